@@ -1,0 +1,313 @@
+"""Run configuration of the Yee engine: the ctypes mirror of ``yee_config``
+(include/yee.h) and a parser for the reference's option table.
+
+The option names, their argument syntax (``x:<int>,y:<int>,z:<int>``) and the
+``--same-size*`` copy rules follow the reference's Settings table and parser
+(Source/Settings/Settings.inc.h:44-343, Source/Settings/Settings.cpp:133-369).
+Floating-point options are parsed the way the reference parses them: with
+``std::stof`` (Source/Helpers/PAssert.h:169-170), i.e. rounded to single
+precision and then widened, so ``--dx 0.0005`` yields the same double as in
+the reference build.
+"""
+from __future__ import annotations
+
+import ctypes
+import struct
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+SCHEME_TMZ = 12
+SCHEME_DIM3 = 13
+
+USE_PML = 0x1
+USE_TFSF = 0x2
+USE_METAMATERIALS = 0x4
+
+COMPONENTS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz",
+              "Dx", "Dy", "Dz", "Bx", "By", "Bz",
+              "D1x", "D1y", "D1z", "B1x", "B1y", "B1z"]
+COMP_INDEX = {name: i for i, name in enumerate(COMPONENTS)}
+
+MATERIALS = ["eps", "mu", "omegape", "omegapm", "gammae", "gammam"]
+MAT_INDEX = {name: i for i, name in enumerate(MATERIALS)}
+
+
+class YeeConfigC(ctypes.Structure):
+    """Binary layout of ``struct yee_config``."""
+    _fields_ = [
+        ("scheme", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("time_steps", ctypes.c_int32),
+        ("size", ctypes.c_int64 * 3),
+        ("pml", ctypes.c_int64 * 3),
+        ("tfsf_left", ctypes.c_int64 * 3),
+        ("tfsf_right", ctypes.c_int64 * 3),
+        ("dx", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("courant", ctypes.c_double),
+        ("angle_deg", ctypes.c_double * 3),
+        ("flags", ctypes.c_uint32),
+        ("point_source", ctypes.c_uint32),
+        ("point_source_pos", ctypes.c_int64 * 3),
+    ]
+
+
+def stof(text) -> float:
+    """Value of ``std::stof(text)`` widened to double (single-precision rounding)."""
+    return struct.unpack("f", struct.pack("f", float(text)))[0]
+
+
+@dataclass
+class Config:
+    """Python view of ``yee_config`` plus the material options of the reference CLI."""
+    scheme: int = SCHEME_DIM3
+    precision: int = 8
+    time_steps: int = 100                     # --time-steps (reference default 100)
+    size: List[int] = field(default_factory=lambda: [100, 100, 100])
+    pml: List[int] = field(default_factory=lambda: [2, 2, 2])
+    tfsf_left: List[int] = field(default_factory=lambda: [4, 4, 4])
+    tfsf_right: List[int] = field(default_factory=lambda: [4, 4, 4])
+    # defaults are the table's FPValue literals (exact doubles); values given on the
+    # command line go through stof
+    dx: float = 0.0005
+    wavelength: float = 0.02
+    courant: float = 0.5
+    angle_deg: List[float] = field(default_factory=lambda: [90.0, 0.0, 90.0])
+    use_pml: bool = False
+    use_tfsf: bool = False
+    use_metamaterials: bool = False
+    point_source: List[str] = field(default_factory=list)   # component names, e.g. ["Ez"]
+    point_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
+    # material options (Settings.inc.h:236-266)
+    eps_file: str = ""
+    mu_file: str = ""
+    omegape_file: str = ""
+    omegapm_file: str = ""
+    gammae_file: str = ""
+    gammam_file: str = ""
+    eps_sphere: float = 1.0
+    eps_sphere_center: List[int] = field(default_factory=lambda: [0, 0, 0])
+    eps_sphere_radius: int = 0
+    mu_sphere: float = 1.0
+    mu_sphere_center: List[int] = field(default_factory=lambda: [0, 0, 0])
+    mu_sphere_radius: int = 0
+    omegape_sphere: float = 0.0
+    omegape_sphere_center: List[int] = field(default_factory=lambda: [0, 0, 0])
+    omegape_sphere_radius: int = 0
+    omegapm_sphere: float = 0.0
+    omegapm_sphere_center: List[int] = field(default_factory=lambda: [0, 0, 0])
+    omegapm_sphere_radius: int = 0
+    sphere_accuracy: int = 100
+    stair_sphere: bool = False
+    layout: int = 0
+
+    @property
+    def dim(self) -> int:
+        return 3 if self.scheme == SCHEME_DIM3 else 2
+
+    @property
+    def shape(self):
+        return tuple(self.size[:3]) if self.dim == 3 else tuple(self.size[:2])
+
+    def to_c(self) -> YeeConfigC:
+        c = YeeConfigC()
+        c.scheme = self.scheme
+        c.layout = self.layout
+        c.precision = self.precision
+        c.time_steps = self.time_steps
+        for a in range(3):
+            c.size[a] = int(self.size[a])
+            c.pml[a] = int(self.pml[a])
+            c.tfsf_left[a] = int(self.tfsf_left[a])
+            c.tfsf_right[a] = int(self.tfsf_right[a])
+            c.angle_deg[a] = float(self.angle_deg[a])
+            c.point_source_pos[a] = int(self.point_source_pos[a])
+        c.dx = self.dx
+        c.wavelength = self.wavelength
+        c.courant = self.courant
+        c.flags = ((USE_PML if self.use_pml else 0) | (USE_TFSF if self.use_tfsf else 0)
+                   | (USE_METAMATERIALS if self.use_metamaterials else 0))
+        mask = 0
+        for name in self.point_source:
+            mask |= 1 << COMP_INDEX[name]
+        c.point_source = mask
+        return c
+
+
+class OptionError(ValueError):
+    """Unknown or malformed option (the reference returns EXIT_UNKNOWN_OPTION, Settings.cpp:364-365)."""
+
+
+def _parse_coord(text: str, default: int) -> List[int]:
+    # Settings::parseCoordinate: comma-separated "x:<int>", "y:<int>", "z:<int>"
+    vals = [default, default, default]
+    for part in text.split(","):
+        if ":" not in part:
+            raise OptionError(f"bad coordinate '{text}'")
+        axis, num = part.split(":", 1)
+        idx = {"x": 0, "y": 1, "z": 2}.get(axis.strip())
+        if idx is None:
+            raise OptionError(f"bad coordinate axis in '{text}'")
+        vals[idx] = int(num)
+    return vals
+
+
+# options that take one argument and are accepted but do not affect the Yee path
+_IGNORED_WITH_ARG = {
+    "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size",
+    "--cuda-buffer-size", "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end",
+    "--ntff-filename", "--ntff-step-angle", "--interm-ntff-step", "--interm-save-step",
+    "--save-start-coord", "--save-end-coord", "--rebalance-step", "--stored-steps",
+    "--current-source", "--save-cmd-to-file",
+}
+_IGNORED_FLAGS = {
+    "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff",
+    "--save-res", "--save-res-per-process", "--save-materials", "--save-interm-res",
+    "--save-scattered-field-res", "--save-scattered--field-interm", "--save-as-bmp", "--save-as-dat",
+    "--save-as-txt", "--save-tfsf-e-incident", "--save-tfsf-h-incident", "--manual-save-coords",
+    "--save-no-pml", "--palette-gray", "--palette-rgb", "--orth-axis-x", "--orth-axis-y",
+    "--orth-axis-z", "--dynamic-grid", "--check-disabling-cond", "--check-enabling-cond",
+    "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
+    "--ntff-to-stdout",
+}
+# options of the reference whose feature is outside this engine: rejected loudly
+_UNSUPPORTED = {
+    "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
+    "--2d-tex", "--2d-tey", "--2d-tez", "--2d-tmx", "--2d-tmy",
+    "--use-double-material-precision", "--eps-normed", "--mu-normed",
+    "--current-source-jx", "--current-source-jy", "--current-source-jz",
+    "--current-source-mx", "--current-source-my", "--current-source-mz",
+}
+
+
+def _read_cmd_file(path: str) -> List[str]:
+    # Settings::loadCmdFromFile: one token per line, '#' and '//' comment lines
+    out = []
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if not line or line.startswith("#") or line.startswith("//"):
+                continue
+            out.append(line)
+    return out
+
+
+def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
+    """Parse the reference's command line (subset that drives the Yee path)."""
+    cfg = Config(precision=precision)
+    args = list(argv)
+    i = 0
+    while i < len(args):
+        a = args[i]
+
+        def val():
+            nonlocal i
+            i += 1
+            if i >= len(args):
+                raise OptionError(f"option {a} needs an argument")
+            return args[i]
+
+        if a == "--cmd-from-file":
+            rest = _read_cmd_file(val())
+            args = args[: i + 1] + rest + args[i + 1:]
+        elif a == "--3d":
+            cfg.scheme = SCHEME_DIM3
+        elif a == "--2d-tmz":
+            cfg.scheme = SCHEME_TMZ
+        elif a == "--size":
+            cfg.size = _parse_coord(val(), 100)
+        elif a == "--same-size":
+            cfg.size = [cfg.size[0]] * 3
+        elif a == "--pml-size":
+            cfg.pml = _parse_coord(val(), 2)
+        elif a == "--same-size-pml":
+            cfg.pml = [cfg.pml[0]] * 3
+        elif a == "--tfsf-size-left":
+            cfg.tfsf_left = _parse_coord(val(), 4)
+        elif a == "--tfsf-size-right":
+            cfg.tfsf_right = _parse_coord(val(), 4)
+        elif a == "--same-size-tfsf":
+            cfg.tfsf_left = [cfg.tfsf_left[0]] * 3
+            cfg.tfsf_right = [cfg.tfsf_right[0]] * 3
+        elif a == "--time-steps":
+            cfg.time_steps = int(val())
+        elif a == "--angle-teta":
+            cfg.angle_deg[0] = stof(val())
+        elif a == "--angle-phi":
+            cfg.angle_deg[1] = stof(val())
+        elif a == "--angle-psi":
+            cfg.angle_deg[2] = stof(val())
+        elif a == "--dx":
+            cfg.dx = stof(val())
+        elif a == "--wavelength":
+            cfg.wavelength = stof(val())
+        elif a == "--courant-factor":
+            cfg.courant = stof(val())
+        elif a == "--use-tfsf":
+            cfg.use_tfsf = True
+        elif a == "--use-pml":
+            cfg.use_pml = True
+        elif a == "--use-metamaterials":
+            cfg.use_metamaterials = True
+        elif a == "--layout-type":
+            cfg.layout = int(val())
+        elif a == "--point-source":
+            cfg.point_source_pos = _parse_coord(val(), 0)
+        elif a.startswith("--point-source-") and a[len("--point-source-"):] in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            comp = a[len("--point-source-"):]
+            cfg.point_source.append(comp[0].upper() + comp[1])
+        elif a == "--load-eps-from-file":
+            cfg.eps_file = val()
+        elif a == "--load-mu-from-file":
+            cfg.mu_file = val()
+        elif a == "--load-omegape-from-file":
+            cfg.omegape_file = val()
+        elif a == "--load-omegapm-from-file":
+            cfg.omegapm_file = val()
+        elif a == "--load-gammae-from-file":
+            cfg.gammae_file = val()
+        elif a == "--load-gammam-from-file":
+            cfg.gammam_file = val()
+        elif a == "--eps-sphere":
+            cfg.eps_sphere = stof(val())
+        elif a == "--eps-sphere-center":
+            cfg.eps_sphere_center = _parse_coord(val(), 0)
+        elif a == "--eps-sphere-radius":
+            cfg.eps_sphere_radius = int(val())
+        elif a == "--mu-sphere":
+            cfg.mu_sphere = stof(val())
+        elif a == "--mu-sphere-center":
+            cfg.mu_sphere_center = _parse_coord(val(), 0)
+        elif a == "--mu-sphere-radius":
+            cfg.mu_sphere_radius = int(val())
+        elif a == "--omegape-sphere":
+            cfg.omegape_sphere = stof(val())
+        elif a == "--omegape-sphere-center":
+            cfg.omegape_sphere_center = _parse_coord(val(), 0)
+        elif a == "--omegape-sphere-radius":
+            cfg.omegape_sphere_radius = int(val())
+        elif a == "--omegapm-sphere":
+            cfg.omegapm_sphere = stof(val())
+        elif a == "--omegapm-sphere-center":
+            cfg.omegapm_sphere_center = _parse_coord(val(), 0)
+        elif a == "--omegapm-sphere-radius":
+            cfg.omegapm_sphere_radius = int(val())
+        elif a == "--sphere-accuracy":
+            cfg.sphere_accuracy = int(val())
+        elif a == "--stair-sphere-approx":
+            cfg.stair_sphere = True
+        elif a in _IGNORED_WITH_ARG:
+            val()
+        elif a in _IGNORED_FLAGS:
+            pass
+        elif a in _UNSUPPORTED:
+            raise OptionError(f"option {a} is outside the Yee engine (3D and TMz, E_CENTERED, real fields)")
+        else:
+            raise OptionError(f"unknown option {a}")
+        i += 1
+    if cfg.use_metamaterials and not cfg.use_pml:
+        raise OptionError("--use-metamaterials requires --use-pml (reference: Scheme.inc.h:889-892)")
+    if cfg.layout != 0:
+        raise OptionError("only --layout-type 0 (E_CENTERED) is supported")
+    return cfg

@@ -1,0 +1,181 @@
+/*
+ * yee.h -- C ABI of the MI355X Yee time-stepping engine (libyee.so).
+ *
+ * The engine replaces the reference's per-cell field update path:
+ *   Scheme::performNStepsForBlock / performFieldSteps / calculateFieldStep
+ *     (Source/Scheme/Scheme.inc.h:28-473, 785-1344)
+ *   InternalScheme::calculateFieldStepIteration{,PML,PMLMetamaterials}, calculateTFSF,
+ *   performPlaneWave{E,H}Steps, performPointSourceCalc
+ *     (Source/Scheme/InternalScheme.inc.h:1386-1835, 2351-2640, 2818-2895, 3276-3357)
+ * together with the host-side set-up those updates read:
+ *   Scheme::initGrids (materials), SchemeHelper::initSigma{X,Y,Z} (PML profiles),
+ *   InternalScheme::initScheme (dt, frequency, incident phase velocity)
+ *     (Source/Scheme/Scheme.inc.h:2962-3165, Source/Scheme/SchemeHelper.h:574-718,
+ *      Source/Scheme/InternalScheme.inc.h:3359-3394).
+ *
+ * The reference has no FFI for this path: its seam is the C++ call from Scheme into
+ * InternalScheme / InternalSchemeGPU.  Every entry point below names the reference
+ * member function whose role it takes.
+ *
+ * Conventions
+ *   - All functions return 0 on success and a negative YEE_E* code on failure; the
+ *     message of the last failure on the calling thread is yee_last_error().
+ *   - Host pointers are borrowed for the duration of the call and copied.
+ *   - Grids are dense [x][y][z] arrays with z fastest (the reference's storage order,
+ *     Source/Grid/Grid.h:585-730); TMz grids are [x][y].
+ *   - A handle owns its device memory and one HIP stream; a handle is not thread-safe,
+ *     distinct handles may be used concurrently.
+ *   - Fields start at zero, as in the reference (field loading is commented out in
+ *     Source/Settings/Settings.inc.h:250-272).
+ */
+#ifndef YEE_H
+#define YEE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YEE_ABI_VERSION 1
+
+/* SchemeType values of the reference (Source/Settings/Settings.h:47-62). */
+#define YEE_SCHEME_TMZ 12
+#define YEE_SCHEME_DIM3 13
+
+/* yee_config.flags */
+#define YEE_USE_PML 0x1u            /* --use-pml */
+#define YEE_USE_TFSF 0x2u           /* --use-tfsf */
+#define YEE_USE_METAMATERIALS 0x4u  /* --use-metamaterials (requires --use-pml, Scheme.inc.h:889-892) */
+
+/* Field components (GridType order of the reference). */
+enum yee_component
+{
+  YEE_EX = 0, YEE_EY = 1, YEE_EZ = 2, YEE_HX = 3, YEE_HY = 4, YEE_HZ = 5,
+  /* auxiliary PML / Drude state (valid inside the PML slabs; see yee_get_field) */
+  YEE_DX = 6, YEE_DY = 7, YEE_DZ = 8, YEE_BX = 9, YEE_BY = 10, YEE_BZ = 11,
+  YEE_D1X = 12, YEE_D1Y = 13, YEE_D1Z = 14, YEE_B1X = 15, YEE_B1Y = 16, YEE_B1Z = 17
+};
+
+/* Material grids (Source/Scheme/Grids.inc.h). */
+enum yee_material
+{
+  YEE_MAT_EPS = 0, YEE_MAT_MU = 1, YEE_MAT_OMEGAPE = 2, YEE_MAT_OMEGAPM = 3, YEE_MAT_GAMMAE = 4, YEE_MAT_GAMMAM = 5
+};
+
+/* Error codes. */
+#define YEE_OK 0
+#define YEE_EINVAL (-1)      /* bad argument (null pointer, size, unsupported option) */
+#define YEE_ENOMEM (-2)      /* device or host allocation failed */
+#define YEE_EDEVICE (-3)     /* HIP runtime error */
+#define YEE_ESTATE (-4)      /* call out of order (e.g. step before prepare) */
+#define YEE_ENOTSUP (-5)     /* feature not built or not supported */
+
+/*
+ * Run configuration, filled from the reference's option table
+ * (Source/Settings/Settings.inc.h) by the CLI front end or by a binding.
+ */
+typedef struct yee_config
+{
+  int32_t scheme;             /* YEE_SCHEME_DIM3 (--3d) or YEE_SCHEME_TMZ (--2d-tmz) */
+  int32_t layout;             /* --layout-type: only 0 (E_CENTERED) is supported */
+  int32_t precision;          /* 4 = FP32 fields, 8 = FP64 fields (reference: -DFLOAT_VALUES / -DDOUBLE_VALUES) */
+  int32_t time_steps;         /* --time-steps T; step t = 0 performs no field update (Scheme.inc.h:1032) */
+  int64_t size[3];            /* --size x:,y:,z: */
+  int64_t pml[3];             /* --pml-size */
+  int64_t tfsf_left[3];       /* --tfsf-size-left */
+  int64_t tfsf_right[3];      /* --tfsf-size-right */
+  double dx;                  /* --dx (m) */
+  double wavelength;          /* --wavelength (m) */
+  double courant;             /* --courant-factor */
+  double angle_deg[3];        /* --angle-teta, --angle-phi, --angle-psi (degrees) */
+  uint32_t flags;             /* YEE_USE_* */
+  uint32_t point_source;      /* bit (1 << yee_component) for each --point-source-<comp> */
+  int64_t point_source_pos[3];/* --point-source x:,y:,z: */
+} yee_config;
+
+typedef struct yee_handle yee_handle;
+
+/* Thread-local message of the last failed call ("" if none). */
+const char *yee_last_error (void);
+
+/* Library / device information: fills up to `len` bytes of `buf` with a one-line description. */
+int yee_device_info (char *buf, int64_t len);
+
+/*
+ * Creates an engine for `cfg` and allocates its device state.
+ * Replaces InternalScheme::init + allocateGrids/allocateGridsInc
+ * (Source/Scheme/InternalScheme.specific.h:37-107) and InternalScheme::initScheme
+ * (Source/Scheme/InternalScheme.inc.h:3359-3394).
+ */
+int yee_create (const yee_config *cfg, yee_handle **out);
+
+void yee_destroy (yee_handle *h);
+
+/*
+ * Sets one material grid (full [x][y][z] FP64 array of size[0]*size[1]*size[2] values,
+ * or [x][y] for TMz).  Defaults: eps = mu = 1, omega/gamma = 0.  Omega values are in
+ * rad/s (the reference scales --omegape-sphere by 2*pi*f before storing, Scheme.inc.h:3091).
+ * Replaces the material part of Scheme::initGrids (Source/Scheme/Scheme.inc.h:2962-3165).
+ */
+int yee_set_material (yee_handle *h, int32_t which, const double *values);
+
+/*
+ * Host FP64 set-up of everything the kernels read: PML sigma profiles
+ * (SchemeHelper::initSigma{X,Y,Z}, Source/Scheme/SchemeHelper.h:574-718), the material
+ * encoding, Drude coefficients, TF/SF face tables and the incident 1D grid parameters.
+ * Must be called after the last yee_set_material and before yee_step.
+ */
+int yee_prepare (yee_handle *h);
+
+/*
+ * Advances `n` time steps (t = current .. current + n - 1), enqueued on the handle's
+ * stream; returns after the work is enqueued.  Replaces Scheme::performNStepsForBlock
+ * (Source/Scheme/Scheme.inc.h:28-473).
+ */
+int yee_step (yee_handle *h, int32_t n);
+
+/* Waits for all enqueued work of the handle. */
+int yee_sync (yee_handle *h);
+
+/* Number of time steps performed so far. */
+int yee_time (yee_handle *h, int32_t *t);
+
+/*
+ * Copies the newest time layer of a component to host memory as FP64
+ * ([x][y][z], or [x][y] for TMz).  Components absent from the scheme return YEE_EINVAL.
+ * D/B (and D1/B1) hold the reference's values inside the PML slabs; in the interior the
+ * engine updates E/H directly, which equals the reference's D->E stage with sigma = 0
+ * up to rounding, and D/B read back as 0 there.
+ */
+int yee_get_field (yee_handle *h, int32_t comp, double *out);
+
+/* Same as yee_get_field, in the handle's own precision (float* when precision == 4). */
+int yee_get_field_raw (yee_handle *h, int32_t comp, void *out);
+
+/*
+ * Mask export for bit-exact checks of the set-up tables.
+ *   kind 0: TF/SF correction mask of component `comp`: bit d set when the reference's
+ *           doNeedTFSFUpdate<comp>Border(pos, dir d) is true, d in LEFT=0, RIGHT=1, DOWN=2,
+ *           UP=3, BACK=4, FRONT=5 (Source/Layout/YeeGridLayout.cpp:272-1740).
+ *   kind 1: PML-slab mask (1 where the engine runs the D/B two-stage update).
+ */
+int yee_get_mask (yee_handle *h, int32_t kind, int32_t comp, uint8_t *out);
+
+/* Copies the incident 1D grids (newest layer), `len` values each; either pointer may be null. */
+int yee_get_incident (yee_handle *h, double *einc, double *hinc, int64_t len);
+
+/* Scalars of InternalScheme::initScheme: out[0] = dt, out[1] = frequency, out[2] = relPhaseVelocity. */
+int yee_get_scalars (yee_handle *h, double out[3]);
+
+/*
+ * Timing hook for benchmarks: average device time (ms) per step of the main stencil kernel
+ * over the last yee_step call, measured with HIP events on the handle's stream.
+ */
+int yee_last_step_ms (yee_handle *h, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YEE_H */

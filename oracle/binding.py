@@ -1,0 +1,184 @@
+"""ctypes binding of the C oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from fdtd3d_amd.config import Config, COMP_INDEX, MAT_INDEX
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.oracle_create.restype = vp
+        L.oracle_create.argtypes = [vp]
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_set_material.argtypes = [vp, ctypes.c_int32, vp]
+        L.oracle_prepare.argtypes = [vp]
+        L.oracle_step.argtypes = [vp, ctypes.c_int32]
+        L.oracle_get_field.argtypes = [vp, ctypes.c_int32, vp]
+        L.oracle_get_material.argtypes = [vp, ctypes.c_int32, vp]
+        L.oracle_tfsf_mask.argtypes = [vp, ctypes.c_int32, vp]
+        L.oracle_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
+        L.oracle_get_scalars.argtypes = [vp, vp]
+        L.oracle_incident_len.restype = ctypes.c_int64
+        L.oracle_incident_len.argtypes = [vp]
+        d = ctypes.c_double
+        L.oracle_sphere_accurate_3d.restype = d
+        L.oracle_sphere_accurate_3d.argtypes = [d] * 9 + [ctypes.c_int]
+        L.oracle_sphere_accurate_2d.restype = d
+        L.oracle_sphere_accurate_2d.argtypes = [d] * 7 + [ctypes.c_int]
+        L.oracle_sphere_stair_3d.restype = d
+        L.oracle_sphere_stair_3d.argtypes = [d] * 9
+        L.oracle_wave_number.restype = d
+        L.oracle_wave_number.argtypes = [d] * 6
+        _lib = L
+    return _lib
+
+
+def sphere_map(cfg: Config, value: float, outside: float, center, radius: int) -> np.ndarray:
+    """Material map of a sphere as Scheme::initGrids builds it (Scheme.inc.h:2983-3012)."""
+    L = lib()
+    shape = cfg.shape
+    out = np.empty(shape, dtype=np.float64)
+    cx, cy, cz = (float(center[0]) + 0.5, float(center[1]) + 0.5, float(center[2]) + 0.5)
+    r = float(radius)
+    it = np.ndindex(*shape)
+    for idx in it:
+        if cfg.dim == 3:
+            i, j, k = idx
+            mx, my, mz = i + 0.5, j + 0.5, k + 0.5
+            if cfg.stair_sphere:
+                out[idx] = L.oracle_sphere_stair_3d(mx, my, mz, cx, cy, cz, r, value, outside)
+            else:
+                out[idx] = L.oracle_sphere_accurate_3d(mx, my, mz, cx, cy, cz, r, value, outside, cfg.sphere_accuracy)
+        else:
+            i, j = idx
+            out[idx] = L.oracle_sphere_accurate_2d(i + 0.5, j + 0.5, cx, cy, r, value, outside, cfg.sphere_accuracy)
+    return out
+
+
+class Oracle:
+    """FP64 CPU restatement of the reference path; same life cycle as the engine handle."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        self.h = lib().oracle_create(ctypes.byref(self._c))
+        if not self.h:
+            raise ValueError("oracle_create rejected the configuration")
+        self.shape = cfg.shape
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_material(self, name: str, values: np.ndarray) -> None:
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.shape)
+        rc = lib().oracle_set_material(self.h, MAT_INDEX[name], arr.ctypes.data)
+        if rc:
+            raise ValueError(name)
+
+    def prepare(self) -> None:
+        lib().oracle_prepare(self.h)
+
+    def step(self, n: int) -> None:
+        if lib().oracle_step(self.h, int(n)):
+            raise RuntimeError("oracle_step")
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        if lib().oracle_get_field(self.h, COMP_INDEX[name], out.ctypes.data):
+            raise KeyError(name)
+        return out
+
+    def material(self, which: int) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        if lib().oracle_get_material(self.h, which, out.ctypes.data):
+            raise KeyError(which)
+        return out
+
+    def tfsf_mask(self, comp: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint8)
+        if lib().oracle_tfsf_mask(self.h, COMP_INDEX[comp], out.ctypes.data):
+            raise KeyError(comp)
+        return out
+
+    def incident(self):
+        n = lib().oracle_incident_len(self.h)
+        e = np.empty(n, dtype=np.float64)
+        h = np.empty(n, dtype=np.float64)
+        lib().oracle_get_incident(self.h, e.ctypes.data, h.ctypes.data, n)
+        return e, h
+
+    def scalars(self):
+        out = np.empty(3, dtype=np.float64)
+        lib().oracle_get_scalars(self.h, out.ctypes.data)
+        return out
+
+
+def run_oracle(cfg: Config, materials=None, steps=None) -> Oracle:
+    o = Oracle(cfg)
+    for name, arr in (materials or {}).items():
+        o.set_material(name, arr)
+    o.prepare()
+    o.step(cfg.time_steps if steps is None else steps)
+    return o
+
+
+def read_ref_output(path: str):
+    """Records written by oracle/ref_driver.cpp: [name:16][int64 n][n doubles]."""
+    out = {}
+    with open(path, "rb") as fh:
+        data = fh.read()
+    pos = 0
+    while pos < len(data):
+        name = data[pos:pos + 16].split(b"\0", 1)[0].decode()
+        n = int(np.frombuffer(data[pos + 16:pos + 24], dtype=np.int64)[0])
+        pos += 24
+        out[name] = np.frombuffer(data[pos:pos + 8 * n], dtype=np.float64).copy()
+        pos += 8 * n
+    return out
+
+
+def ref_driver_path(kind: str = "f64") -> str:
+    return os.path.join(REF_DIR, "refdrv_" + kind)
+
+
+def run_ref_driver(args, out_path: str, kind: str = "f64", timeout: float = 600):
+    env = dict(os.environ)
+    env["YEE_REF_OUT"] = out_path
+    res = subprocess.run([ref_driver_path(kind)] + list(args), env=env, capture_output=True,
+                         text=True, timeout=timeout)
+    if res.returncode != 0:
+        raise RuntimeError(f"ref driver failed ({res.returncode}): {res.stderr[-2000:]}")
+    secs = None
+    for line in res.stdout.splitlines():
+        if line.startswith("Total time ="):
+            secs = float(line.split("=")[1].split()[0])
+    return read_ref_output(out_path), secs

@@ -1,0 +1,575 @@
+/*
+ * ref_driver.cpp -- test infrastructure, NOT product code.
+ *
+ * A small driver that runs the reference's OWN per-cell arithmetic (compiled from
+ * the reference sources by oracle/Makefile into oracle/_ref/) so that the C oracle
+ * in oracle/yee_oracle.c can be checked against it, and golden fixtures can be
+ * written for tests/golden/.
+ *
+ * What comes from the reference (linked, not copied):
+ *   - option parsing                      Settings::SetupFromCmd   (Settings/Settings.cpp:558)
+ *   - Yee layout, TF/SF predicates        YeeGridLayout            (Layout/YeeGridLayout.cpp)
+ *   - per-cell updates                    InternalScheme::calculateFieldStepIteration / ...PML /
+ *                                         ...PMLMetamaterials       (Scheme/InternalScheme.inc.h:2351-2640)
+ *   - incident 1D grid, point source      performPlaneWave{E,H}Steps, performPointSourceCalc
+ *                                         (Scheme/InternalScheme.inc.h:3276-3357, 2818-2895)
+ *   - sphere approximation                Approximation::approximateSphere{Accurate,Stair}
+ *                                         (Layout/Approximation.cpp:563-764)
+ *
+ * What this driver restates itself (because the reference's versions live in
+ * Scheme/SchemeHelper, whose headers pull in the un-vendored EasyBMP):
+ *   - the step ordering of Scheme::performNStepsForBlock (Scheme/Scheme.inc.h:28-473)
+ *     and the three sweeps of Scheme::calculateFieldStep (Scheme/Scheme.inc.h:876-1344);
+ *   - the material set-up of Scheme::initGrids (Scheme/Scheme.inc.h:2962-3165);
+ *   - the PML sigma profiles of SchemeHelper::initSigma{,X,Y,Z} (Scheme/SchemeHelper.h:574-718).
+ *
+ * Output: one binary file (path from env YEE_REF_OUT, default "ref_out.bin") with
+ * records  [char name[16]][int64 count][double values[count]]  holding the latest
+ * time layer of every state grid, the material grids and the incident 1D grids.
+ * Stdout carries "Total time = ..." for the stepping loop only, as the reference CLI
+ * prints it (Source/main.cpp:1211).
+ */
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <sys/time.h>
+
+#include "InternalScheme.h"
+#include "Approximation.h"
+
+namespace
+{
+
+FILE *g_out = NULL;
+
+void putRecord (const char *name, const std::vector<double> &v)
+{
+  char buf[16];
+  memset (buf, 0, sizeof (buf));
+  strncpy (buf, name, 15);
+  int64_t n = (int64_t) v.size ();
+  fwrite (buf, 1, 16, g_out);
+  fwrite (&n, sizeof (n), 1, g_out);
+  if (n)
+  {
+    fwrite (v.data (), sizeof (double), v.size (), g_out);
+  }
+}
+
+/* Polynomial-graded sigma of one cell; restated from SchemeHelper::initSigma (SchemeHelper.h:574-590). */
+FPValue sigmaValue (FPValue dist, FPValue boundary, FPValue gridStep)
+{
+  FPValue eps0 = PhysicsConst::Eps0;
+  FPValue mu0 = PhysicsConst::Mu0;
+  uint32_t exponent = 6;
+  FPValue R_err = 1e-16;
+  FPValue sigma_max_1 = -log (R_err) * (exponent + 1.0) / (2.0 * sqrt (mu0 / eps0) * boundary);
+  FPValue boundaryFactor = sigma_max_1 / (gridStep * (pow (boundary, exponent)) * (exponent + 1));
+  FPValue x1 = (dist + 1) * gridStep;
+  FPValue x2 = dist * gridStep;
+  return boundaryFactor * (pow (x1, (exponent + 1)) - pow (x2, (exponent + 1)));
+}
+
+bool readRaw (const std::string &name, std::vector<FieldValue> &dst, size_t n)
+{
+  FILE *f = fopen (name.c_str (), "rb");
+  if (!f)
+  {
+    fprintf (stderr, "cannot open %s\n", name.c_str ());
+    return false;
+  }
+  dst.resize (n);
+  size_t got = fread (dst.data (), sizeof (FieldValue), n, f);
+  fclose (f);
+  if (got != n)
+  {
+    fprintf (stderr, "short file %s: %zu of %zu values\n", name.c_str (), got, n);
+    return false;
+  }
+  return true;
+}
+
+template <SchemeType_t Type, template <typename, bool> class TCoord>
+class RefRun
+{
+  typedef TCoord<grid_coord, true> TC;
+  typedef TCoord<grid_coord, false> TCS;
+  typedef TCoord<FPValue, true> TCFP;
+  typedef InternalScheme<Type, TCoord, E_CENTERED> IS;
+  typedef YeeGridLayout<Type, TCoord, E_CENTERED> Layout;
+
+  CoordinateType ct1, ct2, ct3;
+  TC size;
+  Layout *layout;
+  IS *is;
+  int dim;
+
+  TC coordOf (grid_coord i, grid_coord j, grid_coord k) const
+  {
+    return TC::initAxesCoordinate (i, j, k, ct1, ct2, ct3);
+  }
+
+  grid_coord sz (int axis) const
+  {
+    if (axis == 0) return size.get1 ();
+    if (axis == 1) return size.get2 ();
+    return sizeZ ();
+  }
+
+  grid_coord sizeZ () const;
+
+  template <class F>
+  void forAll (grid_coord lo, grid_coord hiOff, F f)
+  {
+    grid_coord nx = sz (0), ny = sz (1), nz = sz (2);
+    grid_coord zlo = dim == 3 ? lo : 0;
+    grid_coord zhi = dim == 3 ? nz - hiOff : 1;
+    for (grid_coord i = lo; i < nx - hiOff; ++i)
+      for (grid_coord j = lo; j < ny - hiOff; ++j)
+        for (grid_coord k = zlo; k < zhi; ++k)
+          f (coordOf (i, j, k));
+  }
+
+  std::vector<double> dumpGrid (Grid<TC> *g, int layer)
+  {
+    std::vector<double> v;
+    forAll (0, 0, [&] (TC p) { v.push_back ((double) *g->getFieldValue (p, layer)); });
+    return v;
+  }
+
+  void loadMaterial (Grid<TC> *g, const std::string &file)
+  {
+    if (file.empty ())
+    {
+      return;
+    }
+    size_t n = (size_t) sz (0) * sz (1) * sz (2);
+    std::vector<FieldValue> raw;
+    if (!readRaw (file, raw, n))
+    {
+      exit (2);
+    }
+    size_t idx = 0;
+    forAll (0, 0, [&] (TC p) { g->setFieldValue (raw[idx++], p, 0); });
+  }
+
+  /* Sphere of `value` (outside `outside`) on the material grid; follows Scheme::initGrids (Scheme.inc.h:2983-3012). */
+  void sphere (Grid<TC> *g, FPValue value, FPValue outside, grid_coord cx, grid_coord cy, grid_coord cz, int radius)
+  {
+    TCFP center = TCFP::initAxesCoordinate ((FPValue) cx, (FPValue) cy, (FPValue) cz, ct1, ct2, ct3) * FPValue (1)
+                  + TCFP::initAxesCoordinate (0.5, 0.5, 0.5, ct1, ct2, ct3);
+    forAll (0, 0, [&] (TC p) {
+      TCFP posAbs = layout->getEpsCoordFP (p);
+      FieldValue v;
+      if (SOLVER_SETTINGS.getDoUseStairApproximation ())
+      {
+        v = Approximation::approximateSphereStair (posAbs, center, radius * FPValue (1), value, outside);
+      }
+      else
+      {
+        v = Approximation::approximateSphereAccurate (posAbs, center, radius * FPValue (1), value, outside);
+      }
+      g->setFieldValue (v, p, 0);
+    });
+  }
+
+  void initSigmas ()
+  {
+    FPValue dx = SOLVER_SETTINGS.getGridStep ();
+    TC pml = layout->getLeftBorderPML ();
+    /* x: fractional distance, right threshold N - P (SchemeHelper.h:592-630) */
+    {
+      Grid<TC> *s = is->getSigmaX ();
+      FPValue P = FPValue (pml.get1 ());
+      FPValue boundary = P * dx;
+      FPValue n = FPValue (sz (0));
+      forAll (0, 0, [&] (TC p) {
+        FPValue x = layout->getEpsCoordFP (p).get1 ();
+        FPValue val = 0;
+        if (x < P) val = sigmaValue (P - x, boundary, dx);
+        else if (x >= n - P) val = sigmaValue (x - (n - P), boundary, dx);
+        s->setFieldValue (val, p, 0);
+      });
+    }
+    /* y (and z in 3D): truncated integer distance, right threshold N + 0.5 - P (SchemeHelper.h:632-718) */
+    for (int axis = 1; axis < dim; ++axis)
+    {
+      Grid<TC> *s = axis == 1 ? is->getSigmaY () : is->getSigmaZ ();
+      FPValue P = FPValue (axis == 1 ? pml.get2 () : pmlZ ());
+      FPValue boundary = P * dx;
+      FPValue n = (FPValue) sz (axis) + FPValue (0.5);
+      forAll (0, 0, [&] (TC p) {
+        TCFP fp = layout->getEpsCoordFP (p);
+        FPValue y = axis == 1 ? fp.get2 () : fpZ (fp);
+        FPValue val = 0;
+        if (y < P)
+        {
+          grid_coord d = (grid_coord) (P - y);
+          val = sigmaValue (d, boundary, dx);
+        }
+        else if (y >= n - P)
+        {
+          grid_coord d = (grid_coord) (y - (n - P));
+          val = sigmaValue (d, boundary, dx);
+        }
+        s->setFieldValue (val, p, 0);
+      });
+    }
+  }
+
+  grid_coord pmlZ () const;
+  FPValue fpZ (const TCFP &) const;
+
+  template <uint8_t grid_type, bool usePML, bool useMeta>
+  void fieldStep (time_step t)
+  {
+    Grid<TC> *grid = NULLPTR;
+    GridType gridType = GridType::NONE;
+    Grid<TC> *mg = NULLPTR, *mg1 = NULLPTR, *mg2 = NULLPTR, *mg3 = NULLPTR, *mg4 = NULLPTR, *mg5 = NULLPTR;
+    GridType mgt = GridType::NONE, mgt1 = GridType::NONE, mgt2 = GridType::NONE, mgt3 = GridType::NONE,
+             mgt4 = GridType::NONE, mgt5 = GridType::NONE;
+    Grid<TC> *opp1 = NULLPTR, *opp2 = NULLPTR, *pml1 = NULLPTR, *pml2 = NULLPTR;
+    GridType pmlType1 = GridType::NONE, pmlType2 = GridType::NONE;
+    Grid<TC> *Ca = NULLPTR, *Cb = NULLPTR, *CB0 = NULLPTR, *CB1 = NULLPTR, *CB2 = NULLPTR, *CA1 = NULLPTR,
+             *CA2 = NULLPTR, *CaPML = NULLPTR, *CbPML = NULLPTR, *CcPML = NULLPTR;
+    SourceCallBack rhs = NULLPTR, border = NULLPTR, exact = NULLPTR;
+    FPValue modifier = 0;
+    TCS d11, d12, d21, d22;
+
+    is->template calculateFieldStepInit<grid_type, usePML, useMeta> (&grid, &gridType,
+      &mg, &mgt, &mg1, &mgt1, &mg2, &mgt2, &mg3, &mgt3, &mg4, &mgt4, &mg5, &mgt5,
+      &opp1, &opp2, &pml1, &pmlType1, &pml2, &pmlType2, &rhs, &border, &exact, &modifier,
+      &Ca, &Cb, &CB0, &CB1, &CB2, &CA1, &CA2, &CaPML, &CbPML, &CcPML);
+    is->template calculateFieldStepInitDiff<grid_type> (&d11, &d12, &d21, &d22);
+
+    /* the reference skips every field update at t == 0 (Scheme.inc.h:1032) */
+    if (t > 0)
+    {
+      bool isE = grid_type == static_cast<uint8_t> (GridType::EX) || grid_type == static_cast<uint8_t> (GridType::EY)
+                 || grid_type == static_cast<uint8_t> (GridType::EZ);
+      FPValue timestep = isE ? FPValue (t) : FPValue (t) + FPValue (0.5);
+      TCFP coordFP;
+      forAll (1, 1, [&] (TC p) {
+        is->template calculateFieldStepIteration<grid_type, false> (timestep, p, grid->getTotalPosition (p),
+          d11, d12, d21, d22, grid, coordFP, opp1, opp2, rhs, Ca, Cb, usePML, gridType, mg, mgt, modifier);
+      });
+      if (usePML)
+      {
+        if (useMeta)
+        {
+          forAll (1, 1, [&] (TC p) {
+            is->template calculateFieldStepIterationPMLMetamaterials<false> (t, p, grid, pml1,
+              CB0, CB1, CB2, CA1, CA2, gridType, mg1, mgt1, mg2, mgt2, mg3, mgt3, modifier);
+          });
+        }
+        forAll (1, 1, [&] (TC p) {
+          is->template calculateFieldStepIterationPML<useMeta, false> (t, p, grid, pml1, pml2, CaPML, CbPML, CcPML,
+            pmlType1, mg1, mgt1, mg4, mgt4, mg5, mgt5, modifier);
+        });
+      }
+    }
+
+    bool src = false;
+    switch (grid_type)
+    {
+      case (static_cast<uint8_t> (GridType::EX)): src = SOLVER_SETTINGS.getDoUsePointSourceEx (); break;
+      case (static_cast<uint8_t> (GridType::EY)): src = SOLVER_SETTINGS.getDoUsePointSourceEy (); break;
+      case (static_cast<uint8_t> (GridType::EZ)): src = SOLVER_SETTINGS.getDoUsePointSourceEz (); break;
+      case (static_cast<uint8_t> (GridType::HX)): src = SOLVER_SETTINGS.getDoUsePointSourceHx (); break;
+      case (static_cast<uint8_t> (GridType::HY)): src = SOLVER_SETTINGS.getDoUsePointSourceHy (); break;
+      case (static_cast<uint8_t> (GridType::HZ)): src = SOLVER_SETTINGS.getDoUsePointSourceHz (); break;
+      default: break;
+    }
+    if (src)
+    {
+      is->template performPointSourceCalc<grid_type> (t);
+    }
+  }
+
+  template <uint8_t grid_type>
+  void fieldStepDispatch (time_step t)
+  {
+    bool pml = SOLVER_SETTINGS.getDoUsePML ();
+    bool meta = SOLVER_SETTINGS.getDoUseMetamaterials ();
+    if (pml && meta) fieldStep<grid_type, true, true> (t);
+    else if (pml) fieldStep<grid_type, true, false> (t);
+    else fieldStep<grid_type, false, false> (t);
+  }
+
+  void shiftGroup (Grid<TC> *f, Grid<TC> *d, Grid<TC> *d1)
+  {
+    f->shiftInTime ();
+    if (SOLVER_SETTINGS.getDoUsePML ())
+    {
+      d->shiftInTime ();
+    }
+    if (SOLVER_SETTINGS.getDoUseMetamaterials ())
+    {
+      d1->shiftInTime ();
+    }
+  }
+
+public:
+
+  RefRun (TC s, TC pml, TC tfL, TC tfR, int d)
+    : size (s), dim (d)
+  {
+    ct1 = CoordinateType::X;
+    ct2 = CoordinateType::Y;
+    ct3 = d == 3 ? CoordinateType::Z : CoordinateType::NONE;
+    layout = new Layout (s, pml, tfL, tfR,
+                         SOLVER_SETTINGS.getIncidentWaveAngle1 () * PhysicsConst::Pi / 180.0,
+                         SOLVER_SETTINGS.getIncidentWaveAngle2 () * PhysicsConst::Pi / 180.0,
+                         SOLVER_SETTINGS.getIncidentWaveAngle3 () * PhysicsConst::Pi / 180.0,
+                         SOLVER_SETTINGS.getDoUseDoubleMaterialPrecision ());
+    is = new IS ();
+    is->init (layout, false);
+    is->initScheme (SOLVER_SETTINGS.getGridStep (), SOLVER_SETTINGS.getSourceWaveLength ());
+  }
+
+  void initMaterials ()
+  {
+    is->getEps ()->initialize (FieldValueHelpers::getFieldValueRealOnly (1.0));
+    loadMaterial (is->getEps (), SOLVER_SETTINGS.getEpsFileName ());
+    if (SOLVER_SETTINGS.getEpsSphere () != 1)
+    {
+      sphere (is->getEps (), SOLVER_SETTINGS.getEpsSphere (), 1.0, SOLVER_SETTINGS.getEpsSphereCenterX (),
+              SOLVER_SETTINGS.getEpsSphereCenterY (), SOLVER_SETTINGS.getEpsSphereCenterZ (),
+              SOLVER_SETTINGS.getEpsSphereRadius ());
+    }
+    is->getMu ()->initialize (FieldValueHelpers::getFieldValueRealOnly (1.0));
+    loadMaterial (is->getMu (), SOLVER_SETTINGS.getMuFileName ());
+    if (SOLVER_SETTINGS.getMuSphere () != 1)
+    {
+      sphere (is->getMu (), SOLVER_SETTINGS.getMuSphere (), 1.0, SOLVER_SETTINGS.getMuSphereCenterX (),
+              SOLVER_SETTINGS.getMuSphereCenterY (), SOLVER_SETTINGS.getMuSphereCenterZ (),
+              SOLVER_SETTINGS.getMuSphereRadius ());
+    }
+    if (SOLVER_SETTINGS.getDoUseMetamaterials ())
+    {
+      FPValue w = 2 * PhysicsConst::Pi * is->getSourceFrequency ();
+      is->getOmegaPE ()->initialize (FieldValueHelpers::getFieldValueRealOnly (0.0));
+      loadMaterial (is->getOmegaPE (), SOLVER_SETTINGS.getOmegaPEFileName ());
+      if (SOLVER_SETTINGS.getOmegaPESphere () != 0)
+      {
+        sphere (is->getOmegaPE (), SOLVER_SETTINGS.getOmegaPESphere () * w, 0.0,
+                SOLVER_SETTINGS.getOmegaPESphereCenterX (), SOLVER_SETTINGS.getOmegaPESphereCenterY (),
+                SOLVER_SETTINGS.getOmegaPESphereCenterZ (), SOLVER_SETTINGS.getOmegaPESphereRadius ());
+      }
+      is->getOmegaPM ()->initialize (FieldValueHelpers::getFieldValueRealOnly (0.0));
+      loadMaterial (is->getOmegaPM (), SOLVER_SETTINGS.getOmegaPMFileName ());
+      if (SOLVER_SETTINGS.getOmegaPMSphere () != 0)
+      {
+        sphere (is->getOmegaPM (), SOLVER_SETTINGS.getOmegaPMSphere () * w, 0.0,
+                SOLVER_SETTINGS.getOmegaPMSphereCenterX (), SOLVER_SETTINGS.getOmegaPMSphereCenterY (),
+                SOLVER_SETTINGS.getOmegaPMSphereCenterZ (), SOLVER_SETTINGS.getOmegaPMSphereRadius ());
+      }
+      is->getGammaE ()->initialize (FieldValueHelpers::getFieldValueRealOnly (0.0));
+      loadMaterial (is->getGammaE (), SOLVER_SETTINGS.getGammaEFileName ());
+      is->getGammaM ()->initialize (FieldValueHelpers::getFieldValueRealOnly (0.0));
+      loadMaterial (is->getGammaM (), SOLVER_SETTINGS.getGammaMFileName ());
+    }
+    if (SOLVER_SETTINGS.getDoUsePML ())
+    {
+      initSigmas ();
+    }
+  }
+
+  double run ()
+  {
+    struct timeval tv1, tv2;
+    gettimeofday (&tv1, NULL);
+    bool tfsf = SOLVER_SETTINGS.getDoUseTFSF ();
+    bool pml = SOLVER_SETTINGS.getDoUsePML ();
+    bool meta = SOLVER_SETTINGS.getDoUseMetamaterials ();
+    GridCoordinate1D zero1D = GRID_COORDINATE_1D (0, CoordinateType::X);
+    for (time_step t = 0; t < SOLVER_SETTINGS.getNumTimeSteps (); ++t)
+    {
+      if (tfsf)
+      {
+        is->performPlaneWaveESteps (t, zero1D, is->getEInc ()->getSize ());
+        is->getEInc ()->shiftInTime ();
+      }
+      if (is->getDoNeedEx ()) fieldStepDispatch<static_cast<uint8_t> (GridType::EX)> (t);
+      if (is->getDoNeedEy ()) fieldStepDispatch<static_cast<uint8_t> (GridType::EY)> (t);
+      if (is->getDoNeedEz ()) fieldStepDispatch<static_cast<uint8_t> (GridType::EZ)> (t);
+      if (is->getDoNeedEx ()) shiftGroup (is->getEx (), pml ? is->getDx () : NULLPTR, meta ? is->getD1x () : NULLPTR);
+      if (is->getDoNeedEy ()) shiftGroup (is->getEy (), pml ? is->getDy () : NULLPTR, meta ? is->getD1y () : NULLPTR);
+      if (is->getDoNeedEz ()) shiftGroup (is->getEz (), pml ? is->getDz () : NULLPTR, meta ? is->getD1z () : NULLPTR);
+      if (tfsf)
+      {
+        is->performPlaneWaveHSteps (t, zero1D, is->getHInc ()->getSize ());
+        is->getHInc ()->shiftInTime ();
+      }
+      if (is->getDoNeedHx ()) fieldStepDispatch<static_cast<uint8_t> (GridType::HX)> (t);
+      if (is->getDoNeedHy ()) fieldStepDispatch<static_cast<uint8_t> (GridType::HY)> (t);
+      if (is->getDoNeedHz ()) fieldStepDispatch<static_cast<uint8_t> (GridType::HZ)> (t);
+      if (is->getDoNeedHx ()) shiftGroup (is->getHx (), pml ? is->getBx () : NULLPTR, meta ? is->getB1x () : NULLPTR);
+      if (is->getDoNeedHy ()) shiftGroup (is->getHy (), pml ? is->getBy () : NULLPTR, meta ? is->getB1y () : NULLPTR);
+      if (is->getDoNeedHz ()) shiftGroup (is->getHz (), pml ? is->getBz () : NULLPTR, meta ? is->getB1z () : NULLPTR);
+    }
+    gettimeofday (&tv2, NULL);
+    return (double) (tv2.tv_usec - tv1.tv_usec) / 1000000 + (double) (tv2.tv_sec - tv1.tv_sec);
+  }
+
+  void dump ()
+  {
+    bool pml = SOLVER_SETTINGS.getDoUsePML ();
+    bool meta = SOLVER_SETTINGS.getDoUseMetamaterials ();
+    /* after the final shift the newest values sit in time layer 1 (Grid.h:1304-1319) */
+#define DUMP_IF(need, name, getter) if (need) putRecord (name, dumpGrid (is->getter (), 1));
+    DUMP_IF (is->getDoNeedEx (), "Ex", getEx)
+    DUMP_IF (is->getDoNeedEy (), "Ey", getEy)
+    DUMP_IF (is->getDoNeedEz (), "Ez", getEz)
+    DUMP_IF (is->getDoNeedHx (), "Hx", getHx)
+    DUMP_IF (is->getDoNeedHy (), "Hy", getHy)
+    DUMP_IF (is->getDoNeedHz (), "Hz", getHz)
+    DUMP_IF (pml && is->getDoNeedEx (), "Dx", getDx)
+    DUMP_IF (pml && is->getDoNeedEy (), "Dy", getDy)
+    DUMP_IF (pml && is->getDoNeedEz (), "Dz", getDz)
+    DUMP_IF (pml && is->getDoNeedHx (), "Bx", getBx)
+    DUMP_IF (pml && is->getDoNeedHy (), "By", getBy)
+    DUMP_IF (pml && is->getDoNeedHz (), "Bz", getBz)
+    DUMP_IF (meta && is->getDoNeedEx (), "D1x", getD1x)
+    DUMP_IF (meta && is->getDoNeedEy (), "D1y", getD1y)
+    DUMP_IF (meta && is->getDoNeedEz (), "D1z", getD1z)
+    DUMP_IF (meta && is->getDoNeedHx (), "B1x", getB1x)
+    DUMP_IF (meta && is->getDoNeedHy (), "B1y", getB1y)
+    DUMP_IF (meta && is->getDoNeedHz (), "B1z", getB1z)
+#undef DUMP_IF
+    putRecord ("Eps", dumpGrid (is->getEps (), 0));
+    putRecord ("Mu", dumpGrid (is->getMu (), 0));
+    if (pml)
+    {
+      putRecord ("SigmaX", dumpGrid (is->getSigmaX (), 0));
+      putRecord ("SigmaY", dumpGrid (is->getSigmaY (), 0));
+      if (dim == 3)
+      {
+        putRecord ("SigmaZ", dumpGrid (is->getSigmaZ (), 0));
+      }
+    }
+    if (meta)
+    {
+      putRecord ("OmegaPE", dumpGrid (is->getOmegaPE (), 0));
+      putRecord ("OmegaPM", dumpGrid (is->getOmegaPM (), 0));
+      putRecord ("GammaE", dumpGrid (is->getGammaE (), 0));
+      putRecord ("GammaM", dumpGrid (is->getGammaM (), 0));
+    }
+    if (SOLVER_SETTINGS.getDoUseTFSF ())
+    {
+      std::vector<double> e, h;
+      for (grid_coord i = 0; i < is->getEInc ()->getSize ().get1 (); ++i)
+      {
+        GridCoordinate1D p = GRID_COORDINATE_1D (i, CoordinateType::X);
+        e.push_back ((double) *is->getEInc ()->getFieldValue (p, 1));
+        h.push_back ((double) *is->getHInc ()->getFieldValue (p, 1));
+      }
+      putRecord ("EInc", e);
+      putRecord ("HInc", h);
+    }
+    std::vector<double> scal;
+    scal.push_back ((double) is->getGridTimeStep ());
+    scal.push_back ((double) is->getSourceFrequency ());
+    putRecord ("scalars", scal);
+  }
+};
+
+#ifdef MODE_DIM3
+template <>
+grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate>::sizeZ () const
+{
+  return size.get3 ();
+}
+template <>
+grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate>::pmlZ () const
+{
+  return layout->getLeftBorderPML ().get3 ();
+}
+template <>
+FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate>::fpZ (const GridCoordinateFP3D &c) const
+{
+  return c.get3 ();
+}
+#endif
+#ifdef MODE_TMZ
+template <>
+grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate>::sizeZ () const
+{
+  return 1;
+}
+template <>
+grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate>::pmlZ () const
+{
+  return 0;
+}
+template <>
+FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate>::fpZ (const GridCoordinateFP2D &) const
+{
+  return 0;
+}
+#endif
+
+} /* namespace */
+
+int main (int argc, char **argv)
+{
+  solverSettings.SetupFromCmd (argc, argv);
+  solverSettings.Initialize ();
+  if (SOLVER_SETTINGS.getLayoutType () != E_CENTERED)
+  {
+    fprintf (stderr, "only --layout-type 0 (E_CENTERED) is driven\n");
+    return 3;
+  }
+  const char *outName = getenv ("YEE_REF_OUT");
+  g_out = fopen (outName ? outName : "ref_out.bin", "wb");
+  if (!g_out)
+  {
+    return 4;
+  }
+  double secs = 0;
+  SchemeType st = SOLVER_SETTINGS.getSchemeType ();
+#ifdef MODE_DIM3
+  if (st == SchemeType::Dim3)
+  {
+    GridCoordinate3D s = GRID_COORDINATE_3D (SOLVER_SETTINGS.getSizeX (), SOLVER_SETTINGS.getSizeY (), SOLVER_SETTINGS.getSizeZ (),
+                                             CoordinateType::X, CoordinateType::Y, CoordinateType::Z);
+    GridCoordinate3D p = GRID_COORDINATE_3D (SOLVER_SETTINGS.getPMLSizeX (), SOLVER_SETTINGS.getPMLSizeY (), SOLVER_SETTINGS.getPMLSizeZ (),
+                                             CoordinateType::X, CoordinateType::Y, CoordinateType::Z);
+    GridCoordinate3D l = GRID_COORDINATE_3D (SOLVER_SETTINGS.getTFSFSizeLeftX (), SOLVER_SETTINGS.getTFSFSizeLeftY (), SOLVER_SETTINGS.getTFSFSizeLeftZ (),
+                                             CoordinateType::X, CoordinateType::Y, CoordinateType::Z);
+    GridCoordinate3D r = GRID_COORDINATE_3D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), SOLVER_SETTINGS.getTFSFSizeRightZ (),
+                                             CoordinateType::X, CoordinateType::Y, CoordinateType::Z);
+    RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate> run (s, p, l, r, 3);
+    run.initMaterials ();
+    secs = run.run ();
+    run.dump ();
+  }
+  else
+#endif
+#ifdef MODE_TMZ
+  if (st == SchemeType::Dim2_TMz)
+  {
+    GridCoordinate2D s = GRID_COORDINATE_2D (SOLVER_SETTINGS.getSizeX (), SOLVER_SETTINGS.getSizeY (), CoordinateType::X, CoordinateType::Y);
+    GridCoordinate2D p = GRID_COORDINATE_2D (SOLVER_SETTINGS.getPMLSizeX (), SOLVER_SETTINGS.getPMLSizeY (), CoordinateType::X, CoordinateType::Y);
+    GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeftX (), SOLVER_SETTINGS.getTFSFSizeLeftY (), CoordinateType::X, CoordinateType::Y);
+    GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), CoordinateType::X, CoordinateType::Y);
+    RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate> run (s, p, l, r, 2);
+    run.initMaterials ();
+    secs = run.run ();
+    run.dump ();
+  }
+  else
+#endif
+  {
+    fprintf (stderr, "scheme not driven by this build\n");
+    return 5;
+  }
+  fclose (g_out);
+  printf ("Total time = %f seconds\n", secs);
+  solverSettings.Uninitialize ();
+  return 0;
+}

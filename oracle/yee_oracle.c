@@ -1,0 +1,933 @@
+/*
+ * yee_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker; never shipped or measured).
+ *
+ * FP64 CPU restatement of the reference's Yee update path (E_CENTERED layout,
+ * --3d and --2d-tmz schemes, real fields).  Each function cites the reference code
+ * it restates; expressions keep the reference's operand order so that, compiled
+ * without FP contraction, the results are bit-identical to the reference's own
+ * arithmetic (checked against oracle/_ref in tests/test_oracle_ref.py).
+ *
+ * Storage mirrors the reference: every state grid has `layers` time layers
+ * (2, or 3 with metamaterials + PML: Source/Scheme/InternalScheme.specific.h:83-88);
+ * layer 0 is written by the current update, layer 1 is the newest completed one;
+ * shiftInTime rotates the layers (Source/Grid/Grid.h:1304-1319).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "yee_oracle.h"
+
+/* Source/Physics/PhysicsConst.h:27-33 */
+#define O_PI 3.14159265358979323846   /* M_PI of glibc, as PhysicsConst::Pi = M_PI */
+#define O_C 299792458.0
+
+static double o_mu0 (void) { return 4 * O_PI * 0.0000001; }
+static double o_eps0 (void) { return 1 / (o_mu0 () * (O_C * O_C)); }
+
+enum { LEFT = 0, RIGHT = 1, DOWN = 2, UP = 3, BACK = 4, FRONT = 5 };
+
+/*
+ * Per-component bindings of InternalScheme::calculateFieldStepInit
+ * (Source/Scheme/InternalScheme.inc.h:1902-2295) and the E_CENTERED circuit offsets
+ * (Source/Layout/YeeGridLayout.cpp:1937-1961) used by calculateFieldStepInitDiff (:1837-1900).
+ * sig_curl/sig4/sig5: sigma axis of materialGrid / materialGrid4 / materialGrid5.
+ */
+typedef struct
+{
+  int axis, is_h, opp1, opp2;
+  int d11[3], d12[3], d21[3], d22[3];
+  int dir11, dir12, dir21, dir22;
+  int sig_curl, sig4, sig5;
+} comp_desc;
+
+static const comp_desc CD[6] = {
+  /* Ex: Hz (DOWN/UP), Hy (BACK/FRONT); SigmaY, SigmaX, SigmaZ */
+  { 0, 0, YEE_HZ, YEE_HY, {0, -1, 0}, {0, 0, 0}, {0, 0, -1}, {0, 0, 0}, DOWN, UP, BACK, FRONT, 1, 0, 2 },
+  /* Ey: Hx (BACK/FRONT), Hz (LEFT/RIGHT); SigmaZ, SigmaY, SigmaX */
+  { 1, 0, YEE_HX, YEE_HZ, {0, 0, -1}, {0, 0, 0}, {-1, 0, 0}, {0, 0, 0}, BACK, FRONT, LEFT, RIGHT, 2, 1, 0 },
+  /* Ez: Hy (LEFT/RIGHT), Hx (DOWN/UP); SigmaX, SigmaZ, SigmaY */
+  { 2, 0, YEE_HY, YEE_HX, {-1, 0, 0}, {0, 0, 0}, {0, -1, 0}, {0, 0, 0}, LEFT, RIGHT, DOWN, UP, 0, 2, 1 },
+  /* Hx: Ey (BACK/FRONT), Ez (DOWN/UP); SigmaY, SigmaX, SigmaZ */
+  { 0, 1, YEE_EY, YEE_EZ, {0, 0, 0}, {0, 0, 1}, {0, 0, 0}, {0, 1, 0}, BACK, FRONT, DOWN, UP, 1, 0, 2 },
+  /* Hy: Ez (LEFT/RIGHT), Ex (BACK/FRONT); SigmaZ, SigmaY, SigmaX */
+  { 1, 1, YEE_EZ, YEE_EX, {0, 0, 0}, {1, 0, 0}, {0, 0, 0}, {0, 0, 1}, LEFT, RIGHT, BACK, FRONT, 2, 1, 0 },
+  /* Hz: Ex (DOWN/UP), Ey (LEFT/RIGHT); SigmaX, SigmaZ, SigmaY */
+  { 2, 1, YEE_EX, YEE_EY, {0, 0, 0}, {0, 1, 0}, {0, 0, 0}, {1, 0, 0}, DOWN, UP, LEFT, RIGHT, 0, 2, 1 },
+};
+
+struct oracle
+{
+  yee_config cfg;
+  int dim;                 /* 3 or 2 */
+  int64_t n[3];            /* n[2] == 1 for TMz */
+  int64_t ncell;
+  int present[6];
+  int use_pml, use_tfsf, use_meta;
+  int layers;
+  double dx, dt, freq, rel, courant, lambda;
+  double a1, a2, a3;
+  double zero_inc[3];
+  double bl[3], br[3];     /* TF/SF borders as FP coordinates */
+  double *mat[6];          /* eps, mu, omegaPE, omegaPM, gammaE, gammaM */
+  double *sigma[3];
+  double *f[6][3];         /* E/H layers */
+  double *d[6][3];         /* D/B layers (PML) */
+  double *d1[6][3];        /* D1/B1 layers (metamaterials) */
+  double *einc[2], *hinc[2];
+  int64_t inc_len;
+  int32_t t;
+  int prepared;
+};
+
+static int64_t o_idx (const oracle *o, int64_t i, int64_t j, int64_t k)
+{
+  return (i * o->n[1] + j) * o->n[2] + k;
+}
+
+/* Minimum real coordinate of a component, E_CENTERED (Source/Layout/YeeGridLayout.cpp:1962-1969). */
+static double o_min_coord (int comp, int axis)
+{
+  const comp_desc *c = &CD[comp];
+  if (!c->is_h)
+  {
+    return axis == c->axis ? 1.0 : 0.5;
+  }
+  return axis == c->axis ? 0.5 : 1.0;
+}
+
+static void o_real_coord (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, double out[3])
+{
+  int64_t p[3] = { i, j, k };
+  for (int a = 0; a < 3; ++a)
+  {
+    out[a] = a < o->dim ? (double) p[a] + o_min_coord (comp, a) : 0.0;
+  }
+}
+
+/* getEpsCoord: integer coordinate of a real coordinate on the eps grid (minEpsCoordFP = 0.5). */
+static void o_eps_index (const oracle *o, const double real[3], int64_t out[3])
+{
+  for (int a = 0; a < 3; ++a)
+  {
+    out[a] = a < o->dim ? (int64_t) (real[a] - 0.5) : 0;
+  }
+}
+
+static double o_mval (const oracle *o, const double *grid, const int64_t p[3])
+{
+  return grid[o_idx (o, p[0], p[1], p[2])];
+}
+
+/*
+ * Neighbour eps-grid cells that InternalScheme::getMaterial averages for a field point
+ * (YeeGridLayout::initMaterialCoordinates, Source/Layout/YeeGridLayout.h:806-1041,
+ * non-double-precision branches).  E components: 2 cells along their own axis; H
+ * components: 4 cells in the transverse plane.  Offsets along an axis the scheme does
+ * not have are dropped, as TC::initAxesCoordinate does for 2D coordinates.
+ */
+static int o_material_cells (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, int64_t cells[4][3])
+{
+  const comp_desc *c = &CD[comp];
+  double real[3];
+  o_real_coord (o, comp, i, j, k, real);
+  if (!c->is_h)
+  {
+    double lo[3] = { real[0], real[1], real[2] };
+    double hi[3] = { real[0], real[1], real[2] };
+    if (c->axis < o->dim)
+    {
+      lo[c->axis] -= 0.5;
+      hi[c->axis] += 0.5;
+    }
+    o_eps_index (o, lo, cells[0]);
+    o_eps_index (o, hi, cells[1]);
+    return 2;
+  }
+  int b = c->axis == 0 ? 1 : 0;
+  int cc = c->axis == 2 ? 1 : 2;
+  double sb[4] = { -0.5, -0.5, 0.5, 0.5 };
+  double sc[4] = { -0.5, 0.5, -0.5, 0.5 };
+  for (int q = 0; q < 4; ++q)
+  {
+    double r[3] = { real[0], real[1], real[2] };
+    if (b < o->dim) r[b] += sb[q];
+    if (cc < o->dim) r[cc] += sc[q];
+    o_eps_index (o, r, cells[q]);
+  }
+  return 4;
+}
+
+/* InternalScheme::getMaterial + Approximation::approximateMaterial (Approximation.cpp:36-47). */
+static double o_get_material (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, const double *grid)
+{
+  if (!grid)
+  {
+    return 0;
+  }
+  int64_t cells[4][3];
+  int nc = o_material_cells (o, comp, i, j, k, cells);
+  if (nc == 2)
+  {
+    return (o_mval (o, grid, cells[0]) + o_mval (o, grid, cells[1])) / 2.0;
+  }
+  double a = (o_mval (o, grid, cells[0]) + o_mval (o, grid, cells[1])) / 2.0;
+  double b = (o_mval (o, grid, cells[2]) + o_mval (o, grid, cells[3])) / 2.0;
+  return (a + b) / 2.0;
+}
+
+/* InternalScheme::getMetaMaterial + Approximation::approximateDrudeModel (Approximation.cpp:58-167). */
+static double o_get_meta (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, const double *eps,
+                          const double *om, const double *ga, double *omega, double *gamma)
+{
+  int64_t cells[4][3];
+  int nc = o_material_cells (o, comp, i, j, k, cells);
+  if (nc == 2)
+  {
+    *omega = (o_mval (o, om, cells[0]) + o_mval (o, om, cells[1])) / 2.0;
+    *gamma = (o_mval (o, ga, cells[0]) + o_mval (o, ga, cells[1])) / 2.0;
+    return (o_mval (o, eps, cells[0]) + o_mval (o, eps, cells[1])) / 2.0;
+  }
+  *omega = (o_mval (o, om, cells[0]) + o_mval (o, om, cells[1]) + o_mval (o, om, cells[2]) + o_mval (o, om, cells[3])) / 4.0;
+  *gamma = (o_mval (o, ga, cells[0]) + o_mval (o, ga, cells[1]) + o_mval (o, ga, cells[2]) + o_mval (o, ga, cells[3])) / 4.0;
+  double a = (o_mval (o, eps, cells[0]) + o_mval (o, eps, cells[1])) / 2.0;
+  double b = (o_mval (o, eps, cells[2]) + o_mval (o, eps, cells[3])) / 2.0;
+  return (a + b) / 2.0;
+}
+
+/* --- TF/SF predicates: YeeGridLayoutHelper::tfsfBorder (Source/Layout/YeeGridLayout.h:498-547) --- */
+
+static int o_fp_exact (double a, double b)
+{
+  return (a > b ? a - b : b - a) < 0.0000001;   /* IS_FP_EXACT, FPEXACT_ACCURACY (FieldValue.h:150-160) */
+}
+
+static int o_border_eq (double c, double border, int off)
+{
+  return o_fp_exact (c, border + off * 1.0 / 10.0);
+}
+
+static int o_border_range (double c, double b1, int o1, double b2, int o2)
+{
+  return c > b1 + o1 * 1.0 / 10.0 && c < b2 + o2 * 1.0 / 10.0;
+}
+
+/*
+ * doNeedTFSFUpdate<comp>Border(pos, dir), E_CENTERED, Dim3
+ * (Source/Layout/YeeGridLayout.cpp:272-306, 558-596, 844-882, 1130-1168, 1416-1454, 1702-1740)
+ * and Dim2_TMz (:817-826, 1103-1112, 1389-1398).  All 24 3D predicates share one shape:
+ * for an E component of axis a and a direction along axis b, the coordinate along b
+ * equals left-0.5 / right+0.5, the coordinate along a lies within [left-0.1, right+0.1]
+ * and the third coordinate within [left+0.4, right-0.4]; for H components the b
+ * coordinate equals left / right, a lies within [left+0.4, right-0.4] and the third
+ * within [left-0.1, right+0.1].  The TMz predicates are the same with the z test
+ * removed (tfsfBorder2D*, YeeGridLayout.h:520-547).
+ */
+static int o_tfsf_pred (const oracle *o, int comp, int dir, const double real[3])
+{
+  const comp_desc *c = &CD[comp];
+  int b = dir / 2;
+  int low = (dir % 2) == 0;
+  if (b == c->axis || b >= o->dim)
+  {
+    return 0;
+  }
+  int third = 3 - b - c->axis;
+  int cond_b, cond_a = 1, cond_t = 1;
+  if (!c->is_h)
+  {
+    cond_b = low ? o_border_eq (real[b], o->bl[b], -5) : o_border_eq (real[b], o->br[b], 5);
+    if (c->axis < o->dim) cond_a = o_border_range (real[c->axis], o->bl[c->axis], -1, o->br[c->axis], 1);
+    if (third < o->dim) cond_t = o_border_range (real[third], o->bl[third], 4, o->br[third], -4);
+  }
+  else
+  {
+    cond_b = low ? o_border_eq (real[b], o->bl[b], 0) : o_border_eq (real[b], o->br[b], 0);
+    if (c->axis < o->dim) cond_a = o_border_range (real[c->axis], o->bl[c->axis], 4, o->br[c->axis], -4);
+    if (third < o->dim) cond_t = o_border_range (real[third], o->bl[third], -1, o->br[third], 1);
+  }
+  return cond_b && cond_a && cond_t;
+}
+
+/* Incident-field projections get<comp>FromIncident{E,H} (Source/Layout/YeeGridLayout.h:2194-2356). */
+static double o_project (const oracle *o, int comp, double v)
+{
+  double a1 = o->a1, a2 = o->a2, a3 = o->a3;
+  if (o->dim == 2)
+  {
+    switch (comp)
+    {
+      case YEE_EZ: return v;
+      case YEE_HX: return v * (sin (a2));
+      case YEE_HY: return v * (-cos (a2));
+      default: return 0;
+    }
+  }
+  switch (comp)
+  {
+    case YEE_EX: return v * (cos (a3) * sin (a2) - sin (a3) * cos (a1) * cos (a2));
+    case YEE_EY: return v * (-cos (a3) * cos (a2) - sin (a3) * cos (a1) * sin (a2));
+    case YEE_EZ: return v * (sin (a3) * sin (a1));
+    case YEE_HX: return v * (sin (a3) * sin (a2) + cos (a3) * cos (a1) * cos (a2));
+    case YEE_HY: return v * (-sin (a3) * cos (a2) + cos (a3) * cos (a1) * sin (a2));
+    case YEE_HZ: return -v * (cos (a3) * sin (a1));
+    default: return 0;
+  }
+}
+
+/*
+ * approximateIncidentWave (Source/Scheme/InternalScheme.cpp:375-407) and
+ * approximateIncidentWaveHelper (Source/Scheme/InternalScheme.inc.h:500-515):
+ * projection of the point onto the propagation direction, linear interpolation
+ * of layer 1 of the 1D grid at the truncated index.
+ */
+static double o_incident (const oracle *o, const double real[3], double ddiff, const double *inc)
+{
+  double d;
+  double x = real[0] - o->zero_inc[0];
+  double y = real[1] - o->zero_inc[1];
+  if (o->dim == 3)
+  {
+    double z = real[2] - o->zero_inc[2];
+    d = x * sin (o->a1) * cos (o->a2) + y * sin (o->a1) * sin (o->a2) + z * cos (o->a1) - ddiff;
+  }
+  else
+  {
+    d = x * cos (o->a2) + y * sin (o->a2) - ddiff;
+  }
+  double c1 = (double) ((int64_t) d);
+  double c2 = c1 + 1;
+  double p2 = d - c1;
+  double p1 = 1 - p2;
+  double v1 = inc[(int64_t) c1];
+  double v2 = inc[(int64_t) c2];
+  return v1 * p1 + v2 * p2;
+}
+
+/* Approximation::approximateWaveNumber{,General} (Source/Layout/Approximation.cpp:259-339). */
+double oracle_wave_number (double delta, double lambda, double courant, double nl, double a1, double a2)
+{
+  const double pi = O_PI;
+  if ((a1 == 0 || a1 == pi / 2 || a1 == pi || a1 == 3 * pi / 2)
+      && (a2 == 0 || a2 == pi / 2 || a2 == pi || a2 == 3 * pi / 2))
+  {
+    return 2.0 / delta * asin (sin (pi * courant / nl) / courant);
+  }
+  if (((a1 == 0 || a1 == pi / 2 || a1 == pi || a1 == 3 * pi / 2)
+       && (a2 == pi / 4 || a2 == 3 * pi / 4 || a2 == 5 * pi / 4 || a2 == 7 * pi / 4))
+      || ((a1 == pi / 4 || a1 == 3 * pi / 4 || a1 == 5 * pi / 4 || a1 == 7 * pi / 4)
+          && (a2 == 0 || a2 == pi / 2 || a2 == pi || a2 == 3 * pi / 2)))
+  {
+    return 2.0 * sqrt (2.0) / delta * asin (sin (pi * courant / nl) / (courant * sqrt (2.0)));
+  }
+  const double acc = 0.0000001;
+  double k = 2 * pi;
+  double k_prev = k + 2 * sqrt (acc);
+  double nd = delta / lambda;
+  double A = nd * sin (a1) * cos (a2) / 2;
+  double B = nd * sin (a1) * sin (a2) / 2;
+  double C = nd * cos (a1) / 2;
+  double s = sin (pi * courant / nl);
+  double D = (s * s) / (courant * courant);
+  while ((k_prev - k) * (k_prev - k) >= acc)
+  {
+    k_prev = k;
+    double sa = sin (A * k), sb = sin (B * k), sc = sin (C * k);
+    double diff1 = sa * sa + sb * sb + sc * sc - D;
+    double diff2 = A * sin (2 * A * k) + B * sin (2 * B * k) + C * sin (2 * C * k);
+    double diff = diff1 / diff2;
+    k -= diff;
+  }
+  return k / lambda;
+}
+
+/* Approximation::phaseVelocityIncidentWave (Approximation.cpp:341-353). */
+static double o_phase_velocity (double delta, double lambda, double courant, double nl, double a1, double a2)
+{
+  return O_C * 2 * O_PI / lambda / oracle_wave_number (delta, lambda, courant, nl, a1, a2);
+}
+
+/* SchemeHelper::initSigma (Source/Scheme/SchemeHelper.h:574-590). */
+static double o_sigma_value (double dist, double boundary, double dx)
+{
+  double eps0 = o_eps0 ();
+  double mu0 = o_mu0 ();
+  unsigned exponent = 6;
+  double r_err = 1e-16;
+  double sigma_max_1 = -log (r_err) * (exponent + 1.0) / (2.0 * sqrt (mu0 / eps0) * boundary);
+  double factor = sigma_max_1 / (dx * (pow (boundary, (double) exponent)) * (exponent + 1));
+  double x1 = (dist + 1) * dx;
+  double x2 = dist * dx;
+  return factor * (pow (x1, (double) (exponent + 1)) - pow (x2, (double) (exponent + 1)));
+}
+
+/* Approximation::approximateSphereAccurate, 3D (Source/Layout/Approximation.cpp:563-676). */
+double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,
+                                  double radius, double eps, double outside, int accuracy)
+{
+  double sx = mx - 0.5, sy = my - 0.5, sz = mz - 0.5;
+  double ez = mz + 0.5;
+  int num = accuracy;
+  double step = 1.0 / num;
+  double elem = step * step;
+  double volume = 0;
+  for (int i = 0; i < num; ++i)
+  {
+    for (int j = 0; j < num; ++j)
+    {
+      double px = sx + i * step;
+      double py = sy + j * step;
+      double pz;
+      double temp = radius * radius - (px - cx) * (px - cx) - (py - cy) * (py - cy);
+      if (temp < 0)
+      {
+        pz = 0.0;
+      }
+      else
+      {
+        int above;
+        if (mz > cz)
+        {
+          above = 1;
+          pz = cz + sqrt (temp);
+        }
+        else
+        {
+          above = 0;
+          pz = cz - sqrt (temp);
+        }
+        if (pz < sz) pz = sz;
+        else if (pz > ez) pz = ez;
+        pz = above ? pz - sz : ez - pz;
+      }
+      volume += pz * elem;
+    }
+  }
+  if (volume > 1.0) volume = 1.0;
+  return eps * volume + outside * (1 - volume);
+}
+
+/* Approximation::approximateSphereAccurate, 2D (Source/Layout/Approximation.cpp:471-561). */
+double oracle_sphere_accurate_2d (double mx, double my, double cx, double cy,
+                                  double radius, double eps, double outside, int accuracy)
+{
+  double sx = mx - 0.5, sy = my - 0.5, ey = my + 0.5;
+  int num = accuracy;
+  double step = 1.0 / num;
+  double elem = step;
+  double volume = 0;
+  for (int i = 0; i < num; ++i)
+  {
+    double px = sx + i * step;
+    double py;
+    double temp = radius * radius - (px - cx) * (px - cx);
+    if (temp < 0)
+    {
+      py = 0.0;
+    }
+    else
+    {
+      int above;
+      if (my > cy)
+      {
+        above = 1;
+        py = cy + sqrt (temp);
+      }
+      else
+      {
+        above = 0;
+        py = cy - sqrt (temp);
+      }
+      if (py < sy) py = sy;
+      else if (py > ey) py = ey;
+      py = above ? py - sy : ey - py;
+    }
+    volume += py * elem;
+  }
+  if (volume > 1.0) volume = 1.0;
+  return eps * volume + outside * (1 - volume);
+}
+
+/* Approximation::approximateSphereStair, 3D (Source/Layout/Approximation.cpp:730-764). */
+double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, double cy, double cz,
+                               double radius, double eps, double outside)
+{
+  for (int q = 0; q < 8; ++q)
+  {
+    double x = mx + ((q & 1) ? 0.5 : -0.5);
+    double y = my + ((q & 2) ? 0.5 : -0.5);
+    double z = mz + ((q & 4) ? 0.5 : -0.5);
+    double dist = sqrt ((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+    if (!(dist <= radius))
+    {
+      return outside;
+    }
+  }
+  return eps;
+}
+
+oracle *oracle_create (const yee_config *cfg)
+{
+  if (!cfg || cfg->layout != 0 || (cfg->scheme != YEE_SCHEME_DIM3 && cfg->scheme != YEE_SCHEME_TMZ))
+  {
+    return NULL;
+  }
+  oracle *o = (oracle *) calloc (1, sizeof (oracle));
+  o->cfg = *cfg;
+  o->dim = cfg->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  o->n[0] = cfg->size[0];
+  o->n[1] = cfg->size[1];
+  o->n[2] = o->dim == 3 ? cfg->size[2] : 1;
+  o->ncell = o->n[0] * o->n[1] * o->n[2];
+  o->use_pml = (cfg->flags & YEE_USE_PML) != 0;
+  o->use_tfsf = (cfg->flags & YEE_USE_TFSF) != 0;
+  o->use_meta = (cfg->flags & YEE_USE_METAMATERIALS) != 0;
+  if (o->use_meta && !o->use_pml)
+  {
+    free (o);
+    return NULL;   /* Scheme.inc.h:889-892: metamaterials without PML are unreachable */
+  }
+  o->layers = (o->use_meta && o->use_pml) ? 3 : 2;
+  for (int c = 0; c < 6; ++c)
+  {
+    o->present[c] = o->dim == 3 ? 1 : (c == YEE_EZ || c == YEE_HX || c == YEE_HY);
+  }
+  for (int m = 0; m < 6; ++m)
+  {
+    o->mat[m] = (double *) malloc (sizeof (double) * o->ncell);
+    double v = m < 2 ? 1.0 : 0.0;
+    for (int64_t q = 0; q < o->ncell; ++q) o->mat[m][q] = v;
+  }
+  for (int c = 0; c < 6; ++c)
+  {
+    if (!o->present[c]) continue;
+    for (int l = 0; l < o->layers; ++l)
+    {
+      o->f[c][l] = (double *) calloc (o->ncell, sizeof (double));
+      if (o->use_pml) o->d[c][l] = (double *) calloc (o->ncell, sizeof (double));
+      if (o->use_meta) o->d1[c][l] = (double *) calloc (o->ncell, sizeof (double));
+    }
+  }
+  /* InternalScheme::initScheme (Source/Scheme/InternalScheme.inc.h:3359-3394) */
+  o->dx = cfg->dx;
+  o->lambda = cfg->wavelength;
+  o->courant = cfg->courant;
+  o->freq = O_C / o->lambda;
+  o->dt = o->dx * o->courant / O_C;
+  /* angles as main.cpp:1101-1110 converts them */
+  o->a1 = cfg->angle_deg[0] * O_PI / 180.0;
+  o->a2 = cfg->angle_deg[1] * O_PI / 180.0;
+  o->a3 = cfg->angle_deg[2] * O_PI / 180.0;
+  double nl = o->lambda / o->dx;
+  double pv0 = o_phase_velocity (o->dx, o->lambda, o->courant, nl, O_PI / 2, 0);
+  double pv = o_phase_velocity (o->dx, o->lambda, o->courant, nl, o->a1, o->a2);
+  o->rel = pv0 / pv;
+  /* TF/SF borders and zeroIncCoordFP (YeeGridLayout.cpp:1954-1958, 2583-2599, 2636-2656) */
+  for (int a = 0; a < 3; ++a)
+  {
+    o->bl[a] = 0.0 + (double) cfg->tfsf_left[a];
+    o->br[a] = 0.0 + (double) (cfg->size[a] - cfg->tfsf_right[a]);
+  }
+  if (o->dim == 3)
+  {
+    o->zero_inc[0] = cfg->tfsf_left[0] - 2.5 * sin (o->a1) * cos (o->a2);
+    o->zero_inc[1] = cfg->tfsf_left[1] - 2.5 * sin (o->a1) * sin (o->a2);
+    o->zero_inc[2] = cfg->tfsf_left[2] - 2.5 * cos (o->a1);
+  }
+  else
+  {
+    o->zero_inc[0] = cfg->tfsf_left[0] - 2.5 * cos (o->a2);
+    o->zero_inc[1] = cfg->tfsf_left[1] - 2.5 * sin (o->a2);
+    o->zero_inc[2] = 0;
+  }
+  if (o->use_tfsf)
+  {
+    /* allocateGridsInc: 500 * size.x cells, 2 layers (InternalScheme.specific.h:99-107) */
+    o->inc_len = 500 * o->n[0];
+    for (int l = 0; l < 2; ++l)
+    {
+      o->einc[l] = (double *) calloc (o->inc_len, sizeof (double));
+      o->hinc[l] = (double *) calloc (o->inc_len, sizeof (double));
+    }
+  }
+  return o;
+}
+
+void oracle_destroy (oracle *o)
+{
+  if (!o) return;
+  for (int m = 0; m < 6; ++m) free (o->mat[m]);
+  for (int a = 0; a < 3; ++a) free (o->sigma[a]);
+  for (int c = 0; c < 6; ++c)
+    for (int l = 0; l < 3; ++l)
+    {
+      free (o->f[c][l]);
+      free (o->d[c][l]);
+      free (o->d1[c][l]);
+    }
+  for (int l = 0; l < 2; ++l)
+  {
+    free (o->einc[l]);
+    free (o->hinc[l]);
+  }
+  free (o);
+}
+
+int oracle_set_material (oracle *o, int32_t which, const double *values)
+{
+  if (!o || !values || which < 0 || which > 5) return -1;
+  memcpy (o->mat[which], values, sizeof (double) * o->ncell);
+  return 0;
+}
+
+/* SchemeHelper::initSigmaX/Y/Z (Source/Scheme/SchemeHelper.h:592-718); x and y/z differ on purpose. */
+int oracle_prepare (oracle *o)
+{
+  if (!o) return -1;
+  if (o->use_pml)
+  {
+    for (int a = 0; a < o->dim; ++a)
+    {
+      o->sigma[a] = (double *) calloc (o->ncell, sizeof (double));
+      double P = (double) o->cfg.pml[a];
+      double boundary = P * o->dx;
+      for (int64_t i = 0; i < o->n[0]; ++i)
+        for (int64_t j = 0; j < o->n[1]; ++j)
+          for (int64_t k = 0; k < o->n[2]; ++k)
+          {
+            int64_t p[3] = { i, j, k };
+            double pos = (double) p[a] + 0.5;   /* getEpsCoordFP */
+            double val = 0;
+            if (a == 0)
+            {
+              double size = (double) o->n[0];
+              if (pos < P) val = o_sigma_value (P - pos, boundary, o->dx);
+              else if (pos >= size - P) val = o_sigma_value (pos - (size - P), boundary, o->dx);
+            }
+            else
+            {
+              double size = (double) o->n[a] + 0.5;
+              if (pos < P) val = o_sigma_value ((double) (int64_t) (P - pos), boundary, o->dx);
+              else if (pos >= size - P) val = o_sigma_value ((double) (int64_t) (pos - (size - P)), boundary, o->dx);
+            }
+            o->sigma[a][o_idx (o, i, j, k)] = val;
+          }
+    }
+  }
+  o->prepared = 1;
+  return 0;
+}
+
+static void o_rotate (double **layers, int n)
+{
+  double *oldest = layers[n - 1];
+  for (int l = n - 1; l >= 1; --l) layers[l] = layers[l - 1];
+  layers[0] = oldest;
+}
+
+/* performPlaneWaveESteps (Source/Scheme/InternalScheme.inc.h:3276-3318). */
+static void o_inc_e (oracle *o, int32_t t)
+{
+  double mod = o->dt / (o->rel * o_eps0 () * o->dx);
+  double *e0 = o->einc[0], *e1 = o->einc[1], *h1 = o->hinc[1];
+  for (int64_t i = 1; i < o->inc_len; ++i)
+  {
+    e0[i] = e1[i] + (h1[i - 1] - h1[i]) * mod;
+  }
+  double arg = o->dt * t * 2 * O_PI * o->freq;
+  e0[0] = sin (arg);
+}
+
+/* performPlaneWaveHSteps (Source/Scheme/InternalScheme.inc.h:3320-3357); the last cell is skipped. */
+static void o_inc_h (oracle *o)
+{
+  double mod = o->dt / (o->rel * o_mu0 () * o->dx);
+  double *h0 = o->hinc[0], *h1 = o->hinc[1], *e1 = o->einc[1];
+  for (int64_t i = 0; i < o->inc_len - 1; ++i)
+  {
+    h0[i] = h1[i] + (e1[i] - e1[i + 1]) * mod;
+  }
+}
+
+/* Reads layer 1 of a present grid at an offset position; absent grids read as 0. */
+static double o_read1 (const oracle *o, int comp, int64_t i, int64_t j, int64_t k)
+{
+  if (!o->present[comp]) return 0;
+  return o->f[comp][1][o_idx (o, i, j, k)];
+}
+
+/* calculateTFSF (Source/Scheme/InternalScheme.inc.h:1386-1835), E_CENTERED. */
+static void o_tfsf (const oracle *o, int comp, int64_t i, int64_t j, int64_t k,
+                    double *v11, double *v12, double *v21, double *v22)
+{
+  const comp_desc *c = &CD[comp];
+  double real[3];
+  o_real_coord (o, comp, i, j, k, real);
+  int u11 = o_tfsf_pred (o, comp, c->dir11, real);
+  int u12 = o_tfsf_pred (o, comp, c->dir12, real);
+  int u21 = o_tfsf_pred (o, comp, c->dir21, real);
+  int u22 = o_tfsf_pred (o, comp, c->dir22, real);
+  int revert = !c->is_h;   /* E_CENTERED: E components revert, H do not */
+  const double *inc = c->is_h ? o->einc[1] : o->hinc[1];
+  double ddiff = c->is_h ? 0.0 : 0.5;
+  double diff1 = 0, diff2 = 0;
+  const int *p1 = NULL, *p2 = NULL;
+  if (revert)
+  {
+    if (u11) p1 = c->d12; else if (u12) p1 = c->d11;
+    if (u21) p2 = c->d22; else if (u22) p2 = c->d21;
+  }
+  else
+  {
+    if (u11) p1 = c->d11; else if (u12) p1 = c->d12;
+    if (u21) p2 = c->d21; else if (u22) p2 = c->d22;
+  }
+  if (u11 || u12)
+  {
+    double r[3];
+    o_real_coord (o, c->opp1, i + p1[0], j + p1[1], k + (o->dim == 3 ? p1[2] : 0), r);
+    diff1 = o_project (o, c->opp1, o_incident (o, r, ddiff, inc));
+    if (c->is_h) diff1 *= -1.0;
+  }
+  if (u21 || u22)
+  {
+    double r[3];
+    o_real_coord (o, c->opp2, i + p2[0], j + p2[1], k + (o->dim == 3 ? p2[2] : 0), r);
+    diff2 = o_project (o, c->opp2, o_incident (o, r, ddiff, inc));
+    if (c->is_h) diff2 *= -1.0;
+  }
+  if (revert)
+  {
+    if (u11) *v12 -= diff1; else if (u12) *v11 -= diff1;
+    if (u21) *v22 -= diff2; else if (u22) *v21 -= diff2;
+  }
+  else
+  {
+    if (u11) *v11 -= diff1; else if (u12) *v12 -= diff1;
+    if (u21) *v21 -= diff2; else if (u22) *v22 -= diff2;
+  }
+}
+
+static double o_sig (const oracle *o, int axis, int comp, int64_t i, int64_t j, int64_t k)
+{
+  if (axis >= o->dim) return 0;   /* SigmaZ is absent in TMz: material reads as 0 */
+  return o_get_material (o, comp, i, j, k, o->sigma[axis]);
+}
+
+/* One component's update for step t: Scheme::calculateFieldStep (Scheme.inc.h:876-1344). */
+static void o_field_step (oracle *o, int comp, int32_t t)
+{
+  const comp_desc *c = &CD[comp];
+  const double eps0 = o_eps0 ();
+  const double modifier = c->is_h ? o_mu0 () : eps0;
+  const double *mgrid = c->is_h ? o->mat[YEE_MAT_MU] : o->mat[YEE_MAT_EPS];
+  const double *omg = c->is_h ? o->mat[YEE_MAT_OMEGAPM] : o->mat[YEE_MAT_OMEGAPE];
+  const double *gam = c->is_h ? o->mat[YEE_MAT_GAMMAM] : o->mat[YEE_MAT_GAMMAE];
+  const double dt = o->dt, dx = o->dx;
+  int64_t k0 = o->dim == 3 ? 1 : 0, k1 = o->dim == 3 ? o->n[2] - 1 : 1;
+  int o1 = c->opp1, o2 = c->opp2;
+  int dz = o->dim == 3;
+
+  if (t > 0)
+  {
+    double *out = o->use_pml ? o->d[comp][0] : o->f[comp][0];
+    const double *prev = o->use_pml ? o->d[comp][1] : o->f[comp][1];
+    /* sweep 1: calculateFieldStepIteration (InternalScheme.inc.h:2351-2417) */
+    for (int64_t i = 1; i < o->n[0] - 1; ++i)
+      for (int64_t j = 1; j < o->n[1] - 1; ++j)
+        for (int64_t k = k0; k < k1; ++k)
+        {
+          int64_t q = o_idx (o, i, j, k);
+          double val = prev[q];
+          double ca, cb;
+          /* computeCaCb (InternalScheme.inc.h:2297-2349) */
+          if (o->use_pml)
+          {
+            double material = o_sig (o, c->sig_curl, comp, i, j, k);
+            double dd = (2 * eps0 * 1.0 + material * dt);
+            ca = (2 * eps0 * 1.0 - material * dt) / dd;
+            cb = (2 * eps0 * dt / dx) / dd;
+          }
+          else
+          {
+            double material = o_get_material (o, comp, i, j, k, mgrid);
+            ca = 1.0;
+            cb = dt / (material * modifier * dx);
+          }
+          double v11 = o_read1 (o, o1, i + c->d11[0], j + c->d11[1], k + dz * c->d11[2]);
+          double v12 = o_read1 (o, o1, i + c->d12[0], j + c->d12[1], k + dz * c->d12[2]);
+          double v21 = o_read1 (o, o2, i + c->d21[0], j + c->d21[1], k + dz * c->d21[2]);
+          double v22 = o_read1 (o, o2, i + c->d22[0], j + c->d22[1], k + dz * c->d22[2]);
+          if (o->use_tfsf)
+          {
+            o_tfsf (o, comp, i, j, k, &v11, &v12, &v21, &v22);
+          }
+          /* calcField (InternalScheme.inc.h:765-772), right side J = 0 */
+          double rhs = 0.0;
+          double tmp = v12 - v11 - v22 + v21 - rhs * dx;
+          out[q] = val * ca + tmp * cb;
+        }
+    if (o->use_pml)
+    {
+      if (o->use_meta)
+      {
+        /* sweep 2: calculateFieldStepIterationPMLMetamaterials (InternalScheme.inc.h:2452-2549) */
+        double *d0 = o->d[comp][0], *dp = o->d[comp][1], *dpp = o->d[comp][2];
+        double *m0 = o->d1[comp][0], *mp = o->d1[comp][1], *mpp = o->d1[comp][2];
+        for (int64_t i = 1; i < o->n[0] - 1; ++i)
+          for (int64_t j = 1; j < o->n[1] - 1; ++j)
+            for (int64_t k = k0; k < k1; ++k)
+            {
+              int64_t q = o_idx (o, i, j, k);
+              double m1, m2;
+              double material = o_get_meta (o, comp, i, j, k, mgrid, omg, gam, &m1, &m2);
+              double M = modifier;
+              double A = 4 * M * material + 2 * dt * M * material * m2 + M * ((dt * m1) * (dt * m1));
+              double b0 = (4 + 2 * dt * m2) / A;
+              double b1 = -8 / A;
+              double b2 = (4 - 2 * dt * m2) / A;
+              double a1 = (2 * M * ((dt * m1) * (dt * m1)) - 8 * M * material) / A;
+              double a2 = (4 * M * material - 2 * dt * M * material * m2 + M * ((dt * m1) * (dt * m1))) / A;
+              /* calcFieldDrude (InternalScheme.inc.h:774-780) */
+              m0[q] = d0[q] * b0 + dp[q] * b1 + dpp[q] * b2 - mp[q] * a1 - mpp[q] * a2;
+            }
+      }
+      /* sweep 3: calculateFieldStepIterationPML (InternalScheme.inc.h:2551-2640) */
+      const double *cur = o->use_meta ? o->d1[comp][0] : o->d[comp][0];
+      const double *prv = o->use_meta ? o->d1[comp][1] : o->d[comp][1];
+      double *e0 = o->f[comp][0];
+      const double *e1 = o->f[comp][1];
+      for (int64_t i = 1; i < o->n[0] - 1; ++i)
+        for (int64_t j = 1; j < o->n[1] - 1; ++j)
+          for (int64_t k = k0; k < k1; ++k)
+          {
+            int64_t q = o_idx (o, i, j, k);
+            double material1 = o_get_material (o, comp, i, j, k, mgrid);
+            double material4 = o_sig (o, c->sig4, comp, i, j, k);
+            double material5 = o_sig (o, c->sig5, comp, i, j, k);
+            double mod = material1 * modifier;
+            if (o->use_meta) mod = 1;
+            double dd = (2 * eps0 * 1.0 + material5 * dt);
+            double ca = (2 * eps0 * 1.0 - material5 * dt) / dd;
+            double cb = ((2 * eps0 * 1.0 + material4 * dt) / (mod)) / dd;
+            double cc = ((2 * eps0 * 1.0 - material4 * dt) / (mod)) / dd;
+            /* calcFieldFromDOrB (InternalScheme.inc.h:782-787) */
+            e0[q] = e1[q] * ca + cur[q] * cb - prv[q] * cc;
+          }
+    }
+  }
+
+  /* performPointSourceCalc (InternalScheme.inc.h:2818-2895), also at t == 0 */
+  if (o->cfg.point_source & (1u << comp))
+  {
+    int64_t p[3] = { o->cfg.point_source_pos[0], o->cfg.point_source_pos[1], o->dim == 3 ? o->cfg.point_source_pos[2] : 0 };
+    if (p[0] >= 0 && p[0] < o->n[0] && p[1] >= 0 && p[1] < o->n[1] && p[2] >= 0 && p[2] < o->n[2])
+    {
+      double ts = c->is_h ? t + 1.0 : t + 0.5;
+      o->f[comp][0][o_idx (o, p[0], p[1], p[2])] = sin (o->dt * ts * 2 * O_PI * o->freq);
+    }
+  }
+}
+
+static void o_shift (oracle *o, int comp)
+{
+  o_rotate (o->f[comp], o->layers);
+  if (o->use_pml) o_rotate (o->d[comp], o->layers);
+  if (o->use_meta) o_rotate (o->d1[comp], o->layers);
+}
+
+/* Scheme::performNStepsForBlock (Source/Scheme/Scheme.inc.h:28-473). */
+int oracle_step (oracle *o, int32_t n)
+{
+  if (!o || !o->prepared || n < 0) return -1;
+  for (int32_t s = 0; s < n; ++s)
+  {
+    int32_t t = o->t;
+    if (o->use_tfsf)
+    {
+      o_inc_e (o, t);
+      o_rotate (o->einc, 2);
+    }
+    for (int c = 0; c < 3; ++c) if (o->present[c]) o_field_step (o, c, t);
+    for (int c = 0; c < 3; ++c) if (o->present[c]) o_shift (o, c);
+    if (o->use_tfsf)
+    {
+      o_inc_h (o);
+      o_rotate (o->hinc, 2);
+    }
+    for (int c = 3; c < 6; ++c) if (o->present[c]) o_field_step (o, c, t);
+    for (int c = 3; c < 6; ++c) if (o->present[c]) o_shift (o, c);
+    o->t++;
+  }
+  return 0;
+}
+
+int oracle_get_field (oracle *o, int32_t comp, double *out)
+{
+  if (!o || !out || comp < 0 || comp > 17) return -1;
+  int base = comp % 6;
+  if (!o->present[base]) return -1;
+  double *src;
+  if (comp < 6) src = o->f[base][1];
+  else if (comp < 12) src = o->use_pml ? o->d[base][1] : NULL;
+  else src = o->use_meta ? o->d1[base][1] : NULL;
+  if (!src) return -1;
+  memcpy (out, src, sizeof (double) * o->ncell);
+  return 0;
+}
+
+int oracle_get_material (oracle *o, int32_t which, double *out)
+{
+  if (!o || !out) return -1;
+  if (which >= 0 && which < 6)
+  {
+    memcpy (out, o->mat[which], sizeof (double) * o->ncell);
+    return 0;
+  }
+  if (which >= 6 && which < 9 && o->sigma[which - 6])
+  {
+    memcpy (out, o->sigma[which - 6], sizeof (double) * o->ncell);
+    return 0;
+  }
+  return -1;
+}
+
+int oracle_tfsf_mask (oracle *o, int32_t comp, uint8_t *out)
+{
+  if (!o || !out || comp < 0 || comp > 5 || !o->present[comp]) return -1;
+  for (int64_t i = 0; i < o->n[0]; ++i)
+    for (int64_t j = 0; j < o->n[1]; ++j)
+      for (int64_t k = 0; k < o->n[2]; ++k)
+      {
+        double real[3];
+        o_real_coord (o, comp, i, j, k, real);
+        uint8_t m = 0;
+        for (int d = 0; d < 6; ++d)
+        {
+          if (o_tfsf_pred (o, comp, d, real)) m |= (uint8_t) (1u << d);
+        }
+        out[o_idx (o, i, j, k)] = m;
+      }
+  return 0;
+}
+
+int oracle_get_incident (oracle *o, double *einc, double *hinc, int64_t len)
+{
+  if (!o || !o->use_tfsf || len > o->inc_len) return -1;
+  if (einc) memcpy (einc, o->einc[1], sizeof (double) * len);
+  if (hinc) memcpy (hinc, o->hinc[1], sizeof (double) * len);
+  return 0;
+}
+
+int64_t oracle_incident_len (oracle *o)
+{
+  return o ? o->inc_len : 0;
+}
+
+int oracle_get_scalars (oracle *o, double out[3])
+{
+  if (!o) return -1;
+  out[0] = o->dt;
+  out[1] = o->freq;
+  out[2] = o->rel;
+  return 0;
+}

@@ -1,0 +1,58 @@
+/*
+ * yee_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of the reference's Yee update path, used as the checker by
+ * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.  The product
+ * (fdtd3d_amd, libyee.so) never links, loads or calls it.
+ *
+ * It follows the reference's semantics literally: two (three with Drude+PML) time
+ * layers per grid, D/B computed over the whole grid when PML is on, coefficients
+ * recomputed per cell from averaged materials, TF/SF predicates on floating-point
+ * coordinates with the reference's tolerances.  Citations are per function in
+ * yee_oracle.c.
+ */
+#ifndef YEE_ORACLE_H
+#define YEE_ORACLE_H
+
+#include <stdint.h>
+#include "../include/yee.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle oracle;
+
+oracle *oracle_create (const yee_config *cfg);
+void oracle_destroy (oracle *o);
+/* Same semantics as yee_set_material (FP64 [x][y][z]). */
+int oracle_set_material (oracle *o, int32_t which, const double *values);
+/* Sigma profiles and other set-up; call after the materials. */
+int oracle_prepare (oracle *o);
+int oracle_step (oracle *o, int32_t n);
+/* Newest layer of a component ([x][y][z]); returns -1 if the component is absent. */
+int oracle_get_field (oracle *o, int32_t comp, double *out);
+/* Material grid as stored (after prepare); which: yee_material, or 6/7/8 = SigmaX/Y/Z. */
+int oracle_get_material (oracle *o, int32_t which, double *out);
+/* TF/SF predicate bits per cell for component comp (same encoding as yee_get_mask kind 0). */
+int oracle_tfsf_mask (oracle *o, int32_t comp, uint8_t *out);
+int oracle_get_incident (oracle *o, double *einc, double *hinc, int64_t len);
+int oracle_get_scalars (oracle *o, double out[3]);
+int64_t oracle_incident_len (oracle *o);
+
+/* Material generators, restated from Approximation.cpp (used to build parity inputs). */
+double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,
+                                  double radius, double eps, double outside, int accuracy);
+double oracle_sphere_accurate_2d (double mx, double my, double cx, double cy,
+                                  double radius, double eps, double outside, int accuracy);
+double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, double cy, double cz,
+                               double radius, double eps, double outside);
+
+/* Numerical wave number of the incident wave (Approximation::approximateWaveNumber). */
+double oracle_wave_number (double delta, double lambda, double courant, double nlambda, double a1, double a2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YEE_ORACLE_H */
