@@ -1,0 +1,630 @@
+// Host driver and C ABI of the Yee engine (include/yee.h).
+//
+// One handle = one simulation on one GPU: device state, the HIP stream its work is
+// queued on, and the FP64 set-up tables.  yee_step enqueues per time step:
+//   incident_step (TF/SF only)  ->  E half-step  ->  E point sources  ->  H half-step  ->  H point sources
+// which is the ordering of Scheme::performNStepsForBlock (Source/Scheme/Scheme.inc.h:28-473).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/yee.h"
+#include "yee_kernels.h"
+#include "yee_setup.h"
+
+namespace yee {
+
+template <typename T, int MODE> __global__ void e_half_3d (StepArgs<T>);
+template <typename T, int MODE> __global__ void h_half_3d (StepArgs<T>);
+template <typename T, int MODE> __global__ void e_half_tmz (StepArgs<T>);
+template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
+template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
+template <typename T> __global__ void point_source (T *, int64_t, double);
+template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
+
+static thread_local std::string g_error;
+
+static int fail (int code, const std::string &msg)
+{
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do                                                                                           \
+  {                                                                                            \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail (YEE_EDEVICE, std::string (#expr) + ": " + hipGetErrorString (e_));          \
+  } while (0)
+
+}  // namespace yee
+
+using namespace yee;
+
+struct yee_handle
+{
+  yee_config cfg {};
+  int dim = 3;
+  int64_t n[3] = { 1, 1, 1 };
+  int64_t ncell = 0;
+  size_t wsize = 8;
+  int mode = 0;   // 0 plain, 1 PML, 2 PML + Drude
+  bool present[6] = {};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = { nullptr, nullptr };
+
+  void *f[6] = {}, *d[6] = {}, *dp[6] = {}, *d1[6] = {}, *d1p[6] = {};
+  void *mat[6] = {};
+  double mat_u[6] = { 1, 1, 0, 0, 0, 0 };
+  bool mat_nonzero[6] = {};
+  int64_t box_lo[3] = { 0, 0, 0 }, box_hi[3] = { 0, 0, 0 };   // Drude-material bounding box (eps cells)
+  bool box_any = false;
+
+  void *pml = nullptr;
+  int tab_len = 0;
+  void *einc = nullptr, *hinc[2] = { nullptr, nullptr };
+  int64_t inc_len = 0;
+  Scalars sc {};
+  int32_t t = 0;
+  bool prepared = false;
+  double last_ms = 0;
+  int chunk = 1, nchunks = 1;
+
+  bool fp32 () const { return wsize == 4; }
+  bool use_pml () const { return (cfg.flags & YEE_USE_PML) != 0; }
+  bool use_tfsf () const { return (cfg.flags & YEE_USE_TFSF) != 0; }
+  bool use_meta () const { return (cfg.flags & YEE_USE_METAMATERIALS) != 0; }
+};
+
+namespace {
+
+template <typename T>
+StepArgs<T> make_args (const yee_handle *h, int parity)
+{
+  StepArgs<T> a {};
+  a.nx = (int) h->n[0];
+  a.ny = (int) h->n[1];
+  a.nz = (int) h->n[2];
+  a.sx = h->n[1] * h->n[2];
+  a.sy = (int) h->n[2];
+  a.chunk = h->chunk;
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    // PML two-stage cells: p < P or p >= N - P - 1 (every averaged sigma is zero elsewhere)
+    a.slab_lo[ax] = h->use_pml () ? (int) h->cfg.pml[ax] : -1;
+    a.slab_hi[ax] = h->use_pml () ? (int) (h->n[ax] - h->cfg.pml[ax] - 1) : 1 << 30;
+    a.box_lo[ax] = (int) h->box_lo[ax];
+    a.box_hi[ax] = (int) h->box_hi[ax];
+    a.tl[ax] = (int) h->cfg.tfsf_left[ax];
+    a.tr[ax] = (int) (h->cfg.size[ax] - h->cfg.tfsf_right[ax]);
+    a.zinc[ax] = h->sc.zero_inc[ax];
+  }
+  if (h->dim == 2)
+  {
+    a.slab_lo[2] = -1;
+    a.slab_hi[2] = 1 << 30;
+  }
+  for (int c = 0; c < 6; ++c)
+  {
+    a.f[c] = (T *) h->f[c];
+    a.d[c] = (T *) h->d[c];
+    a.dp[c] = (T *) h->dp[c];
+    a.d1[c] = (T *) h->d1[c];
+    a.d1p[c] = (T *) h->d1p[c];
+    a.proj[c] = h->sc.proj[c];
+  }
+  a.eps = (const T *) h->mat[YEE_MAT_EPS];
+  a.mu = (const T *) h->mat[YEE_MAT_MU];
+  a.ope = (const T *) h->mat[YEE_MAT_OMEGAPE];
+  a.opm = (const T *) h->mat[YEE_MAT_OMEGAPM];
+  a.ge = (const T *) h->mat[YEE_MAT_GAMMAE];
+  a.gm = (const T *) h->mat[YEE_MAT_GAMMAM];
+  a.eps_u = (T) h->mat_u[YEE_MAT_EPS];
+  a.mu_u = (T) h->mat_u[YEE_MAT_MU];
+  a.dt = (T) h->sc.dt;
+  a.dx = (T) h->cfg.dx;
+  a.eps0 = (T) eps0 ();
+  a.mu0 = (T) mu0 ();
+  a.pml = (const T *) h->pml;
+  a.tab_len = h->tab_len;
+  a.tfsf = h->use_tfsf () ? 1 : 0;
+  a.S1 = h->sc.S1;
+  a.C1 = h->sc.C1;
+  a.S2 = h->sc.S2;
+  a.C2 = h->sc.C2;
+  a.einc = (const T *) h->einc;
+  // the E half-step of step t reads HInc after step t-1, kept in buffer (t+1) & 1
+  a.hinc = (const T *) h->hinc[parity];
+  a.inc_len = h->inc_len;
+  return a;
+}
+
+template <typename T>
+int launch_half (yee_handle *h, bool e_half, int parity)
+{
+  StepArgs<T> a = make_args<T> (h, parity);
+  if (h->dim == 3)
+  {
+    dim3 block (64, 4, 1);
+    dim3 grid ((unsigned) ((h->n[2] + 63) / 64), (unsigned) ((h->n[1] + 3) / 4), (unsigned) h->nchunks);
+    switch (h->mode * 2 + (e_half ? 0 : 1))
+    {
+      case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
+      case 1: hipLaunchKernelGGL ((h_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
+      case 2: hipLaunchKernelGGL ((e_half_3d<T, 1>), grid, block, 0, h->stream, a); break;
+      case 3: hipLaunchKernelGGL ((h_half_3d<T, 1>), grid, block, 0, h->stream, a); break;
+      case 4: hipLaunchKernelGGL ((e_half_3d<T, 2>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL ((h_half_3d<T, 2>), grid, block, 0, h->stream, a); break;
+    }
+  }
+  else
+  {
+    dim3 block (64, 4, 1);
+    dim3 grid ((unsigned) ((h->n[1] + 63) / 64), (unsigned) ((h->n[0] + 3) / 4), 1);
+    switch (h->mode * 2 + (e_half ? 0 : 1))
+    {
+      case 0: hipLaunchKernelGGL ((e_half_tmz<T, 0>), grid, block, 0, h->stream, a); break;
+      case 1: hipLaunchKernelGGL ((h_half_tmz<T, 0>), grid, block, 0, h->stream, a); break;
+      case 2: hipLaunchKernelGGL ((e_half_tmz<T, 1>), grid, block, 0, h->stream, a); break;
+      case 3: hipLaunchKernelGGL ((h_half_tmz<T, 1>), grid, block, 0, h->stream, a); break;
+      case 4: hipLaunchKernelGGL ((e_half_tmz<T, 2>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL ((h_half_tmz<T, 2>), grid, block, 0, h->stream, a); break;
+    }
+  }
+  HIP_TRY (hipGetLastError ());
+  return YEE_OK;
+}
+
+template <typename T>
+int launch_sources (yee_handle *h, bool e_half, int32_t t)
+{
+  const int c0 = e_half ? 0 : 3;
+  for (int c = c0; c < c0 + 3; ++c)
+  {
+    if (!(h->cfg.point_source & (1u << c)) || !h->present[c]) continue;
+    const int64_t *p = h->cfg.point_source_pos;
+    const int64_t pz = h->dim == 3 ? p[2] : 0;
+    // getFieldValueOrNullByAbsolutePos: positions outside the grid are ignored
+    if (p[0] < 0 || p[0] >= h->n[0] || p[1] < 0 || p[1] >= h->n[1] || pz < 0 || pz >= h->n[2]) continue;
+    const int64_t q = (p[0] * h->n[1] + p[1]) * h->n[2] + pz;
+    const double ts = e_half ? t + 0.5 : t + 1.0;
+    const double arg = h->sc.dt * ts * 2 * kPi * h->sc.freq;
+    hipLaunchKernelGGL ((point_source<T>), dim3 (1), dim3 (64), 0, h->stream, (T *) h->f[c], q, std::sin (arg));
+    HIP_TRY (hipGetLastError ());
+  }
+  return YEE_OK;
+}
+
+template <typename T>
+int step_once (yee_handle *h)
+{
+  const int32_t t = h->t;
+  const int prev = (t + 1) & 1;   // HInc after step t-1
+  const int cur = t & 1;          // HInc after step t
+  if (h->use_tfsf ())
+  {
+    const double modE = h->sc.dt / (h->sc.rel * eps0 () * h->cfg.dx);
+    const double modH = h->sc.dt / (h->sc.rel * mu0 () * h->cfg.dx);
+    const double arg = h->sc.dt * t * 2 * kPi * h->sc.freq;
+    hipLaunchKernelGGL ((incident_step<T>), dim3 (1), dim3 (1024), 0, h->stream, (T *) h->einc,
+                       (const T *) h->hinc[prev], (T *) h->hinc[cur], h->inc_len, (int) t, modE, modH, std::sin (arg));
+    HIP_TRY (hipGetLastError ());
+  }
+  int rc;
+  // the reference performs no field update at t == 0 (Scheme.inc.h:1032)
+  if (t > 0 && (rc = launch_half<T> (h, true, prev))) return rc;
+  if ((rc = launch_sources<T> (h, true, t))) return rc;
+  if (t > 0 && (rc = launch_half<T> (h, false, prev))) return rc;
+  if ((rc = launch_sources<T> (h, false, t))) return rc;
+  h->t++;
+  return YEE_OK;
+}
+
+int alloc_zero (yee_handle *h, void **p, size_t bytes)
+{
+  HIP_TRY (hipMalloc (p, bytes));
+  HIP_TRY (hipMemsetAsync (*p, 0, bytes, h->stream));
+  return YEE_OK;
+}
+
+template <typename T>
+std::vector<T> convert (const double *src, size_t n)
+{
+  std::vector<T> v (n);
+  for (size_t q = 0; q < n; ++q) v[q] = (T) src[q];
+  return v;
+}
+
+int upload (yee_handle *h, void **dst, const double *src, size_t n)
+{
+  if (!*dst)
+  {
+    HIP_TRY (hipMalloc (dst, n * h->wsize));
+  }
+  if (h->fp32 ())
+  {
+    std::vector<float> v = convert<float> (src, n);
+    HIP_TRY (hipMemcpy (*dst, v.data (), n * 4, hipMemcpyHostToDevice));
+  }
+  else
+  {
+    HIP_TRY (hipMemcpy (*dst, src, n * 8, hipMemcpyHostToDevice));
+  }
+  return YEE_OK;
+}
+
+int validate (const yee_config *c)
+{
+  if (!c) return fail (YEE_EINVAL, "null config");
+  if (c->scheme != YEE_SCHEME_DIM3 && c->scheme != YEE_SCHEME_TMZ)
+    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13) or YEE_SCHEME_TMZ (12)");
+  if (c->layout != 0) return fail (YEE_EINVAL, "only the E_CENTERED layout (0) is supported");
+  if (c->precision != 4 && c->precision != 8) return fail (YEE_EINVAL, "precision must be 4 or 8");
+  const int dim = c->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  for (int a = 0; a < dim; ++a)
+  {
+    if (c->size[a] < 3 || c->size[a] > (1 << 20)) return fail (YEE_EINVAL, "size out of range");
+    if (c->pml[a] < 0 || 2 * c->pml[a] > c->size[a]) return fail (YEE_EINVAL, "pml size out of range");
+    if ((c->flags & YEE_USE_TFSF)
+        && (c->tfsf_left[a] < 1 || c->tfsf_right[a] < 1 || c->tfsf_left[a] + c->tfsf_right[a] >= c->size[a]))
+      return fail (YEE_EINVAL, "TF/SF box out of range");
+  }
+  int64_t cells = c->size[0] * c->size[1] * (dim == 3 ? c->size[2] : 1);
+  if (cells > (int64_t) 1 << 34) return fail (YEE_EINVAL, "grid too large");
+  if (!(c->dx > 0) || !(c->wavelength > 0) || !(c->courant > 0))
+    return fail (YEE_EINVAL, "dx, wavelength and courant factor must be positive");
+  if ((c->flags & YEE_USE_METAMATERIALS) && !(c->flags & YEE_USE_PML))
+    return fail (YEE_EINVAL, "metamaterials require PML (reference: Scheme.inc.h:889-892)");
+  if (c->time_steps < 0) return fail (YEE_EINVAL, "negative time steps");
+  return YEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *yee_last_error (void)
+{
+  return g_error.c_str ();
+}
+
+int yee_device_info (char *buf, int64_t len)
+{
+  if (!buf || len <= 0) return fail (YEE_EINVAL, "null buffer");
+  int dev = 0;
+  hipDeviceProp_t prop;
+  HIP_TRY (hipGetDevice (&dev));
+  HIP_TRY (hipGetDeviceProperties (&prop, dev));
+  snprintf (buf, (size_t) len, "device %d: %s (%s), %d CUs, %.1f GiB", dev, prop.name, prop.gcnArchName,
+            prop.multiProcessorCount, prop.totalGlobalMem / 1073741824.0);
+  return YEE_OK;
+}
+
+int yee_create (const yee_config *cfg, yee_handle **out)
+{
+  if (!out) return fail (YEE_EINVAL, "null output handle");
+  *out = nullptr;
+  int rc = validate (cfg);
+  if (rc) return rc;
+  yee_handle *h = new yee_handle ();
+  h->cfg = *cfg;
+  h->dim = cfg->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  h->n[0] = cfg->size[0];
+  h->n[1] = cfg->size[1];
+  h->n[2] = h->dim == 3 ? cfg->size[2] : 1;
+  h->ncell = h->n[0] * h->n[1] * h->n[2];
+  h->wsize = cfg->precision == 4 ? 4 : 8;
+  h->mode = h->use_meta () ? 2 : (h->use_pml () ? 1 : 0);
+  for (int c = 0; c < 6; ++c)
+    h->present[c] = h->dim == 3 || c == YEE_EZ || c == YEE_HX || c == YEE_HY;
+  auto bail = [&] (int code) {
+    yee_destroy (h);
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags (&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail (fail (YEE_EDEVICE, std::string ("hipStreamCreate: ") + hipGetErrorString (e)));
+  if (hipEventCreate (&h->ev[0]) != hipSuccess || hipEventCreate (&h->ev[1]) != hipSuccess)
+    return bail (fail (YEE_EDEVICE, "hipEventCreate failed"));
+  const size_t bytes = (size_t) h->ncell * h->wsize;
+  for (int c = 0; c < 6; ++c)
+  {
+    if (!h->present[c]) continue;
+    if ((rc = alloc_zero (h, &h->f[c], bytes))) return bail (rc);
+    if (h->mode >= 1 && (rc = alloc_zero (h, &h->d[c], bytes))) return bail (rc);
+    if (h->mode == 2)
+    {
+      if ((rc = alloc_zero (h, &h->dp[c], bytes))) return bail (rc);
+      if ((rc = alloc_zero (h, &h->d1[c], bytes))) return bail (rc);
+      if ((rc = alloc_zero (h, &h->d1p[c], bytes))) return bail (rc);
+    }
+  }
+  if (h->use_tfsf ())
+  {
+    // allocateGridsInc: 500 * size.x cells (InternalScheme.specific.h:99-107)
+    h->inc_len = 500 * h->n[0];
+    if ((rc = alloc_zero (h, &h->einc, h->inc_len * h->wsize))) return bail (rc);
+    if ((rc = alloc_zero (h, &h->hinc[0], h->inc_len * h->wsize))) return bail (rc);
+    if ((rc = alloc_zero (h, &h->hinc[1], h->inc_len * h->wsize))) return bail (rc);
+  }
+  if (hipStreamSynchronize (h->stream) != hipSuccess) return bail (fail (YEE_EDEVICE, "stream sync failed"));
+  h->sc = compute_scalars (h->cfg);
+  *out = h;
+  return YEE_OK;
+}
+
+void yee_destroy (yee_handle *h)
+{
+  if (!h) return;
+  if (h->stream) (void) hipStreamSynchronize (h->stream);
+  for (int c = 0; c < 6; ++c)
+  {
+    void *ptrs[5] = { h->f[c], h->d[c], h->dp[c], h->d1[c], h->d1p[c] };
+    for (void *p : ptrs)
+      if (p) (void) hipFree (p);
+    if (h->mat[c]) (void) hipFree (h->mat[c]);
+  }
+  if (h->pml) (void) hipFree (h->pml);
+  if (h->einc) (void) hipFree (h->einc);
+  if (h->hinc[0]) (void) hipFree (h->hinc[0]);
+  if (h->hinc[1]) (void) hipFree (h->hinc[1]);
+  for (hipEvent_t e : h->ev)
+    if (e) (void) hipEventDestroy (e);
+  if (h->stream) (void) hipStreamDestroy (h->stream);
+  delete h;
+}
+
+int yee_set_material (yee_handle *h, int32_t which, const double *values)
+{
+  if (!h || !values) return fail (YEE_EINVAL, "null argument");
+  if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
+  if (h->prepared) return fail (YEE_ESTATE, "materials must be set before yee_prepare");
+  const size_t n = (size_t) h->ncell;
+  bool uniform = true;
+  bool nonzero = false;
+  int64_t lo[3] = { h->n[0], h->n[1], h->n[2] }, hi[3] = { -1, -1, -1 };
+  for (size_t q = 0; q < n; ++q)
+  {
+    if (!std::isfinite (values[q])) return fail (YEE_EINVAL, "material values must be finite");
+    if (values[q] != values[0]) uniform = false;
+    if (values[q] != 0.0)
+    {
+      nonzero = true;
+      if (which >= 2)
+      {
+        const int64_t k = (int64_t) (q % h->n[2]);
+        const int64_t j = (int64_t) ((q / h->n[2]) % h->n[1]);
+        const int64_t i = (int64_t) (q / ((size_t) h->n[2] * h->n[1]));
+        const int64_t p[3] = { i, j, k };
+        for (int a = 0; a < 3; ++a)
+        {
+          lo[a] = std::min (lo[a], p[a]);
+          hi[a] = std::max (hi[a], p[a]);
+        }
+      }
+    }
+  }
+  if ((which == YEE_MAT_EPS || which == YEE_MAT_MU) && !nonzero) return fail (YEE_EINVAL, "eps/mu must be nonzero");
+  h->mat_nonzero[which] = nonzero;
+  if (which >= 2 && nonzero)
+  {
+    // field cells whose averages reach a nonzero Drude cell: p in [lo - 1, hi]
+    for (int a = 0; a < 3; ++a)
+    {
+      const int64_t l = lo[a] - 1, u = hi[a] + 1;
+      if (!h->box_any)
+      {
+        h->box_lo[a] = l;
+        h->box_hi[a] = u;
+      }
+      else
+      {
+        h->box_lo[a] = std::min (h->box_lo[a], l);
+        h->box_hi[a] = std::max (h->box_hi[a], u);
+      }
+    }
+    h->box_any = true;
+  }
+  if (uniform && which < 2)
+  {
+    h->mat_u[which] = values[0];
+    if (h->mat[which])
+    {
+      (void) hipFree (h->mat[which]);
+      h->mat[which] = nullptr;
+    }
+    return YEE_OK;
+  }
+  if (!nonzero)
+  {
+    if (h->mat[which])
+    {
+      (void) hipFree (h->mat[which]);
+      h->mat[which] = nullptr;
+    }
+    return YEE_OK;
+  }
+  return upload (h, &h->mat[which], values, n);
+}
+
+int yee_prepare (yee_handle *h)
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  if (h->prepared) return YEE_OK;
+  // the Drude ADE divides by A = 4 M eps + ...: eps / mu must be nonzero (checked in set_material)
+  if (h->mode == 2 && !h->box_any)
+  {
+    for (int a = 0; a < 3; ++a)
+    {
+      h->box_lo[a] = 0;
+      h->box_hi[a] = 0;
+    }
+  }
+  // x-march chunking: enough (y, z) columns x chunks to fill the GPU
+  const int64_t cols = h->dim == 3 ? std::max<int64_t> (1, (h->n[1] - 2) * (h->n[2] - 2)) : 1;
+  const int64_t planes = std::max<int64_t> (1, h->n[0] - 2);
+  int64_t nch = std::max<int64_t> (1, (int64_t) 262144 / cols);
+  nch = std::min (nch, planes);
+  h->chunk = (int) ((planes + nch - 1) / nch);
+  h->nchunks = (int) ((planes + h->chunk - 1) / h->chunk);
+  if (h->use_pml ())
+  {
+    h->tab_len = (int) std::max (h->n[0], std::max (h->n[1], h->n[2]));
+    std::vector<double> tab = pml_tables (h->cfg, h->tab_len);
+    int rc = upload (h, &h->pml, tab.data (), tab.size ());
+    if (rc) return rc;
+  }
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  h->prepared = true;
+  return YEE_OK;
+}
+
+int yee_step (yee_handle *h, int32_t n)
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  if (n < 0) return fail (YEE_EINVAL, "negative step count");
+  HIP_TRY (hipEventRecord (h->ev[0], h->stream));
+  for (int32_t s = 0; s < n; ++s)
+  {
+    int rc = h->fp32 () ? step_once<float> (h) : step_once<double> (h);
+    if (rc) return rc;
+  }
+  HIP_TRY (hipEventRecord (h->ev[1], h->stream));
+  HIP_TRY (hipEventSynchronize (h->ev[1]));
+  float ms = 0;
+  HIP_TRY (hipEventElapsedTime (&ms, h->ev[0], h->ev[1]));
+  h->last_ms = n > 0 ? ms / n : 0;
+  return YEE_OK;
+}
+
+int yee_sync (yee_handle *h)
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  return YEE_OK;
+}
+
+int yee_time (yee_handle *h, int32_t *t)
+{
+  if (!h || !t) return fail (YEE_EINVAL, "null argument");
+  *t = h->t;
+  return YEE_OK;
+}
+
+int yee_last_step_ms (yee_handle *h, double *ms)
+{
+  if (!h || !ms) return fail (YEE_EINVAL, "null argument");
+  *ms = h->last_ms;
+  return YEE_OK;
+}
+
+int yee_get_field_raw (yee_handle *h, int32_t comp, void *out)
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (comp < 0 || comp > 17) return fail (YEE_EINVAL, "unknown component");
+  const int base = comp % 6;
+  if (!h->present[base]) return fail (YEE_EINVAL, "component not present in this scheme");
+  void *src = comp < 6 ? h->f[base] : (comp < 12 ? h->d[base] : h->d1[base]);
+  if (!src) return fail (YEE_EINVAL, "component not allocated (enable PML / metamaterials)");
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  HIP_TRY (hipMemcpy (out, src, (size_t) h->ncell * h->wsize, hipMemcpyDeviceToHost));
+  return YEE_OK;
+}
+
+int yee_get_field (yee_handle *h, int32_t comp, double *out)
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (!h->fp32 ()) return yee_get_field_raw (h, comp, out);
+  std::vector<float> tmp ((size_t) h->ncell);
+  int rc = yee_get_field_raw (h, comp, tmp.data ());
+  if (rc) return rc;
+  for (size_t q = 0; q < tmp.size (); ++q) out[q] = tmp[q];
+  return YEE_OK;
+}
+
+int yee_get_mask (yee_handle *h, int32_t kind, int32_t comp, uint8_t *out)
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (kind == 0)
+  {
+    if (comp < 0 || comp > 5 || !h->present[comp]) return fail (YEE_EINVAL, "bad component");
+    uint8_t *dev = nullptr;
+    HIP_TRY (hipMalloc (&dev, (size_t) h->ncell));
+    const int grid = (int) std::min<int64_t> (4096, (h->ncell + 255) / 256);
+    if (h->fp32 ())
+      hipLaunchKernelGGL ((tfsf_mask_kernel<float>), dim3 (grid), dim3 (256), 0, h->stream, make_args<float> (h, 0), comp, h->dim, dev);
+    else
+      hipLaunchKernelGGL ((tfsf_mask_kernel<double>), dim3 (grid), dim3 (256), 0, h->stream, make_args<double> (h, 0), comp, h->dim, dev);
+    hipError_t e = hipGetLastError ();
+    if (e == hipSuccess) e = hipStreamSynchronize (h->stream);
+    if (e == hipSuccess) e = hipMemcpy (out, dev, (size_t) h->ncell, hipMemcpyDeviceToHost);
+    (void) hipFree (dev);
+    if (e != hipSuccess) return fail (YEE_EDEVICE, std::string ("tfsf mask: ") + hipGetErrorString (e));
+    return YEE_OK;
+  }
+  if (kind == 1)
+  {
+    for (int64_t i = 0; i < h->n[0]; ++i)
+      for (int64_t j = 0; j < h->n[1]; ++j)
+        for (int64_t k = 0; k < h->n[2]; ++k)
+        {
+          const int64_t p[3] = { i, j, k };
+          bool s = false;
+          if (h->use_pml ())
+            for (int a = 0; a < h->dim; ++a) s |= p[a] < h->cfg.pml[a] || p[a] >= h->n[a] - h->cfg.pml[a] - 1;
+          if (h->mode == 2)
+          {
+            bool in = true;
+            for (int a = 0; a < h->dim; ++a) in &= p[a] >= h->box_lo[a] && p[a] < h->box_hi[a];
+            s |= in;
+          }
+          out[(i * h->n[1] + j) * h->n[2] + k] = s ? 1 : 0;
+        }
+    return YEE_OK;
+  }
+  return fail (YEE_EINVAL, "unknown mask kind");
+}
+
+int yee_get_incident (yee_handle *h, double *einc, double *hinc, int64_t len)
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
+  if (len < 0 || len > h->inc_len) return fail (YEE_EINVAL, "length out of range");
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  // newest HInc: written by the last completed step into buffer (t - 1) & 1
+  const int newest = (h->t + 1) & 1;
+  void *srcs[2] = { h->einc, h->hinc[newest] };
+  double *dsts[2] = { einc, hinc };
+  for (int s = 0; s < 2; ++s)
+  {
+    if (!dsts[s]) continue;
+    if (h->fp32 ())
+    {
+      std::vector<float> tmp ((size_t) len);
+      HIP_TRY (hipMemcpy (tmp.data (), srcs[s], (size_t) len * 4, hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < len; ++q) dsts[s][q] = tmp[q];
+    }
+    else
+    {
+      HIP_TRY (hipMemcpy (dsts[s], srcs[s], (size_t) len * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  return YEE_OK;
+}
+
+int yee_get_scalars (yee_handle *h, double out[3])
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  out[0] = h->sc.dt;
+  out[1] = h->sc.freq;
+  out[2] = h->sc.rel;
+  return YEE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+// Host-side FP64 set-up of the Yee engine (see yee_setup.h).
+#include "yee_setup.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace yee {
+
+// E_CENTERED bindings (InternalScheme.inc.h:1910-2295):
+//   Ex: Hz, Hy, sigma Y / X / Z    Hx: Ey, Ez, sigma Y / X / Z
+//   Ey: Hx, Hz, sigma Z / Y / X    Hy: Ez, Ex, sigma Z / Y / X
+//   Ez: Hy, Hx, sigma X / Z / Y    Hz: Ex, Ey, sigma X / Z / Y
+const CompDesc kComp[6] = {
+  { 0, 0, YEE_HZ, YEE_HY, 1, 0, 2 },
+  { 1, 0, YEE_HX, YEE_HZ, 2, 1, 0 },
+  { 2, 0, YEE_HY, YEE_HX, 0, 2, 1 },
+  { 0, 1, YEE_EY, YEE_EZ, 1, 0, 2 },
+  { 1, 1, YEE_EZ, YEE_EX, 2, 1, 0 },
+  { 2, 1, YEE_EX, YEE_EY, 0, 2, 1 },
+};
+
+// Approximation::approximateWaveNumber / approximateWaveNumberGeneral (Approximation.cpp:259-339):
+// closed forms for propagation along an axis or at 45 degrees, Newton iteration otherwise.
+double wave_number (double delta, double lambda, double courant, double nl, double a1, double a2)
+{
+  const double pi = kPi;
+  auto axial = [pi] (double a) { return a == 0 || a == pi / 2 || a == pi || a == 3 * pi / 2; };
+  auto diagonal = [pi] (double a) { return a == pi / 4 || a == 3 * pi / 4 || a == 5 * pi / 4 || a == 7 * pi / 4; };
+  if (axial (a1) && axial (a2))
+  {
+    return 2.0 / delta * std::asin (std::sin (pi * courant / nl) / courant);
+  }
+  if ((axial (a1) && diagonal (a2)) || (diagonal (a1) && axial (a2)))
+  {
+    return 2.0 * std::sqrt (2.0) / delta * std::asin (std::sin (pi * courant / nl) / (courant * std::sqrt (2.0)));
+  }
+  const double tol = 0.0000001;
+  double k = 2 * pi;
+  double k_prev = k + 2 * std::sqrt (tol);
+  const double nd = delta / lambda;
+  const double A = nd * std::sin (a1) * std::cos (a2) / 2;
+  const double B = nd * std::sin (a1) * std::sin (a2) / 2;
+  const double C = nd * std::cos (a1) / 2;
+  const double s = std::sin (pi * courant / nl);
+  const double D = (s * s) / (courant * courant);
+  while ((k_prev - k) * (k_prev - k) >= tol)
+  {
+    k_prev = k;
+    const double sa = std::sin (A * k), sb = std::sin (B * k), sc = std::sin (C * k);
+    const double f = sa * sa + sb * sb + sc * sc - D;
+    const double df = A * std::sin (2 * A * k) + B * std::sin (2 * B * k) + C * std::sin (2 * C * k);
+    k -= f / df;
+  }
+  return k / lambda;
+}
+
+Scalars compute_scalars (const yee_config &cfg)
+{
+  Scalars s {};
+  const bool dim3 = cfg.scheme == YEE_SCHEME_DIM3;
+  // InternalScheme::initScheme (InternalScheme.inc.h:3359-3394)
+  s.freq = kC / cfg.wavelength;
+  s.dt = cfg.dx * cfg.courant / kC;
+  s.a1 = cfg.angle_deg[0] * kPi / 180.0;
+  s.a2 = cfg.angle_deg[1] * kPi / 180.0;
+  s.a3 = cfg.angle_deg[2] * kPi / 180.0;
+  const double nl = cfg.wavelength / cfg.dx;
+  auto phase_velocity = [&] (double a1, double a2) {
+    return kC * 2 * kPi / cfg.wavelength / wave_number (cfg.dx, cfg.wavelength, cfg.courant, nl, a1, a2);
+  };
+  s.rel = phase_velocity (kPi / 2, 0) / phase_velocity (s.a1, s.a2);
+  s.S1 = std::sin (s.a1);
+  s.C1 = std::cos (s.a1);
+  s.S2 = std::sin (s.a2);
+  s.C2 = std::cos (s.a2);
+  const double a1 = s.a1, a2 = s.a2, a3 = s.a3;
+  if (dim3)
+  {
+    // YeeGridLayout.h:2279-2356
+    s.proj[YEE_EX] = std::cos (a3) * std::sin (a2) - std::sin (a3) * std::cos (a1) * std::cos (a2);
+    s.proj[YEE_EY] = -std::cos (a3) * std::cos (a2) - std::sin (a3) * std::cos (a1) * std::sin (a2);
+    s.proj[YEE_EZ] = std::sin (a3) * std::sin (a1);
+    s.proj[YEE_HX] = std::sin (a3) * std::sin (a2) + std::cos (a3) * std::cos (a1) * std::cos (a2);
+    s.proj[YEE_HY] = -std::sin (a3) * std::cos (a2) + std::cos (a3) * std::cos (a1) * std::sin (a2);
+    // -valH * f: negating the factor instead gives the same bits
+    s.proj[YEE_HZ] = -(std::cos (a3) * std::sin (a1));
+    // YeeGridLayout.cpp:2636-2656
+    s.zero_inc[0] = cfg.tfsf_left[0] - 2.5 * std::sin (a1) * std::cos (a2);
+    s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a1) * std::sin (a2);
+    s.zero_inc[2] = cfg.tfsf_left[2] - 2.5 * std::cos (a1);
+  }
+  else
+  {
+    // YeeGridLayout.h:2194-2277 and YeeGridLayout.cpp:2583-2599 (TMz)
+    s.proj[YEE_EZ] = 1.0;
+    s.proj[YEE_HX] = std::sin (a2);
+    s.proj[YEE_HY] = -std::cos (a2);
+    s.zero_inc[0] = cfg.tfsf_left[0] - 2.5 * std::cos (a2);
+    s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a2);
+    s.zero_inc[2] = 0;
+  }
+  return s;
+}
+
+// SchemeHelper::initSigma (SchemeHelper.h:574-590): polynomial grading, m = 6, R = 1e-16.
+static double graded_sigma (double dist, double boundary, double dx)
+{
+  const unsigned m = 6;
+  const double r = 1e-16;
+  const double smax = -std::log (r) * (m + 1.0) / (2.0 * std::sqrt (mu0 () / eps0 ()) * boundary);
+  const double factor = smax / (dx * (std::pow (boundary, (double) m)) * (m + 1));
+  const double hi = (dist + 1) * dx;
+  const double lo = dist * dx;
+  return factor * (std::pow (hi, (double) (m + 1)) - std::pow (lo, (double) (m + 1)));
+}
+
+std::vector<double> sigma_profile (const yee_config &cfg, int axis)
+{
+  const int64_t n = cfg.size[axis];
+  std::vector<double> prof (n, 0.0);
+  const double P = (double) cfg.pml[axis];
+  const double boundary = P * cfg.dx;
+  for (int64_t p = 0; p < n; ++p)
+  {
+    const double pos = (double) p + 0.5;   // eps-grid coordinate
+    if (axis == 0)
+    {
+      // x: fractional distance, right edge at N - P
+      const double edge = (double) n;
+      if (pos < P) prof[p] = graded_sigma (P - pos, boundary, cfg.dx);
+      else if (pos >= edge - P) prof[p] = graded_sigma (pos - (edge - P), boundary, cfg.dx);
+    }
+    else
+    {
+      // y, z: distance truncated to an integer, right edge at N + 0.5 - P
+      const double edge = (double) n + 0.5;
+      if (pos < P) prof[p] = graded_sigma ((double) (int64_t) (P - pos), boundary, cfg.dx);
+      else if (pos >= edge - P) prof[p] = graded_sigma ((double) (int64_t) (pos - (edge - P)), boundary, cfg.dx);
+    }
+  }
+  return prof;
+}
+
+// A component's field point lies at an integer coordinate along `axis` when it is an E
+// component of that axis or an H component of another axis; getMaterial then averages the
+// two eps cells p and p+1.  At a half-integer coordinate the cells coincide and the average
+// reproduces the cell value exactly, as do the 4-cell averages of H components.
+double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p, int64_t n)
+{
+  const CompDesc &c = kComp[comp];
+  const bool integer_coord = c.is_h ? (axis != c.axis) : (axis == c.axis);
+  if (!integer_coord)
+  {
+    return prof[p];
+  }
+  const double hi = p + 1 < n ? prof[p + 1] : 0.0;
+  return (prof[p] + hi) / 2.0;
+}
+
+std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
+{
+  const int dim = cfg.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  std::vector<double> tab ((size_t) 6 * kNumRoles * tab_len, 0.0);
+  if (!(cfg.flags & YEE_USE_PML))
+  {
+    return tab;
+  }
+  std::vector<double> prof[3];
+  for (int a = 0; a < dim; ++a) prof[a] = sigma_profile (cfg, a);
+  const Scalars s = compute_scalars (cfg);
+  const double e0 = eps0 (), dt = s.dt, dx = cfg.dx;
+  auto at = [&] (int comp, int axis, int64_t p) -> double {
+    if (axis >= dim) return 0.0;   // SigmaZ is absent in TMz and reads as 0
+    return sigma_at (prof[axis], comp, axis, p, cfg.size[axis]);
+  };
+  for (int comp = 0; comp < 6; ++comp)
+  {
+    const CompDesc &c = kComp[comp];
+    double *T = &tab[(size_t) comp * kNumRoles * tab_len];
+    for (int64_t p = 0; p < tab_len; ++p)
+    {
+      if (c.sig_curl < dim ? p < cfg.size[c.sig_curl] : p == 0)
+      {
+        const double m = at (comp, c.sig_curl, p);
+        const double dd = (2 * e0 * 1.0 + m * dt);
+        T[kCAC * tab_len + p] = (2 * e0 * 1.0 - m * dt) / dd;
+        T[kCBC * tab_len + p] = (2 * e0 * dt / dx) / dd;
+      }
+      if (c.sig4 < dim ? p < cfg.size[c.sig4] : p == 0)
+      {
+        const double m = at (comp, c.sig4, p);
+        T[kP4 * tab_len + p] = 2 * e0 * 1.0 + m * dt;
+        T[kM4 * tab_len + p] = 2 * e0 * 1.0 - m * dt;
+      }
+      if (c.sig5 < dim ? p < cfg.size[c.sig5] : p == 0)
+      {
+        const double m = at (comp, c.sig5, p);
+        const double dd = (2 * e0 * 1.0 + m * dt);
+        T[kCA5 * tab_len + p] = (2 * e0 * 1.0 - m * dt) / dd;
+        T[kDD5 * tab_len + p] = dd;
+      }
+    }
+  }
+  return tab;
+}
+
+double sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,
+                           double radius, double eps, double outside, int accuracy)
+{
+  // Volume fraction of the cell [m-0.5, m+0.5]^3 inside the sphere, integrated column by
+  // column over an accuracy x accuracy raster of the xy face.
+  const double x0 = mx - 0.5, y0 = my - 0.5, z0 = mz - 0.5, z1 = mz + 0.5;
+  const double h = 1.0 / accuracy;
+  const double area = h * h;
+  const bool upper = mz > cz;
+  double vol = 0;
+  for (int i = 0; i < accuracy; ++i)
+  {
+    for (int j = 0; j < accuracy; ++j)
+    {
+      const double px = x0 + i * h;
+      const double py = y0 + j * h;
+      const double r2 = radius * radius - (px - cx) * (px - cx) - (py - cy) * (py - cy);
+      double height = 0.0;
+      if (!(r2 < 0))
+      {
+        double zs = upper ? cz + std::sqrt (r2) : cz - std::sqrt (r2);
+        zs = std::min (std::max (zs, z0), z1);
+        height = upper ? zs - z0 : z1 - zs;
+      }
+      vol += height * area;
+    }
+  }
+  vol = std::min (vol, 1.0);
+  return eps * vol + outside * (1 - vol);
+}
+
+double sphere_accurate_2d (double mx, double my, double cx, double cy, double radius, double eps,
+                           double outside, int accuracy)
+{
+  const double x0 = mx - 0.5, y0 = my - 0.5, y1 = my + 0.5;
+  const double h = 1.0 / accuracy;
+  const bool upper = my > cy;
+  double vol = 0;
+  for (int i = 0; i < accuracy; ++i)
+  {
+    const double px = x0 + i * h;
+    const double r2 = radius * radius - (px - cx) * (px - cx);
+    double height = 0.0;
+    if (!(r2 < 0))
+    {
+      double ys = upper ? cy + std::sqrt (r2) : cy - std::sqrt (r2);
+      ys = std::min (std::max (ys, y0), y1);
+      height = upper ? ys - y0 : y1 - ys;
+    }
+    vol += height * h;
+  }
+  vol = std::min (vol, 1.0);
+  return eps * vol + outside * (1 - vol);
+}
+
+double sphere_stair_3d (double mx, double my, double mz, double cx, double cy, double cz,
+                        double radius, double eps, double outside)
+{
+  for (int q = 0; q < 8; ++q)
+  {
+    const double x = mx + ((q & 1) ? 0.5 : -0.5) - cx;
+    const double y = my + ((q & 2) ? 0.5 : -0.5) - cy;
+    const double z = mz + ((q & 4) ? 0.5 : -0.5) - cz;
+    if (!(std::sqrt (x * x + y * y + z * z) <= radius)) return outside;
+  }
+  return eps;
+}
+
+double sphere_stair_2d (double mx, double my, double cx, double cy, double radius, double eps, double outside)
+{
+  for (int q = 0; q < 4; ++q)
+  {
+    const double x = mx + ((q & 1) ? 0.5 : -0.5) - cx;
+    const double y = my + ((q & 2) ? 0.5 : -0.5) - cy;
+    if (!(std::sqrt (x * x + y * y) <= radius)) return outside;
+  }
+  return eps;
+}
+
+}  // namespace yee

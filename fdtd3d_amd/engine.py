@@ -1,0 +1,183 @@
+"""ctypes binding of libyee.so (include/yee.h).
+
+This is the host-side mirror of the reference's Scheme/InternalScheme life cycle for
+the Yee path:
+
+    Engine(cfg)          ~ InternalScheme::init + initScheme      (InternalScheme.specific.h:37-67)
+    set_material(...)    ~ the material part of Scheme::initGrids (Scheme.inc.h:2962-3165)
+    prepare()            ~ initSigmas + coefficient set-up        (Scheme.cpp:355-388)
+    step(n)              ~ Scheme::performNStepsForBlock          (Scheme.inc.h:28-473)
+    field(name)          ~ reading the newest time layer of a Grid
+
+The binding refuses to run without the native library: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .config import Config, COMP_INDEX, MAT_INDEX
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libyee.so")
+
+EXPORTED = [
+    "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
+    "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw",
+    "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms",
+]
+
+
+class YeeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"yee error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libyee.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m fdtd3d_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    i32 = ctypes.c_int32
+    L.yee_last_error.restype = ctypes.c_char_p
+    L.yee_device_info.argtypes = [ctypes.c_char_p, ctypes.c_int64]
+    L.yee_create.argtypes = [vp, ctypes.POINTER(vp)]
+    L.yee_destroy.argtypes = [vp]
+    L.yee_destroy.restype = None
+    L.yee_set_material.argtypes = [vp, i32, vp]
+    L.yee_prepare.argtypes = [vp]
+    L.yee_step.argtypes = [vp, i32]
+    L.yee_sync.argtypes = [vp]
+    L.yee_time.argtypes = [vp, ctypes.POINTER(i32)]
+    L.yee_get_field.argtypes = [vp, i32, vp]
+    L.yee_get_field_raw.argtypes = [vp, i32, vp]
+    L.yee_get_mask.argtypes = [vp, i32, i32, vp]
+    L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
+    L.yee_get_scalars.argtypes = [vp, vp]
+    L.yee_last_step_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    for name in EXPORTED:
+        if name not in ("yee_last_error", "yee_destroy"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise YeeError(rc, load_library().yee_last_error().decode())
+
+
+def device_info() -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(load_library().yee_device_info(buf, 256))
+    return buf.value.decode()
+
+
+class Engine:
+    """One simulation on the GPU (a yee_handle)."""
+
+    def __init__(self, cfg: Config):
+        L = load_library()
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        h = ctypes.c_void_p()
+        _check(L.yee_create(ctypes.byref(self._c), ctypes.byref(h)))
+        self.h = h
+        self.shape = cfg.shape
+        self.dtype = np.float32 if cfg.precision == 4 else np.float64
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load_library().yee_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_material(self, name: str, values: np.ndarray) -> None:
+        arr = np.ascontiguousarray(values, dtype=np.float64)
+        if arr.size != int(np.prod(self.shape)):
+            raise ValueError(f"material {name}: expected {self.shape}, got {arr.shape}")
+        _check(load_library().yee_set_material(self.h, MAT_INDEX[name], arr.ctypes.data))
+
+    def prepare(self) -> None:
+        _check(load_library().yee_prepare(self.h))
+
+    def step(self, n: int) -> None:
+        _check(load_library().yee_step(self.h, int(n)))
+
+    def sync(self) -> None:
+        _check(load_library().yee_sync(self.h))
+
+    @property
+    def time(self) -> int:
+        t = ctypes.c_int32()
+        _check(load_library().yee_time(self.h, ctypes.byref(t)))
+        return t.value
+
+    def last_step_ms(self) -> float:
+        ms = ctypes.c_double()
+        _check(load_library().yee_last_step_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(load_library().yee_get_field(self.h, COMP_INDEX[name], out.ctypes.data))
+        return out
+
+    def field_raw(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(load_library().yee_get_field_raw(self.h, COMP_INDEX[name], out.ctypes.data))
+        return out
+
+    def tfsf_mask(self, comp: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint8)
+        _check(load_library().yee_get_mask(self.h, 0, COMP_INDEX[comp], out.ctypes.data))
+        return out
+
+    def special_mask(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint8)
+        _check(load_library().yee_get_mask(self.h, 1, 0, out.ctypes.data))
+        return out
+
+    def incident(self, length: Optional[int] = None):
+        n = 500 * self.cfg.size[0] if length is None else length
+        e = np.empty(n, dtype=np.float64)
+        h = np.empty(n, dtype=np.float64)
+        _check(load_library().yee_get_incident(self.h, e.ctypes.data, h.ctypes.data, n))
+        return e, h
+
+    def scalars(self) -> np.ndarray:
+        out = np.empty(3, dtype=np.float64)
+        _check(load_library().yee_get_scalars(self.h, out.ctypes.data))
+        return out
+
+
+def run_engine(cfg: Config, materials: Optional[Dict[str, np.ndarray]] = None, steps: Optional[int] = None) -> Engine:
+    eng = Engine(cfg)
+    for name, arr in (materials or {}).items():
+        eng.set_material(name, arr)
+    eng.prepare()
+    eng.step(cfg.time_steps if steps is None else steps)
+    eng.sync()
+    return eng

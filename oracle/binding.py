@@ -44,6 +44,7 @@ def lib():
         L.oracle_get_scalars.argtypes = [vp, vp]
         L.oracle_incident_len.restype = ctypes.c_int64
         L.oracle_incident_len.argtypes = [vp]
+        L.oracle_set_source_phase.argtypes = [vp, ctypes.c_double]
         d = ctypes.c_double
         L.oracle_sphere_accurate_3d.restype = d
         L.oracle_sphere_accurate_3d.argtypes = [d] * 9 + [ctypes.c_int]
@@ -53,6 +54,8 @@ def lib():
         L.oracle_sphere_stair_3d.argtypes = [d] * 9
         L.oracle_wave_number.restype = d
         L.oracle_wave_number.argtypes = [d] * 6
+        L.oracle_wave_number_general.restype = d
+        L.oracle_wave_number_general.argtypes = [d] * 6
         _lib = L
     return _lib
 
@@ -97,6 +100,9 @@ class Oracle:
 
     def __del__(self):
         self.close()
+
+    def set_source_phase(self, phase: float) -> None:
+        lib().oracle_set_source_phase(self.h, float(phase))
 
     def set_material(self, name: str, values: np.ndarray) -> None:
         arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.shape)

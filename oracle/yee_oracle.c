@@ -77,6 +77,7 @@ struct oracle
   int64_t inc_len;
   int32_t t;
   int prepared;
+  double src_phase;        /* test hook: added to every source argument (0 = reference) */
 };
 
 static int64_t o_idx (const oracle *o, int64_t i, int64_t j, int64_t k)
@@ -303,6 +304,8 @@ static double o_incident (const oracle *o, const double real[3], double ddiff, c
   return v1 * p1 + v2 * p2;
 }
 
+double oracle_wave_number_general (double delta, double lambda, double courant, double nl, double a1, double a2);
+
 /* Approximation::approximateWaveNumber{,General} (Source/Layout/Approximation.cpp:259-339). */
 double oracle_wave_number (double delta, double lambda, double courant, double nl, double a1, double a2)
 {
@@ -319,6 +322,13 @@ double oracle_wave_number (double delta, double lambda, double courant, double n
   {
     return 2.0 * sqrt (2.0) / delta * asin (sin (pi * courant / nl) / (courant * sqrt (2.0)));
   }
+  return oracle_wave_number_general (delta, lambda, courant, nl, a1, a2);
+}
+
+/* Newton solution of the 3D Yee dispersion relation (Approximation.cpp:313-339). */
+double oracle_wave_number_general (double delta, double lambda, double courant, double nl, double a1, double a2)
+{
+  const double pi = O_PI;
   const double acc = 0.0000001;
   double k = 2 * pi;
   double k_prev = k + 2 * sqrt (acc);
@@ -634,7 +644,7 @@ static void o_inc_e (oracle *o, int32_t t)
     e0[i] = e1[i] + (h1[i - 1] - h1[i]) * mod;
   }
   double arg = o->dt * t * 2 * O_PI * o->freq;
-  e0[0] = sin (arg);
+  e0[0] = sin (arg + o->src_phase);
 }
 
 /* performPlaneWaveHSteps (Source/Scheme/InternalScheme.inc.h:3320-3357); the last cell is skipped. */
@@ -823,7 +833,7 @@ static void o_field_step (oracle *o, int comp, int32_t t)
     if (p[0] >= 0 && p[0] < o->n[0] && p[1] >= 0 && p[1] < o->n[1] && p[2] >= 0 && p[2] < o->n[2])
     {
       double ts = c->is_h ? t + 1.0 : t + 0.5;
-      o->f[comp][0][o_idx (o, p[0], p[1], p[2])] = sin (o->dt * ts * 2 * O_PI * o->freq);
+      o->f[comp][0][o_idx (o, p[0], p[1], p[2])] = sin (o->dt * ts * 2 * O_PI * o->freq + o->src_phase);
     }
   }
 }
@@ -915,6 +925,19 @@ int oracle_get_incident (oracle *o, double *einc, double *hinc, int64_t len)
   if (!o || !o->use_tfsf || len > o->inc_len) return -1;
   if (einc) memcpy (einc, o->einc[1], sizeof (double) * len);
   if (hinc) memcpy (hinc, o->hinc[1], sizeof (double) * len);
+  return 0;
+}
+
+/*
+ * Test hook: shifts the phase of every source.  With phase pi/2 the run is the imaginary
+ * part of the reference's complex-field build, whose sources are sin + i cos
+ * (InternalScheme.inc.h:2886-2888, 3311-3312); together with the phase-0 run it gives
+ * the field magnitudes the reference's unit test checks (unit-test-internalscheme.cpp:941-988).
+ */
+int oracle_set_source_phase (oracle *o, double phase)
+{
+  if (!o) return -1;
+  o->src_phase = phase;
   return 0;
 }
 

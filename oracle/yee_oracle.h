@@ -39,6 +39,8 @@ int oracle_tfsf_mask (oracle *o, int32_t comp, uint8_t *out);
 int oracle_get_incident (oracle *o, double *einc, double *hinc, int64_t len);
 int oracle_get_scalars (oracle *o, double out[3]);
 int64_t oracle_incident_len (oracle *o);
+/* Test hook: phase added to every source argument (pi/2 gives the cos-source run). */
+int oracle_set_source_phase (oracle *o, double phase);
 
 /* Material generators, restated from Approximation.cpp (used to build parity inputs). */
 double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,
@@ -50,6 +52,7 @@ double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, doubl
 
 /* Numerical wave number of the incident wave (Approximation::approximateWaveNumber). */
 double oracle_wave_number (double delta, double lambda, double courant, double nlambda, double a1, double a2);
+double oracle_wave_number_general (double delta, double lambda, double courant, double nlambda, double a1, double a2);
 
 #ifdef __cplusplus
 }

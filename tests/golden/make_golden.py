@@ -1,0 +1,81 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own arithmetic.
+
+Runs oracle/_ref/refdrv_f64 (built by `make -C oracle ref` from the reference sources;
+only possible where the reference tree is mounted) on small inputs and stores the
+newest time layer of every field, the material and sigma grids, the head of the
+incident 1D grids and the run scalars, as compressed .npz files.
+
+    python tests/golden/make_golden.py            # regenerate all fixtures
+"""
+from __future__ import annotations
+
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.binding import run_ref_driver  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+W0 = 2 * np.pi * 299792458.0 / np.float64(np.float32(0.02))
+
+
+def drude_files(tmp, n, lo, hi):
+    om = np.zeros((n, n, n))
+    ga = np.zeros((n, n, n))
+    om[lo:hi, lo:hi, lo:hi] = 1.5 * W0
+    ga[lo:hi, lo:hi, lo:hi] = 0.01 * W0
+    p_om = os.path.join(tmp, "omega.dat")
+    p_ga = os.path.join(tmp, "gamma.dat")
+    om.tofile(p_om)
+    ga.tofile(p_ga)
+    return p_om, p_ga
+
+
+def cases(tmp):
+    """name -> (argv, material files written for the run)"""
+    out = {}
+    out["d3_pml_tfsf_oblique_sphere"] = (
+        "--3d --size x:16 --same-size --dx 0.0005 --wavelength 0.02 --courant-factor 0.5 "
+        "--use-pml --pml-size x:3 --same-size-pml --use-tfsf --tfsf-size-left x:4 --tfsf-size-right x:4 "
+        "--same-size-tfsf --angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 4 "
+        "--eps-sphere-center x:8,y:8,z:8 --eps-sphere-radius 4 --time-steps 30").split()
+    out["d3_tfsf_axial_pointsource"] = (
+        "--3d --size x:16,y:14,z:18 --use-tfsf --tfsf-size-left x:4,y:3,z:5 --tfsf-size-right x:3,y:4,z:4 "
+        "--angle-teta 0 --angle-phi 0 --angle-psi 0 --point-source x:3,y:4,z:5 --point-source-hy "
+        "--time-steps 25").split()
+    om, ga = drude_files(tmp, 16, 6, 10)
+    out["d3_drude_pml_tfsf"] = (
+        "--3d --size x:16 --same-size --use-pml --pml-size x:3 --same-size-pml --use-tfsf "
+        "--tfsf-size-left x:4 --tfsf-size-right x:4 --same-size-tfsf --angle-teta 0 --angle-phi 0 "
+        f"--angle-psi 0 --use-metamaterials --load-omegape-from-file {om} --load-omegapm-from-file {om} "
+        f"--load-gammae-from-file {ga} --load-gammam-from-file {ga} --time-steps 30").split()
+    out["tmz_pml_pointsource_sphere"] = (
+        "--2d-tmz --size x:32,y:24 --use-pml --pml-size x:4,y:5 --point-source x:13,y:11 --point-source-ez "
+        "--eps-sphere 3 --eps-sphere-center x:16,y:12 --eps-sphere-radius 5 --time-steps 50").split()
+    out["tmz_pml_tfsf_oblique"] = (
+        "--2d-tmz --size x:32,y:28 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:8 "
+        "--tfsf-size-right x:6,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 50").split()
+    return out
+
+
+def main():
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, argv in cases(tmp).items():
+            rec, _ = run_ref_driver(argv, os.path.join(tmp, name + ".bin"))
+            payload = {"argv": np.array(" ".join(a if not a.startswith(tmp) else os.path.basename(a) for a in argv))}
+            for key, val in rec.items():
+                if key in ("EInc", "HInc"):
+                    val = val[:256]
+                payload[key] = val
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **payload)
+            print(name, sorted(payload))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""C-ABI checks that need no GPU: the library loads and exports every declared entry point,
+and the option parser follows the reference's option table."""
+import os
+import re
+
+import pytest
+
+from fdtd3d_amd.config import OptionError, parse_args, stof, SCHEME_TMZ, SCHEME_DIM3
+from fdtd3d_amd.engine import LIB_PATH, load_library
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for hdr in os.listdir(os.path.join(ROOT, "include")):
+        if hdr.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", hdr)).read()
+            names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(yee_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_built():
+    assert os.path.exists(LIB_PATH), "run python -m fdtd3d_amd.build"
+
+
+def test_exports_every_declared_symbol():
+    L = load_library()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_error_reporting_without_gpu():
+    import ctypes
+    from fdtd3d_amd.config import Config
+    L = load_library()
+    h = ctypes.c_void_p()
+    bad = Config(scheme=5).to_c()
+    assert L.yee_create(ctypes.byref(bad), ctypes.byref(h)) < 0
+    assert b"scheme" in L.yee_last_error()
+    assert L.yee_create(None, ctypes.byref(h)) < 0
+    assert L.yee_step(None, 1) < 0
+
+
+def test_parse_reference_options():
+    cfg = parse_args("--3d --size x:80 --same-size --dx 0.0005 --wavelength 0.02 --angle-teta 30 "
+                     "--angle-phi 45 --angle-psi 90 --use-tfsf --tfsf-size-left x:20 --tfsf-size-right x:20 "
+                     "--same-size-tfsf --use-pml --pml-size x:10 --same-size-pml --courant-factor 0.5 "
+                     "--time-steps 200 --eps-sphere 4 --eps-sphere-center x:40,y:40,z:40 --eps-sphere-radius 20".split())
+    assert cfg.scheme == SCHEME_DIM3 and cfg.size == [80, 80, 80] and cfg.pml == [10, 10, 10]
+    assert cfg.tfsf_left == [20, 20, 20] and cfg.tfsf_right == [20, 20, 20]
+    assert cfg.dx == stof("0.0005") and cfg.dx != 0.0005   # std::stof rounding, as the reference
+    assert cfg.use_pml and cfg.use_tfsf and cfg.time_steps == 200
+    assert cfg.eps_sphere == 4 and cfg.eps_sphere_center == [40, 40, 40] and cfg.eps_sphere_radius == 20
+    tmz = parse_args("--2d-tmz --size x:64,y:48 --point-source x:3,y:4 --point-source-ez".split())
+    assert tmz.scheme == SCHEME_TMZ and tmz.shape == (64, 48) and tmz.point_source == ["Ez"]
+
+
+def test_parse_rejects_unknown_and_unsupported():
+    with pytest.raises(OptionError):
+        parse_args(["--no-such-option"])
+    with pytest.raises(OptionError):
+        parse_args(["--1d-exhy"])
+    with pytest.raises(OptionError):
+        parse_args(["--3d", "--use-metamaterials"])
+
+
+def test_cmd_file(tmp_path):
+    f = tmp_path / "cmd.txt"
+    f.write_text("# comment\n--3d\n--size\nx:12,y:10,z:8\n// another\n--time-steps\n7\n")
+    cfg = parse_args(["--cmd-from-file", str(f)])
+    assert cfg.size == [12, 10, 8] and cfg.time_steps == 7

@@ -1,0 +1,169 @@
+"""HIP engine parity against the oracle and the reference fixtures (needs the GPU).
+
+Bars (per component, after the last step):
+  - FP64, runs without PML: bit-identical to the reference (same operations, no FMA);
+  - FP64 with PML / Drude: relative L2 <= 1e-12 (the engine updates E/H directly outside
+    the PML slabs, which equals the reference's D->E stage with sigma = 0 up to rounding);
+  - FP32: relative L2 of the E and H vectors <= 1e-5 against the FP64 oracle;
+  - TF/SF and PML masks: bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fdtd3d_amd.config import parse_args
+from fdtd3d_amd.engine import Engine, run_engine
+from oracle.binding import Oracle, run_oracle, sphere_map
+from tests.helpers import field_names, golden_names, load_golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+TOL_FP32 = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def vec_rel(eng_fields, ref_fields):
+    num = sum(np.sum((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2) for a, b in zip(eng_fields, ref_fields))
+    den = sum(np.sum(np.asarray(b, np.float64) ** 2) for b in ref_fields)
+    return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_engine_fp64_vs_reference_fixture(name, engine_lib):
+    cfg, mats, data = load_golden(name)
+    say(f"[gpu] fixture {name} fp64")
+    eng = run_engine(cfg, mats)
+    exact = not cfg.use_pml
+    for f in field_names(cfg):
+        got = eng.field(f).ravel()
+        ref = data[f]
+        if exact:
+            assert np.array_equal(got, ref), f"{name}/{f}: max diff {np.abs(got - ref).max()}"
+        else:
+            assert rel_l2(got, ref) <= TOL_FP64, f"{name}/{f}: rel {rel_l2(got, ref)}"
+    if cfg.use_tfsf:
+        e, h = eng.incident(256)
+        np.testing.assert_array_equal(e, data["EInc"])
+        np.testing.assert_array_equal(h, data["HInc"])
+    eng.close()
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_engine_fp32_vs_fp64_oracle(name, engine_lib):
+    cfg, mats, data = load_golden(name)
+    cfg.precision = 4
+    say(f"[gpu] fixture {name} fp32")
+    eng = run_engine(cfg, mats)
+    names = field_names(cfg)
+    e_names = [n for n in names if n[0] == "E"]
+    h_names = [n for n in names if n[0] == "H"]
+    assert vec_rel([eng.field(n) for n in e_names], [data[n].reshape(cfg.shape) for n in e_names]) <= TOL_FP32
+    assert vec_rel([eng.field(n) for n in h_names], [data[n].reshape(cfg.shape) for n in h_names]) <= TOL_FP32
+    eng.close()
+
+
+MASK_CASES = [
+    "--3d --size x:17,y:15,z:19 --use-tfsf --tfsf-size-left x:4,y:3,z:5 --tfsf-size-right x:5,y:4,z:3 "
+    "--angle-teta 30 --angle-phi 45 --angle-psi 90 --time-steps 1",
+    "--2d-tmz --size x:23,y:29 --use-tfsf --tfsf-size-left x:5,y:6 --tfsf-size-right x:4,y:7 "
+    "--angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 1",
+]
+
+
+@pytest.mark.parametrize("args", MASK_CASES)
+def test_tfsf_masks_bit_exact(args, engine_lib):
+    cfg = parse_args(args.split())
+    say(f"[gpu] masks {args[:40]}")
+    o = Oracle(cfg)
+    o.prepare()
+    eng = Engine(cfg)
+    eng.prepare()
+    for comp in field_names(cfg):
+        np.testing.assert_array_equal(eng.tfsf_mask(comp), o.tfsf_mask(comp), err_msg=comp)
+        assert o.tfsf_mask(comp).any()
+    eng.close()
+
+
+LARGER = [
+    # config 1 geometry at half size: sphere, PML, TF/SF, 120 steps
+    ("cfg1_half", "--3d --size x:40 --same-size --dx 0.0005 --wavelength 0.02 --angle-teta 0 --angle-phi 0 "
+     "--angle-psi 0 --use-tfsf --tfsf-size-left x:10 --tfsf-size-right x:10 --same-size-tfsf --use-pml "
+     "--pml-size x:5 --same-size-pml --courant-factor 0.5 --time-steps 120 --eps-sphere 4 "
+     "--eps-sphere-center x:20,y:20,z:20 --eps-sphere-radius 10"),
+    # config 2 features at small size: oblique incidence, block eps in [1, 12]
+    ("cfg2_small", "--3d --size x:48,y:40,z:56 --use-pml --pml-size x:6 --same-size-pml --use-tfsf "
+     "--tfsf-size-left x:10 --tfsf-size-right x:10 --same-size-tfsf --angle-teta 30 --angle-phi 45 "
+     "--angle-psi 90 --time-steps 100"),
+    # TMz point source with PML and random eps blobs
+    ("tmz_blobs", "--2d-tmz --size x:96,y:80 --use-pml --pml-size x:12,y:12 --point-source x:40,y:33 "
+     "--point-source-ez --time-steps 200"),
+]
+
+
+def _materials(name, cfg):
+    rng = np.random.default_rng(20251202)
+    mats = {}
+    if cfg.eps_sphere != 1.0:
+        mats["eps"] = sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius)
+    if name == "cfg2_small":
+        nb = [(s + 7) // 8 for s in cfg.shape]
+        blocks = rng.uniform(1.0, 12.0, size=nb)
+        mats["eps"] = np.kron(blocks, np.ones((8, 8, 8)))[: cfg.shape[0], : cfg.shape[1], : cfg.shape[2]]
+    if name == "tmz_blobs":
+        x, y = np.meshgrid(np.arange(cfg.shape[0]), np.arange(cfg.shape[1]), indexing="ij")
+        eps = np.ones(cfg.shape)
+        for _ in range(5):
+            cx, cy, r = rng.uniform(15, 80), rng.uniform(15, 65), rng.uniform(3, 8)
+            eps += 8 * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * r * r))
+        mats["eps"] = np.clip(eps, 1, 9)
+    return mats
+
+
+@pytest.mark.parametrize("name,args", LARGER)
+@pytest.mark.parametrize("precision", [8, 4])
+def test_engine_vs_oracle_larger(name, args, precision, engine_lib):
+    cfg = parse_args(args.split())
+    mats = _materials(name, cfg)
+    o = run_oracle(cfg, mats)
+    cfg.precision = precision
+    say(f"[gpu] larger {name} precision {precision}")
+    eng = run_engine(cfg, mats)
+    names = field_names(cfg)
+    if precision == 8:
+        for f in names:
+            r = rel_l2(eng.field(f), o.field(f))
+            assert r <= TOL_FP64, f"{name}/{f}: {r}"
+    else:
+        for group in ("E", "H"):
+            ns = [n for n in names if n[0] == group]
+            r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
+            assert r <= TOL_FP32, f"{name}/{group}: {r}"
+    eng.close()
+
+
+def test_special_mask_covers_every_nonzero_sigma(engine_lib):
+    cfg = parse_args("--3d --size x:20,y:18,z:22 --use-pml --pml-size x:4,y:3,z:5 --time-steps 1".split())
+    say("[gpu] special mask")
+    eng = Engine(cfg)
+    eng.prepare()
+    special = eng.special_mask().astype(bool)
+    o = Oracle(cfg)
+    o.prepare()
+    sig = [o.material(6 + a) for a in range(3)]
+    # a field cell averages eps cells p and p+1 along each axis
+    any_sigma = np.zeros(cfg.shape, bool)
+    for s in sig:
+        nz = s != 0
+        grown = nz.copy()
+        grown[:-1, :, :] |= nz[1:, :, :]
+        grown[:, :-1, :] |= nz[:, 1:, :]
+        grown[:, :, :-1] |= nz[:, :, 1:]
+        any_sigma |= grown
+    assert np.all(special[any_sigma])
+    eng.close()

@@ -1,0 +1,48 @@
+"""The C oracle against golden fixtures written from the reference's own arithmetic.
+
+The fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py with
+oracle/_ref/refdrv_f64) hold the newest time layer of every field after a short run
+of 3D and TMz cases with PML, TF/SF at oblique incidence, Drude metamaterials, point
+sources and partial-volume spheres.  The oracle must reproduce them bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle.binding import Oracle, sphere_map
+from tests.helpers import field_names, golden_names, load_golden
+
+NAMES = golden_names()
+
+
+def test_fixtures_present():
+    assert len(NAMES) >= 5
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_matches_reference_fixture_bit_exact(name):
+    cfg, mats, data = load_golden(name)
+    o = Oracle(cfg)
+    for key, arr in mats.items():
+        o.set_material(key, arr)
+    o.prepare()
+    o.step(cfg.time_steps)
+    for fname in field_names(cfg, aux=True):
+        got = o.field(fname).ravel()
+        assert np.array_equal(got, data[fname]), f"{name}: {fname} differs (max {np.abs(got - data[fname]).max()})"
+    np.testing.assert_array_equal(o.scalars()[:2], data["scalars"][:2])
+    if cfg.use_tfsf:
+        e, h = o.incident()
+        np.testing.assert_array_equal(e[:256], data["EInc"])
+        np.testing.assert_array_equal(h[:256], data["HInc"])
+    if cfg.use_pml:
+        for axis, key in enumerate(["SigmaX", "SigmaY", "SigmaZ"][: cfg.dim]):
+            np.testing.assert_array_equal(o.material(6 + axis).ravel(), data[key])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_sphere_generator_matches_reference(name):
+    cfg, mats, data = load_golden(name)
+    if cfg.eps_sphere == 1.0:
+        pytest.skip("no sphere in this fixture")
+    eps = sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius)
+    np.testing.assert_array_equal(eps.ravel(), data["Eps"])

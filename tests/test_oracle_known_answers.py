@@ -1,0 +1,116 @@
+"""The C oracle against known answers the reference's own tests and the mathematics fix.
+
+- unit-test-internalscheme.cpp:941-988: a 3D TF/SF plane wave in vacuum has |Ez| = 1
+  inside the total-field box (to sqrt(1.8e-5)) and exactly 0 outside after 200 steps.
+  The reference checks the magnitude of a complex run whose sources are sin + i cos;
+  here it is assembled from a sin-source run and a cos-source run.
+- unit-test-approximation.cpp:33-60: closed-form and Newton wave numbers agree to 1%.
+- the closed-form wave numbers solve the Yee dispersion relation.
+- the y/z PML sigma profile telescopes: sum of sigma over one slab = sigma_max * L / 7.
+- mirror and transposition symmetry of a centred point source in TMz.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fdtd3d_amd.config import Config, SCHEME_DIM3, SCHEME_TMZ, parse_args
+from oracle.binding import Oracle, lib
+
+
+def _run(cfg, phase=0.0, steps=None):
+    o = Oracle(cfg)
+    o.set_source_phase(phase)
+    o.prepare()
+    o.step(cfg.time_steps if steps is None else steps)
+    return o
+
+
+def test_unit_test_internalscheme_3d_tfsf_amplitude():
+    # SIZE 20, TFSF 7, LAMBDA 0.2, DX 0.002, angles 90/90/90, --time-steps 200 --use-tfsf
+    cfg = Config(scheme=SCHEME_DIM3, size=[20, 20, 20], pml=[5, 5, 5], tfsf_left=[7, 7, 7],
+                 tfsf_right=[7, 7, 7], dx=0.002, wavelength=0.2, angle_deg=[90.0, 90.0, 90.0],
+                 use_tfsf=True, time_steps=200)
+    re = _run(cfg).field("Ez")
+    im = _run(cfg, phase=math.pi / 2).field("Ez")
+    mag = np.sqrt(re ** 2 + im ** 2)
+    for i in range(20):
+        for j in range(20):
+            for k in range(20):
+                x, y, z = i + 0.5, j + 0.5, k + 1.0   # minEzCoordFP
+                if 7 <= x <= 13 and 7 <= y <= 13 and 7 <= z <= 13:
+                    assert (mag[i, j, k] - 1.0) ** 2 < 0.000018, (i, j, k, mag[i, j, k])
+                else:
+                    assert mag[i, j, k] < 1e-7, (i, j, k, mag[i, j, k])
+
+
+def test_wave_number_closed_form_vs_newton():
+    L = lib()
+    lam = 0.04
+    step = math.pi / 12
+    a1 = 0.0
+    while a1 < 2 * math.pi + step:
+        a2 = 0.0
+        while a2 < 2 * math.pi + step:
+            for courant in (0.5, 0.75, 1.0):
+                delta = 0.0002
+                while delta <= 0.002:
+                    nl = lam / delta
+                    g = L.oracle_wave_number_general(delta, lam, courant, nl, a1, a2)
+                    k = L.oracle_wave_number(delta, lam, courant, nl, a1, a2)
+                    assert (k - g) ** 2 < 2 * math.pi / lam / 100.0
+                    delta *= 2
+            a2 += step
+        a1 += step
+
+
+@pytest.mark.parametrize("a1,a2", [(0.0, 0.0), (math.pi / 2, 0.0), (math.pi / 2, math.pi / 4), (0.3, 1.1)])
+def test_wave_number_solves_dispersion_relation(a1, a2):
+    L = lib()
+    dx, lam, S = 0.0005, 0.02, 0.5
+    k = L.oracle_wave_number(dx, lam, S, lam / dx, a1, a2)
+    kx, ky, kz = k * math.sin(a1) * math.cos(a2), k * math.sin(a1) * math.sin(a2), k * math.cos(a1)
+    lhs = sum(math.sin(c * dx / 2) ** 2 for c in (kx, ky, kz))
+    rhs = math.sin(math.pi * S / (lam / dx)) ** 2 / S ** 2
+    assert abs(lhs - rhs) < 1e-9
+
+
+def test_sigma_profiles_closed_form():
+    P, N, dx = 6, 30, 0.0005
+    cfg = Config(scheme=SCHEME_DIM3, size=[N, N, N], pml=[P, P, P], dx=dx, use_pml=True, time_steps=1)
+    o = Oracle(cfg)
+    o.prepare()
+    eps0 = 1 / (4 * math.pi * 1e-7 * 299792458.0 ** 2)
+    mu0 = 4 * math.pi * 1e-7
+    L_ = P * dx
+    smax = -math.log(1e-16) * 7 / (2 * math.sqrt(mu0 / eps0) * L_)
+    sx = o.material(6)[:, 0, 0]
+    sy = o.material(7)[0, :, 0]
+    sz = o.material(8)[0, 0, :]
+    # y/z: integer distances 0..P-1 telescope to sigma_max * L / 7
+    assert math.isclose(sy[:P].sum() * dx, smax * L_ / 7, rel_tol=1e-12)
+    assert math.isclose(sy[N - P:].sum() * dx, smax * L_ / 7, rel_tol=1e-12)
+    np.testing.assert_array_equal(sy, sz)
+    # x: fractional distances P-0.5 .. 0.5 on the left, 0.5 .. P-0.5 on the right
+    f = smax / (dx * L_ ** 6 * 7)
+    for i in range(P):
+        d = P - (i + 0.5)
+        assert math.isclose(sx[i], f * (((d + 1) * dx) ** 7 - (d * dx) ** 7), rel_tol=1e-12)
+    assert np.all(sx[P:N - P] == 0) and np.all(sy[P:N - P] == 0)
+    assert sx[N - P] > 0 and sy[N - P] > 0 and sy[N - P] < sx[N - P]
+
+
+def test_tmz_point_source_symmetry():
+    n = 25
+    c = n // 2
+    cfg = parse_args(f"--2d-tmz --size x:{n},y:{n} --point-source x:{c},y:{c} --point-source-ez --time-steps 10".split())
+    # the Yee stencil spreads one cell per step: after 10 steps no value has reached the
+    # outer boundary, whose treatment is not mirror-symmetric (H at x = 1 is never updated)
+    o = _run(cfg)
+    ez, hx, hy = o.field("Ez"), o.field("Hx"), o.field("Hy")
+    assert np.abs(ez).max() > 0
+    np.testing.assert_allclose(ez, ez[::-1, :], rtol=0, atol=1e-12 * np.abs(ez).max())
+    np.testing.assert_allclose(ez, ez[:, ::-1], rtol=0, atol=1e-12 * np.abs(ez).max())
+    np.testing.assert_allclose(ez, ez.T, rtol=0, atol=1e-12 * np.abs(ez).max())
+    # x <-> y exchange maps Hx onto -Hy (curl orientation)
+    np.testing.assert_allclose(hx, -hy.T, rtol=0, atol=1e-12 * np.abs(hx).max())

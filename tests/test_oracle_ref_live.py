@@ -1,0 +1,49 @@
+"""The C oracle against the reference-arithmetic driver, run live on more inputs.
+
+Runs only where oracle/_ref/refdrv_f64 exists (it is built from the reference
+sources, which are not available on every machine).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fdtd3d_amd.config import parse_args
+from oracle.binding import Oracle, ref_driver_path, run_ref_driver, sphere_map
+from tests.helpers import field_names
+
+pytestmark = pytest.mark.skipif(not os.path.exists(ref_driver_path("f64")), reason="reference driver not built")
+
+CASES = [
+    # oblique 3D with uneven sizes and borders, stair sphere
+    "--3d --size x:15,y:17,z:13 --use-pml --pml-size x:3,y:2,z:4 --use-tfsf --tfsf-size-left x:4,y:5,z:5 "
+    "--tfsf-size-right x:5,y:4,z:6 --angle-teta 60 --angle-phi 20 --angle-psi 30 --eps-sphere 2.5 "
+    "--eps-sphere-center x:7,y:8,z:6 --eps-sphere-radius 3 --stair-sphere-approx --time-steps 20",
+    # 45-degree special case of the wave number, mu sphere
+    "--3d --size x:14 --same-size --use-tfsf --tfsf-size-left x:4 --tfsf-size-right x:4 --same-size-tfsf "
+    "--angle-teta 90 --angle-phi 45 --angle-psi 0 --mu-sphere 2 --mu-sphere-center x:7,y:7,z:7 "
+    "--mu-sphere-radius 3 --time-steps 20",
+    # TMz with every point source component it has and PML
+    "--2d-tmz --size x:30,y:26 --use-pml --pml-size x:5,y:3 --point-source x:12,y:9 --point-source-ez "
+    "--point-source-hx --point-source-hy --time-steps 40",
+    # TMz at phi = 0 with courant 0.25 and a different wavelength
+    "--2d-tmz --size x:28,y:30 --use-pml --pml-size x:3,y:3 --use-tfsf --tfsf-size-left x:6,y:6 "
+    "--tfsf-size-right x:6,y:6 --angle-teta 90 --angle-phi 0 --angle-psi 90 --courant-factor 0.25 "
+    "--wavelength 0.01 --time-steps 60",
+]
+
+
+@pytest.mark.parametrize("args", CASES)
+def test_oracle_matches_reference_driver(args, tmp_path):
+    argv = args.split()
+    cfg = parse_args(argv)
+    rec, _ = run_ref_driver(argv, str(tmp_path / "r.bin"))
+    o = Oracle(cfg)
+    if cfg.eps_sphere != 1.0:
+        o.set_material("eps", sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius))
+    if cfg.mu_sphere != 1.0:
+        o.set_material("mu", sphere_map(cfg, cfg.mu_sphere, 1.0, cfg.mu_sphere_center, cfg.mu_sphere_radius))
+    o.prepare()
+    o.step(cfg.time_steps)
+    for name in field_names(cfg, aux=True):
+        assert np.array_equal(o.field(name).ravel(), rec[name]), name
