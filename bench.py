@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the Yee time-stepping engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2]
+
+Default workload (the configuration the headline metric is quoted on): 3D 1024^3 FP32,
+PML 16 cells, TF/SF plane wave lambda = 40 cells at theta = 30, phi = 45 (psi = 90),
+Courant 0.5, eps piecewise constant over 8^3 blocks drawn U[1, 12] from a fixed seed,
+mu = 1.  A "step" is one Yee time step of the whole grid; the timed region runs K steps
+with every input resident in HBM.  Prints one JSON line.
+
+roofline: algorithmic bytes per cell-update B_alg = w (12 + 12 f_pml + 36 f_drude) + 1
+(every state word read and written once per step, plus a 1-byte material class), times
+the cells of one step, divided by the device time of one step measured with HIP events
+on the engine's stream over the timed region, against the 8 TB/s HBM3E peak.
+
+cpu_baseline: the reference's own per-cell arithmetic (oracle/_ref/refdrv_f32_fast, built
+from the reference sources) timed on one host core on a bounded sample of the same
+workload; the C oracle stands in where that binary is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fdtd3d_amd.config import Config, SCHEME_DIM3, SCHEME_TMZ, stof  # noqa: E402
+
+METRIC = "3D Yee Mcell-updates/s + % HBM roofline (1 GPU); adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+EPS_SEED = 0xFD7D3D
+
+
+def cfg2(n=1024, precision=4, pml=16, inset=24, steps=1000) -> Config:
+    return Config(scheme=SCHEME_DIM3, precision=precision, time_steps=steps, size=[n, n, n], pml=[pml] * 3,
+                  tfsf_left=[inset] * 3, tfsf_right=[inset] * 3, dx=stof("0.0005"), wavelength=stof("0.02"),
+                  courant=stof("0.5"), angle_deg=[stof("30"), stof("45"), stof("90")], use_pml=True, use_tfsf=True)
+
+
+def block_eps(shape, block=8, lo=1.0, hi=12.0, seed=EPS_SEED, dtype=np.float32):
+    """eps piecewise constant over block^3 cubes, i.i.d. U[lo, hi] per cube."""
+    rng = np.random.default_rng(seed)
+    nb = [(s + block - 1) // block for s in shape]
+    vals = rng.uniform(lo, hi, size=nb).astype(dtype)
+    out = np.repeat(np.repeat(np.repeat(vals, block, axis=0), block, axis=1), block, axis=2)
+    return np.ascontiguousarray(out[: shape[0], : shape[1], : shape[2]])
+
+
+WORKLOADS = {
+    "cfg2": dict(desc="3D 1024^3 FP32, PML 16, TF/SF lambda=40 cells theta=30 phi=45 psi=90, Courant 0.5, "
+                      "eps 8^3 blocks U[1,12] seed 0xFD7D3D, mu=1",
+                 make=lambda: cfg2()),
+    "cfg2-768-fp64": dict(desc="3D 768^3 FP64, otherwise as cfg2", make=lambda: cfg2(n=768, precision=8)),
+    "cfg2-512": dict(desc="3D 512^3 FP32, otherwise as cfg2", make=lambda: cfg2(n=512)),
+}
+
+
+def f_pml(cfg: Config) -> float:
+    inner = 1.0
+    for a in range(cfg.dim):
+        inner *= max(0.0, 1.0 - 2.0 * cfg.pml[a] / cfg.size[a])
+    return 1.0 - inner if cfg.use_pml else 0.0
+
+
+def b_alg(cfg: Config, f_drude: float = 0.0) -> float:
+    w = cfg.precision
+    if cfg.dim == 3:
+        return w * (12 + 12 * f_pml(cfg) + 36 * f_drude) + 1
+    return w * (6 + 6 * f_pml(cfg)) + 1
+
+
+def cpu_baseline(sample_n: int = 96, sample_steps: int = 31) -> dict:
+    """Reference arithmetic on one core, config-2 features at sample_n^3 (T-1 updates)."""
+    from oracle.binding import ref_driver_path, run_ref_driver
+    cells = sample_n ** 3
+    upd = cells * (sample_steps - 1)
+    sample = (f"{sample_n}^3 FP32, PML 16, TF/SF inset 24, theta=30 phi=45 psi=90, lambda=40 cells, "
+              f"8^3-block eps U[1,12], {sample_steps} time steps ({sample_steps - 1} field updates)")
+    drv = ref_driver_path("f32_fast")
+    if os.path.exists(drv):
+        with tempfile.TemporaryDirectory() as tmp:
+            epsf = os.path.join(tmp, "eps.dat")
+            block_eps((sample_n,) * 3, dtype=np.float32).tofile(epsf)
+            argv = (f"--3d --size x:{sample_n} --same-size --dx 0.0005 --wavelength 0.02 --courant-factor 0.5 "
+                    f"--use-pml --pml-size x:16 --same-size-pml --use-tfsf --tfsf-size-left x:24 "
+                    f"--tfsf-size-right x:24 --same-size-tfsf --angle-teta 30 --angle-phi 45 --angle-psi 90 "
+                    f"--load-eps-from-file {epsf} --time-steps {sample_steps}").split()
+            _, secs = run_ref_driver(argv, os.path.join(tmp, "out.bin"), kind="f32_fast", timeout=600)
+        return {"value": upd / secs / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "reference",
+                "sample": sample + "; reference InternalScheme per-cell code (default path), g++ -O3"}
+    from fdtd3d_amd.config import Config as _C
+    from oracle.binding import Oracle
+    c = cfg2(n=sample_n, precision=8, steps=sample_steps)
+    o = Oracle(c)
+    o.set_material("eps", block_eps((sample_n,) * 3, dtype=np.float64))
+    o.prepare()
+    t0 = time.perf_counter()
+    o.step(sample_steps)
+    secs = time.perf_counter() - t0
+    return {"value": upd / secs / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": sample + "; C oracle FP64 (reference driver not built)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the grid is not sharded across GPUs; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    from fdtd3d_amd.engine import Engine, device_info
+
+    wl = WORKLOADS[args.workload]
+    cfg = wl["make"]()
+    cells = int(np.prod(cfg.shape))
+    print(f"[bench] {device_info()}", file=sys.stderr, flush=True)
+    print(f"[bench] workload {args.workload}: {wl['desc']}", file=sys.stderr, flush=True)
+    t_setup = time.perf_counter()
+    eps = block_eps(cfg.shape, dtype=np.float32 if cfg.precision == 4 else np.float64)
+    eng = Engine(cfg)
+    eng.set_material("eps", eps)
+    del eps
+    eng.prepare()
+    print(f"[bench] set-up {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    eng.step(args.warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    dev_ms = eng.last_step_ms()          # HIP events on the engine stream over the timed region
+    ms_per_step = wall * 1e3 / args.steps
+    value = cells * args.steps / wall / 1e6
+    prof = eng.profile_steps(min(args.steps, 20))
+    print(f"[bench] {args.steps} steps in {wall:.3f} s; device {dev_ms:.3f} ms/step; kernels {prof}",
+          file=sys.stderr, flush=True)
+
+    balg = b_alg(cfg)
+    achieved = balg * cells / (dev_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "one time step (incident_step + e_half_3d + h_half_3d)",
+                "b_alg_bytes_per_cell_update": round(balg, 2),
+                "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline()
+        except Exception as exc:  # the GPU figure stands on its own
+            cpu = {"error": str(exc)[:200]}
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if cfg.precision == 4 else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "description": wl["desc"], "grid": list(cfg.shape),
+                   "pml": cfg.pml[0], "tfsf_inset": cfg.tfsf_left[0], "time_steps_of_run": cfg.time_steps,
+                   "cells": cells},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -75,6 +75,9 @@ struct yee_handle
   bool prepared = false;
   double last_ms = 0;
   int chunk = 1, nchunks = 1;
+  // kernel timing (yee_profile_steps): events recorded around each launch when set
+  std::vector<hipEvent_t> *prof_events = nullptr;
+  std::vector<int> *prof_kinds = nullptr;
 
   bool fp32 () const { return wsize == 4; }
   bool use_pml () const { return (cfg.flags & YEE_USE_PML) != 0; }
@@ -201,6 +204,32 @@ int launch_sources (yee_handle *h, bool e_half, int32_t t)
   return YEE_OK;
 }
 
+// Records an event pair around the next launch when kernel timing is on.
+struct ProfScope
+{
+  yee_handle *h;
+  int kind;
+  bool on;
+  ProfScope (yee_handle *hh, int k) : h (hh), kind (k), on (hh->prof_events != nullptr)
+  {
+    if (on) record ();
+  }
+  ~ProfScope ()
+  {
+    if (on) record ();
+  }
+  void record ()
+  {
+    hipEvent_t e;
+    if (hipEventCreate (&e) == hipSuccess)
+    {
+      (void) hipEventRecord (e, h->stream);
+      h->prof_events->push_back (e);
+      h->prof_kinds->push_back (kind);
+    }
+  }
+};
+
 template <typename T>
 int step_once (yee_handle *h)
 {
@@ -212,15 +241,24 @@ int step_once (yee_handle *h)
     const double modE = h->sc.dt / (h->sc.rel * eps0 () * h->cfg.dx);
     const double modH = h->sc.dt / (h->sc.rel * mu0 () * h->cfg.dx);
     const double arg = h->sc.dt * t * 2 * kPi * h->sc.freq;
+    ProfScope ps (h, 0);
     hipLaunchKernelGGL ((incident_step<T>), dim3 (1), dim3 (1024), 0, h->stream, (T *) h->einc,
                        (const T *) h->hinc[prev], (T *) h->hinc[cur], h->inc_len, (int) t, modE, modH, std::sin (arg));
     HIP_TRY (hipGetLastError ());
   }
   int rc;
   // the reference performs no field update at t == 0 (Scheme.inc.h:1032)
-  if (t > 0 && (rc = launch_half<T> (h, true, prev))) return rc;
+  if (t > 0)
+  {
+    ProfScope ps (h, 1);
+    if ((rc = launch_half<T> (h, true, prev))) return rc;
+  }
   if ((rc = launch_sources<T> (h, true, t))) return rc;
-  if (t > 0 && (rc = launch_half<T> (h, false, prev))) return rc;
+  if (t > 0)
+  {
+    ProfScope ps (h, 2);
+    if ((rc = launch_half<T> (h, false, prev))) return rc;
+  }
   if ((rc = launch_sources<T> (h, false, t))) return rc;
   h->t++;
   return YEE_OK;
@@ -283,6 +321,91 @@ int validate (const yee_config *c)
     return fail (YEE_EINVAL, "metamaterials require PML (reference: Scheme.inc.h:889-892)");
   if (c->time_steps < 0) return fail (YEE_EINVAL, "negative time steps");
   return YEE_OK;
+}
+
+template <typename S>
+int upload_as (yee_handle *h, void **dst, const S *src, size_t n)
+{
+  if (!*dst)
+  {
+    HIP_TRY (hipMalloc (dst, n * h->wsize));
+  }
+  if (h->fp32 ())
+  {
+    std::vector<float> v (n);
+    for (size_t q = 0; q < n; ++q) v[q] = (float) src[q];
+    HIP_TRY (hipMemcpy (*dst, v.data (), n * 4, hipMemcpyHostToDevice));
+  }
+  else
+  {
+    std::vector<double> v (n);
+    for (size_t q = 0; q < n; ++q) v[q] = (double) src[q];
+    HIP_TRY (hipMemcpy (*dst, v.data (), n * 8, hipMemcpyHostToDevice));
+  }
+  return YEE_OK;
+}
+
+// Scans one material grid: uniform grids become a scalar, nonzero Drude grids extend the
+// Drude box, everything else is converted to the field precision and uploaded.
+template <typename S>
+int set_material_impl (yee_handle *h, int32_t which, const S *values)
+{
+  if (!h || !values) return fail (YEE_EINVAL, "null argument");
+  if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
+  if (h->prepared) return fail (YEE_ESTATE, "materials must be set before yee_prepare");
+  const size_t n = (size_t) h->ncell;
+  bool uniform = true;
+  bool nonzero = false;
+  int64_t lo[3] = { h->n[0], h->n[1], h->n[2] }, hi[3] = { -1, -1, -1 };
+  for (size_t q = 0; q < n; ++q)
+  {
+    if (!std::isfinite ((double) values[q])) return fail (YEE_EINVAL, "material values must be finite");
+    if (values[q] != values[0]) uniform = false;
+    if (values[q] != 0)
+    {
+      nonzero = true;
+      if (which >= 2)
+      {
+        const int64_t k = (int64_t) (q % h->n[2]);
+        const int64_t j = (int64_t) ((q / h->n[2]) % h->n[1]);
+        const int64_t i = (int64_t) (q / ((size_t) h->n[2] * h->n[1]));
+        const int64_t p[3] = { i, j, k };
+        for (int a = 0; a < 3; ++a)
+        {
+          lo[a] = std::min (lo[a], p[a]);
+          hi[a] = std::max (hi[a], p[a]);
+        }
+      }
+    }
+  }
+  if ((which == YEE_MAT_EPS || which == YEE_MAT_MU))
+  {
+    for (size_t q = 0; q < n; ++q)
+      if (values[q] == 0) return fail (YEE_EINVAL, "eps/mu must be nonzero everywhere");
+  }
+  h->mat_nonzero[which] = nonzero;
+  if (which >= 2 && nonzero)
+  {
+    // field cells whose averages reach a nonzero Drude cell: p in [lo - 1, hi]
+    for (int a = 0; a < 3; ++a)
+    {
+      const int64_t l = lo[a] - 1, u = hi[a] + 1;
+      h->box_lo[a] = h->box_any ? std::min (h->box_lo[a], l) : l;
+      h->box_hi[a] = h->box_any ? std::max (h->box_hi[a], u) : u;
+    }
+    h->box_any = true;
+  }
+  if ((uniform && which < 2) || !nonzero)
+  {
+    if (which < 2) h->mat_u[which] = (double) values[0];
+    if (h->mat[which])
+    {
+      (void) hipFree (h->mat[which]);
+      h->mat[which] = nullptr;
+    }
+    return YEE_OK;
+  }
+  return upload_as (h, &h->mat[which], values, n);
 }
 
 }  // namespace
@@ -379,77 +502,14 @@ void yee_destroy (yee_handle *h)
   delete h;
 }
 
+int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values)
+{
+  return set_material_impl (h, which, values);
+}
+
 int yee_set_material (yee_handle *h, int32_t which, const double *values)
 {
-  if (!h || !values) return fail (YEE_EINVAL, "null argument");
-  if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
-  if (h->prepared) return fail (YEE_ESTATE, "materials must be set before yee_prepare");
-  const size_t n = (size_t) h->ncell;
-  bool uniform = true;
-  bool nonzero = false;
-  int64_t lo[3] = { h->n[0], h->n[1], h->n[2] }, hi[3] = { -1, -1, -1 };
-  for (size_t q = 0; q < n; ++q)
-  {
-    if (!std::isfinite (values[q])) return fail (YEE_EINVAL, "material values must be finite");
-    if (values[q] != values[0]) uniform = false;
-    if (values[q] != 0.0)
-    {
-      nonzero = true;
-      if (which >= 2)
-      {
-        const int64_t k = (int64_t) (q % h->n[2]);
-        const int64_t j = (int64_t) ((q / h->n[2]) % h->n[1]);
-        const int64_t i = (int64_t) (q / ((size_t) h->n[2] * h->n[1]));
-        const int64_t p[3] = { i, j, k };
-        for (int a = 0; a < 3; ++a)
-        {
-          lo[a] = std::min (lo[a], p[a]);
-          hi[a] = std::max (hi[a], p[a]);
-        }
-      }
-    }
-  }
-  if ((which == YEE_MAT_EPS || which == YEE_MAT_MU) && !nonzero) return fail (YEE_EINVAL, "eps/mu must be nonzero");
-  h->mat_nonzero[which] = nonzero;
-  if (which >= 2 && nonzero)
-  {
-    // field cells whose averages reach a nonzero Drude cell: p in [lo - 1, hi]
-    for (int a = 0; a < 3; ++a)
-    {
-      const int64_t l = lo[a] - 1, u = hi[a] + 1;
-      if (!h->box_any)
-      {
-        h->box_lo[a] = l;
-        h->box_hi[a] = u;
-      }
-      else
-      {
-        h->box_lo[a] = std::min (h->box_lo[a], l);
-        h->box_hi[a] = std::max (h->box_hi[a], u);
-      }
-    }
-    h->box_any = true;
-  }
-  if (uniform && which < 2)
-  {
-    h->mat_u[which] = values[0];
-    if (h->mat[which])
-    {
-      (void) hipFree (h->mat[which]);
-      h->mat[which] = nullptr;
-    }
-    return YEE_OK;
-  }
-  if (!nonzero)
-  {
-    if (h->mat[which])
-    {
-      (void) hipFree (h->mat[which]);
-      h->mat[which] = nullptr;
-    }
-    return YEE_OK;
-  }
-  return upload (h, &h->mat[which], values, n);
+  return set_material_impl (h, which, values);
 }
 
 int yee_prepare (yee_handle *h)
@@ -500,6 +560,38 @@ int yee_step (yee_handle *h, int32_t n)
   float ms = 0;
   HIP_TRY (hipEventElapsedTime (&ms, h->ev[0], h->ev[1]));
   h->last_ms = n > 0 ? ms / n : 0;
+  return YEE_OK;
+}
+
+int yee_profile_steps (yee_handle *h, int32_t n, double out[4])
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  if (n <= 0) return fail (YEE_EINVAL, "step count must be positive");
+  std::vector<hipEvent_t> events;
+  std::vector<int> kinds;
+  events.reserve ((size_t) n * 6);
+  kinds.reserve ((size_t) n * 6);
+  h->prof_events = &events;
+  h->prof_kinds = &kinds;
+  int rc = yee_step (h, n);
+  h->prof_events = nullptr;
+  h->prof_kinds = nullptr;
+  double sum[3] = { 0, 0, 0 };
+  int cnt[3] = { 0, 0, 0 };
+  for (size_t e = 0; rc == YEE_OK && e + 1 < events.size (); e += 2)
+  {
+    float ms = 0;
+    if (hipEventElapsedTime (&ms, events[e], events[e + 1]) == hipSuccess)
+    {
+      sum[kinds[e]] += ms;
+      cnt[kinds[e]]++;
+    }
+  }
+  for (hipEvent_t e : events) (void) hipEventDestroy (e);
+  if (rc) return rc;
+  for (int k = 0; k < 3; ++k) out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+  out[3] = h->last_ms;
   return YEE_OK;
 }
 

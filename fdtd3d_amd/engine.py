@@ -27,7 +27,8 @@ LIB_PATH = os.path.join(HERE, "libyee.so")
 EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw",
-    "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms",
+    "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
+    "yee_set_material_f32",
 ]
 
 
@@ -66,6 +67,8 @@ def load_library():
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.yee_get_scalars.argtypes = [vp, vp]
     L.yee_last_step_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.yee_profile_steps.argtypes = [vp, i32, vp]
+    L.yee_set_material_f32.argtypes = [vp, i32, vp]
     for name in EXPORTED:
         if name not in ("yee_last_error", "yee_destroy"):
             getattr(L, name).restype = ctypes.c_int
@@ -115,9 +118,13 @@ class Engine:
         self.close()
 
     def set_material(self, name: str, values: np.ndarray) -> None:
+        if int(np.size(values)) != int(np.prod(self.shape)):
+            raise ValueError(f"material {name}: expected {self.shape}, got {np.shape(values)}")
+        if isinstance(values, np.ndarray) and values.dtype == np.float32:
+            arr = np.ascontiguousarray(values)
+            _check(load_library().yee_set_material_f32(self.h, MAT_INDEX[name], arr.ctypes.data))
+            return
         arr = np.ascontiguousarray(values, dtype=np.float64)
-        if arr.size != int(np.prod(self.shape)):
-            raise ValueError(f"material {name}: expected {self.shape}, got {arr.shape}")
         _check(load_library().yee_set_material(self.h, MAT_INDEX[name], arr.ctypes.data))
 
     def prepare(self) -> None:
@@ -139,6 +146,12 @@ class Engine:
         ms = ctypes.c_double()
         _check(load_library().yee_last_step_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def profile_steps(self, n: int) -> dict:
+        """Runs n steps with per-kernel HIP events; average ms per launch of each kernel."""
+        out = np.zeros(4, dtype=np.float64)
+        _check(load_library().yee_profile_steps(self.h, int(n), out.ctypes.data))
+        return {"incident_ms": out[0], "e_half_ms": out[1], "h_half_ms": out[2], "step_ms": out[3]}
 
     def field(self, name: str) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.float64)

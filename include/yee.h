@@ -120,6 +120,9 @@ void yee_destroy (yee_handle *h);
  */
 int yee_set_material (yee_handle *h, int32_t which, const double *values);
 
+/* Same as yee_set_material from an FP32 array (the reference's -DFLOAT_VALUES grids). */
+int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values);
+
 /*
  * Host FP64 set-up of everything the kernels read: PML sigma profiles
  * (SchemeHelper::initSigma{X,Y,Z}, Source/Scheme/SchemeHelper.h:574-718), the material
@@ -146,7 +149,7 @@ int yee_time (yee_handle *h, int32_t *t);
  * ([x][y][z], or [x][y] for TMz).  Components absent from the scheme return YEE_EINVAL.
  * D/B (and D1/B1) hold the reference's values inside the PML slabs; in the interior the
  * engine updates E/H directly, which equals the reference's D->E stage with sigma = 0
- * up to rounding (see DESIGN.md), and D/B read back as 0 there.
+ * up to rounding, and D/B read back as 0 there.
  */
 int yee_get_field (yee_handle *h, int32_t comp, double *out);
 
@@ -173,6 +176,14 @@ int yee_get_scalars (yee_handle *h, double out[3]);
  * over the last yee_step call, measured with HIP events on the handle's stream.
  */
 int yee_last_step_ms (yee_handle *h, double *ms);
+
+/*
+ * Runs `n` steps like yee_step with a HIP event pair around every kernel launch and
+ * returns average device milliseconds per launch: out[0] incident 1D update, out[1] E
+ * half-step, out[2] H half-step, out[3] whole step (first event to last event / n).
+ * Benchmark hook; the extra events cost a few microseconds per step.
+ */
+int yee_profile_steps (yee_handle *h, int32_t n, double out[4]);
 
 #ifdef __cplusplus
 }
