@@ -61,7 +61,15 @@ WORKLOADS = {
                  make=lambda: cfg2()),
     "cfg2-768-fp64": dict(desc="3D 768^3 FP64, otherwise as cfg2", make=lambda: cfg2(n=768, precision=8)),
     "cfg2-512": dict(desc="3D 512^3 FP32, otherwise as cfg2", make=lambda: cfg2(n=512)),
+    "cfg2-plain": dict(desc="3D 1024^3 FP32 without PML and TF/SF (stencil only), eps as cfg2",
+                       make=lambda: _plain(cfg2())),
 }
+
+
+def _plain(c: Config) -> Config:
+    c.use_pml = False
+    c.use_tfsf = False
+    return c
 
 
 def f_pml(cfg: Config) -> float:

@@ -59,7 +59,11 @@ struct yee_handle
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = { nullptr, nullptr };
 
-  void *f[6] = {}, *d[6] = {}, *dp[6] = {}, *d1[6] = {}, *d1p[6] = {};
+  // state: two buffers per array when the fused 3D step is used (old -> new), else one
+  void *f[2][6] = {}, *d[2][6] = {}, *dp[2][6] = {}, *d1[2][6] = {}, *d1p[2][6] = {};
+  int cur = 0;          // buffer holding the newest state
+  bool fused = false;   // 3D fused step kernel (else in-place half-step kernels)
+
   void *mat[6] = {};
   double mat_u[6] = { 1, 1, 0, 0, 0, 0 };
   bool mat_nonzero[6] = {};
@@ -74,7 +78,6 @@ struct yee_handle
   int32_t t = 0;
   bool prepared = false;
   double last_ms = 0;
-  int chunk = 1, nchunks = 1;
   // kernel timing (yee_profile_steps): events recorded around each launch when set
   std::vector<hipEvent_t> *prof_events = nullptr;
   std::vector<int> *prof_kinds = nullptr;
@@ -96,7 +99,6 @@ StepArgs<T> make_args (const yee_handle *h, int parity)
   a.nz = (int) h->n[2];
   a.sx = h->n[1] * h->n[2];
   a.sy = (int) h->n[2];
-  a.chunk = h->chunk;
   for (int ax = 0; ax < 3; ++ax)
   {
     // PML two-stage cells: p < P or p >= N - P - 1 (every averaged sigma is zero elsewhere)
@@ -113,13 +115,19 @@ StepArgs<T> make_args (const yee_handle *h, int parity)
     a.slab_lo[2] = -1;
     a.slab_hi[2] = 1 << 30;
   }
+  const int src = h->cur, dst = h->fused ? 1 - h->cur : h->cur;
   for (int c = 0; c < 6; ++c)
   {
-    a.f[c] = (T *) h->f[c];
-    a.d[c] = (T *) h->d[c];
-    a.dp[c] = (T *) h->dp[c];
-    a.d1[c] = (T *) h->d1[c];
-    a.d1p[c] = (T *) h->d1p[c];
+    a.f[c] = (T *) h->f[src][c];
+    a.d[c] = (T *) h->d[src][c];
+    a.dp[c] = (T *) h->dp[src][c];
+    a.d1[c] = (T *) h->d1[src][c];
+    a.d1p[c] = (T *) h->d1p[src][c];
+    a.f_n[c] = (T *) h->f[dst][c];
+    a.d_n[c] = (T *) h->d[dst][c];
+    a.dp_n[c] = (T *) h->dp[dst][c];
+    a.d1_n[c] = (T *) h->d1[dst][c];
+    a.d1p_n[c] = (T *) h->d1p[dst][c];
     a.proj[c] = h->sc.proj[c];
   }
   a.eps = (const T *) h->mat[YEE_MAT_EPS];
@@ -155,7 +163,7 @@ int launch_half (yee_handle *h, bool e_half, int parity)
   if (h->dim == 3)
   {
     dim3 block (64, 4, 1);
-    dim3 grid ((unsigned) ((h->n[2] + 63) / 64), (unsigned) ((h->n[1] + 3) / 4), (unsigned) h->nchunks);
+    dim3 grid ((unsigned) ((h->n[2] + 63) / 64), (unsigned) ((h->n[1] + 3) / 4), (unsigned) (h->n[0] - 2));
     switch (h->mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
@@ -198,7 +206,7 @@ int launch_sources (yee_handle *h, bool e_half, int32_t t)
     const int64_t q = (p[0] * h->n[1] + p[1]) * h->n[2] + pz;
     const double ts = e_half ? t + 0.5 : t + 1.0;
     const double arg = h->sc.dt * ts * 2 * kPi * h->sc.freq;
-    hipLaunchKernelGGL ((point_source<T>), dim3 (1), dim3 (64), 0, h->stream, (T *) h->f[c], q, std::sin (arg));
+    hipLaunchKernelGGL ((point_source<T>), dim3 (1), dim3 (64), 0, h->stream, (T *) h->f[h->cur][c], q, std::sin (arg));
     HIP_TRY (hipGetLastError ());
   }
   return YEE_OK;
@@ -247,7 +255,6 @@ int step_once (yee_handle *h)
     HIP_TRY (hipGetLastError ());
   }
   int rc;
-  // the reference performs no field update at t == 0 (Scheme.inc.h:1032)
   if (t > 0)
   {
     ProfScope ps (h, 1);
@@ -455,18 +462,22 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (hipEventCreate (&h->ev[0]) != hipSuccess || hipEventCreate (&h->ev[1]) != hipSuccess)
     return bail (fail (YEE_EDEVICE, "hipEventCreate failed"));
   const size_t bytes = (size_t) h->ncell * h->wsize;
-  for (int c = 0; c < 6; ++c)
-  {
-    if (!h->present[c]) continue;
-    if ((rc = alloc_zero (h, &h->f[c], bytes))) return bail (rc);
-    if (h->mode >= 1 && (rc = alloc_zero (h, &h->d[c], bytes))) return bail (rc);
-    if (h->mode == 2)
+  // one buffer per array: the half-step kernels update in place
+  h->fused = false;
+  const int nbuf = 1;
+  for (int bsel = 0; bsel < nbuf; ++bsel)
+    for (int c = 0; c < 6; ++c)
     {
-      if ((rc = alloc_zero (h, &h->dp[c], bytes))) return bail (rc);
-      if ((rc = alloc_zero (h, &h->d1[c], bytes))) return bail (rc);
-      if ((rc = alloc_zero (h, &h->d1p[c], bytes))) return bail (rc);
+      if (!h->present[c]) continue;
+      if ((rc = alloc_zero (h, &h->f[bsel][c], bytes))) return bail (rc);
+      if (h->mode >= 1 && (rc = alloc_zero (h, &h->d[bsel][c], bytes))) return bail (rc);
+      if (h->mode == 2)
+      {
+        if ((rc = alloc_zero (h, &h->dp[bsel][c], bytes))) return bail (rc);
+        if ((rc = alloc_zero (h, &h->d1[bsel][c], bytes))) return bail (rc);
+        if ((rc = alloc_zero (h, &h->d1p[bsel][c], bytes))) return bail (rc);
+      }
     }
-  }
   if (h->use_tfsf ())
   {
     // allocateGridsInc: 500 * size.x cells (InternalScheme.specific.h:99-107)
@@ -487,9 +498,12 @@ void yee_destroy (yee_handle *h)
   if (h->stream) (void) hipStreamSynchronize (h->stream);
   for (int c = 0; c < 6; ++c)
   {
-    void *ptrs[5] = { h->f[c], h->d[c], h->dp[c], h->d1[c], h->d1p[c] };
-    for (void *p : ptrs)
-      if (p) (void) hipFree (p);
+    for (int bsel = 0; bsel < 2; ++bsel)
+    {
+      void *ptrs[5] = { h->f[bsel][c], h->d[bsel][c], h->dp[bsel][c], h->d1[bsel][c], h->d1p[bsel][c] };
+      for (void *p : ptrs)
+        if (p) (void) hipFree (p);
+    }
     if (h->mat[c]) (void) hipFree (h->mat[c]);
   }
   if (h->pml) (void) hipFree (h->pml);
@@ -525,13 +539,6 @@ int yee_prepare (yee_handle *h)
       h->box_hi[a] = 0;
     }
   }
-  // x-march chunking: enough (y, z) columns x chunks to fill the GPU
-  const int64_t cols = h->dim == 3 ? std::max<int64_t> (1, (h->n[1] - 2) * (h->n[2] - 2)) : 1;
-  const int64_t planes = std::max<int64_t> (1, h->n[0] - 2);
-  int64_t nch = std::max<int64_t> (1, (int64_t) 262144 / cols);
-  nch = std::min (nch, planes);
-  h->chunk = (int) ((planes + nch - 1) / nch);
-  h->nchunks = (int) ((planes + h->chunk - 1) / h->chunk);
   if (h->use_pml ())
   {
     h->tab_len = (int) std::max (h->n[0], std::max (h->n[1], h->n[2]));
@@ -622,7 +629,7 @@ int yee_get_field_raw (yee_handle *h, int32_t comp, void *out)
   if (comp < 0 || comp > 17) return fail (YEE_EINVAL, "unknown component");
   const int base = comp % 6;
   if (!h->present[base]) return fail (YEE_EINVAL, "component not present in this scheme");
-  void *src = comp < 6 ? h->f[base] : (comp < 12 ? h->d[base] : h->d1[base]);
+  void *src = comp < 6 ? h->f[h->cur][base] : (comp < 12 ? h->d[h->cur][base] : h->d1[h->cur][base]);
   if (!src) return fail (YEE_EINVAL, "component not allocated (enable PML / metamaterials)");
   HIP_TRY (hipStreamSynchronize (h->stream));
   HIP_TRY (hipMemcpy (out, src, (size_t) h->ncell * h->wsize, hipMemcpyDeviceToHost));

@@ -15,19 +15,21 @@ struct StepArgs
   int nx, ny, nz;
   int64_t sx;   // ny * nz
   int sy;       // nz
-  int chunk;    // x-march chunk length per thread
 
   // cells that run the reference's two-stage D/B update: the PML slabs
   // (p < slab_lo[a] or p >= slab_hi[a]) and, with metamaterials, the Drude box
   int slab_lo[3], slab_hi[3];
   int box_lo[3], box_hi[3];
 
-  // state
+  // state read by this half-step / step
   T *f[6];      // Ex, Ey, Ez, Hx, Hy, Hz
   T *d[6];      // D/B (PML and Drude cells)
   T *dp[6];     // D/B one step older (Drude)
   T *d1[6];     // D1/B1 (Drude)
   T *d1p[6];    // D1/B1 one step older (Drude)
+  // state written (the same arrays for the in-place half-step kernels, the other
+  // buffer of each pair for the fused step kernel)
+  T *f_n[6], *d_n[6], *dp_n[6], *d1_n[6], *d1p_n[6];
 
   // materials: per-cell arrays, or a uniform value when the pointer is null
   const T *eps, *mu, *ope, *opm, *ge, *gm;

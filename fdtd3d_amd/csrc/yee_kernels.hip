@@ -1,8 +1,7 @@
 // Yee E/H half-step kernels for gfx950 (3D and TMz, E_CENTERED layout).
 //
-// One thread owns a (y, z) column and marches along x over a chunk of planes, so the
-// x-neighbours of the stencil stay in registers; z is the lane-contiguous axis, so every
-// plane access of a wavefront is one coalesced 256-byte (FP32) / 512-byte (FP64) row.
+// One thread per cell; z is the lane-contiguous axis, so every access of a wavefront is
+// one coalesced 256-byte (FP32) / 512-byte (FP64) row.
 //
 // Per cell and half-step the kernel applies, in the reference's order of operations:
 //   - the curl update                       calcField                 (InternalScheme.inc.h:765-772)
@@ -19,27 +18,45 @@
 
 namespace yee {
 
-// E_CENTERED component tables (device copies of the host CompDesc plus the circuit offsets
-// of YeeGridLayout.cpp:1937-1961).
-__constant__ int kAxisD[6] = { 0, 1, 2, 0, 1, 2 };
-// slot directions (LEFT=0, RIGHT=1, DOWN=2, UP=3, BACK=4, FRONT=5) of d11, d12, d21, d22
-__constant__ int kDirD[6][4] = {
-  { 2, 3, 4, 5 },   // Ex: DOWN UP BACK FRONT
-  { 4, 5, 0, 1 },   // Ey: BACK FRONT LEFT RIGHT
-  { 0, 1, 2, 3 },   // Ez: LEFT RIGHT DOWN UP
-  { 4, 5, 2, 3 },   // Hx: BACK FRONT DOWN UP
-  { 0, 1, 4, 5 },   // Hy: LEFT RIGHT BACK FRONT
-  { 2, 3, 0, 1 },   // Hz: DOWN UP LEFT RIGHT
-};
-__constant__ int kOppD[6][2] = { { 5, 4 }, { 3, 5 }, { 4, 3 }, { 1, 2 }, { 2, 0 }, { 0, 1 } };
-// offsets [slot][axis] of the four opposite-field operands
-__constant__ int kOffD[6][4][3] = {
-  { { 0, -1, 0 }, { 0, 0, 0 }, { 0, 0, -1 }, { 0, 0, 0 } },
-  { { 0, 0, -1 }, { 0, 0, 0 }, { -1, 0, 0 }, { 0, 0, 0 } },
-  { { -1, 0, 0 }, { 0, 0, 0 }, { 0, -1, 0 }, { 0, 0, 0 } },
-  { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 0, 0 }, { 0, 1, 0 } },
-  { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, 0 }, { 0, 0, 1 } },
-  { { 0, 0, 0 }, { 0, 1, 0 }, { 0, 0, 0 }, { 1, 0, 0 } },
+// Global-memory accessors: the state pointers live in the kernel argument block, where the
+// compiler cannot see their address space; casting to address space 1 keeps every access
+// a global_load/global_store instead of a flat one.
+template <typename T>
+__device__ __forceinline__ T gld (const T *p)
+{
+  return *(const __attribute__ ((address_space (1))) T *) (p);
+}
+
+template <typename T>
+__device__ __forceinline__ void gst (T *p, T v)
+{
+  *(__attribute__ ((address_space (1))) T *) (p) = v;
+}
+
+// E_CENTERED component tables (the host CompDesc plus the circuit offsets of
+// YeeGridLayout.cpp:1937-1961).  Every lookup below is resolved at compile time: runtime
+// indexing of small arrays would push them to scratch memory.
+struct Tab
+{
+  // slot directions (LEFT=0, RIGHT=1, DOWN=2, UP=3, BACK=4, FRONT=5) of d11, d12, d21, d22
+  static constexpr int dir[6][4] = {
+    { 2, 3, 4, 5 },   // Ex: DOWN UP BACK FRONT
+    { 4, 5, 0, 1 },   // Ey: BACK FRONT LEFT RIGHT
+    { 0, 1, 2, 3 },   // Ez: LEFT RIGHT DOWN UP
+    { 4, 5, 2, 3 },   // Hx: BACK FRONT DOWN UP
+    { 0, 1, 4, 5 },   // Hy: LEFT RIGHT BACK FRONT
+    { 2, 3, 0, 1 },   // Hz: DOWN UP LEFT RIGHT
+  };
+  static constexpr int opp[6][2] = { { 5, 4 }, { 3, 5 }, { 4, 3 }, { 1, 2 }, { 2, 0 }, { 0, 1 } };
+  // offsets [slot][axis] of the four opposite-field operands
+  static constexpr int off[6][4][3] = {
+    { { 0, -1, 0 }, { 0, 0, 0 }, { 0, 0, -1 }, { 0, 0, 0 } },
+    { { 0, 0, -1 }, { 0, 0, 0 }, { -1, 0, 0 }, { 0, 0, 0 } },
+    { { -1, 0, 0 }, { 0, 0, 0 }, { 0, -1, 0 }, { 0, 0, 0 } },
+    { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 0, 0 }, { 0, 1, 0 } },
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, 0 }, { 0, 0, 1 } },
+    { { 0, 0, 0 }, { 0, 1, 0 }, { 0, 0, 0 }, { 1, 0, 0 } },
+  };
 };
 
 // sigma axes (curl, sigma4, sigma5) per component
@@ -51,47 +68,66 @@ template <> struct SigAxes<3> { static constexpr int c = 1, s4 = 0, s5 = 2; };
 template <> struct SigAxes<4> { static constexpr int c = 2, s4 = 1, s5 = 0; };
 template <> struct SigAxes<5> { static constexpr int c = 0, s4 = 2, s5 = 1; };
 
-__device__ __forceinline__ int twice_coord_d (int comp, int axis, int p)
+template <int AX>
+__device__ __forceinline__ int pick (int i, int j, int k)
 {
-  const bool h = comp >= 3;
-  const int own = kAxisD[comp];
-  return 2 * p + (h ? (axis == own ? 1 : 2) : (axis == own ? 2 : 1));
+  return AX == 0 ? i : (AX == 1 ? j : k);
+}
+
+// Twice the real coordinate of component COMP along axis AX at integer index p.
+template <int COMP, int AX>
+__device__ __forceinline__ int twice_coord (int p)
+{
+  constexpr bool h = COMP >= 3;
+  constexpr int own = COMP % 3;
+  return 2 * p + (h ? (AX == own ? 1 : 2) : (AX == own ? 2 : 1));
 }
 
 // doNeedTFSFUpdate<comp>Border(pos, dir) on doubled integer coordinates (YeeGridLayout.cpp:272-1740;
 // the FP tests c == b +- 0.5, c in (L - 0.1, R + 0.1), c in (L + 0.4, R - 0.4) are exact on half-integers).
-template <typename T>
-__device__ bool tfsf_pred (const StepArgs<T> &a, int comp, int dir, const int p[3], int dim)
+template <typename T, int COMP, int DIR, int DIM>
+__device__ __forceinline__ bool tfsf_pred (const StepArgs<T> &a, int i, int j, int k)
 {
-  const int b = dir >> 1;
-  const bool low = (dir & 1) == 0;
-  const int ax = kAxisD[comp];
-  const bool h = comp >= 3;
-  if (b == ax || b >= dim) return false;
-  const int t = 3 - b - ax;
-  const int cb2 = twice_coord_d (comp, b, p[b]);
+  constexpr int b = DIR >> 1;
+  constexpr bool low = (DIR & 1) == 0;
+  constexpr int ax = COMP % 3;
+  constexpr bool h = COMP >= 3;
+  constexpr int t = 3 - b - ax;
+  if (b == ax || b >= DIM) return false;
+  const int cb2 = twice_coord<COMP, b> (pick<b> (i, j, k));
   bool ok = h ? (low ? cb2 == 2 * a.tl[b] : cb2 == 2 * a.tr[b])
               : (low ? cb2 == 2 * a.tl[b] - 1 : cb2 == 2 * a.tr[b] + 1);
-  if (ok && ax < dim)
+  if (ax < DIM)
   {
-    const int c2 = twice_coord_d (comp, ax, p[ax]);
-    ok = h ? (c2 >= 2 * a.tl[ax] + 1 && c2 <= 2 * a.tr[ax] - 1) : (c2 >= 2 * a.tl[ax] && c2 <= 2 * a.tr[ax]);
+    const int c2 = twice_coord<COMP, ax> (pick<ax> (i, j, k));
+    ok = ok && (h ? (c2 >= 2 * a.tl[ax] + 1 && c2 <= 2 * a.tr[ax] - 1) : (c2 >= 2 * a.tl[ax] && c2 <= 2 * a.tr[ax]));
   }
-  if (ok && t < dim)
+  if (t < DIM)
   {
-    const int c2 = twice_coord_d (comp, t, p[t]);
-    ok = h ? (c2 >= 2 * a.tl[t] && c2 <= 2 * a.tr[t]) : (c2 >= 2 * a.tl[t] + 1 && c2 <= 2 * a.tr[t] - 1);
+    const int c2 = twice_coord<COMP, t> (pick<t> (i, j, k));
+    ok = ok && (h ? (c2 >= 2 * a.tl[t] && c2 <= 2 * a.tr[t]) : (c2 >= 2 * a.tl[t] + 1 && c2 <= 2 * a.tr[t] - 1));
   }
   return ok;
 }
 
+// Runtime-direction form (mask export only).
+template <typename T, int COMP, int DIM>
+__device__ uint8_t tfsf_bits (const StepArgs<T> &a, int i, int j, int k)
+{
+  return (uint8_t) ((tfsf_pred<T, COMP, 0, DIM> (a, i, j, k) ? 1 : 0) | (tfsf_pred<T, COMP, 1, DIM> (a, i, j, k) ? 2 : 0)
+                    | (tfsf_pred<T, COMP, 2, DIM> (a, i, j, k) ? 4 : 0) | (tfsf_pred<T, COMP, 3, DIM> (a, i, j, k) ? 8 : 0)
+                    | (tfsf_pred<T, COMP, 4, DIM> (a, i, j, k) ? 16 : 0) | (tfsf_pred<T, COMP, 5, DIM> (a, i, j, k) ? 32 : 0));
+}
+
 // A cell can carry a TF/SF correction only on the planes L-1, R-1, R of some axis while
 // lying inside [L-1, R] on every axis.
-template <typename T>
-__device__ __forceinline__ bool tfsf_near (const StepArgs<T> &a, const int p[3], int dim)
+template <typename T, int DIM>
+__device__ __forceinline__ bool tfsf_near (const StepArgs<T> &a, int i, int j, int k)
 {
+  const int p[3] = { i, j, k };
   bool face = false;
-  for (int ax = 0; ax < dim; ++ax)
+#pragma unroll
+  for (int ax = 0; ax < DIM; ++ax)
   {
     if (p[ax] < a.tl[ax] - 1 || p[ax] > a.tr[ax]) return false;
     face |= (p[ax] == a.tl[ax] - 1) | (p[ax] == a.tr[ax] - 1) | (p[ax] == a.tr[ax]);
@@ -99,22 +135,17 @@ __device__ __forceinline__ bool tfsf_near (const StepArgs<T> &a, const int p[3],
   return face;
 }
 
-// Incident value of component `opp` at integer position pos: approximateIncidentWave +
+// Incident value of component OPP at integer position (i, j, k): approximateIncidentWave +
 // get<opp>FromIncident (InternalScheme.cpp:375-407, InternalScheme.inc.h:500-515).
-template <typename T>
-__device__ double incident_at (const StepArgs<T> &a, int opp, const int pos[3], int dim, const T *inc, double ddiff)
+template <typename T, int OPP, int DIM>
+__device__ double incident_at (const StepArgs<T> &a, int i, int j, int k, const T *inc, double ddiff)
 {
-  double r[3];
-  for (int ax = 0; ax < 3; ++ax)
-  {
-    r[ax] = ax < dim ? 0.5 * (double) twice_coord_d (opp, ax, pos[ax]) : 0.0;
-  }
-  const double x = r[0] - a.zinc[0];
-  const double y = r[1] - a.zinc[1];
+  const double x = 0.5 * (double) twice_coord<OPP, 0> (i) - a.zinc[0];
+  const double y = 0.5 * (double) twice_coord<OPP, 1> (j) - a.zinc[1];
   double d;
-  if (dim == 3)
+  if (DIM == 3)
   {
-    const double z = r[2] - a.zinc[2];
+    const double z = 0.5 * (double) twice_coord<OPP, 2> (k) - a.zinc[2];
     d = x * a.S1 * a.C2 + y * a.S1 * a.S2 + z * a.C1 - ddiff;
   }
   else
@@ -128,35 +159,43 @@ __device__ double incident_at (const StepArgs<T> &a, int opp, const int pos[3], 
   if (i1 > a.inc_len - 2) i1 = a.inc_len - 2;
   const double p2 = d - c1;
   const double p1 = 1 - p2;
-  const double v = (double) inc[i1] * p1 + (double) inc[i1 + 1] * p2;
-  return v * a.proj[opp];
+  const double v = (double) gld (inc + i1) * p1 + (double) gld (inc + i1 + 1) * p2;
+  return v * a.proj[OPP];
 }
 
-// calculateTFSF for one component at cell p: corrects the operands v[0..3] = v11, v12, v21, v22.
-template <typename T>
-__device__ void tfsf_apply (const StepArgs<T> &a, int comp, const int p[3], int dim, T v[4])
+// One operand pair of calculateTFSF.  E components (E_CENTERED) correct the operand opposite
+// to the border side; H components the operand on the border side, with the sign flipped.
+template <typename T, int COMP, int PAIR, int DIM>
+__device__ __forceinline__ void tfsf_pair (const StepArgs<T> &a, int i, int j, int k, T &v_lo, T &v_hi)
 {
-  const bool h = comp >= 3;
-  const bool u[4] = { tfsf_pred (a, comp, kDirD[comp][0], p, dim), tfsf_pred (a, comp, kDirD[comp][1], p, dim),
-                      tfsf_pred (a, comp, kDirD[comp][2], p, dim), tfsf_pred (a, comp, kDirD[comp][3], p, dim) };
+  constexpr bool h = COMP >= 3;
+  constexpr int OPP = Tab::opp[COMP][PAIR];
+  const bool lo = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR], DIM> (a, i, j, k);
+  const bool hi = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR + 1], DIM> (a, i, j, k);
+  if (!lo && !hi) return;
   const T *inc = h ? a.einc : a.hinc;
   const double ddiff = h ? 0.0 : 0.5;
-  for (int pair = 0; pair < 2; ++pair)
-  {
-    const bool lo = u[2 * pair], hi = u[2 * pair + 1];
-    if (!lo && !hi) continue;
-    // E components (E_CENTERED) correct the operand opposite to the border side;
-    // H components the operand on the border side, with the sign flipped.
-    int slot;
-    if (!h) slot = lo ? 2 * pair + 1 : 2 * pair;
-    else slot = lo ? 2 * pair : 2 * pair + 1;
-    const int opp = kOppD[comp][pair];
-    int pos[3];
-    for (int ax = 0; ax < 3; ++ax) pos[ax] = p[ax] + (ax < dim ? kOffD[comp][slot][ax] : 0);
-    double diff = incident_at (a, opp, pos, dim, inc, ddiff);
-    if (h) diff *= -1.0;
-    v[slot] = (T) ((double) v[slot] - diff);
-  }
+  // slot 2*PAIR is "lo" (d11 / d21), slot 2*PAIR+1 is "hi" (d12 / d22)
+  const bool use_hi_slot = h ? !lo : lo;
+  constexpr int s_lo = 2 * PAIR, s_hi = 2 * PAIR + 1;
+  double diff;
+  if (use_hi_slot)
+    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_hi][0], j + Tab::off[COMP][s_hi][1],
+                                     k + (DIM == 3 ? Tab::off[COMP][s_hi][2] : 0), inc, ddiff);
+  else
+    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_lo][0], j + Tab::off[COMP][s_lo][1],
+                                     k + (DIM == 3 ? Tab::off[COMP][s_lo][2] : 0), inc, ddiff);
+  if (h) diff *= -1.0;
+  if (use_hi_slot) v_hi = (T) ((double) v_hi - diff);
+  else v_lo = (T) ((double) v_lo - diff);
+}
+
+// calculateTFSF for one component at cell (i, j, k): corrects the operands v11, v12, v21, v22.
+template <typename T, int COMP, int DIM>
+__device__ __forceinline__ void tfsf_apply (const StepArgs<T> &a, int i, int j, int k, T v[4])
+{
+  tfsf_pair<T, COMP, 0, DIM> (a, i, j, k, v[0], v[1]);
+  tfsf_pair<T, COMP, 1, DIM> (a, i, j, k, v[2], v[3]);
 }
 
 template <typename T>
@@ -164,51 +203,54 @@ __device__ __forceinline__ bool is_special (const StepArgs<T> &a, const int p[3]
 {
   if (mode == 0) return false;
   bool s = false;
-  for (int ax = 0; ax < dim; ++ax) s |= (p[ax] < a.slab_lo[ax]) | (p[ax] >= a.slab_hi[ax]);
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    if (ax < dim) s |= (p[ax] < a.slab_lo[ax]) | (p[ax] >= a.slab_hi[ax]);
   if (mode == 2)
   {
     bool in = true;
-    for (int ax = 0; ax < dim; ++ax) in &= (p[ax] >= a.box_lo[ax]) & (p[ax] < a.box_hi[ax]);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      if (ax < dim) in &= (p[ax] >= a.box_lo[ax]) & (p[ax] < a.box_hi[ax]);
     s |= in;
   }
   return s;
 }
 
-// Field update of one component at flat index q.
-//   tmp  : curl operand combination (o12 - o11 - o22 + o21)
-//   mat  : averaged eps (E) or mu (H)
-//   om/ga: averaged Drude omega / gamma (MODE 2 special cells)
+// New value of one field component at flat index q (reference formulas, see file header).
+//   f_old : the component's previous value
+//   tmp   : curl operand combination (o12 - o11 - o22 + o21)
+//   mat   : averaged eps (E) or mu (H)
+//   om/ga : averaged Drude omega / gamma (MODE 2 special cells)
+//   store : write the auxiliary D/B (D1/B1) state of this cell
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, const int p[3], T tmp, T mat,
-                                             bool special, T om, T ga)
+__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, T tmp, T mat,
+                                         bool special, T om, T ga, bool store)
 {
-  T *F = a.f[COMP];
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
-  const T f_old = F[q];
   if (MODE == 0 || !special)
   {
     // computeCaCb, non-PML branch: Ca = 1, Cb = dt / (material * modifier * dx)
     const T cb = a.dt / (mat * modc * a.dx);
-    F[q] = f_old * (T) 1.0 + tmp * cb;
-    return;
+    return f_old * (T) 1.0 + tmp * cb;
   }
   const int L = a.tab_len;
   const T *tab = a.pml + (size_t) COMP * 6 * L;
-  const T cac = tab[0 * L + p[SigAxes<COMP>::c]];
-  const T cbc = tab[1 * L + p[SigAxes<COMP>::c]];
-  const T p4 = tab[2 * L + p[SigAxes<COMP>::s4]];
-  const T m4 = tab[3 * L + p[SigAxes<COMP>::s4]];
-  const T ca5 = tab[4 * L + p[SigAxes<COMP>::s5]];
-  const T dd5 = tab[5 * L + p[SigAxes<COMP>::s5]];
-  T *D = a.d[COMP];
-  const T d_old = D[q];
+  const T cac = gld (tab + 0 * L + p[SigAxes<COMP>::c]);
+  const T cbc = gld (tab + 1 * L + p[SigAxes<COMP>::c]);
+  const T p4 = gld (tab + 2 * L + p[SigAxes<COMP>::s4]);
+  const T m4 = gld (tab + 3 * L + p[SigAxes<COMP>::s4]);
+  const T ca5 = gld (tab + 4 * L + p[SigAxes<COMP>::s5]);
+  const T dd5 = gld (tab + 5 * L + p[SigAxes<COMP>::s5]);
+  const T d_old = gld (a.d[COMP] + q);
   // curl update of D/B with the PML Ca/Cb (computeCaCb, PML branch)
   const T d_new = d_old * cac + tmp * cbc;
+  T f_new;
   if (MODE == 2)
   {
-    const T d_pp = a.dp[COMP][q];
-    const T m_old = a.d1[COMP][q];
-    const T m_pp = a.d1p[COMP][q];
+    const T d_pp = gld (a.dp[COMP] + q);
+    const T m_old = gld (a.d1[COMP] + q);
+    const T m_pp = gld (a.d1p[COMP] + q);
     // Drude ADE coefficients (InternalScheme.inc.h:2513-2518), M = eps0 or mu0
     const T M = modc, dt = a.dt;
     const T wdt2 = (dt * om) * (dt * om);
@@ -221,17 +263,30 @@ __device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, co
     const T m_new = d_new * b0 + d_old * b1 + d_pp * b2 - m_old * a1 - m_pp * a2;
     // D->E from D1 with modifier 1 (calculateFieldStepIterationPML, useMetamaterials)
     const T one = (T) 1;
-    F[q] = f_old * ca5 + m_new * ((p4 / one) / dd5) - m_old * ((m4 / one) / dd5);
-    a.dp[COMP][q] = d_old;
-    a.d1[COMP][q] = m_new;
-    a.d1p[COMP][q] = m_old;
+    f_new = f_old * ca5 + m_new * ((p4 / one) / dd5) - m_old * ((m4 / one) / dd5);
+    if (store)
+    {
+      gst (a.dp_n[COMP] + q, d_old);
+      gst (a.d1_n[COMP] + q, m_new);
+      gst (a.d1p_n[COMP] + q, m_old);
+    }
   }
   else
   {
     const T mod = mat * modc;
-    F[q] = f_old * ca5 + d_new * ((p4 / mod) / dd5) - d_old * ((m4 / mod) / dd5);
+    f_new = f_old * ca5 + d_new * ((p4 / mod) / dd5) - d_old * ((m4 / mod) / dd5);
   }
-  D[q] = d_new;
+  if (store) gst (a.d_n[COMP] + q, d_new);
+  return f_new;
+}
+
+// In-place update of one component (half-step kernels).
+template <typename T, int MODE, int COMP>
+__device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, const int p[3], T tmp, T mat,
+                                             bool special, T om, T ga)
+{
+  const T f_old = gld (a.f[COMP] + q);
+  gst (a.f_n[COMP] + q, cell_value<T, MODE, COMP> (a, q, p, f_old, tmp, mat, special, om, ga, true));
 }
 
 template <typename T>
@@ -259,150 +314,119 @@ __device__ __forceinline__ T sum4 (T m11, T m12, T m21, T m22)
 }
 
 // ---------------------------------------------------------------------------------------
-// 3D E half-step: Ex, Ey, Ez on [1, N-1)^3
+// 3D E half-step: Ex, Ey, Ez on [1, N-1)^3.  One thread per cell, lane = z, so each
+// wavefront access is a coalesced row; the y/z/x neighbours of the stencil are served by
+// L1/L2 (the neighbouring rows and planes are read by the neighbouring waves).
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = (int) blockIdx.z + 1;
   if (j < 1 || j >= a.ny - 1 || k < 1 || k >= a.nz - 1) return;
-  const int i0 = 1 + (int) blockIdx.z * a.chunk;
-  const int i1 = min (a.nx - 1, i0 + a.chunk);
-  if (i0 >= i1) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
-  const T *__restrict__ Hx = a.f[3];
-  const T *__restrict__ Hy = a.f[4];
-  const T *__restrict__ Hz = a.f[5];
-  int64_t q = (int64_t) i0 * sx + (int64_t) j * sy + k;
-  T hy_m = Hy[q - sx];
-  T hz_m = Hz[q - sx];
-  T e_here = a.eps ? a.eps[q] : a.eps_u;
-  for (int i = i0; i < i1; ++i, q += sx)
+  const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
+  const T *Hx = a.f[3], *Hy = a.f[4], *Hz = a.f[5];
+  const T hx = gld (Hx + q), hy = gld (Hy + q), hz = gld (Hz + q);
+  T vx[4] = { gld (Hz + q - sy), hz, gld (Hy + q - 1), hy };
+  T vy[4] = { gld (Hx + q - 1), hx, gld (Hz + q - sx), hz };
+  T vz[4] = { gld (Hy + q - sx), hy, gld (Hx + q - sy), hx };
+  T ex_m = a.eps_u, ey_m = a.eps_u, ez_m = a.eps_u;
+  if (a.eps)
   {
-    const T hx = Hx[q], hy = Hy[q], hz = Hz[q];
-    const int p[3] = { i, j, k };
-    T vx[4] = { Hz[q - sy], hz, Hy[q - 1], hy };
-    T vy[4] = { Hx[q - 1], hx, hz_m, hz };
-    T vz[4] = { hy_m, hy, Hx[q - sy], hx };
-    hy_m = hy;
-    hz_m = hz;
-    if (a.tfsf && tfsf_near (a, p, 3))
-    {
-      tfsf_apply (a, 0, p, 3, vx);
-      tfsf_apply (a, 1, p, 3, vy);
-      tfsf_apply (a, 2, p, 3, vz);
-    }
-    // materials: 2-cell averages along each component's axis (getMaterial, E_CENTERED)
-    T ex_m, ey_m, ez_m, e_next;
-    if (a.eps)
-    {
-      e_next = a.eps[q + sx];
-      ex_m = avg2 (e_here, e_next);
-      ey_m = avg2 (e_here, a.eps[q + sy]);
-      ez_m = avg2 (e_here, a.eps[q + 1]);
-    }
-    else
-    {
-      e_next = a.eps_u;
-      ex_m = ey_m = ez_m = a.eps_u;
-    }
-    const bool special = is_special (a, p, 3, MODE);
-    T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
-    if (MODE == 2 && special)
-    {
-      // Drude 2-cell averages (getMetaMaterial / approximateDrudeModel); read only in
-      // PML and Drude cells
-      if (a.ope)
-      {
-        const T *o = a.ope;
-        oxm = avg2 (o[q], o[q + sx]); oym = avg2 (o[q], o[q + sy]); ozm = avg2 (o[q], o[q + 1]);
-      }
-      if (a.ge)
-      {
-        const T *g = a.ge;
-        gxm = avg2 (g[q], g[q + sx]); gym = avg2 (g[q], g[q + sy]); gzm = avg2 (g[q], g[q + 1]);
-      }
-    }
-    update_cell<T, MODE, 0> (a, q, p, curl (vx), ex_m, special, oxm, gxm);
-    update_cell<T, MODE, 1> (a, q, p, curl (vy), ey_m, special, oym, gym);
-    update_cell<T, MODE, 2> (a, q, p, curl (vz), ez_m, special, ozm, gzm);
-    e_here = e_next;
+    // 2-cell averages along each component's axis (getMaterial, E_CENTERED)
+    const T e = gld (a.eps + q);
+    ex_m = avg2 (e, gld (a.eps + q + sx));
+    ey_m = avg2 (e, gld (a.eps + q + sy));
+    ez_m = avg2 (e, gld (a.eps + q + 1));
   }
+  if (a.tfsf && tfsf_near<T, 3> (a, i, j, k))
+  {
+    tfsf_apply<T, 0, 3> (a, i, j, k, vx);
+    tfsf_apply<T, 1, 3> (a, i, j, k, vy);
+    tfsf_apply<T, 2, 3> (a, i, j, k, vz);
+  }
+  const int p[3] = { i, j, k };
+  const bool special = is_special (a, p, 3, MODE);
+  T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
+  if (MODE == 2 && special)
+  {
+    // Drude 2-cell averages (getMetaMaterial / approximateDrudeModel)
+    if (a.ope)
+    {
+      const T *o = a.ope;
+      oxm = avg2 (gld (o + q), gld (o + q + sx)); oym = avg2 (gld (o + q), gld (o + q + sy));
+      ozm = avg2 (gld (o + q), gld (o + q + 1));
+    }
+    if (a.ge)
+    {
+      const T *g = a.ge;
+      gxm = avg2 (gld (g + q), gld (g + q + sx)); gym = avg2 (gld (g + q), gld (g + q + sy));
+      gzm = avg2 (gld (g + q), gld (g + q + 1));
+    }
+  }
+  update_cell<T, MODE, 0> (a, q, p, curl (vx), ex_m, special, oxm, gxm);
+  update_cell<T, MODE, 1> (a, q, p, curl (vy), ey_m, special, oym, gym);
+  update_cell<T, MODE, 2> (a, q, p, curl (vz), ez_m, special, ozm, gzm);
 }
 
 // ---------------------------------------------------------------------------------------
-// 3D H half-step: Hx, Hy, Hz on [1, N-1)^3
+// 3D H half-step: Hx, Hy, Hz on [1, N-1)^3 (one thread per cell)
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = (int) blockIdx.z + 1;
   if (j < 1 || j >= a.ny - 1 || k < 1 || k >= a.nz - 1) return;
-  const int i0 = 1 + (int) blockIdx.z * a.chunk;
-  const int i1 = min (a.nx - 1, i0 + a.chunk);
-  if (i0 >= i1) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
-  const T *__restrict__ Ex = a.f[0];
-  const T *__restrict__ Ey = a.f[1];
-  const T *__restrict__ Ez = a.f[2];
-  int64_t q = (int64_t) i0 * sx + (int64_t) j * sy + k;
-  T ey_c = Ey[q];
-  T ez_c = Ez[q];
-  for (int i = i0; i < i1; ++i, q += sx)
+  const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
+  const T *Ex = a.f[0], *Ey = a.f[1], *Ez = a.f[2];
+  const T ex = gld (Ex + q), ey = gld (Ey + q), ez = gld (Ez + q);
+  T vx[4] = { ey, gld (Ey + q + 1), ez, gld (Ez + q + sy) };
+  T vy[4] = { ez, gld (Ez + q + sx), ex, gld (Ex + q + 1) };
+  T vz[4] = { ex, gld (Ex + q + sy), ey, gld (Ey + q + sx) };
+  if (a.tfsf && tfsf_near<T, 3> (a, i, j, k))
   {
-    const T ex = Ex[q];
-    const T ey_n = Ey[q + sx], ez_n = Ez[q + sx];
-    const int p[3] = { i, j, k };
-    T vx[4] = { ey_c, Ey[q + 1], ez_c, Ez[q + sy] };
-    T vy[4] = { ez_c, ez_n, ex, Ex[q + 1] };
-    T vz[4] = { ex, Ex[q + sy], ey_c, ey_n };
-    if (a.tfsf && tfsf_near (a, p, 3))
-    {
-      tfsf_apply (a, 3, p, 3, vx);
-      tfsf_apply (a, 4, p, 3, vy);
-      tfsf_apply (a, 5, p, 3, vz);
-    }
-    ey_c = ey_n;
-    ez_c = ez_n;
-    // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
-    T hx_m, hy_m, hz_m;
-    if (a.mu)
-    {
-      const T *m = a.mu;
-      hx_m = avg4 (m[q], m[q + 1], m[q + sy], m[q + sy + 1]);
-      hy_m = avg4 (m[q], m[q + 1], m[q + sx], m[q + sx + 1]);
-      hz_m = avg4 (m[q], m[q + sy], m[q + sx], m[q + sx + sy]);
-    }
-    else
-    {
-      hx_m = hy_m = hz_m = a.mu_u;
-    }
-    const bool special = is_special (a, p, 3, MODE);
-    T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
-    if (MODE == 2 && special)
-    {
-      const T *o = a.opm, *g = a.gm;
-      if (o)
-      {
-        oxm = sum4 (o[q], o[q + 1], o[q + sy], o[q + sy + 1]);
-        oym = sum4 (o[q], o[q + 1], o[q + sx], o[q + sx + 1]);
-        ozm = sum4 (o[q], o[q + sy], o[q + sx], o[q + sx + sy]);
-      }
-      if (g)
-      {
-        gxm = sum4 (g[q], g[q + 1], g[q + sy], g[q + sy + 1]);
-        gym = sum4 (g[q], g[q + 1], g[q + sx], g[q + sx + 1]);
-        gzm = sum4 (g[q], g[q + sy], g[q + sx], g[q + sx + sy]);
-      }
-    }
-    update_cell<T, MODE, 3> (a, q, p, curl (vx), hx_m, special, oxm, gxm);
-    update_cell<T, MODE, 4> (a, q, p, curl (vy), hy_m, special, oym, gym);
-    update_cell<T, MODE, 5> (a, q, p, curl (vz), hz_m, special, ozm, gzm);
+    tfsf_apply<T, 3, 3> (a, i, j, k, vx);
+    tfsf_apply<T, 4, 3> (a, i, j, k, vy);
+    tfsf_apply<T, 5, 3> (a, i, j, k, vz);
   }
+  // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
+  T hx_m = a.mu_u, hy_m = a.mu_u, hz_m = a.mu_u;
+  if (a.mu)
+  {
+    const T *m = a.mu;
+    hx_m = avg4 (gld (m + q), gld (m + q + 1), gld (m + q + sy), gld (m + q + sy + 1));
+    hy_m = avg4 (gld (m + q), gld (m + q + 1), gld (m + q + sx), gld (m + q + sx + 1));
+    hz_m = avg4 (gld (m + q), gld (m + q + sy), gld (m + q + sx), gld (m + q + sx + sy));
+  }
+  const int p[3] = { i, j, k };
+  const bool special = is_special (a, p, 3, MODE);
+  T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
+  if (MODE == 2 && special)
+  {
+    const T *o = a.opm, *g = a.gm;
+    if (o)
+    {
+      oxm = sum4 (gld (o + q), gld (o + q + 1), gld (o + q + sy), gld (o + q + sy + 1));
+      oym = sum4 (gld (o + q), gld (o + q + 1), gld (o + q + sx), gld (o + q + sx + 1));
+      ozm = sum4 (gld (o + q), gld (o + q + sy), gld (o + q + sx), gld (o + q + sx + sy));
+    }
+    if (g)
+    {
+      gxm = sum4 (gld (g + q), gld (g + q + 1), gld (g + q + sy), gld (g + q + sy + 1));
+      gym = sum4 (gld (g + q), gld (g + q + 1), gld (g + q + sx), gld (g + q + sx + 1));
+      gzm = sum4 (gld (g + q), gld (g + q + sy), gld (g + q + sx), gld (g + q + sx + sy));
+    }
+  }
+  update_cell<T, MODE, 3> (a, q, p, curl (vx), hx_m, special, oxm, gxm);
+  update_cell<T, MODE, 4> (a, q, p, curl (vy), hy_m, special, oym, gym);
+  update_cell<T, MODE, 5> (a, q, p, curl (vz), hz_m, special, ozm, gzm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -420,7 +444,7 @@ __global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
   const int p[3] = { i, j, 0 };
   const T *Hx = a.f[3], *Hy = a.f[4];
   T v[4] = { Hy[q - sx], Hy[q], Hx[q - sy], Hx[q] };
-  if (a.tfsf && tfsf_near (a, p, 2)) tfsf_apply (a, 2, p, 2, v);
+  if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0)) tfsf_apply<T, 2, 2> (a, i, j, 0, v);
   // Ez sits on the eps point: the 2-cell average reads the same cell twice
   const T e = a.eps ? a.eps[q] : a.eps_u;
   const T mat = avg2 (e, e);
@@ -450,11 +474,10 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
   // Hx: Ey operands absent (0), Ez at (i, j) and (i, j+1); Hy: Ez at (i, j) and (i+1, j), Ex absent
   T vx[4] = { (T) 0, (T) 0, ez, Ez[q + sy] };
   T vy[4] = { ez, Ez[q + sx], (T) 0, (T) 0 };
-  const bool near = a.tfsf && tfsf_near (a, p, 2);
-  if (near)
+  if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0))
   {
-    tfsf_apply (a, 3, p, 2, vx);
-    tfsf_apply (a, 4, p, 2, vy);
+    tfsf_apply<T, 3, 2> (a, i, j, 0, vx);
+    tfsf_apply<T, 4, 2> (a, i, j, 0, vy);
   }
   T mx, my;
   if (a.mu)
@@ -534,11 +557,29 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
     const int k = (int) (q % a.nz);
     const int j = (int) ((q / a.nz) % a.ny);
     const int i = (int) (q / ((int64_t) a.nz * a.ny));
-    const int p[3] = { i, j, k };
     uint8_t m = 0;
-    for (int d = 0; d < 6; ++d)
-      if (tfsf_pred (a, comp, d, p, dim)) m |= (uint8_t) (1u << d);
-    out[q] = m;
+    if (dim == 3)
+    {
+      switch (comp)
+      {
+        case 0: m = tfsf_bits<T, 0, 3> (a, i, j, k); break;
+        case 1: m = tfsf_bits<T, 1, 3> (a, i, j, k); break;
+        case 2: m = tfsf_bits<T, 2, 3> (a, i, j, k); break;
+        case 3: m = tfsf_bits<T, 3, 3> (a, i, j, k); break;
+        case 4: m = tfsf_bits<T, 4, 3> (a, i, j, k); break;
+        default: m = tfsf_bits<T, 5, 3> (a, i, j, k); break;
+      }
+    }
+    else
+    {
+      switch (comp)
+      {
+        case 2: m = tfsf_bits<T, 2, 2> (a, i, j, k); break;
+        case 3: m = tfsf_bits<T, 3, 2> (a, i, j, k); break;
+        default: m = tfsf_bits<T, 4, 2> (a, i, j, k); break;
+      }
+    }
+    gst (out + q, m);
   }
 }
 
