@@ -118,6 +118,20 @@ def cpu_baseline(sample_n: int = 96, sample_steps: int = 31) -> dict:
             "sample": sample + "; C oracle FP64 (reference driver not built)"}
 
 
+def copy_probe(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB (torch is only the allocator here)."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.neg(src, out=dst)   # one elementwise kernel: reads mib MiB, writes mib MiB
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +139,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the run, copy a buffer of this many MiB device-to-device (a plain copy "
+                         "of known size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -132,6 +149,9 @@ def main():
         sys.exit(2)
 
     from fdtd3d_amd.engine import Engine, device_info
+
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
 
     wl = WORKLOADS[args.workload]
     cfg = wl["make"]()

@@ -1,0 +1,293 @@
+// fdtd3d-compatible command line front end of the Yee engine (libyee.so).
+//
+// Accepts the reference's flags for the Yee time-stepping path (Source/Settings/Settings.inc.h,
+// parsed as Source/Settings/Settings.cpp:133-369 does: "x:<int>,y:<int>,z:<int>" coordinates,
+// --same-size* copy rules, std::stof for floating-point values, --cmd-from-file), sets the run
+// up as Scheme::initScheme / initGrids do (materials from DAT files and spheres), performs
+// --time-steps steps and prints the reference's "Total time = ... seconds" line
+// (Source/main.cpp:1175-1211).  Options of the reference outside this path are either
+// accepted without effect (output and logging options) or rejected with an error (other
+// schemes and physics).
+//
+// Built twice: fdtd3d (FP64 fields, the reference's default value type) and fdtd3d-f32.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/yee.h"
+
+#ifndef YEE_CLI_PRECISION
+#define YEE_CLI_PRECISION 8
+#endif
+
+namespace {
+
+struct Sphere
+{
+  double value;
+  int64_t center[3] = { 0, 0, 0 };
+  int64_t radius = 0;
+};
+
+struct Options
+{
+  yee_config cfg {};
+  std::string file[6];            // --load-<material>-from-file
+  Sphere sphere[4] = { { 1.0 }, { 1.0 }, { 0.0 }, { 0.0 } };   // eps, mu, omegape, omegapm
+  int accuracy = 100;
+  bool stair = false;
+};
+
+[[noreturn]] void die (const std::string &msg)
+{
+  std::fprintf (stderr, "fdtd3d: %s\n", msg.c_str ());
+  std::exit (1);
+}
+
+void parse_coord (const std::string &text, int64_t def, int64_t out[3])
+{
+  out[0] = out[1] = out[2] = def;
+  size_t start = 0;
+  while (start <= text.size ())
+  {
+    size_t end = text.find (',', start);
+    if (end == std::string::npos) end = text.size ();
+    const std::string part = text.substr (start, end - start);
+    const size_t colon = part.find (':');
+    if (colon == std::string::npos || colon == 0) die ("bad coordinate '" + text + "'");
+    const char axis = part[0];
+    const int idx = axis == 'x' ? 0 : axis == 'y' ? 1 : axis == 'z' ? 2 : -1;
+    if (idx < 0) die ("bad coordinate axis in '" + text + "'");
+    out[idx] = std::stoll (part.substr (colon + 1));
+    start = end + 1;
+  }
+}
+
+// std::stof, then widened: the reference stores every FP option through stof.
+double fp_option (const std::string &text)
+{
+  return (double) std::stof (text);
+}
+
+std::vector<std::string> read_cmd_file (const std::string &path)
+{
+  std::ifstream in (path);
+  if (!in) die ("cannot open " + path);
+  std::vector<std::string> out;
+  std::string line;
+  while (std::getline (in, line))
+  {
+    const size_t b = line.find_first_not_of (" \t\r");
+    if (b == std::string::npos) continue;
+    const size_t e = line.find_last_not_of (" \t\r");
+    line = line.substr (b, e - b + 1);
+    if (line[0] == '#' || line.rfind ("//", 0) == 0) continue;
+    out.push_back (line);
+  }
+  return out;
+}
+
+const std::set<std::string> kIgnoredWithArg = {
+  "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size", "--cuda-buffer-size",
+  "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end", "--ntff-filename", "--ntff-step-angle",
+  "--interm-ntff-step", "--interm-save-step", "--save-start-coord", "--save-end-coord", "--rebalance-step",
+  "--stored-steps", "--current-source", "--save-cmd-to-file",
+};
+const std::set<std::string> kIgnoredFlags = {
+  "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff", "--save-res",
+  "--save-res-per-process", "--save-materials", "--save-interm-res", "--save-scattered-field-res",
+  "--save-scattered--field-interm", "--save-as-bmp", "--save-as-dat", "--save-as-txt", "--save-tfsf-e-incident",
+  "--save-tfsf-h-incident", "--manual-save-coords", "--save-no-pml", "--palette-gray", "--palette-rgb",
+  "--orth-axis-x", "--orth-axis-y", "--orth-axis-z", "--dynamic-grid", "--check-disabling-cond",
+  "--check-enabling-cond", "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
+  "--ntff-to-stdout",
+};
+const std::set<std::string> kUnsupported = {
+  "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy", "--2d-tex", "--2d-tey",
+  "--2d-tez", "--2d-tmx", "--2d-tmy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
+  "--current-source-jx", "--current-source-jy", "--current-source-jz", "--current-source-mx",
+  "--current-source-my", "--current-source-mz",
+};
+
+Options parse (int argc, char **argv)
+{
+  Options o;
+  yee_config &c = o.cfg;
+  // reference defaults (Settings.inc.h)
+  c.scheme = YEE_SCHEME_DIM3;
+  c.layout = 0;
+  c.precision = YEE_CLI_PRECISION;
+  c.time_steps = 100;
+  for (int a = 0; a < 3; ++a)
+  {
+    c.size[a] = 100;
+    c.pml[a] = 2;
+    c.tfsf_left[a] = 4;
+    c.tfsf_right[a] = 4;
+  }
+  c.dx = 0.0005;
+  c.wavelength = 0.02;
+  c.courant = 0.5;
+  c.angle_deg[0] = 90;
+  c.angle_deg[1] = 0;
+  c.angle_deg[2] = 90;
+
+  std::vector<std::string> args (argv + 1, argv + argc);
+  for (size_t i = 0; i < args.size (); ++i)
+  {
+    const std::string a = args[i];
+    auto val = [&] () -> std::string {
+      if (i + 1 >= args.size ()) die ("option " + a + " needs an argument");
+      return args[++i];
+    };
+    auto copy3 = [] (int64_t v[3]) { v[1] = v[2] = v[0]; };
+    static const char *comps[6] = { "ex", "ey", "ez", "hx", "hy", "hz" };
+    static const char *mats[6] = { "eps", "mu", "omegape", "omegapm", "gammae", "gammam" };
+    bool done = true;
+    if (a == "--cmd-from-file")
+    {
+      std::vector<std::string> rest = read_cmd_file (val ());
+      args.insert (args.begin () + (long) i + 1, rest.begin (), rest.end ());
+    }
+    else if (a == "--3d") c.scheme = YEE_SCHEME_DIM3;
+    else if (a == "--2d-tmz") c.scheme = YEE_SCHEME_TMZ;
+    else if (a == "--size") parse_coord (val (), 100, c.size);
+    else if (a == "--same-size") copy3 (c.size);
+    else if (a == "--pml-size") parse_coord (val (), 2, c.pml);
+    else if (a == "--same-size-pml") copy3 (c.pml);
+    else if (a == "--tfsf-size-left") parse_coord (val (), 4, c.tfsf_left);
+    else if (a == "--tfsf-size-right") parse_coord (val (), 4, c.tfsf_right);
+    else if (a == "--same-size-tfsf")
+    {
+      copy3 (c.tfsf_left);
+      copy3 (c.tfsf_right);
+    }
+    else if (a == "--time-steps") c.time_steps = std::stoi (val ());
+    else if (a == "--angle-teta") c.angle_deg[0] = fp_option (val ());
+    else if (a == "--angle-phi") c.angle_deg[1] = fp_option (val ());
+    else if (a == "--angle-psi") c.angle_deg[2] = fp_option (val ());
+    else if (a == "--dx") c.dx = fp_option (val ());
+    else if (a == "--wavelength") c.wavelength = fp_option (val ());
+    else if (a == "--courant-factor") c.courant = fp_option (val ());
+    else if (a == "--use-tfsf") c.flags |= YEE_USE_TFSF;
+    else if (a == "--use-pml") c.flags |= YEE_USE_PML;
+    else if (a == "--use-metamaterials") c.flags |= YEE_USE_METAMATERIALS;
+    else if (a == "--layout-type") c.layout = std::stoi (val ());
+    else if (a == "--point-source") parse_coord (val (), 0, c.point_source_pos);
+    else if (a == "--sphere-accuracy") o.accuracy = std::stoi (val ());
+    else if (a == "--stair-sphere-approx") o.stair = true;
+    else done = false;
+    if (done) continue;
+    for (int k = 0; k < 6 && !done; ++k)
+      if (a == std::string ("--point-source-") + comps[k])
+      {
+        c.point_source |= 1u << k;
+        done = true;
+      }
+    for (int m = 0; m < 6 && !done; ++m)
+    {
+      const std::string base = std::string ("--") + mats[m];
+      if (a == "--load-" + std::string (mats[m]) + "-from-file")
+      {
+        o.file[m] = val ();
+        done = true;
+      }
+      else if (m < 4 && a == base + "-sphere")
+      {
+        o.sphere[m].value = fp_option (val ());
+        done = true;
+      }
+      else if (m < 4 && a == base + "-sphere-center")
+      {
+        parse_coord (val (), 0, o.sphere[m].center);
+        done = true;
+      }
+      else if (m < 4 && a == base + "-sphere-radius")
+      {
+        o.sphere[m].radius = std::stoll (val ());
+        done = true;
+      }
+    }
+    if (done) continue;
+    if (kIgnoredWithArg.count (a)) (void) val ();
+    else if (kIgnoredFlags.count (a)) continue;
+    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (3D and TMz, E_CENTERED, real fields)");
+    else die ("unknown option " + a);
+  }
+  return o;
+}
+
+void check (int rc)
+{
+  if (rc != YEE_OK) die (yee_last_error ());
+}
+
+// Raw double array [x][y][z] of the grid's size (DATLoader, Source/Loader/DATLoader.h:62-86).
+std::vector<double> load_dat (const std::string &path, int64_t n)
+{
+  std::ifstream in (path, std::ios::binary);
+  if (!in) die ("cannot open " + path);
+  std::vector<double> v ((size_t) n);
+  in.read (reinterpret_cast<char *> (v.data ()), (std::streamsize) (n * sizeof (double)));
+  if (in.gcount () != (std::streamsize) (n * sizeof (double))) die (path + ": expected " + std::to_string (n) + " doubles");
+  return v;
+}
+
+}  // namespace
+
+int main (int argc, char **argv)
+{
+  Options o = parse (argc, argv);
+  yee_config &c = o.cfg;
+  if (c.scheme == YEE_SCHEME_TMZ) c.size[2] = 1;
+  const int64_t n = c.size[0] * c.size[1] * (c.scheme == YEE_SCHEME_DIM3 ? c.size[2] : 1);
+
+  char info[256];
+  check (yee_device_info (info, sizeof info));
+  std::printf ("%s\n", info);
+
+  yee_handle *h = nullptr;
+  check (yee_create (&c, &h));
+  double sc[3];
+  check (yee_get_scalars (h, sc));
+  const double two_pi_f = 2 * 3.14159265358979323846 * sc[1];
+
+  // Scheme::initGrids order: file, then sphere (which replaces the grid); omega/gamma only
+  // with metamaterials (Scheme.inc.h:2962-3165)
+  const bool meta = (c.flags & YEE_USE_METAMATERIALS) != 0;
+  for (int m = 0; m < 6; ++m)
+  {
+    if (m >= 2 && !meta) continue;
+    if (!o.file[m].empty ())
+    {
+      std::vector<double> v = load_dat (o.file[m], n);
+      check (yee_set_material (h, m, v.data ()));
+    }
+    if (m < 4)
+    {
+      const Sphere &s = o.sphere[m];
+      const double neutral = m < 2 ? 1.0 : 0.0;
+      if (s.value != neutral)
+      {
+        const double value = m < 2 ? s.value : s.value * two_pi_f;
+        check (yee_set_material_sphere (h, m, value, neutral, s.center, (double) s.radius, o.accuracy, o.stair ? 1 : 0));
+      }
+    }
+  }
+  check (yee_prepare (h));
+
+  const auto t0 = std::chrono::steady_clock::now ();
+  check (yee_step (h, c.time_steps));
+  check (yee_sync (h));
+  const double secs = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
+  std::printf ("Total time = %f seconds\n", secs);
+  const double upd = (double) n * (double) (c.time_steps > 0 ? c.time_steps - 1 : 0);
+  std::printf ("Cell updates = %.0f (%.2f Mcell-updates/s)\n", upd, secs > 0 ? upd / secs / 1e6 : 0.0);
+  yee_destroy (h);
+  return 0;
+}

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/yee.h"
@@ -519,6 +520,40 @@ void yee_destroy (yee_handle *h)
 int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values)
 {
   return set_material_impl (h, which, values);
+}
+
+int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double outside, const int64_t center[3],
+                             double radius, int32_t accuracy, int32_t stair)
+{
+  if (!h || !center) return fail (YEE_EINVAL, "null argument");
+  if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
+  if (!stair && accuracy <= 0) return fail (YEE_EINVAL, "sphere accuracy must be positive");
+  std::vector<double> v ((size_t) h->ncell);
+  const double cx = (double) center[0] + 0.5, cy = (double) center[1] + 0.5, cz = (double) center[2] + 0.5;
+  const int64_t nx = h->n[0], ny = h->n[1], nz = h->n[2];
+  // the accurate approximation integrates accuracy^2 columns per cell: split x planes over threads
+  const unsigned nthr = std::max (1u, std::min (16u, std::thread::hardware_concurrency ()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nthr; ++t)
+    pool.emplace_back ([&, t] () {
+      for (int64_t i = t; i < nx; i += nthr)
+        for (int64_t j = 0; j < ny; ++j)
+          for (int64_t k = 0; k < nz; ++k)
+          {
+            // eps-point coordinates (i + 0.5, j + 0.5, k + 0.5)
+            const double mx = i + 0.5, my = j + 0.5, mz = k + 0.5;
+            double val;
+            if (h->dim == 3)
+              val = stair ? sphere_stair_3d (mx, my, mz, cx, cy, cz, radius, value, outside)
+                          : sphere_accurate_3d (mx, my, mz, cx, cy, cz, radius, value, outside, accuracy);
+            else
+              val = stair ? sphere_stair_2d (mx, my, cx, cy, radius, value, outside)
+                          : sphere_accurate_2d (mx, my, cx, cy, radius, value, outside, accuracy);
+            v[(size_t) ((i * ny + j) * nz + k)] = val;
+          }
+    });
+  for (std::thread &th : pool) th.join ();
+  return set_material_impl (h, which, v.data ());
 }
 
 int yee_set_material (yee_handle *h, int32_t which, const double *values)

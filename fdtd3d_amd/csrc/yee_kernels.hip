@@ -219,13 +219,14 @@ __device__ __forceinline__ bool is_special (const StepArgs<T> &a, const int p[3]
 
 // New value of one field component at flat index q (reference formulas, see file header).
 //   f_old : the component's previous value
+//   d_old : its D/B value (special cells only; loaded by the caller)
 //   tmp   : curl operand combination (o12 - o11 - o22 + o21)
 //   mat   : averaged eps (E) or mu (H)
 //   om/ga : averaged Drude omega / gamma (MODE 2 special cells)
 //   store : write the auxiliary D/B (D1/B1) state of this cell
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, T tmp, T mat,
-                                         bool special, T om, T ga, bool store)
+__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, T d_old, T tmp,
+                                         T mat, bool special, T om, T ga, bool store)
 {
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
   if (MODE == 0 || !special)
@@ -242,7 +243,6 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
   const T m4 = gld (tab + 3 * L + p[SigAxes<COMP>::s4]);
   const T ca5 = gld (tab + 4 * L + p[SigAxes<COMP>::s5]);
   const T dd5 = gld (tab + 5 * L + p[SigAxes<COMP>::s5]);
-  const T d_old = gld (a.d[COMP] + q);
   // curl update of D/B with the PML Ca/Cb (computeCaCb, PML branch)
   const T d_new = d_old * cac + tmp * cbc;
   T f_new;
@@ -280,13 +280,22 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
   return f_new;
 }
 
-// In-place update of one component (half-step kernels).
+// In-place update of one component (half-step kernels).  The old values are loaded by the
+// caller before any store of the cell, so every load of a thread is in flight at once (the
+// state pointers may alias as far as the compiler knows).
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, const int p[3], T tmp, T mat,
-                                             bool special, T om, T ga)
+__device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, T d_old,
+                                             T tmp, T mat, bool special, T om, T ga)
 {
-  const T f_old = gld (a.f[COMP] + q);
-  gst (a.f_n[COMP] + q, cell_value<T, MODE, COMP> (a, q, p, f_old, tmp, mat, special, om, ga, true));
+  gst (a.f_n[COMP] + q, cell_value<T, MODE, COMP> (a, q, p, f_old, d_old, tmp, mat, special, om, ga, true));
+}
+
+// Previous value and (special cells) D/B value of component COMP at q.
+template <typename T, int MODE, int COMP>
+__device__ __forceinline__ void load_old (const StepArgs<T> &a, int64_t q, bool special, T &f_old, T &d_old)
+{
+  f_old = gld (a.f[COMP] + q);
+  d_old = (MODE != 0 && special) ? gld (a.d[COMP] + q) : (T) 0;
 }
 
 template <typename T>
@@ -328,6 +337,12 @@ __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
+  const int p[3] = { i, j, k };
+  const bool special = is_special (a, p, 3, MODE);
+  T fo0, fo1, fo2, do0, do1, do2;
+  load_old<T, MODE, 0> (a, q, special, fo0, do0);
+  load_old<T, MODE, 1> (a, q, special, fo1, do1);
+  load_old<T, MODE, 2> (a, q, special, fo2, do2);
   const T *Hx = a.f[3], *Hy = a.f[4], *Hz = a.f[5];
   const T hx = gld (Hx + q), hy = gld (Hy + q), hz = gld (Hz + q);
   T vx[4] = { gld (Hz + q - sy), hz, gld (Hy + q - 1), hy };
@@ -348,8 +363,6 @@ __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
     tfsf_apply<T, 1, 3> (a, i, j, k, vy);
     tfsf_apply<T, 2, 3> (a, i, j, k, vz);
   }
-  const int p[3] = { i, j, k };
-  const bool special = is_special (a, p, 3, MODE);
   T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
   if (MODE == 2 && special)
   {
@@ -367,9 +380,9 @@ __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
       gzm = avg2 (gld (g + q), gld (g + q + 1));
     }
   }
-  update_cell<T, MODE, 0> (a, q, p, curl (vx), ex_m, special, oxm, gxm);
-  update_cell<T, MODE, 1> (a, q, p, curl (vy), ey_m, special, oym, gym);
-  update_cell<T, MODE, 2> (a, q, p, curl (vz), ez_m, special, ozm, gzm);
+  update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl (vx), ex_m, special, oxm, gxm);
+  update_cell<T, MODE, 1> (a, q, p, fo1, do1, curl (vy), ey_m, special, oym, gym);
+  update_cell<T, MODE, 2> (a, q, p, fo2, do2, curl (vz), ez_m, special, ozm, gzm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -385,6 +398,12 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
+  const int p[3] = { i, j, k };
+  const bool special = is_special (a, p, 3, MODE);
+  T fo3, fo4, fo5, do3, do4, do5;
+  load_old<T, MODE, 3> (a, q, special, fo3, do3);
+  load_old<T, MODE, 4> (a, q, special, fo4, do4);
+  load_old<T, MODE, 5> (a, q, special, fo5, do5);
   const T *Ex = a.f[0], *Ey = a.f[1], *Ez = a.f[2];
   const T ex = gld (Ex + q), ey = gld (Ey + q), ez = gld (Ez + q);
   T vx[4] = { ey, gld (Ey + q + 1), ez, gld (Ez + q + sy) };
@@ -405,8 +424,6 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
     hy_m = avg4 (gld (m + q), gld (m + q + 1), gld (m + q + sx), gld (m + q + sx + 1));
     hz_m = avg4 (gld (m + q), gld (m + q + sy), gld (m + q + sx), gld (m + q + sx + sy));
   }
-  const int p[3] = { i, j, k };
-  const bool special = is_special (a, p, 3, MODE);
   T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
   if (MODE == 2 && special)
   {
@@ -424,9 +441,9 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
       gzm = sum4 (gld (g + q), gld (g + q + sy), gld (g + q + sx), gld (g + q + sx + sy));
     }
   }
-  update_cell<T, MODE, 3> (a, q, p, curl (vx), hx_m, special, oxm, gxm);
-  update_cell<T, MODE, 4> (a, q, p, curl (vy), hy_m, special, oym, gym);
-  update_cell<T, MODE, 5> (a, q, p, curl (vz), hz_m, special, ozm, gzm);
+  update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), hx_m, special, oxm, gxm);
+  update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), hy_m, special, oym, gym);
+  update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl (vz), hz_m, special, ozm, gzm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -442,13 +459,15 @@ __global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
   const int sy = a.sy;   // == 1
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
   const int p[3] = { i, j, 0 };
+  const bool special = is_special (a, p, 2, MODE);
+  T fo, dold;
+  load_old<T, MODE, 2> (a, q, special, fo, dold);
   const T *Hx = a.f[3], *Hy = a.f[4];
   T v[4] = { Hy[q - sx], Hy[q], Hx[q - sy], Hx[q] };
   if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0)) tfsf_apply<T, 2, 2> (a, i, j, 0, v);
   // Ez sits on the eps point: the 2-cell average reads the same cell twice
   const T e = a.eps ? a.eps[q] : a.eps_u;
   const T mat = avg2 (e, e);
-  const bool special = is_special (a, p, 2, MODE);
   T om = 0, ga = 0;
   if (MODE == 2 && special)
   {
@@ -456,7 +475,7 @@ __global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
     om = avg2 (o, o);
     ga = avg2 (g, g);
   }
-  update_cell<T, MODE, 2> (a, q, p, curl (v), mat, special, om, ga);
+  update_cell<T, MODE, 2> (a, q, p, fo, dold, curl (v), mat, special, om, ga);
 }
 
 template <typename T, int MODE>
@@ -469,6 +488,10 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
   const int p[3] = { i, j, 0 };
+  const bool special = is_special (a, p, 2, MODE);
+  T fo3, fo4, do3, do4;
+  load_old<T, MODE, 3> (a, q, special, fo3, do3);
+  load_old<T, MODE, 4> (a, q, special, fo4, do4);
   const T *Ez = a.f[2];
   const T ez = Ez[q];
   // Hx: Ey operands absent (0), Ez at (i, j) and (i, j+1); Hy: Ez at (i, j) and (i+1, j), Ex absent
@@ -490,7 +513,6 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
   {
     mx = my = a.mu_u;
   }
-  const bool special = is_special (a, p, 2, MODE);
   T omx = 0, gmx = 0, omy = 0, gmy = 0;
   if (MODE == 2 && special)
   {
@@ -506,8 +528,8 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
       gmy = sum4 (g[q], g[q], g[q + sx], g[q + sx]);
     }
   }
-  update_cell<T, MODE, 3> (a, q, p, curl (vx), mx, special, omx, gmx);
-  update_cell<T, MODE, 4> (a, q, p, curl (vy), my, special, omy, gmy);
+  update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), mx, special, omx, gmx);
+  update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), my, special, omy, gmy);
 }
 
 // ---------------------------------------------------------------------------------------

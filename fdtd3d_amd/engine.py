@@ -28,7 +28,7 @@ EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
-    "yee_set_material_f32",
+    "yee_set_material_f32", "yee_set_material_sphere",
 ]
 
 
@@ -69,6 +69,8 @@ def load_library():
     L.yee_last_step_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.yee_profile_steps.argtypes = [vp, i32, vp]
     L.yee_set_material_f32.argtypes = [vp, i32, vp]
+    L.yee_set_material_sphere.argtypes = [vp, i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.c_double, i32, i32]
     for name in EXPORTED:
         if name not in ("yee_last_error", "yee_destroy"):
             getattr(L, name).restype = ctypes.c_int
@@ -126,6 +128,13 @@ class Engine:
             return
         arr = np.ascontiguousarray(values, dtype=np.float64)
         _check(load_library().yee_set_material(self.h, MAT_INDEX[name], arr.ctypes.data))
+
+    def set_material_sphere(self, name: str, value: float, outside: float, center, radius: float,
+                            accuracy: int = 100, stair: bool = False) -> None:
+        """Sphere material as Scheme::initGrids builds it (center in cell indices)."""
+        c = (ctypes.c_int64 * 3)(*[int(v) for v in (list(center) + [0, 0, 0])[:3]])
+        _check(load_library().yee_set_material_sphere(self.h, MAT_INDEX[name], float(value), float(outside), c,
+                                                      float(radius), int(accuracy), 1 if stair else 0))
 
     def prepare(self) -> None:
         _check(load_library().yee_prepare(self.h))

@@ -124,6 +124,19 @@ int yee_set_material (yee_handle *h, int32_t which, const double *values);
 int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values);
 
 /*
+ * Fills material `which` with a sphere the way Scheme::initGrids does
+ * (Source/Scheme/Scheme.inc.h:2979-3012 for eps; mu, omegaPE, omegaPM likewise at :3020-3165):
+ * every cell gets Approximation::approximateSphereAccurate (volume fraction over an
+ * accuracy x accuracy raster; Source/Physics/Approximation.cpp:563-676) or, when stair != 0,
+ * approximateSphereStair (:730-764) of `value` inside and `outside` elsewhere.  `center` is in
+ * cell indices (the reference adds 0.5, the eps-point offset); `radius` in cells.  The value
+ * replaces the whole grid, as in the reference (a sphere overrides a loaded file).  Omega values
+ * are in rad/s: the reference passes --omegape-sphere * 2*pi*f here.
+ */
+int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double outside, const int64_t center[3],
+                             double radius, int32_t accuracy, int32_t stair);
+
+/*
  * Host FP64 set-up of everything the kernels read: PML sigma profiles
  * (SchemeHelper::initSigma{X,Y,Z}, Source/Scheme/SchemeHelper.h:574-718), the material
  * encoding, Drude coefficients, TF/SF face tables and the incident 1D grid parameters.

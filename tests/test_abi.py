@@ -72,3 +72,14 @@ def test_cmd_file(tmp_path):
     f.write_text("# comment\n--3d\n--size\nx:12,y:10,z:8\n// another\n--time-steps\n7\n")
     cfg = parse_args(["--cmd-from-file", str(f)])
     assert cfg.size == [12, 10, 8] and cfg.time_steps == 7
+
+
+def test_cli_rejects_options_outside_the_path():
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fdtd3d_amd", "bin", "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    import subprocess
+    for argv in (["--1d-exhy"], ["--no-such-flag"], ["--size"]):
+        out = subprocess.run([exe] + argv, capture_output=True, text=True, timeout=60)
+        assert out.returncode != 0, argv
+        assert out.stderr.startswith("fdtd3d: "), out.stderr
