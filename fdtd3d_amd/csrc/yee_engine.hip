@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -27,6 +28,7 @@ template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
 template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
+template <typename T, int BY> __global__ void step_3d_fused (StepArgs<T>);
 
 static thread_local std::string g_error;
 
@@ -63,7 +65,12 @@ struct yee_handle
   // state: two buffers per array when the fused 3D step is used (old -> new), else one
   void *f[2][6] = {}, *d[2][6] = {}, *dp[2][6] = {}, *d1[2][6] = {}, *d1p[2][6] = {};
   int cur = 0;          // buffer holding the newest state
-  bool fused = false;   // 3D fused step kernel (else in-place half-step kernels)
+  bool fused = false;   // 3D fused step kernel on the PML-free box + slab half-steps
+  int64_t fbox_lo[3] = { 1, 1, 1 }, fbox_hi[3] = { 1, 1, 1 };   // PML-free box of the fused kernel
+  int fused_lx = 0, fused_nstrips = 0, fused_nchunks = 0, fused_threads = 0;
+  // padded grid allocations (every grid-sized array; the views above point inside them)
+  std::vector<void *> grids;
+  void *tg = nullptr;   // TfsfGeom<T> in device memory
 
   void *mat[6] = {};
   double mat_u[6] = { 1, 1, 0, 0, 0, 0 };
@@ -91,8 +98,16 @@ struct yee_handle
 
 namespace {
 
+// Which buffers a launch reads and writes (double-buffered state only; in place otherwise).
+enum ArgRole
+{
+  kRoleE,       // E update: old E, old H -> new E
+  kRoleH,       // H update: new E, old H -> new H
+  kRoleFused,   // whole step: old E, old H -> new E, new H
+};
+
 template <typename T>
-StepArgs<T> make_args (const yee_handle *h, int parity)
+StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
 {
   StepArgs<T> a {};
   a.nx = (int) h->n[0];
@@ -109,7 +124,6 @@ StepArgs<T> make_args (const yee_handle *h, int parity)
     a.box_hi[ax] = (int) h->box_hi[ax];
     a.tl[ax] = (int) h->cfg.tfsf_left[ax];
     a.tr[ax] = (int) (h->cfg.size[ax] - h->cfg.tfsf_right[ax]);
-    a.zinc[ax] = h->sc.zero_inc[ax];
   }
   if (h->dim == 2)
   {
@@ -119,7 +133,7 @@ StepArgs<T> make_args (const yee_handle *h, int parity)
   const int src = h->cur, dst = h->fused ? 1 - h->cur : h->cur;
   for (int c = 0; c < 6; ++c)
   {
-    a.f[c] = (T *) h->f[src][c];
+    a.f[c] = (T *) h->f[(role == kRoleH && c < 3) ? dst : src][c];
     a.d[c] = (T *) h->d[src][c];
     a.dp[c] = (T *) h->dp[src][c];
     a.d1[c] = (T *) h->d1[src][c];
@@ -129,7 +143,11 @@ StepArgs<T> make_args (const yee_handle *h, int parity)
     a.dp_n[c] = (T *) h->dp[dst][c];
     a.d1_n[c] = (T *) h->d1[dst][c];
     a.d1p_n[c] = (T *) h->d1p[dst][c];
-    a.proj[c] = h->sc.proj[c];
+  }
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    a.run_lo[ax] = 1;
+    a.run_hi[ax] = (int) h->n[ax] - 1;
   }
   a.eps = (const T *) h->mat[YEE_MAT_EPS];
   a.mu = (const T *) h->mat[YEE_MAT_MU];
@@ -146,25 +164,28 @@ StepArgs<T> make_args (const yee_handle *h, int parity)
   a.pml = (const T *) h->pml;
   a.tab_len = h->tab_len;
   a.tfsf = h->use_tfsf () ? 1 : 0;
-  a.S1 = h->sc.S1;
-  a.C1 = h->sc.C1;
-  a.S2 = h->sc.S2;
-  a.C2 = h->sc.C2;
-  a.einc = (const T *) h->einc;
+  a.tg = (const TfsfGeom<T> *) h->tg;
   // the E half-step of step t reads HInc after step t-1, kept in buffer (t+1) & 1
-  a.hinc = (const T *) h->hinc[parity];
-  a.inc_len = h->inc_len;
+  a.hinc_sel = parity;
   return a;
 }
 
 template <typename T>
-int launch_half (yee_handle *h, bool e_half, int parity)
+int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nullptr, const int64_t *hi = nullptr)
 {
-  StepArgs<T> a = make_args<T> (h, parity);
+  StepArgs<T> a = make_args<T> (h, parity, e_half ? kRoleE : kRoleH);
+  if (lo)
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      a.run_lo[ax] = (int) lo[ax];
+      a.run_hi[ax] = (int) hi[ax];
+    }
   if (h->dim == 3)
   {
     dim3 block (64, 4, 1);
-    dim3 grid ((unsigned) ((h->n[2] + 63) / 64), (unsigned) ((h->n[1] + 3) / 4), (unsigned) (h->n[0] - 2));
+    const int64_t ext[3] = { a.run_hi[0] - a.run_lo[0], a.run_hi[1] - a.run_lo[1], a.run_hi[2] - a.run_lo[2] };
+    if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) return YEE_OK;
+    dim3 grid ((unsigned) ((ext[2] + 63) / 64), (unsigned) ((ext[1] + 3) / 4), (unsigned) ext[0]);
     switch (h->mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
@@ -193,8 +214,26 @@ int launch_half (yee_handle *h, bool e_half, int parity)
   return YEE_OK;
 }
 
+// Hard point-source value of the E (t + 1/2) or H (t + 1) update of step t
+// (performPointSourceCalc, InternalScheme.inc.h:2818-2895), with the C library's sin.
+double source_value (const yee_handle *h, bool e_half, int32_t t)
+{
+  const double ts = e_half ? t + 0.5 : t + 1.0;
+  return std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+}
+
+bool source_in_fused_box (const yee_handle *h)
+{
+  const int64_t *p = h->cfg.point_source_pos;
+  for (int ax = 0; ax < 3; ++ax)
+    if (p[ax] < h->fbox_lo[ax] || p[ax] >= h->fbox_hi[ax]) return false;
+  return true;
+}
+
+// Point sources as separate one-cell launches into buffer `buf`; with the fused step the
+// sources inside its box are applied by that kernel instead (skip_box).
 template <typename T>
-int launch_sources (yee_handle *h, bool e_half, int32_t t)
+int launch_sources (yee_handle *h, bool e_half, int32_t t, int buf, bool skip_box)
 {
   const int c0 = e_half ? 0 : 3;
   for (int c = c0; c < c0 + 3; ++c)
@@ -204,10 +243,10 @@ int launch_sources (yee_handle *h, bool e_half, int32_t t)
     const int64_t pz = h->dim == 3 ? p[2] : 0;
     // getFieldValueOrNullByAbsolutePos: positions outside the grid are ignored
     if (p[0] < 0 || p[0] >= h->n[0] || p[1] < 0 || p[1] >= h->n[1] || pz < 0 || pz >= h->n[2]) continue;
+    if (skip_box && source_in_fused_box (h)) continue;
     const int64_t q = (p[0] * h->n[1] + p[1]) * h->n[2] + pz;
-    const double ts = e_half ? t + 0.5 : t + 1.0;
-    const double arg = h->sc.dt * ts * 2 * kPi * h->sc.freq;
-    hipLaunchKernelGGL ((point_source<T>), dim3 (1), dim3 (64), 0, h->stream, (T *) h->f[h->cur][c], q, std::sin (arg));
+    hipLaunchKernelGGL ((point_source<T>), dim3 (1), dim3 (64), 0, h->stream, (T *) h->f[buf][c], q,
+                       source_value (h, e_half, t));
     HIP_TRY (hipGetLastError ());
   }
   return YEE_OK;
@@ -239,6 +278,77 @@ struct ProfScope
   }
 };
 
+// The six PML slabs around the fused kernel's box: [lo, hi) boxes covering
+// [1, N-1)^3 minus the box.
+int slab_boxes (const yee_handle *h, int64_t lo[6][3], int64_t hi[6][3])
+{
+  const int64_t *bl = h->fbox_lo, *bh = h->fbox_hi;
+  const int64_t n0 = h->n[0] - 1, n1 = h->n[1] - 1, n2 = h->n[2] - 1;
+  const int64_t L[6][3] = { { 1, 1, 1 }, { bh[0], 1, 1 }, { bl[0], 1, 1 }, { bl[0], bh[1], 1 },
+                            { bl[0], bl[1], 1 }, { bl[0], bl[1], bh[2] } };
+  const int64_t H[6][3] = { { bl[0], n1, n2 }, { n0, n1, n2 }, { bh[0], bl[1], n2 }, { bh[0], n1, n2 },
+                            { bh[0], bh[1], bl[2] }, { bh[0], bh[1], n2 } };
+  int m = 0;
+  for (int s = 0; s < 6; ++s)
+  {
+    if (H[s][0] <= L[s][0] || H[s][1] <= L[s][1] || H[s][2] <= L[s][2]) continue;
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      lo[m][ax] = L[s][ax];
+      hi[m][ax] = H[s][ax];
+    }
+    ++m;
+  }
+  return m;
+}
+
+// One time step (t > 0) with the fused kernel: E slabs, E sources outside the box, fused
+// E+H sweep of the box, H slabs, H sources outside the box; old state in buffer cur, new
+// state in 1 - cur.
+template <typename T>
+int fused_step (yee_handle *h, int32_t t, int prev)
+{
+  int rc;
+  const int dst = 1 - h->cur;
+  int64_t lo[6][3], hi[6][3];
+  const int nslab = slab_boxes (h, lo, hi);
+  {
+    ProfScope ps (h, 1);
+    for (int s = 0; s < nslab; ++s)
+      if ((rc = launch_half<T> (h, true, prev, lo[s], hi[s]))) return rc;
+  }
+  if ((rc = launch_sources<T> (h, true, t, dst, true))) return rc;
+  {
+    StepArgs<T> a = make_args<T> (h, prev, kRoleFused);
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      a.run_lo[ax] = (int) h->fbox_lo[ax];
+      a.run_hi[ax] = (int) h->fbox_hi[ax];
+    }
+    a.lx = h->fused_lx;
+    a.nstrips = h->fused_nstrips;
+    if (source_in_fused_box (h))
+    {
+      const int64_t *p = h->cfg.point_source_pos;
+      a.src_mask = h->cfg.point_source & 0x3fu;
+      a.src_q = (p[0] * h->n[1] + p[1]) * h->n[2] + p[2];
+      a.src_e = (T) source_value (h, true, t);
+      a.src_h = (T) source_value (h, false, t);
+    }
+    ProfScope ps (h, 3);
+    const unsigned blocks = (unsigned) (h->fused_nstrips * h->fused_nchunks);
+    hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T)>), dim3 (blocks), dim3 ((unsigned) h->fused_threads), 0,
+                       h->stream, a);
+    HIP_TRY (hipGetLastError ());
+  }
+  {
+    ProfScope ps (h, 2);
+    for (int s = 0; s < nslab; ++s)
+      if ((rc = launch_half<T> (h, false, prev, lo[s], hi[s]))) return rc;
+  }
+  return launch_sources<T> (h, false, t, dst, true);
+}
+
 template <typename T>
 int step_once (yee_handle *h)
 {
@@ -256,19 +366,43 @@ int step_once (yee_handle *h)
     HIP_TRY (hipGetLastError ());
   }
   int rc;
+  if (h->fused && t > 0)
+  {
+    if ((rc = fused_step<T> (h, t, prev))) return rc;
+    h->cur = 1 - h->cur;
+    h->t++;
+    return YEE_OK;
+  }
   if (t > 0)
   {
     ProfScope ps (h, 1);
     if ((rc = launch_half<T> (h, true, prev))) return rc;
   }
-  if ((rc = launch_sources<T> (h, true, t))) return rc;
+  if ((rc = launch_sources<T> (h, true, t, h->cur, false))) return rc;
   if (t > 0)
   {
     ProfScope ps (h, 2);
     if ((rc = launch_half<T> (h, false, prev))) return rc;
   }
-  if ((rc = launch_sources<T> (h, false, t))) return rc;
+  if ((rc = launch_sources<T> (h, false, t, h->cur, false))) return rc;
   h->t++;
+  return YEE_OK;
+}
+
+// Grid-sized arrays carry padding on both sides: the fused kernel reads the z neighbours of
+// a row as immediate offsets (one element before the first row) and whole rows past the last
+// one (up to FUSED_BY + 1 rows); those values only reach cells that are not updated.
+constexpr int64_t kPadFront = 64;
+int64_t pad_back (const yee_handle *h) { return 4 * h->n[2] + 64; }
+
+int alloc_grid (yee_handle *h, void **view)
+{
+  const size_t elems = (size_t) (kPadFront + h->ncell + pad_back (h));
+  void *raw = nullptr;
+  HIP_TRY (hipMalloc (&raw, elems * h->wsize));
+  h->grids.push_back (raw);
+  HIP_TRY (hipMemsetAsync (raw, 0, elems * h->wsize, h->stream));
+  *view = (char *) raw + kPadFront * h->wsize;
   return YEE_OK;
 }
 
@@ -336,7 +470,9 @@ int upload_as (yee_handle *h, void **dst, const S *src, size_t n)
 {
   if (!*dst)
   {
-    HIP_TRY (hipMalloc (dst, n * h->wsize));
+    int rc = alloc_grid (h, dst);
+    if (rc) return rc;
+    HIP_TRY (hipStreamSynchronize (h->stream));
   }
   if (h->fp32 ())
   {
@@ -406,14 +542,29 @@ int set_material_impl (yee_handle *h, int32_t which, const S *values)
   if ((uniform && which < 2) || !nonzero)
   {
     if (which < 2) h->mat_u[which] = (double) values[0];
-    if (h->mat[which])
-    {
-      (void) hipFree (h->mat[which]);
-      h->mat[which] = nullptr;
-    }
+    h->mat[which] = nullptr;   // an earlier grid allocation stays owned by h->grids
     return YEE_OK;
   }
   return upload_as (h, &h->mat[which], values, n);
+}
+
+template <typename T>
+int upload_tfsf_geom (yee_handle *h)
+{
+  TfsfGeom<T> g {};
+  for (int ax = 0; ax < 3; ++ax) g.zinc[ax] = h->sc.zero_inc[ax];
+  g.S1 = h->sc.S1;
+  g.C1 = h->sc.C1;
+  g.S2 = h->sc.S2;
+  g.C2 = h->sc.C2;
+  for (int c = 0; c < 6; ++c) g.proj[c] = h->sc.proj[c];
+  g.einc = (const T *) h->einc;
+  g.hinc[0] = (const T *) h->hinc[0];
+  g.hinc[1] = (const T *) h->hinc[1];
+  g.inc_len = h->inc_len;
+  if (!h->tg) HIP_TRY (hipMalloc (&h->tg, sizeof g));
+  HIP_TRY (hipMemcpy (h->tg, &g, sizeof g, hipMemcpyHostToDevice));
+  return YEE_OK;
 }
 
 }  // namespace
@@ -462,21 +613,51 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (e != hipSuccess) return bail (fail (YEE_EDEVICE, std::string ("hipStreamCreate: ") + hipGetErrorString (e)));
   if (hipEventCreate (&h->ev[0]) != hipSuccess || hipEventCreate (&h->ev[1]) != hipSuccess)
     return bail (fail (YEE_EDEVICE, "hipEventCreate failed"));
-  const size_t bytes = (size_t) h->ncell * h->wsize;
-  // one buffer per array: the half-step kernels update in place
+  // 3D runs without Drude terms can use the fused step kernel on the PML-free box
+  // (double-buffered state); TMz and Drude runs update in place with the half-step kernels.
+  // YEE_KERNEL=fused / split in the environment selects the 3D path (A/B checks).
+  const char *kern = std::getenv ("YEE_KERNEL");
+  const bool want_fused = kern && std::strcmp (kern, "fused") == 0;
   h->fused = false;
-  const int nbuf = 1;
+  if (h->dim == 3 && h->mode <= 1 && h->n[2] <= 1024 && want_fused)
+  {
+    bool ok = true;
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      const int64_t P = h->use_pml () ? cfg->pml[ax] : 0;
+      h->fbox_lo[ax] = std::max<int64_t> (1, P);
+      h->fbox_hi[ax] = std::min<int64_t> (h->n[ax] - 1, h->n[ax] - P - 1);
+      ok = ok && h->fbox_hi[ax] > h->fbox_lo[ax];
+    }
+    h->fused = ok;
+  }
+  if (h->fused)
+  {
+    const int by = h->fp32 () ? FUSED_BY (float) : FUSED_BY (double);
+    int dev = 0, ncu = 256;
+    if (hipGetDevice (&dev) == hipSuccess) (void) hipDeviceGetAttribute (&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t rows = h->fbox_hi[1] - h->fbox_lo[1], planes = h->fbox_hi[0] - h->fbox_lo[0];
+    h->fused_nstrips = (int) ((rows + by - 1) / by);
+    // one 1024-thread workgroup per CU at a time: chunk x so that there are >= 2 per CU
+    int64_t nch = std::max<int64_t> (1, (2 * (int64_t) ncu + h->fused_nstrips - 1) / h->fused_nstrips);
+    if (const char *e = std::getenv ("YEE_FUSED_CHUNKS")) nch = std::max (1, std::atoi (e));
+    nch = std::min (nch, planes);
+    h->fused_lx = (int) ((planes + nch - 1) / nch);
+    h->fused_nchunks = (int) ((planes + h->fused_lx - 1) / h->fused_lx);
+    h->fused_threads = (int) ((h->n[2] + 63) / 64 * 64);
+  }
+  const int nbuf = h->fused ? 2 : 1;
   for (int bsel = 0; bsel < nbuf; ++bsel)
     for (int c = 0; c < 6; ++c)
     {
       if (!h->present[c]) continue;
-      if ((rc = alloc_zero (h, &h->f[bsel][c], bytes))) return bail (rc);
-      if (h->mode >= 1 && (rc = alloc_zero (h, &h->d[bsel][c], bytes))) return bail (rc);
+      if ((rc = alloc_grid (h, &h->f[bsel][c]))) return bail (rc);
+      if (h->mode >= 1 && (rc = alloc_grid (h, &h->d[bsel][c]))) return bail (rc);
       if (h->mode == 2)
       {
-        if ((rc = alloc_zero (h, &h->dp[bsel][c], bytes))) return bail (rc);
-        if ((rc = alloc_zero (h, &h->d1[bsel][c], bytes))) return bail (rc);
-        if ((rc = alloc_zero (h, &h->d1p[bsel][c], bytes))) return bail (rc);
+        if ((rc = alloc_grid (h, &h->dp[bsel][c]))) return bail (rc);
+        if ((rc = alloc_grid (h, &h->d1[bsel][c]))) return bail (rc);
+        if ((rc = alloc_grid (h, &h->d1p[bsel][c]))) return bail (rc);
       }
     }
   if (h->use_tfsf ())
@@ -497,16 +678,8 @@ void yee_destroy (yee_handle *h)
 {
   if (!h) return;
   if (h->stream) (void) hipStreamSynchronize (h->stream);
-  for (int c = 0; c < 6; ++c)
-  {
-    for (int bsel = 0; bsel < 2; ++bsel)
-    {
-      void *ptrs[5] = { h->f[bsel][c], h->d[bsel][c], h->dp[bsel][c], h->d1[bsel][c], h->d1p[bsel][c] };
-      for (void *p : ptrs)
-        if (p) (void) hipFree (p);
-    }
-    if (h->mat[c]) (void) hipFree (h->mat[c]);
-  }
+  for (void *p : h->grids) (void) hipFree (p);
+  if (h->tg) (void) hipFree (h->tg);
   if (h->pml) (void) hipFree (h->pml);
   if (h->einc) (void) hipFree (h->einc);
   if (h->hinc[0]) (void) hipFree (h->hinc[0]);
@@ -574,6 +747,11 @@ int yee_prepare (yee_handle *h)
       h->box_hi[a] = 0;
     }
   }
+  {
+    // incident-wave geometry of the TF/SF correction, read by the corrected cells only
+    int rc = h->fp32 () ? upload_tfsf_geom<float> (h) : upload_tfsf_geom<double> (h);
+    if (rc) return rc;
+  }
   if (h->use_pml ())
   {
     h->tab_len = (int) std::max (h->n[0], std::max (h->n[1], h->n[2]));
@@ -605,7 +783,7 @@ int yee_step (yee_handle *h, int32_t n)
   return YEE_OK;
 }
 
-int yee_profile_steps (yee_handle *h, int32_t n, double out[4])
+int yee_profile_steps (yee_handle *h, int32_t n, double out[5])
 {
   if (!h || !out) return fail (YEE_EINVAL, "null argument");
   if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
@@ -619,8 +797,8 @@ int yee_profile_steps (yee_handle *h, int32_t n, double out[4])
   int rc = yee_step (h, n);
   h->prof_events = nullptr;
   h->prof_kinds = nullptr;
-  double sum[3] = { 0, 0, 0 };
-  int cnt[3] = { 0, 0, 0 };
+  double sum[4] = { 0, 0, 0, 0 };
+  int cnt[4] = { 0, 0, 0, 0 };
   for (size_t e = 0; rc == YEE_OK && e + 1 < events.size (); e += 2)
   {
     float ms = 0;
@@ -632,8 +810,8 @@ int yee_profile_steps (yee_handle *h, int32_t n, double out[4])
   }
   for (hipEvent_t e : events) (void) hipEventDestroy (e);
   if (rc) return rc;
-  for (int k = 0; k < 3; ++k) out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
-  out[3] = h->last_ms;
+  for (int k = 0; k < 4; ++k) out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+  out[4] = h->last_ms;
   return YEE_OK;
 }
 

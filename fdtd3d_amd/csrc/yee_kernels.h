@@ -7,6 +7,21 @@ namespace yee {
 
 constexpr int kMaxDim = 3;
 
+// y rows per thread of the fused step kernel (FP64 holds half as many values per register)
+#define FUSED_BY(T) (sizeof (T) == 4 ? 2 : 1)
+
+// Incident-wave geometry of the TF/SF correction (FP64; InternalScheme::approximateIncidentWave).
+template <typename T>
+struct TfsfGeom
+{
+  double zinc[3];              // zeroIncCoordFP
+  double S1, C1, S2, C2;       // sin / cos of the incidence angles
+  double proj[6];              // get<comp>FromIncident factors
+  const T *einc;               // EInc after this step's E update (read by H)
+  const T *hinc[2];            // HInc by step parity (E reads the layer after the previous step)
+  int64_t inc_len;
+};
+
 // Geometry and tables common to every half-step kernel.
 template <typename T>
 struct StepArgs
@@ -20,6 +35,9 @@ struct StepArgs
   // (p < slab_lo[a] or p >= slab_hi[a]) and, with metamaterials, the Drude box
   int slab_lo[3], slab_hi[3];
   int box_lo[3], box_hi[3];
+  // cells a launch updates: [run_lo, run_hi) on every axis (the full interior [1, N-1), a
+  // PML slab, or the PML-free box of the fused step kernel)
+  int run_lo[3], run_hi[3];
 
   // state read by this half-step / step
   T *f[6];      // Ex, Ey, Ez, Hx, Hy, Hz
@@ -41,15 +59,19 @@ struct StepArgs
   const T *pml;
   int tab_len;
 
-  // TF/SF (geometry kept in FP64; corrections touch O(N^2) cells only)
+  // TF/SF: the box bounds are tested for every cell; the incident-wave geometry is only read
+  // by the O(N^2) corrected cells and lives in device memory (keeps scalar registers free)
   int tfsf;
   int tl[3], tr[3];            // total-field box [tl, tr] in integer coordinates
-  double zinc[3];
-  double S1, C1, S2, C2;
-  double proj[6];
-  const T *einc;               // EInc after this step's E update (read by H)
-  const T *hinc;               // HInc after the previous step (read by E)
-  int64_t inc_len;
+  const struct TfsfGeom<T> *tg;
+  int hinc_sel;                // HInc buffer holding the layer after the previous step
+
+  // fused step kernel: work decomposition and the point sources it applies in registers
+  int lx;                      // x planes per chunk
+  int nstrips;                 // y strips per chunk
+  uint32_t src_mask;           // bit c: hard source on component c at flat index src_q
+  int64_t src_q;
+  T src_e, src_h;              // source values of this step's E and H updates
 };
 
 }  // namespace yee

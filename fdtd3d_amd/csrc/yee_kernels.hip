@@ -33,6 +33,30 @@ __device__ __forceinline__ void gst (T *p, T v)
   *(__attribute__ ((address_space (1))) T *) (p) = v;
 }
 
+// Row-relative accessors for the fused kernel.  `row` MUST be uniform over the wavefront:
+// it is pinned to scalar registers (the empty asm also keeps the compiler from folding the
+// lane offset into it), so every access is selected as scalar base + 32-bit lane offset
+// `boff` + immediate `delta` (elements), with no per-access 64-bit address registers.
+template <typename T>
+__device__ __forceinline__ T rld (const T *row, uint32_t boff, int delta = 0)
+{
+  typedef const __attribute__ ((address_space (1))) char gchar;
+  typedef const __attribute__ ((address_space (1))) T gT;
+  asm volatile ("" : "+s"(row));
+  gchar *p = (gchar *) row + boff;
+  return *((gT *) p + delta);
+}
+
+template <typename T>
+__device__ __forceinline__ void rst (T *row, uint32_t boff, T v)
+{
+  typedef __attribute__ ((address_space (1))) char gchar;
+  typedef __attribute__ ((address_space (1))) T gT;
+  asm volatile ("" : "+s"(row));
+  gchar *p = (gchar *) row + boff;
+  *(gT *) p = v;
+}
+
 // E_CENTERED component tables (the host CompDesc plus the circuit offsets of
 // YeeGridLayout.cpp:1937-1961).  Every lookup below is resolved at compile time: runtime
 // indexing of small arrays would push them to scratch memory.
@@ -140,27 +164,28 @@ __device__ __forceinline__ bool tfsf_near (const StepArgs<T> &a, int i, int j, i
 template <typename T, int OPP, int DIM>
 __device__ double incident_at (const StepArgs<T> &a, int i, int j, int k, const T *inc, double ddiff)
 {
-  const double x = 0.5 * (double) twice_coord<OPP, 0> (i) - a.zinc[0];
-  const double y = 0.5 * (double) twice_coord<OPP, 1> (j) - a.zinc[1];
+  const TfsfGeom<T> &tg = *a.tg;
+  const double x = 0.5 * (double) twice_coord<OPP, 0> (i) - tg.zinc[0];
+  const double y = 0.5 * (double) twice_coord<OPP, 1> (j) - tg.zinc[1];
   double d;
   if (DIM == 3)
   {
-    const double z = 0.5 * (double) twice_coord<OPP, 2> (k) - a.zinc[2];
-    d = x * a.S1 * a.C2 + y * a.S1 * a.S2 + z * a.C1 - ddiff;
+    const double z = 0.5 * (double) twice_coord<OPP, 2> (k) - tg.zinc[2];
+    d = x * tg.S1 * tg.C2 + y * tg.S1 * tg.S2 + z * tg.C1 - ddiff;
   }
   else
   {
-    d = x * a.C2 + y * a.S2 - ddiff;
+    d = x * tg.C2 + y * tg.S2 - ddiff;
   }
   const double c1 = (double) (long long) d;
   long long i1 = (long long) c1;
   // the reference asserts d > 0; clamp so a degenerate box can never read out of bounds
   if (i1 < 0) i1 = 0;
-  if (i1 > a.inc_len - 2) i1 = a.inc_len - 2;
+  if (i1 > tg.inc_len - 2) i1 = tg.inc_len - 2;
   const double p2 = d - c1;
   const double p1 = 1 - p2;
   const double v = (double) gld (inc + i1) * p1 + (double) gld (inc + i1 + 1) * p2;
-  return v * a.proj[OPP];
+  return v * tg.proj[OPP];
 }
 
 // One operand pair of calculateTFSF.  E components (E_CENTERED) correct the operand opposite
@@ -173,7 +198,7 @@ __device__ __forceinline__ void tfsf_pair (const StepArgs<T> &a, int i, int j, i
   const bool lo = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR], DIM> (a, i, j, k);
   const bool hi = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR + 1], DIM> (a, i, j, k);
   if (!lo && !hi) return;
-  const T *inc = h ? a.einc : a.hinc;
+  const T *inc = h ? a.tg->einc : a.tg->hinc[a.hinc_sel];
   const double ddiff = h ? 0.0 : 0.5;
   // slot 2*PAIR is "lo" (d11 / d21), slot 2*PAIR+1 is "hi" (d12 / d22)
   const bool use_hi_slot = h ? !lo : lo;
@@ -323,17 +348,19 @@ __device__ __forceinline__ T sum4 (T m11, T m12, T m21, T m22)
 }
 
 // ---------------------------------------------------------------------------------------
-// 3D E half-step: Ex, Ey, Ez on [1, N-1)^3.  One thread per cell, lane = z, so each
-// wavefront access is a coalesced row; the y/z/x neighbours of the stencil are served by
-// L1/L2 (the neighbouring rows and planes are read by the neighbouring waves).
+// 3D E half-step: Ex, Ey, Ez on the run box (all of [1, N-1)^3, or one PML slab when the
+// fused step kernel covers the rest).  One thread per cell, lane = z, so each wavefront
+// access is a coalesced row; the y/z/x neighbours of the stencil are served by L1/L2 (the
+// neighbouring rows and planes are read by the neighbouring waves).  Reads f, writes f_n
+// (the same arrays when updating in place).
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = (int) blockIdx.z + 1;
-  if (j < 1 || j >= a.ny - 1 || k < 1 || k >= a.nz - 1) return;
+  const int k = a.run_lo[2] + (int) (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = a.run_lo[1] + (int) (blockIdx.y * blockDim.y + threadIdx.y);
+  const int i = a.run_lo[0] + (int) blockIdx.z;
+  if (j >= a.run_hi[1] || k >= a.run_hi[2]) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
@@ -386,15 +413,16 @@ __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------------------
-// 3D H half-step: Hx, Hy, Hz on [1, N-1)^3 (one thread per cell)
+// 3D H half-step: Hx, Hy, Hz on the run box (one thread per cell).  E is read from f[0..2],
+// which the host points at the new E when the state is double-buffered.
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = (int) blockIdx.z + 1;
-  if (j < 1 || j >= a.ny - 1 || k < 1 || k >= a.nz - 1) return;
+  const int k = a.run_lo[2] + (int) (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = a.run_lo[1] + (int) (blockIdx.y * blockDim.y + threadIdx.y);
+  const int i = a.run_lo[0] + (int) blockIdx.z;
+  if (j >= a.run_hi[1] || k >= a.run_hi[2]) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
@@ -444,6 +472,365 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
   update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), hx_m, special, oxm, gxm);
   update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), hy_m, special, oym, gym);
   update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl (vz), hz_m, special, ozm, gzm);
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused 3D time step over the PML-free box: E of step t, then H of step t, in one sweep.
+//
+// The box [run_lo, run_hi) holds every cell whose update is the direct one (sigma = 0 on all
+// axes); the PML slabs around it are updated by e_half_3d / h_half_3d on slab boxes, E slabs
+// before this kernel and H slabs after it.  State is double-buffered: every old value comes
+// from f, every new one goes to f_n, and E outside the box is read back from f_n.
+//
+// Each workgroup owns a full z row (one lane per z cell, up to 16 waves) of a strip of BY
+// y rows and marches along x over a chunk of planes.  Per x plane it computes E one plane
+// ahead of H: E at plane x needs H before the step at x and x-1 (x-1 carried in registers);
+// H at plane x-1 needs E after the step at x-1 (carried) and x (just computed).  The y
+// neighbour row j0+BY is recomputed (Ex, Ez only; not stored), the z neighbour of a lane
+// comes from the next lane by a cross-lane shuffle and, for the last lane of a wave, from the
+// next wave through LDS.  Every state word crosses HBM once per step, except the recomputed
+// row and the first plane of each chunk.
+// ---------------------------------------------------------------------------------------
+
+// Everything the E update of one x plane produces for the fused sweep.
+template <typename T, int BY>
+struct EPlane
+{
+  T ex[BY + 1], ey[BY + 1], ez[BY + 1];   // E after the step, rows 0..BY (Ey: rows 0..BY-1)
+  T hx[BY + 1], hy[BY + 1], hz[BY + 1];   // H before the step at this plane (Hx, Hz: rows 0..BY-1)
+  T ep1[BY + 1];                          // eps at the next plane, rows 0..BY
+};
+
+// Thread geometry of the fused sweep.
+struct FusedPos
+{
+  int j0, k, kc;
+  bool kbox;   // lane inside the box along z
+};
+
+// Direct E/H update (computeCaCb non-PML branch, calcField; same expressions as cell_value).
+template <typename T, int COMP>
+__device__ __forceinline__ T direct_value (const StepArgs<T> &a, T f_old, T tmp, T mat)
+{
+  const T modc = COMP >= 3 ? a.mu0 : a.eps0;
+  const T cb = a.dt / (mat * modc * a.dx);
+  return f_old * (T) 1.0 + tmp * cb;
+}
+
+// Lane byte offsets of the rows j0-1 .. j0+BY relative to row j0-1 (kept opaque, so that all
+// arrays of a plane share them and each array needs one scalar base per plane).  Rows past
+// the grid edge read the allocation padding; their values only reach cells that are not
+// updated.
+template <typename T, int BY>
+__device__ __forceinline__ void row_offsets (const StepArgs<T> &a, const FusedPos &g, uint32_t (&ro)[BY + 2])
+{
+#pragma unroll
+  for (int r = 0; r < BY + 2; ++r)
+  {
+    ro[r] = (uint32_t) g.kc * (uint32_t) sizeof (T) + (uint32_t) (r * a.sy) * (uint32_t) sizeof (T);
+    asm volatile ("" : "+v"(ro[r]));
+  }
+}
+
+// Base of plane x, row j0-1 of array `arr` (uniform).
+template <typename T>
+__device__ __forceinline__ const T *plane_row (const StepArgs<T> &a, const T *arr, int x, const FusedPos &g)
+{
+  return arr + (int64_t) x * a.sx + (int64_t) (g.j0 - 1) * a.sy;
+}
+
+template <typename T, int BY>
+__device__ __forceinline__ void e_plane (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
+                                         const T (&hy_prev)[BY + 1], const T (&hz_prev)[BY + 1],
+                                         const T (&ep0)[BY + 1], EPlane<T, BY> &o)
+{
+  const int nx = a.nx;
+  const T *Ex = plane_row (a, a.f[0], x, g), *Ey = plane_row (a, a.f[1], x, g), *Ez = plane_row (a, a.f[2], x, g);
+  const T *Hx = plane_row (a, a.f[3], x, g), *Hy = plane_row (a, a.f[4], x, g), *Hz = plane_row (a, a.f[5], x, g);
+  T hxr[BY + 2], hzr[BY + 2];              // rows -1..BY
+  T hym[BY + 1], hxm[BY + 1];              // z-1 neighbours
+  T exo[BY + 1], eyo[BY + 1], ezo[BY + 1];
+  T epk[BY + 1];                           // eps at z+1
+#pragma unroll
+  for (int r = -1; r <= BY; ++r)
+  {
+    hxr[r + 1] = rld (Hx, ro[r + 1]);
+    hzr[r + 1] = rld (Hz, ro[r + 1]);
+  }
+  const T *E0 = a.eps ? plane_row (a, a.eps, x, g) : nullptr;
+  const T *E1 = a.eps ? plane_row (a, a.eps, min (x + 1, nx - 1), g) : nullptr;
+#pragma unroll
+  for (int r = 0; r <= BY; ++r)
+  {
+    const uint32_t off = ro[r + 1];
+    o.hy[r] = rld (Hy, off);
+    hym[r] = rld (Hy, off, -1);
+    exo[r] = rld (Ex, off);
+    ezo[r] = rld (Ez, off);
+    if (r < BY)
+    {
+      hxm[r] = rld (Hx, off, -1);
+      eyo[r] = rld (Ey, off);
+    }
+    if (a.eps)
+    {
+      o.ep1[r] = rld (E1, off);
+      epk[r] = rld (E0, off, 1);
+    }
+    else
+    {
+      o.ep1[r] = a.eps_u;
+      epk[r] = a.eps_u;
+    }
+  }
+  // ---- E of rows 0..BY (row BY: Ex and Ez only, for the H update of row BY-1); cells
+  // outside the box take the new value the slab kernel has already written
+  const bool xbox = x >= a.run_lo[0] && x < a.run_hi[0];
+  const int64_t base = (int64_t) x * a.sx;
+#pragma unroll
+  for (int r = 0; r <= BY; ++r)
+  {
+    const int j = g.j0 + r;
+    const bool in = xbox && j >= a.run_lo[1] && j < a.run_hi[1] && g.kbox;
+    T vx[4] = { hzr[r], hzr[r + 1], hym[r], o.hy[r] };
+    T vz[4] = { hy_prev[r], o.hy[r], hxr[r], hxr[r + 1] };
+    T vy[4] = { hxm[r], hxr[r + 1], hz_prev[r], hzr[r + 1] };
+    if (a.tfsf && tfsf_near<T, 3> (a, x, j, g.k))
+    {
+      tfsf_apply<T, 0, 3> (a, x, j, g.k, vx);
+      tfsf_apply<T, 2, 3> (a, x, j, g.k, vz);
+      if (r < BY) tfsf_apply<T, 1, 3> (a, x, j, g.k, vy);
+    }
+    const T e = ep0[r];
+    const uint32_t off = ro[r + 1];
+    const int64_t qt = base + (int64_t) j * a.sy + g.k;
+    T v;
+    if (in)
+    {
+      v = direct_value<T, 0> (a, exo[r], curl (vx), avg2 (e, o.ep1[r]));
+      if ((a.src_mask & 1u) && qt == a.src_q) v = a.src_e;
+    }
+    else
+      v = rld (plane_row (a, (const T *) a.f_n[0], x, g), off);
+    o.ex[r] = v;
+    if (in)
+    {
+      v = direct_value<T, 2> (a, ezo[r], curl (vz), avg2 (e, epk[r]));
+      if ((a.src_mask & 4u) && qt == a.src_q) v = a.src_e;
+    }
+    else
+      v = rld (plane_row (a, (const T *) a.f_n[2], x, g), off);
+    o.ez[r] = v;
+    if (r < BY)
+    {
+      if (in)
+      {
+        v = direct_value<T, 1> (a, eyo[r], curl (vy), avg2 (e, ep0[r + 1]));
+        if ((a.src_mask & 2u) && qt == a.src_q) v = a.src_e;
+      }
+      else
+        v = rld (plane_row (a, (const T *) a.f_n[1], x, g), off);
+      o.ey[r] = v;
+      o.hx[r] = hxr[r + 1];
+      o.hz[r] = hzr[r + 1];
+    }
+  }
+}
+
+// Stores the box rows of an E plane.
+template <typename T, int BY>
+__device__ __forceinline__ void e_store (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
+                                         const EPlane<T, BY> &o)
+{
+  if (x < a.run_lo[0] || x >= a.run_hi[0] || !g.kbox) return;
+  T *Ex = (T *) plane_row (a, (const T *) a.f_n[0], x, g);
+  T *Ey = (T *) plane_row (a, (const T *) a.f_n[1], x, g);
+  T *Ez = (T *) plane_row (a, (const T *) a.f_n[2], x, g);
+#pragma unroll
+  for (int r = 0; r < BY; ++r)
+  {
+    if (g.j0 + r >= a.run_hi[1]) continue;
+    rst (Ex, ro[r + 1], o.ex[r]);
+    rst (Ey, ro[r + 1], o.ey[r]);
+    rst (Ez, ro[r + 1], o.ez[r]);
+  }
+}
+
+template <typename T, int BY>
+__global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
+{
+  __shared__ T xch[2][16][BY][2];   // Ex, Ey after the step at lane 0 of every wave, by plane parity
+
+  FusedPos g;
+  g.k = (int) threadIdx.x;
+  const int lane = g.k & 63;
+  const int wave = g.k >> 6;
+  const int nw = (int) (blockDim.x >> 6);
+  // block -> (chunk, strip): blocks are dispatched round-robin over the 8 XCDs; with
+  // nstrips % 8 == 0 each XCD gets a contiguous band of strips, so the recomputed neighbour
+  // row of a strip is mostly read from the same XCD's L2
+  const int b = (int) blockIdx.x;
+  const int chunk = b / a.nstrips;
+  const int rr = b - chunk * a.nstrips;
+  const int strip = (a.nstrips % 8 == 0) ? (rr % 8) * (a.nstrips / 8) + rr / 8 : rr;
+  g.j0 = a.run_lo[1] + strip * BY;
+  const int xa = a.run_lo[0] + chunk * a.lx;
+  const int xb = min (xa + a.lx, a.run_hi[0]);   // H (and stored E) planes [xa, xb)
+  g.kc = min (g.k, a.nz - 1);
+  g.kbox = g.k >= a.run_lo[2] && g.k < a.run_hi[2];
+  const int64_t sx = a.sx;
+  const int sy = a.sy;
+
+  // carried state: E after the step and H before the step at plane i, eps at plane i+1
+  T exc[BY + 1], eyc[BY + 1], ezc[BY + 1];
+  T hxc[BY + 1], hyc[BY + 1], hzc[BY + 1];
+  T ep0[BY + 1];
+  {
+    // prologue: E at plane xa (H before the step at xa-1 and eps at xa are loaded here)
+    uint32_t ro[BY + 2];
+    row_offsets<T, BY> (a, g, ro);
+    T hyp[BY + 1], hzp[BY + 1], e0[BY + 1];
+    const T *Hy = plane_row (a, a.f[4], xa - 1, g), *Hz = plane_row (a, a.f[5], xa - 1, g);
+    const T *E0 = a.eps ? plane_row (a, a.eps, xa, g) : nullptr;
+#pragma unroll
+    for (int r = 0; r <= BY; ++r)
+    {
+      hyp[r] = rld (Hy, ro[r + 1]);
+      hzp[r] = rld (Hz, ro[r + 1]);
+      e0[r] = a.eps ? rld (E0, ro[r + 1]) : a.eps_u;
+    }
+    EPlane<T, BY> o;
+    e_plane<T, BY> (a, xa, g, ro, hyp, hzp, e0, o);
+    e_store<T, BY> (a, xa, g, ro, o);
+#pragma unroll
+    for (int r = 0; r <= BY; ++r)
+    {
+      exc[r] = o.ex[r];
+      eyc[r] = o.ey[r];
+      ezc[r] = o.ez[r];
+      hxc[r] = o.hx[r];
+      hyc[r] = o.hy[r];
+      hzc[r] = o.hz[r];
+      ep0[r] = o.ep1[r];
+    }
+    if (lane == 0)
+    {
+#pragma unroll
+      for (int r = 0; r < BY; ++r)
+      {
+        xch[0][wave][r][0] = exc[r];
+        xch[0][wave][r][1] = eyc[r];
+      }
+    }
+    __syncthreads ();
+  }
+
+  int par = 0;
+  for (int x = xa + 1; x <= xb; ++x)
+  {
+    const int i = x - 1;   // H plane
+    uint32_t ro[BY + 2];
+    row_offsets<T, BY> (a, g, ro);
+    EPlane<T, BY> o;
+    e_plane<T, BY> (a, x, g, ro, hyc, hzc, ep0, o);
+    par ^= 1;
+    if (lane == 0)
+    {
+#pragma unroll
+      for (int r = 0; r < BY; ++r)
+      {
+        xch[par][wave][r][0] = o.ex[r];
+        xch[par][wave][r][1] = o.ey[r];
+      }
+    }
+    // z+1 neighbours of Ex, Ey after the step at plane i
+    T exk[BY + 1], eyk[BY + 1];
+#pragma unroll
+    for (int r = 0; r < BY; ++r)
+    {
+      exk[r] = __shfl_down (exc[r], 1);
+      eyk[r] = __shfl_down (eyc[r], 1);
+      if (lane == 63 && wave + 1 < nw)
+      {
+        exk[r] = xch[par ^ 1][wave + 1][r][0];
+        eyk[r] = xch[par ^ 1][wave + 1][r][1];
+      }
+    }
+    // ---- H of plane i, rows 0..BY-1
+    T hn[3][BY + 1];
+    T mxm[BY + 1], mym[BY + 1], mzm[BY + 1];
+    const T *M0 = a.mu ? plane_row (a, a.mu, i, g) : nullptr;
+    const T *M1 = a.mu ? plane_row (a, a.mu, i + 1, g) : nullptr;   // i + 1 <= nx - 2
+#pragma unroll
+    for (int r = 0; r < BY; ++r)
+    {
+      if (a.mu)
+      {
+        // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
+        const uint32_t o0 = ro[r + 1], o1 = ro[r + 2];
+        mxm[r] = avg4 (rld (M0, o0), rld (M0, o0, 1), rld (M0, o1), rld (M0, o1, 1));
+        mym[r] = avg4 (rld (M0, o0), rld (M0, o0, 1), rld (M1, o0), rld (M1, o0, 1));
+        mzm[r] = avg4 (rld (M0, o0), rld (M0, o1), rld (M1, o0), rld (M1, o1));
+      }
+      else
+      {
+        mxm[r] = mym[r] = mzm[r] = a.mu_u;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < BY; ++r)
+    {
+      const int j = g.j0 + r;
+      T vx[4] = { eyc[r], eyk[r], ezc[r], ezc[r + 1] };
+      T vy[4] = { ezc[r], o.ez[r], exc[r], exk[r] };
+      T vz[4] = { exc[r], exc[r + 1], eyc[r], o.ey[r] };
+      if (a.tfsf && tfsf_near<T, 3> (a, i, j, g.k))
+      {
+        tfsf_apply<T, 3, 3> (a, i, j, g.k, vx);
+        tfsf_apply<T, 4, 3> (a, i, j, g.k, vy);
+        tfsf_apply<T, 5, 3> (a, i, j, g.k, vz);
+      }
+      const int64_t qt = (int64_t) i * sx + (int64_t) j * sy + g.k;
+      T v = direct_value<T, 3> (a, hxc[r], curl (vx), mxm[r]);
+      if ((a.src_mask & 8u) && qt == a.src_q) v = a.src_h;
+      hn[0][r] = v;
+      v = direct_value<T, 4> (a, hyc[r], curl (vy), mym[r]);
+      if ((a.src_mask & 16u) && qt == a.src_q) v = a.src_h;
+      hn[1][r] = v;
+      v = direct_value<T, 5> (a, hzc[r], curl (vz), mzm[r]);
+      if ((a.src_mask & 32u) && qt == a.src_q) v = a.src_h;
+      hn[2][r] = v;
+    }
+    // ---- stores: E of plane x when this chunk owns it, H of plane i (box cells only)
+    if (x < xb) e_store<T, BY> (a, x, g, ro, o);
+    if (g.kbox)
+    {
+      T *Hx = (T *) plane_row (a, (const T *) a.f_n[3], i, g);
+      T *Hy = (T *) plane_row (a, (const T *) a.f_n[4], i, g);
+      T *Hz = (T *) plane_row (a, (const T *) a.f_n[5], i, g);
+#pragma unroll
+      for (int r = 0; r < BY; ++r)
+      {
+        if (g.j0 + r >= a.run_hi[1]) continue;
+        rst (Hx, ro[r + 1], hn[0][r]);
+        rst (Hy, ro[r + 1], hn[1][r]);
+        rst (Hz, ro[r + 1], hn[2][r]);
+      }
+    }
+    // ---- rotate the carried state
+#pragma unroll
+    for (int r = 0; r <= BY; ++r)
+    {
+      exc[r] = o.ex[r];
+      eyc[r] = o.ey[r];
+      ezc[r] = o.ez[r];
+      hxc[r] = o.hx[r];
+      hyc[r] = o.hy[r];
+      hzc[r] = o.hz[r];
+      ep0[r] = o.ep1[r];
+    }
+    __syncthreads ();
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -620,6 +1007,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void h_half_tmz<T, 1> (StepArgs<T>);                                 \
   template __global__ void h_half_tmz<T, 2> (StepArgs<T>);                                 \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
+  template __global__ void step_3d_fused<T, FUSED_BY (T)> (StepArgs<T>);                   \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *);
 

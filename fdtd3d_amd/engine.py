@@ -158,9 +158,10 @@ class Engine:
 
     def profile_steps(self, n: int) -> dict:
         """Runs n steps with per-kernel HIP events; average ms per launch of each kernel."""
-        out = np.zeros(4, dtype=np.float64)
+        out = np.zeros(5, dtype=np.float64)
         _check(load_library().yee_profile_steps(self.h, int(n), out.ctypes.data))
-        return {"incident_ms": out[0], "e_half_ms": out[1], "h_half_ms": out[2], "step_ms": out[3]}
+        return {"incident_ms": float(out[0]), "e_half_ms": float(out[1]), "h_half_ms": float(out[2]),
+                "fused_ms": float(out[3]), "step_ms": float(out[4])}
 
     def field(self, name: str) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.float64)

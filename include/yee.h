@@ -191,12 +191,14 @@ int yee_get_scalars (yee_handle *h, double out[3]);
 int yee_last_step_ms (yee_handle *h, double *ms);
 
 /*
- * Runs `n` steps like yee_step with a HIP event pair around every kernel launch and
- * returns average device milliseconds per launch: out[0] incident 1D update, out[1] E
- * half-step, out[2] H half-step, out[3] whole step (first event to last event / n).
+ * Runs `n` steps like yee_step with a HIP event pair around every group of kernel launches
+ * and returns average device milliseconds per step of each group: out[0] incident 1D
+ * update, out[1] E half-step (with the fused step: the E update of the PML slabs), out[2]
+ * H half-step (fused: the H update of the PML slabs), out[3] fused E+H step kernel of the
+ * PML-free box (0 when not used), out[4] whole step (yee_last_step_ms of the run).
  * Benchmark hook; the extra events cost a few microseconds per step.
  */
-int yee_profile_steps (yee_handle *h, int32_t n, double out[4]);
+int yee_profile_steps (yee_handle *h, int32_t n, double out[5]);
 
 #ifdef __cplusplus
 }
