@@ -83,6 +83,7 @@ struct yee_handle
   void *einc = nullptr, *hinc[2] = { nullptr, nullptr };
   int64_t inc_len = 0;
   Scalars sc {};
+  double src_t0 = 0, src_sigma = 0;   // Gaussian envelope of the point source (sigma <= 0: none)
   int32_t t = 0;
   bool prepared = false;
   double last_ms = 0;
@@ -219,15 +220,40 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
 double source_value (const yee_handle *h, bool e_half, int32_t t)
 {
   const double ts = e_half ? t + 0.5 : t + 1.0;
-  return std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+  double v = std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+  if (h->src_sigma > 0)
+  {
+    const double u = (ts - h->src_t0) / h->src_sigma;
+    v *= std::exp (-u * u);
+  }
+  return v;
+}
+
+// tfsf_near of the kernels (yee_kernels.hip): the cell lies in [L-1, R] on every axis and on
+// one of the planes L-1, R-1, R of some axis.
+bool tfsf_near_host (const yee_handle *h, const int64_t p[3])
+{
+  bool face = false;
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    const int64_t L = h->cfg.tfsf_left[ax], R = h->n[ax] - h->cfg.tfsf_right[ax];
+    if (p[ax] < L - 1 || p[ax] > R) return false;
+    face = face || p[ax] == L - 1 || p[ax] == R - 1 || p[ax] == R;
+  }
+  return face;
+}
+
+// Does the fused kernel update cell p (fused_owns of the kernels)?
+bool fused_owns_host (const yee_handle *h, const int64_t p[3])
+{
+  for (int ax = 0; ax < 3; ++ax)
+    if (p[ax] < h->fbox_lo[ax] || p[ax] >= h->fbox_hi[ax]) return false;
+  return !(h->use_tfsf () && tfsf_near_host (h, p));
 }
 
 bool source_in_fused_box (const yee_handle *h)
 {
-  const int64_t *p = h->cfg.point_source_pos;
-  for (int ax = 0; ax < 3; ++ax)
-    if (p[ax] < h->fbox_lo[ax] || p[ax] >= h->fbox_hi[ax]) return false;
-  return true;
+  return fused_owns_host (h, h->cfg.point_source_pos);
 }
 
 // Point sources as separate one-cell launches into buffer `buf`; with the fused step the
@@ -302,16 +328,45 @@ int slab_boxes (const yee_handle *h, int64_t lo[6][3], int64_t hi[6][3])
   return m;
 }
 
-// One time step (t > 0) with the fused kernel: E slabs, E sources outside the box, fused
-// E+H sweep of the box, H slabs, H sources outside the box; old state in buffer cur, new
-// state in 1 - cur.
+// The cells next to the TF/SF faces (tfsf_near): for each axis the planes L-1, R-1 and R,
+// spanning [L-1, R] on the other axes, clipped to the interior.  Boxes may overlap; a cell
+// updated twice from the old buffer gets the same value both times.
+int face_boxes (const yee_handle *h, int64_t lo[9][3], int64_t hi[9][3])
+{
+  if (!h->use_tfsf ()) return 0;
+  int m = 0;
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    const int64_t L = h->cfg.tfsf_left[ax], R = h->n[ax] - h->cfg.tfsf_right[ax];
+    const int64_t planes[3] = { L - 1, R - 1, R };
+    for (int64_t pl : planes)
+    {
+      bool ok = true;
+      for (int b = 0; b < 3; ++b)
+      {
+        const int64_t Lb = h->cfg.tfsf_left[b], Rb = h->n[b] - h->cfg.tfsf_right[b];
+        const int64_t l = b == ax ? pl : Lb - 1, u = b == ax ? pl + 1 : Rb + 1;
+        lo[m][b] = std::max<int64_t> (l, 1);
+        hi[m][b] = std::min<int64_t> (u, h->n[b] - 1);
+        ok = ok && hi[m][b] > lo[m][b];
+      }
+      if (ok) ++m;
+    }
+  }
+  return m;
+}
+
+// One time step (t > 0) with the fused kernel: E of the PML slabs and TF/SF faces, E sources
+// the fused kernel does not own, fused E+H sweep, H of the slabs and faces, H sources; old
+// state in buffer cur, new state in 1 - cur.
 template <typename T>
 int fused_step (yee_handle *h, int32_t t, int prev)
 {
   int rc;
   const int dst = 1 - h->cur;
-  int64_t lo[6][3], hi[6][3];
-  const int nslab = slab_boxes (h, lo, hi);
+  int64_t lo[15][3], hi[15][3];
+  int nslab = slab_boxes (h, lo, hi);
+  nslab += face_boxes (h, lo + nslab, hi + nslab);
   {
     ProfScope ps (h, 1);
     for (int s = 0; s < nslab; ++s)
@@ -812,6 +867,15 @@ int yee_profile_steps (yee_handle *h, int32_t n, double out[5])
   if (rc) return rc;
   for (int k = 0; k < 4; ++k) out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
   out[4] = h->last_ms;
+  return YEE_OK;
+}
+
+int yee_set_point_source_gaussian (yee_handle *h, double t0, double sigma)
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  if (!std::isfinite (t0) || !std::isfinite (sigma)) return fail (YEE_EINVAL, "t0 and sigma must be finite");
+  h->src_t0 = t0;
+  h->src_sigma = sigma;
   return YEE_OK;
 }
 

@@ -475,31 +475,24 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------------------
-// Fused 3D time step over the PML-free box: E of step t, then H of step t, in one sweep.
+// Fused 3D time step: E of step t, then H of step t, in one sweep of the cells it owns.
 //
-// The box [run_lo, run_hi) holds every cell whose update is the direct one (sigma = 0 on all
-// axes); the PML slabs around it are updated by e_half_3d / h_half_3d on slab boxes, E slabs
-// before this kernel and H slabs after it.  State is double-buffered: every old value comes
-// from f, every new one goes to f_n, and E outside the box is read back from f_n.
+// Owned cells: the PML-free box [run_lo, run_hi) minus the cells next to the TF/SF faces
+// (tfsf_near).  The other cells are updated by e_half_3d / h_half_3d on slab and face boxes,
+// the E updates before this kernel and the H updates after it.  State is double-buffered:
+// every old value comes from f, every new one goes to f_n, and the new E of cells this kernel
+// does not own is read back from f_n.
 //
 // Each workgroup owns a full z row (one lane per z cell, up to 16 waves) of a strip of BY
 // y rows and marches along x over a chunk of planes.  Per x plane it computes E one plane
 // ahead of H: E at plane x needs H before the step at x and x-1 (x-1 carried in registers);
-// H at plane x-1 needs E after the step at x-1 (carried) and x (just computed).  The y
-// neighbour row j0+BY is recomputed (Ex, Ez only; not stored), the z neighbour of a lane
-// comes from the next lane by a cross-lane shuffle and, for the last lane of a wave, from the
-// next wave through LDS.  Every state word crosses HBM once per step, except the recomputed
-// row and the first plane of each chunk.
+// H at plane x-1 needs E after the step at x-1 (carried) and x (just computed).  The loads of
+// plane x+1 are issued before plane x is computed, so their latency overlaps the arithmetic,
+// the stores and the barrier.  The y neighbour row j0+BY is recomputed (Ex, Ez only; not
+// stored); the z neighbour of a lane comes from the next lane by a cross-lane shuffle and,
+// for the last lane of a wave, from the next wave through LDS.  Every state word crosses HBM
+// once per step, except the recomputed row and the first plane of each chunk.
 // ---------------------------------------------------------------------------------------
-
-// Everything the E update of one x plane produces for the fused sweep.
-template <typename T, int BY>
-struct EPlane
-{
-  T ex[BY + 1], ey[BY + 1], ez[BY + 1];   // E after the step, rows 0..BY (Ey: rows 0..BY-1)
-  T hx[BY + 1], hy[BY + 1], hz[BY + 1];   // H before the step at this plane (Hx, Hz: rows 0..BY-1)
-  T ep1[BY + 1];                          // eps at the next plane, rows 0..BY
-};
 
 // Thread geometry of the fused sweep.
 struct FusedPos
@@ -515,6 +508,13 @@ __device__ __forceinline__ T direct_value (const StepArgs<T> &a, T f_old, T tmp,
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
   const T cb = a.dt / (mat * modc * a.dx);
   return f_old * (T) 1.0 + tmp * cb;
+}
+
+// Does the fused kernel update cell (x, j, k)?  (xbox, kbox: the x and z box tests.)
+template <typename T>
+__device__ __forceinline__ bool fused_owns (const StepArgs<T> &a, bool xbox, int x, int j, int k, bool kbox)
+{
+  return xbox && kbox && j >= a.run_lo[1] && j < a.run_hi[1] && !(a.tfsf && tfsf_near<T, 3> (a, x, j, k));
 }
 
 // Lane byte offsets of the rows j0-1 .. j0+BY relative to row j0-1 (kept opaque, so that all
@@ -539,117 +539,139 @@ __device__ __forceinline__ const T *plane_row (const StepArgs<T> &a, const T *ar
   return arr + (int64_t) x * a.sx + (int64_t) (g.j0 - 1) * a.sy;
 }
 
+// Everything the E update of plane x reads.
 template <typename T, int BY>
-__device__ __forceinline__ void e_plane (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
-                                         const T (&hy_prev)[BY + 1], const T (&hz_prev)[BY + 1],
-                                         const T (&ep0)[BY + 1], EPlane<T, BY> &o)
+struct PlaneLoads
+{
+  T hxr[BY + 2], hzr[BY + 2];          // H before the step, rows -1..BY
+  T hy[BY + 1], hym[BY + 1], hxm[BY + 1];   // Hy rows 0..BY; z-1 neighbours of Hy, Hx
+  T ex[BY + 1], ey[BY + 1], ez[BY + 1];     // owned cells: E before the step; others: E after it
+  T ep1[BY + 1], epk[BY + 1];               // eps at plane x+1 and at z+1
+  bool own[BY + 1];
+};
+
+template <typename T, int BY>
+__device__ __forceinline__ void e_load (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
+                                        PlaneLoads<T, BY> &L)
 {
   const int nx = a.nx;
-  const T *Ex = plane_row (a, a.f[0], x, g), *Ey = plane_row (a, a.f[1], x, g), *Ez = plane_row (a, a.f[2], x, g);
   const T *Hx = plane_row (a, a.f[3], x, g), *Hy = plane_row (a, a.f[4], x, g), *Hz = plane_row (a, a.f[5], x, g);
-  T hxr[BY + 2], hzr[BY + 2];              // rows -1..BY
-  T hym[BY + 1], hxm[BY + 1];              // z-1 neighbours
-  T exo[BY + 1], eyo[BY + 1], ezo[BY + 1];
-  T epk[BY + 1];                           // eps at z+1
 #pragma unroll
   for (int r = -1; r <= BY; ++r)
   {
-    hxr[r + 1] = rld (Hx, ro[r + 1]);
-    hzr[r + 1] = rld (Hz, ro[r + 1]);
+    L.hxr[r + 1] = rld (Hx, ro[r + 1]);
+    L.hzr[r + 1] = rld (Hz, ro[r + 1]);
   }
+  const bool xbox = x >= a.run_lo[0] && x < a.run_hi[0];
   const T *E0 = a.eps ? plane_row (a, a.eps, x, g) : nullptr;
   const T *E1 = a.eps ? plane_row (a, a.eps, min (x + 1, nx - 1), g) : nullptr;
 #pragma unroll
   for (int r = 0; r <= BY; ++r)
   {
     const uint32_t off = ro[r + 1];
-    o.hy[r] = rld (Hy, off);
-    hym[r] = rld (Hy, off, -1);
-    exo[r] = rld (Ex, off);
-    ezo[r] = rld (Ez, off);
-    if (r < BY)
+    L.hy[r] = rld (Hy, off);
+    L.hym[r] = rld (Hy, off, -1);
+    if (r < BY) L.hxm[r] = rld (Hx, off, -1);
+    const bool own = fused_owns (a, xbox, x, g.j0 + r, g.k, g.kbox);
+    L.own[r] = own;
+    const int sel = own ? 0 : 1;   // old E from f, or the new E another kernel wrote to f_n
+    const T *const *fe = sel ? (const T *const *) a.f_n : (const T *const *) a.f;
+    (void) fe;
+    if (own)
     {
-      hxm[r] = rld (Hx, off, -1);
-      eyo[r] = rld (Ey, off);
-    }
-    if (a.eps)
-    {
-      o.ep1[r] = rld (E1, off);
-      epk[r] = rld (E0, off, 1);
+      L.ex[r] = rld (plane_row (a, a.f[0], x, g), off);
+      L.ez[r] = rld (plane_row (a, a.f[2], x, g), off);
+      if (r < BY) L.ey[r] = rld (plane_row (a, a.f[1], x, g), off);
     }
     else
     {
-      o.ep1[r] = a.eps_u;
-      epk[r] = a.eps_u;
+      L.ex[r] = rld (plane_row (a, (const T *) a.f_n[0], x, g), off);
+      L.ez[r] = rld (plane_row (a, (const T *) a.f_n[2], x, g), off);
+      if (r < BY) L.ey[r] = rld (plane_row (a, (const T *) a.f_n[1], x, g), off);
+    }
+    if (a.eps)
+    {
+      L.ep1[r] = rld (E1, off);
+      L.epk[r] = rld (E0, off, 1);
+    }
+    else
+    {
+      L.ep1[r] = a.eps_u;
+      L.epk[r] = a.eps_u;
     }
   }
-  // ---- E of rows 0..BY (row BY: Ex and Ez only, for the H update of row BY-1); cells
-  // outside the box take the new value the slab kernel has already written
-  const bool xbox = x >= a.run_lo[0] && x < a.run_hi[0];
+}
+
+// Everything the E update of one x plane produces for the fused sweep.
+template <typename T, int BY>
+struct EPlane
+{
+  T ex[BY + 1], ey[BY + 1], ez[BY + 1];   // E after the step, rows 0..BY (Ey: rows 0..BY-1)
+  T hx[BY + 1], hy[BY + 1], hz[BY + 1];   // H before the step at this plane (Hx, Hz: rows 0..BY-1)
+  T ep1[BY + 1];                          // eps at the next plane, rows 0..BY
+  bool own[BY + 1];
+};
+
+template <typename T, int BY>
+__device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const FusedPos &g, const PlaneLoads<T, BY> &L,
+                                           const T (&hy_prev)[BY + 1], const T (&hz_prev)[BY + 1],
+                                           const T (&ep0)[BY + 1], EPlane<T, BY> &o)
+{
   const int64_t base = (int64_t) x * a.sx;
 #pragma unroll
   for (int r = 0; r <= BY; ++r)
   {
     const int j = g.j0 + r;
-    const bool in = xbox && j >= a.run_lo[1] && j < a.run_hi[1] && g.kbox;
-    T vx[4] = { hzr[r], hzr[r + 1], hym[r], o.hy[r] };
-    T vz[4] = { hy_prev[r], o.hy[r], hxr[r], hxr[r + 1] };
-    T vy[4] = { hxm[r], hxr[r + 1], hz_prev[r], hzr[r + 1] };
-    if (a.tfsf && tfsf_near<T, 3> (a, x, j, g.k))
-    {
-      tfsf_apply<T, 0, 3> (a, x, j, g.k, vx);
-      tfsf_apply<T, 2, 3> (a, x, j, g.k, vz);
-      if (r < BY) tfsf_apply<T, 1, 3> (a, x, j, g.k, vy);
-    }
+    const T vx[4] = { L.hzr[r], L.hzr[r + 1], L.hym[r], L.hy[r] };
+    const T vz[4] = { hy_prev[r], L.hy[r], L.hxr[r], L.hxr[r + 1] };
+    const T vy[4] = { L.hxm[r], L.hxr[r + 1], hz_prev[r], L.hzr[r + 1] };
     const T e = ep0[r];
-    const uint32_t off = ro[r + 1];
+    const bool own = L.own[r];
     const int64_t qt = base + (int64_t) j * a.sy + g.k;
-    T v;
-    if (in)
+    T v = L.ex[r];
+    if (own)
     {
-      v = direct_value<T, 0> (a, exo[r], curl (vx), avg2 (e, o.ep1[r]));
+      v = direct_value<T, 0> (a, v, curl (vx), avg2 (e, L.ep1[r]));
       if ((a.src_mask & 1u) && qt == a.src_q) v = a.src_e;
     }
-    else
-      v = rld (plane_row (a, (const T *) a.f_n[0], x, g), off);
     o.ex[r] = v;
-    if (in)
+    v = L.ez[r];
+    if (own)
     {
-      v = direct_value<T, 2> (a, ezo[r], curl (vz), avg2 (e, epk[r]));
+      v = direct_value<T, 2> (a, v, curl (vz), avg2 (e, L.epk[r]));
       if ((a.src_mask & 4u) && qt == a.src_q) v = a.src_e;
     }
-    else
-      v = rld (plane_row (a, (const T *) a.f_n[2], x, g), off);
     o.ez[r] = v;
     if (r < BY)
     {
-      if (in)
+      v = L.ey[r];
+      if (own)
       {
-        v = direct_value<T, 1> (a, eyo[r], curl (vy), avg2 (e, ep0[r + 1]));
+        v = direct_value<T, 1> (a, v, curl (vy), avg2 (e, ep0[r + 1]));
         if ((a.src_mask & 2u) && qt == a.src_q) v = a.src_e;
       }
-      else
-        v = rld (plane_row (a, (const T *) a.f_n[1], x, g), off);
       o.ey[r] = v;
-      o.hx[r] = hxr[r + 1];
-      o.hz[r] = hzr[r + 1];
+      o.hx[r] = L.hxr[r + 1];
+      o.hz[r] = L.hzr[r + 1];
     }
+    o.hy[r] = L.hy[r];
+    o.ep1[r] = L.ep1[r];
+    o.own[r] = own;
   }
 }
 
-// Stores the box rows of an E plane.
+// Stores the owned cells of rows 0..BY-1 of an E plane.
 template <typename T, int BY>
 __device__ __forceinline__ void e_store (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
                                          const EPlane<T, BY> &o)
 {
-  if (x < a.run_lo[0] || x >= a.run_hi[0] || !g.kbox) return;
   T *Ex = (T *) plane_row (a, (const T *) a.f_n[0], x, g);
   T *Ey = (T *) plane_row (a, (const T *) a.f_n[1], x, g);
   T *Ez = (T *) plane_row (a, (const T *) a.f_n[2], x, g);
 #pragma unroll
   for (int r = 0; r < BY; ++r)
   {
-    if (g.j0 + r >= a.run_hi[1]) continue;
+    if (!o.own[r]) continue;
     rst (Ex, ro[r + 1], o.ex[r]);
     rst (Ey, ro[r + 1], o.ey[r]);
     rst (Ez, ro[r + 1], o.ez[r]);
@@ -678,17 +700,17 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
   const int xb = min (xa + a.lx, a.run_hi[0]);   // H (and stored E) planes [xa, xb)
   g.kc = min (g.k, a.nz - 1);
   g.kbox = g.k >= a.run_lo[2] && g.k < a.run_hi[2];
-  const int64_t sx = a.sx;
-  const int sy = a.sy;
+
+  uint32_t ro[BY + 2];
+  row_offsets<T, BY> (a, g, ro);
 
   // carried state: E after the step and H before the step at plane i, eps at plane i+1
   T exc[BY + 1], eyc[BY + 1], ezc[BY + 1];
   T hxc[BY + 1], hyc[BY + 1], hzc[BY + 1];
   T ep0[BY + 1];
+  PlaneLoads<T, BY> nxt;
   {
     // prologue: E at plane xa (H before the step at xa-1 and eps at xa are loaded here)
-    uint32_t ro[BY + 2];
-    row_offsets<T, BY> (a, g, ro);
     T hyp[BY + 1], hzp[BY + 1], e0[BY + 1];
     const T *Hy = plane_row (a, a.f[4], xa - 1, g), *Hz = plane_row (a, a.f[5], xa - 1, g);
     const T *E0 = a.eps ? plane_row (a, a.eps, xa, g) : nullptr;
@@ -699,9 +721,12 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
       hzp[r] = rld (Hz, ro[r + 1]);
       e0[r] = a.eps ? rld (E0, ro[r + 1]) : a.eps_u;
     }
+    PlaneLoads<T, BY> L;
+    e_load<T, BY> (a, xa, g, ro, L);
     EPlane<T, BY> o;
-    e_plane<T, BY> (a, xa, g, ro, hyp, hzp, e0, o);
+    e_compute<T, BY> (a, xa, g, L, hyp, hzp, e0, o);
     e_store<T, BY> (a, xa, g, ro, o);
+    e_load<T, BY> (a, min (xa + 1, xb), g, ro, nxt);
 #pragma unroll
     for (int r = 0; r <= BY; ++r)
     {
@@ -729,10 +754,10 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
   for (int x = xa + 1; x <= xb; ++x)
   {
     const int i = x - 1;   // H plane
-    uint32_t ro[BY + 2];
-    row_offsets<T, BY> (a, g, ro);
+    const PlaneLoads<T, BY> L = nxt;
+    e_load<T, BY> (a, min (x + 1, xb), g, ro, nxt);   // prefetch the next plane
     EPlane<T, BY> o;
-    e_plane<T, BY> (a, x, g, ro, hyc, hzc, ep0, o);
+    e_compute<T, BY> (a, x, g, L, hyc, hzc, ep0, o);
     par ^= 1;
     if (lane == 0)
     {
@@ -761,9 +786,12 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     T mxm[BY + 1], mym[BY + 1], mzm[BY + 1];
     const T *M0 = a.mu ? plane_row (a, a.mu, i, g) : nullptr;
     const T *M1 = a.mu ? plane_row (a, a.mu, i + 1, g) : nullptr;   // i + 1 <= nx - 2
+    const bool ibox = i >= a.run_lo[0] && i < a.run_hi[0];
+    bool hown[BY + 1];
 #pragma unroll
     for (int r = 0; r < BY; ++r)
     {
+      hown[r] = fused_owns (a, ibox, i, g.j0 + r, g.k, g.kbox);
       if (a.mu)
       {
         // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
@@ -781,16 +809,10 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     for (int r = 0; r < BY; ++r)
     {
       const int j = g.j0 + r;
-      T vx[4] = { eyc[r], eyk[r], ezc[r], ezc[r + 1] };
-      T vy[4] = { ezc[r], o.ez[r], exc[r], exk[r] };
-      T vz[4] = { exc[r], exc[r + 1], eyc[r], o.ey[r] };
-      if (a.tfsf && tfsf_near<T, 3> (a, i, j, g.k))
-      {
-        tfsf_apply<T, 3, 3> (a, i, j, g.k, vx);
-        tfsf_apply<T, 4, 3> (a, i, j, g.k, vy);
-        tfsf_apply<T, 5, 3> (a, i, j, g.k, vz);
-      }
-      const int64_t qt = (int64_t) i * sx + (int64_t) j * sy + g.k;
+      const T vx[4] = { eyc[r], eyk[r], ezc[r], ezc[r + 1] };
+      const T vy[4] = { ezc[r], o.ez[r], exc[r], exk[r] };
+      const T vz[4] = { exc[r], exc[r + 1], eyc[r], o.ey[r] };
+      const int64_t qt = (int64_t) i * a.sx + (int64_t) j * a.sy + g.k;
       T v = direct_value<T, 3> (a, hxc[r], curl (vx), mxm[r]);
       if ((a.src_mask & 8u) && qt == a.src_q) v = a.src_h;
       hn[0][r] = v;
@@ -801,9 +823,8 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
       if ((a.src_mask & 32u) && qt == a.src_q) v = a.src_h;
       hn[2][r] = v;
     }
-    // ---- stores: E of plane x when this chunk owns it, H of plane i (box cells only)
+    // ---- stores: E of plane x when this chunk owns it, H of plane i (owned cells only)
     if (x < xb) e_store<T, BY> (a, x, g, ro, o);
-    if (g.kbox)
     {
       T *Hx = (T *) plane_row (a, (const T *) a.f_n[3], i, g);
       T *Hy = (T *) plane_row (a, (const T *) a.f_n[4], i, g);
@@ -811,7 +832,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
 #pragma unroll
       for (int r = 0; r < BY; ++r)
       {
-        if (g.j0 + r >= a.run_hi[1]) continue;
+        if (!hown[r]) continue;
         rst (Hx, ro[r + 1], hn[0][r]);
         rst (Hy, ro[r + 1], hn[1][r]);
         rst (Hz, ro[r + 1], hn[2][r]);
@@ -837,11 +858,8 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
 // TMz half-steps (Ez; Hx, Hy) on [1, N-1)^2; one thread per cell, y lane-contiguous
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
+__device__ __forceinline__ void tmz_e_cell (const StepArgs<T> &a, int i, int j)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;   // == 1
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
@@ -866,11 +884,8 @@ __global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
+__device__ __forceinline__ void tmz_h_cell (const StepArgs<T> &a, int i, int j)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
@@ -917,6 +932,92 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
   }
   update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), mx, special, omx, gmx);
   update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), my, special, omy, gmy);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
+  tmz_e_cell<T, MODE> (a, i, j);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
+  tmz_h_cell<T, MODE> (a, i, j);
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched TMz half-steps: many independent simulations of different sizes in one launch.
+//
+// Persistent workgroups walk a tile table (a static round-robin queue): tile = (simulation,
+// x0, y0) covers TX x rows and 64 y cells of that simulation's interior.  The per-simulation
+// set-up (grid pointers, sizes, PML tables, coefficients) is a StepArgs record in device
+// memory, read with scalar loads (uniform per tile).  The time step comes from device
+// memory, so a captured graph of several steps can be replayed; hard point sources are
+// applied by the thread that updates the source cell, from a per-simulation table of values.
+// ---------------------------------------------------------------------------------------
+template <typename T, int TX>
+__global__ __launch_bounds__ (64 * TX) void batch_tmz_half (const StepArgs<T> *sims, const int4 *tiles, int ntiles,
+                                                          int e_half, const int *step, const T *src_tab, int src_len)
+{
+  const int t = *step;
+  for (int tile = (int) blockIdx.x; tile < ntiles; tile += (int) gridDim.x)
+  {
+    const int4 td = tiles[tile];   // sim, x0, y0, mode
+    const StepArgs<T> &a = sims[td.x];
+    const int i = td.y + (int) threadIdx.y;
+    const int j = td.z + (int) threadIdx.x;
+    if (i >= a.nx - 1 || j >= a.ny - 1) continue;
+    if (e_half)
+    {
+      if (td.w) tmz_e_cell<T, 1> (a, i, j);
+      else tmz_e_cell<T, 0> (a, i, j);
+    }
+    else
+    {
+      if (td.w) tmz_h_cell<T, 1> (a, i, j);
+      else tmz_h_cell<T, 0> (a, i, j);
+    }
+    if (a.src_mask)
+    {
+      const int64_t q = (int64_t) i * a.sx + j;
+      if (q == a.src_q)
+      {
+        const T v = src_tab[((int64_t) td.x * src_len + t) * 2 + (e_half ? 0 : 1)];
+        if (e_half && (a.src_mask & 4u)) gst (a.f_n[2] + q, v);
+        if (!e_half && (a.src_mask & 8u)) gst (a.f_n[3] + q, v);
+        if (!e_half && (a.src_mask & 16u)) gst (a.f_n[4] + q, v);
+      }
+    }
+  }
+}
+
+// Step 0 performs no field update (Scheme.inc.h:1032) but the hard sources still fire.
+template <typename T>
+__global__ void batch_tmz_sources (const StepArgs<T> *sims, int nsim, const int *step, const T *src_tab, int src_len)
+{
+  const int s = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= nsim) return;
+  const int t = *step;
+  const StepArgs<T> &a = sims[s];
+  if (!a.src_mask) return;
+  const T ve = src_tab[((int64_t) s * src_len + t) * 2 + 0];
+  const T vh = src_tab[((int64_t) s * src_len + t) * 2 + 1];
+  if (a.src_mask & 4u) gst (a.f_n[2] + a.src_q, ve);
+  if (a.src_mask & 8u) gst (a.f_n[3] + a.src_q, vh);
+  if (a.src_mask & 16u) gst (a.f_n[4] + a.src_q, vh);
+}
+
+// Advances the device-side step counter (last node of every captured step).
+__global__ void batch_advance (int *step)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1006,6 +1107,11 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void h_half_tmz<T, 0> (StepArgs<T>);                                 \
   template __global__ void h_half_tmz<T, 1> (StepArgs<T>);                                 \
   template __global__ void h_half_tmz<T, 2> (StepArgs<T>);                                 \
+  template __global__ void batch_tmz_half<T, 1> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
+  template __global__ void batch_tmz_half<T, 2> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
+  template __global__ void batch_tmz_half<T, 4> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
+  template __global__ void batch_tmz_half<T, 8> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
+  template __global__ void batch_tmz_sources<T> (const StepArgs<T> *, int, const int *, const T *, int); \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
   template __global__ void step_3d_fused<T, FUSED_BY (T)> (StepArgs<T>);                   \
   template __global__ void point_source<T> (T *, int64_t, double);                         \

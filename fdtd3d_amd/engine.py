@@ -19,7 +19,7 @@ from typing import Dict, Optional
 
 import numpy as np
 
-from .config import Config, COMP_INDEX, MAT_INDEX
+from .config import Config, COMP_INDEX, MAT_INDEX, YeeConfigC
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libyee.so")
@@ -28,7 +28,10 @@ EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
-    "yee_set_material_f32", "yee_set_material_sphere",
+    "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
+    "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
+    "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
+    "yee_batch_sync", "yee_batch_time", "yee_batch_last_step_ms", "yee_batch_tuning", "yee_batch_get_field",
 ]
 
 
@@ -69,10 +72,25 @@ def load_library():
     L.yee_last_step_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.yee_profile_steps.argtypes = [vp, i32, vp]
     L.yee_set_material_f32.argtypes = [vp, i32, vp]
+    L.yee_set_point_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.yee_set_material_sphere.argtypes = [vp, i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
                                           ctypes.c_double, i32, i32]
+    L.yee_batch_last_error.restype = ctypes.c_char_p
+    L.yee_batch_create.argtypes = [vp, i32, ctypes.POINTER(vp)]
+    L.yee_batch_destroy.argtypes = [vp]
+    L.yee_batch_destroy.restype = None
+    L.yee_batch_set_material.argtypes = [vp, i32, i32, vp]
+    L.yee_batch_set_material_f32.argtypes = [vp, i32, i32, vp]
+    L.yee_batch_set_point_source_gaussian.argtypes = [vp, i32, ctypes.c_double, ctypes.c_double]
+    L.yee_batch_prepare.argtypes = [vp]
+    L.yee_batch_step.argtypes = [vp, i32]
+    L.yee_batch_sync.argtypes = [vp]
+    L.yee_batch_time.argtypes = [vp, ctypes.POINTER(i32)]
+    L.yee_batch_last_step_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.yee_batch_tuning.argtypes = [vp, vp, vp, i32]
+    L.yee_batch_get_field.argtypes = [vp, i32, i32, vp]
     for name in EXPORTED:
-        if name not in ("yee_last_error", "yee_destroy"):
+        if name not in ("yee_last_error", "yee_destroy", "yee_batch_last_error", "yee_batch_destroy"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -136,6 +154,10 @@ class Engine:
         _check(load_library().yee_set_material_sphere(self.h, MAT_INDEX[name], float(value), float(outside), c,
                                                       float(radius), int(accuracy), 1 if stair else 0))
 
+    def set_point_source_gaussian(self, t0: float, sigma: float) -> None:
+        """Gaussian envelope exp(-((ts - t0) / sigma)^2) on the hard point source (steps)."""
+        _check(load_library().yee_set_point_source_gaussian(self.h, float(t0), float(sigma)))
+
     def prepare(self) -> None:
         _check(load_library().yee_prepare(self.h))
 
@@ -193,6 +215,81 @@ class Engine:
     def scalars(self) -> np.ndarray:
         out = np.empty(3, dtype=np.float64)
         _check(load_library().yee_get_scalars(self.h, out.ctypes.data))
+        return out
+
+
+def _bcheck(rc: int) -> None:
+    if rc != 0:
+        raise YeeError(rc, load_library().yee_batch_last_error().decode())
+
+
+class Batch:
+    """Batched TMz sweep (yee_batch): independent TMz simulations advanced together."""
+
+    def __init__(self, cfgs):
+        L = load_library()
+        self.cfgs = list(cfgs)
+        arr = (YeeConfigC * len(self.cfgs))(*[c.to_c() for c in self.cfgs])
+        self._arr = arr
+        h = ctypes.c_void_p()
+        _bcheck(L.yee_batch_create(arr, len(self.cfgs), ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load_library().yee_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_material(self, sim: int, name: str, values: np.ndarray) -> None:
+        c = self.cfgs[sim]
+        if int(np.size(values)) != int(np.prod(c.shape)):
+            raise ValueError(f"material {name}: expected {c.shape}, got {np.shape(values)}")
+        if isinstance(values, np.ndarray) and values.dtype == np.float32:
+            arr = np.ascontiguousarray(values)
+            _bcheck(load_library().yee_batch_set_material_f32(self.h, sim, MAT_INDEX[name], arr.ctypes.data))
+            return
+        arr = np.ascontiguousarray(values, dtype=np.float64)
+        _bcheck(load_library().yee_batch_set_material(self.h, sim, MAT_INDEX[name], arr.ctypes.data))
+
+    def set_point_source_gaussian(self, sim: int, t0: float, sigma: float) -> None:
+        _bcheck(load_library().yee_batch_set_point_source_gaussian(self.h, sim, float(t0), float(sigma)))
+
+    def prepare(self) -> None:
+        _bcheck(load_library().yee_batch_prepare(self.h))
+
+    def step(self, n: int) -> None:
+        _bcheck(load_library().yee_batch_step(self.h, int(n)))
+
+    def sync(self) -> None:
+        _bcheck(load_library().yee_batch_sync(self.h))
+
+    @property
+    def time(self) -> int:
+        t = ctypes.c_int32()
+        _bcheck(load_library().yee_batch_time(self.h, ctypes.byref(t)))
+        return t.value
+
+    def last_step_ms(self) -> float:
+        ms = ctypes.c_double()
+        _bcheck(load_library().yee_batch_last_step_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def tuning(self, ncand: int = 8):
+        out = np.zeros(2, dtype=np.int32)
+        ms = np.zeros(ncand, dtype=np.float64)
+        _bcheck(load_library().yee_batch_tuning(self.h, out.ctypes.data, ms.ctypes.data, ncand))
+        return {"tile_rows": int(out[0]), "workgroups": int(out[1]), "candidate_ms": [float(v) for v in ms]}
+
+    def field(self, sim: int, name: str) -> np.ndarray:
+        c = self.cfgs[sim]
+        out = np.empty(c.shape, dtype=np.float64)
+        _bcheck(load_library().yee_batch_get_field(self.h, sim, COMP_INDEX[name], out.ctypes.data))
         return out
 
 

@@ -151,6 +151,16 @@ int yee_prepare (yee_handle *h);
  */
 int yee_step (yee_handle *h, int32_t n);
 
+/*
+ * Gaussian-modulated hard point source: the value the reference's performPointSourceCalc
+ * assigns (Source/Scheme/InternalScheme.inc.h:2818-2895, sin(2 pi f dt ts) with ts = t + 1/2
+ * for E and t + 1 for H) is multiplied by exp(-((ts - t0) / sigma)^2), t0 and sigma in time
+ * steps.  No counterpart in the reference (it only has the plain sinusoid, sigma <= 0 here);
+ * it is the source of the batched TMz sweep.  May be called at any time; applies from the
+ * next step on.
+ */
+int yee_set_point_source_gaussian (yee_handle *h, double t0, double sigma);
+
 /* Waits for all enqueued work of the handle. */
 int yee_sync (yee_handle *h);
 
@@ -199,6 +209,38 @@ int yee_last_step_ms (yee_handle *h, double *ms);
  * Benchmark hook; the extra events cost a few microseconds per step.
  */
 int yee_profile_steps (yee_handle *h, int32_t n, double out[5]);
+
+/*
+ * Batched TMz sweep: `nsim` independent TMz simulations (each its own yee_config: size, PML,
+ * point source; one precision for the batch; no TF/SF, no metamaterials) advanced together,
+ * two launches per time step for the whole batch, replayed from a captured HIP graph.  The
+ * reference runs one simulation per process (Source/main.cpp); the batch has no counterpart
+ * there and replaces running the same simulations one by one.  Every member computes exactly
+ * what a yee_handle with its configuration computes.  Errors: yee_batch_last_error().
+ */
+typedef struct yee_batch yee_batch;
+
+const char *yee_batch_last_error (void);
+int yee_batch_create (const yee_config *cfgs, int32_t nsim, yee_batch **out);
+void yee_batch_destroy (yee_batch *b);
+/* eps (YEE_MAT_EPS) or mu (YEE_MAT_MU) of simulation `sim`, [x][y] */
+int yee_batch_set_material (yee_batch *b, int32_t sim, int32_t which, const double *values);
+int yee_batch_set_material_f32 (yee_batch *b, int32_t sim, int32_t which, const float *values);
+/* Gaussian envelope of simulation sim's point source (see yee_set_point_source_gaussian) */
+int yee_batch_set_point_source_gaussian (yee_batch *b, int32_t sim, double t0, double sigma);
+int yee_batch_prepare (yee_batch *b);
+/* Advances every simulation by n steps; the first call with enough steps also times the
+ * tile-shape candidates on its first steps and keeps the fastest (out: yee_batch_tuning). */
+int yee_batch_step (yee_batch *b, int32_t n);
+int yee_batch_sync (yee_batch *b);
+int yee_batch_time (yee_batch *b, int32_t *t);
+/* average device ms per step of the last yee_batch_step call (HIP events on the batch stream) */
+int yee_batch_last_step_ms (yee_batch *b, double *ms);
+/* out[0] = chosen tile height (x rows per 64-cell tile), out[1] = persistent workgroups;
+ * cand_ms[0..ncand) = ms per step the tuner measured for each candidate */
+int yee_batch_tuning (yee_batch *b, int32_t out[2], double *cand_ms, int32_t ncand);
+/* newest layer of Ez, Hx, Hy (or Dz, Bx, By) of simulation sim as FP64 [x][y] */
+int yee_batch_get_field (yee_batch *b, int32_t sim, int32_t comp, double *out);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,7 @@ def lib():
         L.oracle_incident_len.restype = ctypes.c_int64
         L.oracle_incident_len.argtypes = [vp]
         L.oracle_set_source_phase.argtypes = [vp, ctypes.c_double]
+        L.oracle_set_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         d = ctypes.c_double
         L.oracle_sphere_accurate_3d.restype = d
         L.oracle_sphere_accurate_3d.argtypes = [d] * 9 + [ctypes.c_int]
@@ -103,6 +104,9 @@ class Oracle:
 
     def set_source_phase(self, phase: float) -> None:
         lib().oracle_set_source_phase(self.h, float(phase))
+
+    def set_source_gaussian(self, t0: float, sigma: float) -> None:
+        lib().oracle_set_source_gaussian(self.h, float(t0), float(sigma))
 
     def set_material(self, name: str, values: np.ndarray) -> None:
         arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.shape)

@@ -78,6 +78,7 @@ struct oracle
   int32_t t;
   int prepared;
   double src_phase;        /* test hook: added to every source argument (0 = reference) */
+  double src_t0, src_sigma; /* Gaussian envelope of the point source in time steps (sigma <= 0: none) */
 };
 
 static int64_t o_idx (const oracle *o, int64_t i, int64_t j, int64_t k)
@@ -833,7 +834,10 @@ static void o_field_step (oracle *o, int comp, int32_t t)
     if (p[0] >= 0 && p[0] < o->n[0] && p[1] >= 0 && p[1] < o->n[1] && p[2] >= 0 && p[2] < o->n[2])
     {
       double ts = c->is_h ? t + 1.0 : t + 0.5;
-      o->f[comp][0][o_idx (o, p[0], p[1], p[2])] = sin (o->dt * ts * 2 * O_PI * o->freq + o->src_phase);
+      double v = sin (o->dt * ts * 2 * O_PI * o->freq + o->src_phase);
+      /* Gaussian envelope (no reference counterpart; oracle_set_source_gaussian) */
+      if (o->src_sigma > 0) v *= exp (-((ts - o->src_t0) / o->src_sigma) * ((ts - o->src_t0) / o->src_sigma));
+      o->f[comp][0][o_idx (o, p[0], p[1], p[2])] = v;
     }
   }
 }
@@ -938,6 +942,17 @@ int oracle_set_source_phase (oracle *o, double phase)
 {
   if (!o) return -1;
   o->src_phase = phase;
+  return 0;
+}
+
+/* Gaussian-modulated hard point source: the reference's sine times exp(-((ts - t0) / sigma)^2),
+ * ts = t + 1/2 (E) or t + 1 (H) in time steps.  No reference counterpart (the batched TMz
+ * sweep's source); pinned by the closed form in tests/test_oracle_known_answers.py. */
+int oracle_set_source_gaussian (oracle *o, double t0, double sigma)
+{
+  if (!o) return -1;
+  o->src_t0 = t0;
+  o->src_sigma = sigma;
   return 0;
 }
 

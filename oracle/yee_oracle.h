@@ -41,6 +41,7 @@ int oracle_get_scalars (oracle *o, double out[3]);
 int64_t oracle_incident_len (oracle *o);
 /* Test hook: phase added to every source argument (pi/2 gives the cos-source run). */
 int oracle_set_source_phase (oracle *o, double phase);
+int oracle_set_source_gaussian (oracle *o, double t0, double sigma);
 
 /* Material generators, restated from Approximation.cpp (used to build parity inputs). */
 double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,

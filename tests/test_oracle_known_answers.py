@@ -114,3 +114,24 @@ def test_tmz_point_source_symmetry():
     np.testing.assert_allclose(ez, ez.T, rtol=0, atol=1e-12 * np.abs(ez).max())
     # x <-> y exchange maps Hx onto -Hy (curl orientation)
     np.testing.assert_allclose(hx, -hy.T, rtol=0, atol=1e-12 * np.abs(hx).max())
+
+
+@pytest.mark.parametrize("sigma", [0.0, 30.0])
+@pytest.mark.parametrize("steps", [1, 37, 120])
+def test_point_source_closed_form(sigma, steps):
+    """A hard source assigns its value after the update of its component, so after the last
+    step the source cell holds exactly sin(2 pi f dt ts) * exp(-((ts - t0)/sigma)^2) with
+    ts = T - 1/2 (E) or T (H); sigma = 0 is the reference's plain sinusoid."""
+    t0 = 60.0
+    cfg = parse_args(f"--2d-tmz --size x:30,y:28 --use-pml --pml-size x:4,y:4 --point-source x:11,y:17 "
+                     f"--point-source-ez --point-source-hy --time-steps {steps}".split())
+    o = Oracle(cfg)
+    o.set_source_gaussian(t0, sigma)
+    o.prepare()
+    o.step(steps)
+    dt, f, _ = o.scalars()
+    for name, ts in (("Ez", steps - 0.5), ("Hy", float(steps))):
+        want = math.sin(dt * ts * 2 * math.pi * f)
+        if sigma > 0:
+            want *= math.exp(-((ts - t0) / sigma) ** 2)
+        assert o.field(name)[11, 17] == want, name
