@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -64,6 +65,74 @@ WORKLOADS = {
     "cfg2-plain": dict(desc="3D 1024^3 FP32 without PML and TF/SF (stencil only), eps as cfg2",
                        make=lambda: _plain(cfg2())),
 }
+
+
+def cfg4_batch(nsim=64, seed=0xB47C4, steps=2000, precision=4):
+    """64 TMz simulations: edges log-uniform in [256, 2048], PML 12, Gaussian-modulated hard Ez
+    source at a random interior cell, eps = 1 + 8 * sum of Gaussian blobs clamped to [1, 9]."""
+    rng = np.random.default_rng(seed)
+    sims = []
+    for s in range(nsim):
+        nx, ny = (int(round(math.exp(rng.uniform(math.log(256), math.log(2048))))) for _ in range(2))
+        px, py = int(rng.integers(12 + 1, nx - 12 - 1)), int(rng.integers(12 + 1, ny - 12 - 1))
+        c = Config(scheme=SCHEME_TMZ, precision=precision, time_steps=steps, size=[nx, ny, 1], pml=[12, 12, 0],
+                   use_pml=True, point_source=["Ez"], point_source_pos=[px, py, 0])
+        blobs = [(rng.uniform(0, nx), rng.uniform(0, ny), rng.uniform(8, 40)) for _ in range(int(rng.integers(4, 10)))]
+        sims.append((c, blobs))
+    return sims
+
+
+def blob_eps(shape, blobs, dtype=np.float32):
+    x = np.arange(shape[0], dtype=np.float64)[:, None]
+    y = np.arange(shape[1], dtype=np.float64)[None, :]
+    eps = np.ones(shape)
+    for cx, cy, r in blobs:
+        eps += 8 * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * r * r))
+    return np.clip(eps, 1, 9).astype(dtype)
+
+
+# source envelope of the batched sweep: one period of the source is
+# lambda / (c dt) = (cells per wavelength) / Courant = 40 / 0.5 = 80 steps
+GAUSS_SIGMA = 80.0
+GAUSS_T0 = 4 * GAUSS_SIGMA
+
+
+def run_cfg4(args) -> dict:
+    """Batched sweep vs the same simulations run one by one (both on the GPU, same kernels)."""
+    from fdtd3d_amd.engine import Batch, Engine
+    sims = cfg4_batch(steps=args.batch_steps)
+    cfgs = [c for c, _ in sims]
+    cells = sum(c.size[0] * c.size[1] for c in cfgs)
+    upd = cells * (args.batch_steps - 1)
+    print(f"[bench] cfg4: {len(cfgs)} TMz simulations, {cells} cells, {args.batch_steps} steps", file=sys.stderr, flush=True)
+    b = Batch(cfgs)
+    for s, (c, blobs) in enumerate(sims):
+        b.set_material(s, "eps", blob_eps(c.shape, blobs))
+        b.set_point_source_gaussian(s, GAUSS_T0, GAUSS_SIGMA)
+    b.prepare()
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(args.batch_steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    tun = b.tuning()
+    print(f"[bench] batched: {t_batch:.3f} s ({b.last_step_ms():.4f} ms/step); tuning {tun}", file=sys.stderr, flush=True)
+    b.close()
+    t_seq = 0.0
+    for s, (c, blobs) in enumerate(sims):
+        e = Engine(c)
+        e.set_material("eps", blob_eps(c.shape, blobs))
+        e.set_point_source_gaussian(GAUSS_T0, GAUSS_SIGMA)
+        e.prepare()
+        e.sync()
+        t0 = time.perf_counter()
+        e.step(args.batch_steps)
+        e.sync()
+        t_seq += time.perf_counter() - t0
+        e.close()
+    print(f"[bench] sequential: {t_seq:.3f} s", file=sys.stderr, flush=True)
+    return {"cells": cells, "updates": upd, "t_batch": t_batch, "t_seq": t_seq, "tuning": tun,
+            "ms_per_step": t_batch * 1e3 / args.batch_steps}
 
 
 def _plain(c: Config) -> Config:
@@ -137,8 +206,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4-batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2000, help="time steps of the cfg4-batch workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the run, copy a buffer of this many MiB device-to-device (a plain copy "
                          "of known size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -152,6 +222,23 @@ def main():
 
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+
+    if args.workload == "cfg4-batch":
+        r = run_cfg4(args)
+        line = {
+            "metric": "batched TMz sweep: Mcell-updates/s and speed-up over the same simulations run one by one",
+            "value": round(r["updates"] / r["t_batch"] / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
+            "steps": args.batch_steps, "warmup": 0, "ms_per_step": round(r["ms_per_step"], 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "cfg4-batch", "simulations": 64, "cells": r["cells"],
+                       "description": "64 TMz sims, edges log-uniform [256, 2048] seed 0xB47C4, PML 12, "
+                                      "Gaussian-modulated Ez source (t0 = 320, sigma = 80 steps), eps blobs in [1, 9]"},
+            "sequential_s": round(r["t_seq"], 3), "batched_s": round(r["t_batch"], 3),
+            "speedup_vs_sequential": round(r["t_seq"] / r["t_batch"], 2), "tuning": r["tuning"],
+        }
+        print(json.dumps(line), flush=True)
+        return
 
     wl = WORKLOADS[args.workload]
     cfg = wl["make"]()

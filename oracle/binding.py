@@ -46,6 +46,8 @@ def lib():
         L.oracle_incident_len.argtypes = [vp]
         L.oracle_set_source_phase.argtypes = [vp, ctypes.c_double]
         L.oracle_set_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.oracle_adjoint.argtypes = [vp, ctypes.POINTER(OracleAdjSpec), ctypes.c_int32,
+                                     ctypes.POINTER(ctypes.c_double), vp]
         d = ctypes.c_double
         L.oracle_sphere_accurate_3d.restype = d
         L.oracle_sphere_accurate_3d.argtypes = [d] * 9 + [ctypes.c_int]
@@ -59,6 +61,49 @@ def lib():
         L.oracle_wave_number_general.argtypes = [d] * 6
         _lib = L
     return _lib
+
+
+class OracleAdjSpec(ctypes.Structure):
+    _fields_ = [("dlo", ctypes.c_int64 * 3), ("dhi", ctypes.c_int64 * 3), ("mz", ctypes.c_int64),
+                ("mlo", ctypes.c_int64 * 2), ("mhi", ctypes.c_int64 * 2), ("window", ctypes.c_int32)]
+
+
+def oracle_adjoint(cfg: Config, eps: np.ndarray, design_lo, design_hi, monitor_z: int, monitor_lo, monitor_hi,
+                   window: int, steps: int):
+    """FP64 CPU adjoint of the reference-form update: (J, dJ/deps over the design box)."""
+    o = Oracle(cfg)
+    o.set_material("eps", eps)
+    o.prepare()
+    sp = OracleAdjSpec()
+    for a in range(3):
+        sp.dlo[a], sp.dhi[a] = int(design_lo[a]), int(design_hi[a])
+    sp.mz = int(monitor_z)
+    for a in range(2):
+        sp.mlo[a], sp.mhi[a] = int(monitor_lo[a]), int(monitor_hi[a])
+    sp.window = int(window)
+    shape = tuple(int(design_hi[a]) - int(design_lo[a]) for a in range(3))
+    grad = np.zeros(shape, dtype=np.float64)
+    J = ctypes.c_double()
+    rc = lib().oracle_adjoint(o.h, ctypes.byref(sp), int(steps), ctypes.byref(J), grad.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_adjoint failed ({rc})")
+    o.close()
+    return J.value, grad
+
+
+def oracle_objective(cfg: Config, eps: np.ndarray, monitor_z: int, monitor_lo, monitor_hi, window: int, steps: int):
+    """J of the forward oracle alone (for finite differences)."""
+    o = Oracle(cfg)
+    o.set_material("eps", eps)
+    o.prepare()
+    J = 0.0
+    for t in range(steps):
+        o.step(1)
+        if t >= steps - window:
+            ez = o.field("Ez")[monitor_lo[0]:monitor_hi[0], monitor_lo[1]:monitor_hi[1], monitor_z]
+            J += float(np.sum(ez * ez))
+    o.close()
+    return J
 
 
 def sphere_map(cfg: Config, value: float, outside: float, center, radius: int) -> np.ndarray:

@@ -969,3 +969,190 @@ int oracle_get_scalars (oracle *o, double out[3])
   out[2] = o->rel;
   return 0;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * FP64 CPU adjoint (dJ/d eps over a design box) of the reference-form update above.
+ *
+ * No reference counterpart: the reference has no differentiation.  The forward map of one
+ * step is linear in the state (E, D, H, B) plus field-independent TF/SF and source terms, so
+ * the adjoint step is its transpose, applied in reverse (H first, then E):
+ *   H:  mB = aB + kp aH;  w = cb_curl mB;  aB <- cac mB - km aH;  aH <- ca5 aH;
+ *       every E operand of the H curl receives +-w (the transpose of the curl gather);
+ *   E:  the same with (E, D) and the H operands.
+ * Without PML the cell update is V' = V + cb tmp (w = cb aV).  The update of an E cell
+ * depends on eps only through the averaged material m = (eps(v1) + eps(v2)) / 2 inside kp, km
+ * (or cb), each proportional to 1/m, so dE'/dm = -(E' - ca5 E) / m, and each of the two
+ * eps voxels receives half of it.
+ * A hard point source replaces the new value of its cell, so no adjoint flows back from it.
+ * J = sum over the last `window` steps of sum over the monitor rectangle of Ez^2.
+ * Pinned by central finite differences of the forward oracle (tests).
+ * ------------------------------------------------------------------------------------------ */
+
+typedef struct
+{
+  double cac, cbc, ca5, kp, km, cb;   /* two-stage (PML) or direct (cb) coefficients */
+  double m;                           /* averaged material */
+} o_coef;
+
+static void o_cell_coef (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, o_coef *r)
+{
+  const comp_desc *c = &CD[comp];
+  const double eps0 = o_eps0 ();
+  const double modifier = c->is_h ? o_mu0 () : eps0;
+  const double *mgrid = c->is_h ? o->mat[YEE_MAT_MU] : o->mat[YEE_MAT_EPS];
+  const double dt = o->dt, dx = o->dx;
+  r->m = o_get_material (o, comp, i, j, k, mgrid);
+  if (o->use_pml)
+  {
+    double s = o_sig (o, c->sig_curl, comp, i, j, k);
+    double dd = (2 * eps0 * 1.0 + s * dt);
+    r->cac = (2 * eps0 * 1.0 - s * dt) / dd;
+    r->cbc = (2 * eps0 * dt / dx) / dd;
+    double s4 = o_sig (o, c->sig4, comp, i, j, k);
+    double s5 = o_sig (o, c->sig5, comp, i, j, k);
+    double mod = r->m * modifier;
+    double dd5 = (2 * eps0 * 1.0 + s5 * dt);
+    r->ca5 = (2 * eps0 * 1.0 - s5 * dt) / dd5;
+    r->kp = ((2 * eps0 * 1.0 + s4 * dt) / (mod)) / dd5;
+    r->km = ((2 * eps0 * 1.0 - s4 * dt) / (mod)) / dd5;
+    r->cb = 0;
+  }
+  else
+  {
+    r->cb = dt / (r->m * modifier * dx);
+    r->cac = r->cbc = r->kp = r->km = 0;
+    r->ca5 = 1.0;
+  }
+}
+
+/* Adds s * w to the four curl operands of component comp at (i, j, k) in the adjoint
+ * arrays a[] (tmp = v12 - v11 - v22 + v21). */
+static void o_scatter (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, double w, double *a[6])
+{
+  const comp_desc *c = &CD[comp];
+  const int dz = o->dim == 3;
+  if (o->present[c->opp1])
+  {
+    a[c->opp1][o_idx (o, i + c->d12[0], j + c->d12[1], k + dz * c->d12[2])] += w;
+    a[c->opp1][o_idx (o, i + c->d11[0], j + c->d11[1], k + dz * c->d11[2])] -= w;
+  }
+  if (o->present[c->opp2])
+  {
+    a[c->opp2][o_idx (o, i + c->d22[0], j + c->d22[1], k + dz * c->d22[2])] -= w;
+    a[c->opp2][o_idx (o, i + c->d21[0], j + c->d21[1], k + dz * c->d21[2])] += w;
+  }
+}
+
+int oracle_adjoint (oracle *o, const oracle_adj_spec *sp, int32_t T, double *J_out, double *grad)
+{
+  if (!o || !sp || !J_out || !grad || !o->prepared || o->t != 0 || o->dim != 3 || o->use_meta || T < 2) return -1;
+  if (sp->window < 1 || sp->window > T - 1) return -1;
+  const int64_t nc = o->ncell;
+  const int nstate = o->use_pml ? 12 : 6;
+  /* trajectory: state after steps 0 .. T-1 */
+  double *traj = (double *) malloc (sizeof (double) * (size_t) nstate * nc * T);
+  if (!traj) return -2;
+  double J = 0;
+  for (int32_t t = 0; t < T; ++t)
+  {
+    oracle_step (o, 1);
+    for (int c = 0; c < 6; ++c)
+    {
+      memcpy (traj + ((size_t) t * nstate + c) * nc, o->f[c][1], sizeof (double) * nc);
+      if (o->use_pml) memcpy (traj + ((size_t) t * nstate + 6 + c) * nc, o->d[c][1], sizeof (double) * nc);
+    }
+    if (t >= T - sp->window)
+    {
+      for (int64_t x = sp->mlo[0]; x < sp->mhi[0]; ++x)
+        for (int64_t y = sp->mlo[1]; y < sp->mhi[1]; ++y)
+        {
+          double v = o->f[YEE_EZ][1][o_idx (o, x, y, sp->mz)];
+          J += v * v;
+        }
+    }
+  }
+  *J_out = J;
+
+  const int64_t dn[3] = { sp->dhi[0] - sp->dlo[0], sp->dhi[1] - sp->dlo[1], sp->dhi[2] - sp->dlo[2] };
+  memset (grad, 0, sizeof (double) * (size_t) (dn[0] * dn[1] * dn[2]));
+  double *aF[6], *aD[6], *w = (double *) calloc ((size_t) nc, sizeof (double));
+  for (int c = 0; c < 6; ++c)
+  {
+    aF[c] = (double *) calloc ((size_t) nc, sizeof (double));
+    aD[c] = (double *) calloc ((size_t) nc, sizeof (double));
+  }
+  for (int32_t t = T - 1; t >= 1; --t)
+  {
+    const double *Scur = traj + (size_t) t * nstate * nc, *Sprev = traj + (size_t) (t - 1) * nstate * nc;
+    if (t >= T - sp->window)
+      for (int64_t x = sp->mlo[0]; x < sp->mhi[0]; ++x)
+        for (int64_t y = sp->mlo[1]; y < sp->mhi[1]; ++y)
+        {
+          int64_t q = o_idx (o, x, y, sp->mz);
+          aF[YEE_EZ][q] += 2 * Scur[(size_t) YEE_EZ * nc + q];
+        }
+    for (int pass = 0; pass < 2; ++pass)
+    {
+      const int c0 = pass == 0 ? 3 : 0;   /* H transpose first, then E */
+      for (int comp = c0; comp < c0 + 3; ++comp)
+      {
+        memset (w, 0, sizeof (double) * (size_t) nc);
+        for (int64_t i = 1; i < o->n[0] - 1; ++i)
+          for (int64_t j = 1; j < o->n[1] - 1; ++j)
+            for (int64_t k = 1; k < o->n[2] - 1; ++k)
+            {
+              int64_t q = o_idx (o, i, j, k);
+              o_coef cf;
+              o_cell_coef (o, comp, i, j, k, &cf);
+              double a = aF[comp][q];
+              /* a hard source overwrites the new value: nothing flows back from it */
+              if ((o->cfg.point_source & (1u << comp)) && i == o->cfg.point_source_pos[0]
+                  && j == o->cfg.point_source_pos[1] && k == o->cfg.point_source_pos[2])
+                a = 0;
+              if (comp < 3)
+              {
+                /* dE'/dm = -(E' - ca5 E) / m, half to each eps voxel of the average */
+                double dEdm = -(Scur[(size_t) comp * nc + q] - cf.ca5 * Sprev[(size_t) comp * nc + q]) / cf.m;
+                int64_t cells[4][3];
+                o_material_cells (o, comp, i, j, k, cells);
+                for (int h = 0; h < 2; ++h)
+                {
+                  int64_t *v = cells[h];
+                  if (v[0] < sp->dlo[0] || v[0] >= sp->dhi[0] || v[1] < sp->dlo[1] || v[1] >= sp->dhi[1]
+                      || v[2] < sp->dlo[2] || v[2] >= sp->dhi[2])
+                    continue;
+                  grad[((v[0] - sp->dlo[0]) * dn[1] + (v[1] - sp->dlo[1])) * dn[2] + (v[2] - sp->dlo[2])] += a * dEdm * 0.5;
+                }
+              }
+              if (o->use_pml)
+              {
+                double mD = aD[comp][q] + cf.kp * a;
+                w[q] = cf.cbc * mD;
+                aD[comp][q] = cf.cac * mD - cf.km * a;
+                aF[comp][q] = cf.ca5 * a;
+              }
+              else
+              {
+                w[q] = cf.cb * a;
+                aF[comp][q] = a;
+              }
+            }
+        for (int64_t i = 1; i < o->n[0] - 1; ++i)
+          for (int64_t j = 1; j < o->n[1] - 1; ++j)
+            for (int64_t k = 1; k < o->n[2] - 1; ++k)
+            {
+              int64_t q = o_idx (o, i, j, k);
+              if (w[q] != 0) o_scatter (o, comp, i, j, k, w[q], aF);
+            }
+      }
+    }
+  }
+  for (int c = 0; c < 6; ++c)
+  {
+    free (aF[c]);
+    free (aD[c]);
+  }
+  free (w);
+  free (traj);
+  return 0;
+}

@@ -43,6 +43,17 @@ int64_t oracle_incident_len (oracle *o);
 int oracle_set_source_phase (oracle *o, double phase);
 int oracle_set_source_gaussian (oracle *o, double t0, double sigma);
 
+/* FP64 adjoint of the reference-form update (see yee_oracle.c): runs T steps from a freshly
+ * prepared oracle, returns J = sum over the last `window` steps of sum of Ez^2 on the monitor
+ * rectangle [mlo, mhi) x {mz}, and dJ/deps over the design box [dlo, dhi) (x-major). */
+typedef struct
+{
+  int64_t dlo[3], dhi[3];
+  int64_t mz, mlo[2], mhi[2];
+  int32_t window;
+} oracle_adj_spec;
+int oracle_adjoint (oracle *o, const oracle_adj_spec *sp, int32_t T, double *J, double *grad);
+
 /* Material generators, restated from Approximation.cpp (used to build parity inputs). */
 double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,
                                   double radius, double eps, double outside, int accuracy);
