@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
 
-SOURCES = ["yee_setup.cpp", "yee_kernels.hip", "yee_engine.hip", "yee_batch.hip"]
+SOURCES = ["yee_setup.cpp", "yee_kernels.hip", "yee_engine.hip", "yee_batch.hip", "yee_adjoint.hip"]
 
 
 def hipcc() -> str:

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/yee.h"
+#include "yee_internal.h"
 #include "yee_kernels.h"
 #include "yee_setup.h"
 
@@ -32,70 +33,17 @@ template <typename T, int BY> __global__ void step_3d_fused (StepArgs<T>);
 
 static thread_local std::string g_error;
 
-static int fail (int code, const std::string &msg)
+int fail (int code, const std::string &msg)
 {
   g_error = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-  do                                                                                           \
-  {                                                                                            \
-    hipError_t e_ = (expr);                                                                    \
-    if (e_ != hipSuccess)                                                                      \
-      return fail (YEE_EDEVICE, std::string (#expr) + ": " + hipGetErrorString (e_));          \
-  } while (0)
 
 }  // namespace yee
 
 using namespace yee;
 
-struct yee_handle
-{
-  yee_config cfg {};
-  int dim = 3;
-  int64_t n[3] = { 1, 1, 1 };
-  int64_t ncell = 0;
-  size_t wsize = 8;
-  int mode = 0;   // 0 plain, 1 PML, 2 PML + Drude
-  bool present[6] = {};
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[2] = { nullptr, nullptr };
-
-  // state: two buffers per array when the fused 3D step is used (old -> new), else one
-  void *f[2][6] = {}, *d[2][6] = {}, *dp[2][6] = {}, *d1[2][6] = {}, *d1p[2][6] = {};
-  int cur = 0;          // buffer holding the newest state
-  bool fused = false;   // 3D fused step kernel on the PML-free box + slab half-steps
-  int64_t fbox_lo[3] = { 1, 1, 1 }, fbox_hi[3] = { 1, 1, 1 };   // PML-free box of the fused kernel
-  int fused_lx = 0, fused_nstrips = 0, fused_nchunks = 0, fused_threads = 0;
-  // padded grid allocations (every grid-sized array; the views above point inside them)
-  std::vector<void *> grids;
-  void *tg = nullptr;   // TfsfGeom<T> in device memory
-
-  void *mat[6] = {};
-  double mat_u[6] = { 1, 1, 0, 0, 0, 0 };
-  bool mat_nonzero[6] = {};
-  int64_t box_lo[3] = { 0, 0, 0 }, box_hi[3] = { 0, 0, 0 };   // Drude-material bounding box (eps cells)
-  bool box_any = false;
-
-  void *pml = nullptr;
-  int tab_len = 0;
-  void *einc = nullptr, *hinc[2] = { nullptr, nullptr };
-  int64_t inc_len = 0;
-  Scalars sc {};
-  double src_t0 = 0, src_sigma = 0;   // Gaussian envelope of the point source (sigma <= 0: none)
-  int32_t t = 0;
-  bool prepared = false;
-  double last_ms = 0;
-  // kernel timing (yee_profile_steps): events recorded around each launch when set
-  std::vector<hipEvent_t> *prof_events = nullptr;
-  std::vector<int> *prof_kinds = nullptr;
-
-  bool fp32 () const { return wsize == 4; }
-  bool use_pml () const { return (cfg.flags & YEE_USE_PML) != 0; }
-  bool use_tfsf () const { return (cfg.flags & YEE_USE_TFSF) != 0; }
-  bool use_meta () const { return (cfg.flags & YEE_USE_METAMATERIALS) != 0; }
-};
 
 namespace {
 
@@ -624,6 +572,46 @@ int upload_tfsf_geom (yee_handle *h)
 
 }  // namespace
 
+namespace yee {
+
+int engine_step_once (yee_handle *h)
+{
+  return h->fp32 () ? step_once<float> (h) : step_once<double> (h);
+}
+
+int engine_alloc_grid (yee_handle *h, void **view)
+{
+  return alloc_grid (h, view);
+}
+
+template <typename T>
+StepArgs<T> engine_args (const yee_handle *h, int parity)
+{
+  return make_args<T> (h, parity);
+}
+template StepArgs<float> engine_args<float> (const yee_handle *, int);
+template StepArgs<double> engine_args<double> (const yee_handle *, int);
+
+int engine_reset (yee_handle *h)
+{
+  const size_t bytes = (size_t) (kPadFront + h->ncell + pad_back (h)) * h->wsize;
+  for (int bsel = 0; bsel < 2; ++bsel)
+    for (int c = 0; c < 6; ++c)
+    {
+      void *ptrs[5] = { h->f[bsel][c], h->d[bsel][c], h->dp[bsel][c], h->d1[bsel][c], h->d1p[bsel][c] };
+      for (void *p : ptrs)
+        if (p) HIP_TRY (hipMemsetAsync ((char *) p - kPadFront * h->wsize, 0, bytes, h->stream));
+    }
+  if (h->einc) HIP_TRY (hipMemsetAsync (h->einc, 0, (size_t) h->inc_len * h->wsize, h->stream));
+  for (void *p : h->hinc)
+    if (p) HIP_TRY (hipMemsetAsync (p, 0, (size_t) h->inc_len * h->wsize, h->stream));
+  h->t = 0;
+  h->cur = 0;
+  return YEE_OK;
+}
+
+}  // namespace yee
+
 extern "C" {
 
 const char *yee_last_error (void)
@@ -733,6 +721,7 @@ void yee_destroy (yee_handle *h)
 {
   if (!h) return;
   if (h->stream) (void) hipStreamSynchronize (h->stream);
+  adjoint_free (h);
   for (void *p : h->grids) (void) hipFree (p);
   if (h->tg) (void) hipFree (h->tg);
   if (h->pml) (void) hipFree (h->pml);

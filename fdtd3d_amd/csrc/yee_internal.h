@@ -1,0 +1,93 @@
+// Internal declarations shared by the host files of libyee.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/yee.h"
+#include "yee_kernels.h"
+#include "yee_setup.h"
+
+namespace yee {
+
+// Records `msg` as the calling thread's last error (yee_last_error) and returns `code`.
+int fail (int code, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                          \
+  do                                                                                           \
+  {                                                                                            \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return ::yee::fail (YEE_EDEVICE, std::string (#expr) + ": " + hipGetErrorString (e_));   \
+  } while (0)
+
+struct AdjointState;
+
+}  // namespace yee
+
+struct yee_handle
+{
+  yee_config cfg {};
+  int dim = 3;
+  int64_t n[3] = { 1, 1, 1 };
+  int64_t ncell = 0;
+  size_t wsize = 8;
+  int mode = 0;   // 0 plain, 1 PML, 2 PML + Drude
+  bool present[6] = {};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = { nullptr, nullptr };
+
+  // state: two buffers per array when the fused 3D step is used (old -> new), else one
+  void *f[2][6] = {}, *d[2][6] = {}, *dp[2][6] = {}, *d1[2][6] = {}, *d1p[2][6] = {};
+  int cur = 0;          // buffer holding the newest state
+  bool fused = false;   // 3D fused step kernel on the PML-free box + slab half-steps
+  int64_t fbox_lo[3] = { 1, 1, 1 }, fbox_hi[3] = { 1, 1, 1 };   // PML-free box of the fused kernel
+  int fused_lx = 0, fused_nstrips = 0, fused_nchunks = 0, fused_threads = 0;
+  // padded grid allocations (every grid-sized array; the views above point inside them)
+  std::vector<void *> grids;
+  void *tg = nullptr;   // TfsfGeom<T> in device memory
+
+  void *mat[6] = {};
+  double mat_u[6] = { 1, 1, 0, 0, 0, 0 };
+  bool mat_nonzero[6] = {};
+  int64_t box_lo[3] = { 0, 0, 0 }, box_hi[3] = { 0, 0, 0 };   // Drude-material bounding box (eps cells)
+  bool box_any = false;
+
+  void *pml = nullptr;
+  int tab_len = 0;
+  void *einc = nullptr, *hinc[2] = { nullptr, nullptr };
+  int64_t inc_len = 0;
+  yee::Scalars sc {};
+  double src_t0 = 0, src_sigma = 0;   // Gaussian envelope of the point source (sigma <= 0: none)
+  int32_t t = 0;
+  bool prepared = false;
+  double last_ms = 0;
+  // kernel timing (yee_profile_steps): events recorded around each launch when set
+  std::vector<hipEvent_t> *prof_events = nullptr;
+  std::vector<int> *prof_kinds = nullptr;
+
+  bool fp32 () const { return wsize == 4; }
+  bool use_pml () const { return (cfg.flags & YEE_USE_PML) != 0; }
+  bool use_tfsf () const { return (cfg.flags & YEE_USE_TFSF) != 0; }
+  bool use_meta () const { return (cfg.flags & YEE_USE_METAMATERIALS) != 0; }
+
+  yee::AdjointState *adj = nullptr;   // forward/backward pair state (yee_adjoint_setup)
+};
+
+namespace yee {
+
+// One time step of the handle (the body of yee_step), enqueued on its stream.
+int engine_step_once (yee_handle *h);
+// Padded, zero-filled grid allocation owned by the handle.
+int engine_alloc_grid (yee_handle *h, void **view);
+// Zeroes every field, auxiliary and incident array and resets the step counter.
+int engine_reset (yee_handle *h);
+// Kernel argument block of the handle's current state (in-place half-step view).
+template <typename T>
+StepArgs<T> engine_args (const yee_handle *h, int parity);
+// Frees the adjoint state (called by yee_destroy).
+void adjoint_free (yee_handle *h);
+
+}  // namespace yee

@@ -14,233 +14,9 @@
 // reference's D->E stage with sigma = 0 up to rounding.
 #include <hip/hip_runtime.h>
 
-#include "yee_kernels.h"
+#include "yee_device.h"
 
 namespace yee {
-
-// Global-memory accessors: the state pointers live in the kernel argument block, where the
-// compiler cannot see their address space; casting to address space 1 keeps every access
-// a global_load/global_store instead of a flat one.
-template <typename T>
-__device__ __forceinline__ T gld (const T *p)
-{
-  return *(const __attribute__ ((address_space (1))) T *) (p);
-}
-
-template <typename T>
-__device__ __forceinline__ void gst (T *p, T v)
-{
-  *(__attribute__ ((address_space (1))) T *) (p) = v;
-}
-
-// Row-relative accessors for the fused kernel.  `row` MUST be uniform over the wavefront:
-// it is pinned to scalar registers (the empty asm also keeps the compiler from folding the
-// lane offset into it), so every access is selected as scalar base + 32-bit lane offset
-// `boff` + immediate `delta` (elements), with no per-access 64-bit address registers.
-template <typename T>
-__device__ __forceinline__ T rld (const T *row, uint32_t boff, int delta = 0)
-{
-  typedef const __attribute__ ((address_space (1))) char gchar;
-  typedef const __attribute__ ((address_space (1))) T gT;
-  asm volatile ("" : "+s"(row));
-  gchar *p = (gchar *) row + boff;
-  return *((gT *) p + delta);
-}
-
-template <typename T>
-__device__ __forceinline__ void rst (T *row, uint32_t boff, T v)
-{
-  typedef __attribute__ ((address_space (1))) char gchar;
-  typedef __attribute__ ((address_space (1))) T gT;
-  asm volatile ("" : "+s"(row));
-  gchar *p = (gchar *) row + boff;
-  *(gT *) p = v;
-}
-
-// E_CENTERED component tables (the host CompDesc plus the circuit offsets of
-// YeeGridLayout.cpp:1937-1961).  Every lookup below is resolved at compile time: runtime
-// indexing of small arrays would push them to scratch memory.
-struct Tab
-{
-  // slot directions (LEFT=0, RIGHT=1, DOWN=2, UP=3, BACK=4, FRONT=5) of d11, d12, d21, d22
-  static constexpr int dir[6][4] = {
-    { 2, 3, 4, 5 },   // Ex: DOWN UP BACK FRONT
-    { 4, 5, 0, 1 },   // Ey: BACK FRONT LEFT RIGHT
-    { 0, 1, 2, 3 },   // Ez: LEFT RIGHT DOWN UP
-    { 4, 5, 2, 3 },   // Hx: BACK FRONT DOWN UP
-    { 0, 1, 4, 5 },   // Hy: LEFT RIGHT BACK FRONT
-    { 2, 3, 0, 1 },   // Hz: DOWN UP LEFT RIGHT
-  };
-  static constexpr int opp[6][2] = { { 5, 4 }, { 3, 5 }, { 4, 3 }, { 1, 2 }, { 2, 0 }, { 0, 1 } };
-  // offsets [slot][axis] of the four opposite-field operands
-  static constexpr int off[6][4][3] = {
-    { { 0, -1, 0 }, { 0, 0, 0 }, { 0, 0, -1 }, { 0, 0, 0 } },
-    { { 0, 0, -1 }, { 0, 0, 0 }, { -1, 0, 0 }, { 0, 0, 0 } },
-    { { -1, 0, 0 }, { 0, 0, 0 }, { 0, -1, 0 }, { 0, 0, 0 } },
-    { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 0, 0 }, { 0, 1, 0 } },
-    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, 0 }, { 0, 0, 1 } },
-    { { 0, 0, 0 }, { 0, 1, 0 }, { 0, 0, 0 }, { 1, 0, 0 } },
-  };
-};
-
-// sigma axes (curl, sigma4, sigma5) per component
-template <int COMP> struct SigAxes;
-template <> struct SigAxes<0> { static constexpr int c = 1, s4 = 0, s5 = 2; };
-template <> struct SigAxes<1> { static constexpr int c = 2, s4 = 1, s5 = 0; };
-template <> struct SigAxes<2> { static constexpr int c = 0, s4 = 2, s5 = 1; };
-template <> struct SigAxes<3> { static constexpr int c = 1, s4 = 0, s5 = 2; };
-template <> struct SigAxes<4> { static constexpr int c = 2, s4 = 1, s5 = 0; };
-template <> struct SigAxes<5> { static constexpr int c = 0, s4 = 2, s5 = 1; };
-
-template <int AX>
-__device__ __forceinline__ int pick (int i, int j, int k)
-{
-  return AX == 0 ? i : (AX == 1 ? j : k);
-}
-
-// Twice the real coordinate of component COMP along axis AX at integer index p.
-template <int COMP, int AX>
-__device__ __forceinline__ int twice_coord (int p)
-{
-  constexpr bool h = COMP >= 3;
-  constexpr int own = COMP % 3;
-  return 2 * p + (h ? (AX == own ? 1 : 2) : (AX == own ? 2 : 1));
-}
-
-// doNeedTFSFUpdate<comp>Border(pos, dir) on doubled integer coordinates (YeeGridLayout.cpp:272-1740;
-// the FP tests c == b +- 0.5, c in (L - 0.1, R + 0.1), c in (L + 0.4, R - 0.4) are exact on half-integers).
-template <typename T, int COMP, int DIR, int DIM>
-__device__ __forceinline__ bool tfsf_pred (const StepArgs<T> &a, int i, int j, int k)
-{
-  constexpr int b = DIR >> 1;
-  constexpr bool low = (DIR & 1) == 0;
-  constexpr int ax = COMP % 3;
-  constexpr bool h = COMP >= 3;
-  constexpr int t = 3 - b - ax;
-  if (b == ax || b >= DIM) return false;
-  const int cb2 = twice_coord<COMP, b> (pick<b> (i, j, k));
-  bool ok = h ? (low ? cb2 == 2 * a.tl[b] : cb2 == 2 * a.tr[b])
-              : (low ? cb2 == 2 * a.tl[b] - 1 : cb2 == 2 * a.tr[b] + 1);
-  if (ax < DIM)
-  {
-    const int c2 = twice_coord<COMP, ax> (pick<ax> (i, j, k));
-    ok = ok && (h ? (c2 >= 2 * a.tl[ax] + 1 && c2 <= 2 * a.tr[ax] - 1) : (c2 >= 2 * a.tl[ax] && c2 <= 2 * a.tr[ax]));
-  }
-  if (t < DIM)
-  {
-    const int c2 = twice_coord<COMP, t> (pick<t> (i, j, k));
-    ok = ok && (h ? (c2 >= 2 * a.tl[t] && c2 <= 2 * a.tr[t]) : (c2 >= 2 * a.tl[t] + 1 && c2 <= 2 * a.tr[t] - 1));
-  }
-  return ok;
-}
-
-// Runtime-direction form (mask export only).
-template <typename T, int COMP, int DIM>
-__device__ uint8_t tfsf_bits (const StepArgs<T> &a, int i, int j, int k)
-{
-  return (uint8_t) ((tfsf_pred<T, COMP, 0, DIM> (a, i, j, k) ? 1 : 0) | (tfsf_pred<T, COMP, 1, DIM> (a, i, j, k) ? 2 : 0)
-                    | (tfsf_pred<T, COMP, 2, DIM> (a, i, j, k) ? 4 : 0) | (tfsf_pred<T, COMP, 3, DIM> (a, i, j, k) ? 8 : 0)
-                    | (tfsf_pred<T, COMP, 4, DIM> (a, i, j, k) ? 16 : 0) | (tfsf_pred<T, COMP, 5, DIM> (a, i, j, k) ? 32 : 0));
-}
-
-// A cell can carry a TF/SF correction only on the planes L-1, R-1, R of some axis while
-// lying inside [L-1, R] on every axis.
-template <typename T, int DIM>
-__device__ __forceinline__ bool tfsf_near (const StepArgs<T> &a, int i, int j, int k)
-{
-  const int p[3] = { i, j, k };
-  bool face = false;
-#pragma unroll
-  for (int ax = 0; ax < DIM; ++ax)
-  {
-    if (p[ax] < a.tl[ax] - 1 || p[ax] > a.tr[ax]) return false;
-    face |= (p[ax] == a.tl[ax] - 1) | (p[ax] == a.tr[ax] - 1) | (p[ax] == a.tr[ax]);
-  }
-  return face;
-}
-
-// Incident value of component OPP at integer position (i, j, k): approximateIncidentWave +
-// get<opp>FromIncident (InternalScheme.cpp:375-407, InternalScheme.inc.h:500-515).
-template <typename T, int OPP, int DIM>
-__device__ double incident_at (const StepArgs<T> &a, int i, int j, int k, const T *inc, double ddiff)
-{
-  const TfsfGeom<T> &tg = *a.tg;
-  const double x = 0.5 * (double) twice_coord<OPP, 0> (i) - tg.zinc[0];
-  const double y = 0.5 * (double) twice_coord<OPP, 1> (j) - tg.zinc[1];
-  double d;
-  if (DIM == 3)
-  {
-    const double z = 0.5 * (double) twice_coord<OPP, 2> (k) - tg.zinc[2];
-    d = x * tg.S1 * tg.C2 + y * tg.S1 * tg.S2 + z * tg.C1 - ddiff;
-  }
-  else
-  {
-    d = x * tg.C2 + y * tg.S2 - ddiff;
-  }
-  const double c1 = (double) (long long) d;
-  long long i1 = (long long) c1;
-  // the reference asserts d > 0; clamp so a degenerate box can never read out of bounds
-  if (i1 < 0) i1 = 0;
-  if (i1 > tg.inc_len - 2) i1 = tg.inc_len - 2;
-  const double p2 = d - c1;
-  const double p1 = 1 - p2;
-  const double v = (double) gld (inc + i1) * p1 + (double) gld (inc + i1 + 1) * p2;
-  return v * tg.proj[OPP];
-}
-
-// One operand pair of calculateTFSF.  E components (E_CENTERED) correct the operand opposite
-// to the border side; H components the operand on the border side, with the sign flipped.
-template <typename T, int COMP, int PAIR, int DIM>
-__device__ __forceinline__ void tfsf_pair (const StepArgs<T> &a, int i, int j, int k, T &v_lo, T &v_hi)
-{
-  constexpr bool h = COMP >= 3;
-  constexpr int OPP = Tab::opp[COMP][PAIR];
-  const bool lo = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR], DIM> (a, i, j, k);
-  const bool hi = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR + 1], DIM> (a, i, j, k);
-  if (!lo && !hi) return;
-  const T *inc = h ? a.tg->einc : a.tg->hinc[a.hinc_sel];
-  const double ddiff = h ? 0.0 : 0.5;
-  // slot 2*PAIR is "lo" (d11 / d21), slot 2*PAIR+1 is "hi" (d12 / d22)
-  const bool use_hi_slot = h ? !lo : lo;
-  constexpr int s_lo = 2 * PAIR, s_hi = 2 * PAIR + 1;
-  double diff;
-  if (use_hi_slot)
-    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_hi][0], j + Tab::off[COMP][s_hi][1],
-                                     k + (DIM == 3 ? Tab::off[COMP][s_hi][2] : 0), inc, ddiff);
-  else
-    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_lo][0], j + Tab::off[COMP][s_lo][1],
-                                     k + (DIM == 3 ? Tab::off[COMP][s_lo][2] : 0), inc, ddiff);
-  if (h) diff *= -1.0;
-  if (use_hi_slot) v_hi = (T) ((double) v_hi - diff);
-  else v_lo = (T) ((double) v_lo - diff);
-}
-
-// calculateTFSF for one component at cell (i, j, k): corrects the operands v11, v12, v21, v22.
-template <typename T, int COMP, int DIM>
-__device__ __forceinline__ void tfsf_apply (const StepArgs<T> &a, int i, int j, int k, T v[4])
-{
-  tfsf_pair<T, COMP, 0, DIM> (a, i, j, k, v[0], v[1]);
-  tfsf_pair<T, COMP, 1, DIM> (a, i, j, k, v[2], v[3]);
-}
-
-template <typename T>
-__device__ __forceinline__ bool is_special (const StepArgs<T> &a, const int p[3], int dim, int mode)
-{
-  if (mode == 0) return false;
-  bool s = false;
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax)
-    if (ax < dim) s |= (p[ax] < a.slab_lo[ax]) | (p[ax] >= a.slab_hi[ax]);
-  if (mode == 2)
-  {
-    bool in = true;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
-      if (ax < dim) in &= (p[ax] >= a.box_lo[ax]) & (p[ax] < a.box_hi[ax]);
-    s |= in;
-  }
-  return s;
-}
 
 // New value of one field component at flat index q (reference formulas, see file header).
 //   f_old : the component's previous value
@@ -323,29 +99,6 @@ __device__ __forceinline__ void load_old (const StepArgs<T> &a, int64_t q, bool 
   d_old = (MODE != 0 && special) ? gld (a.d[COMP] + q) : (T) 0;
 }
 
-template <typename T>
-__device__ __forceinline__ T curl (const T v[4])
-{
-  return v[1] - v[0] - v[3] + v[2];
-}
-
-template <typename T>
-__device__ __forceinline__ T avg2 (T a, T b)
-{
-  return (a + b) / (T) 2.0;
-}
-
-template <typename T>
-__device__ __forceinline__ T avg4 (T m11, T m12, T m21, T m22)
-{
-  return (avg2 (m11, m12) + avg2 (m21, m22)) / (T) 2.0;
-}
-
-template <typename T>
-__device__ __forceinline__ T sum4 (T m11, T m12, T m21, T m22)
-{
-  return (m11 + m12 + m21 + m22) / (T) 4.0;
-}
 
 // ---------------------------------------------------------------------------------------
 // 3D E half-step: Ex, Ey, Ez on the run box (all of [1, N-1)^3, or one PML slab when the

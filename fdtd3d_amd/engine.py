@@ -35,6 +35,13 @@ EXPORTED = [
 ]
 
 
+class AdjointSpecC(ctypes.Structure):
+    """Binary layout of ``struct yee_adjoint_spec``."""
+    _fields_ = [("design_lo", ctypes.c_int64 * 3), ("design_hi", ctypes.c_int64 * 3), ("monitor_z", ctypes.c_int64),
+                ("monitor_lo", ctypes.c_int64 * 2), ("monitor_hi", ctypes.c_int64 * 2), ("window", ctypes.c_int32),
+                ("checkpoint_every", ctypes.c_int32)]
+
+
 class YeeError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"yee error {code}: {msg}")
@@ -75,6 +82,10 @@ def load_library():
     L.yee_set_point_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.yee_set_material_sphere.argtypes = [vp, i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
                                           ctypes.c_double, i32, i32]
+    L.yee_adjoint_setup.argtypes = [vp, ctypes.POINTER(AdjointSpecC)]
+    L.yee_forward.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_double)]
+    L.yee_backward.argtypes = [vp, ctypes.c_double, vp]
+    L.yee_adjoint_timing.argtypes = [vp, vp]
     L.yee_batch_last_error.restype = ctypes.c_char_p
     L.yee_batch_create.argtypes = [vp, i32, ctypes.POINTER(vp)]
     L.yee_batch_destroy.argtypes = [vp]
@@ -184,6 +195,36 @@ class Engine:
         _check(load_library().yee_profile_steps(self.h, int(n), out.ctypes.data))
         return {"incident_ms": float(out[0]), "e_half_ms": float(out[1]), "h_half_ms": float(out[2]),
                 "fused_ms": float(out[3]), "step_ms": float(out[4])}
+
+    def adjoint_setup(self, design_lo, design_hi, monitor_z, monitor_lo, monitor_hi, window, checkpoint_every=0):
+        sp = AdjointSpecC()
+        for a in range(3):
+            sp.design_lo[a], sp.design_hi[a] = int(design_lo[a]), int(design_hi[a])
+        sp.monitor_z = int(monitor_z)
+        for a in range(2):
+            sp.monitor_lo[a], sp.monitor_hi[a] = int(monitor_lo[a]), int(monitor_hi[a])
+        sp.window = int(window)
+        sp.checkpoint_every = int(checkpoint_every)
+        self._design_shape = tuple(int(design_hi[a]) - int(design_lo[a]) for a in range(3))
+        _check(load_library().yee_adjoint_setup(self.h, ctypes.byref(sp)))
+
+    def forward(self, design_eps: np.ndarray) -> float:
+        arr = np.ascontiguousarray(design_eps, dtype=np.float64)
+        if arr.shape != self._design_shape:
+            raise ValueError(f"design eps: expected {self._design_shape}, got {arr.shape}")
+        J = ctypes.c_double()
+        _check(load_library().yee_forward(self.h, arr.ctypes.data, ctypes.byref(J)))
+        return J.value
+
+    def backward(self, dJ: float = 1.0) -> np.ndarray:
+        out = np.empty(self._design_shape, dtype=np.float64)
+        _check(load_library().yee_backward(self.h, float(dJ), out.ctypes.data))
+        return out
+
+    def adjoint_timing(self):
+        out = np.zeros(2, dtype=np.float64)
+        _check(load_library().yee_adjoint_timing(self.h, out.ctypes.data))
+        return {"forward_ms": float(out[0]), "backward_ms": float(out[1])}
 
     def field(self, name: str) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.float64)

@@ -211,6 +211,36 @@ int yee_last_step_ms (yee_handle *h, double *ms);
 int yee_profile_steps (yee_handle *h, int32_t n, double out[5]);
 
 /*
+ * Forward/backward pair (shape of a torch.autograd.Function): J(eps) and dJ/d eps over a
+ * design box, by the discrete adjoint (time-reversed transpose) of the engine's own 3D update.
+ * No counterpart in the reference, which has no differentiation.
+ *   J = sum over the last `window` steps of the sum of Ez^2 on the rectangle
+ *       [monitor_lo[0], monitor_hi[0]) x [monitor_lo[1], monitor_hi[1]) of plane z = monitor_z.
+ * checkpoint_every = 0 records E on the design box after every step during yee_forward;
+ * K > 0 keeps a full-state snapshot every K steps instead and re-runs each segment during
+ * yee_backward (memory: one snapshot per K steps, K + 1 design-box slots).
+ */
+typedef struct yee_adjoint_spec
+{
+  int64_t design_lo[3], design_hi[3];   /* eps voxels [lo, hi) (inside the PML-free interior) */
+  int64_t monitor_z;
+  int64_t monitor_lo[2], monitor_hi[2];
+  int32_t window;                       /* 1 <= window <= time_steps - 1 */
+  int32_t checkpoint_every;
+} yee_adjoint_spec;
+
+/* After yee_prepare: allocates the adjoint state (3D, no metamaterials). */
+int yee_adjoint_setup (yee_handle *h, const yee_adjoint_spec *spec);
+/* Sets the design-box eps (x-major, FP64), resets every field to zero and runs time_steps
+ * steps; returns J.  The rest of the eps grid is the one set with yee_set_material. */
+int yee_forward (yee_handle *h, const double *design_eps, double *J);
+/* dJ/d eps of the last yee_forward, times the upstream gradient dJ, into grad_eps (design box,
+ * x-major).  Leaves the fields in an unspecified state (re-run yee_forward to use them). */
+int yee_backward (yee_handle *h, double dJ, double *grad_eps);
+/* out[0] = device ms of the last yee_forward, out[1] = of the last yee_backward */
+int yee_adjoint_timing (yee_handle *h, double out[2]);
+
+/*
  * Batched TMz sweep: `nsim` independent TMz simulations (each its own yee_config: size, PML,
  * point source; one precision for the batch; no TF/SF, no metamaterials) advanced together,
  * two launches per time step for the whole batch, replayed from a captured HIP graph.  The

@@ -1,0 +1,123 @@
+"""Forward/backward pair (yee_forward / yee_backward) on the GPU (needs the GPU).
+
+Bars:
+  - FP64 GPU gradient vs the FP64 CPU adjoint of the oracle (reference-form update): relative
+    L2 <= 1e-9 (the two forwards differ by rounding: the engine updates E/H directly outside
+    the PML slabs);
+  - FP64 GPU gradient vs central finite differences of the GPU forward (h = 1e-4 eps):
+    relative error <= 1e-6 per voxel (relative to max(|fd|, 1e-3 max|grad|));
+  - checkpointed and traced backward passes: bit-identical gradients;
+  - FP32 gradient vs FP64 gradient: relative L2 <= 1e-3;
+  - J of yee_forward equals J of the oracle to 1e-12 (FP64).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fdtd3d_amd.config import parse_args
+from fdtd3d_amd.engine import Engine
+from oracle.binding import oracle_adjoint
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("--3d --size x:18,y:16,z:20 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:5,y:5,z:5 "
+     "--tfsf-size-right x:5,y:5,z:5 --angle-teta 30 --angle-phi 45 --angle-psi 90 --wavelength 0.005 --time-steps 36",
+     (7, 6, 8), (11, 10, 12), 13, (5, 5), (13, 11), 12),
+    ("--3d --size x:14,y:14,z:14 --point-source x:4,y:5,z:6 --point-source-ez --wavelength 0.004 --time-steps 30",
+     (6, 6, 6), (9, 10, 9), 8, (3, 3), (11, 11), 10),
+]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _eps(cfg, seed):
+    return np.random.default_rng(seed).uniform(1.0, 4.0, size=cfg.shape)
+
+
+def _engine(cfg, eps, dlo, dhi, mz, mlo, mhi, window, precision=8, ckpt=0):
+    cfg.precision = precision
+    e = Engine(cfg)
+    e.set_material("eps", eps)
+    e.prepare()
+    e.adjoint_setup(dlo, dhi, mz, mlo, mhi, window, ckpt)
+    return e
+
+
+def _design(eps, dlo, dhi):
+    return eps[dlo[0]:dhi[0], dlo[1]:dhi[1], dlo[2]:dhi[2]].copy()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_gpu_adjoint_vs_cpu_adjoint(case, engine_lib):
+    args, dlo, dhi, mz, mlo, mhi, window = case
+    cfg = parse_args(args.split())
+    eps = _eps(cfg, 5)
+    Jo, go = oracle_adjoint(cfg, eps, dlo, dhi, mz, mlo, mhi, window, cfg.time_steps)
+    say(f"[gpu] adjoint vs oracle: {args[:50]}")
+    e = _engine(cfg, eps, dlo, dhi, mz, mlo, mhi, window)
+    J = e.forward(_design(eps, dlo, dhi))
+    g = e.backward(1.0)
+    assert abs(J - Jo) <= 1e-12 * Jo
+    assert rel_l2(g, go) <= 1e-9
+    e.close()
+
+
+@pytest.mark.parametrize("case", CASES[:1])
+def test_gpu_adjoint_vs_finite_differences(case, engine_lib):
+    args, dlo, dhi, mz, mlo, mhi, window = case
+    cfg = parse_args(args.split())
+    eps = _eps(cfg, 5)
+    say("[gpu] adjoint vs finite differences")
+    e = _engine(cfg, eps, dlo, dhi, mz, mlo, mhi, window)
+    d = _design(eps, dlo, dhi)
+    e.forward(d)
+    g = e.backward(1.0)
+    rng = np.random.default_rng(3)
+    scale = np.max(np.abs(g))
+    for _ in range(6):
+        v = tuple(int(rng.integers(0, d.shape[a])) for a in range(3))
+        h = 1e-4 * d[v]
+        dp, dm = d.copy(), d.copy()
+        dp[v] += h
+        dm[v] -= h
+        fd = (e.forward(dp) - e.forward(dm)) / (2 * h)
+        assert abs(g[v] - fd) <= 1e-6 * max(abs(fd), 1e-3 * scale), (v, g[v], fd)
+    e.close()
+
+
+@pytest.mark.parametrize("kernel", ["split", "fused"])
+def test_checkpointed_backward_matches_traced(kernel, engine_lib, monkeypatch):
+    args, dlo, dhi, mz, mlo, mhi, window = CASES[0]
+    cfg = parse_args(args.split())
+    eps = _eps(cfg, 9)
+    monkeypatch.setenv("YEE_KERNEL", kernel)
+    say(f"[gpu] checkpointed vs traced backward ({kernel})")
+    a = _engine(cfg, eps, dlo, dhi, mz, mlo, mhi, window, ckpt=0)
+    b = _engine(cfg, eps, dlo, dhi, mz, mlo, mhi, window, ckpt=7)
+    d = _design(eps, dlo, dhi)
+    Ja, Jb = a.forward(d), b.forward(d)
+    assert Ja == Jb
+    ga, gb = a.backward(2.5), b.backward(2.5)
+    assert np.array_equal(ga, gb)
+    a.close()
+    b.close()
+
+
+def test_fp32_gradient_vs_fp64(engine_lib):
+    args, dlo, dhi, mz, mlo, mhi, window = CASES[0]
+    cfg = parse_args(args.split())
+    eps = _eps(cfg, 5)
+    say("[gpu] adjoint fp32 vs fp64")
+    g = []
+    for prec in (8, 4):
+        e = _engine(parse_args(args.split()), eps, dlo, dhi, mz, mlo, mhi, window, precision=prec)
+        e.forward(_design(eps, dlo, dhi))
+        g.append(e.backward(1.0))
+        e.close()
+    assert rel_l2(g[1], g[0]) <= 1e-3
