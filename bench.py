@@ -135,6 +135,77 @@ def run_cfg4(args) -> dict:
             "ms_per_step": t_batch * 1e3 / args.batch_steps}
 
 
+def cfg5_adjoint(precision=4, n=256, steps=800):
+    """256^3, PML 16, TF/SF plane wave lambda = 32 cells (dx 0.000625, lambda 0.02), 800 steps;
+    design 64^3 centred, eps U[1, 4] per voxel (seeded); J = sum of Ez^2 over the 128 x 128
+    centre z-plane over the last 200 steps."""
+    c = Config(scheme=SCHEME_DIM3, precision=precision, time_steps=steps, size=[n, n, n], pml=[16] * 3,
+               tfsf_left=[24] * 3, tfsf_right=[24] * 3, dx=stof("0.000625"), wavelength=stof("0.02"),
+               courant=stof("0.5"), use_pml=True, use_tfsf=True)
+    lo, hi = n // 2 - 32, n // 2 + 32
+    spec = dict(design_lo=(lo, lo, lo), design_hi=(hi, hi, hi), monitor_z=n // 2, monitor_lo=(n // 2 - 64, n // 2 - 64),
+                monitor_hi=(n // 2 + 64, n // 2 + 64), window=200)
+    design = np.random.default_rng(0xAD701).uniform(1.0, 4.0, size=(hi - lo,) * 3)
+    return c, spec, design
+
+
+def run_cfg5(args) -> dict:
+    """Forward + backward time over plain forward time; FP64 gradient vs central differences."""
+    from fdtd3d_amd.engine import Engine
+    out = {}
+    c, spec, design = cfg5_adjoint(precision=4)
+    # plain forward: the same run without recording
+    e = Engine(c)
+    eps = np.ones(c.shape, dtype=np.float32)
+    lo, hi = spec["design_lo"], spec["design_hi"]
+    eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = design
+    e.set_material("eps", eps)
+    e.prepare()
+    e.step(c.time_steps)
+    e.sync()
+    out["plain_forward_ms"] = e.last_step_ms() * c.time_steps
+    e.close()
+    for ck in (50, 0):
+        e = Engine(c)
+        e.set_material("eps", eps)
+        e.prepare()
+        e.adjoint_setup(spec["design_lo"], spec["design_hi"], spec["monitor_z"], spec["monitor_lo"],
+                        spec["monitor_hi"], spec["window"], checkpoint_every=ck)
+        J = e.forward(design)
+        g32 = e.backward(1.0)
+        tm = e.adjoint_timing()
+        key = f"checkpoint{ck}" if ck else "trace"
+        out[key] = {"J": J, "forward_ms": round(tm["forward_ms"], 2), "backward_ms": round(tm["backward_ms"], 2),
+                    "ratio": round((tm["forward_ms"] + tm["backward_ms"]) / out["plain_forward_ms"], 3)}
+        print(f"[bench] cfg5 {key}: {out[key]}", file=sys.stderr, flush=True)
+        e.close()
+    # FP64: gradient vs central finite differences on 16 seeded voxels, and the FP32 gradient error
+    c64, _, _ = cfg5_adjoint(precision=8)
+    e = Engine(c64)
+    e.set_material("eps", eps.astype(np.float64))
+    e.prepare()
+    e.adjoint_setup(spec["design_lo"], spec["design_hi"], spec["monitor_z"], spec["monitor_lo"], spec["monitor_hi"],
+                    spec["window"], checkpoint_every=50)
+    e.forward(design)
+    g64 = e.backward(1.0)
+    rng = np.random.default_rng(0xFD16)
+    errs = []
+    for _ in range(args.fd_voxels):
+        v = tuple(int(rng.integers(0, design.shape[a])) for a in range(3))
+        h = 1e-3 * design[v]
+        dp, dm = design.copy(), design.copy()
+        dp[v] += h
+        dm[v] -= h
+        fd = (e.forward(dp) - e.forward(dm)) / (2 * h)
+        errs.append(abs(g64[v] - fd) / max(abs(fd), 1e-30))
+        print(f"[bench] fd voxel {v}: adjoint {g64[v]:.6e} fd {fd:.6e} rel {errs[-1]:.2e}", file=sys.stderr, flush=True)
+    e.close()
+    out["fd_max_rel_err_fp64"] = float(max(errs)) if errs else None
+    out["fd_voxels"] = len(errs)
+    out["fp32_vs_fp64_grad_rel_l2"] = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    return out
+
+
 def _plain(c: Config) -> Config:
     c.use_pml = False
     c.use_tfsf = False
@@ -206,9 +277,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4-batch"])
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4-batch", "cfg5-adjoint"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000, help="time steps of the cfg4-batch workload")
+    ap.add_argument("--fd-voxels", type=int, default=16, help="finite-difference voxels of the cfg5-adjoint workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the run, copy a buffer of this many MiB device-to-device (a plain copy "
                          "of known size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -236,6 +308,21 @@ def main():
                                       "Gaussian-modulated Ez source (t0 = 320, sigma = 80 steps), eps blobs in [1, 9]"},
             "sequential_s": round(r["t_seq"], 3), "batched_s": round(r["t_batch"], 3),
             "speedup_vs_sequential": round(r["t_seq"] / r["t_batch"], 2), "tuning": r["tuning"],
+        }
+        print(json.dumps(line), flush=True)
+        return
+
+    if args.workload == "cfg5-adjoint":
+        r = run_cfg5(args)
+        line = {
+            "metric": "adjoint/forward time ratio: (forward + backward) / plain forward, 3D 256^3 FP32",
+            "value": r["checkpoint50"]["ratio"], "unit": "ratio", "n_gpus": 1, "steps": 800, "warmup": 0,
+            "ms_per_step": round(r["plain_forward_ms"] / 800, 4), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "cfg5-adjoint", "description": "256^3 PML 16, TF/SF lambda 32 cells, 800 steps, "
+                       "design 64^3 eps U[1,4] seed 0xAD701, J = sum Ez^2 on the 128x128 centre z-plane over "
+                       "the last 200 steps, checkpoint every 50 steps"},
+            "detail": r,
         }
         print(json.dumps(line), flush=True)
         return

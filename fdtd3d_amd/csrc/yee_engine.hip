@@ -656,11 +656,11 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (e != hipSuccess) return bail (fail (YEE_EDEVICE, std::string ("hipStreamCreate: ") + hipGetErrorString (e)));
   if (hipEventCreate (&h->ev[0]) != hipSuccess || hipEventCreate (&h->ev[1]) != hipSuccess)
     return bail (fail (YEE_EDEVICE, "hipEventCreate failed"));
-  // 3D runs without Drude terms can use the fused step kernel on the PML-free box
-  // (double-buffered state); TMz and Drude runs update in place with the half-step kernels.
-  // YEE_KERNEL=fused / split in the environment selects the 3D path (A/B checks).
+  // 3D runs without Drude terms use the fused step kernel on the PML-free box (double-buffered
+  // state); TMz and Drude runs update in place with the half-step kernels.  YEE_KERNEL=split
+  // in the environment selects the half-step kernels for 3D as well (A/B checks).
   const char *kern = std::getenv ("YEE_KERNEL");
-  const bool want_fused = kern && std::strcmp (kern, "fused") == 0;
+  const bool want_fused = !(kern && std::strcmp (kern, "split") == 0);
   h->fused = false;
   if (h->dim == 3 && h->mode <= 1 && h->n[2] <= 1024 && want_fused)
   {

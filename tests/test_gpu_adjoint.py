@@ -5,7 +5,8 @@ Bars:
     L2 <= 1e-9 (the two forwards differ by rounding: the engine updates E/H directly outside
     the PML slabs);
   - FP64 GPU gradient vs central finite differences of the GPU forward (h = 1e-4 eps):
-    relative error <= 1e-6 per voxel (relative to max(|fd|, 1e-3 max|grad|));
+    relative error <= 1e-5 per voxel (relative to max(|fd|, 1e-3 max|grad|); the O(h^2)
+    truncation error of the difference is about 1e-6);
   - checkpointed and traced backward passes: bit-identical gradients;
   - FP32 gradient vs FP64 gradient: relative L2 <= 1e-3;
   - J of yee_forward equals J of the oracle to 1e-12 (FP64).
@@ -87,7 +88,7 @@ def test_gpu_adjoint_vs_finite_differences(case, engine_lib):
         dp[v] += h
         dm[v] -= h
         fd = (e.forward(dp) - e.forward(dm)) / (2 * h)
-        assert abs(g[v] - fd) <= 1e-6 * max(abs(fd), 1e-3 * scale), (v, g[v], fd)
+        assert abs(g[v] - fd) <= 1e-5 * max(abs(fd), 1e-3 * scale), (v, g[v], fd)
     e.close()
 
 
