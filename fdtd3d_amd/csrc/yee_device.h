@@ -181,6 +181,34 @@ __device__ double incident_at (const StepArgs<T> &a, int i, int j, int k, const 
 
 // One operand pair of calculateTFSF.  E components (E_CENTERED) correct the operand opposite
 // to the border side; H components the operand on the border side, with the sign flipped.
+// The correction of one operand pair as values to subtract (in FP64) from the lo / hi slot:
+// 0 for a slot that is not corrected.  Returns false when neither slot is.
+template <typename T, int COMP, int PAIR, int DIM>
+__device__ __forceinline__ bool tfsf_pair_diff (const StepArgs<T> &a, int i, int j, int k, double &d_lo, double &d_hi)
+{
+  constexpr bool h = COMP >= 3;
+  constexpr int OPP = Tab::opp[COMP][PAIR];
+  d_lo = d_hi = 0.0;
+  const bool lo = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR], DIM> (a, i, j, k);
+  const bool hi = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR + 1], DIM> (a, i, j, k);
+  if (!lo && !hi) return false;
+  const T *inc = h ? a.tg->einc : a.tg->hinc[a.hinc_sel];
+  const double ddiff = h ? 0.0 : 0.5;
+  const bool use_hi_slot = h ? !lo : lo;
+  constexpr int s_lo = 2 * PAIR, s_hi = 2 * PAIR + 1;
+  double diff;
+  if (use_hi_slot)
+    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_hi][0], j + Tab::off[COMP][s_hi][1],
+                                     k + (DIM == 3 ? Tab::off[COMP][s_hi][2] : 0), inc, ddiff);
+  else
+    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_lo][0], j + Tab::off[COMP][s_lo][1],
+                                     k + (DIM == 3 ? Tab::off[COMP][s_lo][2] : 0), inc, ddiff);
+  if (h) diff *= -1.0;
+  if (use_hi_slot) d_hi = diff;
+  else d_lo = diff;
+  return true;
+}
+
 template <typename T, int COMP, int PAIR, int DIM>
 __device__ __forceinline__ void tfsf_pair (const StepArgs<T> &a, int i, int j, int k, T &v_lo, T &v_hi)
 {

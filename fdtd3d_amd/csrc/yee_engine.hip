@@ -30,6 +30,7 @@ template <typename T> __global__ void incident_step (T *, const T *, T *, int64_
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
 template <typename T, int BY> __global__ void step_3d_fused (StepArgs<T>);
+template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 
 static thread_local std::string g_error;
 
@@ -114,6 +115,9 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   a.tab_len = h->tab_len;
   a.tfsf = h->use_tfsf () ? 1 : 0;
   a.tg = (const TfsfGeom<T> *) h->tg;
+  a.zcorr = (const double *) h->zcorr;
+  a.zc_n[0] = (int) h->zc_n[0];
+  a.zc_n[1] = (int) h->zc_n[1];
   // the E half-step of step t reads HInc after step t-1, kept in buffer (t+1) & 1
   a.hinc_sel = parity;
   return a;
@@ -131,10 +135,14 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
     }
   if (h->dim == 3)
   {
-    dim3 block (64, 4, 1);
     const int64_t ext[3] = { a.run_hi[0] - a.run_lo[0], a.run_hi[1] - a.run_lo[1], a.run_hi[2] - a.run_lo[2] };
     if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) return YEE_OK;
-    dim3 grid ((unsigned) ((ext[2] + 63) / 64), (unsigned) ((ext[1] + 3) / 4), (unsigned) ext[0]);
+    // 64 z cells x 4 rows; boxes thin in z (PML z slabs) take 16 z cells x 16 rows, boxes one
+    // row high (TF/SF y faces) 64 x 1, so that few lanes idle
+    const unsigned bx = ext[2] <= 32 ? 16 : 64;
+    const unsigned by = (unsigned) std::min<int64_t> (256 / bx, std::max<int64_t> (1, ext[1]));
+    dim3 block (bx, by, 1);
+    dim3 grid ((unsigned) ((ext[2] + bx - 1) / bx), (unsigned) ((ext[1] + by - 1) / by), (unsigned) ext[0]);
     switch (h->mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
@@ -177,8 +185,8 @@ double source_value (const yee_handle *h, bool e_half, int32_t t)
   return v;
 }
 
-// tfsf_near of the kernels (yee_kernels.hip): the cell lies in [L-1, R] on every axis and on
-// one of the planes L-1, R-1, R of some axis.
+// tfsf_xy_near of the kernels (yee_kernels.hip): the cell lies in [L-1, R] on every axis and
+// on one of the planes L-1, R-1, R of the x or y axis.
 bool tfsf_near_host (const yee_handle *h, const int64_t p[3])
 {
   bool face = false;
@@ -186,7 +194,7 @@ bool tfsf_near_host (const yee_handle *h, const int64_t p[3])
   {
     const int64_t L = h->cfg.tfsf_left[ax], R = h->n[ax] - h->cfg.tfsf_right[ax];
     if (p[ax] < L - 1 || p[ax] > R) return false;
-    face = face || p[ax] == L - 1 || p[ax] == R - 1 || p[ax] == R;
+    face = face || (ax < 2 && (p[ax] == L - 1 || p[ax] == R - 1 || p[ax] == R));
   }
   return face;
 }
@@ -276,14 +284,15 @@ int slab_boxes (const yee_handle *h, int64_t lo[6][3], int64_t hi[6][3])
   return m;
 }
 
-// The cells next to the TF/SF faces (tfsf_near): for each axis the planes L-1, R-1 and R,
-// spanning [L-1, R] on the other axes, clipped to the interior.  Boxes may overlap; a cell
-// updated twice from the old buffer gets the same value both times.
+// The cells next to the x and y faces of the TF/SF box (tfsf_xy_near): for x and y the planes
+// L-1, R-1 and R, spanning [L-1, R] on the other axes, clipped to the interior (the z faces
+// are corrected inside the fused kernel).  Boxes may overlap; a cell updated twice from the
+// old buffer gets the same value both times.
 int face_boxes (const yee_handle *h, int64_t lo[9][3], int64_t hi[9][3])
 {
   if (!h->use_tfsf ()) return 0;
   int m = 0;
-  for (int ax = 0; ax < 3; ++ax)
+  for (int ax = 0; ax < 2; ++ax)
   {
     const int64_t L = h->cfg.tfsf_left[ax], R = h->n[ax] - h->cfg.tfsf_right[ax];
     const int64_t planes[3] = { L - 1, R - 1, R };
@@ -321,6 +330,14 @@ int fused_step (yee_handle *h, int32_t t, int prev)
       if ((rc = launch_half<T> (h, true, prev, lo[s], hi[s]))) return rc;
   }
   if ((rc = launch_sources<T> (h, true, t, dst, true))) return rc;
+  if (h->zcorr)
+  {
+    StepArgs<T> a = make_args<T> (h, prev);
+    const int64_t n = 3 * h->zc_n[0] * h->zc_n[1];
+    hipLaunchKernelGGL ((tfsf_ztable<T>), dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, h->stream, a,
+                       (double *) h->zcorr);
+    HIP_TRY (hipGetLastError ());
+  }
   {
     StepArgs<T> a = make_args<T> (h, prev, kRoleFused);
     for (int ax = 0; ax < 3; ++ax)
@@ -689,6 +706,13 @@ int yee_create (const yee_config *cfg, yee_handle **out)
     h->fused_nchunks = (int) ((planes + h->fused_lx - 1) / h->fused_lx);
     h->fused_threads = (int) ((h->n[2] + 63) / 64 * 64);
   }
+  if (h->fused && h->use_tfsf ())
+  {
+    // z-face TF/SF correction table of the fused kernel: 2 parts x 3 faces x [L-1, R]^2 x 4
+    for (int ax = 0; ax < 2; ++ax)
+      h->zc_n[ax] = (h->n[ax] - cfg->tfsf_right[ax]) - cfg->tfsf_left[ax] + 2;
+    if ((rc = alloc_zero (h, &h->zcorr, (size_t) (6 * h->zc_n[0] * h->zc_n[1] * 4) * sizeof (double)))) return bail (rc);
+  }
   const int nbuf = h->fused ? 2 : 1;
   for (int bsel = 0; bsel < nbuf; ++bsel)
     for (int c = 0; c < 6; ++c)
@@ -724,6 +748,7 @@ void yee_destroy (yee_handle *h)
   adjoint_free (h);
   for (void *p : h->grids) (void) hipFree (p);
   if (h->tg) (void) hipFree (h->tg);
+  if (h->zcorr) (void) hipFree (h->zcorr);
   if (h->pml) (void) hipFree (h->pml);
   if (h->einc) (void) hipFree (h->einc);
   if (h->hinc[0]) (void) hipFree (h->hinc[0]);

@@ -48,6 +48,8 @@ struct yee_handle
   // padded grid allocations (every grid-sized array; the views above point inside them)
   std::vector<void *> grids;
   void *tg = nullptr;   // TfsfGeom<T> in device memory
+  void *zcorr = nullptr;              // z-face TF/SF corrections of the fused kernel (FP64)
+  int64_t zc_n[2] = { 0, 0 };
 
   void *mat[6] = {};
   double mat_u[6] = { 1, 1, 0, 0, 0, 0 };

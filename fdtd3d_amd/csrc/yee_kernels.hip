@@ -263,11 +263,76 @@ __device__ __forceinline__ T direct_value (const StepArgs<T> &a, T f_old, T tmp,
   return f_old * (T) 1.0 + tmp * cb;
 }
 
+// A cell next to an x or y face of the TF/SF box: inside [L-1, R] on every axis, with x or y
+// on one of the planes L-1, R-1, R.  (Cells next to z faces only are corrected in the fused
+// kernel from a per-step table, tfsf_ztable.)
+template <typename T>
+__device__ __forceinline__ bool tfsf_xy_near (const StepArgs<T> &a, int x, int j, int k)
+{
+  const int p[3] = { x, j, k };
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    if (p[ax] < a.tl[ax] - 1 || p[ax] > a.tr[ax]) return false;
+  return x == a.tl[0] - 1 || x == a.tr[0] - 1 || x == a.tr[0] || j == a.tl[1] - 1 || j == a.tr[1] - 1 || j == a.tr[1];
+}
+
 // Does the fused kernel update cell (x, j, k)?  (xbox, kbox: the x and z box tests.)
 template <typename T>
 __device__ __forceinline__ bool fused_owns (const StepArgs<T> &a, bool xbox, int x, int j, int k, bool kbox)
 {
-  return xbox && kbox && j >= a.run_lo[1] && j < a.run_hi[1] && !(a.tfsf && tfsf_near<T, 3> (a, x, j, k));
+  return xbox && kbox && j >= a.run_lo[1] && j < a.run_hi[1] && !(a.tfsf && tfsf_xy_near<T> (a, x, j, k));
+}
+
+// z face (0: k = L-1, 1: R-1, 2: R) of a lane, or -1.
+template <typename T>
+__device__ __forceinline__ int zface_of (const StepArgs<T> &a, int k)
+{
+  return k == a.tl[2] - 1 ? 0 : (k == a.tr[2] - 1 ? 1 : (k == a.tr[2] ? 2 : -1));
+}
+
+// One value of the z-face table for the LDS staging of the fused kernel: thread t of the
+// first 24 (BY + 1) loads entry (part, face, row, slot) of plane xe (E part) or xe - 1 (H part)
+// of the strip's rows; 0 outside the table.
+template <typename T, int BY>
+__device__ __forceinline__ double zstage_load (const StepArgs<T> &a, int t, int xe, int j0)
+{
+  constexpr int per_part = 3 * (BY + 1) * 4;
+  if (!a.tfsf || t >= 2 * per_part) return 0.0;
+  const int part = t / per_part, rem = t % per_part;
+  const int f = rem / (4 * (BY + 1)), row = (rem / 4) % (BY + 1), slot = rem % 4;
+  const int x = part == 0 ? xe : xe - 1, j = j0 + row;
+  if (x < a.tl[0] - 1 || x > a.tr[0] || j < a.tl[1] - 1 || j > a.tr[1]) return 0.0;
+  return a.zcorr[((((int64_t) part * 3 + f) * a.zc_n[0] + (x - (a.tl[0] - 1))) * a.zc_n[1] + (j - (a.tl[1] - 1))) * 4 + slot];
+}
+
+// Subtracts a tabulated correction pair (FP64) from two operands: (T) ((double) v - d), which is
+// the identity for an uncorrected slot (d = 0).
+template <typename T>
+__device__ __forceinline__ void zface_apply (const double *c, T &v_lo, T &v_hi)
+{
+  v_lo = (T) ((double) v_lo - c[0]);
+  v_hi = (T) ((double) v_hi - c[1]);
+}
+
+// Per-step table of the TF/SF corrections on the z faces of the TF/SF box: part 0 for the E
+// update (Ex pair BACK/FRONT, Ey pair BACK/FRONT), part 1 for the H update (Hx, Hy likewise).
+// The same predicates and incident interpolation as calculateTFSF (tfsf_pair_diff).
+template <typename T>
+__global__ void tfsf_ztable (StepArgs<T> a, double *out)
+{
+  const int64_t u = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t) a.zc_n[0] * a.zc_n[1];
+  if (u >= 3 * per) return;
+  const int f = (int) (u / per);
+  const int ix = (int) ((u % per) / a.zc_n[1]), iy = (int) (u % a.zc_n[1]);
+  const int x = a.tl[0] - 1 + ix, j = a.tl[1] - 1 + iy;
+  const int k = f == 0 ? a.tl[2] - 1 : (f == 1 ? a.tr[2] - 1 : a.tr[2]);
+  double *e = out + (((int64_t) 0 * 3 + f) * per + (int64_t) ix * a.zc_n[1] + iy) * 4;
+  double *h = out + (((int64_t) 1 * 3 + f) * per + (int64_t) ix * a.zc_n[1] + iy) * 4;
+  tfsf_pair_diff<T, 0, 1, 3> (a, x, j, k, e[0], e[1]);   // Ex: Hy operands (BACK, FRONT)
+  tfsf_pair_diff<T, 1, 0, 3> (a, x, j, k, e[2], e[3]);   // Ey: Hx operands (BACK, FRONT)
+  tfsf_pair_diff<T, 3, 0, 3> (a, x, j, k, h[0], h[1]);   // Hx: Ey operands (BACK, FRONT)
+  tfsf_pair_diff<T, 4, 1, 3> (a, x, j, k, h[2], h[3]);   // Hy: Ex operands (BACK, FRONT)
 }
 
 // Lane byte offsets of the rows j0-1 .. j0+BY relative to row j0-1 (kept opaque, so that all
@@ -368,18 +433,26 @@ struct EPlane
 template <typename T, int BY>
 __device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const FusedPos &g, const PlaneLoads<T, BY> &L,
                                            const T (&hy_prev)[BY + 1], const T (&hz_prev)[BY + 1],
-                                           const T (&ep0)[BY + 1], EPlane<T, BY> &o)
+                                           const T (&ep0)[BY + 1], const double (*zl)[BY + 1][4], EPlane<T, BY> &o)
 {
+  const int zf = zface_of (a, g.k);
+  const bool zrows = a.tfsf && zf >= 0 && x >= a.tl[0] - 1 && x <= a.tr[0];
   const int64_t base = (int64_t) x * a.sx;
 #pragma unroll
   for (int r = 0; r <= BY; ++r)
   {
     const int j = g.j0 + r;
-    const T vx[4] = { L.hzr[r], L.hzr[r + 1], L.hym[r], L.hy[r] };
+    T vx[4] = { L.hzr[r], L.hzr[r + 1], L.hym[r], L.hy[r] };
     const T vz[4] = { hy_prev[r], L.hy[r], L.hxr[r], L.hxr[r + 1] };
-    const T vy[4] = { L.hxm[r], L.hxr[r + 1], hz_prev[r], L.hzr[r + 1] };
+    T vy[4] = { L.hxm[r], L.hxr[r + 1], hz_prev[r], L.hzr[r + 1] };
     const T e = ep0[r];
     const bool own = L.own[r];
+    if (own && zrows && j >= a.tl[1] - 1 && j <= a.tr[1])
+    {
+      // z-face TF/SF corrections, staged in LDS (Ex: Hy operands, Ey: Hx operands)
+      zface_apply (&zl[zf][r][0], vx[2], vx[3]);
+      zface_apply (&zl[zf][r][2], vy[0], vy[1]);
+    }
     const int64_t qt = base + (int64_t) j * a.sy + g.k;
     T v = L.ex[r];
     if (own)
@@ -435,6 +508,8 @@ template <typename T, int BY>
 __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
 {
   __shared__ T xch[2][16][BY][2];   // Ex, Ey after the step at lane 0 of every wave, by plane parity
+  // z-face TF/SF corrections of the strip, by iteration parity: [parity][part][face][row][slot]
+  __shared__ double zl[2][2][3][BY + 1][4];
 
   FusedPos g;
   g.k = (int) threadIdx.x;
@@ -476,8 +551,16 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     }
     PlaneLoads<T, BY> L;
     e_load<T, BY> (a, xa, g, ro, L);
+    if (a.tfsf)
+    {
+      // corrections of plane xa (E part) into zl[0], then those of the first iteration
+      constexpr int n = 2 * 3 * (BY + 1) * 4;
+      if (g.k < n) (&zl[0][0][0][0][0])[g.k] = zstage_load<T, BY> (a, g.k, xa, g.j0);
+      __syncthreads ();
+    }
     EPlane<T, BY> o;
-    e_compute<T, BY> (a, xa, g, L, hyp, hzp, e0, o);
+    e_compute<T, BY> (a, xa, g, L, hyp, hzp, e0, zl[0][0], o);
+    const double zpre = zstage_load<T, BY> (a, g.k, min (xa + 1, xb), g.j0);
     e_store<T, BY> (a, xa, g, ro, o);
     e_load<T, BY> (a, min (xa + 1, xb), g, ro, nxt);
 #pragma unroll
@@ -500,6 +583,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
         xch[0][wave][r][1] = eyc[r];
       }
     }
+    if (a.tfsf && g.k < 2 * 3 * (BY + 1) * 4) (&zl[1][0][0][0][0])[g.k] = zpre;
     __syncthreads ();
   }
 
@@ -509,9 +593,11 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     const int i = x - 1;   // H plane
     const PlaneLoads<T, BY> L = nxt;
     e_load<T, BY> (a, min (x + 1, xb), g, ro, nxt);   // prefetch the next plane
-    EPlane<T, BY> o;
-    e_compute<T, BY> (a, x, g, L, hyc, hzc, ep0, o);
     par ^= 1;
+    // z-face corrections of the next iteration (written to LDS before the barrier)
+    const double zpre = zstage_load<T, BY> (a, g.k, min (x + 1, xb), g.j0);
+    EPlane<T, BY> o;
+    e_compute<T, BY> (a, x, g, L, hyc, hzc, ep0, zl[par][0], o);
     if (lane == 0)
     {
 #pragma unroll
@@ -540,6 +626,8 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     const T *M0 = a.mu ? plane_row (a, a.mu, i, g) : nullptr;
     const T *M1 = a.mu ? plane_row (a, a.mu, i + 1, g) : nullptr;   // i + 1 <= nx - 2
     const bool ibox = i >= a.run_lo[0] && i < a.run_hi[0];
+    const int hzf = zface_of (a, g.k);
+    const bool hzrows = a.tfsf && hzf >= 0 && i >= a.tl[0] - 1 && i <= a.tr[0];
     bool hown[BY + 1];
 #pragma unroll
     for (int r = 0; r < BY; ++r)
@@ -562,9 +650,15 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     for (int r = 0; r < BY; ++r)
     {
       const int j = g.j0 + r;
-      const T vx[4] = { eyc[r], eyk[r], ezc[r], ezc[r + 1] };
-      const T vy[4] = { ezc[r], o.ez[r], exc[r], exk[r] };
+      T vx[4] = { eyc[r], eyk[r], ezc[r], ezc[r + 1] };
+      T vy[4] = { ezc[r], o.ez[r], exc[r], exk[r] };
       const T vz[4] = { exc[r], exc[r + 1], eyc[r], o.ey[r] };
+      if (hown[r] && hzrows && j >= a.tl[1] - 1 && j <= a.tr[1])
+      {
+        // z-face TF/SF corrections (Hx: Ey operands, Hy: Ex operands)
+        zface_apply (&zl[par][1][hzf][r][0], vx[0], vx[1]);
+        zface_apply (&zl[par][1][hzf][r][2], vy[2], vy[3]);
+      }
       const int64_t qt = (int64_t) i * a.sx + (int64_t) j * a.sy + g.k;
       T v = direct_value<T, 3> (a, hxc[r], curl (vx), mxm[r]);
       if ((a.src_mask & 8u) && qt == a.src_q) v = a.src_h;
@@ -603,6 +697,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
       hzc[r] = o.hz[r];
       ep0[r] = o.ep1[r];
     }
+    if (a.tfsf && g.k < 2 * 3 * (BY + 1) * 4) (&zl[par ^ 1][0][0][0][0])[g.k] = zpre;
     __syncthreads ();
   }
 }
@@ -867,6 +962,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void batch_tmz_sources<T> (const StepArgs<T> *, int, const int *, const T *, int); \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
   template __global__ void step_3d_fused<T, FUSED_BY (T)> (StepArgs<T>);                   \
+  template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *);
 
