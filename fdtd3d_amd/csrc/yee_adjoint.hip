@@ -9,7 +9,8 @@
 //             with cb = dt / (m mu0 dx), kp = (p4 / mod) / dd5, km = (m4 / mod) / dd5 (the
 //             coefficients of cell_value), a = adjoint of the cell's new value;
 //   E cells:  a = aE - curl_E(w_H) first (the transpose of the H curl that read E), then the
-//             same per-cell rule; afterwards aH -= curl_H(w_E).
+//             same per-cell rule; aH -= curl_H(w_E) is applied at the start of the next
+//             (earlier) reverse step, fused with its H transpose.
 // A hard point source replaces the new value of its cell, so its adjoint is dropped.
 // J = sum over the last `window` steps of the sum of Ez^2 on a monitor rectangle; its
 // derivative 2 Ez is added to aEz before the reverse step of each of those steps.
@@ -74,7 +75,9 @@ __device__ __forceinline__ T adj_cell (const AdjArgs<T> &g, int64_t q, const int
   return w;
 }
 
-// Transpose of the H update of one step (H cells of [1, N-1)^3).
+// aH -= curl_H(w_E) of the previous reverse step (the transpose of the E curl that read H;
+// w_E is zero before the first one), then the transpose of this step's H update (H cells of
+// [1, N-1)^3).
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void adj_h (AdjArgs<T> g)
 {
@@ -96,7 +99,12 @@ __global__ __launch_bounds__ (256) void adj_h (AdjArgs<T> g)
     my = avg4 (m[q], m[q + 1], m[q + sx], m[q + sx + 1]);
     mz = avg4 (m[q], m[q + sy], m[q + sx], m[q + sx + sy]);
   }
-  const T a3 = g.af[3][q], a4 = g.af[4][q], a5 = g.af[5][q];
+  const T *ex = g.w[0], *ey = g.w[1], *ez = g.w[2];
+  // the H-update stencils applied to w_E (Hx: Ey BACK/FRONT, Ez DOWN/UP; cyclic)
+  const T vx[4] = { ey[q], ey[q + 1], ez[q], ez[q + sy] };
+  const T vy[4] = { ez[q], ez[q + sx], ex[q], ex[q + 1] };
+  const T vz[4] = { ex[q], ex[q + sy], ey[q], ey[q + sx] };
+  const T a3 = g.af[3][q] - curl (vx), a4 = g.af[4][q] - curl (vy), a5 = g.af[5][q] - curl (vz);
   g.w[3][q] = adj_cell<T, 3> (g, q, p, a3, mx, special);
   g.w[4][q] = adj_cell<T, 4> (g, q, p, a4, my, special);
   g.w[5][q] = adj_cell<T, 5> (g, q, p, a5, mz, special);
@@ -133,28 +141,6 @@ __global__ __launch_bounds__ (256) void adj_e (AdjArgs<T> g)
   g.w[0][q] = adj_cell<T, 0> (g, q, p, a0, ex, special);
   g.w[1][q] = adj_cell<T, 1> (g, q, p, a1, ey, special);
   g.w[2][q] = adj_cell<T, 2> (g, q, p, a2, ez, special);
-}
-
-// aH -= curl_H(w_E) (H cells of [1, N-1)^3).
-template <typename T>
-__global__ __launch_bounds__ (256) void adj_h_gather (AdjArgs<T> g)
-{
-  const StepArgs<T> &a = g.a;
-  const int k = 1 + (int) (blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int) (blockIdx.y * blockDim.y + threadIdx.y);
-  const int i = 1 + (int) blockIdx.z;
-  if (j >= a.ny - 1 || k >= a.nz - 1) return;
-  const int64_t sx = a.sx;
-  const int sy = a.sy;
-  const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
-  const T *ex = g.w[0], *ey = g.w[1], *ez = g.w[2];
-  // the H-update stencils applied to w_E (Hx: Ey BACK/FRONT, Ez DOWN/UP; cyclic)
-  const T vx[4] = { ey[q], ey[q + 1], ez[q], ez[q + sy] };
-  const T vy[4] = { ez[q], ez[q + sx], ex[q], ex[q + 1] };
-  const T vz[4] = { ex[q], ex[q + sy], ey[q], ey[q + sx] };
-  g.af[3][q] -= curl (vx);
-  g.af[4][q] -= curl (vy);
-  g.af[5][q] -= curl (vz);
 }
 
 // aEz += 2 Ez on the monitor rectangle (mon: that step's recorded Ez, [x][y]).
@@ -397,7 +383,6 @@ int reverse_step (yee_handle *h, int32_t t, int slot_cur, int slot_prev)
                      (const T *) s->trace + (size_t) slot_cur * 3 * nc, (const T *) s->trace + (size_t) slot_prev * 3 * nc,
                      (int) s->sp.design_lo[0], (int) s->sp.design_lo[1], (int) s->sp.design_lo[2], (int) s->dn[0],
                      (int) s->dn[1], (int) s->dn[2], s->grad);
-  hipLaunchKernelGGL ((adj_h_gather<T>), grid, block, 0, h->stream, g);
   HIP_TRY (hipGetLastError ());
   return YEE_OK;
 }

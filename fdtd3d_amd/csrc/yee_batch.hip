@@ -240,13 +240,16 @@ struct Cand
 {
   int sel, per_cu;
 };
-const Cand kCands[] = { { 1, 4 }, { 2, 4 }, { 3, 2 }, { 1, 8 }, { 2, 8 }, { 3, 4 }, { 1, 16 }, { 2, 16 } };
+// per_cu = 0: one tile per workgroup (grid = number of tiles, the hardware dispatcher is the
+// queue); otherwise that many persistent workgroups per CU walk the tile table.
+const Cand kCands[] = { { 2, 0 }, { 3, 0 }, { 1, 0 }, { 2, 8 }, { 3, 4 }, { 1, 16 }, { 2, 16 }, { 0, 0 } };
 constexpr int kNumCands = (int) (sizeof kCands / sizeof kCands[0]);
 constexpr int kTuneSteps = 3;
 constexpr int kGraphSteps = 16;
 
 int cand_grid (const yee_batch *b, const Cand &c)
 {
+  if (c.per_cu == 0) return std::max (1, b->ntiles[c.sel]);
   return std::max (1, std::min (b->ntiles[c.sel], c.per_cu * b->ncu));
 }
 

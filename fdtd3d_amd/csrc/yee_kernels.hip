@@ -18,17 +18,26 @@
 
 namespace yee {
 
+// D/B state of a special cell before the update: D/B, D/B one step older, D1/B1 and D1/B1 one
+// step older (the last three with Drude terms only).
+template <typename T>
+struct Old
+{
+  T d, dp, m, mp;
+};
+
 // New value of one field component at flat index q (reference formulas, see file header).
 //   f_old : the component's previous value
-//   d_old : its D/B value (special cells only; loaded by the caller)
+//   old   : its D/B history (special cells only; loaded by the caller)
 //   tmp   : curl operand combination (o12 - o11 - o22 + o21)
 //   mat   : averaged eps (E) or mu (H)
 //   om/ga : averaged Drude omega / gamma (MODE 2 special cells)
 //   store : write the auxiliary D/B (D1/B1) state of this cell
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, T d_old, T tmp,
-                                         T mat, bool special, T om, T ga, bool store)
+__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, const Old<T> &old,
+                                         T tmp, T mat, bool special, T om, T ga, bool store)
 {
+  const T d_old = old.d;
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
   if (MODE == 0 || !special)
   {
@@ -49,9 +58,7 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
   T f_new;
   if (MODE == 2)
   {
-    const T d_pp = gld (a.dp[COMP] + q);
-    const T m_old = gld (a.d1[COMP] + q);
-    const T m_pp = gld (a.d1p[COMP] + q);
+    const T d_pp = old.dp, m_old = old.m, m_pp = old.mp;
     // Drude ADE coefficients (InternalScheme.inc.h:2513-2518), M = eps0 or mu0
     const T M = modc, dt = a.dt;
     const T wdt2 = (dt * om) * (dt * om);
@@ -85,18 +92,28 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
 // caller before any store of the cell, so every load of a thread is in flight at once (the
 // state pointers may alias as far as the compiler knows).
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, T d_old,
-                                             T tmp, T mat, bool special, T om, T ga)
+__device__ __forceinline__ void update_cell (const StepArgs<T> &a, int64_t q, const int p[3], T f_old,
+                                             const Old<T> &old, T tmp, T mat, bool special, T om, T ga)
 {
-  gst (a.f_n[COMP] + q, cell_value<T, MODE, COMP> (a, q, p, f_old, d_old, tmp, mat, special, om, ga, true));
+  gst (a.f_n[COMP] + q, cell_value<T, MODE, COMP> (a, q, p, f_old, old, tmp, mat, special, om, ga, true));
 }
 
-// Previous value and (special cells) D/B value of component COMP at q.
+// Previous value and (special cells) the D/B history of component COMP at q.
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ void load_old (const StepArgs<T> &a, int64_t q, bool special, T &f_old, T &d_old)
+__device__ __forceinline__ void load_old (const StepArgs<T> &a, int64_t q, bool special, T &f_old, Old<T> &old)
 {
   f_old = gld (a.f[COMP] + q);
-  d_old = (MODE != 0 && special) ? gld (a.d[COMP] + q) : (T) 0;
+  old.d = old.dp = old.m = old.mp = (T) 0;
+  if (MODE != 0 && special)
+  {
+    old.d = gld (a.d[COMP] + q);
+    if (MODE == 2)
+    {
+      old.dp = gld (a.dp[COMP] + q);
+      old.m = gld (a.d1[COMP] + q);
+      old.mp = gld (a.d1p[COMP] + q);
+    }
+  }
 }
 
 
@@ -119,7 +136,8 @@ __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
   const int p[3] = { i, j, k };
   const bool special = is_special (a, p, 3, MODE);
-  T fo0, fo1, fo2, do0, do1, do2;
+  T fo0, fo1, fo2;
+  Old<T> do0, do1, do2;
   load_old<T, MODE, 0> (a, q, special, fo0, do0);
   load_old<T, MODE, 1> (a, q, special, fo1, do1);
   load_old<T, MODE, 2> (a, q, special, fo2, do2);
@@ -181,7 +199,8 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
   const int p[3] = { i, j, k };
   const bool special = is_special (a, p, 3, MODE);
-  T fo3, fo4, fo5, do3, do4, do5;
+  T fo3, fo4, fo5;
+  Old<T> do3, do4, do5;
   load_old<T, MODE, 3> (a, q, special, fo3, do3);
   load_old<T, MODE, 4> (a, q, special, fo4, do4);
   load_old<T, MODE, 5> (a, q, special, fo5, do5);
@@ -510,6 +529,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
   __shared__ T xch[2][16][BY][2];   // Ex, Ey after the step at lane 0 of every wave, by plane parity
   // z-face TF/SF corrections of the strip, by iteration parity: [parity][part][face][row][slot]
   __shared__ double zl[2][2][3][BY + 1][4];
+  constexpr int kZStage = 2 * 3 * (BY + 1) * 4;   // entries per parity
 
   FusedPos g;
   g.k = (int) threadIdx.x;
@@ -553,9 +573,9 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     e_load<T, BY> (a, xa, g, ro, L);
     if (a.tfsf)
     {
-      // corrections of plane xa (E part) into zl[0], then those of the first iteration
-      constexpr int n = 2 * 3 * (BY + 1) * 4;
-      if (g.k < n) (&zl[0][0][0][0][0])[g.k] = zstage_load<T, BY> (a, g.k, xa, g.j0);
+      // corrections of plane xa (E part) into zl[0], then those of the first iteration; the
+      // 24 (BY + 1) entries take two passes of a 64-thread block
+      for (int t = g.k; t < kZStage; t += (int) blockDim.x) (&zl[0][0][0][0][0])[t] = zstage_load<T, BY> (a, t, xa, g.j0);
       __syncthreads ();
     }
     EPlane<T, BY> o;
@@ -583,7 +603,13 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
         xch[0][wave][r][1] = eyc[r];
       }
     }
-    if (a.tfsf && g.k < 2 * 3 * (BY + 1) * 4) (&zl[1][0][0][0][0])[g.k] = zpre;
+    if (a.tfsf)
+    {
+      if (g.k < kZStage) (&zl[1][0][0][0][0])[g.k] = zpre;
+      // a 64-thread block (short z rows) loads the entries past its width here
+      for (int t = g.k + (int) blockDim.x; t < kZStage; t += (int) blockDim.x)
+        (&zl[1][0][0][0][0])[t] = zstage_load<T, BY> (a, t, min (xa + 1, xb), g.j0);
+    }
     __syncthreads ();
   }
 
@@ -697,7 +723,12 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
       hzc[r] = o.hz[r];
       ep0[r] = o.ep1[r];
     }
-    if (a.tfsf && g.k < 2 * 3 * (BY + 1) * 4) (&zl[par ^ 1][0][0][0][0])[g.k] = zpre;
+    if (a.tfsf)
+    {
+      if (g.k < kZStage) (&zl[par ^ 1][0][0][0][0])[g.k] = zpre;
+      for (int t = g.k + (int) blockDim.x; t < kZStage; t += (int) blockDim.x)
+        (&zl[par ^ 1][0][0][0][0])[t] = zstage_load<T, BY> (a, t, min (x + 1, xb), g.j0);
+    }
     __syncthreads ();
   }
 }
@@ -713,7 +744,8 @@ __device__ __forceinline__ void tmz_e_cell (const StepArgs<T> &a, int i, int j)
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
   const int p[3] = { i, j, 0 };
   const bool special = is_special (a, p, 2, MODE);
-  T fo, dold;
+  T fo;
+  Old<T> dold;
   load_old<T, MODE, 2> (a, q, special, fo, dold);
   const T *Hx = a.f[3], *Hy = a.f[4];
   T v[4] = { Hy[q - sx], Hy[q], Hx[q - sy], Hx[q] };
@@ -739,7 +771,8 @@ __device__ __forceinline__ void tmz_h_cell (const StepArgs<T> &a, int i, int j)
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
   const int p[3] = { i, j, 0 };
   const bool special = is_special (a, p, 2, MODE);
-  T fo3, fo4, do3, do4;
+  T fo3, fo4;
+  Old<T> do3, do4;
   load_old<T, MODE, 3> (a, q, special, fo3, do3);
   load_old<T, MODE, 4> (a, q, special, fo4, do4);
   const T *Ez = a.f[2];
