@@ -202,9 +202,13 @@ bool tfsf_near_host (const yee_handle *h, const int64_t p[3])
 // Does the fused kernel update cell p (fused_owns of the kernels)?
 bool fused_owns_host (const yee_handle *h, const int64_t p[3])
 {
+  bool drude = h->mode == 2;
   for (int ax = 0; ax < 3; ++ax)
+  {
     if (p[ax] < h->fbox_lo[ax] || p[ax] >= h->fbox_hi[ax]) return false;
-  return !(h->use_tfsf () && tfsf_near_host (h, p));
+    drude = drude && p[ax] >= h->box_lo[ax] && p[ax] < h->box_hi[ax];
+  }
+  return !(h->use_tfsf () && tfsf_near_host (h, p)) && !drude;
 }
 
 bool source_in_fused_box (const yee_handle *h)
@@ -321,9 +325,21 @@ int fused_step (yee_handle *h, int32_t t, int prev)
 {
   int rc;
   const int dst = 1 - h->cur;
-  int64_t lo[15][3], hi[15][3];
+  int64_t lo[16][3], hi[16][3];
   int nslab = slab_boxes (h, lo, hi);
   nslab += face_boxes (h, lo + nslab, hi + nslab);
+  if (h->mode == 2)
+  {
+    // the Drude box (its cells run the two-stage update with the ADE terms), clipped to the interior
+    bool ok = true;
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      lo[nslab][ax] = std::max<int64_t> (1, h->box_lo[ax]);
+      hi[nslab][ax] = std::min<int64_t> (h->n[ax] - 1, h->box_hi[ax]);
+      ok = ok && hi[nslab][ax] > lo[nslab][ax];
+    }
+    if (ok) ++nslab;
+  }
   {
     ProfScope ps (h, 1);
     for (int s = 0; s < nslab; ++s)
@@ -673,13 +689,14 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (e != hipSuccess) return bail (fail (YEE_EDEVICE, std::string ("hipStreamCreate: ") + hipGetErrorString (e)));
   if (hipEventCreate (&h->ev[0]) != hipSuccess || hipEventCreate (&h->ev[1]) != hipSuccess)
     return bail (fail (YEE_EDEVICE, "hipEventCreate failed"));
-  // 3D runs without Drude terms use the fused step kernel on the PML-free box (double-buffered
-  // state); TMz and Drude runs update in place with the half-step kernels.  YEE_KERNEL=split
-  // in the environment selects the half-step kernels for 3D as well (A/B checks).
+  // 3D runs use the fused step kernel on the PML-free box outside the Drude box (double-
+  // buffered state) when the doubled state fits in HBM; TMz runs update in place with the
+  // half-step kernels.  YEE_KERNEL=split in the environment selects the half-step kernels for 3D
+  // as well (A/B checks).
   const char *kern = std::getenv ("YEE_KERNEL");
   const bool want_fused = !(kern && std::strcmp (kern, "split") == 0);
   h->fused = false;
-  if (h->dim == 3 && h->mode <= 1 && h->n[2] <= 1024 && want_fused)
+  if (h->dim == 3 && h->n[2] <= 1024 && want_fused)
   {
     bool ok = true;
     for (int ax = 0; ax < 3; ++ax)
@@ -689,7 +706,10 @@ int yee_create (const yee_config *cfg, yee_handle **out)
       h->fbox_hi[ax] = std::min<int64_t> (h->n[ax] - 1, h->n[ax] - P - 1);
       ok = ok && h->fbox_hi[ax] > h->fbox_lo[ax];
     }
-    h->fused = ok;
+    // two buffers of E/H (6), D/B (6 with PML) and the Drude history (18 more)
+    const int arrays = 6 + (h->mode >= 1 ? 6 : 0) + (h->mode == 2 ? 18 : 0);
+    const double bytes = 2.0 * arrays * (double) h->ncell * (double) h->wsize;
+    h->fused = ok && bytes < 200e9;
   }
   if (h->fused)
   {

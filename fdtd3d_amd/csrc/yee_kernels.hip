@@ -295,11 +295,15 @@ __device__ __forceinline__ bool tfsf_xy_near (const StepArgs<T> &a, int x, int j
   return x == a.tl[0] - 1 || x == a.tr[0] - 1 || x == a.tr[0] || j == a.tl[1] - 1 || j == a.tr[1] - 1 || j == a.tr[1];
 }
 
-// Does the fused kernel update cell (x, j, k)?  (xbox, kbox: the x and z box tests.)
+// Does the fused kernel update cell (x, j, k)?  (xbox, kbox: the x and z box tests.)  Not the
+// TF/SF x/y face cells and not the Drude box (empty without metamaterials): the half-step
+// kernels update those.
 template <typename T>
 __device__ __forceinline__ bool fused_owns (const StepArgs<T> &a, bool xbox, int x, int j, int k, bool kbox)
 {
-  return xbox && kbox && j >= a.run_lo[1] && j < a.run_hi[1] && !(a.tfsf && tfsf_xy_near<T> (a, x, j, k));
+  const bool drude = x >= a.box_lo[0] && x < a.box_hi[0] && j >= a.box_lo[1] && j < a.box_hi[1] && k >= a.box_lo[2]
+                     && k < a.box_hi[2];
+  return xbox && kbox && j >= a.run_lo[1] && j < a.run_hi[1] && !(a.tfsf && tfsf_xy_near<T> (a, x, j, k)) && !drude;
 }
 
 // z face (0: k = L-1, 1: R-1, 2: R) of a lane, or -1.

@@ -205,3 +205,29 @@ def test_fused_step_matches_half_steps(args, precision, engine_lib, monkeypatch)
         assert np.array_equal(fa, fb), f"{f}: max diff {np.max(np.abs(fa.astype(np.float64) - fb))}"
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_fused_step_with_drude_box_matches_half_steps(precision, engine_lib, monkeypatch):
+    """Drude runs: the fused kernel leaves the Drude box to the half-step kernels; bit-identical
+    to the all-half-step run."""
+    cfg = parse_args("--3d --size x:36,y:32,z:40 --use-pml --pml-size x:4,y:4,z:5 --use-tfsf "
+                     "--tfsf-size-left x:8,y:8,z:9 --tfsf-size-right x:8,y:8,z:9 --angle-teta 0 --angle-phi 0 "
+                     "--angle-psi 0 --use-metamaterials --time-steps 50".split())
+    cfg.precision = precision
+    w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
+    om = np.zeros(cfg.shape)
+    ga = np.zeros(cfg.shape)
+    om[14:22, 12:20, 16:25] = 1.5 * w0
+    ga[14:22, 12:20, 16:25] = 0.01 * w0
+    mats = {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
+    say(f"[gpu] drude split vs fused precision {precision}")
+    monkeypatch.setenv("YEE_KERNEL", "split")
+    a = run_engine(cfg, mats)
+    monkeypatch.setenv("YEE_KERNEL", "fused")
+    b = run_engine(cfg, mats)
+    for f in field_names(cfg, aux=True):
+        fa, fb = a.field_raw(f), b.field_raw(f)
+        assert np.array_equal(fa, fb), f"{f}: max diff {np.max(np.abs(fa.astype(np.float64) - fb))}"
+    a.close()
+    b.close()
