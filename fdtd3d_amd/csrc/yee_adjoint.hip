@@ -51,7 +51,7 @@ __device__ __forceinline__ T adj_cell (const AdjArgs<T> &g, int64_t q, const int
   T w;
   if (!special)
   {
-    const T cb = a.dt / (mat * modc * a.dx);
+    const T cb = direct_cb (a.dt, mat, modc, a.dx);
     w = av * cb;
     g.af[COMP][q] = av;
   }

@@ -368,6 +368,7 @@ int fused_step (yee_handle *h, int32_t t, int prev)
       const int64_t *p = h->cfg.point_source_pos;
       a.src_mask = h->cfg.point_source & 0x3fu;
       a.src_q = (p[0] * h->n[1] + p[1]) * h->n[2] + p[2];
+      for (int ax = 0; ax < 3; ++ax) a.src_pos[ax] = (int) p[ax];
       a.src_e = (T) source_value (h, true, t);
       a.src_h = (T) source_value (h, false, t);
     }

@@ -42,7 +42,7 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
   if (MODE == 0 || !special)
   {
     // computeCaCb, non-PML branch: Ca = 1, Cb = dt / (material * modifier * dx)
-    const T cb = a.dt / (mat * modc * a.dx);
+    const T cb = direct_cb (a.dt, mat, modc, a.dx);
     return f_old * (T) 1.0 + tmp * cb;
   }
   const int L = a.tab_len;
@@ -278,7 +278,14 @@ template <typename T, int COMP>
 __device__ __forceinline__ T direct_value (const StepArgs<T> &a, T f_old, T tmp, T mat)
 {
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
-  const T cb = a.dt / (mat * modc * a.dx);
+  const T cb = direct_cb (a.dt, mat, modc, a.dx);
+  return f_old * (T) 1.0 + tmp * cb;
+}
+
+// The same with the coefficient already computed (uniform material).
+template <typename T>
+__device__ __forceinline__ T direct_value_cb (T f_old, T tmp, T cb)
+{
   return f_old * (T) 1.0 + tmp * cb;
 }
 
@@ -301,9 +308,13 @@ __device__ __forceinline__ bool tfsf_xy_near (const StepArgs<T> &a, int x, int j
 template <typename T>
 __device__ __forceinline__ bool fused_owns (const StepArgs<T> &a, bool xbox, int x, int j, int k, bool kbox)
 {
-  const bool drude = x >= a.box_lo[0] && x < a.box_hi[0] && j >= a.box_lo[1] && j < a.box_hi[1] && k >= a.box_lo[2]
-                     && k < a.box_hi[2];
-  return xbox && kbox && j >= a.run_lo[1] && j < a.run_hi[1] && !(a.tfsf && tfsf_xy_near<T> (a, x, j, k)) && !drude;
+  // uniform over the row first, then the lane's z tests
+  if (!xbox || j < a.run_lo[1] || j >= a.run_hi[1]) return false;
+  const bool face_row = a.tfsf && x >= a.tl[0] - 1 && x <= a.tr[0] && j >= a.tl[1] - 1 && j <= a.tr[1]
+                        && (x == a.tl[0] - 1 || x == a.tr[0] - 1 || x == a.tr[0] || j == a.tl[1] - 1 || j == a.tr[1] - 1
+                            || j == a.tr[1]);
+  const bool drude_row = x >= a.box_lo[0] && x < a.box_hi[0] && j >= a.box_lo[1] && j < a.box_hi[1];
+  return kbox && !(face_row && k >= a.tl[2] - 1 && k <= a.tr[2]) && !(drude_row && k >= a.box_lo[2] && k < a.box_hi[2]);
 }
 
 // z face (0: k = L-1, 1: R-1, 2: R) of a lane, or -1.
@@ -470,25 +481,29 @@ __device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const Fu
     T vy[4] = { L.hxm[r], L.hxr[r + 1], hz_prev[r], L.hzr[r + 1] };
     const T e = ep0[r];
     const bool own = L.own[r];
+    const bool src = a.src_mask && x == a.src_pos[0] && j == a.src_pos[1] && g.k == a.src_pos[2];
     if (own && zrows && j >= a.tl[1] - 1 && j <= a.tr[1])
     {
       // z-face TF/SF corrections, staged in LDS (Ex: Hy operands, Ey: Hx operands)
       zface_apply (&zl[zf][r][0], vx[2], vx[3]);
       zface_apply (&zl[zf][r][2], vy[0], vy[1]);
     }
-    const int64_t qt = base + (int64_t) j * a.sy + g.k;
+    // cb = dt / (m eps0 dx), m the 2-cell average (a uniform eps averages to itself)
+    const T cbx = direct_cb (a.dt, avg2 (e, L.ep1[r]), a.eps0, a.dx);
+    const T cbz = direct_cb (a.dt, avg2 (e, L.epk[r]), a.eps0, a.dx);
+    const T cby = r < BY ? direct_cb (a.dt, avg2 (e, ep0[r + 1]), a.eps0, a.dx) : (T) 0;
     T v = L.ex[r];
     if (own)
     {
-      v = direct_value<T, 0> (a, v, curl (vx), avg2 (e, L.ep1[r]));
-      if ((a.src_mask & 1u) && qt == a.src_q) v = a.src_e;
+      v = direct_value_cb (v, curl (vx), cbx);
+      if (src && (a.src_mask & 1u)) v = a.src_e;
     }
     o.ex[r] = v;
     v = L.ez[r];
     if (own)
     {
-      v = direct_value<T, 2> (a, v, curl (vz), avg2 (e, L.epk[r]));
-      if ((a.src_mask & 4u) && qt == a.src_q) v = a.src_e;
+      v = direct_value_cb (v, curl (vz), cbz);
+      if (src && (a.src_mask & 4u)) v = a.src_e;
     }
     o.ez[r] = v;
     if (r < BY)
@@ -496,8 +511,8 @@ __device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const Fu
       v = L.ey[r];
       if (own)
       {
-        v = direct_value<T, 1> (a, v, curl (vy), avg2 (e, ep0[r + 1]));
-        if ((a.src_mask & 2u) && qt == a.src_q) v = a.src_e;
+        v = direct_value_cb (v, curl (vy), cby);
+        if (src && (a.src_mask & 2u)) v = a.src_e;
       }
       o.ey[r] = v;
       o.hx[r] = L.hxr[r + 1];
@@ -689,16 +704,20 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
         zface_apply (&zl[par][1][hzf][r][0], vx[0], vx[1]);
         zface_apply (&zl[par][1][hzf][r][2], vy[2], vy[3]);
       }
-      const int64_t qt = (int64_t) i * a.sx + (int64_t) j * a.sy + g.k;
-      T v = direct_value<T, 3> (a, hxc[r], curl (vx), mxm[r]);
-      if ((a.src_mask & 8u) && qt == a.src_q) v = a.src_h;
+      const bool src = a.src_mask && i == a.src_pos[0] && j == a.src_pos[1] && g.k == a.src_pos[2];
+      // cb = dt / (m mu0 dx), m the 4-cell average (or the uniform mu)
+      const T cbx = direct_cb (a.dt, mxm[r], a.mu0, a.dx);
+      const T cby = direct_cb (a.dt, mym[r], a.mu0, a.dx);
+      const T cbz = direct_cb (a.dt, mzm[r], a.mu0, a.dx);
+      T v = direct_value_cb (hxc[r], curl (vx), cbx);
+      T w2 = direct_value_cb (hyc[r], curl (vy), cby);
+      T w3 = direct_value_cb (hzc[r], curl (vz), cbz);
+      if (src && (a.src_mask & 8u)) v = a.src_h;
+      if (src && (a.src_mask & 16u)) w2 = a.src_h;
+      if (src && (a.src_mask & 32u)) w3 = a.src_h;
       hn[0][r] = v;
-      v = direct_value<T, 4> (a, hyc[r], curl (vy), mym[r]);
-      if ((a.src_mask & 16u) && qt == a.src_q) v = a.src_h;
-      hn[1][r] = v;
-      v = direct_value<T, 5> (a, hzc[r], curl (vz), mzm[r]);
-      if ((a.src_mask & 32u) && qt == a.src_q) v = a.src_h;
-      hn[2][r] = v;
+      hn[1][r] = w2;
+      hn[2][r] = w3;
     }
     // ---- stores: E of plane x when this chunk owns it, H of plane i (owned cells only)
     if (x < xb) e_store<T, BY> (a, x, g, ro, o);
