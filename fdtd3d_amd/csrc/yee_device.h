@@ -24,16 +24,16 @@ __device__ __forceinline__ void gst (T *p, T v)
   *(__attribute__ ((address_space (1))) T *) (p) = v;
 }
 
-// Row-relative accessors for the fused kernel.  `row` MUST be uniform over the wavefront:
-// it is pinned to scalar registers (the empty asm also keeps the compiler from folding the
-// lane offset into it), so every access is selected as scalar base + 32-bit lane offset
-// `boff` + immediate `delta` (elements), with no per-access 64-bit address registers.
+// Row-relative accessors for the fused kernel: `row` is a uniform base pinned to scalar
+// registers (plane_row), `boff` the lane's byte offset and `delta` an element offset folded
+// into the instruction.  The caller passes offsets taken with fresh_offsets (or an equivalent
+// empty asm) in the same block, so that each access is selected as scalar base + 32-bit lane
+// offset + immediate, with no 64-bit vector address.
 template <typename T>
 __device__ __forceinline__ T rld (const T *row, uint32_t boff, int delta = 0)
 {
   typedef const __attribute__ ((address_space (1))) char gchar;
   typedef const __attribute__ ((address_space (1))) T gT;
-  asm volatile ("" : "+s"(row));
   gchar *p = (gchar *) row + boff;
   return *((gT *) p + delta);
 }
@@ -43,7 +43,6 @@ __device__ __forceinline__ void rst (T *row, uint32_t boff, T v)
 {
   typedef __attribute__ ((address_space (1))) char gchar;
   typedef __attribute__ ((address_space (1))) T gT;
-  asm volatile ("" : "+s"(row));
   gchar *p = (gchar *) row + boff;
   *(gT *) p = v;
 }

@@ -29,7 +29,7 @@ template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
 template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
-template <typename T, int BY> __global__ void step_3d_fused (StepArgs<T>);
+template <typename T, int BY, int MAT> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 
 static thread_local std::string g_error;
@@ -340,12 +340,24 @@ int fused_step (yee_handle *h, int32_t t, int prev)
     }
     if (ok) ++nslab;
   }
+  const bool src_box = source_in_fused_box (h);
+  if (src_box)
+  {
+    // the point-source cell: updated by the half-step kernels, then overwritten by the source
+    // (the fused kernel leaves it out, as it does the TF/SF faces)
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      lo[nslab][ax] = h->cfg.point_source_pos[ax];
+      hi[nslab][ax] = h->cfg.point_source_pos[ax] + 1;
+    }
+    ++nslab;
+  }
   {
     ProfScope ps (h, 1);
     for (int s = 0; s < nslab; ++s)
       if ((rc = launch_half<T> (h, true, prev, lo[s], hi[s]))) return rc;
   }
-  if ((rc = launch_sources<T> (h, true, t, dst, true))) return rc;
+  if ((rc = launch_sources<T> (h, true, t, dst, false))) return rc;
   if (h->zcorr)
   {
     StepArgs<T> a = make_args<T> (h, prev);
@@ -363,19 +375,22 @@ int fused_step (yee_handle *h, int32_t t, int prev)
     }
     a.lx = h->fused_lx;
     a.nstrips = h->fused_nstrips;
-    if (source_in_fused_box (h))
+    if (src_box)
     {
-      const int64_t *p = h->cfg.point_source_pos;
+      // excluded from the kernel's cells (axis_bits, kAxSrc)
       a.src_mask = h->cfg.point_source & 0x3fu;
-      a.src_q = (p[0] * h->n[1] + p[1]) * h->n[2] + p[2];
-      for (int ax = 0; ax < 3; ++ax) a.src_pos[ax] = (int) p[ax];
-      a.src_e = (T) source_value (h, true, t);
-      a.src_h = (T) source_value (h, false, t);
+      for (int ax = 0; ax < 3; ++ax) a.src_pos[ax] = (int) h->cfg.point_source_pos[ax];
     }
     ProfScope ps (h, 3);
     const unsigned blocks = (unsigned) (h->fused_nstrips * h->fused_nchunks);
-    hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T)>), dim3 (blocks), dim3 ((unsigned) h->fused_threads), 0,
-                       h->stream, a);
+    const dim3 grid (blocks), block ((unsigned) h->fused_threads);
+    switch ((a.eps ? 1 : 0) | (a.mu ? 2 : 0))
+    {
+      case 0: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 0>), grid, block, 0, h->stream, a); break;
+      case 1: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 1>), grid, block, 0, h->stream, a); break;
+      case 2: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 2>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 3>), grid, block, 0, h->stream, a); break;
+    }
     HIP_TRY (hipGetLastError ());
   }
   {
@@ -383,7 +398,7 @@ int fused_step (yee_handle *h, int32_t t, int prev)
     for (int s = 0; s < nslab; ++s)
       if ((rc = launch_half<T> (h, false, prev, lo[s], hi[s]))) return rc;
   }
-  return launch_sources<T> (h, false, t, dst, true);
+  return launch_sources<T> (h, false, t, dst, false);
 }
 
 template <typename T>

@@ -72,7 +72,6 @@ struct StepArgs
   uint32_t src_mask;           // bit c: hard source on component c at flat index src_q
   int64_t src_q;
   int src_pos[3];              // the same position as (x, y, z)
-  T src_e, src_h;              // source values of this step's E and H updates
   // TF/SF corrections of the cells on the z faces of the TF/SF box (planes k = L-1, R-1, R),
   // precomputed per step: [part (E, H)][face][x - (L-1)][y - (L-1)][comp (Ex/Ey or Hx/Hy)][slot lo/hi]
   const double *zcorr;

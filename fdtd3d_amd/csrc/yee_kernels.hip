@@ -266,11 +266,24 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 // once per step, except the recomputed row and the first plane of each chunk.
 // ---------------------------------------------------------------------------------------
 
-// Thread geometry of the fused sweep.
+// Ownership bits of one coordinate of a cell (axis_bits): inside the run box, inside
+// [L-1, R] of the TF/SF box (with TF/SF on), on a face plane L-1, R-1 or R of the TF/SF box
+// (x and y only), inside the Drude box, on the point source (with a source on).
+constexpr uint32_t kAxRun = 1, kAxFRange = 2, kAxFPlane = 4, kAxDrude = 8, kAxSrc = 16;
+
+// Thread geometry of the fused sweep.  A cell is updated by the fused kernel when it lies in
+// the run box, is not next to an x or y face of the TF/SF box, not in the Drude box and not the
+// point-source cell (the half-step kernels update those).  The test is split by axis: plane bits (uniform per x),
+// row bits (uniform per strip row, packed 8 bits a row in rowb) and lane bits (zb, per thread),
+// so that it costs a few scalar operations and one vector compare per row.
 struct FusedPos
 {
   int j0, k, kc;
-  bool kbox;   // lane inside the box along z
+  int zf;            // z face of the TF/SF box the lane lies on (zface_of), -1 if none
+  uint32_t zb;       // axis_bits of k
+  uint32_t rowb;     // axis_bits of rows j0 .. j0+BY, 8 bits each
+  bool wave_box;     // every lane of the wave inside the run box along z
+  bool wave_zface;   // a lane of the wave on a z face of the TF/SF box
 };
 
 // Direct E/H update (computeCaCb non-PML branch, calcField; same expressions as cell_value).
@@ -289,32 +302,33 @@ __device__ __forceinline__ T direct_value_cb (T f_old, T tmp, T cb)
   return f_old * (T) 1.0 + tmp * cb;
 }
 
-// A cell next to an x or y face of the TF/SF box: inside [L-1, R] on every axis, with x or y
-// on one of the planes L-1, R-1, R.  (Cells next to z faces only are corrected in the fused
-// kernel from a per-step table, tfsf_ztable.)
 template <typename T>
-__device__ __forceinline__ bool tfsf_xy_near (const StepArgs<T> &a, int x, int j, int k)
+__device__ __forceinline__ uint32_t axis_bits (const StepArgs<T> &a, int ax, int p)
 {
-  const int p[3] = { x, j, k };
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax)
-    if (p[ax] < a.tl[ax] - 1 || p[ax] > a.tr[ax]) return false;
-  return x == a.tl[0] - 1 || x == a.tr[0] - 1 || x == a.tr[0] || j == a.tl[1] - 1 || j == a.tr[1] - 1 || j == a.tr[1];
+  uint32_t b = (p >= a.run_lo[ax] && p < a.run_hi[ax]) ? kAxRun : 0u;
+  b |= (a.tfsf && p >= a.tl[ax] - 1 && p <= a.tr[ax]) ? kAxFRange : 0u;
+  b |= (ax < 2 && (p == a.tl[ax] - 1 || p == a.tr[ax] - 1 || p == a.tr[ax])) ? kAxFPlane : 0u;
+  b |= (p >= a.box_lo[ax] && p < a.box_hi[ax]) ? kAxDrude : 0u;
+  b |= (a.src_mask && p == a.src_pos[ax]) ? kAxSrc : 0u;
+  return b;
 }
 
-// Does the fused kernel update cell (x, j, k)?  (xbox, kbox: the x and z box tests.)  Not the
-// TF/SF x/y face cells and not the Drude box (empty without metamaterials): the half-step
-// kernels update those.
-template <typename T>
-__device__ __forceinline__ bool fused_owns (const StepArgs<T> &a, bool xbox, int x, int j, int k, bool kbox)
+// The z bits a lane must show, masked with the result, for the fused kernel to own cell
+// (x, j, k) of plane bits xb and row bits yb: (zb & m) == kAxRun.  m = 0 owns no lane; m = kAxRun
+// owns every lane inside the run box.  The point-source cell is left to the half-step kernels
+// (the source overwrites it after their update).
+__device__ __forceinline__ uint32_t own_mask (uint32_t xb, uint32_t yb)
 {
-  // uniform over the row first, then the lane's z tests
-  if (!xbox || j < a.run_lo[1] || j >= a.run_hi[1]) return false;
-  const bool face_row = a.tfsf && x >= a.tl[0] - 1 && x <= a.tr[0] && j >= a.tl[1] - 1 && j <= a.tr[1]
-                        && (x == a.tl[0] - 1 || x == a.tr[0] - 1 || x == a.tr[0] || j == a.tl[1] - 1 || j == a.tr[1] - 1
-                            || j == a.tr[1]);
-  const bool drude_row = x >= a.box_lo[0] && x < a.box_hi[0] && j >= a.box_lo[1] && j < a.box_hi[1];
-  return kbox && !(face_row && k >= a.tl[2] - 1 && k <= a.tr[2]) && !(drude_row && k >= a.box_lo[2] && k < a.box_hi[2]);
+  // branch-free: kAxFRange when inside the TF/SF range on x and y and on an x or y face plane
+  // (kAxFPlane shifted onto kAxFRange), kAxDrude when inside the Drude box on x and y
+  const uint32_t u = xb & yb;
+  const uint32_t face = u & ((xb | yb) >> 1) & kAxFRange;
+  return (0u - (u & kAxRun)) & (kAxRun | face | (u & (kAxDrude | kAxSrc)));
+}
+
+__device__ __forceinline__ bool lane_owns (uint32_t zb, uint32_t m)
+{
+  return (zb & m) == kAxRun;
 }
 
 // z face (0: k = L-1, 1: R-1, 2: R) of a lane, or -1.
@@ -388,7 +402,28 @@ __device__ __forceinline__ void row_offsets (const StepArgs<T> &a, const FusedPo
 template <typename T>
 __device__ __forceinline__ const T *plane_row (const StepArgs<T> &a, const T *arr, int x, const FusedPos &g)
 {
-  return arr + (int64_t) x * a.sx + (int64_t) (g.j0 - 1) * a.sy;
+  const T *p = arr + (int64_t) x * a.sx + (int64_t) (g.j0 - 1) * a.sy;
+  asm volatile ("" : "+s"(p));   // a scalar base: row accesses add only the 32-bit lane offset
+  return p;
+}
+
+// Material arrays the fused kernel reads (template MAT): eps per cell, mu per cell; a uniform
+// material (eps_u, mu_u) otherwise.
+constexpr int kMatEps = 1, kMatMu = 2;
+
+// A copy of the lane offsets that the compiler cannot relate to the originals, taken inside
+// the block that uses it: each access then keeps zext (offset) beside its add and is selected
+// as scalar base + 32-bit lane offset (the loop-invariant originals would otherwise be widened
+// once, outside the loop, into 64-bit vector addresses).
+template <int N>
+__device__ __forceinline__ void fresh_offsets (const uint32_t (&ro)[N], uint32_t (&o)[N])
+{
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+  {
+    o[r] = ro[r];
+    asm volatile ("" : "+v"(o[r]));
+  }
 }
 
 // Everything the E update of plane x reads.
@@ -399,135 +434,140 @@ struct PlaneLoads
   T hy[BY + 1], hym[BY + 1], hxm[BY + 1];   // Hy rows 0..BY; z-1 neighbours of Hy, Hx
   T ex[BY + 1], ey[BY + 1], ez[BY + 1];     // owned cells: E before the step; others: E after it
   T ep1[BY + 1], epk[BY + 1];               // eps at plane x+1 and at z+1
-  bool own[BY + 1];
+  uint32_t ownb;                            // bit r: the kernel owns the lane's cell of row r
+  uint32_t xb;                              // axis_bits of x
 };
 
-template <typename T, int BY>
+template <typename T, int BY, int MAT>
 __device__ __forceinline__ void e_load (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
                                         PlaneLoads<T, BY> &L)
 {
-  const int nx = a.nx;
+  uint32_t o[BY + 2];
+  fresh_offsets (ro, o);
   const T *Hx = plane_row (a, a.f[3], x, g), *Hy = plane_row (a, a.f[4], x, g), *Hz = plane_row (a, a.f[5], x, g);
+  const T *Ex = plane_row (a, a.f[0], x, g), *Ey = plane_row (a, a.f[1], x, g), *Ez = plane_row (a, a.f[2], x, g);
 #pragma unroll
   for (int r = -1; r <= BY; ++r)
   {
-    L.hxr[r + 1] = rld (Hx, ro[r + 1]);
-    L.hzr[r + 1] = rld (Hz, ro[r + 1]);
+    L.hxr[r + 1] = rld (Hx, o[r + 1]);
+    L.hzr[r + 1] = rld (Hz, o[r + 1]);
   }
-  const bool xbox = x >= a.run_lo[0] && x < a.run_hi[0];
-  const T *E0 = a.eps ? plane_row (a, a.eps, x, g) : nullptr;
-  const T *E1 = a.eps ? plane_row (a, a.eps, min (x + 1, nx - 1), g) : nullptr;
+  const uint32_t xb = axis_bits (a, 0, x);
+  L.xb = xb;
+  uint32_t ownb = 0;
+  bool any_new = false;   // uniform: a lane of the wave reads E after the step for some row
 #pragma unroll
   for (int r = 0; r <= BY; ++r)
   {
-    const uint32_t off = ro[r + 1];
+    const uint32_t off = o[r + 1];
     L.hy[r] = rld (Hy, off);
     L.hym[r] = rld (Hy, off, -1);
     if (r < BY) L.hxm[r] = rld (Hx, off, -1);
-    const bool own = fused_owns (a, xbox, x, g.j0 + r, g.k, g.kbox);
-    L.own[r] = own;
-    const int sel = own ? 0 : 1;   // old E from f, or the new E another kernel wrote to f_n
-    const T *const *fe = sel ? (const T *const *) a.f_n : (const T *const *) a.f;
-    (void) fe;
-    if (own)
+    L.ex[r] = rld (Ex, off);
+    L.ez[r] = rld (Ez, off);
+    if (r < BY) L.ey[r] = rld (Ey, off);
+    const uint32_t m = own_mask (xb, g.rowb >> (8 * r));
+    ownb |= lane_owns (g.zb, m) ? 1u << r : 0u;
+    any_new |= (m != kAxRun) | !g.wave_box;
+  }
+  if (MAT & kMatEps)
+  {
+    const T *E0 = plane_row (a, a.eps, x, g), *E1 = plane_row (a, a.eps, min (x + 1, a.nx - 1), g);
+#pragma unroll
+    for (int r = 0; r <= BY; ++r)
     {
-      L.ex[r] = rld (plane_row (a, a.f[0], x, g), off);
-      L.ez[r] = rld (plane_row (a, a.f[2], x, g), off);
-      if (r < BY) L.ey[r] = rld (plane_row (a, a.f[1], x, g), off);
+      L.ep1[r] = rld (E1, o[r + 1]);
+      L.epk[r] = rld (E0, o[r + 1], 1);
     }
-    else
+  }
+  else
+  {
+#pragma unroll
+    for (int r = 0; r <= BY; ++r) L.ep1[r] = L.epk[r] = a.eps_u;
+  }
+  asm volatile ("" : "+v"(ownb));   // kept as lane bits, not as scalar lane masks
+  L.ownb = ownb;
+  if (any_new)
+  {
+    // cells the kernel does not own take E after the step, which the half-step kernels wrote
+    // to f_n before this kernel
+    const T *Nx = plane_row (a, (const T *) a.f_n[0], x, g), *Ny = plane_row (a, (const T *) a.f_n[1], x, g);
+    const T *Nz = plane_row (a, (const T *) a.f_n[2], x, g);
+#pragma unroll
+    for (int r = 0; r <= BY; ++r)
     {
-      L.ex[r] = rld (plane_row (a, (const T *) a.f_n[0], x, g), off);
-      L.ez[r] = rld (plane_row (a, (const T *) a.f_n[2], x, g), off);
-      if (r < BY) L.ey[r] = rld (plane_row (a, (const T *) a.f_n[1], x, g), off);
-    }
-    if (a.eps)
-    {
-      L.ep1[r] = rld (E1, off);
-      L.epk[r] = rld (E0, off, 1);
-    }
-    else
-    {
-      L.ep1[r] = a.eps_u;
-      L.epk[r] = a.eps_u;
+      if ((ownb >> r) & 1u) continue;
+      uint32_t off = ro[r + 1];
+      asm volatile ("" : "+v"(off));
+      L.ex[r] = rld (Nx, off);
+      L.ez[r] = rld (Nz, off);
+      if (r < BY) L.ey[r] = rld (Ny, off);
     }
   }
 }
 
-// Everything the E update of one x plane produces for the fused sweep.
+// State carried from one x plane to the next: E after the step and H before it at plane i,
+// eps at plane i+1, the plane bits of i.
 template <typename T, int BY>
-struct EPlane
+struct Carry
 {
-  T ex[BY + 1], ey[BY + 1], ez[BY + 1];   // E after the step, rows 0..BY (Ey: rows 0..BY-1)
-  T hx[BY + 1], hy[BY + 1], hz[BY + 1];   // H before the step at this plane (Hx, Hz: rows 0..BY-1)
-  T ep1[BY + 1];                          // eps at the next plane, rows 0..BY
-  bool own[BY + 1];
+  T ex[BY + 1], ey[BY + 1], ez[BY + 1];   // E rows 0..BY (Ey: rows 0..BY-1)
+  T hx[BY + 1], hy[BY + 1], hz[BY + 1];   // H rows 0..BY-1 (Hy: 0..BY)
+  T ep1[BY + 1];                          // eps at plane i+1
+  uint32_t ownb;                          // E ownership bits of plane i
+  uint32_t xb;                            // axis_bits of i
 };
 
+// E update of plane x (rows 0..BY) from its loads L and the H / eps of plane x-1 carried in
+// `prev`; the result becomes the carried state of plane x.
 template <typename T, int BY>
 __device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const FusedPos &g, const PlaneLoads<T, BY> &L,
-                                           const T (&hy_prev)[BY + 1], const T (&hz_prev)[BY + 1],
-                                           const T (&ep0)[BY + 1], const double (*zl)[BY + 1][4], EPlane<T, BY> &o)
+                                           const Carry<T, BY> &prev, const double (*zl)[BY + 1][4], Carry<T, BY> &o)
 {
-  const int zf = zface_of (a, g.k);
-  const bool zrows = a.tfsf && zf >= 0 && x >= a.tl[0] - 1 && x <= a.tr[0];
-  const int64_t base = (int64_t) x * a.sx;
+  const uint32_t xb = L.xb;
+  // z-face corrections: only waves holding a face lane, on planes inside the TF/SF box
+  const bool zx = g.wave_zface && (xb & kAxFRange);
 #pragma unroll
   for (int r = 0; r <= BY; ++r)
   {
-    const int j = g.j0 + r;
+    const uint32_t yb = g.rowb >> (8 * r);
     T vx[4] = { L.hzr[r], L.hzr[r + 1], L.hym[r], L.hy[r] };
-    const T vz[4] = { hy_prev[r], L.hy[r], L.hxr[r], L.hxr[r + 1] };
-    T vy[4] = { L.hxm[r], L.hxr[r + 1], hz_prev[r], L.hzr[r + 1] };
-    const T e = ep0[r];
-    const bool own = L.own[r];
-    const bool src = a.src_mask && x == a.src_pos[0] && j == a.src_pos[1] && g.k == a.src_pos[2];
-    if (own && zrows && j >= a.tl[1] - 1 && j <= a.tr[1])
+    const T vz[4] = { prev.hy[r], L.hy[r], L.hxr[r], L.hxr[r + 1] };
+    T vy[4] = { L.hxm[r], L.hxr[r + 1], prev.hz[r], L.hzr[r + 1] };
+    const T e = prev.ep1[r];
+    const bool own = (L.ownb >> r) & 1u;
+    if (zx && (yb & kAxFRange) && own && g.zf >= 0)
     {
       // z-face TF/SF corrections, staged in LDS (Ex: Hy operands, Ey: Hx operands)
-      zface_apply (&zl[zf][r][0], vx[2], vx[3]);
-      zface_apply (&zl[zf][r][2], vy[0], vy[1]);
+      zface_apply (&zl[g.zf][r][0], vx[2], vx[3]);
+      zface_apply (&zl[g.zf][r][2], vy[0], vy[1]);
     }
     // cb = dt / (m eps0 dx), m the 2-cell average (a uniform eps averages to itself)
     const T cbx = direct_cb (a.dt, avg2 (e, L.ep1[r]), a.eps0, a.dx);
     const T cbz = direct_cb (a.dt, avg2 (e, L.epk[r]), a.eps0, a.dx);
-    const T cby = r < BY ? direct_cb (a.dt, avg2 (e, ep0[r + 1]), a.eps0, a.dx) : (T) 0;
-    T v = L.ex[r];
-    if (own)
-    {
-      v = direct_value_cb (v, curl (vx), cbx);
-      if (src && (a.src_mask & 1u)) v = a.src_e;
-    }
-    o.ex[r] = v;
-    v = L.ez[r];
-    if (own)
-    {
-      v = direct_value_cb (v, curl (vz), cbz);
-      if (src && (a.src_mask & 4u)) v = a.src_e;
-    }
-    o.ez[r] = v;
+    const T cby = r < BY ? direct_cb (a.dt, avg2 (e, prev.ep1[r + 1]), a.eps0, a.dx) : (T) 0;
+    const T nx = direct_value_cb (L.ex[r], curl (vx), cbx);
+    const T nz = direct_value_cb (L.ez[r], curl (vz), cbz);
+    o.ex[r] = own ? nx : L.ex[r];
+    o.ez[r] = own ? nz : L.ez[r];
     if (r < BY)
     {
-      v = L.ey[r];
-      if (own)
-      {
-        v = direct_value_cb (v, curl (vy), cby);
-        if (src && (a.src_mask & 2u)) v = a.src_e;
-      }
-      o.ey[r] = v;
+      const T ny = direct_value_cb (L.ey[r], curl (vy), cby);
+      o.ey[r] = own ? ny : L.ey[r];
       o.hx[r] = L.hxr[r + 1];
       o.hz[r] = L.hzr[r + 1];
     }
     o.hy[r] = L.hy[r];
     o.ep1[r] = L.ep1[r];
-    o.own[r] = own;
   }
+  o.ownb = L.ownb;
+  o.xb = xb;
 }
 
 // Stores the owned cells of rows 0..BY-1 of an E plane.
 template <typename T, int BY>
 __device__ __forceinline__ void e_store (const StepArgs<T> &a, int x, const FusedPos &g, const uint32_t (&ro)[BY + 2],
-                                         const EPlane<T, BY> &o)
+                                         const Carry<T, BY> &o)
 {
   T *Ex = (T *) plane_row (a, (const T *) a.f_n[0], x, g);
   T *Ey = (T *) plane_row (a, (const T *) a.f_n[1], x, g);
@@ -535,20 +575,142 @@ __device__ __forceinline__ void e_store (const StepArgs<T> &a, int x, const Fuse
 #pragma unroll
   for (int r = 0; r < BY; ++r)
   {
-    if (!o.own[r]) continue;
-    rst (Ex, ro[r + 1], o.ex[r]);
-    rst (Ey, ro[r + 1], o.ey[r]);
-    rst (Ez, ro[r + 1], o.ez[r]);
+    if (!((o.ownb >> r) & 1u)) continue;
+    uint32_t off = ro[r + 1];
+    asm volatile ("" : "+v"(off));
+    rst (Ex, off, o.ex[r]);
+    rst (Ey, off, o.ey[r]);
+    rst (Ez, off, o.ez[r]);
   }
 }
 
+// LDS of the fused kernel: Ex, Ey after the step at lane 0 of every wave (by plane parity),
+// and the z-face TF/SF corrections of the strip ([parity][part][face][row][slot]).
 template <typename T, int BY>
+struct FusedLds
+{
+  T xch[2][16][BY][2];
+  double zl[2][2][3][BY + 1][4];
+};
+
+// One plane of the sweep: E of plane x (from L and the state `c` of plane i = x-1), H of
+// plane i, the stores, and the prefetch of plane x+1 into `nxt`.  PAR is the parity of the
+// LDS slots of this plane.
+template <typename T, int BY, int MAT>
+__device__ __forceinline__ void fused_plane (const StepArgs<T> &a, const FusedPos &g, const uint32_t (&ro)[BY + 2],
+                                             int x, int xend, int PAR, int lane, int wave, int nw, FusedLds<T, BY> &s,
+                                             const Carry<T, BY> &c, const PlaneLoads<T, BY> &L, Carry<T, BY> &o,
+                                             PlaneLoads<T, BY> &nxt)
+{
+  constexpr int kZStage = 2 * 3 * (BY + 1) * 4;   // z-face table entries per parity
+  const int i = x - 1;                            // H plane
+  e_load<T, BY, MAT> (a, min (x + 1, xend), g, ro, nxt);   // prefetch the next plane
+  // z-face corrections of the next plane (written to LDS before the barrier)
+  const double zpre = zstage_load<T, BY> (a, g.k, min (x + 1, xend), g.j0);
+  e_compute<T, BY> (a, x, g, L, c, s.zl[PAR][0], o);
+  if (lane == 0)
+  {
+#pragma unroll
+    for (int r = 0; r < BY; ++r)
+    {
+      s.xch[PAR][wave][r][0] = o.ex[r];
+      s.xch[PAR][wave][r][1] = o.ey[r];
+    }
+  }
+  // z+1 neighbours of Ex, Ey after the step at plane i
+  T exk[BY + 1], eyk[BY + 1];
+#pragma unroll
+  for (int r = 0; r < BY; ++r)
+  {
+    exk[r] = __shfl_down (c.ex[r], 1);
+    eyk[r] = __shfl_down (c.ey[r], 1);
+    if (lane == 63 && wave + 1 < nw)
+    {
+      exk[r] = s.xch[PAR ^ 1][wave + 1][r][0];
+      eyk[r] = s.xch[PAR ^ 1][wave + 1][r][1];
+    }
+  }
+  // ---- H of plane i, rows 0..BY-1
+  const uint32_t ib = c.xb;
+  T hn[3][BY + 1];
+  T mxm[BY + 1], mym[BY + 1], mzm[BY + 1];
+  uint32_t hownb = 0;
+  if (MAT & kMatMu)
+  {
+    uint32_t om[BY + 2];
+    fresh_offsets (ro, om);
+    const T *M0 = plane_row (a, a.mu, i, g), *M1 = plane_row (a, a.mu, i + 1, g);   // i + 1 <= nx - 2
+#pragma unroll
+    for (int r = 0; r < BY; ++r)
+    {
+      // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
+      const uint32_t o0 = om[r + 1], o1 = om[r + 2];
+      mxm[r] = avg4 (rld (M0, o0), rld (M0, o0, 1), rld (M0, o1), rld (M0, o1, 1));
+      mym[r] = avg4 (rld (M0, o0), rld (M0, o0, 1), rld (M1, o0), rld (M1, o0, 1));
+      mzm[r] = avg4 (rld (M0, o0), rld (M0, o1), rld (M1, o0), rld (M1, o1));
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int r = 0; r < BY; ++r) mxm[r] = mym[r] = mzm[r] = a.mu_u;
+  }
+  const bool zx = g.wave_zface && (ib & kAxFRange);
+#pragma unroll
+  for (int r = 0; r < BY; ++r)
+  {
+    const uint32_t yb = g.rowb >> (8 * r);
+    const bool own = lane_owns (g.zb, own_mask (ib, yb));
+    hownb |= own ? 1u << r : 0u;
+    T vx[4] = { c.ey[r], eyk[r], c.ez[r], c.ez[r + 1] };
+    T vy[4] = { c.ez[r], o.ez[r], c.ex[r], exk[r] };
+    const T vz[4] = { c.ex[r], c.ex[r + 1], c.ey[r], o.ey[r] };
+    if (zx && (yb & kAxFRange) && own && g.zf >= 0)
+    {
+      // z-face TF/SF corrections (Hx: Ey operands, Hy: Ex operands)
+      zface_apply (&s.zl[PAR][1][g.zf][r][0], vx[0], vx[1]);
+      zface_apply (&s.zl[PAR][1][g.zf][r][2], vy[2], vy[3]);
+    }
+    // cb = dt / (m mu0 dx), m the 4-cell average (or the uniform mu)
+    const T cbx = direct_cb (a.dt, mxm[r], a.mu0, a.dx);
+    const T cby = direct_cb (a.dt, mym[r], a.mu0, a.dx);
+    const T cbz = direct_cb (a.dt, mzm[r], a.mu0, a.dx);
+    hn[0][r] = direct_value_cb (c.hx[r], curl (vx), cbx);
+    hn[1][r] = direct_value_cb (c.hy[r], curl (vy), cby);
+    hn[2][r] = direct_value_cb (c.hz[r], curl (vz), cbz);
+  }
+  // ---- stores: E of plane x when this chunk owns it, H of plane i (owned cells only)
+  if (x < xend) e_store<T, BY> (a, x, g, ro, o);
+  {
+    T *Hx = (T *) plane_row (a, (const T *) a.f_n[3], i, g);
+    T *Hy = (T *) plane_row (a, (const T *) a.f_n[4], i, g);
+    T *Hz = (T *) plane_row (a, (const T *) a.f_n[5], i, g);
+#pragma unroll
+    for (int r = 0; r < BY; ++r)
+    {
+      if (!((hownb >> r) & 1u)) continue;
+      uint32_t off = ro[r + 1];
+      asm volatile ("" : "+v"(off));
+      rst (Hx, off, hn[0][r]);
+      rst (Hy, off, hn[1][r]);
+      rst (Hz, off, hn[2][r]);
+    }
+  }
+  if (a.tfsf)
+  {
+    if (g.k < kZStage) (&s.zl[PAR ^ 1][0][0][0][0])[g.k] = zpre;
+    // a 64-thread block (short z rows) loads the entries past its width here
+    for (int t = g.k + (int) blockDim.x; t < kZStage; t += (int) blockDim.x)
+      (&s.zl[PAR ^ 1][0][0][0][0])[t] = zstage_load<T, BY> (a, t, min (x + 1, xend), g.j0);
+  }
+  __syncthreads ();
+}
+
+template <typename T, int BY, int MAT>
 __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
 {
-  __shared__ T xch[2][16][BY][2];   // Ex, Ey after the step at lane 0 of every wave, by plane parity
-  // z-face TF/SF corrections of the strip, by iteration parity: [parity][part][face][row][slot]
-  __shared__ double zl[2][2][3][BY + 1][4];
-  constexpr int kZStage = 2 * 3 * (BY + 1) * 4;   // entries per parity
+  __shared__ FusedLds<T, BY> s;
+  constexpr int kZStage = 2 * 3 * (BY + 1) * 4;   // z-face table entries per parity
 
   FusedPos g;
   g.k = (int) threadIdx.x;
@@ -564,195 +726,105 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
   const int strip = (a.nstrips % 8 == 0) ? (rr % 8) * (a.nstrips / 8) + rr / 8 : rr;
   g.j0 = a.run_lo[1] + strip * BY;
   const int xa = a.run_lo[0] + chunk * a.lx;
-  const int xb = min (xa + a.lx, a.run_hi[0]);   // H (and stored E) planes [xa, xb)
+  const int xend = min (xa + a.lx, a.run_hi[0]);   // H (and stored E) planes [xa, xend)
   g.kc = min (g.k, a.nz - 1);
-  g.kbox = g.k >= a.run_lo[2] && g.k < a.run_hi[2];
+  g.zb = axis_bits (a, 2, g.k);
+  asm volatile ("" : "+v"(g.zb));
+  g.zf = zface_of (a, g.k);
+  g.rowb = 0;
+#pragma unroll
+  for (int r = 0; r <= BY; ++r) g.rowb |= axis_bits (a, 1, g.j0 + r) << (8 * r);
+  {
+    const int w0 = __builtin_amdgcn_readfirstlane (wave * 64);   // first k of the wave
+    g.wave_box = w0 >= a.run_lo[2] && w0 + 63 < a.run_hi[2];
+    const int f0 = a.tl[2] - 1, f1 = a.tr[2] - 1, f2 = a.tr[2];
+    g.wave_zface = a.tfsf && ((f0 >= w0 && f0 < w0 + 64) || (f1 >= w0 && f1 < w0 + 64) || (f2 >= w0 && f2 < w0 + 64));
+  }
 
   uint32_t ro[BY + 2];
   row_offsets<T, BY> (a, g, ro);
 
-  // carried state: E after the step and H before the step at plane i, eps at plane i+1
-  T exc[BY + 1], eyc[BY + 1], ezc[BY + 1];
-  T hxc[BY + 1], hyc[BY + 1], hzc[BY + 1];
-  T ep0[BY + 1];
-  PlaneLoads<T, BY> nxt;
+  // two copies of the carried state and the plane loads, alternating between the two planes
+  // of one loop iteration (no register copies between planes)
+  Carry<T, BY> c0, c1;
+  PlaneLoads<T, BY> p0, p1;
   {
     // prologue: E at plane xa (H before the step at xa-1 and eps at xa are loaded here)
-    T hyp[BY + 1], hzp[BY + 1], e0[BY + 1];
-    const T *Hy = plane_row (a, a.f[4], xa - 1, g), *Hz = plane_row (a, a.f[5], xa - 1, g);
-    const T *E0 = a.eps ? plane_row (a, a.eps, xa, g) : nullptr;
-#pragma unroll
-    for (int r = 0; r <= BY; ++r)
+    Carry<T, BY> cm;
     {
-      hyp[r] = rld (Hy, ro[r + 1]);
-      hzp[r] = rld (Hz, ro[r + 1]);
-      e0[r] = a.eps ? rld (E0, ro[r + 1]) : a.eps_u;
+      uint32_t o[BY + 2];
+      fresh_offsets (ro, o);
+      const T *Hy = plane_row (a, a.f[4], xa - 1, g), *Hz = plane_row (a, a.f[5], xa - 1, g);
+#pragma unroll
+      for (int r = 0; r <= BY; ++r)
+      {
+        cm.hy[r] = rld (Hy, o[r + 1]);
+        cm.hz[r] = rld (Hz, o[r + 1]);
+      }
+      if (MAT & kMatEps)
+      {
+        const T *E0 = plane_row (a, a.eps, xa, g);
+#pragma unroll
+        for (int r = 0; r <= BY; ++r) cm.ep1[r] = rld (E0, o[r + 1]);
+      }
+      else
+      {
+#pragma unroll
+        for (int r = 0; r <= BY; ++r) cm.ep1[r] = a.eps_u;
+      }
     }
     PlaneLoads<T, BY> L;
-    e_load<T, BY> (a, xa, g, ro, L);
+    e_load<T, BY, MAT> (a, xa, g, ro, L);
     if (a.tfsf)
     {
       // corrections of plane xa (E part) into zl[0], then those of the first iteration; the
       // 24 (BY + 1) entries take two passes of a 64-thread block
-      for (int t = g.k; t < kZStage; t += (int) blockDim.x) (&zl[0][0][0][0][0])[t] = zstage_load<T, BY> (a, t, xa, g.j0);
+      for (int t = g.k; t < kZStage; t += (int) blockDim.x) (&s.zl[0][0][0][0][0])[t] = zstage_load<T, BY> (a, t, xa, g.j0);
       __syncthreads ();
     }
-    EPlane<T, BY> o;
-    e_compute<T, BY> (a, xa, g, L, hyp, hzp, e0, zl[0][0], o);
-    const double zpre = zstage_load<T, BY> (a, g.k, min (xa + 1, xb), g.j0);
-    e_store<T, BY> (a, xa, g, ro, o);
-    e_load<T, BY> (a, min (xa + 1, xb), g, ro, nxt);
-#pragma unroll
-    for (int r = 0; r <= BY; ++r)
-    {
-      exc[r] = o.ex[r];
-      eyc[r] = o.ey[r];
-      ezc[r] = o.ez[r];
-      hxc[r] = o.hx[r];
-      hyc[r] = o.hy[r];
-      hzc[r] = o.hz[r];
-      ep0[r] = o.ep1[r];
-    }
+    e_compute<T, BY> (a, xa, g, L, cm, s.zl[0][0], c0);
+    const double zpre = zstage_load<T, BY> (a, g.k, min (xa + 1, xend), g.j0);
+    e_store<T, BY> (a, xa, g, ro, c0);
+    e_load<T, BY, MAT> (a, min (xa + 1, xend), g, ro, p0);
     if (lane == 0)
     {
 #pragma unroll
       for (int r = 0; r < BY; ++r)
       {
-        xch[0][wave][r][0] = exc[r];
-        xch[0][wave][r][1] = eyc[r];
+        s.xch[0][wave][r][0] = c0.ex[r];
+        s.xch[0][wave][r][1] = c0.ey[r];
       }
     }
     if (a.tfsf)
     {
-      if (g.k < kZStage) (&zl[1][0][0][0][0])[g.k] = zpre;
+      if (g.k < kZStage) (&s.zl[1][0][0][0][0])[g.k] = zpre;
       // a 64-thread block (short z rows) loads the entries past its width here
       for (int t = g.k + (int) blockDim.x; t < kZStage; t += (int) blockDim.x)
-        (&zl[1][0][0][0][0])[t] = zstage_load<T, BY> (a, t, min (xa + 1, xb), g.j0);
+        (&s.zl[1][0][0][0][0])[t] = zstage_load<T, BY> (a, t, min (xa + 1, xend), g.j0);
     }
     __syncthreads ();
   }
 
-  int par = 0;
-  for (int x = xa + 1; x <= xb; ++x)
+  if (sizeof (T) == 4)
   {
-    const int i = x - 1;   // H plane
-    const PlaneLoads<T, BY> L = nxt;
-    e_load<T, BY> (a, min (x + 1, xb), g, ro, nxt);   // prefetch the next plane
-    par ^= 1;
-    // z-face corrections of the next iteration (written to LDS before the barrier)
-    const double zpre = zstage_load<T, BY> (a, g.k, min (x + 1, xb), g.j0);
-    EPlane<T, BY> o;
-    e_compute<T, BY> (a, x, g, L, hyc, hzc, ep0, zl[par][0], o);
-    if (lane == 0)
+    // unrolled by two planes: the two copies of the state swap roles without register copies
+    for (int x = xa + 1; x <= xend; x += 2)
     {
-#pragma unroll
-      for (int r = 0; r < BY; ++r)
-      {
-        xch[par][wave][r][0] = o.ex[r];
-        xch[par][wave][r][1] = o.ey[r];
-      }
+      fused_plane<T, BY, MAT> (a, g, ro, x, xend, 1, lane, wave, nw, s, c0, p0, c1, p1);
+      if (x + 1 > xend) break;
+      fused_plane<T, BY, MAT> (a, g, ro, x + 1, xend, 0, lane, wave, nw, s, c1, p1, c0, p0);
     }
-    // z+1 neighbours of Ex, Ey after the step at plane i
-    T exk[BY + 1], eyk[BY + 1];
-#pragma unroll
-    for (int r = 0; r < BY; ++r)
+  }
+  else
+  {
+    // FP64 (twice the registers per value): one plane per iteration, fewer live values
+    int par = 1;
+    for (int x = xa + 1; x <= xend; ++x, par ^= 1)
     {
-      exk[r] = __shfl_down (exc[r], 1);
-      eyk[r] = __shfl_down (eyc[r], 1);
-      if (lane == 63 && wave + 1 < nw)
-      {
-        exk[r] = xch[par ^ 1][wave + 1][r][0];
-        eyk[r] = xch[par ^ 1][wave + 1][r][1];
-      }
+      fused_plane<T, BY, MAT> (a, g, ro, x, xend, par, lane, wave, nw, s, c0, p0, c1, p1);
+      c0 = c1;
+      p0 = p1;
     }
-    // ---- H of plane i, rows 0..BY-1
-    T hn[3][BY + 1];
-    T mxm[BY + 1], mym[BY + 1], mzm[BY + 1];
-    const T *M0 = a.mu ? plane_row (a, a.mu, i, g) : nullptr;
-    const T *M1 = a.mu ? plane_row (a, a.mu, i + 1, g) : nullptr;   // i + 1 <= nx - 2
-    const bool ibox = i >= a.run_lo[0] && i < a.run_hi[0];
-    const int hzf = zface_of (a, g.k);
-    const bool hzrows = a.tfsf && hzf >= 0 && i >= a.tl[0] - 1 && i <= a.tr[0];
-    bool hown[BY + 1];
-#pragma unroll
-    for (int r = 0; r < BY; ++r)
-    {
-      hown[r] = fused_owns (a, ibox, i, g.j0 + r, g.k, g.kbox);
-      if (a.mu)
-      {
-        // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
-        const uint32_t o0 = ro[r + 1], o1 = ro[r + 2];
-        mxm[r] = avg4 (rld (M0, o0), rld (M0, o0, 1), rld (M0, o1), rld (M0, o1, 1));
-        mym[r] = avg4 (rld (M0, o0), rld (M0, o0, 1), rld (M1, o0), rld (M1, o0, 1));
-        mzm[r] = avg4 (rld (M0, o0), rld (M0, o1), rld (M1, o0), rld (M1, o1));
-      }
-      else
-      {
-        mxm[r] = mym[r] = mzm[r] = a.mu_u;
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < BY; ++r)
-    {
-      const int j = g.j0 + r;
-      T vx[4] = { eyc[r], eyk[r], ezc[r], ezc[r + 1] };
-      T vy[4] = { ezc[r], o.ez[r], exc[r], exk[r] };
-      const T vz[4] = { exc[r], exc[r + 1], eyc[r], o.ey[r] };
-      if (hown[r] && hzrows && j >= a.tl[1] - 1 && j <= a.tr[1])
-      {
-        // z-face TF/SF corrections (Hx: Ey operands, Hy: Ex operands)
-        zface_apply (&zl[par][1][hzf][r][0], vx[0], vx[1]);
-        zface_apply (&zl[par][1][hzf][r][2], vy[2], vy[3]);
-      }
-      const bool src = a.src_mask && i == a.src_pos[0] && j == a.src_pos[1] && g.k == a.src_pos[2];
-      // cb = dt / (m mu0 dx), m the 4-cell average (or the uniform mu)
-      const T cbx = direct_cb (a.dt, mxm[r], a.mu0, a.dx);
-      const T cby = direct_cb (a.dt, mym[r], a.mu0, a.dx);
-      const T cbz = direct_cb (a.dt, mzm[r], a.mu0, a.dx);
-      T v = direct_value_cb (hxc[r], curl (vx), cbx);
-      T w2 = direct_value_cb (hyc[r], curl (vy), cby);
-      T w3 = direct_value_cb (hzc[r], curl (vz), cbz);
-      if (src && (a.src_mask & 8u)) v = a.src_h;
-      if (src && (a.src_mask & 16u)) w2 = a.src_h;
-      if (src && (a.src_mask & 32u)) w3 = a.src_h;
-      hn[0][r] = v;
-      hn[1][r] = w2;
-      hn[2][r] = w3;
-    }
-    // ---- stores: E of plane x when this chunk owns it, H of plane i (owned cells only)
-    if (x < xb) e_store<T, BY> (a, x, g, ro, o);
-    {
-      T *Hx = (T *) plane_row (a, (const T *) a.f_n[3], i, g);
-      T *Hy = (T *) plane_row (a, (const T *) a.f_n[4], i, g);
-      T *Hz = (T *) plane_row (a, (const T *) a.f_n[5], i, g);
-#pragma unroll
-      for (int r = 0; r < BY; ++r)
-      {
-        if (!hown[r]) continue;
-        rst (Hx, ro[r + 1], hn[0][r]);
-        rst (Hy, ro[r + 1], hn[1][r]);
-        rst (Hz, ro[r + 1], hn[2][r]);
-      }
-    }
-    // ---- rotate the carried state
-#pragma unroll
-    for (int r = 0; r <= BY; ++r)
-    {
-      exc[r] = o.ex[r];
-      eyc[r] = o.ey[r];
-      ezc[r] = o.ez[r];
-      hxc[r] = o.hx[r];
-      hyc[r] = o.hy[r];
-      hzc[r] = o.hz[r];
-      ep0[r] = o.ep1[r];
-    }
-    if (a.tfsf)
-    {
-      if (g.k < kZStage) (&zl[par ^ 1][0][0][0][0])[g.k] = zpre;
-      for (int t = g.k + (int) blockDim.x; t < kZStage; t += (int) blockDim.x)
-        (&zl[par ^ 1][0][0][0][0])[t] = zstage_load<T, BY> (a, t, min (x + 1, xb), g.j0);
-    }
-    __syncthreads ();
   }
 }
 
@@ -1017,7 +1089,10 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void batch_tmz_half<T, 8> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
   template __global__ void batch_tmz_sources<T> (const StepArgs<T> *, int, const int *, const T *, int); \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
-  template __global__ void step_3d_fused<T, FUSED_BY (T)> (StepArgs<T>);                   \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 0> (StepArgs<T>);                \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 1> (StepArgs<T>);                \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 2> (StepArgs<T>);                \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 3> (StepArgs<T>);                \
   template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *);
