@@ -137,12 +137,21 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
   {
     const int64_t ext[3] = { a.run_hi[0] - a.run_lo[0], a.run_hi[1] - a.run_lo[1], a.run_hi[2] - a.run_lo[2] };
     if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) return YEE_OK;
-    // 64 z cells x 4 rows; boxes thin in z (PML z slabs) take 16 z cells x 16 rows, boxes one
-    // row high (TF/SF y faces) 64 x 1, so that few lanes idle
-    const unsigned bx = ext[2] <= 32 ? 16 : 64;
+    // 64 z cells x 4 rows; boxes thin in z (PML z slabs) take 32 z cells x 8 rows, boxes one
+    // row high (TF/SF y faces) 64 x 1, so that few lanes idle.  Tiles start at a z multiple
+    // of their width (whole cache lines per row).
+    const unsigned bx = ext[2] <= 32 ? 32 : 64;
     const unsigned by = (unsigned) std::min<int64_t> (256 / bx, std::max<int64_t> (1, ext[1]));
+    a.tz0 = a.run_lo[2] & ~(int) (bx - 1);
+    a.ntz = (int) ((a.run_hi[2] - a.tz0 + bx - 1) / bx);
+    a.nty = (int) ((ext[1] + by - 1) / by);
+    a.ntx = (int) ext[0];
+    const int64_t ntile = (int64_t) a.ntz * a.nty * a.ntx;
+    if (ntile > (int64_t) 1 << 30) return fail (YEE_EINVAL, "half-step box too large for one launch");
+    a.ntile = (int) ntile;
+    a.per_xcd = (int) ((ntile + 7) / 8);
     dim3 block (bx, by, 1);
-    dim3 grid ((unsigned) ((ext[2] + bx - 1) / bx), (unsigned) ((ext[1] + by - 1) / by), (unsigned) ext[0]);
+    dim3 grid ((unsigned) (8 * a.per_xcd), 1, 1);
     switch (h->mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;

@@ -69,6 +69,9 @@ struct StepArgs
   // fused step kernel: work decomposition and the point sources it applies in registers
   int lx;                      // x planes per chunk
   int nstrips;                 // y strips per chunk
+  // half-step tile grid (e_half_3d / h_half_3d): z origin of the tiles, tiles along z and y,
+  // planes, tiles per XCD band, total tiles
+  int tz0, ntz, nty, ntx, per_xcd, ntile;
   uint32_t src_mask;           // bit c: hard source on component c at flat index src_q
   int64_t src_q;
   int src_pos[3];              // the same position as (x, y, z)
