@@ -6,11 +6,16 @@
 // computes (the same cell code, tmz_e_cell / tmz_h_cell), so a batch member is bit-identical
 // to the run it replaces.
 //
-// Per step: batch_tmz_half (E) then batch_tmz_half (H) over a tile table of every
-// simulation's interior, then batch_advance.  The step counter lives in device memory, so a
-// graph captured over several steps is replayed as often as needed.  The tile height TX and
-// the number of persistent workgroups are picked by a short timing of the first steps
-// (every candidate computes the same values, so the tuning steps are ordinary steps).
+// The simulations are split into groups whose state fits a cache-sized budget (the 256 MB
+// Infinity Cache of the MI355X sits in front of HBM): a call of yee_batch_step advances one
+// group by all its steps before the next group starts, so after the first step a group's
+// fields are read from the cache rather than from HBM.  Per step of a group: batch_tmz_step
+// (E and H fused, double-buffered state) over the group's tile table, then batch_advance of
+// its simulations' step counters.  The counters live in device memory (one per simulation),
+// so a graph captured over several steps of a group is replayed as often as needed.  The
+// grouping budget and the tile length (x rows per tile) are picked by a short timing of the
+// first steps (every candidate computes the same values, so the tuning steps are ordinary
+// steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,17 +31,27 @@
 
 namespace yee {
 
-template <typename T, int TX>
-__global__ void batch_tmz_half (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int);
+template <typename T>
+__global__ void batch_tmz_step (const StepArgs<T> *, const int4 *, int, const int *, const T *, int);
 template <typename T>
 __global__ void batch_tmz_sources (const StepArgs<T> *, int, const int *, const T *, int);
-__global__ void batch_advance (int *);
+__global__ void batch_advance (int *, const int *, int);
 
 int batch_fail (int code, const std::string &msg);
 
 }  // namespace yee
 
 using namespace yee;
+
+// Simulations advanced together, step after step, with their own tile tables and graph.
+struct BatchGroup
+{
+  std::vector<int> sims;
+  int *list = nullptr;                  // the same indices in device memory (batch_advance)
+  int4 *tiles[4] = {};                  // tile tables for tiles of kLx[0..3] x rows
+  int ntiles[4] = {};
+  hipGraphExec_t graph = nullptr;
+};
 
 struct yee_batch
 {
@@ -48,22 +63,21 @@ struct yee_batch
   std::vector<double> mat_u;            // [sim][2] uniform eps, mu
   std::vector<void *> mat;              // [sim][2] eps, mu grids (nullptr: uniform)
   std::vector<void *> grids;            // every device allocation of the batch
-  std::vector<void *> f, d;             // [sim][3] Ez, Hx, Hy and Dz, Bx, By
+  std::vector<void *> f[2], d[2];       // [parity][sim][3] Ez, Hx, Hy and Dz, Bx, By
   std::vector<void *> pml;              // [sim] PML tables
   std::vector<int> tab_len;
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = { nullptr, nullptr };
-  void *sims = nullptr;                 // StepArgs<T>[nsim]
-  int4 *tiles[4] = {};                  // tile tables for TX = 1, 2, 4, 8
-  int ntiles[4] = {};
-  int *step = nullptr;                  // device step counter
+  void *sims = nullptr;                 // StepArgs<T>[nsim][2] (by step parity)
+  std::vector<std::vector<BatchGroup>> groupings;   // [grouping][group], one per cache budget
+  int *step = nullptr;                  // device step counters, one per simulation
+  int *all = nullptr;                   // 0 .. nsim-1 in device memory
   void *src_tab = nullptr;              // [sim][src_len][2] source values
   int src_len = 0;
   int32_t t = 0;
   bool prepared = false;
-  int tx_sel = -1, grid = 0;            // chosen tile height index and workgroup count
+  int cand = -1;                        // chosen tuner candidate
   int ncu = 256;
-  hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   double last_ms = 0;
   std::vector<double> tune_ms;          // [candidate] ms per step measured by the tuner
@@ -80,7 +94,7 @@ static thread_local std::string g_berr;
     if (e_ != hipSuccess) return yee::batch_fail (YEE_EDEVICE, std::string (#expr) + ": " + hipGetErrorString (e_)); \
   } while (0)
 
-constexpr int kTx[4] = { 1, 2, 4, 8 };
+constexpr int kLx[4] = { 13, 24, 40, 72 };   // x rows per tile
 
 int alloc_zero (yee_batch *b, void **p, size_t bytes)
 {
@@ -130,9 +144,10 @@ int upload_grid (yee_batch *b, int sim, int which, const S *src)
   return YEE_OK;
 }
 
-// Per-simulation StepArgs: the same set-up make_args performs for a single TMz handle.
+// Per-simulation StepArgs: the same set-up make_args performs for a single TMz handle, with
+// the old state in buffer `parity` and the new one in the other.
 template <typename T>
-StepArgs<T> sim_args (const yee_batch *b, int s)
+StepArgs<T> sim_args (const yee_batch *b, int s, int parity)
 {
   const yee_config &c = b->cfg[s];
   const bool pml = (c.flags & YEE_USE_PML) != 0;
@@ -152,8 +167,10 @@ StepArgs<T> sim_args (const yee_batch *b, int s)
   for (int k = 0; k < 3; ++k)
   {
     const int comp = k == 0 ? YEE_EZ : (k == 1 ? YEE_HX : YEE_HY);
-    a.f[comp] = a.f_n[comp] = (T *) b->f[3 * s + k];
-    a.d[comp] = a.d_n[comp] = (T *) b->d[3 * s + k];
+    a.f[comp] = (T *) b->f[parity][3 * s + k];
+    a.f_n[comp] = (T *) b->f[1 - parity][3 * s + k];
+    a.d[comp] = (T *) b->d[parity][3 * s + k];
+    a.d_n[comp] = (T *) b->d[1 - parity][3 * s + k];
   }
   a.eps = (const T *) b->mat[2 * s];
   a.mu = (const T *) b->mat[2 * s + 1];
@@ -169,7 +186,22 @@ StepArgs<T> sim_args (const yee_batch *b, int s)
   const int64_t *p = c.point_source_pos;
   a.src_mask = c.point_source & ((1u << YEE_EZ) | (1u << YEE_HX) | (1u << YEE_HY));
   a.src_q = p[0] * c.size[1] + p[1];
+  a.src_pos[0] = (int) p[0];
+  a.src_pos[1] = (int) p[1];
   return a;
+}
+
+// Destroys every captured graph (they hold the source table's address).
+int drop_graphs (yee_batch *b)
+{
+  for (auto &gs : b->groupings)
+    for (BatchGroup &g : gs)
+      if (g.graph)
+      {
+        BHIP (hipGraphExecDestroy (g.graph));
+        g.graph = nullptr;
+      }
+  return YEE_OK;
 }
 
 // Source values of steps [0, len) for every simulation (host FP64 C library, as the single
@@ -192,12 +224,8 @@ int upload_sources (yee_batch *b, int len)
         v[((size_t) s * len + t) * 2 + h] = (T) val;
       }
   BHIP (hipStreamSynchronize (b->stream));   // queued steps may still read the old table
-  if (b->graph)
-  {
-    // the captured steps hold the old table's address
-    BHIP (hipGraphExecDestroy (b->graph));
-    b->graph = nullptr;
-  }
+  int rc = drop_graphs (b);                  // the captured steps hold the old table's address
+  if (rc) return rc;
   if (b->src_tab) BHIP (hipFree (b->src_tab));
   b->src_tab = nullptr;
   BHIP (hipMalloc (&b->src_tab, v.size () * sizeof (T)));
@@ -207,73 +235,64 @@ int upload_sources (yee_batch *b, int len)
 }
 
 template <typename T>
-void launch_half (yee_batch *b, int sel, int grid, bool e_half)
+void launch_step (yee_batch *b, const BatchGroup &g, int sel)
 {
-  const int tx = kTx[sel];
-  const dim3 block (64, (unsigned) tx, 1);
-  const int nt = b->ntiles[sel];
-  const StepArgs<T> *sims = (const StepArgs<T> *) b->sims;
-  const T *tab = (const T *) b->src_tab;
-  const int eh = e_half ? 1 : 0;
-  switch (tx)
-  {
-    case 1: hipLaunchKernelGGL ((batch_tmz_half<T, 1>), dim3 (grid), block, 0, b->stream, sims, b->tiles[sel], nt, eh, b->step, tab, b->src_len); break;
-    case 2: hipLaunchKernelGGL ((batch_tmz_half<T, 2>), dim3 (grid), block, 0, b->stream, sims, b->tiles[sel], nt, eh, b->step, tab, b->src_len); break;
-    case 4: hipLaunchKernelGGL ((batch_tmz_half<T, 4>), dim3 (grid), block, 0, b->stream, sims, b->tiles[sel], nt, eh, b->step, tab, b->src_len); break;
-    default: hipLaunchKernelGGL ((batch_tmz_half<T, 8>), dim3 (grid), block, 0, b->stream, sims, b->tiles[sel], nt, eh, b->step, tab, b->src_len); break;
-  }
+  const int nt = g.ntiles[sel];
+  hipLaunchKernelGGL ((batch_tmz_step<T>), dim3 ((unsigned) std::max (1, (nt + 3) / 4)), dim3 (256), 0, b->stream,
+                     (const StepArgs<T> *) b->sims, g.tiles[sel], nt, b->step, (const T *) b->src_tab, b->src_len);
 }
 
-// One step t >= 1 (E, H, counter) with tile height kTx[sel] and `grid` workgroups.
-template <typename T>
-int enqueue_step (yee_batch *b, int sel, int grid)
+void launch_advance (yee_batch *b, const int *list, int n)
 {
-  launch_half<T> (b, sel, grid, true);
-  launch_half<T> (b, sel, grid, false);
-  hipLaunchKernelGGL (batch_advance, dim3 (1), dim3 (64), 0, b->stream, b->step);
-  BHIP (hipGetLastError ());
-  return YEE_OK;
+  hipLaunchKernelGGL (batch_advance, dim3 ((unsigned) ((n + 63) / 64)), dim3 (64), 0, b->stream, b->step, list, n);
 }
 
-// Candidates of the tuner: (tile height index, workgroups per CU).
+// Candidates of the tuner: (grouping, tile length index).
 struct Cand
 {
-  int sel, per_cu;
+  int grouping, sel;
 };
-// per_cu = 0: one tile per workgroup (grid = number of tiles, the hardware dispatcher is the
-// queue); otherwise that many persistent workgroups per CU walk the tile table.
-const Cand kCands[] = { { 2, 0 }, { 3, 0 }, { 1, 0 }, { 2, 8 }, { 3, 4 }, { 1, 16 }, { 2, 16 }, { 0, 0 } };
+// Cache budgets of the groupings (MiB of state per group; 0: the whole batch in one group).
+constexpr double kBudgetMiB[] = { 0, 160, 64 };
+constexpr int kNumGroupings = (int) (sizeof kBudgetMiB / sizeof kBudgetMiB[0]);
+const Cand kCands[] = { { 0, 0 }, { 0, 1 }, { 0, 2 }, { 0, 3 }, { 1, 0 }, { 1, 1 },
+                        { 1, 2 }, { 1, 3 }, { 2, 0 }, { 2, 1 }, { 2, 2 }, { 2, 3 } };
 constexpr int kNumCands = (int) (sizeof kCands / sizeof kCands[0]);
 constexpr int kTuneSteps = 3;
 constexpr int kGraphSteps = 16;
 
-int cand_grid (const yee_batch *b, const Cand &c)
+// One step t >= 1 of group g (fused E/H, counters) with candidate c's tiles.
+template <typename T>
+int enqueue_step (yee_batch *b, const BatchGroup &g, const Cand &c)
 {
-  if (c.per_cu == 0) return std::max (1, b->ntiles[c.sel]);
-  return std::max (1, std::min (b->ntiles[c.sel], c.per_cu * b->ncu));
+  launch_step<T> (b, g, c.sel);
+  launch_advance (b, g.list, (int) g.sims.size ());
+  BHIP (hipGetLastError ());
+  return YEE_OK;
 }
 
 template <typename T>
-int build_graph (yee_batch *b)
+int build_graph (yee_batch *b, BatchGroup &grp, const Cand &c)
 {
-  if (b->graph)
+  if (grp.graph)
   {
-    BHIP (hipGraphExecDestroy (b->graph));
-    b->graph = nullptr;
+    BHIP (hipGraphExecDestroy (grp.graph));
+    grp.graph = nullptr;
   }
   hipGraph_t g = nullptr;
   BHIP (hipStreamBeginCapture (b->stream, hipStreamCaptureModeThreadLocal));
   int rc = YEE_OK;
-  for (int s = 0; s < kGraphSteps && rc == YEE_OK; ++s) rc = enqueue_step<T> (b, b->tx_sel, b->grid);
+  for (int s = 0; s < kGraphSteps && rc == YEE_OK; ++s) rc = enqueue_step<T> (b, grp, c);
   hipError_t e = hipStreamEndCapture (b->stream, &g);
   if (rc) return rc;
   if (e != hipSuccess) return batch_fail (YEE_EDEVICE, std::string ("hipStreamEndCapture: ") + hipGetErrorString (e));
-  e = hipGraphInstantiate (&b->graph, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate (&grp.graph, g, nullptr, nullptr, 0);
   (void) hipGraphDestroy (g);
   if (e != hipSuccess) return batch_fail (YEE_EDEVICE, std::string ("hipGraphInstantiate: ") + hipGetErrorString (e));
   b->graph_steps = kGraphSteps;
   return YEE_OK;
 }
+
 
 template <typename T>
 int step_impl (yee_batch *b, int32_t n)
@@ -287,13 +306,14 @@ int step_impl (yee_batch *b, int32_t n)
   {
     hipLaunchKernelGGL ((batch_tmz_sources<T>), dim3 ((unsigned) ((b->nsim + 63) / 64)), dim3 (64), 0, b->stream,
                        (const StepArgs<T> *) b->sims, b->nsim, b->step, (const T *) b->src_tab, b->src_len);
-    hipLaunchKernelGGL (batch_advance, dim3 (1), dim3 (64), 0, b->stream, b->step);
+    launch_advance (b, b->all, b->nsim);
     BHIP (hipGetLastError ());
     ++done;
     ++b->t;
   }
-  // tuner: kTuneSteps ordinary steps per candidate, timed, on the first call that has room
-  if (b->tx_sel < 0 && n - done >= kNumCands * kTuneSteps)
+  // tuner: kTuneSteps ordinary steps of every simulation per candidate, timed, on the first
+  // call that has room
+  if (b->cand < 0 && n - done >= kNumCands * kTuneSteps)
   {
     b->tune_ms.assign (kNumCands, 0.0);
     hipEvent_t e0, e1;
@@ -302,10 +322,10 @@ int step_impl (yee_batch *b, int32_t n)
     double best = 1e30;
     for (int c = 0; c < kNumCands; ++c)
     {
-      const int grid = cand_grid (b, kCands[c]);
       BHIP (hipEventRecord (e0, b->stream));
-      for (int s = 0; s < kTuneSteps; ++s)
-        if ((rc = enqueue_step<T> (b, kCands[c].sel, grid))) return rc;
+      for (const BatchGroup &g : b->groupings[kCands[c].grouping])
+        for (int s = 0; s < kTuneSteps; ++s)
+          if ((rc = enqueue_step<T> (b, g, kCands[c]))) return rc;
       BHIP (hipEventRecord (e1, b->stream));
       BHIP (hipEventSynchronize (e1));
       float ms = 0;
@@ -314,8 +334,7 @@ int step_impl (yee_batch *b, int32_t n)
       if (ms < best)
       {
         best = ms;
-        b->tx_sel = kCands[c].sel;
-        b->grid = grid;
+        b->cand = c;
       }
       done += kTuneSteps;
       b->t += kTuneSteps;
@@ -323,30 +342,31 @@ int step_impl (yee_batch *b, int32_t n)
     (void) hipEventDestroy (e0);
     (void) hipEventDestroy (e1);
   }
-  if (b->tx_sel < 0)
-  {
-    // too few steps to tune: a middle candidate
-    b->tx_sel = kCands[1].sel;
-    b->grid = cand_grid (b, kCands[1]);
-  }
+  if (b->cand < 0) b->cand = 2;   // too few steps to tune: one group, 64-row tiles
+  const Cand &c = kCands[b->cand];
   const char *use_graph = std::getenv ("YEE_BATCH_GRAPH");
   const bool graphs = !(use_graph && std::strcmp (use_graph, "0") == 0);
-  if (graphs && !b->graph && n - done >= kGraphSteps && (rc = build_graph<T> (b))) return rc;
-  while (n - done > 0)
+  // each group runs all the remaining steps before the next one starts
+  const int32_t left = n - done;
+  for (BatchGroup &g : b->groupings[c.grouping])
   {
-    if (graphs && b->graph && n - done >= b->graph_steps)
+    if (graphs && !g.graph && left >= kGraphSteps && (rc = build_graph<T> (b, g, c))) return rc;
+    int32_t k = 0;
+    while (k < left)
     {
-      BHIP (hipGraphLaunch (b->graph, b->stream));
-      done += b->graph_steps;
-      b->t += b->graph_steps;
-    }
-    else
-    {
-      if ((rc = enqueue_step<T> (b, b->tx_sel, b->grid))) return rc;
-      ++done;
-      ++b->t;
+      if (graphs && g.graph && left - k >= b->graph_steps)
+      {
+        BHIP (hipGraphLaunch (g.graph, b->stream));
+        k += b->graph_steps;
+      }
+      else
+      {
+        if ((rc = enqueue_step<T> (b, g, c))) return rc;
+        ++k;
+      }
     }
   }
+  b->t += left;
   BHIP (hipEventRecord (b->ev[1], b->stream));
   BHIP (hipEventSynchronize (b->ev[1]));
   float ms = 0;
@@ -355,11 +375,27 @@ int step_impl (yee_batch *b, int32_t n)
   return YEE_OK;
 }
 
+// Device bytes a simulation's steps touch: the field grids, eps/mu grids, and the D/B grids
+// inside its PML frame.
+double sim_bytes (const yee_batch *b, int s)
+{
+  const yee_config &c = b->cfg[s];
+  const double cells = (double) c.size[0] * (double) c.size[1];
+  double arrays = 3 + (b->mat[2 * s] ? 1 : 0) + (b->mat[2 * s + 1] ? 1 : 0);
+  double bytes = cells * arrays * (double) b->wsize;
+  if (c.flags & YEE_USE_PML)
+  {
+    const double inner = (double) std::max<int64_t> (0, c.size[0] - 2 * c.pml[0]) * (double) std::max<int64_t> (0, c.size[1] - 2 * c.pml[1]);
+    bytes += (cells - inner) * 3 * (double) b->wsize;
+  }
+  return bytes;
+}
+
 template <typename T>
 int prepare_impl (yee_batch *b)
 {
   int rc;
-  std::vector<StepArgs<T>> args ((size_t) b->nsim);
+  std::vector<StepArgs<T>> args (2 * (size_t) b->nsim);
   for (int s = 0; s < b->nsim; ++s)
   {
     const yee_config &c = b->cfg[s];
@@ -374,36 +410,70 @@ int prepare_impl (yee_batch *b)
       BHIP (hipStreamSynchronize (b->stream));
       b->pml[s] = p;
     }
-    args[s] = sim_args<T> (b, s);
+    args[2 * s] = sim_args<T> (b, s, 0);
+    args[2 * s + 1] = sim_args<T> (b, s, 1);
   }
   if ((rc = alloc_zero (b, &b->sims, args.size () * sizeof (StepArgs<T>)))) return rc;
   BHIP (hipMemcpyAsync (b->sims, args.data (), args.size () * sizeof (StepArgs<T>), hipMemcpyHostToDevice, b->stream));
-  // tile tables, biggest simulations first so the tail of a sweep is made of small tiles
+  // groupings: simulations biggest first, packed greedily into groups of at most the budget
+  // (a simulation above it gets a group of its own); tile tables per group, so the tail of
+  // a sweep is made of small tiles
   std::vector<int> order ((size_t) b->nsim);
   for (int s = 0; s < b->nsim; ++s) order[s] = s;
   std::stable_sort (order.begin (), order.end (), [&] (int x, int y) {
     return b->cfg[x].size[0] * b->cfg[x].size[1] > b->cfg[y].size[0] * b->cfg[y].size[1];
   });
-  for (int sel = 0; sel < 4; ++sel)
+  b->groupings.assign (kNumGroupings, {});
+  for (int gi = 0; gi < kNumGroupings; ++gi)
   {
-    std::vector<int4> tv;
+    std::vector<BatchGroup> &gs = b->groupings[gi];
+    const double budget = kBudgetMiB[gi] * 1048576.0;
+    double used = 0;
     for (int s : order)
     {
-      const yee_config &c = b->cfg[s];
-      const int mode = (c.flags & YEE_USE_PML) ? 1 : 0;
-      for (int64_t x0 = 1; x0 < c.size[0] - 1; x0 += kTx[sel])
-        for (int64_t y0 = 1; y0 < c.size[1] - 1; y0 += 64)
-          tv.push_back (make_int4 (s, (int) x0, (int) y0, mode));
+      const double by = sim_bytes (b, s);
+      if (gs.empty () || (budget > 0 && used + by > budget))
+      {
+        gs.emplace_back ();
+        used = 0;
+      }
+      gs.back ().sims.push_back (s);
+      used += by;
     }
-    b->ntiles[sel] = (int) tv.size ();
-    void *p = nullptr;
-    if ((rc = alloc_zero (b, &p, std::max<size_t> (1, tv.size ()) * sizeof (int4)))) return rc;
-    BHIP (hipMemcpyAsync (p, tv.data (), tv.size () * sizeof (int4), hipMemcpyHostToDevice, b->stream));
-    b->tiles[sel] = (int4 *) p;
+    for (BatchGroup &g : gs)
+    {
+      void *p = nullptr;
+      if ((rc = alloc_zero (b, &p, g.sims.size () * sizeof (int)))) return rc;
+      BHIP (hipMemcpyAsync (p, g.sims.data (), g.sims.size () * sizeof (int), hipMemcpyHostToDevice, b->stream));
+      g.list = (int *) p;
+      for (int sel = 0; sel < 4; ++sel)
+      {
+        std::vector<int4> tv;
+        // tiles (sim, xa, y0, xb): rows [xa, xb) of columns [y0, y0 + 63)
+        for (int s : g.sims)
+        {
+          const yee_config &c = b->cfg[s];
+          for (int64_t x0 = 1; x0 < c.size[0] - 1; x0 += kLx[sel])
+            for (int64_t y0 = 1; y0 < c.size[1] - 1; y0 += 63)
+              tv.push_back (make_int4 (s, (int) x0, (int) y0, (int) std::min<int64_t> (x0 + kLx[sel], c.size[0] - 1)));
+        }
+        g.ntiles[sel] = (int) tv.size ();
+        if ((rc = alloc_zero (b, &p, std::max<size_t> (1, tv.size ()) * sizeof (int4)))) return rc;
+        BHIP (hipMemcpyAsync (p, tv.data (), tv.size () * sizeof (int4), hipMemcpyHostToDevice, b->stream));
+        g.tiles[sel] = (int4 *) p;
+      }
+    }
   }
-  void *st = nullptr;
-  if ((rc = alloc_zero (b, &st, 256))) return rc;
-  b->step = (int *) st;
+  {
+    void *st = nullptr;
+    if ((rc = alloc_zero (b, &st, std::max<size_t> (256, (size_t) b->nsim * sizeof (int))))) return rc;
+    b->step = (int *) st;
+    std::vector<int> all ((size_t) b->nsim);
+    for (int s = 0; s < b->nsim; ++s) all[s] = s;
+    if ((rc = alloc_zero (b, &st, all.size () * sizeof (int)))) return rc;
+    BHIP (hipMemcpyAsync (st, all.data (), all.size () * sizeof (int), hipMemcpyHostToDevice, b->stream));
+    b->all = (int *) st;
+  }
   BHIP (hipStreamSynchronize (b->stream));
   int maxT = 1;
   for (const yee_config &c : b->cfg) maxT = std::max (maxT, (int) c.time_steps);
@@ -460,8 +530,11 @@ int yee_batch_create (const yee_config *cfgs, int32_t nsim, yee_batch **out)
   b->sigma.assign (nsim, 0.0);
   b->mat_u.assign (2 * (size_t) nsim, 1.0);
   b->mat.assign (2 * (size_t) nsim, nullptr);
-  b->f.assign (3 * (size_t) nsim, nullptr);
-  b->d.assign (3 * (size_t) nsim, nullptr);
+  for (int par = 0; par < 2; ++par)
+  {
+    b->f[par].assign (3 * (size_t) nsim, nullptr);
+    b->d[par].assign (3 * (size_t) nsim, nullptr);
+  }
   b->pml.assign (nsim, nullptr);
   b->tab_len.assign (nsim, 0);
   auto bail = [&] (int code) {
@@ -476,12 +549,13 @@ int yee_batch_create (const yee_config *cfgs, int32_t nsim, yee_batch **out)
   for (int32_t s = 0; s < nsim; ++s)
   {
     const size_t bytes = (size_t) (cfgs[s].size[0] * cfgs[s].size[1]) * b->wsize;
-    for (int k = 0; k < 3; ++k)
-    {
-      int rc = alloc_zero (b, &b->f[3 * s + k], bytes);
-      if (rc) return bail (rc);
-      if ((cfgs[s].flags & YEE_USE_PML) && (rc = alloc_zero (b, &b->d[3 * s + k], bytes))) return bail (rc);
-    }
+    for (int par = 0; par < 2; ++par)
+      for (int k = 0; k < 3; ++k)
+      {
+        int rc = alloc_zero (b, &b->f[par][3 * s + k], bytes);
+        if (rc) return bail (rc);
+        if ((cfgs[s].flags & YEE_USE_PML) && (rc = alloc_zero (b, &b->d[par][3 * s + k], bytes))) return bail (rc);
+      }
     b->sc.push_back (compute_scalars (cfgs[s]));
   }
   if (hipStreamSynchronize (b->stream) != hipSuccess) return bail (batch_fail (YEE_EDEVICE, "stream sync failed"));
@@ -493,7 +567,7 @@ void yee_batch_destroy (yee_batch *b)
 {
   if (!b) return;
   if (b->stream) (void) hipStreamSynchronize (b->stream);
-  if (b->graph) (void) hipGraphExecDestroy (b->graph);
+  (void) drop_graphs (b);
   for (void *p : b->grids) (void) hipFree (p);
   if (b->src_tab) (void) hipFree (b->src_tab);
   for (hipEvent_t e : b->ev)
@@ -571,11 +645,21 @@ int yee_batch_last_step_ms (yee_batch *b, double *ms)
   return YEE_OK;
 }
 
-int yee_batch_tuning (yee_batch *b, int32_t out[2], double *cand_ms, int32_t ncand)
+int yee_batch_tuning (yee_batch *b, int32_t out[3], double *cand_ms, int32_t ncand)
 {
   if (!b || !out) return batch_fail (YEE_EINVAL, "null argument");
-  out[0] = b->tx_sel >= 0 ? kTx[b->tx_sel] : 0;
-  out[1] = b->grid;
+  if (b->cand >= 0)
+  {
+    const Cand &c = kCands[b->cand];
+    const std::vector<BatchGroup> &gs = b->groupings[c.grouping];
+    out[0] = kLx[c.sel];
+    out[1] = gs.empty () ? 0 : (gs[0].ntiles[c.sel] + 3) / 4;
+    out[2] = (int32_t) gs.size ();
+  }
+  else
+  {
+    out[0] = out[1] = out[2] = 0;
+  }
   for (int32_t c = 0; cand_ms && c < ncand && c < (int32_t) b->tune_ms.size (); ++c) cand_ms[c] = b->tune_ms[c];
   return YEE_OK;
 }
@@ -588,12 +672,13 @@ int yee_batch_get_field (yee_batch *b, int32_t sim, int32_t comp, double *out)
   void *src;
   switch (comp)
   {
-    case YEE_EZ: k = 0; src = b->f[3 * sim]; break;
-    case YEE_HX: k = 1; src = b->f[3 * sim + 1]; break;
-    case YEE_HY: k = 2; src = b->f[3 * sim + 2]; break;
-    case YEE_DZ: k = 0; src = b->d[3 * sim]; break;
-    case YEE_BX: k = 1; src = b->d[3 * sim + 1]; break;
-    case YEE_BY: k = 2; src = b->d[3 * sim + 2]; break;
+    // the newest state is in buffer t & 1 (step t reads buffer t & 1 and writes the other)
+    case YEE_EZ: k = 0; src = b->f[b->t & 1][3 * sim]; break;
+    case YEE_HX: k = 1; src = b->f[b->t & 1][3 * sim + 1]; break;
+    case YEE_HY: k = 2; src = b->f[b->t & 1][3 * sim + 2]; break;
+    case YEE_DZ: k = 0; src = b->d[b->t & 1][3 * sim]; break;
+    case YEE_BX: k = 1; src = b->d[b->t & 1][3 * sim + 1]; break;
+    case YEE_BY: k = 2; src = b->d[b->t & 1][3 * sim + 2]; break;
     default: return batch_fail (YEE_EINVAL, "TMz components are Ez, Hx, Hy (and Dz, Bx, By with PML)");
   }
   (void) k;

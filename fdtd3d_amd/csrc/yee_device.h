@@ -47,6 +47,23 @@ __device__ __forceinline__ void rst (T *row, uint32_t boff, T v)
   *(gT *) p = v;
 }
 
+// The value of lane + 1 of the wavefront (lane 63: `edge`): a DPP wave shift (wave_shl:1),
+// one vector op, with no LDS traffic or wait.
+__device__ __forceinline__ float lane_next (float v, float edge)
+{
+  return __builtin_bit_cast (float, __builtin_amdgcn_update_dpp (__builtin_bit_cast (int, edge),
+                                                                __builtin_bit_cast (int, v), 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double lane_next (double v, double edge)
+{
+  const int2 vv = __builtin_bit_cast (int2, v), ee = __builtin_bit_cast (int2, edge);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp (ee.x, vv.x, 0x130, 0xf, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp (ee.y, vv.y, 0x130, 0xf, 0xf, false);
+  return __builtin_bit_cast (double, r);
+}
+
 // Direct-update coefficient cb = dt / (m m0 dx) of computeCaCb (InternalScheme.inc.h:2340-2344).
 // FP64 divides exactly (the reference's operations); FP32 multiplies by the hardware reciprocal
 // (1 ulp), several times cheaper than an IEEE division and far inside the FP32 tolerance.

@@ -26,7 +26,31 @@ struct Old
   T d, dp, m, mp;
 };
 
-// New value of one field component at flat index q (reference formulas, see file header).
+// PML coefficients of component COMP at cell p (computeCaCb, PML branch): Ca, Cb of the
+// D/B update along the component's axis, the D/B -> field factors along the other two.
+template <typename T>
+struct PmlCoef
+{
+  T cac, cbc, p4, m4, ca5, dd5;
+};
+
+template <typename T, int COMP>
+__device__ __forceinline__ PmlCoef<T> pml_coef (const StepArgs<T> &a, const int p[3])
+{
+  const int L = a.tab_len;
+  const T *tab = a.pml + (size_t) COMP * 6 * L;
+  PmlCoef<T> c;
+  c.cac = gld (tab + 0 * L + p[SigAxes<COMP>::c]);
+  c.cbc = gld (tab + 1 * L + p[SigAxes<COMP>::c]);
+  c.p4 = gld (tab + 2 * L + p[SigAxes<COMP>::s4]);
+  c.m4 = gld (tab + 3 * L + p[SigAxes<COMP>::s4]);
+  c.ca5 = gld (tab + 4 * L + p[SigAxes<COMP>::s5]);
+  c.dd5 = gld (tab + 5 * L + p[SigAxes<COMP>::s5]);
+  return c;
+}
+
+// New value of one field component at flat index q (reference formulas, see file header),
+// with the PML coefficients of the cell given (special cells of MODE 1 and 2).
 //   f_old : the component's previous value
 //   old   : its D/B history (special cells only; loaded by the caller)
 //   tmp   : curl operand combination (o12 - o11 - o22 + o21)
@@ -34,8 +58,8 @@ struct Old
 //   om/ga : averaged Drude omega / gamma (MODE 2 special cells)
 //   store : write the auxiliary D/B (D1/B1) state of this cell
 template <typename T, int MODE, int COMP>
-__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, const Old<T> &old,
-                                         T tmp, T mat, bool special, T om, T ga, bool store)
+__device__ __forceinline__ T cell_value_pc (const StepArgs<T> &a, int64_t q, T f_old, const Old<T> &old, T tmp, T mat,
+                                            bool special, T om, T ga, bool store, const PmlCoef<T> &pc)
 {
   const T d_old = old.d;
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
@@ -45,14 +69,7 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
     const T cb = direct_cb (a.dt, mat, modc, a.dx);
     return f_old * (T) 1.0 + tmp * cb;
   }
-  const int L = a.tab_len;
-  const T *tab = a.pml + (size_t) COMP * 6 * L;
-  const T cac = gld (tab + 0 * L + p[SigAxes<COMP>::c]);
-  const T cbc = gld (tab + 1 * L + p[SigAxes<COMP>::c]);
-  const T p4 = gld (tab + 2 * L + p[SigAxes<COMP>::s4]);
-  const T m4 = gld (tab + 3 * L + p[SigAxes<COMP>::s4]);
-  const T ca5 = gld (tab + 4 * L + p[SigAxes<COMP>::s5]);
-  const T dd5 = gld (tab + 5 * L + p[SigAxes<COMP>::s5]);
+  const T cac = pc.cac, cbc = pc.cbc, p4 = pc.p4, m4 = pc.m4, ca5 = pc.ca5, dd5 = pc.dd5;
   // curl update of D/B with the PML Ca/Cb (computeCaCb, PML branch)
   const T d_new = d_old * cac + tmp * cbc;
   T f_new;
@@ -86,6 +103,16 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
   }
   if (store) gst (a.d_n[COMP] + q, d_new);
   return f_new;
+}
+
+// The same, loading the PML coefficients of cell p where they are needed.
+template <typename T, int MODE, int COMP>
+__device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const int p[3], T f_old, const Old<T> &old,
+                                         T tmp, T mat, bool special, T om, T ga, bool store)
+{
+  PmlCoef<T> pc {};
+  if (MODE != 0 && special) pc = pml_coef<T, COMP> (a, p);
+  return cell_value_pc<T, MODE, COMP> (a, q, f_old, old, tmp, mat, special, om, ga, store, pc);
 }
 
 // In-place update of one component (half-step kernels).  The old values are loaded by the
@@ -449,7 +476,7 @@ struct PlaneLoads
   T hxr[BY + 2], hzr[BY + 2];          // H before the step, rows -1..BY
   T hy[BY + 1], hym[BY + 1], hxm[BY + 1];   // Hy rows 0..BY; z-1 neighbours of Hy, Hx
   T ex[BY + 1], ey[BY + 1], ez[BY + 1];     // owned cells: E before the step; others: E after it
-  T ep1[BY + 1], epk[BY + 1];               // eps at plane x+1 and at z+1
+  T ep1[BY + 1];                            // eps at plane x+1
   uint32_t ownb;                            // bit r: the kernel owns the lane's cell of row r
   uint32_t xb;                              // axis_bits of x
 };
@@ -488,18 +515,17 @@ __device__ __forceinline__ void e_load (const StepArgs<T> &a, int x, const Fused
   }
   if (MAT & kMatEps)
   {
-    const T *E0 = plane_row (a, a.eps, x, g), *E1 = plane_row (a, a.eps, min (x + 1, a.nx - 1), g);
+    const T *E1 = plane_row (a, a.eps, min (x + 1, a.nx - 1), g);
 #pragma unroll
     for (int r = 0; r <= BY; ++r)
     {
       L.ep1[r] = rld (E1, o[r + 1]);
-      L.epk[r] = rld (E0, o[r + 1], 1);
     }
   }
   else
   {
 #pragma unroll
-    for (int r = 0; r <= BY; ++r) L.ep1[r] = L.epk[r] = a.eps_u;
+    for (int r = 0; r <= BY; ++r) L.ep1[r] = a.eps_u;
   }
   asm volatile ("" : "+v"(ownb));   // kept as lane bits, not as scalar lane masks
   L.ownb = ownb;
@@ -534,11 +560,12 @@ struct Carry
   uint32_t xb;                            // axis_bits of i
 };
 
-// E update of plane x (rows 0..BY) from its loads L and the H / eps of plane x-1 carried in
-// `prev`; the result becomes the carried state of plane x.
+// E update of plane x (rows 0..BY) from its loads L, the H / eps of plane x-1 carried in
+// `prev` and eps of plane x at z+1 (epk); the result becomes the carried state of plane x.
 template <typename T, int BY>
 __device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const FusedPos &g, const PlaneLoads<T, BY> &L,
-                                           const Carry<T, BY> &prev, const double (*zl)[BY + 1][4], Carry<T, BY> &o)
+                                           const Carry<T, BY> &prev, const T (&epk)[BY + 1],
+                                           const double (*zl)[BY + 1][4], Carry<T, BY> &o)
 {
   const uint32_t xb = L.xb;
   // z-face corrections: only waves holding a face lane, on planes inside the TF/SF box
@@ -560,7 +587,7 @@ __device__ __forceinline__ void e_compute (const StepArgs<T> &a, int x, const Fu
     }
     // cb = dt / (m eps0 dx), m the 2-cell average (a uniform eps averages to itself)
     const T cbx = direct_cb (a.dt, avg2 (e, L.ep1[r]), a.eps0, a.dx);
-    const T cbz = direct_cb (a.dt, avg2 (e, L.epk[r]), a.eps0, a.dx);
+    const T cbz = direct_cb (a.dt, avg2 (e, epk[r]), a.eps0, a.dx);
     const T cby = r < BY ? direct_cb (a.dt, avg2 (e, prev.ep1[r + 1]), a.eps0, a.dx) : (T) 0;
     const T nx = direct_value_cb (L.ex[r], curl (vx), cbx);
     const T nz = direct_value_cb (L.ez[r], curl (vz), cbz);
@@ -600,12 +627,14 @@ __device__ __forceinline__ void e_store (const StepArgs<T> &a, int x, const Fuse
   }
 }
 
-// LDS of the fused kernel: Ex, Ey after the step at lane 0 of every wave (by plane parity),
-// and the z-face TF/SF corrections of the strip ([parity][part][face][row][slot]).
+// LDS of the fused kernel: Ex, Ey after the step and eps of the next plane at lane 0 of every
+// wave (the z+1 neighbours of lane 63 of the wave before; by plane parity), and the z-face
+// TF/SF corrections of the strip ([parity][part][face][row][slot]).
 template <typename T, int BY>
 struct FusedLds
 {
   T xch[2][16][BY][2];
+  T epx[2][16][BY + 1];
   double zl[2][2][3][BY + 1][4];
 };
 
@@ -623,7 +652,13 @@ __device__ __forceinline__ void fused_plane (const StepArgs<T> &a, const FusedPo
   e_load<T, BY, MAT> (a, min (x + 1, xend), g, ro, nxt);   // prefetch the next plane
   // z-face corrections of the next plane (written to LDS before the barrier)
   const double zpre = zstage_load<T, BY> (a, g.k, min (x + 1, xend), g.j0);
-  e_compute<T, BY> (a, x, g, L, c, s.zl[PAR][0], o);
+  // z+1 neighbours from the next lane (DPP shift); lane 63 takes lane 0 of the next wave,
+  // which that wave left in LDS before the last barrier
+  const bool edge = lane == 63 && wave + 1 < nw;
+  T epk[BY + 1];
+#pragma unroll
+  for (int r = 0; r <= BY; ++r) epk[r] = lane_next (c.ep1[r], edge ? s.epx[PAR ^ 1][wave + 1][r] : c.ep1[r]);
+  e_compute<T, BY> (a, x, g, L, c, epk, s.zl[PAR][0], o);
   if (lane == 0)
   {
 #pragma unroll
@@ -632,19 +667,16 @@ __device__ __forceinline__ void fused_plane (const StepArgs<T> &a, const FusedPo
       s.xch[PAR][wave][r][0] = o.ex[r];
       s.xch[PAR][wave][r][1] = o.ey[r];
     }
+#pragma unroll
+    for (int r = 0; r <= BY; ++r) s.epx[PAR][wave][r] = o.ep1[r];
   }
   // z+1 neighbours of Ex, Ey after the step at plane i
   T exk[BY + 1], eyk[BY + 1];
 #pragma unroll
   for (int r = 0; r < BY; ++r)
   {
-    exk[r] = __shfl_down (c.ex[r], 1);
-    eyk[r] = __shfl_down (c.ey[r], 1);
-    if (lane == 63 && wave + 1 < nw)
-    {
-      exk[r] = s.xch[PAR ^ 1][wave + 1][r][0];
-      eyk[r] = s.xch[PAR ^ 1][wave + 1][r][1];
-    }
+    exk[r] = lane_next (c.ex[r], edge ? s.xch[PAR ^ 1][wave + 1][r][0] : c.ex[r]);
+    eyk[r] = lane_next (c.ey[r], edge ? s.xch[PAR ^ 1][wave + 1][r][1] : c.ey[r]);
   }
   // ---- H of plane i, rows 0..BY-1
   const uint32_t ib = c.xb;
@@ -767,6 +799,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
   {
     // prologue: E at plane xa (H before the step at xa-1 and eps at xa are loaded here)
     Carry<T, BY> cm;
+    T epk0[BY + 1];   // eps of plane xa at z+1
     {
       uint32_t o[BY + 2];
       fresh_offsets (ro, o);
@@ -781,12 +814,16 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
       {
         const T *E0 = plane_row (a, a.eps, xa, g);
 #pragma unroll
-        for (int r = 0; r <= BY; ++r) cm.ep1[r] = rld (E0, o[r + 1]);
+        for (int r = 0; r <= BY; ++r)
+        {
+          cm.ep1[r] = rld (E0, o[r + 1]);
+          epk0[r] = rld (E0, o[r + 1], 1);
+        }
       }
       else
       {
 #pragma unroll
-        for (int r = 0; r <= BY; ++r) cm.ep1[r] = a.eps_u;
+        for (int r = 0; r <= BY; ++r) cm.ep1[r] = epk0[r] = a.eps_u;
       }
     }
     PlaneLoads<T, BY> L;
@@ -798,7 +835,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
       for (int t = g.k; t < kZStage; t += (int) blockDim.x) (&s.zl[0][0][0][0][0])[t] = zstage_load<T, BY> (a, t, xa, g.j0);
       __syncthreads ();
     }
-    e_compute<T, BY> (a, xa, g, L, cm, s.zl[0][0], c0);
+    e_compute<T, BY> (a, xa, g, L, cm, epk0, s.zl[0][0], c0);
     const double zpre = zstage_load<T, BY> (a, g.k, min (xa + 1, xend), g.j0);
     e_store<T, BY> (a, xa, g, ro, c0);
     e_load<T, BY, MAT> (a, min (xa + 1, xend), g, ro, p0);
@@ -810,6 +847,8 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
         s.xch[0][wave][r][0] = c0.ex[r];
         s.xch[0][wave][r][1] = c0.ey[r];
       }
+#pragma unroll
+      for (int r = 0; r <= BY; ++r) s.epx[0][wave][r] = c0.ep1[r];
     }
     if (a.tfsf)
     {
@@ -945,49 +984,205 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------------------
-// Batched TMz half-steps: many independent simulations of different sizes in one launch.
+// Batched TMz step: many independent simulations of different sizes, E then H of one time
+// step fused in one launch.
 //
-// Persistent workgroups walk a tile table (a static round-robin queue): tile = (simulation,
-// x0, y0) covers TX x rows and 64 y cells of that simulation's interior.  The per-simulation
-// set-up (grid pointers, sizes, PML tables, coefficients) is a StepArgs record in device
-// memory, read with scalar loads (uniform per tile).  The time step comes from device
-// memory, so a captured graph of several steps can be replayed; hard point sources are
-// applied by the thread that updates the source cell, from a per-simulation table of values.
+// Each wave takes one tile of a tile table: (simulation, x rows [xa, xb), first column y0).
+// Lane l holds column y0 + l; lanes 0..62 own their column, lane 63 recomputes E of the next
+// tile's first column, which H of column y0 + 62 needs.  The wave marches along x with E one
+// row ahead of H: E at row x needs H before the step at rows x and x-1 (x-1 carried in
+// registers) and at column j-1 (the same row, one element back); H at row x-1 needs E after
+// the step at rows x-1 (carried, and at column j+1 from the next lane) and x.  E of row xb is
+// recomputed for H of row xb-1 and not stored.  The state is double-buffered by step parity
+// (old values from f / d, new ones to f_n / d_n), so recomputed halo cells read the same old
+// values as their owner.  Each cell's arithmetic is cell_value with the operands of
+// tmz_e_cell / tmz_h_cell, so a batch member is bit-identical to the single-engine run.
+//
+// The per-simulation set-up (grid pointers of both parities, sizes, PML tables, coefficients)
+// is a StepArgs record in device memory, read with scalar loads (uniform per tile).  The time
+// step of each simulation comes from device memory (step[sim]), so a captured graph of several
+// steps can be replayed and groups of simulations can run ahead of one another; hard point
+// sources are applied to the source cell right after its update, from a per-simulation table.
 // ---------------------------------------------------------------------------------------
-template <typename T, int TX>
-__global__ __launch_bounds__ (64 * TX) void batch_tmz_half (const StepArgs<T> *sims, const int4 *tiles, int ntiles,
-                                                          int e_half, const int *step, const T *src_tab, int src_len)
+// Old values of one row of a TMz tile (the lane's column), loaded one row ahead.
+template <typename T>
+struct TmzRow
 {
-  const int t = *step;
-  for (int tile = (int) blockIdx.x; tile < ntiles; tile += (int) gridDim.x)
+  T ez, hx, hy, hx_l;   // Ez, Hx, Hy before the step; Hx at column j-1
+  T eps;                // eps (a.eps_u without an eps grid)
+  T dz, bx, by;         // D/B before the step (PML cells; 0 elsewhere)
+  // PML coefficients that depend on x (uniform over the row): Ez Ca/Cb, Hx p4/m4, Hy ca5/dd5
+  T ez_cac, ez_cbc, hx_p4, hx_m4, hy_ca5, hy_dd5;
+  bool special;
+};
+
+// The PML coefficients of a TMz cell (x, j, 0) by the axis they depend on (SigAxes): x (the
+// row, TmzRow), j (the lane's column, constant over a tile) or z = 0 (constant).
+static_assert (SigAxes<2>::c == 0 && SigAxes<2>::s4 == 2 && SigAxes<2>::s5 == 1, "Ez PML axes");
+static_assert (SigAxes<3>::c == 1 && SigAxes<3>::s4 == 0 && SigAxes<3>::s5 == 2, "Hx PML axes");
+static_assert (SigAxes<4>::c == 2 && SigAxes<4>::s4 == 1 && SigAxes<4>::s5 == 0, "Hy PML axes");
+
+template <typename T>
+struct TmzColCoef
+{
+  T ez_ca5, ez_dd5, hx_cac, hx_cbc, hy_p4, hy_m4;   // column j
+  T ez_p4, ez_m4, hx_ca5, hx_dd5, hy_cac, hy_cbc;   // z = 0
+};
+
+template <typename T>
+__device__ __forceinline__ T pml_slot (const StepArgs<T> &a, int comp, int slot, int idx)
+{
+  return gld (a.pml + ((size_t) comp * 6 + slot) * a.tab_len + idx);
+}
+
+// The same as a vector load even for a uniform index (vector loads complete in order, so a
+// value prefetched one row ahead is not waited for with the current row's; scalar loads are
+// waited for all together).
+template <typename T>
+__device__ __forceinline__ T pml_slot_v (const StepArgs<T> &a, int comp, int slot, int idx)
+{
+  int z = 0;
+  asm volatile ("" : "+v"(z));
+  return gld (a.pml + ((size_t) comp * 6 + slot) * a.tab_len + idx + z);
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void tmz_row_load (const StepArgs<T> &a, int x, int j, int jc, TmzRow<T> &r)
+{
+  const int64_t q = (int64_t) x * a.sx + jc;
+  r.ez = a.f[2][q];
+  r.hx = a.f[3][q];
+  r.hy = a.f[4][q];
+  r.hx_l = a.f[3][q - 1];
+  r.eps = a.eps ? a.eps[q] : a.eps_u;
+  const int p[3] = { x, j, 0 };
+  r.special = is_special (a, p, 2, MODE);
+  r.dz = r.bx = r.by = (T) 0;
+  if (MODE != 0)
   {
-    const int4 td = tiles[tile];   // sim, x0, y0, mode
-    const StepArgs<T> &a = sims[td.x];
-    const int i = td.y + (int) threadIdx.y;
-    const int j = td.z + (int) threadIdx.x;
-    if (i >= a.nx - 1 || j >= a.ny - 1) continue;
-    if (e_half)
+    r.ez_cac = pml_slot_v (a, 2, 0, x);
+    r.ez_cbc = pml_slot_v (a, 2, 1, x);
+    r.hx_p4 = pml_slot_v (a, 3, 2, x);
+    r.hx_m4 = pml_slot_v (a, 3, 3, x);
+    r.hy_ca5 = pml_slot_v (a, 4, 4, x);
+    r.hy_dd5 = pml_slot_v (a, 4, 5, x);
+    if (r.special)
     {
-      if (td.w) tmz_e_cell<T, 1> (a, i, j);
-      else tmz_e_cell<T, 0> (a, i, j);
-    }
-    else
-    {
-      if (td.w) tmz_h_cell<T, 1> (a, i, j);
-      else tmz_h_cell<T, 0> (a, i, j);
-    }
-    if (a.src_mask)
-    {
-      const int64_t q = (int64_t) i * a.sx + j;
-      if (q == a.src_q)
-      {
-        const T v = src_tab[((int64_t) td.x * src_len + t) * 2 + (e_half ? 0 : 1)];
-        if (e_half && (a.src_mask & 4u)) gst (a.f_n[2] + q, v);
-        if (!e_half && (a.src_mask & 8u)) gst (a.f_n[3] + q, v);
-        if (!e_half && (a.src_mask & 16u)) gst (a.f_n[4] + q, v);
-      }
+      r.dz = a.d[2][q];
+      r.bx = a.d[3][q];
+      r.by = a.d[4][q];
     }
   }
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void tmz_fused_tile (const StepArgs<T> &a, int xa, int xb, int j, int lane, T se, T sh)
+{
+  const int nx = a.nx, ny = a.ny;
+  const int64_t sx = a.sx;   // == ny
+  const int jc = min (j, ny - 1);   // lanes past the grid read its last column
+  const bool jin = j < ny - 1;      // interior column (j >= 1 always)
+  const bool own = jin && lane < 63;
+  const bool src_col = j == a.src_pos[1];
+  T *Ezn = a.f_n[2], *Hxn = a.f_n[3], *Hyn = a.f_n[4];
+  TmzColCoef<T> cc {};
+  if (MODE != 0)
+  {
+    cc.ez_ca5 = pml_slot (a, 2, 4, jc);
+    cc.ez_dd5 = pml_slot (a, 2, 5, jc);
+    cc.hx_cac = pml_slot (a, 3, 0, jc);
+    cc.hx_cbc = pml_slot (a, 3, 1, jc);
+    cc.hy_p4 = pml_slot (a, 4, 2, jc);
+    cc.hy_m4 = pml_slot (a, 4, 3, jc);
+    cc.ez_p4 = pml_slot (a, 2, 2, 0);
+    cc.ez_m4 = pml_slot (a, 2, 3, 0);
+    cc.hx_ca5 = pml_slot (a, 3, 4, 0);
+    cc.hx_dd5 = pml_slot (a, 3, 5, 0);
+    cc.hy_cac = pml_slot (a, 4, 0, 0);
+    cc.hy_cbc = pml_slot (a, 4, 1, 0);
+  }
+  // carried: the old row x-1 and E after the step at row x-1; rows x+1 and x+2 are loaded
+  // ahead (two rows of loads in flight per wave hide the memory latency of the march)
+  TmzRow<T> prev, cur, nx1, nx2;
+  prev.hy = a.f[4][(int64_t) (xa - 1) * sx + jc];
+  prev.hx = prev.bx = prev.by = (T) 0;
+  prev.special = false;
+  T ez_p = (T) 0;
+  tmz_row_load<T, MODE> (a, xa, j, jc, cur);
+  if (xa < xb) tmz_row_load<T, MODE> (a, xa + 1, j, jc, nx1);
+  for (int x = xa; x <= xb; ++x)
+  {
+    if (x + 2 <= xb) tmz_row_load<T, MODE> (a, x + 2, j, jc, nx2);   // prefetch
+    const int64_t q = (int64_t) x * sx + jc;
+    const bool xin = x < nx - 1;   // x >= 1 always
+    T ez_n = cur.ez;
+    if (jin && xin)
+    {
+      Old<T> dold;
+      dold.d = cur.dz;
+      dold.dp = dold.m = dold.mp = (T) 0;
+      T v[4] = { prev.hy, cur.hy, cur.hx_l, cur.hx };
+      const PmlCoef<T> pc = { cur.ez_cac, cur.ez_cbc, cc.ez_p4, cc.ez_m4, cc.ez_ca5, cc.ez_dd5 };
+      ez_n = cell_value_pc<T, MODE, 2> (a, q, cur.ez, dold, curl (v), avg2 (cur.eps, cur.eps), cur.special, (T) 0,
+                                        (T) 0, own && x < xb, pc);
+      if ((a.src_mask & 4u) && src_col && x == a.src_pos[0]) ez_n = se;
+      if (own && x < xb) Ezn[q] = ez_n;
+    }
+    const T ez_r = lane_next (ez_p, ez_p);   // E after the step at (x-1, j+1)
+    const int i = x - 1;
+    if (x > xa && own && i < nx - 1)
+    {
+      const int64_t qi = q - sx;
+      Old<T> o3, o4;
+      o3.dp = o3.m = o3.mp = (T) 0;
+      o4 = o3;
+      o3.d = prev.bx;
+      o4.d = prev.by;
+      // Hx: Ez at (i, j) and (i, j+1); Hy: Ez at (i, j) and (i+1, j)
+      T vx[4] = { (T) 0, (T) 0, ez_p, ez_r };
+      T vy[4] = { ez_p, ez_n, (T) 0, (T) 0 };
+      T mx = a.mu_u, my = a.mu_u;
+      if (a.mu)
+      {
+        const T *m = a.mu;
+        mx = avg4 (m[qi], m[qi], m[qi + 1], m[qi + 1]);
+        my = avg4 (m[qi], m[qi], m[q], m[q]);
+      }
+      const PmlCoef<T> px = { cc.hx_cac, cc.hx_cbc, prev.hx_p4, prev.hx_m4, cc.hx_ca5, cc.hx_dd5 };
+      const PmlCoef<T> py = { cc.hy_cac, cc.hy_cbc, cc.hy_p4, cc.hy_m4, prev.hy_ca5, prev.hy_dd5 };
+      T hxn = cell_value_pc<T, MODE, 3> (a, qi, prev.hx, o3, curl (vx), mx, prev.special, (T) 0, (T) 0, true, px);
+      T hyn = cell_value_pc<T, MODE, 4> (a, qi, prev.hy, o4, curl (vy), my, prev.special, (T) 0, (T) 0, true, py);
+      if (src_col && i == a.src_pos[0])
+      {
+        if (a.src_mask & 8u) hxn = sh;
+        if (a.src_mask & 16u) hyn = sh;
+      }
+      Hxn[qi] = hxn;
+      Hyn[qi] = hyn;
+    }
+    prev = cur;
+    cur = nx1;
+    nx1 = nx2;
+    ez_p = ez_n;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__ (256) void batch_tmz_step (const StepArgs<T> *sims, const int4 *tiles, int ntiles,
+                                                       const int *step, const T *src_tab, int src_len)
+{
+  const int lane = (int) (threadIdx.x & 63);
+  const int tile = (int) blockIdx.x * 4 + __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
+  if (tile >= ntiles) return;   // no barriers below
+  const int4 td = tiles[tile];   // sim, xa, y0, xb
+  const int t = step[td.x];
+  // a private copy: loads through a reference could not be kept across the tile's stores
+  // (the compiler cannot rule out that they alias the record)
+  const StepArgs<T> a = sims[2 * td.x + (t & 1)];
+  const T se = src_tab[((int64_t) td.x * src_len + t) * 2 + 0];
+  const T sh = src_tab[((int64_t) td.x * src_len + t) * 2 + 1];
+  if (a.pml) tmz_fused_tile<T, 1> (a, td.y, td.w, td.z + lane, lane, se, sh);
+  else tmz_fused_tile<T, 0> (a, td.y, td.w, td.z + lane, lane, se, sh);
 }
 
 // Step 0 performs no field update (Scheme.inc.h:1032) but the hard sources still fire.
@@ -996,8 +1191,8 @@ __global__ void batch_tmz_sources (const StepArgs<T> *sims, int nsim, const int 
 {
   const int s = (int) (blockIdx.x * blockDim.x + threadIdx.x);
   if (s >= nsim) return;
-  const int t = *step;
-  const StepArgs<T> &a = sims[s];
+  const int t = step[s];
+  const StepArgs<T> &a = sims[2 * s + (t & 1)];
   if (!a.src_mask) return;
   const T ve = src_tab[((int64_t) s * src_len + t) * 2 + 0];
   const T vh = src_tab[((int64_t) s * src_len + t) * 2 + 1];
@@ -1006,10 +1201,11 @@ __global__ void batch_tmz_sources (const StepArgs<T> *sims, int nsim, const int 
   if (a.src_mask & 16u) gst (a.f_n[4] + a.src_q, vh);
 }
 
-// Advances the device-side step counter (last node of every captured step).
-__global__ void batch_advance (int *step)
+// Advances the step counters of simulations sims[0..n) (last node of every captured step).
+__global__ void batch_advance (int *step, const int *sims, int n)
 {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+  const int u = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+  if (u < n) step[sims[u]] += 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1099,10 +1295,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void h_half_tmz<T, 0> (StepArgs<T>);                                 \
   template __global__ void h_half_tmz<T, 1> (StepArgs<T>);                                 \
   template __global__ void h_half_tmz<T, 2> (StepArgs<T>);                                 \
-  template __global__ void batch_tmz_half<T, 1> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
-  template __global__ void batch_tmz_half<T, 2> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
-  template __global__ void batch_tmz_half<T, 4> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
-  template __global__ void batch_tmz_half<T, 8> (const StepArgs<T> *, const int4 *, int, int, const int *, const T *, int); \
+  template __global__ void batch_tmz_step<T> (const StepArgs<T> *, const int4 *, int, const int *, const T *, int); \
   template __global__ void batch_tmz_sources<T> (const StepArgs<T> *, int, const int *, const T *, int); \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
   template __global__ void step_3d_fused<T, FUSED_BY (T), 0> (StepArgs<T>);                \

@@ -321,11 +321,12 @@ class Batch:
         _bcheck(load_library().yee_batch_last_step_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
-    def tuning(self, ncand: int = 8):
-        out = np.zeros(2, dtype=np.int32)
+    def tuning(self, ncand: int = 12):
+        out = np.zeros(3, dtype=np.int32)
         ms = np.zeros(ncand, dtype=np.float64)
         _bcheck(load_library().yee_batch_tuning(self.h, out.ctypes.data, ms.ctypes.data, ncand))
-        return {"tile_rows": int(out[0]), "workgroups": int(out[1]), "candidate_ms": [float(v) for v in ms]}
+        return {"tile_rows": int(out[0]), "workgroups": int(out[1]), "groups": int(out[2]),
+                "candidate_ms": [float(v) for v in ms]}
 
     def field(self, sim: int, name: str) -> np.ndarray:
         c = self.cfgs[sim]

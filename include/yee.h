@@ -266,9 +266,11 @@ int yee_batch_sync (yee_batch *b);
 int yee_batch_time (yee_batch *b, int32_t *t);
 /* average device ms per step of the last yee_batch_step call (HIP events on the batch stream) */
 int yee_batch_last_step_ms (yee_batch *b, double *ms);
-/* out[0] = chosen tile height (x rows per 64-cell tile), out[1] = persistent workgroups;
- * cand_ms[0..ncand) = ms per step the tuner measured for each candidate */
-int yee_batch_tuning (yee_batch *b, int32_t out[2], double *cand_ms, int32_t ncand);
+/* out[0] = chosen tile height (x rows per 64-cell tile), out[1] = workgroups per launch of
+ * the first group, out[2] = groups (simulations advanced together through all their steps
+ * before the next group, sized to stay cache resident); cand_ms[0..ncand) = ms per step the
+ * tuner measured for each candidate */
+int yee_batch_tuning (yee_batch *b, int32_t out[3], double *cand_ms, int32_t ncand);
 /* newest layer of Ez, Hx, Hy (or Dz, Bx, By) of simulation sim as FP64 [x][y] */
 int yee_batch_get_field (yee_batch *b, int32_t sim, int32_t comp, double *out);
 
