@@ -34,8 +34,10 @@ namespace yee {
 template <typename T>
 __global__ void batch_tmz_step (const StepArgs<T> *, const int4 *, int, const int *, const T *, int);
 template <typename T>
+__global__ void batch_tmz_step2 (const StepArgs<T> *, const int4 *, int, const int *, const T *, int);
+template <typename T>
 __global__ void batch_tmz_sources (const StepArgs<T> *, int, const int *, const T *, int);
-__global__ void batch_advance (int *, const int *, int);
+__global__ void batch_advance (int *, const int *, int, int);
 
 int batch_fail (int code, const std::string &msg);
 
@@ -48,8 +50,8 @@ struct BatchGroup
 {
   std::vector<int> sims;
   int *list = nullptr;                  // the same indices in device memory (batch_advance)
-  int4 *tiles[4] = {};                  // tile tables for tiles of kLx[0..3] x rows
-  int ntiles[4] = {};
+  int4 *tiles[2][4] = {};               // [two-step][length] tile tables (kLx[0..3] x rows)
+  int ntiles[2][4] = {};
   hipGraphExec_t graph = nullptr;
 };
 
@@ -70,7 +72,8 @@ struct yee_batch
   hipEvent_t ev[2] = { nullptr, nullptr };
   void *sims = nullptr;                 // StepArgs<T>[nsim][2] (by step parity)
   std::vector<std::vector<BatchGroup>> groupings;   // [grouping][group], one per cache budget
-  int *step = nullptr;                  // device step counters, one per simulation
+  int *state = nullptr;                 // device: [sim] steps done, buffer of the newest fields
+  bool two_ok = true;                   // two-step launches allowed (no mu grid in the batch)
   int *all = nullptr;                   // 0 .. nsim-1 in device memory
   void *src_tab = nullptr;              // [sim][src_len][2] source values
   int src_len = 0;
@@ -234,41 +237,55 @@ int upload_sources (yee_batch *b, int len)
   return YEE_OK;
 }
 
+// One launch of group g: one step (batch_tmz_step) or two (batch_tmz_step2), tiles of
+// length kLx[sel].
 template <typename T>
-void launch_step (yee_batch *b, const BatchGroup &g, int sel)
+void launch_step (yee_batch *b, const BatchGroup &g, int two, int sel)
 {
-  const int nt = g.ntiles[sel];
-  hipLaunchKernelGGL ((batch_tmz_step<T>), dim3 ((unsigned) std::max (1, (nt + 3) / 4)), dim3 (256), 0, b->stream,
-                     (const StepArgs<T> *) b->sims, g.tiles[sel], nt, b->step, (const T *) b->src_tab, b->src_len);
+  const int nt = g.ntiles[two][sel];
+  const dim3 grid ((unsigned) std::max (1, (nt + 3) / 4)), block (256);
+  if (two)
+    hipLaunchKernelGGL ((batch_tmz_step2<T>), grid, block, 0, b->stream, (const StepArgs<T> *) b->sims, g.tiles[1][sel], nt,
+                       b->state, (const T *) b->src_tab, b->src_len);
+  else
+    hipLaunchKernelGGL ((batch_tmz_step<T>), grid, block, 0, b->stream, (const StepArgs<T> *) b->sims, g.tiles[0][sel], nt,
+                       b->state, (const T *) b->src_tab, b->src_len);
 }
 
-void launch_advance (yee_batch *b, const int *list, int n)
+void launch_advance (yee_batch *b, const int *list, int n, int k)
 {
-  hipLaunchKernelGGL (batch_advance, dim3 ((unsigned) ((n + 63) / 64)), dim3 (64), 0, b->stream, b->step, list, n);
+  hipLaunchKernelGGL (batch_advance, dim3 ((unsigned) ((n + 63) / 64)), dim3 (64), 0, b->stream, b->state, list, n, k);
 }
 
-// Candidates of the tuner: (grouping, tile length index).
+// Candidates of the tuner: (grouping, steps per launch, tile length index).
 struct Cand
 {
-  int grouping, sel;
+  int grouping, two, sel;
 };
 // Cache budgets of the groupings (MiB of state per group; 0: the whole batch in one group).
 constexpr double kBudgetMiB[] = { 0, 160, 64 };
 constexpr int kNumGroupings = (int) (sizeof kBudgetMiB / sizeof kBudgetMiB[0]);
-const Cand kCands[] = { { 0, 0 }, { 0, 1 }, { 0, 2 }, { 0, 3 }, { 1, 0 }, { 1, 1 },
-                        { 1, 2 }, { 1, 3 }, { 2, 0 }, { 2, 1 }, { 2, 2 }, { 2, 3 } };
+const Cand kCands[] = { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 0, 2 }, { 0, 1, 0 }, { 0, 1, 1 }, { 0, 1, 2 },
+                        { 0, 1, 3 }, { 1, 1, 1 }, { 1, 1, 2 }, { 2, 1, 0 }, { 2, 1, 1 }, { 2, 1, 2 } };
 constexpr int kNumCands = (int) (sizeof kCands / sizeof kCands[0]);
-constexpr int kTuneSteps = 3;
+constexpr int kTuneSteps = 4;
 constexpr int kGraphSteps = 16;
 
-// One step t >= 1 of group g (fused E/H, counters) with candidate c's tiles.
+// `k` steps (1, or 2 with a two-step candidate) of group g with candidate c's tiles.
 template <typename T>
-int enqueue_step (yee_batch *b, const BatchGroup &g, const Cand &c)
+int enqueue_steps (yee_batch *b, const BatchGroup &g, const Cand &c, int k)
 {
-  launch_step<T> (b, g, c.sel);
-  launch_advance (b, g.list, (int) g.sims.size ());
+  const int two = (c.two && k == 2) ? 1 : 0;
+  launch_step<T> (b, g, two, c.sel);
+  launch_advance (b, g.list, (int) g.sims.size (), two ? 2 : 1);
   BHIP (hipGetLastError ());
   return YEE_OK;
+}
+
+// Steps per launch of a candidate.
+int cand_stride (const Cand &c)
+{
+  return c.two ? 2 : 1;
 }
 
 template <typename T>
@@ -282,7 +299,7 @@ int build_graph (yee_batch *b, BatchGroup &grp, const Cand &c)
   hipGraph_t g = nullptr;
   BHIP (hipStreamBeginCapture (b->stream, hipStreamCaptureModeThreadLocal));
   int rc = YEE_OK;
-  for (int s = 0; s < kGraphSteps && rc == YEE_OK; ++s) rc = enqueue_step<T> (b, grp, c);
+  for (int s = 0; s < kGraphSteps && rc == YEE_OK; s += cand_stride (c)) rc = enqueue_steps<T> (b, grp, c, cand_stride (c));
   hipError_t e = hipStreamEndCapture (b->stream, &g);
   if (rc) return rc;
   if (e != hipSuccess) return batch_fail (YEE_EDEVICE, std::string ("hipStreamEndCapture: ") + hipGetErrorString (e));
@@ -297,7 +314,7 @@ int build_graph (yee_batch *b, BatchGroup &grp, const Cand &c)
 template <typename T>
 int step_impl (yee_batch *b, int32_t n)
 {
-  int rc;
+  int rc = YEE_OK;
   if (b->t + n > b->src_len && (rc = upload_sources<T> (b, std::max (b->t + n, 2 * b->src_len)))) return rc;
   // device pointers inside StepArgs do not change, but the source table may have moved
   BHIP (hipEventRecord (b->ev[0], b->stream));
@@ -305,11 +322,20 @@ int step_impl (yee_batch *b, int32_t n)
   if (b->t == 0 && n > 0)
   {
     hipLaunchKernelGGL ((batch_tmz_sources<T>), dim3 ((unsigned) ((b->nsim + 63) / 64)), dim3 (64), 0, b->stream,
-                       (const StepArgs<T> *) b->sims, b->nsim, b->step, (const T *) b->src_tab, b->src_len);
-    launch_advance (b, b->all, b->nsim);
+                       (const StepArgs<T> *) b->sims, b->nsim, b->state, (const T *) b->src_tab, b->src_len);
+    launch_advance (b, b->all, b->nsim, 1);
     BHIP (hipGetLastError ());
     ++done;
     ++b->t;
+  }
+  // YEE_BATCH_CAND=<index> fixes the candidate (tests of every launch shape)
+  if (const char *fc = std::getenv ("YEE_BATCH_CAND"))
+  {
+    const int ci = std::atoi (fc);
+    if (ci < 0 || ci >= kNumCands || (kCands[ci].two && !b->two_ok)) return batch_fail (YEE_EINVAL, "YEE_BATCH_CAND out of range");
+    if (b->cand != ci) rc = drop_graphs (b);
+    if (rc) return rc;
+    b->cand = ci;
   }
   // tuner: kTuneSteps ordinary steps of every simulation per candidate, timed, on the first
   // call that has room
@@ -322,10 +348,16 @@ int step_impl (yee_batch *b, int32_t n)
     double best = 1e30;
     for (int c = 0; c < kNumCands; ++c)
     {
+      if (kCands[c].two && !b->two_ok)
+      {
+        b->tune_ms[c] = 0;
+        continue;
+      }
       BHIP (hipEventRecord (e0, b->stream));
+      const int stride = cand_stride (kCands[c]);
       for (const BatchGroup &g : b->groupings[kCands[c].grouping])
-        for (int s = 0; s < kTuneSteps; ++s)
-          if ((rc = enqueue_step<T> (b, g, kCands[c]))) return rc;
+        for (int s = 0; s < kTuneSteps; s += stride)
+          if ((rc = enqueue_steps<T> (b, g, kCands[c], stride))) return rc;
       BHIP (hipEventRecord (e1, b->stream));
       BHIP (hipEventSynchronize (e1));
       float ms = 0;
@@ -361,8 +393,9 @@ int step_impl (yee_batch *b, int32_t n)
       }
       else
       {
-        if ((rc = enqueue_step<T> (b, g, c))) return rc;
-        ++k;
+        const int kk = std::min (cand_stride (c), left - k);
+        if ((rc = enqueue_steps<T> (b, g, c, kk))) return rc;
+        k += kk;
       }
     }
   }
@@ -446,28 +479,32 @@ int prepare_impl (yee_batch *b)
       if ((rc = alloc_zero (b, &p, g.sims.size () * sizeof (int)))) return rc;
       BHIP (hipMemcpyAsync (p, g.sims.data (), g.sims.size () * sizeof (int), hipMemcpyHostToDevice, b->stream));
       g.list = (int *) p;
-      for (int sel = 0; sel < 4; ++sel)
-      {
-        std::vector<int4> tv;
-        // tiles (sim, xa, y0, xb): rows [xa, xb) of columns [y0, y0 + 63)
-        for (int s : g.sims)
+      for (int two = 0; two < 2; ++two)
+        for (int sel = 0; sel < 4; ++sel)
         {
-          const yee_config &c = b->cfg[s];
-          for (int64_t x0 = 1; x0 < c.size[0] - 1; x0 += kLx[sel])
-            for (int64_t y0 = 1; y0 < c.size[1] - 1; y0 += 63)
-              tv.push_back (make_int4 (s, (int) x0, (int) y0, (int) std::min<int64_t> (x0 + kLx[sel], c.size[0] - 1)));
+          // tiles (sim, xa, y0, xb): rows [xa, xb) of columns [y0, y0 + 63) (one step per
+          // launch) or [y0, y0 + 61) (two steps)
+          const int w = two ? 61 : 63;
+          std::vector<int4> tv;
+          for (int s : g.sims)
+          {
+            const yee_config &c = b->cfg[s];
+            for (int64_t x0 = 1; x0 < c.size[0] - 1; x0 += kLx[sel])
+              for (int64_t y0 = 1; y0 < c.size[1] - 1; y0 += w)
+                tv.push_back (make_int4 (s, (int) x0, (int) y0, (int) std::min<int64_t> (x0 + kLx[sel], c.size[0] - 1)));
+          }
+          g.ntiles[two][sel] = (int) tv.size ();
+          if ((rc = alloc_zero (b, &p, std::max<size_t> (1, tv.size ()) * sizeof (int4)))) return rc;
+          BHIP (hipMemcpyAsync (p, tv.data (), tv.size () * sizeof (int4), hipMemcpyHostToDevice, b->stream));
+          g.tiles[two][sel] = (int4 *) p;
         }
-        g.ntiles[sel] = (int) tv.size ();
-        if ((rc = alloc_zero (b, &p, std::max<size_t> (1, tv.size ()) * sizeof (int4)))) return rc;
-        BHIP (hipMemcpyAsync (p, tv.data (), tv.size () * sizeof (int4), hipMemcpyHostToDevice, b->stream));
-        g.tiles[sel] = (int4 *) p;
-      }
     }
   }
   {
     void *st = nullptr;
-    if ((rc = alloc_zero (b, &st, std::max<size_t> (256, (size_t) b->nsim * sizeof (int))))) return rc;
-    b->step = (int *) st;
+    if ((rc = alloc_zero (b, &st, std::max<size_t> (256, 2 * (size_t) b->nsim * sizeof (int))))) return rc;
+    b->state = (int *) st;
+    for (int s = 0; s < b->nsim; ++s) b->two_ok = b->two_ok && b->mat[2 * s + 1] == nullptr;
     std::vector<int> all ((size_t) b->nsim);
     for (int s = 0; s < b->nsim; ++s) all[s] = s;
     if ((rc = alloc_zero (b, &st, all.size () * sizeof (int)))) return rc;
@@ -653,7 +690,7 @@ int yee_batch_tuning (yee_batch *b, int32_t out[3], double *cand_ms, int32_t nca
     const Cand &c = kCands[b->cand];
     const std::vector<BatchGroup> &gs = b->groupings[c.grouping];
     out[0] = kLx[c.sel];
-    out[1] = gs.empty () ? 0 : (gs[0].ntiles[c.sel] + 3) / 4;
+    out[1] = gs.empty () ? 0 : (gs[0].ntiles[c.two][c.sel] + 3) / 4;
     out[2] = (int32_t) gs.size ();
   }
   else
@@ -672,23 +709,25 @@ int yee_batch_get_field (yee_batch *b, int32_t sim, int32_t comp, double *out)
   void *src;
   switch (comp)
   {
-    // the newest state is in buffer t & 1 (step t reads buffer t & 1 and writes the other)
-    case YEE_EZ: k = 0; src = b->f[b->t & 1][3 * sim]; break;
-    case YEE_HX: k = 1; src = b->f[b->t & 1][3 * sim + 1]; break;
-    case YEE_HY: k = 2; src = b->f[b->t & 1][3 * sim + 2]; break;
-    case YEE_DZ: k = 0; src = b->d[b->t & 1][3 * sim]; break;
-    case YEE_BX: k = 1; src = b->d[b->t & 1][3 * sim + 1]; break;
-    case YEE_BY: k = 2; src = b->d[b->t & 1][3 * sim + 2]; break;
+    case YEE_EZ: k = 0; break;
+    case YEE_HX: k = 1; break;
+    case YEE_HY: k = 2; break;
+    case YEE_DZ: k = 3; break;
+    case YEE_BX: k = 4; break;
+    case YEE_BY: k = 5; break;
     default: return batch_fail (YEE_EINVAL, "TMz components are Ez, Hx, Hy (and Dz, Bx, By with PML)");
   }
-  (void) k;
   const size_t n = (size_t) (b->cfg[sim].size[0] * b->cfg[sim].size[1]);
+  BHIP (hipStreamSynchronize (b->stream));
+  // the buffer holding the newest fields of this simulation (flipped by every launch)
+  int buf = 0;
+  if (b->state) BHIP (hipMemcpy (&buf, b->state + 2 * sim + 1, sizeof (int), hipMemcpyDeviceToHost));
+  src = k < 3 ? b->f[buf][3 * sim + k] : b->d[buf][3 * sim + k - 3];
   if (!src)
   {
     std::fill (out, out + n, 0.0);
     return YEE_OK;
   }
-  BHIP (hipStreamSynchronize (b->stream));
   if (b->wsize == 4)
   {
     std::vector<float> v (n);

@@ -64,6 +64,22 @@ __device__ __forceinline__ double lane_next (double v, double edge)
   return __builtin_bit_cast (double, r);
 }
 
+// The value of lane - 1 (lane 0: `edge`): DPP wave_shr:1.
+__device__ __forceinline__ float lane_prev (float v, float edge)
+{
+  return __builtin_bit_cast (float, __builtin_amdgcn_update_dpp (__builtin_bit_cast (int, edge),
+                                                                __builtin_bit_cast (int, v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double lane_prev (double v, double edge)
+{
+  const int2 vv = __builtin_bit_cast (int2, v), ee = __builtin_bit_cast (int2, edge);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp (ee.x, vv.x, 0x138, 0xf, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp (ee.y, vv.y, 0x138, 0xf, 0xf, false);
+  return __builtin_bit_cast (double, r);
+}
+
 // Direct-update coefficient cb = dt / (m m0 dx) of computeCaCb (InternalScheme.inc.h:2340-2344).
 // FP64 divides exactly (the reference's operations); FP32 multiplies by the hardware reciprocal
 // (1 ulp), several times cheaper than an IEEE division and far inside the FP32 tolerance.

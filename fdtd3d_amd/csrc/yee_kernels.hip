@@ -1053,7 +1053,7 @@ __device__ __forceinline__ void tmz_row_load (const StepArgs<T> &a, int x, int j
   r.ez = a.f[2][q];
   r.hx = a.f[3][q];
   r.hy = a.f[4][q];
-  r.hx_l = a.f[3][q - 1];
+  r.hx_l = a.f[3][q - (jc > 0 ? 1 : 0)];   // column 0 (a halo lane of the two-step tile) is never updated
   r.eps = a.eps ? a.eps[q] : a.eps_u;
   const int p[3] = { x, j, 0 };
   r.special = is_special (a, p, 2, MODE);
@@ -1167,18 +1167,201 @@ __device__ __forceinline__ void tmz_fused_tile (const StepArgs<T> &a, int xa, in
   }
 }
 
+// One TMz component update for the two-step tile: cell_value_pc's arithmetic (direct, or the
+// PML two-stage update of MODE 1), returning the new D/B in d_new (d_old for direct cells).
+template <typename T, int MODE, int COMP>
+__device__ __forceinline__ T tmz_update (const StepArgs<T> &a, T f_old, T d_old, T tmp, T mat, bool special,
+                                         const PmlCoef<T> &pc, T &d_new)
+{
+  const T modc = COMP >= 3 ? a.mu0 : a.eps0;
+  if (MODE == 0 || !special)
+  {
+    d_new = d_old;
+    const T cb = direct_cb (a.dt, mat, modc, a.dx);
+    return f_old * (T) 1.0 + tmp * cb;
+  }
+  d_new = d_old * pc.cac + tmp * pc.cbc;
+  const T mod = mat * modc;
+  return f_old * pc.ca5 + d_new * ((pc.p4 / mod) / pc.dd5) - d_old * ((pc.m4 / mod) / pc.dd5);
+}
+
+// Row-dependent H coefficients and PML flag of a row, kept for the second step's H update.
 template <typename T>
-__global__ __launch_bounds__ (256) void batch_tmz_step (const StepArgs<T> *sims, const int4 *tiles, int ntiles,
-                                                       const int *step, const T *src_tab, int src_len)
+struct TmzRowH
+{
+  T hx_p4, hx_m4, hy_ca5, hy_dd5;
+  bool special;
+};
+
+// Two time steps (t+1 and t+2) of a TMz tile in one sweep.  Lane l holds column y0 - 1 + l:
+// lanes 1..61 own their column; lane 0 (column y0 - 1) recomputes E and H of step t+1 for the
+// E of step t+2 at column y0; lanes 62 and 63 recompute what H of step t+2 at column y0 + 60
+// needs.  Along x the stages lag by one row: at row x, E(t+1) of row x, H(t+1) of x-1, E(t+2)
+// of x-1 and H(t+2) of x-2; the sweep starts one row before the owned rows [xa, xb) and ends
+// one row after, so that the owned rows of step t+2 see their neighbours of step t+1.  Only
+// step t+2 is stored; step t+1 lives in registers.  Uniform mu only (no mu grid).
+template <typename T, int MODE>
+__device__ __forceinline__ void tmz_fused2_tile (const StepArgs<T> &a, int xa, int xb, int y0, int lane, T se0, T sh0,
+                                                 T se1, T sh1)
+{
+  const int nx = a.nx, ny = a.ny;
+  const int64_t sx = a.sx;   // == ny
+  const int j = y0 - 1 + lane;
+  const int jc = min (j, ny - 1);
+  const bool jin = j >= 1 && j < ny - 1;
+  const bool own = jin && lane >= 1 && lane <= 61;
+  const bool src_col = j == a.src_pos[1];
+  const int sxr = a.src_pos[0];
+  const T mu = a.mu_u;
+  T *Ezn = a.f_n[2], *Hxn = a.f_n[3], *Hyn = a.f_n[4];
+  TmzColCoef<T> cc {};
+  if (MODE != 0)
+  {
+    cc.ez_ca5 = pml_slot (a, 2, 4, jc);
+    cc.ez_dd5 = pml_slot (a, 2, 5, jc);
+    cc.hx_cac = pml_slot (a, 3, 0, jc);
+    cc.hx_cbc = pml_slot (a, 3, 1, jc);
+    cc.hy_p4 = pml_slot (a, 4, 2, jc);
+    cc.hy_m4 = pml_slot (a, 4, 3, jc);
+    cc.ez_p4 = pml_slot (a, 2, 2, 0);
+    cc.ez_m4 = pml_slot (a, 2, 3, 0);
+    cc.hx_ca5 = pml_slot (a, 3, 4, 0);
+    cc.hx_dd5 = pml_slot (a, 3, 5, 0);
+    cc.hy_cac = pml_slot (a, 4, 0, 0);
+    cc.hy_cbc = pml_slot (a, 4, 1, 0);
+  }
+  // state t: rows x-1 (p), x (cur), x+1 and x+2 (loaded ahead)
+  TmzRow<T> p, cur, nx1, nx2;
+  p.hy = xa >= 2 ? a.f[4][(int64_t) (xa - 2) * sx + jc] : (T) 0;
+  p.hx = p.bx = p.by = p.ez = p.dz = p.eps = (T) 0;
+  p.special = false;
+  tmz_row_load<T, MODE> (a, xa - 1, j, jc, cur);
+  tmz_row_load<T, MODE> (a, xa, j, jc, nx1);
+  // step t+1 at row x-1 (E, D) and x-2 (H, B); step t+2 E at row x-2; coefficients of row x-2
+  T e1_p = (T) 0, dz1_p = (T) 0;
+  T h1x_pp = (T) 0, h1y_pp = (T) 0, b1x_pp = (T) 0, b1y_pp = (T) 0, e2_pp = (T) 0;
+  TmzRowH<T> rh_pp {};
+  for (int x = xa - 1; x <= xb + 1; ++x)
+  {
+    if (x + 2 <= xb + 1) tmz_row_load<T, MODE> (a, min (x + 2, nx - 1), j, jc, nx2);   // prefetch
+    const bool xin = x >= 1 && x < nx - 1;
+    // S1: E(t+1) at row x
+    T e1 = cur.ez, dz1 = cur.dz;
+    if (jin && xin)
+    {
+      T v[4] = { p.hy, cur.hy, cur.hx_l, cur.hx };
+      const PmlCoef<T> pc = { cur.ez_cac, cur.ez_cbc, cc.ez_p4, cc.ez_m4, cc.ez_ca5, cc.ez_dd5 };
+      e1 = tmz_update<T, MODE, 2> (a, cur.ez, cur.dz, curl (v), avg2 (cur.eps, cur.eps), cur.special, pc, dz1);
+      if ((a.src_mask & 4u) && src_col && x == sxr) e1 = se0;
+    }
+    // S2: H(t+1) at row x-1
+    const int i1 = x - 1;
+    const bool i1in = i1 >= 1 && i1 < nx - 1;
+    const T e1r = lane_next (e1_p, e1_p);
+    T h1x = p.hx, h1y = p.hy, b1x = p.bx, b1y = p.by;
+    if (x >= xa && jin && i1in)
+    {
+      T vx[4] = { (T) 0, (T) 0, e1_p, e1r };
+      T vy[4] = { e1_p, e1, (T) 0, (T) 0 };
+      const PmlCoef<T> px = { cc.hx_cac, cc.hx_cbc, p.hx_p4, p.hx_m4, cc.hx_ca5, cc.hx_dd5 };
+      const PmlCoef<T> py = { cc.hy_cac, cc.hy_cbc, cc.hy_p4, cc.hy_m4, p.hy_ca5, p.hy_dd5 };
+      h1x = tmz_update<T, MODE, 3> (a, p.hx, p.bx, curl (vx), mu, p.special, px, b1x);
+      h1y = tmz_update<T, MODE, 4> (a, p.hy, p.by, curl (vy), mu, p.special, py, b1y);
+      if (src_col && i1 == sxr)
+      {
+        if (a.src_mask & 8u) h1x = sh0;
+        if (a.src_mask & 16u) h1y = sh0;
+      }
+    }
+    // S3: E(t+2) at row x-1
+    const T h1xl = lane_prev (h1x, h1x);
+    T e2 = e1_p;
+    if (x >= xa + 1 && jin && i1in)
+    {
+      T v[4] = { h1y_pp, h1y, h1xl, h1x };
+      const PmlCoef<T> pc = { p.ez_cac, p.ez_cbc, cc.ez_p4, cc.ez_m4, cc.ez_ca5, cc.ez_dd5 };
+      T dz2;
+      e2 = tmz_update<T, MODE, 2> (a, e1_p, dz1_p, curl (v), avg2 (p.eps, p.eps), p.special, pc, dz2);
+      if ((a.src_mask & 4u) && src_col && i1 == sxr) e2 = se1;
+      if (own && i1 < xb)
+      {
+        const int64_t q = (int64_t) i1 * sx + jc;
+        Ezn[q] = e2;
+        if (MODE != 0 && p.special) a.d_n[2][q] = dz2;
+      }
+    }
+    // S4: H(t+2) at row x-2
+    const int i2 = x - 2;
+    const T e2r = lane_next (e2_pp, e2_pp);
+    if (x >= xa + 2 && own && i2 >= 1 && i2 < nx - 1)
+    {
+      T vx[4] = { (T) 0, (T) 0, e2_pp, e2r };
+      T vy[4] = { e2_pp, e2, (T) 0, (T) 0 };
+      const PmlCoef<T> px = { cc.hx_cac, cc.hx_cbc, rh_pp.hx_p4, rh_pp.hx_m4, cc.hx_ca5, cc.hx_dd5 };
+      const PmlCoef<T> py = { cc.hy_cac, cc.hy_cbc, cc.hy_p4, cc.hy_m4, rh_pp.hy_ca5, rh_pp.hy_dd5 };
+      T b2x, b2y;
+      T h2x = tmz_update<T, MODE, 3> (a, h1x_pp, b1x_pp, curl (vx), mu, rh_pp.special, px, b2x);
+      T h2y = tmz_update<T, MODE, 4> (a, h1y_pp, b1y_pp, curl (vy), mu, rh_pp.special, py, b2y);
+      if (src_col && i2 == sxr)
+      {
+        if (a.src_mask & 8u) h2x = sh1;
+        if (a.src_mask & 16u) h2y = sh1;
+      }
+      const int64_t q = (int64_t) i2 * sx + jc;
+      Hxn[q] = h2x;
+      Hyn[q] = h2y;
+      if (MODE != 0 && rh_pp.special)
+      {
+        a.d_n[3][q] = b2x;
+        a.d_n[4][q] = b2y;
+      }
+    }
+    // rotate
+    h1x_pp = h1x;
+    h1y_pp = h1y;
+    b1x_pp = b1x;
+    b1y_pp = b1y;
+    e2_pp = e2;
+    rh_pp.hx_p4 = p.hx_p4;
+    rh_pp.hx_m4 = p.hx_m4;
+    rh_pp.hy_ca5 = p.hy_ca5;
+    rh_pp.hy_dd5 = p.hy_dd5;
+    rh_pp.special = p.special;
+    p = cur;
+    e1_p = e1;
+    dz1_p = dz1;
+    cur = nx1;
+    nx1 = nx2;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__ (256) void batch_tmz_step2 (const StepArgs<T> *sims, const int4 *tiles, int ntiles,
+                                                        const int *state, const T *src_tab, int src_len)
 {
   const int lane = (int) (threadIdx.x & 63);
   const int tile = (int) blockIdx.x * 4 + __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
   if (tile >= ntiles) return;   // no barriers below
   const int4 td = tiles[tile];   // sim, xa, y0, xb
-  const int t = step[td.x];
+  const int t = state[2 * td.x];
+  const StepArgs<T> a = sims[2 * td.x + state[2 * td.x + 1]];
+  const T *st = src_tab + ((int64_t) td.x * src_len + t) * 2;
+  if (a.pml) tmz_fused2_tile<T, 1> (a, td.y, td.w, td.z, lane, st[0], st[1], st[2], st[3]);
+  else tmz_fused2_tile<T, 0> (a, td.y, td.w, td.z, lane, st[0], st[1], st[2], st[3]);
+}
+
+template <typename T>
+__global__ __launch_bounds__ (256) void batch_tmz_step (const StepArgs<T> *sims, const int4 *tiles, int ntiles,
+                                                       const int *state, const T *src_tab, int src_len)
+{
+  const int lane = (int) (threadIdx.x & 63);
+  const int tile = (int) blockIdx.x * 4 + __builtin_amdgcn_readfirstlane ((int) (threadIdx.x >> 6));
+  if (tile >= ntiles) return;   // no barriers below
+  const int4 td = tiles[tile];   // sim, xa, y0, xb
+  const int t = state[2 * td.x];
   // a private copy: loads through a reference could not be kept across the tile's stores
   // (the compiler cannot rule out that they alias the record)
-  const StepArgs<T> a = sims[2 * td.x + (t & 1)];
+  const StepArgs<T> a = sims[2 * td.x + state[2 * td.x + 1]];
   const T se = src_tab[((int64_t) td.x * src_len + t) * 2 + 0];
   const T sh = src_tab[((int64_t) td.x * src_len + t) * 2 + 1];
   if (a.pml) tmz_fused_tile<T, 1> (a, td.y, td.w, td.z + lane, lane, se, sh);
@@ -1187,12 +1370,12 @@ __global__ __launch_bounds__ (256) void batch_tmz_step (const StepArgs<T> *sims,
 
 // Step 0 performs no field update (Scheme.inc.h:1032) but the hard sources still fire.
 template <typename T>
-__global__ void batch_tmz_sources (const StepArgs<T> *sims, int nsim, const int *step, const T *src_tab, int src_len)
+__global__ void batch_tmz_sources (const StepArgs<T> *sims, int nsim, const int *state, const T *src_tab, int src_len)
 {
   const int s = (int) (blockIdx.x * blockDim.x + threadIdx.x);
   if (s >= nsim) return;
-  const int t = step[s];
-  const StepArgs<T> &a = sims[2 * s + (t & 1)];
+  const int t = state[2 * s];
+  const StepArgs<T> &a = sims[2 * s + state[2 * s + 1]];
   if (!a.src_mask) return;
   const T ve = src_tab[((int64_t) s * src_len + t) * 2 + 0];
   const T vh = src_tab[((int64_t) s * src_len + t) * 2 + 1];
@@ -1201,11 +1384,17 @@ __global__ void batch_tmz_sources (const StepArgs<T> *sims, int nsim, const int 
   if (a.src_mask & 16u) gst (a.f_n[4] + a.src_q, vh);
 }
 
-// Advances the step counters of simulations sims[0..n) (last node of every captured step).
-__global__ void batch_advance (int *step, const int *sims, int n)
+// After a launch that advanced simulations sims[0..n) by k steps (the last node of every
+// captured step): their step counters move on by k and their newest fields are now in the
+// other buffer.
+__global__ void batch_advance (int *state, const int *sims, int n, int k)
 {
   const int u = (int) (blockIdx.x * blockDim.x + threadIdx.x);
-  if (u < n) step[sims[u]] += 1;
+  if (u < n)
+  {
+    state[2 * sims[u]] += k;
+    state[2 * sims[u] + 1] ^= 1;
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1296,6 +1485,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void h_half_tmz<T, 1> (StepArgs<T>);                                 \
   template __global__ void h_half_tmz<T, 2> (StepArgs<T>);                                 \
   template __global__ void batch_tmz_step<T> (const StepArgs<T> *, const int4 *, int, const int *, const T *, int); \
+  template __global__ void batch_tmz_step2<T> (const StepArgs<T> *, const int4 *, int, const int *, const T *, int); \
   template __global__ void batch_tmz_sources<T> (const StepArgs<T> *, int, const int *, const T *, int); \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
   template __global__ void step_3d_fused<T, FUSED_BY (T), 0> (StepArgs<T>);                \

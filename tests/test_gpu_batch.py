@@ -127,3 +127,54 @@ def test_batch_steps_in_several_calls(engine_lib):
             assert np.array_equal(a.field(s, f), b.field(s, f)), f"sim {s} {f}"
     a.close()
     b.close()
+
+
+# sims whose tiles split into several x chunks and column strips, with sources on the halo
+# rows and columns of the two-step tiles (chunks of 13 rows start at x = 1, 14, 27, ...;
+# strips of 61 columns at y = 1, 62, 123, ...)
+SIMS_EDGE = [
+    ("--2d-tmz --size x:60,y:150 --use-pml --pml-size x:6,y:8 --point-source x:14,y:61 --point-source-ez", 20.0, 9.0),
+    ("--2d-tmz --size x:45,y:130 --use-pml --pml-size x:5,y:5 --point-source x:13,y:62 --point-source-ez "
+     "--point-source-hx --point-source-hy", 15.0, 6.0),
+    ("--2d-tmz --size x:31,y:130 --point-source x:27,y:122 --point-source-ez", 10.0, 5.0),
+    ("--2d-tmz --size x:29,y:140 --use-pml --pml-size x:4,y:4 --point-source x:26,y:123 --point-source-hy", 12.0, 4.0),
+]
+
+# the two-step launch shapes of the tuner (index into its candidate table: grouping,
+# two steps, tile length) and one single-step shape
+CANDS_TWO = [3, 4, 6, 9]
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("cand", CANDS_TWO + [0])
+def test_batch_launch_shapes_match_single_runs(precision, cand, engine_lib, monkeypatch):
+    steps = 24   # one source step, a graph of 16 steps, three two-step launches, one single step
+    cfgs = []
+    for args, _, _ in SIMS_EDGE:
+        c = parse_args((args + f" --time-steps {steps}").split())
+        c.precision = precision
+        cfgs.append(c)
+    say(f"[gpu] batch launch shape {cand} precision {precision}")
+    monkeypatch.setenv("YEE_BATCH_CAND", str(cand))
+    b = Batch(cfgs)
+    for s, (c, (_, t0, sig)) in enumerate(zip(cfgs, SIMS_EDGE)):
+        b.set_material(s, "eps", blobs(c.shape, 200 + s))
+        b.set_point_source_gaussian(s, t0, sig)
+    b.prepare()
+    b.step(steps)
+    b.sync()
+    monkeypatch.delenv("YEE_BATCH_CAND")
+    for s, (c, (_, t0, sig)) in enumerate(zip(cfgs, SIMS_EDGE)):
+        e = Engine(c)
+        e.set_material("eps", blobs(c.shape, 200 + s))
+        e.set_point_source_gaussian(t0, sig)
+        e.prepare()
+        e.step(steps)
+        e.sync()
+        for f in ("Ez", "Hx", "Hy"):
+            assert np.array_equal(b.field(s, f), e.field(f)), f"sim {s} {f}"
+        if c.use_pml:
+            for f in ("Dz", "Bx", "By"):
+                assert np.array_equal(b.field(s, f), e.field(f)), f"sim {s} {f}"
+        e.close()
+    b.close()
