@@ -42,8 +42,33 @@ struct AdjArgs
   int64_t src_q;
 };
 
+// The adjoint D/B and PML coefficients of a special cell, loaded before any store of the
+// thread (the state pointers may alias as far as the compiler knows), so that every load of a
+// cell is in flight at once.
+template <typename T>
+struct AdjOld
+{
+  T ad, cac, cbc, p4, m4, ca5, dd5;
+};
+
 template <typename T, int COMP>
-__device__ __forceinline__ T adj_cell (const AdjArgs<T> &g, int64_t q, const int p[3], T av, T mat, bool special)
+__device__ __forceinline__ void adj_load (const AdjArgs<T> &g, int64_t q, const int p[3], bool special, AdjOld<T> &o)
+{
+  if (!special) return;
+  const StepArgs<T> &a = g.a;
+  const int L = a.tab_len;
+  const T *tab = a.pml + (size_t) COMP * 6 * L;
+  o.ad = g.ad[COMP][q];
+  o.cac = tab[0 * L + p[SigAxes<COMP>::c]];
+  o.cbc = tab[1 * L + p[SigAxes<COMP>::c]];
+  o.p4 = tab[2 * L + p[SigAxes<COMP>::s4]];
+  o.m4 = tab[3 * L + p[SigAxes<COMP>::s4]];
+  o.ca5 = tab[4 * L + p[SigAxes<COMP>::s5]];
+  o.dd5 = tab[5 * L + p[SigAxes<COMP>::s5]];
+}
+
+template <typename T, int COMP>
+__device__ __forceinline__ T adj_cell (const AdjArgs<T> &g, int64_t q, T av, T mat, bool special, const AdjOld<T> &o)
 {
   const StepArgs<T> &a = g.a;
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
@@ -57,35 +82,25 @@ __device__ __forceinline__ T adj_cell (const AdjArgs<T> &g, int64_t q, const int
   }
   else
   {
-    const int L = a.tab_len;
-    const T *tab = a.pml + (size_t) COMP * 6 * L;
-    const T cac = tab[0 * L + p[SigAxes<COMP>::c]];
-    const T cbc = tab[1 * L + p[SigAxes<COMP>::c]];
-    const T p4 = tab[2 * L + p[SigAxes<COMP>::s4]];
-    const T m4 = tab[3 * L + p[SigAxes<COMP>::s4]];
-    const T ca5 = tab[4 * L + p[SigAxes<COMP>::s5]];
-    const T dd5 = tab[5 * L + p[SigAxes<COMP>::s5]];
     const T mod = mat * modc;
-    const T kp = (p4 / mod) / dd5, km = (m4 / mod) / dd5;
-    const T mD = g.ad[COMP][q] + kp * av;
-    w = cbc * mD;
-    g.ad[COMP][q] = cac * mD - km * av;
-    g.af[COMP][q] = ca5 * av;
+    const T kp = (o.p4 / mod) / o.dd5, km = (o.m4 / mod) / o.dd5;
+    const T mD = o.ad + kp * av;
+    w = o.cbc * mD;
+    g.ad[COMP][q] = o.cac * mD - km * av;
+    g.af[COMP][q] = o.ca5 * av;
   }
   return w;
 }
 
 // aH -= curl_H(w_E) of the previous reverse step (the transpose of the E curl that read H;
 // w_E is zero before the first one), then the transpose of this step's H update (H cells of
-// [1, N-1)^3).
+// [1, N-1)^3; tiles as half_cell).
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void adj_h (AdjArgs<T> g)
 {
   const StepArgs<T> &a = g.a;
-  const int k = 1 + (int) (blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int) (blockIdx.y * blockDim.y + threadIdx.y);
-  const int i = 1 + (int) blockIdx.z;
-  if (j >= a.ny - 1 || k >= a.nz - 1) return;
+  int i, j, k;
+  if (!half_cell (a, i, j, k)) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
@@ -104,10 +119,14 @@ __global__ __launch_bounds__ (256) void adj_h (AdjArgs<T> g)
   const T vx[4] = { ey[q], ey[q + 1], ez[q], ez[q + sy] };
   const T vy[4] = { ez[q], ez[q + sx], ex[q], ex[q + 1] };
   const T vz[4] = { ex[q], ex[q + sy], ey[q], ey[q + sx] };
+  AdjOld<T> o3 {}, o4 {}, o5 {};
+  adj_load<T, 3> (g, q, p, special, o3);
+  adj_load<T, 4> (g, q, p, special, o4);
+  adj_load<T, 5> (g, q, p, special, o5);
   const T a3 = g.af[3][q] - curl (vx), a4 = g.af[4][q] - curl (vy), a5 = g.af[5][q] - curl (vz);
-  g.w[3][q] = adj_cell<T, 3> (g, q, p, a3, mx, special);
-  g.w[4][q] = adj_cell<T, 4> (g, q, p, a4, my, special);
-  g.w[5][q] = adj_cell<T, 5> (g, q, p, a5, mz, special);
+  g.w[3][q] = adj_cell<T, 3> (g, q, a3, mx, special, o3);
+  g.w[4][q] = adj_cell<T, 4> (g, q, a4, my, special, o4);
+  g.w[5][q] = adj_cell<T, 5> (g, q, a5, mz, special, o5);
 }
 
 // aE -= curl_E(w_H), then the transpose of the E update (E cells of [1, N-1)^3).
@@ -115,10 +134,8 @@ template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void adj_e (AdjArgs<T> g)
 {
   const StepArgs<T> &a = g.a;
-  const int k = 1 + (int) (blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int) (blockIdx.y * blockDim.y + threadIdx.y);
-  const int i = 1 + (int) blockIdx.z;
-  if (j >= a.ny - 1 || k >= a.nz - 1) return;
+  int i, j, k;
+  if (!half_cell (a, i, j, k)) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
@@ -137,10 +154,14 @@ __global__ __launch_bounds__ (256) void adj_e (AdjArgs<T> g)
     ey = avg2 (e, a.eps[q + sy]);
     ez = avg2 (e, a.eps[q + 1]);
   }
+  AdjOld<T> o0 {}, o1 {}, o2 {};
+  adj_load<T, 0> (g, q, p, special, o0);
+  adj_load<T, 1> (g, q, p, special, o1);
+  adj_load<T, 2> (g, q, p, special, o2);
   const T a0 = g.af[0][q] - curl (vx), a1 = g.af[1][q] - curl (vy), a2 = g.af[2][q] - curl (vz);
-  g.w[0][q] = adj_cell<T, 0> (g, q, p, a0, ex, special);
-  g.w[1][q] = adj_cell<T, 1> (g, q, p, a1, ey, special);
-  g.w[2][q] = adj_cell<T, 2> (g, q, p, a2, ez, special);
+  g.w[0][q] = adj_cell<T, 0> (g, q, a0, ex, special, o0);
+  g.w[1][q] = adj_cell<T, 1> (g, q, a1, ey, special, o1);
+  g.w[2][q] = adj_cell<T, 2> (g, q, a2, ez, special, o2);
 }
 
 // aEz += 2 Ez on the monitor rectangle (mon: that step's recorded Ez, [x][y]).
@@ -287,10 +308,6 @@ AdjArgs<T> adj_args (const yee_handle *h)
   return g;
 }
 
-dim3 interior_grid (const yee_handle *h)
-{
-  return dim3 ((unsigned) ((h->n[2] - 2 + 63) / 64), (unsigned) ((h->n[1] - 2 + 3) / 4), (unsigned) (h->n[0] - 2));
-}
 
 template <typename T>
 int record (yee_handle *h, int slot)
@@ -358,7 +375,15 @@ int reverse_step (yee_handle *h, int32_t t, int slot_cur, int slot_prev)
 {
   AdjointState *s = h->adj;
   AdjArgs<T> g = adj_args<T> (h);
-  const dim3 grid = interior_grid (h), block (64, 4, 1);
+  // the interior [1, N-1)^3 in 64 x 4 tiles (half_cell)
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    g.a.run_lo[ax] = 1;
+    g.a.run_hi[ax] = (int) h->n[ax] - 1;
+  }
+  dim3 grid;
+  const dim3 block (64, 4, 1);
+  if (!half_tiles (g.a, 64, 4, grid)) return fail (YEE_EINVAL, "adjoint grid too large for one launch");
   const int W = s->sp.window;
   if (t >= s->T - W)
   {

@@ -80,6 +80,26 @@ __device__ __forceinline__ double lane_prev (double v, double edge)
   return __builtin_bit_cast (double, r);
 }
 
+// Cell of a half-step thread.  The tiles (blockDim.x z cells, from the aligned origin tz0, by
+// blockDim.y rows, one x plane) are numbered plane-fastest, and the workgroups, which the
+// hardware hands to the 8 XCDs round-robin, are mapped so that each XCD takes one contiguous
+// band of tile numbers: the next plane of a tile then runs on the same XCD right after it,
+// and its x-neighbour plane (and the y-neighbour rows of the next tile row) come from that
+// XCD's L2 instead of HBM.  Returns false for threads outside the run box.
+template <typename T>
+__device__ __forceinline__ bool half_cell (const StepArgs<T> &a, int &i, int &j, int &k)
+{
+  const int b = (int) blockIdx.x;
+  const int u = (b & 7) * a.per_xcd + (b >> 3);
+  if (u >= a.ntile) return false;
+  const int t = u / a.ntx;
+  i = a.run_lo[0] + (u - t * a.ntx);
+  const int ty = t / a.ntz;
+  k = a.tz0 + (t - ty * a.ntz) * (int) blockDim.x + (int) threadIdx.x;
+  j = a.run_lo[1] + ty * (int) blockDim.y + (int) threadIdx.y;
+  return j < a.run_hi[1] && k >= a.run_lo[2] && k < a.run_hi[2];
+}
+
 // Direct-update coefficient cb = dt / (m m0 dx) of computeCaCb (InternalScheme.inc.h:2340-2344).
 // FP64 divides exactly (the reference's operations); FP32 multiplies by the hardware reciprocal
 // (1 ulp), several times cheaper than an IEEE division and far inside the FP32 tolerance.

@@ -41,6 +41,25 @@ int fail (int code, const std::string &msg)
 }
 
 
+
+// Tile grid of a half-step launch over the run box of `a` (half_cell): bx z cells from a
+// multiple of bx, by rows, one plane; one workgroup per tile on a 1-D grid.
+template <typename T>
+bool half_tiles (StepArgs<T> &a, unsigned bx, unsigned by, dim3 &grid)
+{
+  a.tz0 = a.run_lo[2] & ~(int) (bx - 1);
+  a.ntz = (int) ((a.run_hi[2] - a.tz0 + (int) bx - 1) / (int) bx);
+  a.nty = (int) ((a.run_hi[1] - a.run_lo[1] + (int) by - 1) / (int) by);
+  a.ntx = a.run_hi[0] - a.run_lo[0];
+  const int64_t ntile = (int64_t) a.ntz * a.nty * a.ntx;
+  if (ntile <= 0 || ntile > (int64_t) 1 << 30) return false;
+  a.ntile = (int) ntile;
+  a.per_xcd = (int) ((ntile + 7) / 8);
+  grid = dim3 ((unsigned) (8 * a.per_xcd), 1, 1);
+  return true;
+}
+template bool half_tiles<float> (StepArgs<float> &, unsigned, unsigned, dim3 &);
+template bool half_tiles<double> (StepArgs<double> &, unsigned, unsigned, dim3 &);
 }  // namespace yee
 
 using namespace yee;
@@ -123,6 +142,7 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   return a;
 }
 
+
 template <typename T>
 int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nullptr, const int64_t *hi = nullptr)
 {
@@ -142,16 +162,8 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
     // of their width (whole cache lines per row).
     const unsigned bx = ext[2] <= 32 ? 32 : 64;
     const unsigned by = (unsigned) std::min<int64_t> (256 / bx, std::max<int64_t> (1, ext[1]));
-    a.tz0 = a.run_lo[2] & ~(int) (bx - 1);
-    a.ntz = (int) ((a.run_hi[2] - a.tz0 + bx - 1) / bx);
-    a.nty = (int) ((ext[1] + by - 1) / by);
-    a.ntx = (int) ext[0];
-    const int64_t ntile = (int64_t) a.ntz * a.nty * a.ntx;
-    if (ntile > (int64_t) 1 << 30) return fail (YEE_EINVAL, "half-step box too large for one launch");
-    a.ntile = (int) ntile;
-    a.per_xcd = (int) ((ntile + 7) / 8);
-    dim3 block (bx, by, 1);
-    dim3 grid ((unsigned) (8 * a.per_xcd), 1, 1);
+    dim3 block (bx, by, 1), grid;
+    if (!half_tiles (a, bx, by, grid)) return fail (YEE_EINVAL, "half-step box too large for one launch");
     switch (h->mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;

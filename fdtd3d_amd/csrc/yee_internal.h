@@ -91,5 +91,9 @@ template <typename T>
 StepArgs<T> engine_args (const yee_handle *h, int parity);
 // Frees the adjoint state (called by yee_destroy).
 void adjoint_free (yee_handle *h);
+// Tile grid of a half-step style launch over a.run_lo .. a.run_hi (fills the tile fields of a;
+// false when the box is empty or too large for one launch).
+template <typename T>
+bool half_tiles (StepArgs<T> &a, unsigned bx, unsigned by, dim3 &grid);
 
 }  // namespace yee
