@@ -277,6 +277,7 @@ struct AdjointState
   std::vector<int> snap_cur;
   size_t snap_bytes = 0;
   bool forward_done = false;
+  bool last_traced = false;   // the forward recorded the trace of the last checkpoint segment
   double J = 0;
   double fwd_ms = 0, bwd_ms = 0;
 };
@@ -436,6 +437,9 @@ int forward_impl (yee_handle *h, const double *design, double *J)
   if ((rc = engine_reset (h))) return rc;
   const int K = s->sp.checkpoint_every;
   const int W = s->sp.window;
+  // with checkpoints, the last segment's trace is recorded here (the backward starts with it
+  // and need not re-run it)
+  const int32_t t0_last = K > 0 ? ((s->T - 1 + K - 1) / K - 1) * K : 0;
   HIP_TRY (hipEventRecord (h->ev[0], h->stream));
   for (int32_t t = 0; t < s->T; ++t)
   {
@@ -444,9 +448,10 @@ int forward_impl (yee_handle *h, const double *design, double *J)
     {
       if ((rc = record<T> (h, t))) return rc;
     }
-    else if (t % K == 0)
+    else
     {
-      if ((rc = snapshot (h, t / K, false))) return rc;
+      if (t % K == 0 && (rc = snapshot (h, t / K, false))) return rc;
+      if (t >= t0_last && (rc = record<T> (h, t - t0_last))) return rc;
     }
     if (t >= s->T - W && (rc = monitor<T> (h, t - (s->T - W)))) return rc;
   }
@@ -463,6 +468,7 @@ int forward_impl (yee_handle *h, const double *design, double *J)
   s->J = acc;
   *J = acc;
   s->forward_done = true;
+  s->last_traced = K > 0;
   return YEE_OK;
 }
 
@@ -492,17 +498,21 @@ int backward_impl (yee_handle *h, double dJ, double *grad)
     const int nseg = (s->T - 1 + K - 1) / K;   // segments of steps (kK, min((k+1)K, T-1)]
     for (int k = nseg - 1; k >= 0; --k)
     {
-      if ((rc = snapshot (h, k, true))) return rc;
-      if ((rc = record<T> (h, 0))) return rc;   // state after step kK
       const int32_t t0 = k * K, t1 = std::min ((k + 1) * K, s->T - 1);
-      for (int32_t t = t0 + 1; t <= t1; ++t)
+      if (!(k == nseg - 1 && s->last_traced))
       {
-        if ((rc = engine_step_once (h))) return rc;
-        if ((rc = record<T> (h, t - t0))) return rc;
+        if ((rc = snapshot (h, k, true))) return rc;
+        if ((rc = record<T> (h, 0))) return rc;   // state after step kK
+        for (int32_t t = t0 + 1; t <= t1; ++t)
+        {
+          if ((rc = engine_step_once (h))) return rc;
+          if ((rc = record<T> (h, t - t0))) return rc;
+        }
       }
       for (int32_t t = t1; t >= t0 + 1; --t)
         if ((rc = reverse_step<T> (h, t, t - t0, t - t0 - 1))) return rc;
     }
+    s->last_traced = false;   // the trace now holds segment 0
   }
   HIP_TRY (hipEventRecord (h->ev[1], h->stream));
   HIP_TRY (hipEventSynchronize (h->ev[1]));

@@ -755,8 +755,20 @@ int yee_create (const yee_config *cfg, yee_handle **out)
     if (hipGetDevice (&dev) == hipSuccess) (void) hipDeviceGetAttribute (&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int64_t rows = h->fbox_hi[1] - h->fbox_lo[1], planes = h->fbox_hi[0] - h->fbox_lo[0];
     h->fused_nstrips = (int) ((rows + by - 1) / by);
-    // one 1024-thread workgroup per CU at a time: chunk x so that there are >= 2 per CU
-    int64_t nch = std::max<int64_t> (1, (2 * (int64_t) ncu + h->fused_nstrips - 1) / h->fused_nstrips);
+    // x chunks: a CU holds 16 waves of this kernel (<= 128 VGPRs each), i.e. 16 / (waves per
+    // row) workgroups.  The chunk count minimises rounds of resident workgroups x planes per
+    // chunk (+1 for the chunk's first plane, read twice): enough workgroups to fill every CU,
+    // and no last round left mostly empty.
+    const int64_t waves = (h->n[2] + 63) / 64;
+    const int64_t slots = (int64_t) ncu * std::max<int64_t> (1, 16 / waves);
+    auto chunk_cost = [&] (int64_t c) {
+      const int64_t rounds = (h->fused_nstrips * c + slots - 1) / slots;
+      return (double) rounds * (double) ((planes + c - 1) / c + 1);
+    };
+    double best = 1e300;
+    for (int64_t c = 1; c <= planes; ++c) best = std::min (best, chunk_cost (c));
+    int64_t nch = 1;   // the fewest chunks within 1 % of the best
+    while (nch < planes && chunk_cost (nch) > 1.01 * best) ++nch;
     if (const char *e = std::getenv ("YEE_FUSED_CHUNKS")) nch = std::max (1, std::atoi (e));
     nch = std::min (nch, planes);
     h->fused_lx = (int) ((planes + nch - 1) / nch);
