@@ -463,8 +463,17 @@ int forward_impl (yee_handle *h, const double *design, double *J)
   const size_t nm = (size_t) s->mnx * s->mny * W;
   std::vector<T> mon (nm);
   HIP_TRY (hipMemcpy (mon.data (), s->mon, nm * sizeof (T), hipMemcpyDeviceToHost));
-  double acc = 0;
-  for (size_t u = 0; u < nm; ++u) acc += (double) mon[u] * (double) mon[u];
+  // compensated (Neumaier) sum: J is differenced by finite-difference checks, where the
+  // rounding of a plain sum over millions of terms would dominate
+  double acc = 0, comp = 0;
+  for (size_t u = 0; u < nm; ++u)
+  {
+    const double v = (double) mon[u] * (double) mon[u];
+    const double t = acc + v;
+    comp += std::fabs (acc) >= std::fabs (v) ? (acc - t) + v : (v - t) + acc;
+    acc = t;
+  }
+  acc += comp;
   s->J = acc;
   *J = acc;
   s->forward_done = true;
