@@ -243,6 +243,50 @@ def run_cfg5(args) -> dict:
     return out
 
 
+def precision_check(args) -> dict:
+    """FP32 against FP64 on the same inputs (the FP32-rounded materials), both on the GPU, after
+    --check-steps time steps: relative L2 of every field component.  The FP64 engine is itself
+    pinned to the oracle at 1e-12, so this is the FP32 error of the full-size run."""
+    from fdtd3d_amd.engine import Engine
+    wl = WORKLOADS[args.workload]
+    steps = args.check_steps
+    ref = {}
+    out = {}
+    for prec in (8, 4):
+        cfg = wl["make"]()
+        cfg.precision = prec
+        cfg.time_steps = steps
+        eng = Engine(cfg)
+        if args.workload == "cfg3-drude":
+            for name, arr in drude_materials(cfg).items():
+                eng.set_material(name, arr)
+        else:
+            eng.set_material("eps", block_eps(cfg.shape, dtype=np.float32))
+        eng.prepare()
+        t0 = time.perf_counter()
+        eng.step(steps)
+        eng.sync()
+        print(f"[bench] precision check: FP{8 * prec} {steps} steps in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        comps = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+        for comp in comps:
+            f = eng.field_raw(comp)
+            if prec == 8:
+                ref[comp] = f
+                continue
+            num = den = 0.0
+            for x0 in range(0, f.shape[0], 64):   # in slabs of planes: no full-size temporaries
+                r = ref[comp][x0:x0 + 64]
+                d = f[x0:x0 + 64].astype(np.float64) - r
+                num += float(np.vdot(d, d))
+                den += float(np.vdot(r, r))
+            out[comp] = math.sqrt(num / den) if den > 0 else 0.0
+            del f
+            ref[comp] = None
+        eng.close()
+    return {"steps": steps, "rel_l2": out, "max_rel_l2": max(out.values())}
+
+
 def _plain(c: Config) -> Config:
     c.use_pml = False
     c.use_tfsf = False
@@ -351,6 +395,9 @@ def main():
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS) + ["cfg4-batch", "cfg5-adjoint"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000, help="time steps of the cfg4-batch workload")
+    ap.add_argument("--precision-check", action="store_true",
+                    help="instead of timing: FP32 vs FP64 relative L2 per component after --check-steps steps")
+    ap.add_argument("--check-steps", type=int, default=1000, help="time steps of --precision-check")
     ap.add_argument("--batch-only", action="store_true",
                     help="cfg4-batch: time the batched run only (for profiling; no speed-up figure)")
     ap.add_argument("--fd-voxels", type=int, default=16, help="finite-difference voxels of the cfg5-adjoint workload")
@@ -400,6 +447,13 @@ def main():
             "detail": r,
         }
         print(json.dumps(line), flush=True)
+        return
+
+    if args.precision_check:
+        if args.workload not in WORKLOADS:
+            sys.exit("--precision-check applies to the 3D workloads")
+        r = precision_check(args)
+        print(json.dumps({"workload": args.workload, "check": "fp32 vs fp64 on the GPU, same inputs", **r}), flush=True)
         return
 
     wl = WORKLOADS[args.workload]
