@@ -206,8 +206,8 @@ double source_value (const yee_handle *h, bool e_half, int32_t t)
   return v;
 }
 
-// tfsf_xy_near of the kernels (yee_kernels.hip): the cell lies in [L-1, R] on every axis and
-// on one of the planes L-1, R-1, R of the x or y axis.
+// A cell next to an x or y face of the TF/SF box (the kAxFRange / kAxFPlane bits of the fused
+// kernel): inside [L-1, R] on every axis and on one of the planes L-1, R-1, R of x or y.
 bool tfsf_near_host (const yee_handle *h, const int64_t p[3])
 {
   bool face = false;
@@ -220,7 +220,9 @@ bool tfsf_near_host (const yee_handle *h, const int64_t p[3])
   return face;
 }
 
-// Does the fused kernel update cell p (fused_owns of the kernels)?
+// Is cell p inside the fused kernel's box and not one of the cells it leaves to the half-step
+// kernels for other reasons (TF/SF x/y faces, Drude box)?  (own_mask of the kernels, without
+// the source cell.)
 bool fused_owns_host (const yee_handle *h, const int64_t p[3])
 {
   bool drude = h->mode == 2;
@@ -309,7 +311,7 @@ int slab_boxes (const yee_handle *h, int64_t lo[6][3], int64_t hi[6][3])
   return m;
 }
 
-// The cells next to the x and y faces of the TF/SF box (tfsf_xy_near): for x and y the planes
+// The cells next to the x and y faces of the TF/SF box (tfsf_near_host): for x and y the planes
 // L-1, R-1 and R, spanning [L-1, R] on the other axes, clipped to the interior (the z faces
 // are corrected inside the fused kernel).  Boxes may overlap; a cell updated twice from the
 // old buffer gets the same value both times.

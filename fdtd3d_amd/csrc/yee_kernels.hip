@@ -272,11 +272,12 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 // ---------------------------------------------------------------------------------------
 // Fused 3D time step: E of step t, then H of step t, in one sweep of the cells it owns.
 //
-// Owned cells: the PML-free box [run_lo, run_hi) minus the cells next to the TF/SF faces
-// (tfsf_near).  The other cells are updated by e_half_3d / h_half_3d on slab and face boxes,
-// the E updates before this kernel and the H updates after it.  State is double-buffered:
-// every old value comes from f, every new one goes to f_n, and the new E of cells this kernel
-// does not own is read back from f_n.
+// Owned cells: the PML-free box [run_lo, run_hi) minus the cells next to the TF/SF x and y
+// faces, the Drude box and the point-source cell (own_mask).  The other cells are updated by
+// e_half_3d / h_half_3d on slab and face boxes, the E updates before this kernel and the H
+// updates after it; the z faces of the TF/SF box are corrected here from a per-step table.
+// State is double-buffered: every old value comes from f, every new one goes to f_n, and the
+// new E of cells this kernel does not own is read back from f_n.
 //
 // Each workgroup owns a full z row (one lane per z cell, up to 16 waves) of a strip of BY
 // y rows and marches along x over a chunk of planes.  Per x plane it computes E one plane
@@ -284,9 +285,10 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 // H at plane x-1 needs E after the step at x-1 (carried) and x (just computed).  The loads of
 // plane x+1 are issued before plane x is computed, so their latency overlaps the arithmetic,
 // the stores and the barrier.  The y neighbour row j0+BY is recomputed (Ex, Ez only; not
-// stored); the z neighbour of a lane comes from the next lane by a cross-lane shuffle and,
-// for the last lane of a wave, from the next wave through LDS.  Every state word crosses HBM
-// once per step, except the recomputed row and the first plane of each chunk.
+// stored); the z+1 neighbours of a lane (new Ex, Ey; eps) come from the next lane by a DPP
+// wave shift and, for the last lane of a wave, from lane 0 of the next wave through LDS.  Every
+// state word crosses HBM once per step, except the recomputed row and the first plane of each
+// chunk.
 // ---------------------------------------------------------------------------------------
 
 // Ownership bits of one coordinate of a cell (axis_bits): inside the run box, inside
@@ -309,16 +311,8 @@ struct FusedPos
   bool wave_zface;   // a lane of the wave on a z face of the TF/SF box
 };
 
-// Direct E/H update (computeCaCb non-PML branch, calcField; same expressions as cell_value).
-template <typename T, int COMP>
-__device__ __forceinline__ T direct_value (const StepArgs<T> &a, T f_old, T tmp, T mat)
-{
-  const T modc = COMP >= 3 ? a.mu0 : a.eps0;
-  const T cb = direct_cb (a.dt, mat, modc, a.dx);
-  return f_old * (T) 1.0 + tmp * cb;
-}
-
-// The same with the coefficient already computed (uniform material).
+// Direct E/H update (computeCaCb non-PML branch, calcField; the expression of cell_value) with
+// the coefficient cb = direct_cb (dt, m, m0, dx) already computed.
 template <typename T>
 __device__ __forceinline__ T direct_value_cb (T f_old, T tmp, T cb)
 {
