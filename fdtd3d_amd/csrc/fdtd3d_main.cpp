@@ -5,11 +5,16 @@
 // --same-size* copy rules, std::stof for floating-point values, --cmd-from-file), sets the run
 // up as Scheme::initScheme / initGrids do (materials from DAT files and spheres), performs
 // --time-steps steps and prints the reference's "Total time = ... seconds" line
-// (Source/main.cpp:1175-1211).  Options of the reference outside this path are either
-// accepted without effect (output and logging options) or rejected with an error (other
-// schemes and physics).
+// (Source/main.cpp:1175-1211).  Result dumps follow Scheme::saveGrids
+// (Source/Scheme/Scheme.inc.h:4367-4531) in the text format (TXTDumper,
+// Source/File-Management/Dumper/TXTDumper.h:50-130) under the reference's file names
+// (GridFileManager::setFileNames, Source/File-Management/Commons.h:64-122).  Options of the
+// reference outside this path are either accepted without effect (logging, NTFF and other
+// options that do not change the fields or the text dumps) or rejected with an error (other
+// schemes and physics, image and binary dumps, scattered-field dumps).
 //
 // Built twice: fdtd3d (FP64 fields, the reference's default value type) and fdtd3d-f32.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -34,8 +39,20 @@ struct Sphere
   int64_t radius = 0;
 };
 
+// Result dump options (Settings.inc.h:199-217, 279-281).
+struct SaveOptions
+{
+  bool res = false, interm = false, txt = false, per_process = false, manual = false, no_pml = false;
+  bool einc = false, hinc = false;
+  int64_t interm_step = 100;
+  int64_t start[3] = { 0, 0, 0 }, end[3] = { 0, 0, 0 };
+  int orth = 2;   // orthogonal axis of the default 3D dump range (--orth-axis-x/y/z)
+  std::vector<std::string> refused;   // image / binary / scattered-field dumps asked for
+};
+
 struct Options
 {
+  SaveOptions save;
   yee_config cfg {};
   std::string file[6];            // --load-<material>-from-file
   Sphere sphere[4] = { { 1.0 }, { 1.0 }, { 0.0 }, { 0.0 } };   // eps, mu, omegape, omegapm
@@ -95,15 +112,11 @@ std::vector<std::string> read_cmd_file (const std::string &path)
 const std::set<std::string> kIgnoredWithArg = {
   "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size", "--cuda-buffer-size",
   "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end", "--ntff-filename", "--ntff-step-angle",
-  "--interm-ntff-step", "--interm-save-step", "--save-start-coord", "--save-end-coord", "--rebalance-step",
-  "--stored-steps", "--current-source", "--save-cmd-to-file",
+  "--interm-ntff-step", "--rebalance-step", "--stored-steps", "--current-source", "--save-cmd-to-file",
 };
 const std::set<std::string> kIgnoredFlags = {
-  "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff", "--save-res",
-  "--save-res-per-process", "--save-materials", "--save-interm-res", "--save-scattered-field-res",
-  "--save-scattered--field-interm", "--save-as-bmp", "--save-as-dat", "--save-as-txt", "--save-tfsf-e-incident",
-  "--save-tfsf-h-incident", "--manual-save-coords", "--save-no-pml", "--palette-gray", "--palette-rgb",
-  "--orth-axis-x", "--orth-axis-y", "--orth-axis-z", "--dynamic-grid", "--check-disabling-cond",
+  "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff",
+  "--save-materials", "--palette-gray", "--palette-rgb", "--dynamic-grid", "--check-disabling-cond",
   "--check-enabling-cond", "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
   "--ntff-to-stdout",
 };
@@ -181,6 +194,23 @@ Options parse (int argc, char **argv)
     else if (a == "--point-source") parse_coord (val (), 0, c.point_source_pos);
     else if (a == "--sphere-accuracy") o.accuracy = std::stoi (val ());
     else if (a == "--stair-sphere-approx") o.stair = true;
+    else if (a == "--save-res") o.save.res = true;
+    else if (a == "--save-interm-res") o.save.interm = true;
+    else if (a == "--interm-save-step") o.save.interm_step = std::stoll (val ());
+    else if (a == "--save-as-txt") o.save.txt = true;
+    else if (a == "--save-res-per-process") o.save.per_process = true;
+    else if (a == "--manual-save-coords") o.save.manual = true;
+    else if (a == "--save-start-coord") parse_coord (val (), 0, o.save.start);
+    else if (a == "--save-end-coord") parse_coord (val (), 0, o.save.end);
+    else if (a == "--save-no-pml") o.save.no_pml = true;
+    else if (a == "--save-tfsf-e-incident") o.save.einc = true;
+    else if (a == "--save-tfsf-h-incident") o.save.hinc = true;
+    else if (a == "--orth-axis-x") o.save.orth = 0;
+    else if (a == "--orth-axis-y") o.save.orth = 1;
+    else if (a == "--orth-axis-z") o.save.orth = 2;
+    else if (a == "--save-as-bmp" || a == "--save-as-dat" || a == "--save-scattered-field-res"
+             || a == "--save-scattered--field-interm")
+      o.save.refused.push_back (a);
     else done = false;
     if (done) continue;
     for (int k = 0; k < 6 && !done; ++k)
@@ -238,6 +268,118 @@ std::vector<double> load_dat (const std::string &path, int64_t n)
   return v;
 }
 
+// Minimum coordinate of a component in the E_CENTERED layout, doubled (YeeGridLayout.cpp:1937-2043):
+// Ex (1, .5, .5), Ey (.5, 1, .5), Ez (.5, .5, 1), Hx (.5, 1, 1), Hy (1, .5, 1), Hz (1, 1, .5).
+int min_coord2 (int comp, int axis)
+{
+  static const int m[6][3] = { { 2, 1, 1 }, { 1, 2, 1 }, { 1, 1, 2 }, { 1, 2, 2 }, { 2, 1, 2 }, { 2, 2, 1 } };
+  return m[comp][axis];
+}
+
+// Dump range [start, end) of a component (Scheme::getStartCoord / getEndCoord, Scheme.inc.h:4647-4790;
+// the middle plane along the orthogonal axis in 3D, SchemeHelper.h:860-990).
+void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int64_t start[3], int64_t end[3])
+{
+  const SaveOptions &sv = o.save;
+  if (sv.manual)
+  {
+    for (int a = 0; a < 3; ++a)
+    {
+      start[a] = sv.start[a];
+      end[a] = sv.end[a];
+    }
+    return;
+  }
+  for (int a = 0; a < 3; ++a)
+  {
+    start[a] = 0;
+    end[a] = size[a];
+  }
+  if (sv.no_pml && (o.cfg.flags & YEE_USE_PML))
+  {
+    // convertCoord (border - min) truncates the half-integer coordinate; start adds 1
+    for (int a = 0; a < dim; ++a)
+    {
+      const int64_t L = o.cfg.pml[a], R = size[a] - o.cfg.pml[a];
+      start[a] = (2 * L - min_coord2 (comp, a)) / 2 + 1;
+      end[a] = (2 * R - min_coord2 (comp, a)) / 2;
+    }
+  }
+  if (dim == 3)
+  {
+    start[sv.orth] = size[sv.orth] / 2;
+    end[sv.orth] = size[sv.orth] / 2 + 1;
+  }
+}
+
+// One text dump, TXTDumper::writeToFile: "x [y [z]] value" per cell of [start, end) in [x][y][z]
+// order, values with 15 significant digits; file "previous-1_[timestep=T]_[pid=0]_[name=N].txt"
+// (the layer after the step, index 1 of the two stored layers).
+void write_txt (const std::string &name, int64_t t, int dim, const int64_t size[3], const int64_t start[3],
+                const int64_t end[3], const double *v)
+{
+  int64_t total = 1;
+  for (int a = 0; a < dim; ++a) total *= size[a];
+  std::printf ("Saving grid '%s' to text. Time step: 1. Size: %lld (from startCoord to endCoord). \n", name.c_str (),
+               (long long) total);
+  const std::string path = "previous-1_[timestep=" + std::to_string (t) + "]_[pid=0]_[name=" + name + "].txt";
+  FILE *f = std::fopen (path.c_str (), "w");
+  if (!f) die ("cannot write " + path);
+  const int64_t nz = dim == 3 ? size[2] : 1;
+  for (int64_t x = start[0]; x < end[0]; ++x)
+    for (int64_t y = (dim >= 2 ? start[1] : 0); y < (dim >= 2 ? end[1] : 1); ++y)
+      for (int64_t z = (dim == 3 ? start[2] : 0); z < (dim == 3 ? end[2] : 1); ++z)
+      {
+        const double val = v[(x * (dim >= 2 ? size[1] : 1) + y) * nz + z];
+        if (dim == 3) std::fprintf (f, "%lld %lld %lld %.15g\n", (long long) x, (long long) y, (long long) z, val);
+        else if (dim == 2) std::fprintf (f, "%lld %lld %.15g\n", (long long) x, (long long) y, val);
+        else std::fprintf (f, "%lld %.15g\n", (long long) x, val);
+      }
+  std::fclose (f);
+  std::printf ("Saved. \n");
+}
+
+// Scheme::saveGrids (time step t): every component of the scheme, then EInc / HInc if asked.
+void save_grids (const Options &o, yee_handle *h, int64_t t)
+{
+  const yee_config &c = o.cfg;
+  const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  const int64_t size[3] = { c.size[0], c.size[1], dim == 3 ? c.size[2] : 1 };
+  const int64_t n = size[0] * size[1] * size[2];
+  std::vector<double> v ((size_t) n);
+  static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
+  for (int comp = 0; comp < 6; ++comp)
+  {
+    if (dim == 2 && comp != YEE_EZ && comp != YEE_HX && comp != YEE_HY) continue;
+    check (yee_get_field (h, comp, v.data ()));
+    int64_t start[3], end[3];
+    if (o.save.per_process)
+    {
+      for (int a = 0; a < 3; ++a)
+      {
+        start[a] = 0;
+        end[a] = size[a];
+      }
+    }
+    else
+    {
+      dump_range (o, comp, dim, size, start, end);
+    }
+    for (int a = 0; a < dim; ++a)
+      if (start[a] < 0 || end[a] > size[a] || start[a] >= end[a]) die ("dump coordinates out of the grid");
+    write_txt (names[comp], t, dim, size, start, end, v.data ());
+  }
+  if ((o.save.einc || o.save.hinc) && (c.flags & YEE_USE_TFSF))
+  {
+    const int64_t len = 500 * c.size[0];
+    std::vector<double> e ((size_t) len), hv ((size_t) len);
+    check (yee_get_incident (h, e.data (), hv.data (), len));
+    const int64_t s1[3] = { len, 1, 1 }, b[3] = { 0, 0, 0 };
+    if (o.save.einc) write_txt ("EInc", t, 1, s1, b, s1, e.data ());
+    if (o.save.hinc) write_txt ("HInc", t, 1, s1, b, s1, hv.data ());
+  }
+}
+
 }  // namespace
 
 int main (int argc, char **argv)
@@ -281,8 +423,30 @@ int main (int argc, char **argv)
   }
   check (yee_prepare (h));
 
+  const SaveOptions &sv = o.save;
+  const bool dumping = (sv.res || sv.interm) && sv.txt;
+  if ((sv.res || sv.interm) && !sv.refused.empty ())
+    die ("option " + sv.refused[0] + " is not supported (text dumps only: --save-as-txt)");
+  if (sv.interm && sv.interm_step <= 0) die ("--interm-save-step must be positive");
   const auto t0 = std::chrono::steady_clock::now ();
-  check (yee_step (h, c.time_steps));
+  if (dumping && sv.interm)
+  {
+    // SchemeHelper::performNSteps3D (SchemeHelper.h:100-118): after step t, save when
+    // t / step < (t + 1) / step, as time step t + 1
+    int64_t done = 0;
+    while (done < c.time_steps)
+    {
+      const int64_t next = std::min<int64_t> (c.time_steps, (done / sv.interm_step + 1) * sv.interm_step);
+      check (yee_step (h, (int32_t) (next - done)));
+      done = next;
+      if (done % sv.interm_step == 0) save_grids (o, h, done);
+    }
+  }
+  else
+  {
+    check (yee_step (h, c.time_steps));
+  }
+  if (dumping && sv.res) save_grids (o, h, c.time_steps);
   check (yee_sync (h));
   const double secs = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
   std::printf ("Total time = %f seconds\n", secs);

@@ -62,3 +62,69 @@ def test_cli_runs_reference_command_line(binary, engine_lib):
     assert "Total time = " in out.stdout
     assert "Cell updates = 3136000 " in out.stdout   # 40^3 x 49
 
+
+
+def _read_txt(path):
+    rows = np.loadtxt(path, ndmin=2)
+    return rows[:, :-1].astype(np.int64), rows[:, -1]
+
+
+def test_cli_text_dumps(engine_lib, tmp_path):
+    """--save-res / --save-interm-res with --save-as-txt write the reference's text dumps
+    (Scheme::saveGrids, TXTDumper): file names, the default 3D range (the middle z plane),
+    --manual-save-coords, the incident grid, and the values of the engine's fields."""
+    from fdtd3d_amd.config import parse_args
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    base = ("--3d --size x:24,y:20,z:22 --use-pml --pml-size x:4 --same-size-pml --use-tfsf "
+            "--tfsf-size-left x:8 --tfsf-size-right x:8 --same-size-tfsf --angle-teta 30 --angle-phi 45 "
+            "--angle-psi 90 --time-steps 50")
+    args = base + " --save-res --save-as-txt --save-interm-res --interm-save-step 20 --save-tfsf-e-incident"
+    say("[gpu] cli text dumps 3D")
+    out = subprocess.run([exe] + args.split(), capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    assert "Saving grid 'Ex' to text. Time step: 1. Size: 10560 (from startCoord to endCoord). " in out.stdout
+    for t in (20, 40, 50):
+        for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "EInc"):
+            assert (tmp_path / f"previous-1_[timestep={t}]_[pid=0]_[name={name}].txt").exists(), (t, name)
+    cfg = parse_args(base.split())
+    eng = Engine(cfg)
+    eng.prepare()
+    eng.step(50)
+    eng.sync()
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        xyz, val = _read_txt(tmp_path / f"previous-1_[timestep=50]_[pid=0]_[name={name}].txt")
+        # the middle z plane, [x][y] order
+        assert xyz.shape == (24 * 20, 3)
+        assert (xyz[:, 2] == 11).all()
+        assert (xyz[:, 0] == np.repeat(np.arange(24), 20)).all() and (xyz[:, 1] == np.tile(np.arange(20), 24)).all()
+        ref = eng.field(name)[:, :, 11].ravel()
+        assert np.allclose(val, ref, rtol=1e-14, atol=1e-300), name
+    xi, ei = _read_txt(tmp_path / "previous-1_[timestep=50]_[pid=0]_[name=EInc].txt")
+    assert xi.shape == (500 * 24, 1) and (xi[:, 0] == np.arange(500 * 24)).all()
+    einc, _ = eng.incident()
+    assert np.allclose(ei, einc[: 500 * 24], rtol=1e-14, atol=1e-300)
+    eng.close()
+
+    say("[gpu] cli text dumps TMz, manual range")
+    tmz = ("--2d-tmz --size x:30,y:26 --use-pml --pml-size x:4,y:4 --point-source x:12,y:10 "
+           "--point-source-ez --time-steps 35")
+    out = subprocess.run([exe] + (tmz + " --save-res --save-as-txt --manual-save-coords "
+                                  "--save-start-coord x:3,y:5 --save-end-coord x:20,y:11").split(),
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    eng = Engine(parse_args(tmz.split()))
+    eng.prepare()
+    eng.step(35)
+    eng.sync()
+    for name in ("Ez", "Hx", "Hy"):
+        xy, val = _read_txt(tmp_path / f"previous-1_[timestep=35]_[pid=0]_[name={name}].txt")
+        assert xy.shape == (17 * 6, 2) and xy[0].tolist() == [3, 5] and xy[-1].tolist() == [19, 10]
+        assert np.allclose(val, eng.field(name)[3:20, 5:11].ravel(), rtol=1e-14, atol=1e-300), name
+    assert not (tmp_path / "previous-1_[timestep=35]_[pid=0]_[name=Ex].txt").exists()
+    eng.close()
+
+    out = subprocess.run([exe] + (tmz + " --save-res --save-as-bmp").split(), capture_output=True, text=True,
+                         timeout=120, cwd=tmp_path)
+    assert out.returncode != 0 and "--save-as-bmp" in out.stderr
