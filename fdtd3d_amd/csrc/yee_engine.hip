@@ -29,7 +29,7 @@ template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
 template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
-template <typename T, int BY, int MAT> __global__ void step_3d_fused (StepArgs<T>);
+template <typename T, int BY, int MAT, int MAXW> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 
 static thread_local std::string g_error;
@@ -407,12 +407,18 @@ int fused_step (yee_handle *h, int32_t t, int prev)
     ProfScope ps (h, 3);
     const unsigned blocks = (unsigned) (h->fused_nstrips * h->fused_nchunks);
     const dim3 grid (blocks), block ((unsigned) h->fused_threads);
-    switch ((a.eps ? 1 : 0) | (a.mu ? 2 : 0))
+    // rows of 9-12 waves: the 12-wave instantiation (more registers, the same one workgroup per CU)
+    const int w12 = (h->fused_threads > 512 && h->fused_threads <= 768) ? 4 : 0;
+    switch (w12 | (a.eps ? 1 : 0) | (a.mu ? 2 : 0))
     {
-      case 0: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 0>), grid, block, 0, h->stream, a); break;
-      case 1: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 1>), grid, block, 0, h->stream, a); break;
-      case 2: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 2>), grid, block, 0, h->stream, a); break;
-      default: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 3>), grid, block, 0, h->stream, a); break;
+      case 0: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 0, 16>), grid, block, 0, h->stream, a); break;
+      case 1: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 1, 16>), grid, block, 0, h->stream, a); break;
+      case 2: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 2, 16>), grid, block, 0, h->stream, a); break;
+      case 3: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 3, 16>), grid, block, 0, h->stream, a); break;
+      case 4: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 0, 12>), grid, block, 0, h->stream, a); break;
+      case 5: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 1, 12>), grid, block, 0, h->stream, a); break;
+      case 6: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 2, 12>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL ((step_3d_fused<T, FUSED_BY (T), 3, 12>), grid, block, 0, h->stream, a); break;
     }
     HIP_TRY (hipGetLastError ());
   }

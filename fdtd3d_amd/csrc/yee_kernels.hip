@@ -728,8 +728,10 @@ __device__ __forceinline__ void fused_plane (const StepArgs<T> &a, const FusedPo
   __syncthreads ();
 }
 
-template <typename T, int BY, int MAT>
-__global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
+// MAXW: the most waves of a workgroup (rows of up to 64 MAXW cells); 12 lets the compiler use
+// up to 170 VGPRs per lane, where 16 waves allow 128 (FP64 state would spill).
+template <typename T, int BY, int MAT, int MAXW>
+__global__ __launch_bounds__ (64 * MAXW) void step_3d_fused (StepArgs<T> a)
 {
   __shared__ FusedLds<T, BY> s;
   constexpr int kZStage = 2 * 3 * (BY + 1) * 4;   // z-face table entries per parity
@@ -834,7 +836,7 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
     __syncthreads ();
   }
 
-  if (sizeof (T) == 4)
+  if (sizeof (T) == 4 || MAXW <= 12)
   {
     // unrolled by two planes: the two copies of the state swap roles without register copies
     for (int x = xa + 1; x <= xend; x += 2)
@@ -846,7 +848,8 @@ __global__ __launch_bounds__ (1024) void step_3d_fused (StepArgs<T> a)
   }
   else
   {
-    // FP64 (twice the registers per value): one plane per iteration, fewer live values
+    // FP64 in 16-wave workgroups (128 VGPRs; twice the registers per value): one plane per
+    // iteration, fewer live values
     int par = 1;
     for (int x = xa + 1; x <= xend; ++x, par ^= 1)
     {
@@ -1462,10 +1465,14 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void batch_tmz_step2<T> (const StepArgs<T> *, const int4 *, int, const int *, const T *, int); \
   template __global__ void batch_tmz_sources<T> (const StepArgs<T> *, int, const int *, const T *, int); \
   template __global__ void incident_step<T> (T *, const T *, T *, int64_t, int, double, double, double); \
-  template __global__ void step_3d_fused<T, FUSED_BY (T), 0> (StepArgs<T>);                \
-  template __global__ void step_3d_fused<T, FUSED_BY (T), 1> (StepArgs<T>);                \
-  template __global__ void step_3d_fused<T, FUSED_BY (T), 2> (StepArgs<T>);                \
-  template __global__ void step_3d_fused<T, FUSED_BY (T), 3> (StepArgs<T>);                \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 0, 16> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 1, 16> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 2, 16> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 3, 16> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 0, 12> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 1, 12> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 2, 12> (StepArgs<T>);            \
+  template __global__ void step_3d_fused<T, FUSED_BY (T), 3, 12> (StepArgs<T>);            \
   template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *);

@@ -181,6 +181,9 @@ SPLIT_VS_FUSED = [
     # point source inside a PML slab, nz not a multiple of 64
     ("--3d --size x:30,y:26,z:90 --use-pml --pml-size x:4,y:3,z:7 --point-source x:2,y:13,z:45 "
      "--point-source-hz --point-source-ex --time-steps 50"),
+    # rows of 10 waves (the 12-wave instantiation of the fused kernel), PML + TF/SF
+    ("--3d --size x:14,y:12,z:620 --use-pml --pml-size x:3,y:2,z:5 --use-tfsf --tfsf-size-left x:5,y:4,z:6 "
+     "--tfsf-size-right x:5,y:4,z:7 --angle-teta 20 --angle-phi 70 --angle-psi 10 --time-steps 30"),
 ]
 
 
