@@ -11,7 +11,7 @@
 // (GridFileManager::setFileNames, Source/File-Management/Commons.h:64-122).  Options of the
 // reference outside this path are either accepted without effect (logging, NTFF and other
 // options that do not change the fields or the text dumps) or rejected with an error (other
-// schemes and physics, image and binary dumps, scattered-field dumps).
+// schemes and physics, image and binary dumps).
 //
 // Built twice: fdtd3d (FP64 fields, the reference's default value type) and fdtd3d-f32.
 #include <algorithm>
@@ -44,10 +44,11 @@ struct SaveOptions
 {
   bool res = false, interm = false, txt = false, per_process = false, manual = false, no_pml = false;
   bool einc = false, hinc = false;
+  bool scat_res = false, scat_interm = false;   // --save-scattered-field-res / --save-scattered--field-interm
   int64_t interm_step = 100;
   int64_t start[3] = { 0, 0, 0 }, end[3] = { 0, 0, 0 };
   int orth = 2;   // orthogonal axis of the default 3D dump range (--orth-axis-x/y/z)
-  std::vector<std::string> refused;   // image / binary / scattered-field dumps asked for
+  std::vector<std::string> refused;   // image / binary dumps asked for
 };
 
 struct Options
@@ -208,9 +209,9 @@ Options parse (int argc, char **argv)
     else if (a == "--orth-axis-x") o.save.orth = 0;
     else if (a == "--orth-axis-y") o.save.orth = 1;
     else if (a == "--orth-axis-z") o.save.orth = 2;
-    else if (a == "--save-as-bmp" || a == "--save-as-dat" || a == "--save-scattered-field-res"
-             || a == "--save-scattered--field-interm")
-      o.save.refused.push_back (a);
+    else if (a == "--save-scattered-field-res") o.save.scat_res = true;
+    else if (a == "--save-scattered--field-interm") o.save.scat_interm = true;
+    else if (a == "--save-as-bmp" || a == "--save-as-dat") o.save.refused.push_back (a);
     else done = false;
     if (done) continue;
     for (int k = 0; k < 6 && !done; ++k)
@@ -340,7 +341,9 @@ void write_txt (const std::string &name, int64_t t, int dim, const int64_t size[
 }
 
 // Scheme::saveGrids (time step t): every component of the scheme, then EInc / HInc if asked.
-void save_grids (const Options &o, yee_handle *h, int64_t t)
+// With scattered set, the components are first made scattered (Scheme::gatherFieldsTotal,
+// Scheme.inc.h:4335-4362); per-process dumps write the fields as they are.
+void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
 {
   const yee_config &c = o.cfg;
   const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
@@ -351,7 +354,8 @@ void save_grids (const Options &o, yee_handle *h, int64_t t)
   for (int comp = 0; comp < 6; ++comp)
   {
     if (dim == 2 && comp != YEE_EZ && comp != YEE_HX && comp != YEE_HY) continue;
-    check (yee_get_field (h, comp, v.data ()));
+    if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, v.data ()));
+    else check (yee_get_field (h, comp, v.data ()));
     int64_t start[3], end[3];
     if (o.save.per_process)
     {
@@ -428,6 +432,12 @@ int main (int argc, char **argv)
   if ((sv.res || sv.interm) && !sv.refused.empty ())
     die ("option " + sv.refused[0] + " is not supported (text dumps only: --save-as-txt)");
   if (sv.interm && sv.interm_step <= 0) die ("--interm-save-step must be positive");
+  // Scheme.h:195-204 asserts that per-process result dumps are not scattered; the incident
+  // wave that is subtracted exists only with TF/SF
+  if (dumping && sv.res && sv.per_process && sv.scat_res)
+    die ("--save-scattered-field-res cannot be combined with --save-res-per-process");
+  if (dumping && ((sv.res && sv.scat_res) || (sv.interm && sv.scat_interm)) && !(c.flags & YEE_USE_TFSF))
+    die ("scattered-field dumps need --use-tfsf");
   const auto t0 = std::chrono::steady_clock::now ();
   if (dumping && sv.interm)
   {
@@ -439,14 +449,14 @@ int main (int argc, char **argv)
       const int64_t next = std::min<int64_t> (c.time_steps, (done / sv.interm_step + 1) * sv.interm_step);
       check (yee_step (h, (int32_t) (next - done)));
       done = next;
-      if (done % sv.interm_step == 0) save_grids (o, h, done);
+      if (done % sv.interm_step == 0) save_grids (o, h, done, sv.scat_interm);
     }
   }
   else
   {
     check (yee_step (h, c.time_steps));
   }
-  if (dumping && sv.res) save_grids (o, h, c.time_steps);
+  if (dumping && sv.res) save_grids (o, h, c.time_steps, sv.scat_res);
   check (yee_sync (h));
   const double secs = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
   std::printf ("Total time = %f seconds\n", secs);

@@ -29,6 +29,7 @@ template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
 template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
+template <typename T, int COMP, int DIM> __global__ void scattered_kernel (StepArgs<T>, const T *, T *, int);
 template <typename T, int BY, int MAT, int MAXW> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 
@@ -1012,6 +1013,74 @@ int yee_get_field (yee_handle *h, int32_t comp, double *out)
   int rc = yee_get_field_raw (h, comp, tmp.data ());
   if (rc) return rc;
   for (size_t q = 0; q < tmp.size (); ++q) out[q] = tmp[q];
+  return YEE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int scattered (yee_handle *h, int comp, void *dev)
+{
+  const StepArgs<T> a = make_args<T> (h, 0);
+  const T *f = (const T *) h->f[h->cur][comp];
+  T *o = (T *) dev;
+  const int hsel = (h->t + 1) & 1;   // buffer of the newest HInc (yee_get_incident)
+  const dim3 grid ((unsigned) ((h->ncell + 255) / 256)), block (256);
+  if (h->dim == 3)
+    switch (comp)
+    {
+      case 0: hipLaunchKernelGGL ((scattered_kernel<T, 0, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 1: hipLaunchKernelGGL ((scattered_kernel<T, 1, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 4: hipLaunchKernelGGL ((scattered_kernel<T, 4, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      default: hipLaunchKernelGGL ((scattered_kernel<T, 5, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+    }
+  else
+    switch (comp)
+    {
+      case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      default: hipLaunchKernelGGL ((scattered_kernel<T, 4, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+    }
+  HIP_TRY (hipGetLastError ());
+  return YEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yee_get_scattered_field (yee_handle *h, int32_t comp, double *out)
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (comp < 0 || comp > 5 || !h->present[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
+  // the reference interpolates EInc / HInc, which exist only with TF/SF
+  if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
+  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  void *dev = nullptr;
+  HIP_TRY (hipMalloc (&dev, (size_t) h->ncell * h->wsize));
+  int rc = h->fp32 () ? scattered<float> (h, comp, dev) : scattered<double> (h, comp, dev);
+  hipError_t e = rc ? hipSuccess : hipStreamSynchronize (h->stream);
+  if (!rc && e == hipSuccess)
+  {
+    if (h->fp32 ())
+    {
+      std::vector<float> tmp ((size_t) h->ncell);
+      e = hipMemcpy (tmp.data (), dev, (size_t) h->ncell * 4, hipMemcpyDeviceToHost);
+      for (size_t q = 0; q < tmp.size (); ++q) out[q] = tmp[q];
+    }
+    else
+    {
+      e = hipMemcpy (out, dev, (size_t) h->ncell * 8, hipMemcpyDeviceToHost);
+    }
+  }
+  (void) hipFree (dev);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail (YEE_EDEVICE, std::string ("scattered field: ") + hipGetErrorString (e));
   return YEE_OK;
 }
 

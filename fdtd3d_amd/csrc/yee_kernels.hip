@@ -1412,6 +1412,33 @@ __global__ void point_source (T *f, int64_t q, double value)
 }
 
 // TF/SF predicate mask of one component for every cell (set-up check export).
+// Scattered field of one component (Scheme::makeGridScattered, Scheme.inc.h:4113-4228): the
+// newest field minus the incident plane wave at the component's grid points inside the TF/SF
+// box [L, R] (SchemeHelper::doSkipMakeScattered, SchemeHelper.h:905-911: points outside keep
+// the total field).  E takes EInc, H the newest HInc buffer (hsel), interpolated as the TF/SF
+// correction does.
+template <typename T, int COMP, int DIM>
+__global__ void scattered_kernel (StepArgs<T> a, const T *f, T *out, int hsel)
+{
+  const int nzz = DIM == 3 ? a.nz : 1;
+  const int64_t ncell = (int64_t) a.nx * a.ny * nzz;
+  const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncell) return;
+  const int k = (int) (q % nzz);
+  const int j = (int) ((q / nzz) % a.ny);
+  const int i = (int) (q / ((int64_t) nzz * a.ny));
+  double v = (double) f[q];
+  bool in = twice_coord<COMP, 0> (i) >= 2 * a.tl[0] && twice_coord<COMP, 0> (i) <= 2 * a.tr[0]
+            && twice_coord<COMP, 1> (j) >= 2 * a.tl[1] && twice_coord<COMP, 1> (j) <= 2 * a.tr[1];
+  if (DIM == 3) in = in && twice_coord<COMP, 2> (k) >= 2 * a.tl[2] && twice_coord<COMP, 2> (k) <= 2 * a.tr[2];
+  if (in)
+  {
+    const T *inc = COMP >= 3 ? a.tg->hinc[hsel] : a.tg->einc;
+    v -= incident_at<T, COMP, DIM> (a, i, j, k, inc, COMP >= 3 ? 0.5 : 0.0);
+  }
+  out[q] = (T) v;
+}
+
 template <typename T>
 __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out)
 {
@@ -1475,7 +1502,16 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void step_3d_fused<T, FUSED_BY (T), 3, 12> (StepArgs<T>);            \
   template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
-  template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *);
+  template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *); \
+  template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 1, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 2, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 3, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 4, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 5, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 2, 2> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 3, 2> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 4, 2> (StepArgs<T>, const T *, T *, int);
 
 YEE_INST (float)
 YEE_INST (double)

@@ -179,6 +179,12 @@ int yee_get_field (yee_handle *h, int32_t comp, double *out);
 /* Same as yee_get_field, in the handle's own precision (float* when precision == 4). */
 int yee_get_field_raw (yee_handle *h, int32_t comp, void *out);
 
+/* Scattered field of component comp (EX..HZ) as FP64 [x][y][z]: the newest field minus the
+ * incident plane wave at the component's grid points inside the TF/SF box, the total field
+ * outside it.  YEE_EINVAL without TF/SF.  Replaces Scheme::makeGridScattered (Scheme.inc.h:4113-4228),
+ * used by --save-scattered-field-res. */
+int yee_get_scattered_field (yee_handle *h, int32_t comp, double *out);
+
 /*
  * Mask export for bit-exact checks of the set-up tables.
  *   kind 0: TF/SF correction mask of component `comp`: bit d set when the reference's

@@ -41,6 +41,7 @@ def lib():
         L.oracle_get_material.argtypes = [vp, ctypes.c_int32, vp]
         L.oracle_tfsf_mask.argtypes = [vp, ctypes.c_int32, vp]
         L.oracle_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
+        L.oracle_get_scattered.argtypes = [vp, ctypes.c_int32, vp]
         L.oracle_get_scalars.argtypes = [vp, vp]
         L.oracle_incident_len.restype = ctypes.c_int64
         L.oracle_incident_len.argtypes = [vp]
@@ -182,6 +183,12 @@ class Oracle:
         out = np.empty(self.shape, dtype=np.uint8)
         if lib().oracle_tfsf_mask(self.h, COMP_INDEX[comp], out.ctypes.data):
             raise KeyError(comp)
+        return out
+
+    def scattered(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        if lib().oracle_get_scattered(self.h, COMP_INDEX[name], out.ctypes.data):
+            raise KeyError(name)
         return out
 
     def incident(self):

@@ -933,6 +933,37 @@ int oracle_get_incident (oracle *o, double *einc, double *hinc, int64_t len)
 }
 
 /*
+ * Scheme::makeGridScattered (Source/Scheme/Scheme.inc.h:4113-4228): layer 1 of the component
+ * minus the projected incident wave at its real coordinate (E from EInc with diff 0, H from
+ * HInc with diff 0.5, InternalScheme.inc.h:525-536), except where doSkipMakeScattered
+ * (Source/Scheme/SchemeHelper.h:897-911) skips the point: a real coordinate below the left
+ * or above the right TF/SF border on some axis keeps the total field.
+ */
+int oracle_get_scattered (oracle *o, int32_t comp, double *out)
+{
+  if (!o || !out || comp < 0 || comp > 5 || !o->present[comp] || !o->use_tfsf) return -1;
+  const int is_h = CD[comp].is_h;
+  const double *inc = is_h ? o->hinc[1] : o->einc[1];
+  const double ddiff = is_h ? 0.5 : 0.0;
+  for (int64_t i = 0; i < o->n[0]; ++i)
+    for (int64_t j = 0; j < o->n[1]; ++j)
+      for (int64_t k = 0; k < o->n[2]; ++k)
+      {
+        double real[3];
+        o_real_coord (o, comp, i, j, k, real);
+        int skip = 0;
+        for (int a = 0; a < o->dim; ++a)
+        {
+          if (real[a] < o->bl[a] || real[a] > o->br[a]) skip = 1;
+        }
+        const int64_t q = o_idx (o, i, j, k);
+        const double v = o->f[comp][1][q];
+        out[q] = skip ? v : v - o_project (o, comp, o_incident (o, real, ddiff, inc));
+      }
+  return 0;
+}
+
+/*
  * Test hook: shifts the phase of every source.  With phase pi/2 the run is the imaginary
  * part of the reference's complex-field build, whose sources are sin + i cos
  * (InternalScheme.inc.h:2886-2888, 3311-3312); together with the phase-0 run it gives

@@ -37,6 +37,8 @@ int oracle_get_material (oracle *o, int32_t which, double *out);
 /* TF/SF predicate bits per cell for component comp (same encoding as yee_get_mask kind 0). */
 int oracle_tfsf_mask (oracle *o, int32_t comp, uint8_t *out);
 int oracle_get_incident (oracle *o, double *einc, double *hinc, int64_t len);
+/* Scattered field of a component (same semantics as yee_get_scattered_field); -1 without TF/SF. */
+int oracle_get_scattered (oracle *o, int32_t comp, double *out);
 int oracle_get_scalars (oracle *o, double out[3]);
 int64_t oracle_incident_len (oracle *o);
 /* Test hook: phase added to every source argument (pi/2 gives the cos-source run). */

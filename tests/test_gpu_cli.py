@@ -128,3 +128,52 @@ def test_cli_text_dumps(engine_lib, tmp_path):
     out = subprocess.run([exe] + (tmz + " --save-res --save-as-bmp").split(), capture_output=True, text=True,
                          timeout=120, cwd=tmp_path)
     assert out.returncode != 0 and "--save-as-bmp" in out.stderr
+
+
+def test_cli_scattered_field_dumps(engine_lib, tmp_path):
+    """--save-scattered-field-res (Tests/suite/t6.1, t6.7 on Examples/vacuum3D_scattered.txt and
+    vacuum2D_TMz_scattered.txt with text dumps in place of images): the dumped scattered field of
+    a normally incident plane wave in vacuum is <= 1e-14 with minimum 0; --save-scattered--field-interm
+    applies to the intermediate dumps only."""
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    cases = [
+        ("--time-steps 100 --size x:40,y:40,z:40 --3d --angle-teta 0 --angle-phi 0 --angle-psi 0 --dx 0.0005 "
+         "--wavelength 0.02 --orth-axis-z --use-tfsf --tfsf-size-left x:12,y:12,z:12 "
+         "--tfsf-size-right x:12,y:12,z:12", "Ey"),
+        ("--time-steps 100 --size x:40,y:40 --2d-tmz --angle-teta 90 --angle-phi 0 --angle-psi 90 --dx 0.0005 "
+         "--wavelength 0.02 --orth-axis-z --use-tfsf --tfsf-size-left x:12,y:12 --tfsf-size-right x:12,y:12", "Ez"),
+    ]
+    for args, main in cases:
+        say(f"[gpu] cli scattered dumps {main}")
+        run = args + " --save-res --save-as-txt --save-scattered-field-res --save-interm-res --interm-save-step 60"
+        out = subprocess.run([exe] + run.split(), capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert out.returncode == 0, out.stderr
+        _, scat = _read_txt(tmp_path / f"previous-1_[timestep=100]_[pid=0]_[name={main}].txt")
+        assert np.abs(scat).max() <= 1e-14 and np.abs(scat).min() == 0.0, np.abs(scat).max()
+        # the intermediate dump at step 60 is the total field: the wave is there
+        _, tot = _read_txt(tmp_path / f"previous-1_[timestep=60]_[pid=0]_[name={main}].txt")
+        assert np.abs(tot).max() > 1e-2
+
+    say("[gpu] cli scattered dumps oblique 3D vs engine")
+    obl = ("--3d --size x:24,y:20,z:22 --use-pml --pml-size x:4 --same-size-pml --use-tfsf "
+           "--tfsf-size-left x:8 --tfsf-size-right x:8 --same-size-tfsf --angle-teta 30 --angle-phi 45 "
+           "--angle-psi 90 --time-steps 50")
+    out = subprocess.run([exe] + (obl + " --save-res --save-as-txt --save-scattered-field-res").split(),
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    from fdtd3d_amd.config import parse_args
+    eng = Engine(parse_args(obl.split()))
+    eng.prepare()
+    eng.step(50)
+    eng.sync()
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        _, val = _read_txt(tmp_path / f"previous-1_[timestep=50]_[pid=0]_[name={name}].txt")
+        assert np.allclose(val, eng.scattered(name)[:, :, 11].ravel(), rtol=1e-14, atol=1e-300), name
+    eng.close()
+
+    out = subprocess.run([exe] + ("--2d-tmz --size x:30,y:26 --time-steps 5 --save-res --save-as-txt "
+                                  "--save-scattered-field-res").split(), capture_output=True, text=True,
+                         timeout=120, cwd=tmp_path)
+    assert out.returncode != 0 and "--use-tfsf" in out.stderr

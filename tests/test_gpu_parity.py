@@ -147,6 +147,38 @@ def test_engine_vs_oracle_larger(name, args, precision, engine_lib):
     eng.close()
 
 
+SCATTERED_CASES = [
+    "--3d --size x:30,y:26,z:28 --use-pml --pml-size x:5 --same-size-pml --use-tfsf --tfsf-size-left x:8,y:7,z:9 "
+    "--tfsf-size-right x:9,y:8,z:7 --angle-teta 30 --angle-phi 40 --angle-psi 20 --time-steps 60",
+    "--2d-tmz --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:10,y:9 "
+    "--tfsf-size-right x:9,y:10 --angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 80",
+]
+
+
+@pytest.mark.parametrize("args", SCATTERED_CASES)
+@pytest.mark.parametrize("precision", [8, 4])
+def test_scattered_field_vs_oracle(args, precision, engine_lib):
+    """yee_get_scattered_field against the oracle's makeGridScattered restatement (oblique
+    incidence, so the subtracted wave is interpolated and projected on every component)."""
+    cfg = parse_args(args.split())
+    o = run_oracle(cfg)
+    cfg.precision = precision
+    say(f"[gpu] scattered {cfg.scheme} precision {precision}")
+    eng = run_engine(cfg)
+    for f in field_names(cfg):
+        ref = o.scattered(f)
+        got = eng.scattered(f)
+        if precision == 8:
+            assert rel_l2(got, ref) <= TOL_FP64, f"{f}: {rel_l2(got, ref)}"
+        else:
+            # scattered = total - incident: each term carries the FP32 bar relative to its own
+            # size, so the bar scales with (|total| + |incident|) / |scattered|
+            tot = o.field(f)
+            scale = (np.linalg.norm(tot) + np.linalg.norm(tot - ref)) / np.linalg.norm(ref)
+            assert rel_l2(got, ref) <= TOL_FP32 * scale, f"{f}: {rel_l2(got, ref)} (scale {scale:.2f})"
+    eng.close()
+
+
 def test_special_mask_covers_every_nonzero_sigma(engine_lib):
     cfg = parse_args("--3d --size x:20,y:18,z:22 --use-pml --pml-size x:4,y:3,z:5 --time-steps 1".split())
     say("[gpu] special mask")

@@ -4,6 +4,10 @@
   inside the total-field box (to sqrt(1.8e-5)) and exactly 0 outside after 200 steps.
   The reference checks the magnitude of a complex run whose sources are sin + i cos;
   here it is assembled from a sin-source run and a cos-source run.
+- Tests/suite/t6.1 and t6.7: the scattered field of a normally incident plane wave in vacuum
+  (Examples/vacuum3D_scattered.txt, vacuum2D_TMz_scattered.txt) has magnitude <= 1e-14 with
+  minimum 0; here it is exactly 0 for every component, which fixes the E/H incident grid,
+  the half-cell shift of HInc, the projection and the inclusive box borders.
 - unit-test-approximation.cpp:33-60: closed-form and Newton wave numbers agree to 1%.
 - the closed-form wave numbers solve the Yee dispersion relation.
 - the y/z PML sigma profile telescopes: sum of sigma over one slab = sigma_max * L / 7.
@@ -42,6 +46,27 @@ def test_unit_test_internalscheme_3d_tfsf_amplitude():
                     assert (mag[i, j, k] - 1.0) ** 2 < 0.000018, (i, j, k, mag[i, j, k])
                 else:
                     assert mag[i, j, k] < 1e-7, (i, j, k, mag[i, j, k])
+
+
+# Examples/vacuum3D_scattered.txt and vacuum2D_TMz_scattered.txt without the image options
+VACUUM_SCATTERED = [
+    ("--time-steps 100 --size x:40,y:40,z:40 --3d --angle-teta 0 --angle-phi 0 --angle-psi 0 --dx 0.0005 "
+     "--wavelength 0.02 --use-tfsf --tfsf-size-left x:12,y:12,z:12 --tfsf-size-right x:12,y:12,z:12", "Ey"),
+    ("--time-steps 100 --size x:40,y:40 --2d-tmz --angle-teta 90 --angle-phi 0 --angle-psi 90 --dx 0.0005 "
+     "--wavelength 0.02 --use-tfsf --tfsf-size-left x:12,y:12 --tfsf-size-right x:12,y:12", "Ez"),
+]
+
+
+@pytest.mark.parametrize("args,main", VACUUM_SCATTERED)
+def test_suite_t6_scattered_field_vanishes(args, main):
+    cfg = parse_args(args.split())
+    runs = [_run(cfg), _run(cfg, phase=math.pi / 2)]
+    comps = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"] if cfg.scheme == SCHEME_DIM3 else ["Ez", "Hx", "Hy"]
+    total = np.hypot(runs[0].field(main), runs[1].field(main))
+    assert total.max() > 0.9   # the wave fills the total-field box
+    for c in comps:
+        scat = np.hypot(runs[0].scattered(c), runs[1].scattered(c))
+        assert scat.max() <= 1e-14 and scat.min() == 0.0, (c, scat.max())
 
 
 def test_wave_number_closed_form_vs_newton():
