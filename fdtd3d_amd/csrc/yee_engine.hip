@@ -48,7 +48,9 @@ int fail (int code, const std::string &msg)
 template <typename T>
 bool half_tiles (StepArgs<T> &a, unsigned bx, unsigned by, dim3 &grid)
 {
-  a.tz0 = a.run_lo[2] & ~(int) (bx - 1);
+  // tiles start at a z multiple of their width; a box no wider than one tile starts it at the
+  // box (a 16-cell PML z slab row then fills the tile instead of half of an aligned one)
+  a.tz0 = (a.run_hi[2] - a.run_lo[2] <= (int) bx) ? a.run_lo[2] : (a.run_lo[2] & ~(int) (bx - 1));
   a.ntz = (int) ((a.run_hi[2] - a.tz0 + (int) bx - 1) / (int) bx);
   a.nty = (int) ((a.run_hi[1] - a.run_lo[1] + (int) by - 1) / (int) by);
   a.ntx = a.run_hi[0] - a.run_lo[0];
@@ -158,10 +160,9 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
   {
     const int64_t ext[3] = { a.run_hi[0] - a.run_lo[0], a.run_hi[1] - a.run_lo[1], a.run_hi[2] - a.run_lo[2] };
     if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) return YEE_OK;
-    // 64 z cells x 4 rows; boxes thin in z (PML z slabs) take 32 z cells x 8 rows, boxes one
-    // row high (TF/SF y faces) 64 x 1, so that few lanes idle.  Tiles start at a z multiple
-    // of their width (whole cache lines per row).
-    const unsigned bx = ext[2] <= 32 ? 32 : 64;
+    // 64 z cells x 4 rows; boxes thin in z (PML z slabs) take 16 or 32 z cells x 16 or 8 rows,
+    // boxes one row high (TF/SF y faces) 64 x 1, so that few lanes idle.
+    const unsigned bx = ext[2] <= 16 ? 16 : (ext[2] <= 32 ? 32 : 64);
     const unsigned by = (unsigned) std::min<int64_t> (256 / bx, std::max<int64_t> (1, ext[1]));
     dim3 block (bx, by, 1), grid;
     if (!half_tiles (a, bx, by, grid)) return fail (YEE_EINVAL, "half-step box too large for one launch");
