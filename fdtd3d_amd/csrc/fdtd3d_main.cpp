@@ -6,12 +6,14 @@
 // up as Scheme::initScheme / initGrids do (materials from DAT files and spheres), performs
 // --time-steps steps and prints the reference's "Total time = ... seconds" line
 // (Source/main.cpp:1175-1211).  Result dumps follow Scheme::saveGrids
-// (Source/Scheme/Scheme.inc.h:4367-4531) in the text format (TXTDumper,
-// Source/File-Management/Dumper/TXTDumper.h:50-130) under the reference's file names
+// (Source/Scheme/Scheme.inc.h:4367-4531) and the material dumps of Scheme::initGrids
+// (Scheme.inc.h:3166-3200) in the text format (TXTDumper,
+// Source/File-Management/Dumper/TXTDumper.h:50-130) and the binary format (DATDumper,
+// Source/File-Management/Dumper/DATDumper.h:28-86) under the reference's file names
 // (GridFileManager::setFileNames, Source/File-Management/Commons.h:64-122).  Options of the
 // reference outside this path are either accepted without effect (logging, NTFF and other
-// options that do not change the fields or the text dumps) or rejected with an error (other
-// schemes and physics, image and binary dumps).
+// options that do not change the fields or the dumps) or rejected with an error (other
+// schemes and physics, image dumps).
 //
 // Built twice: fdtd3d (FP64 fields, the reference's default value type) and fdtd3d-f32.
 #include <algorithm>
@@ -22,6 +24,7 @@
 #include <fstream>
 #include <set>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/yee.h"
@@ -42,13 +45,14 @@ struct Sphere
 // Result dump options (Settings.inc.h:199-217, 279-281).
 struct SaveOptions
 {
-  bool res = false, interm = false, txt = false, per_process = false, manual = false, no_pml = false;
+  bool res = false, interm = false, txt = false, dat = false, per_process = false, manual = false, no_pml = false;
+  bool materials = false;   // --save-materials
   bool einc = false, hinc = false;
   bool scat_res = false, scat_interm = false;   // --save-scattered-field-res / --save-scattered--field-interm
   int64_t interm_step = 100;
   int64_t start[3] = { 0, 0, 0 }, end[3] = { 0, 0, 0 };
   int orth = 2;   // orthogonal axis of the default 3D dump range (--orth-axis-x/y/z)
-  std::vector<std::string> refused;   // image / binary dumps asked for
+  std::vector<std::string> refused;   // image dumps asked for
 };
 
 struct Options
@@ -117,7 +121,7 @@ const std::set<std::string> kIgnoredWithArg = {
 };
 const std::set<std::string> kIgnoredFlags = {
   "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff",
-  "--save-materials", "--palette-gray", "--palette-rgb", "--dynamic-grid", "--check-disabling-cond",
+  "--palette-gray", "--palette-rgb", "--dynamic-grid", "--check-disabling-cond",
   "--check-enabling-cond", "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
   "--ntff-to-stdout",
 };
@@ -211,7 +215,9 @@ Options parse (int argc, char **argv)
     else if (a == "--orth-axis-z") o.save.orth = 2;
     else if (a == "--save-scattered-field-res") o.save.scat_res = true;
     else if (a == "--save-scattered--field-interm") o.save.scat_interm = true;
-    else if (a == "--save-as-bmp" || a == "--save-as-dat") o.save.refused.push_back (a);
+    else if (a == "--save-as-dat") o.save.dat = true;
+    else if (a == "--save-materials") o.save.materials = true;
+    else if (a == "--save-as-bmp") o.save.refused.push_back (a);
     else done = false;
     if (done) continue;
     for (int k = 0; k < 6 && !done; ++k)
@@ -259,14 +265,20 @@ void check (int rc)
 }
 
 // Raw double array [x][y][z] of the grid's size (DATLoader, Source/Loader/DATLoader.h:62-86).
+// The value type of the DAT files (FieldValue: double in the FP64 build, float in the FP32 one).
+using FileValue = std::conditional<YEE_CLI_PRECISION == 4, float, double>::type;
+
+// DATLoader::loadFromFile (Source/File-Management/Loader/DATLoader.h:55-90): N raw FieldValues in
+// [x][y][z] order.
 std::vector<double> load_dat (const std::string &path, int64_t n)
 {
   std::ifstream in (path, std::ios::binary);
   if (!in) die ("cannot open " + path);
-  std::vector<double> v ((size_t) n);
-  in.read (reinterpret_cast<char *> (v.data ()), (std::streamsize) (n * sizeof (double)));
-  if (in.gcount () != (std::streamsize) (n * sizeof (double))) die (path + ": expected " + std::to_string (n) + " doubles");
-  return v;
+  std::vector<FileValue> raw ((size_t) n);
+  in.read (reinterpret_cast<char *> (raw.data ()), (std::streamsize) (n * sizeof (FileValue)));
+  if (in.gcount () != (std::streamsize) (n * sizeof (FileValue)))
+    die (path + ": expected " + std::to_string (n) + " values of " + std::to_string (sizeof (FileValue)) + " bytes");
+  return std::vector<double> (raw.begin (), raw.end ());
 }
 
 // Minimum coordinate of a component in the E_CENTERED layout, doubled (YeeGridLayout.cpp:1937-2043):
@@ -278,7 +290,8 @@ int min_coord2 (int comp, int axis)
 }
 
 // Dump range [start, end) of a component (Scheme::getStartCoord / getEndCoord, Scheme.inc.h:4647-4790;
-// the middle plane along the orthogonal axis in 3D, SchemeHelper.h:860-990).
+// the middle plane along the orthogonal axis in 3D, SchemeHelper.h:860-990).  comp -1: a material
+// grid, whose minimum coordinate is taken as 0 there.
 void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int64_t start[3], int64_t end[3])
 {
   const SaveOptions &sv = o.save;
@@ -302,8 +315,9 @@ void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int
     for (int a = 0; a < dim; ++a)
     {
       const int64_t L = o.cfg.pml[a], R = size[a] - o.cfg.pml[a];
-      start[a] = (2 * L - min_coord2 (comp, a)) / 2 + 1;
-      end[a] = (2 * R - min_coord2 (comp, a)) / 2;
+      const int m2 = comp < 0 ? 0 : min_coord2 (comp, a);
+      start[a] = (2 * L - m2) / 2 + 1;
+      end[a] = (2 * R - m2) / 2;
     }
   }
   if (dim == 3)
@@ -313,18 +327,24 @@ void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int
   }
 }
 
-// One text dump, TXTDumper::writeToFile: "x [y [z]] value" per cell of [start, end) in [x][y][z]
-// order, values with 15 significant digits; file "previous-1_[timestep=T]_[pid=0]_[name=N].txt"
-// (the layer after the step, index 1 of the two stored layers).
-void write_txt (const std::string &name, int64_t t, int dim, const int64_t size[3], const int64_t start[3],
-                const int64_t end[3], const double *v)
+// One dump of layer `layer` at time step t, file "previous-<layer>_[timestep=T]_[pid=0]_[name=N].<ext>"
+// (GridFileManager::setFileNames; fields are dumped at layer 1, the layer after the step,
+// materials at layer 0), of the cells [start, end) in [x][y][z] order:
+//   text (TXTDumper::writeToFile): "x [y [z]] value" lines, values with 15 significant digits;
+//   binary (DATDumper::writeToFile): the raw FieldValues.  The reference names its binary field
+//   dumps from a list sized by the layer index and then reads the entry past its end
+//   (DATDumper.h:62, 129), so it writes no usable file; here the binary dump takes the name its
+//   text counterpart has, with the .dat extension.
+void write_dump (bool binary, const std::string &name, int layer, int64_t t, int dim, const int64_t size[3],
+                 const int64_t start[3], const int64_t end[3], const double *v)
 {
   int64_t total = 1;
   for (int a = 0; a < dim; ++a) total *= size[a];
-  std::printf ("Saving grid '%s' to text. Time step: 1. Size: %lld (from startCoord to endCoord). \n", name.c_str (),
-               (long long) total);
-  const std::string path = "previous-1_[timestep=" + std::to_string (t) + "]_[pid=0]_[name=" + name + "].txt";
-  FILE *f = std::fopen (path.c_str (), "w");
+  std::printf ("Saving grid '%s' to %s. Time step: %d. Size: %lld (from startCoord to endCoord). \n", name.c_str (),
+               binary ? "binary" : "text", layer, (long long) total);
+  const std::string path = "previous-" + std::to_string (layer) + "_[timestep=" + std::to_string (t) + "]_[pid=0]_[name="
+                           + name + "]" + (binary ? ".dat" : ".txt");
+  FILE *f = std::fopen (path.c_str (), binary ? "wb" : "w");
   if (!f) die ("cannot write " + path);
   const int64_t nz = dim == 3 ? size[2] : 1;
   for (int64_t x = start[0]; x < end[0]; ++x)
@@ -332,55 +352,106 @@ void write_txt (const std::string &name, int64_t t, int dim, const int64_t size[
       for (int64_t z = (dim == 3 ? start[2] : 0); z < (dim == 3 ? end[2] : 1); ++z)
       {
         const double val = v[(x * (dim >= 2 ? size[1] : 1) + y) * nz + z];
-        if (dim == 3) std::fprintf (f, "%lld %lld %lld %.15g\n", (long long) x, (long long) y, (long long) z, val);
+        if (binary)
+        {
+          const FileValue fv = (FileValue) val;
+          if (std::fwrite (&fv, sizeof fv, 1, f) != 1) die ("cannot write " + path);
+        }
+        else if (dim == 3) std::fprintf (f, "%lld %lld %lld %.15g\n", (long long) x, (long long) y, (long long) z, val);
         else if (dim == 2) std::fprintf (f, "%lld %lld %.15g\n", (long long) x, (long long) y, val);
         else std::fprintf (f, "%lld %.15g\n", (long long) x, val);
       }
-  std::fclose (f);
+  if (std::fclose (f) != 0) die ("cannot write " + path);
   std::printf ("Saved. \n");
 }
 
-// Scheme::saveGrids (time step t): every component of the scheme, then EInc / HInc if asked.
-// With scattered set, the components are first made scattered (Scheme::gatherFieldsTotal,
-// Scheme.inc.h:4335-4362); per-process dumps write the fields as they are.
+// The dump formats asked for, in the reference's order (FILE_TYPE_DAT before FILE_TYPE_TXT).
+std::vector<bool> dump_formats (const SaveOptions &sv)
+{
+  std::vector<bool> out;
+  if (sv.dat) out.push_back (true);
+  if (sv.txt) out.push_back (false);
+  return out;
+}
+
+// Material dumps of Scheme::initGrids with --save-materials (Scheme.inc.h:3166-3200): Eps, Mu and,
+// with metamaterials, OmegaPE, OmegaPM, GammaE, GammaM, at time step 0.
+void save_materials (const Options &o, yee_handle *h)
+{
+  const yee_config &c = o.cfg;
+  const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  const int64_t size[3] = { c.size[0], c.size[1], dim == 3 ? c.size[2] : 1 };
+  std::vector<double> v ((size_t) (size[0] * size[1] * size[2]));
+  static const char *names[6] = { "Eps", "Mu", "OmegaPE", "OmegaPM", "GammaE", "GammaM" };
+  const int count = (c.flags & YEE_USE_METAMATERIALS) ? 6 : 2;
+  for (bool binary : dump_formats (o.save))
+    for (int m = 0; m < count; ++m)
+    {
+      check (yee_get_material (h, m, v.data ()));
+      int64_t start[3], end[3];
+      dump_range (o, -1, dim, size, start, end);
+      for (int a = 0; a < dim; ++a)
+        if (start[a] < 0 || end[a] > size[a] || start[a] >= end[a]) die ("dump coordinates out of the grid");
+      write_dump (binary, names[m], 0, 0, dim, size, start, end, v.data ());
+    }
+}
+
+// Scheme::saveGrids (time step t): per format, every component of the scheme, then EInc / HInc
+// if asked.  With scattered set, the components are first made scattered
+// (Scheme::gatherFieldsTotal, Scheme.inc.h:4335-4362); per-process dumps write the fields as
+// they are.
 void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
 {
   const yee_config &c = o.cfg;
   const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
   const int64_t size[3] = { c.size[0], c.size[1], dim == 3 ? c.size[2] : 1 };
   const int64_t n = size[0] * size[1] * size[2];
-  std::vector<double> v ((size_t) n);
   static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
+  std::vector<std::vector<double>> fields (6);
   for (int comp = 0; comp < 6; ++comp)
   {
     if (dim == 2 && comp != YEE_EZ && comp != YEE_HX && comp != YEE_HY) continue;
-    if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, v.data ()));
-    else check (yee_get_field (h, comp, v.data ()));
-    int64_t start[3], end[3];
-    if (o.save.per_process)
-    {
-      for (int a = 0; a < 3; ++a)
-      {
-        start[a] = 0;
-        end[a] = size[a];
-      }
-    }
-    else
-    {
-      dump_range (o, comp, dim, size, start, end);
-    }
-    for (int a = 0; a < dim; ++a)
-      if (start[a] < 0 || end[a] > size[a] || start[a] >= end[a]) die ("dump coordinates out of the grid");
-    write_txt (names[comp], t, dim, size, start, end, v.data ());
+    fields[comp].resize ((size_t) n);
+    if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, fields[comp].data ()));
+    else check (yee_get_field (h, comp, fields[comp].data ()));
   }
-  if ((o.save.einc || o.save.hinc) && (c.flags & YEE_USE_TFSF))
+  std::vector<double> e, hv;
+  const int64_t len = 500 * c.size[0];
+  const bool inc = (o.save.einc || o.save.hinc) && (c.flags & YEE_USE_TFSF);
+  if (inc)
   {
-    const int64_t len = 500 * c.size[0];
-    std::vector<double> e ((size_t) len), hv ((size_t) len);
+    e.resize ((size_t) len);
+    hv.resize ((size_t) len);
     check (yee_get_incident (h, e.data (), hv.data (), len));
-    const int64_t s1[3] = { len, 1, 1 }, b[3] = { 0, 0, 0 };
-    if (o.save.einc) write_txt ("EInc", t, 1, s1, b, s1, e.data ());
-    if (o.save.hinc) write_txt ("HInc", t, 1, s1, b, s1, hv.data ());
+  }
+  for (bool binary : dump_formats (o.save))
+  {
+    for (int comp = 0; comp < 6; ++comp)
+    {
+      if (fields[comp].empty ()) continue;
+      int64_t start[3], end[3];
+      if (o.save.per_process)
+      {
+        for (int a = 0; a < 3; ++a)
+        {
+          start[a] = 0;
+          end[a] = size[a];
+        }
+      }
+      else
+      {
+        dump_range (o, comp, dim, size, start, end);
+      }
+      for (int a = 0; a < dim; ++a)
+        if (start[a] < 0 || end[a] > size[a] || start[a] >= end[a]) die ("dump coordinates out of the grid");
+      write_dump (binary, names[comp], 1, t, dim, size, start, end, fields[comp].data ());
+    }
+    if (inc)
+    {
+      const int64_t s1[3] = { len, 1, 1 }, b[3] = { 0, 0, 0 };
+      if (o.save.einc) write_dump (binary, "EInc", 1, t, 1, s1, b, s1, e.data ());
+      if (o.save.hinc) write_dump (binary, "HInc", 1, t, 1, s1, b, s1, hv.data ());
+    }
   }
 }
 
@@ -428,9 +499,12 @@ int main (int argc, char **argv)
   check (yee_prepare (h));
 
   const SaveOptions &sv = o.save;
-  const bool dumping = (sv.res || sv.interm) && sv.txt;
+  if (sv.materials && !sv.refused.empty ())
+    die ("option " + sv.refused[0] + " is not supported (text and binary dumps: --save-as-txt, --save-as-dat)");
+  if (sv.materials) save_materials (o, h);
+  const bool dumping = (sv.res || sv.interm) && (sv.txt || sv.dat);
   if ((sv.res || sv.interm) && !sv.refused.empty ())
-    die ("option " + sv.refused[0] + " is not supported (text dumps only: --save-as-txt)");
+    die ("option " + sv.refused[0] + " is not supported (text and binary dumps: --save-as-txt, --save-as-dat)");
   if (sv.interm && sv.interm_step <= 0) die ("--interm-save-step must be positive");
   // Scheme.h:195-204 asserts that per-process result dumps are not scattered; the incident
   // wave that is subtracted exists only with TF/SF

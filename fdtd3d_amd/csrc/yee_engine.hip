@@ -1017,6 +1017,33 @@ int yee_get_field (yee_handle *h, int32_t comp, double *out)
   return YEE_OK;
 }
 
+int yee_get_material (yee_handle *h, int32_t which, double *out)
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
+  const size_t n = (size_t) h->ncell;
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  if (!h->mat[which])
+  {
+    // a uniform grid is kept as a scalar (eps / mu) or not at all (Drude terms, all zero)
+    const double u = which < 2 ? h->mat_u[which] : 0.0;
+    const double v = h->fp32 () ? (double) (float) u : u;
+    std::fill (out, out + n, v);
+    return YEE_OK;
+  }
+  if (h->fp32 ())
+  {
+    std::vector<float> tmp (n);
+    HIP_TRY (hipMemcpy (tmp.data (), h->mat[which], n * 4, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) out[q] = tmp[q];
+  }
+  else
+  {
+    HIP_TRY (hipMemcpy (out, h->mat[which], n * 8, hipMemcpyDeviceToHost));
+  }
+  return YEE_OK;
+}
+
 }  // extern "C"
 
 namespace {

@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "libyee.so")
 
 EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
-    "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field",
+    "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
@@ -74,6 +74,7 @@ def load_library():
     L.yee_get_field.argtypes = [vp, i32, vp]
     L.yee_get_field_raw.argtypes = [vp, i32, vp]
     L.yee_get_scattered_field.argtypes = [vp, i32, vp]
+    L.yee_get_material.argtypes = [vp, i32, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.yee_get_scalars.argtypes = [vp, vp]
@@ -236,6 +237,12 @@ class Engine:
         """Scattered field: the newest field minus the incident wave inside the TF/SF box."""
         out = np.empty(self.shape, dtype=np.float64)
         _check(load_library().yee_get_scattered_field(self.h, COMP_INDEX[name], out.ctypes.data))
+        return out
+
+    def material(self, name: str) -> np.ndarray:
+        """Material grid as the engine holds it (relative eps / mu; omega_p, gamma in rad/s)."""
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(load_library().yee_get_material(self.h, MAT_INDEX[name], out.ctypes.data))
         return out
 
     def field_raw(self, name: str) -> np.ndarray:

@@ -179,6 +179,12 @@ int yee_get_field (yee_handle *h, int32_t comp, double *out);
 /* Same as yee_get_field, in the handle's own precision (float* when precision == 4). */
 int yee_get_field_raw (yee_handle *h, int32_t comp, void *out);
 
+/* Material grid `which` (yee_material) as FP64 [x][y][z], as the engine holds it: the values given
+ * to yee_set_material* (relative eps / mu, omega_p and gamma in rad/s) in the handle's precision;
+ * a grid never set reads as its default (1 for eps / mu, 0 for the Drude terms).  Replaces the
+ * material grids Scheme::initGrids dumps with --save-materials (Scheme.inc.h:3166-3200). */
+int yee_get_material (yee_handle *h, int32_t which, double *out);
+
 /* Scattered field of component comp (EX..HZ) as FP64 [x][y][z]: the newest field minus the
  * incident plane wave at the component's grid points inside the TF/SF box, the total field
  * outside it.  YEE_EINVAL without TF/SF.  Replaces Scheme::makeGridScattered (Scheme.inc.h:4113-4228),

@@ -177,3 +177,73 @@ def test_cli_scattered_field_dumps(engine_lib, tmp_path):
                                   "--save-scattered-field-res").split(), capture_output=True, text=True,
                          timeout=120, cwd=tmp_path)
     assert out.returncode != 0 and "--use-tfsf" in out.stderr
+
+
+def _write_dat(path, arr, dtype):
+    np.ascontiguousarray(arr, dtype=dtype).tofile(path)
+
+
+@pytest.mark.parametrize("binary,dtype,precision", [("fdtd3d", np.float64, 8), ("fdtd3d-f32", np.float32, 4)])
+def test_cli_binary_and_material_dumps(binary, dtype, precision, engine_lib, tmp_path):
+    """--save-as-dat writes the raw FieldValues of the dump range (the FP64 build doubles, the FP32
+    build floats, as DATLoader reads them back); --save-materials dumps Eps and Mu at time step 0
+    (Scheme::initGrids); a material dumped in binary loads back unchanged."""
+    from fdtd3d_amd.config import parse_args
+    exe = os.path.join(BIN, binary)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    shape = (20, 18, 22)
+    base = ("--3d --size x:20,y:18,z:22 --use-pml --pml-size x:3 --same-size-pml --use-tfsf "
+            "--tfsf-size-left x:6 --tfsf-size-right x:6 --same-size-tfsf --angle-teta 30 --angle-phi 45 "
+            "--angle-psi 90 --time-steps 30")
+    rng = np.random.default_rng(7)
+    eps = rng.uniform(1.0, 6.0, size=shape).astype(dtype)
+    _write_dat(tmp_path / "eps.dat", eps, dtype)
+    full = " --manual-save-coords --save-start-coord x:0,y:0,z:0 --save-end-coord x:20,y:18,z:22"
+    run = (base + " --load-eps-from-file eps.dat --save-materials --save-res --save-as-dat --save-as-txt" + full)
+    say(f"[gpu] cli binary dumps {binary}")
+    out = subprocess.run([exe] + run.split(), capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    assert "Saving grid 'Eps' to binary. Time step: 0. Size: 7920 (from startCoord to endCoord). " in out.stdout
+    assert "Saving grid 'Ex' to binary. Time step: 1. Size: 7920 (from startCoord to endCoord). " in out.stdout
+    # binary before text, as the reference's dumper types are ordered
+    assert out.stdout.index("'Ex' to binary") < out.stdout.index("'Ex' to text")
+    m_eps = np.fromfile(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Eps].dat", dtype=dtype)
+    np.testing.assert_array_equal(m_eps, eps.ravel())
+    m_mu = np.fromfile(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Mu].dat", dtype=dtype)
+    np.testing.assert_array_equal(m_mu, np.ones(eps.size, dtype))
+    _, t_eps = _read_txt(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Eps].txt")
+    assert np.allclose(t_eps, eps.ravel().astype(np.float64), rtol=1e-14, atol=0)
+
+    cfg = parse_args(base.split())
+    cfg.precision = precision
+    eng = Engine(cfg)
+    eng.set_material("eps", eps.astype(np.float64))
+    eng.prepare()
+    eng.step(30)
+    eng.sync()
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        got = np.fromfile(tmp_path / f"previous-1_[timestep=30]_[pid=0]_[name={name}].dat", dtype=dtype)
+        np.testing.assert_array_equal(got, eng.field_raw(name).ravel(), err_msg=name)
+    np.testing.assert_array_equal(eng.material("eps"), eps.astype(np.float64))
+    eng.close()
+
+    # the binary material dump loads back as the same material
+    say(f"[gpu] cli binary round trip {binary}")
+    again = tmp_path / "again"
+    again.mkdir()
+    os.replace(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Eps].dat", again / "eps_back.dat")
+    out = subprocess.run([exe] + (base + " --load-eps-from-file eps_back.dat --save-res --save-as-dat" + full).split(),
+                         capture_output=True, text=True, timeout=120, cwd=again)
+    assert out.returncode == 0, out.stderr
+    for name in ("Ex", "Hz"):
+        a = np.fromfile(again / f"previous-1_[timestep=30]_[pid=0]_[name={name}].dat", dtype=dtype)
+        b = np.fromfile(tmp_path / f"previous-1_[timestep=30]_[pid=0]_[name={name}].dat", dtype=dtype)
+        np.testing.assert_array_equal(a, b)
+
+    # a file too short for the grid is refused
+    other = np.float32 if dtype == np.float64 else np.float64
+    _write_dat(again / "short.dat", eps[:5], other)
+    out = subprocess.run([exe] + (base + " --load-eps-from-file short.dat").split(), capture_output=True, text=True,
+                         timeout=120, cwd=again)
+    assert out.returncode != 0 and "short.dat" in out.stderr
