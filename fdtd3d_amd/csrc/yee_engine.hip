@@ -533,7 +533,9 @@ int validate (const yee_config *c)
   for (int a = 0; a < dim; ++a)
   {
     if (c->size[a] < 3 || c->size[a] > (1 << 20)) return fail (YEE_EINVAL, "size out of range");
-    if (c->pml[a] < 0 || 2 * c->pml[a] > c->size[a]) return fail (YEE_EINVAL, "pml size out of range");
+    // the PML sizes matter only with PML on (a 3-cell grid keeps the default size 2 otherwise)
+    if ((c->flags & YEE_USE_PML) && (c->pml[a] < 0 || 2 * c->pml[a] > c->size[a]))
+      return fail (YEE_EINVAL, "pml size out of range");
     if ((c->flags & YEE_USE_TFSF)
         && (c->tfsf_left[a] < 1 || c->tfsf_right[a] < 1 || c->tfsf_left[a] + c->tfsf_right[a] >= c->size[a]))
       return fail (YEE_EINVAL, "TF/SF box out of range");

@@ -83,3 +83,34 @@ def test_cli_rejects_options_outside_the_path():
         out = subprocess.run([exe] + argv, capture_output=True, text=True, timeout=60)
         assert out.returncode != 0, argv
         assert out.stderr.startswith("fdtd3d: "), out.stderr
+
+
+INVALID_CONFIGS = [
+    (dict(precision=2), b"precision"),
+    (dict(layout=1), b"E_CENTERED"),
+    (dict(size=[2, 10, 10]), b"size"),
+    (dict(size=[10, 10, 1 << 21]), b"size"),
+    (dict(size=[8, 8, 8], pml=[5, 1, 1], use_pml=True), b"pml"),
+    (dict(size=[10, 10, 10], pml=[-1, 1, 1], use_pml=True), b"pml"),
+    (dict(size=[10, 10, 10], use_tfsf=True, tfsf_left=[0, 3, 3], tfsf_right=[3, 3, 3]), b"TF/SF"),
+    (dict(size=[10, 10, 10], use_tfsf=True, tfsf_left=[5, 3, 3], tfsf_right=[5, 3, 3]), b"TF/SF"),
+    (dict(size=[4096, 4096, 2048]), b"too large"),
+    (dict(size=[10, 10, 10], dx=0.0), b"positive"),
+    (dict(size=[10, 10, 10], courant=-0.5), b"positive"),
+    (dict(size=[10, 10, 10], use_metamaterials=True), b"PML"),
+    (dict(size=[10, 10, 10], time_steps=-1), b"time steps"),
+]
+
+
+@pytest.mark.parametrize("kw,msg", INVALID_CONFIGS)
+def test_create_rejects_invalid_configs(kw, msg):
+    """yee_create validates the configuration before touching the device: every invalid
+    configuration returns YEE_EINVAL with a message naming the problem."""
+    import ctypes
+    from fdtd3d_amd.config import Config
+    L = load_library()
+    h = ctypes.c_void_p()
+    c = Config(**kw).to_c()
+    assert L.yee_create(ctypes.byref(c), ctypes.byref(h)) == -1
+    assert msg in L.yee_last_error(), L.yee_last_error()
+    assert not h.value

@@ -1,0 +1,133 @@
+"""Edge cases of the engine against the oracle (needs the GPU): grids whose rows are longer
+than one fused workgroup (N_z > 1024: in-place half steps over the whole grid), the smallest
+grids the reference accepts, a grid that is PML everywhere (no fused box), the t = 0 rule
+(no field update, the point source still fires), step counts split over several calls, and
+a point source off the grid (ignored, as getFieldValueOrNullByAbsolutePos ignores it).
+
+Bars as in the parity tests: FP64 bit-identical without PML and relative L2 <= 1e-12 with it;
+FP32 relative L2 of the E and H vectors <= 1e-5 against the FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from fdtd3d_amd.config import parse_args
+from fdtd3d_amd.engine import Engine, run_engine
+from oracle.binding import run_oracle
+from tests.helpers import field_names, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+TOL_FP32 = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _vec_rel(got, want):
+    num = sum(float(np.sum((g - w) ** 2)) for g, w in zip(got, want))
+    den = sum(float(np.sum(w ** 2)) for w in want)
+    return np.sqrt(num / den) if den > 0 else np.sqrt(num)
+
+
+def _check(cfg, eng, o, exact):
+    if cfg.precision == 4:
+        # the E and H vectors (a component can hold only FP64 rounding residue, ~1e-19)
+        for group in ("E", "H"):
+            ns = [n for n in field_names(cfg) if n[0] == group]
+            r = _vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
+            assert r <= TOL_FP32, (group, r)
+        return
+    for f in field_names(cfg):
+        got, want = eng.field(f), o.field(f)
+        if exact:
+            assert np.array_equal(got, want), (f, float(np.abs(got - want).max()))
+        else:
+            assert rel_l2(got, want) <= TOL_FP64, (f, rel_l2(got, want))
+
+
+CASES = [
+    # rows longer than 1024 cells: no fused kernel, in-place half steps; PML + oblique TF/SF
+    ("long_rows", "--3d --size x:8,y:7,z:1100 --use-pml --pml-size x:1,y:1,z:10 --use-tfsf "
+     "--tfsf-size-left x:3,y:3,z:20 --tfsf-size-right x:3,y:3,z:20 --angle-teta 20 --angle-phi 35 "
+     "--angle-psi 60 --time-steps 40"),
+    # N_z = 1025 with a point source and eps blocks
+    ("rows_1025", "--3d --size x:6,y:6,z:1025 --point-source x:3,y:2,z:1000 --point-source-ex "
+     "--point-source-hy --time-steps 30"),
+    # the smallest grid: one updated cell
+    ("min_3d", "--3d --size x:3,y:3,z:3 --point-source x:1,y:1,z:1 --point-source-ez --point-source-hx --time-steps 9"),
+    ("min_tmz", "--2d-tmz --size x:3,y:3 --point-source x:1,y:1 --point-source-ez --time-steps 9"),
+    ("small_uneven", "--3d --size x:4,y:3,z:5 --point-source x:2,y:1,z:3 --point-source-ey --time-steps 12"),
+    # PML everywhere: no PML-free box for the fused kernel
+    ("all_pml", "--3d --size x:8,y:8,z:8 --use-pml --pml-size x:4,y:4,z:4 --point-source x:4,y:3,z:4 "
+     "--point-source-ez --time-steps 25"),
+    ("all_pml_tmz", "--2d-tmz --size x:10,y:8 --use-pml --pml-size x:5,y:4 --point-source x:5,y:4 "
+     "--point-source-ez --time-steps 25"),
+    # TF/SF box one cell wide inside a thin PML-free box
+    ("thin_tfsf", "--3d --size x:14,y:13,z:15 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf "
+     "--tfsf-size-left x:6,y:6,z:7 --tfsf-size-right x:7,y:6,z:7 --angle-teta 40 --angle-phi 20 "
+     "--angle-psi 80 --time-steps 30"),
+    # t = 0 performs no field update; the point source still fires
+    ("one_step", "--3d --size x:10,y:9,z:11 --use-pml --pml-size x:2,y:2,z:2 --point-source x:5,y:4,z:5 "
+     "--point-source-ez --point-source-hz --time-steps 1"),
+    ("zero_steps", "--3d --size x:10,y:9,z:11 --use-tfsf --tfsf-size-left x:3,y:3,z:3 "
+     "--tfsf-size-right x:3,y:3,z:3 --time-steps 0"),
+    # a point source off the grid is ignored
+    ("source_off_grid", "--3d --size x:12,y:10,z:14 --use-pml --pml-size x:2,y:2,z:2 --point-source x:30,y:4,z:5 "
+     "--point-source-ez --time-steps 15"),
+]
+
+
+def _materials(name, cfg):
+    if name != "rows_1025":
+        return {}
+    rng = np.random.default_rng(11)
+    nb = [(s + 7) // 8 for s in cfg.shape]
+    eps = np.kron(rng.uniform(1.0, 5.0, size=nb), np.ones((8, 8, 8)))[: cfg.shape[0], : cfg.shape[1], : cfg.shape[2]]
+    return {"eps": eps}
+
+
+@pytest.mark.parametrize("name,args", CASES)
+@pytest.mark.parametrize("precision", [8, 4])
+def test_edge_case_vs_oracle(name, args, precision, engine_lib):
+    cfg = parse_args(args.split())
+    mats = _materials(name, cfg)
+    o = run_oracle(cfg, mats)
+    cfg.precision = precision
+    say(f"[gpu] edge {name} precision {precision}")
+    eng = run_engine(cfg, mats)
+    _check(cfg, eng, o, exact=not cfg.use_pml)
+    if name == "zero_steps":
+        for f in field_names(cfg):
+            assert not np.any(eng.field(f)), f
+    if name == "one_step":
+        ez = eng.field("Ez")
+        assert np.count_nonzero(ez) == 1 and ez[5, 4, 5] != 0
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_steps_split_over_calls(precision, engine_lib):
+    """yee_step (n) in several calls equals one call of the total (the step counter, the
+    double-buffer parity and the incident grid continue across calls)."""
+    cfg = parse_args("--3d --size x:40,y:36,z:44 --use-pml --pml-size x:5,y:4,z:6 --use-tfsf --tfsf-size-left x:9 "
+                     "--tfsf-size-right x:8 --same-size-tfsf --angle-teta 30 --angle-phi 45 --angle-psi 90 "
+                     "--point-source x:20,y:18,z:22 --point-source-hy --time-steps 41".split())
+    cfg.precision = precision
+    say(f"[gpu] split step calls precision {precision}")
+    a = run_engine(cfg)
+    b = Engine(cfg)
+    b.prepare()
+    for n in (1, 6, 13, 0, 21):
+        b.step(n)
+    b.sync()
+    for f in field_names(cfg, aux=True):
+        assert np.array_equal(a.field_raw(f), b.field_raw(f)), f
+    ea, ha = a.incident(1000)
+    eb, hb = b.incident(1000)
+    assert np.array_equal(ea, eb) and np.array_equal(ha, hb)
+    a.close()
+    b.close()
