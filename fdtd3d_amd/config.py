@@ -162,7 +162,7 @@ _IGNORED_WITH_ARG = {
     "--current-source", "--save-cmd-to-file",
 }
 _IGNORED_FLAGS = {
-    "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff",
+    "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
     "--save-res", "--save-res-per-process", "--save-materials", "--save-interm-res",
     "--save-scattered-field-res", "--save-scattered--field-interm", "--save-as-bmp", "--save-as-dat",
     "--save-as-txt", "--save-tfsf-e-incident", "--save-tfsf-h-incident", "--manual-save-coords",
@@ -178,6 +178,8 @@ _UNSUPPORTED = {
     "--use-double-material-precision", "--eps-normed", "--mu-normed",
     "--current-source-jx", "--current-source-jy", "--current-source-jz",
     "--current-source-mx", "--current-source-my", "--current-source-mz",
+    # the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
+    "--use-ntff",
 }
 
 

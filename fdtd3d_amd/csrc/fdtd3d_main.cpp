@@ -120,7 +120,7 @@ const std::set<std::string> kIgnoredWithArg = {
   "--interm-ntff-step", "--rebalance-step", "--stored-steps", "--current-source", "--save-cmd-to-file",
 };
 const std::set<std::string> kIgnoredFlags = {
-  "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda", "--use-ntff",
+  "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
   "--palette-gray", "--palette-rgb", "--dynamic-grid", "--check-disabling-cond",
   "--check-enabling-cond", "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
   "--ntff-to-stdout",
@@ -130,6 +130,8 @@ const std::set<std::string> kUnsupported = {
   "--2d-tez", "--2d-tmx", "--2d-tmy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
   "--current-source-jx", "--current-source-jy", "--current-source-jz", "--current-source-mx",
   "--current-source-my", "--current-source-mz",
+  // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
+  "--use-ntff",
 };
 
 Options parse (int argc, char **argv)

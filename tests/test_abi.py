@@ -65,6 +65,8 @@ def test_parse_rejects_unknown_and_unsupported():
         parse_args(["--1d-exhy"])
     with pytest.raises(OptionError):
         parse_args(["--3d", "--use-metamaterials"])
+    with pytest.raises(OptionError):
+        parse_args(["--3d", "--use-ntff"])   # the far field needs complex fields
 
 
 def test_cmd_file(tmp_path):
@@ -79,7 +81,7 @@ def test_cli_rejects_options_outside_the_path():
     if not os.path.exists(exe):
         pytest.skip(f"{exe} not built")
     import subprocess
-    for argv in (["--1d-exhy"], ["--no-such-flag"], ["--size"]):
+    for argv in (["--1d-exhy"], ["--no-such-flag"], ["--size"], ["--use-ntff"]):
         out = subprocess.run([exe] + argv, capture_output=True, text=True, timeout=60)
         assert out.returncode != 0, argv
         assert out.stderr.startswith("fdtd3d: "), out.stderr
