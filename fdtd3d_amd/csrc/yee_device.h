@@ -308,6 +308,17 @@ __device__ __forceinline__ void tfsf_apply (const StepArgs<T> &a, int i, int j, 
   tfsf_pair<T, COMP, 1, DIM> (a, i, j, k, v[2], v[3]);
 }
 
+// Inside the Drude box: the field cells whose 2- or 4-cell averages reach a nonzero Drude cell.
+template <typename T>
+__device__ __forceinline__ bool in_drude_box (const StepArgs<T> &a, const int p[3], int dim)
+{
+  bool in = true;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    if (ax < dim) in &= (p[ax] >= a.box_lo[ax]) & (p[ax] < a.box_hi[ax]);
+  return in;
+}
+
 template <typename T>
 __device__ __forceinline__ bool is_special (const StepArgs<T> &a, const int p[3], int dim, int mode)
 {
@@ -316,14 +327,7 @@ __device__ __forceinline__ bool is_special (const StepArgs<T> &a, const int p[3]
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax)
     if (ax < dim) s |= (p[ax] < a.slab_lo[ax]) | (p[ax] >= a.slab_hi[ax]);
-  if (mode == 2)
-  {
-    bool in = true;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
-      if (ax < dim) in &= (p[ax] >= a.box_lo[ax]) & (p[ax] < a.box_hi[ax]);
-    s |= in;
-  }
+  if (mode == 2) s |= in_drude_box (a, p, dim);
   return s;
 }
 

@@ -166,7 +166,16 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
     const unsigned by = (unsigned) std::min<int64_t> (256 / bx, std::max<int64_t> (1, ext[1]));
     dim3 block (bx, by, 1), grid;
     if (!half_tiles (a, bx, by, grid)) return fail (YEE_EINVAL, "half-step box too large for one launch");
-    switch (h->mode * 2 + (e_half ? 0 : 1))
+    // a box that misses the Drude box takes the PML kernels, which every such cell of a Drude
+    // run takes anyway (e_half_3d)
+    int mode = h->mode;
+    if (mode == 2)
+    {
+      bool meets = true;
+      for (int ax = 0; ax < 3; ++ax) meets = meets && a.run_lo[ax] < a.box_hi[ax] && a.box_lo[ax] < a.run_hi[ax];
+      if (!meets) mode = 1;
+    }
+    switch (mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
       case 1: hipLaunchKernelGGL ((h_half_3d<T, 0>), grid, block, 0, h->stream, a); break;

@@ -152,10 +152,8 @@ __device__ __forceinline__ void load_old (const StepArgs<T> &a, int64_t q, bool 
 // updating in place).
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
+__device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j, int k)
 {
-  int i, j, k;
-  if (!half_cell (a, i, j, k)) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
@@ -213,10 +211,8 @@ __global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
 // which the host points at the new E when the state is double-buffered.
 // ---------------------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
+__device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j, int k)
 {
-  int i, j, k;
-  if (!half_cell (a, i, j, k)) return;
   const int64_t sx = a.sx;
   const int sy = a.sy;
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
@@ -267,6 +263,32 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
   update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), hx_m, special, oxm, gxm);
   update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), hy_m, special, oym, gym);
   update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl (vz), hz_m, special, ozm, gzm);
+}
+
+// Drude runs (MODE 2): a cell outside the Drude box has omega_p = gamma = 0 in every average,
+// where the reference's ADE recursion (b0 + b1 + b2 = 0, a1 = -2, a2 = 1) keeps
+// D1 = D / (M eps) exactly in real arithmetic, so its D1 -> field stage equals the PML D -> field
+// stage: such a PML cell takes the PML update (MODE 1: D / B state only), as the cells outside
+// the PML take the direct one.  Equal to the reference up to rounding, and without the ADE's
+// cancellation (whose FP32 rounding the recursion amplifies).
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void e_half_3d (StepArgs<T> a)
+{
+  int i, j, k;
+  if (!half_cell (a, i, j, k)) return;
+  const int p[3] = { i, j, k };
+  if (MODE == 2 && !in_drude_box (a, p, 3)) e_half_cell<T, 1> (a, i, j, k);
+  else e_half_cell<T, MODE> (a, i, j, k);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
+{
+  int i, j, k;
+  if (!half_cell (a, i, j, k)) return;
+  const int p[3] = { i, j, k };
+  if (MODE == 2 && !in_drude_box (a, p, 3)) h_half_cell<T, 1> (a, i, j, k);
+  else h_half_cell<T, MODE> (a, i, j, k);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -948,7 +970,10 @@ __global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
-  tmz_e_cell<T, MODE> (a, i, j);
+  const int p[3] = { i, j, 0 };
+  // outside the Drude box: the PML update (see e_half_3d)
+  if (MODE == 2 && !in_drude_box (a, p, 2)) tmz_e_cell<T, 1> (a, i, j);
+  else tmz_e_cell<T, MODE> (a, i, j);
 }
 
 template <typename T, int MODE>
@@ -957,7 +982,9 @@ __global__ __launch_bounds__ (256) void h_half_tmz (StepArgs<T> a)
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
-  tmz_h_cell<T, MODE> (a, i, j);
+  const int p[3] = { i, j, 0 };
+  if (MODE == 2 && !in_drude_box (a, p, 2)) tmz_h_cell<T, 1> (a, i, j);
+  else tmz_h_cell<T, MODE> (a, i, j);
 }
 
 // ---------------------------------------------------------------------------------------
