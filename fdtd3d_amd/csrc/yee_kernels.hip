@@ -70,6 +70,38 @@ __device__ __forceinline__ T cell_value_pc (const StepArgs<T> &a, int64_t q, T f
     return f_old * (T) 1.0 + tmp * cb;
   }
   const T cac = pc.cac, cbc = pc.cbc, p4 = pc.p4, m4 = pc.m4, ca5 = pc.ca5, dd5 = pc.dd5;
+  if constexpr (MODE == 2 && sizeof (T) == 4)
+  {
+    // FP32 Drude cells: the same recursion carried in increments.  The reference's form
+    // b0 D + b1 D' + b2 D'' (b0 + b1 + b2 = 0) is a second difference of D, which FP32 values
+    // of D (rounded to 6e-8 of D, while the difference is ~(w dt)^2 D) cannot resolve, and the
+    // ADE's roots near 1 accumulate that error.  With dD = D - D', dD1 = D1 - D1' the algebra
+    // gives (Drude ADE coefficients as below, c = a1 + a2 + 1 = 4 M (dt w_p)^2 / A):
+    //   dD_new  = D (cac - 1) + tmp cbc,       D_new  = D + dD_new
+    //   dD1_new = b0 dD_new - b2 dD - c D1 + a2 dD1,   D1_new = D1 + dD1_new
+    //   f_new   = f ca5 + dD1_new p4 / dd5 + D1 (p4 - m4) / dd5
+    // equal to the reference's expressions in real arithmetic.  dp / d1p hold dD / dD1 here
+    // (values one step older in the FP64 state).  FP64 keeps the reference's operations.
+    const T dn = d_old * (cac - (T) 1) + tmp * cbc;
+    const T dd_old = old.dp, m_old = old.m, dm_old = old.mp;
+    const T M = modc, dt = a.dt;
+    const T wdt2 = (dt * om) * (dt * om);
+    const T A = (T) 4 * M * mat + (T) 2 * dt * M * mat * ga + M * wdt2;
+    const T b0 = ((T) 4 + (T) 2 * dt * ga) / A;
+    const T b2 = ((T) 4 - (T) 2 * dt * ga) / A;
+    const T a2 = ((T) 4 * M * mat - (T) 2 * dt * M * mat * ga + M * wdt2) / A;
+    const T c = (T) 4 * M * wdt2 / A;
+    const T dm = dn * b0 - dd_old * b2 - m_old * c + dm_old * a2;
+    const T f_new = f_old * ca5 + dm * (p4 / dd5) + m_old * ((p4 - m4) / dd5);
+    if (store)
+    {
+      gst (a.dp_n[COMP] + q, dn);
+      gst (a.d1_n[COMP] + q, m_old + dm);
+      gst (a.d1p_n[COMP] + q, dm);
+      gst (a.d_n[COMP] + q, d_old + dn);
+    }
+    return f_new;
+  }
   // curl update of D/B with the PML Ca/Cb (computeCaCb, PML branch)
   const T d_new = d_old * cac + tmp * cbc;
   T f_new;

@@ -266,3 +266,29 @@ def test_fused_step_with_drude_box_matches_half_steps(precision, engine_lib, mon
         assert np.array_equal(fa, fb), f"{f}: max diff {np.max(np.abs(fa.astype(np.float64) - fb))}"
     a.close()
     b.close()
+
+
+def test_fp32_drude_resonance_long_run(engine_lib):
+    """A resonant Drude cube (omega_p = 1.5 omega, gamma = 0.01 omega) driven for 500 steps: the
+    FP32 run stays within the FP32 bar of the FP64 oracle (the FP32 Drude cells carry the ADE in
+    increments; in the reference's form the second difference of FP32 D drifts by ~2e-5 a step)."""
+    cfg = parse_args("--3d --size x:24,y:24,z:24 --use-pml --pml-size x:5,y:5,z:5 --use-tfsf --tfsf-size-left x:7 "
+                     "--tfsf-size-right x:7 --same-size-tfsf --angle-teta 0 --angle-phi 0 --angle-psi 0 "
+                     "--use-metamaterials --time-steps 500".split())
+    w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
+    om = np.zeros(cfg.shape)
+    ga = np.zeros(cfg.shape)
+    om[9:15, 9:15, 9:15] = 1.5 * w0
+    ga[9:15, 9:15, 9:15] = 0.01 * w0
+    mats = {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
+    o = run_oracle(cfg, mats)
+    cfg.precision = 4
+    say("[gpu] fp32 drude resonance 500 steps")
+    eng = run_engine(cfg, mats)
+    names = field_names(cfg)
+    for group in ("E", "H"):
+        ns = [n for n in names if n[0] == group]
+        r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
+        say(f"[gpu] {group}: {r:.2e}")
+        assert r <= TOL_FP32, (group, r)
+    eng.close()
