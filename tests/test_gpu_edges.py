@@ -75,6 +75,9 @@ CASES = [
      "--point-source-ez --point-source-hz --time-steps 1"),
     ("zero_steps", "--3d --size x:10,y:9,z:11 --use-tfsf --tfsf-size-left x:3,y:3,z:3 "
      "--tfsf-size-right x:3,y:3,z:3 --time-steps 0"),
+    # TMz with a resonant Drude square inside PML + TF/SF (the single-run TMz half-step kernels, MODE 2)
+    ("tmz_drude", "--2d-tmz --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:9,y:8 "
+     "--tfsf-size-right x:9,y:8 --angle-teta 90 --angle-phi 30 --angle-psi 90 --use-metamaterials --time-steps 300"),
     # a point source off the grid is ignored
     ("source_off_grid", "--3d --size x:12,y:10,z:14 --use-pml --pml-size x:2,y:2,z:2 --point-source x:30,y:4,z:5 "
      "--point-source-ez --time-steps 15"),
@@ -82,6 +85,13 @@ CASES = [
 
 
 def _materials(name, cfg):
+    if name == "tmz_drude":
+        w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
+        om = np.zeros(cfg.shape)
+        ga = np.zeros(cfg.shape)
+        om[15:25, 13:23] = 1.5 * w0
+        ga[15:25, 13:23] = 0.01 * w0
+        return {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
     if name != "rows_1025":
         return {}
     rng = np.random.default_rng(11)
