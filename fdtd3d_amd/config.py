@@ -16,6 +16,7 @@ import struct
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
+SCHEME_TEZ = 9
 SCHEME_TMZ = 12
 SCHEME_DIM3 = 13
 
@@ -174,7 +175,7 @@ _IGNORED_FLAGS = {
 # options of the reference whose feature is outside this engine: rejected loudly
 _UNSUPPORTED = {
     "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
-    "--2d-tex", "--2d-tey", "--2d-tez", "--2d-tmx", "--2d-tmy",
+    "--2d-tex", "--2d-tey", "--2d-tmx", "--2d-tmy",
     "--use-double-material-precision", "--eps-normed", "--mu-normed",
     "--current-source-jx", "--current-source-jy", "--current-source-jz",
     "--current-source-mx", "--current-source-my", "--current-source-mz",
@@ -217,6 +218,8 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             cfg.scheme = SCHEME_DIM3
         elif a == "--2d-tmz":
             cfg.scheme = SCHEME_TMZ
+        elif a == "--2d-tez":
+            cfg.scheme = SCHEME_TEZ
         elif a == "--size":
             cfg.size = _parse_coord(val(), 100)
         elif a == "--same-size":

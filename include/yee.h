@@ -40,6 +40,7 @@ extern "C" {
 #define YEE_ABI_VERSION 1
 
 /* SchemeType values of the reference (Source/Settings/Settings.h:47-62). */
+#define YEE_SCHEME_TEZ 9
 #define YEE_SCHEME_TMZ 12
 #define YEE_SCHEME_DIM3 13
 
@@ -77,7 +78,7 @@ enum yee_material
  */
 typedef struct yee_config
 {
-  int32_t scheme;             /* YEE_SCHEME_DIM3 (--3d) or YEE_SCHEME_TMZ (--2d-tmz) */
+  int32_t scheme;             /* YEE_SCHEME_DIM3 (--3d), YEE_SCHEME_TMZ (--2d-tmz) or YEE_SCHEME_TEZ (--2d-tez) */
   int32_t layout;             /* --layout-type: only 0 (E_CENTERED) is supported */
   int32_t precision;          /* 4 = FP32 fields, 8 = FP64 fields (reference: -DFLOAT_VALUES / -DDOUBLE_VALUES) */
   int32_t time_steps;         /* --time-steps T; step t = 0 performs no field update (Scheme.inc.h:1032) */

@@ -512,6 +512,23 @@ FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2
   return 0;
 }
 #endif
+#ifdef MODE_TEZ
+template <>
+grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate>::sizeZ () const
+{
+  return 1;
+}
+template <>
+grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate>::pmlZ () const
+{
+  return 0;
+}
+template <>
+FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate>::fpZ (const GridCoordinateFP2D &) const
+{
+  return 0;
+}
+#endif
 
 } /* namespace */
 
@@ -558,6 +575,20 @@ int main (int argc, char **argv)
     GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeftX (), SOLVER_SETTINGS.getTFSFSizeLeftY (), CoordinateType::X, CoordinateType::Y);
     GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), CoordinateType::X, CoordinateType::Y);
     RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate> run (s, p, l, r, 2);
+    run.initMaterials ();
+    secs = run.run ();
+    run.dump ();
+  }
+  else
+#endif
+#ifdef MODE_TEZ
+  if (st == SchemeType::Dim2_TEz)
+  {
+    GridCoordinate2D s = GRID_COORDINATE_2D (SOLVER_SETTINGS.getSizeX (), SOLVER_SETTINGS.getSizeY (), CoordinateType::X, CoordinateType::Y);
+    GridCoordinate2D p = GRID_COORDINATE_2D (SOLVER_SETTINGS.getPMLSizeX (), SOLVER_SETTINGS.getPMLSizeY (), CoordinateType::X, CoordinateType::Y);
+    GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeftX (), SOLVER_SETTINGS.getTFSFSizeLeftY (), CoordinateType::X, CoordinateType::Y);
+    GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), CoordinateType::X, CoordinateType::Y);
+    RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate> run (s, p, l, r, 2);
     run.initMaterials ();
     secs = run.run ();
     run.dump ();

@@ -250,10 +250,21 @@ static int o_tfsf_pred (const oracle *o, int comp, int dir, const double real[3]
   return cond_b && cond_a && cond_t;
 }
 
-/* Incident-field projections get<comp>FromIncident{E,H} (Source/Layout/YeeGridLayout.h:2194-2356). */
+/* Incident-field projections get<comp>FromIncident{E,H} (Source/Layout/YeeGridLayout.h:2194-2356;
+ * Dim2_TEz :1939-2024). */
 static double o_project (const oracle *o, int comp, double v)
 {
   double a1 = o->a1, a2 = o->a2, a3 = o->a3;
+  if (o->dim == 2 && o->cfg.scheme == YEE_SCHEME_TEZ)
+  {
+    switch (comp)
+    {
+      case YEE_EX: return v * (sin (a2));
+      case YEE_EY: return v * (-cos (a2));
+      case YEE_HZ: return -v;
+      default: return 0;
+    }
+  }
   if (o->dim == 2)
   {
     switch (comp)
@@ -478,7 +489,8 @@ double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, doubl
 
 oracle *oracle_create (const yee_config *cfg)
 {
-  if (!cfg || cfg->layout != 0 || (cfg->scheme != YEE_SCHEME_DIM3 && cfg->scheme != YEE_SCHEME_TMZ))
+  if (!cfg || cfg->layout != 0
+      || (cfg->scheme != YEE_SCHEME_DIM3 && cfg->scheme != YEE_SCHEME_TMZ && cfg->scheme != YEE_SCHEME_TEZ))
   {
     return NULL;
   }
@@ -500,7 +512,10 @@ oracle *oracle_create (const yee_config *cfg)
   o->layers = (o->use_meta && o->use_pml) ? 3 : 2;
   for (int c = 0; c < 6; ++c)
   {
-    o->present[c] = o->dim == 3 ? 1 : (c == YEE_EZ || c == YEE_HX || c == YEE_HY);
+    /* InternalScheme component flags (InternalScheme.inc.h:3243-3272): TMz Ez, Hx, Hy; TEz Ex, Ey, Hz */
+    if (o->dim == 3) o->present[c] = 1;
+    else if (cfg->scheme == YEE_SCHEME_TMZ) o->present[c] = (c == YEE_EZ || c == YEE_HX || c == YEE_HY);
+    else o->present[c] = (c == YEE_EX || c == YEE_EY || c == YEE_HZ);
   }
   for (int m = 0; m < 6; ++m)
   {

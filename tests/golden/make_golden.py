@@ -61,12 +61,23 @@ def cases(tmp):
     out["tmz_pml_tfsf_oblique"] = (
         "--2d-tmz --size x:32,y:28 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:8 "
         "--tfsf-size-right x:6,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 50").split()
+    # TEz: the TEz layout asserts theta = 90 and psi = 0 (YeeGridLayout.cpp:2442)
+    out["tez_pml_tfsf_oblique_sphere"] = (
+        "--2d-tez --size x:32,y:28 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:8 "
+        "--tfsf-size-right x:6,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 0 --eps-sphere 3 "
+        "--eps-sphere-center x:16,y:14 --eps-sphere-radius 5 --time-steps 50").split()
+    out["tez_pml_pointsource"] = (
+        "--2d-tez --size x:30,y:26 --use-pml --pml-size x:4,y:5 --point-source x:13,y:11 --point-source-hz "
+        "--point-source-ex --angle-psi 0 --time-steps 50").split()
     return out
 
 
 def main():
+    only = set(sys.argv[1:])   # optional: the fixture names to (re)generate
     with tempfile.TemporaryDirectory() as tmp:
         for name, argv in cases(tmp).items():
+            if only and name not in only:
+                continue
             rec, _ = run_ref_driver(argv, os.path.join(tmp, name + ".bin"))
             payload = {"argv": np.array(" ".join(a if not a.startswith(tmp) else os.path.basename(a) for a in argv))}
             for key, val in rec.items():

@@ -13,6 +13,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 FIELDS_3D = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
 FIELDS_TMZ = ["Ez", "Hx", "Hy"]
+FIELDS_TEZ = ["Ex", "Ey", "Hz"]
 
 
 def golden_names():
@@ -55,7 +56,7 @@ def rel_l2(a, b):
 
 
 def field_names(cfg, aux=False):
-    base = FIELDS_3D if cfg.dim == 3 else FIELDS_TMZ
+    base = FIELDS_3D if cfg.dim == 3 else (FIELDS_TEZ if cfg.scheme == 9 else FIELDS_TMZ)
     names = list(base)
     if aux and cfg.use_pml:
         names += [("D" if n[0] == "E" else "B") + n[1] for n in base]

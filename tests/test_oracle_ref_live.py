@@ -30,6 +30,18 @@ CASES = [
     "--2d-tmz --size x:28,y:30 --use-pml --pml-size x:3,y:3 --use-tfsf --tfsf-size-left x:6,y:6 "
     "--tfsf-size-right x:6,y:6 --angle-teta 90 --angle-phi 0 --angle-psi 90 --courant-factor 0.25 "
     "--wavelength 0.01 --time-steps 60",
+    # TEz: oblique TF/SF in PML with an eps sphere
+    "--2d-tez --size x:30,y:26 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:8,y:7 "
+    "--tfsf-size-right x:7,y:8 --angle-teta 90 --angle-phi 30 --angle-psi 0 --eps-sphere 3 "
+    "--eps-sphere-center x:15,y:13 --eps-sphere-radius 5 --time-steps 60",
+    # TEz with every point source component it has, PML and a mu sphere (the TEz layout asserts
+    # theta = 90, psi = 0: YeeGridLayout.cpp:2442)
+    "--2d-tez --size x:26,y:31 --use-pml --pml-size x:3,y:5 --point-source x:11,y:14 --point-source-ex "
+    "--point-source-ey --point-source-hz --mu-sphere 2 --mu-sphere-center x:13,y:15 --mu-sphere-radius 4 "
+    "--angle-psi 0 --time-steps 45",
+    # TEz at phi = 0 without PML
+    "--2d-tez --size x:24,y:20 --use-tfsf --tfsf-size-left x:5,y:5 --tfsf-size-right x:5,y:5 --angle-teta 90 "
+    "--angle-phi 0 --angle-psi 0 --time-steps 40",
 ]
 
 
