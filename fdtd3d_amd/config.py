@@ -307,7 +307,7 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
         elif a in _IGNORED_FLAGS:
             pass
         elif a in _UNSUPPORTED:
-            raise OptionError(f"option {a} is outside the Yee engine (3D and TMz, E_CENTERED, real fields)")
+            raise OptionError(f"option {a} is outside the Yee engine (3D, TMz and TEz, E_CENTERED, real fields)")
         else:
             raise OptionError(f"unknown option {a}")
         i += 1

@@ -127,7 +127,7 @@ const std::set<std::string> kIgnoredFlags = {
 };
 const std::set<std::string> kUnsupported = {
   "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy", "--2d-tex", "--2d-tey",
-  "--2d-tez", "--2d-tmx", "--2d-tmy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
+  "--2d-tmx", "--2d-tmy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
   "--current-source-jx", "--current-source-jy", "--current-source-jz", "--current-source-mx",
   "--current-source-my", "--current-source-mz",
   // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
@@ -176,6 +176,7 @@ Options parse (int argc, char **argv)
     }
     else if (a == "--3d") c.scheme = YEE_SCHEME_DIM3;
     else if (a == "--2d-tmz") c.scheme = YEE_SCHEME_TMZ;
+    else if (a == "--2d-tez") c.scheme = YEE_SCHEME_TEZ;
     else if (a == "--size") parse_coord (val (), 100, c.size);
     else if (a == "--same-size") copy3 (c.size);
     else if (a == "--pml-size") parse_coord (val (), 2, c.pml);
@@ -255,7 +256,7 @@ Options parse (int argc, char **argv)
     if (done) continue;
     if (kIgnoredWithArg.count (a)) (void) val ();
     else if (kIgnoredFlags.count (a)) continue;
-    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (3D and TMz, E_CENTERED, real fields)");
+    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (3D, TMz and TEz, E_CENTERED, real fields)");
     else die ("unknown option " + a);
   }
   return o;
@@ -412,7 +413,9 @@ void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
   std::vector<std::vector<double>> fields (6);
   for (int comp = 0; comp < 6; ++comp)
   {
-    if (dim == 2 && comp != YEE_EZ && comp != YEE_HX && comp != YEE_HY) continue;
+    if (dim == 2 && (c.scheme == YEE_SCHEME_TMZ ? (comp != YEE_EZ && comp != YEE_HX && comp != YEE_HY)
+                                                : (comp != YEE_EX && comp != YEE_EY && comp != YEE_HZ)))
+      continue;
     fields[comp].resize ((size_t) n);
     if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, fields[comp].data ()));
     else check (yee_get_field (h, comp, fields[comp].data ()));
@@ -463,7 +466,7 @@ int main (int argc, char **argv)
 {
   Options o = parse (argc, argv);
   yee_config &c = o.cfg;
-  if (c.scheme == YEE_SCHEME_TMZ) c.size[2] = 1;
+  if (c.scheme != YEE_SCHEME_DIM3) c.size[2] = 1;
   const int64_t n = c.size[0] * c.size[1] * (c.scheme == YEE_SCHEME_DIM3 ? c.size[2] : 1);
 
   char info[256];

@@ -26,6 +26,8 @@ template <typename T, int MODE> __global__ void e_half_3d (StepArgs<T>);
 template <typename T, int MODE> __global__ void h_half_3d (StepArgs<T>);
 template <typename T, int MODE> __global__ void e_half_tmz (StepArgs<T>);
 template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
+template <typename T, int MODE> __global__ void e_half_tez (StepArgs<T>);
+template <typename T, int MODE> __global__ void h_half_tez (StepArgs<T>);
 template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
@@ -183,6 +185,20 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
       case 3: hipLaunchKernelGGL ((h_half_3d<T, 1>), grid, block, 0, h->stream, a); break;
       case 4: hipLaunchKernelGGL ((e_half_3d<T, 2>), grid, block, 0, h->stream, a); break;
       default: hipLaunchKernelGGL ((h_half_3d<T, 2>), grid, block, 0, h->stream, a); break;
+    }
+  }
+  else if (h->cfg.scheme == YEE_SCHEME_TEZ)
+  {
+    dim3 block (64, 4, 1);
+    dim3 grid ((unsigned) ((h->n[1] + 63) / 64), (unsigned) ((h->n[0] + 3) / 4), 1);
+    switch (h->mode * 2 + (e_half ? 0 : 1))
+    {
+      case 0: hipLaunchKernelGGL ((e_half_tez<T, 0>), grid, block, 0, h->stream, a); break;
+      case 1: hipLaunchKernelGGL ((h_half_tez<T, 0>), grid, block, 0, h->stream, a); break;
+      case 2: hipLaunchKernelGGL ((e_half_tez<T, 1>), grid, block, 0, h->stream, a); break;
+      case 3: hipLaunchKernelGGL ((h_half_tez<T, 1>), grid, block, 0, h->stream, a); break;
+      case 4: hipLaunchKernelGGL ((e_half_tez<T, 2>), grid, block, 0, h->stream, a); break;
+      default: hipLaunchKernelGGL ((h_half_tez<T, 2>), grid, block, 0, h->stream, a); break;
     }
   }
   else
@@ -534,8 +550,8 @@ int upload (yee_handle *h, void **dst, const double *src, size_t n)
 int validate (const yee_config *c)
 {
   if (!c) return fail (YEE_EINVAL, "null config");
-  if (c->scheme != YEE_SCHEME_DIM3 && c->scheme != YEE_SCHEME_TMZ)
-    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13) or YEE_SCHEME_TMZ (12)");
+  if (c->scheme != YEE_SCHEME_DIM3 && c->scheme != YEE_SCHEME_TMZ && c->scheme != YEE_SCHEME_TEZ)
+    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13), YEE_SCHEME_TMZ (12) or YEE_SCHEME_TEZ (9)");
   if (c->layout != 0) return fail (YEE_EINVAL, "only the E_CENTERED layout (0) is supported");
   if (c->precision != 4 && c->precision != 8) return fail (YEE_EINVAL, "precision must be 4 or 8");
   const int dim = c->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
@@ -738,7 +754,9 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   h->wsize = cfg->precision == 4 ? 4 : 8;
   h->mode = h->use_meta () ? 2 : (h->use_pml () ? 1 : 0);
   for (int c = 0; c < 6; ++c)
-    h->present[c] = h->dim == 3 || c == YEE_EZ || c == YEE_HX || c == YEE_HY;
+    // InternalScheme component flags (InternalScheme.inc.h:3243-3272)
+    h->present[c] = h->dim == 3 || (cfg->scheme == YEE_SCHEME_TMZ ? (c == YEE_EZ || c == YEE_HX || c == YEE_HY)
+                                                                : (c == YEE_EX || c == YEE_EY || c == YEE_HZ));
   auto bail = [&] (int code) {
     yee_destroy (h);
     return code;
@@ -1080,9 +1098,12 @@ int scattered (yee_handle *h, int comp, void *dev)
   else
     switch (comp)
     {
+      case 0: hipLaunchKernelGGL ((scattered_kernel<T, 0, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 1: hipLaunchKernelGGL ((scattered_kernel<T, 1, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
       case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
       case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      default: hipLaunchKernelGGL ((scattered_kernel<T, 4, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 4: hipLaunchKernelGGL ((scattered_kernel<T, 4, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      default: hipLaunchKernelGGL ((scattered_kernel<T, 5, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
     }
   HIP_TRY (hipGetLastError ());
   return YEE_OK;

@@ -996,6 +996,113 @@ __device__ __forceinline__ void tmz_h_cell (const StepArgs<T> &a, int i, int j)
   update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), my, special, omy, gmy);
 }
 
+// ---------------------------------------------------------------------------------------
+// TEz half-steps (Ex, Ey; Hz) on [1, N-1)^2: the 3D expressions with the absent operands
+// (Hy for Ex, Hx for Ey) read as 0, as for TMz; y lane-contiguous, updated in place
+// ---------------------------------------------------------------------------------------
+template <typename T, int MODE>
+__device__ __forceinline__ void tez_e_cell (const StepArgs<T> &a, int i, int j)
+{
+  const int64_t sx = a.sx;
+  const int sy = a.sy;   // == 1
+  const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
+  const int p[3] = { i, j, 0 };
+  const bool special = is_special (a, p, 2, MODE);
+  T fo0, fo1;
+  Old<T> do0, do1;
+  load_old<T, MODE, 0> (a, q, special, fo0, do0);
+  load_old<T, MODE, 1> (a, q, special, fo1, do1);
+  const T *Hz = a.f[5];
+  const T hz = Hz[q];
+  // Ex: Hz at (i, j-1) and (i, j), Hy absent; Ey: Hx absent, Hz at (i-1, j) and (i, j)
+  T vx[4] = { Hz[q - sy], hz, (T) 0, (T) 0 };
+  T vy[4] = { (T) 0, (T) 0, Hz[q - sx], hz };
+  if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0))
+  {
+    tfsf_apply<T, 0, 2> (a, i, j, 0, vx);
+    tfsf_apply<T, 1, 2> (a, i, j, 0, vy);
+  }
+  // 2-cell averages along each component's axis
+  T ex_m = a.eps_u, ey_m = a.eps_u;
+  if (a.eps)
+  {
+    const T e = a.eps[q];
+    ex_m = avg2 (e, a.eps[q + sx]);
+    ey_m = avg2 (e, a.eps[q + sy]);
+  }
+  T oxm = 0, oym = 0, gxm = 0, gym = 0;
+  if (MODE == 2 && special)
+  {
+    if (a.ope)
+    {
+      oxm = avg2 (a.ope[q], a.ope[q + sx]);
+      oym = avg2 (a.ope[q], a.ope[q + sy]);
+    }
+    if (a.ge)
+    {
+      gxm = avg2 (a.ge[q], a.ge[q + sx]);
+      gym = avg2 (a.ge[q], a.ge[q + sy]);
+    }
+  }
+  update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl (vx), ex_m, special, oxm, gxm);
+  update_cell<T, MODE, 1> (a, q, p, fo1, do1, curl (vy), ey_m, special, oym, gym);
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void tez_h_cell (const StepArgs<T> &a, int i, int j)
+{
+  const int64_t sx = a.sx;
+  const int sy = a.sy;
+  const int64_t q = (int64_t) i * sx + (int64_t) j * sy;
+  const int p[3] = { i, j, 0 };
+  const bool special = is_special (a, p, 2, MODE);
+  T fo5;
+  Old<T> do5;
+  load_old<T, MODE, 5> (a, q, special, fo5, do5);
+  const T *Ex = a.f[0], *Ey = a.f[1];
+  // Hz: Ex at (i, j) and (i, j+1), Ey at (i, j) and (i+1, j)
+  T vz[4] = { Ex[q], Ex[q + sy], Ey[q], Ey[q + sx] };
+  if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0)) tfsf_apply<T, 5, 2> (a, i, j, 0, vz);
+  // 4-cell transverse average
+  T hz_m = a.mu_u;
+  if (a.mu)
+  {
+    const T *m = a.mu;
+    hz_m = avg4 (m[q], m[q + sy], m[q + sx], m[q + sx + sy]);
+  }
+  T ozm = 0, gzm = 0;
+  if (MODE == 2 && special)
+  {
+    const T *o = a.opm, *g = a.gm;
+    if (o) ozm = sum4 (o[q], o[q + sy], o[q + sx], o[q + sx + sy]);
+    if (g) gzm = sum4 (g[q], g[q + sy], g[q + sx], g[q + sx + sy]);
+  }
+  update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl (vz), hz_m, special, ozm, gzm);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void e_half_tez (StepArgs<T> a)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
+  const int p[3] = { i, j, 0 };
+  // outside the Drude box: the PML update (see e_half_3d)
+  if (MODE == 2 && !in_drude_box (a, p, 2)) tez_e_cell<T, 1> (a, i, j);
+  else tez_e_cell<T, MODE> (a, i, j);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void h_half_tez (StepArgs<T> a)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.ny - 1) return;
+  const int p[3] = { i, j, 0 };
+  if (MODE == 2 && !in_drude_box (a, p, 2)) tez_h_cell<T, 1> (a, i, j);
+  else tez_h_cell<T, MODE> (a, i, j);
+}
+
 template <typename T, int MODE>
 __global__ __launch_bounds__ (256) void e_half_tmz (StepArgs<T> a)
 {
@@ -1524,9 +1631,12 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
     {
       switch (comp)
       {
+        case 0: m = tfsf_bits<T, 0, 2> (a, i, j, k); break;
+        case 1: m = tfsf_bits<T, 1, 2> (a, i, j, k); break;
         case 2: m = tfsf_bits<T, 2, 2> (a, i, j, k); break;
         case 3: m = tfsf_bits<T, 3, 2> (a, i, j, k); break;
-        default: m = tfsf_bits<T, 4, 2> (a, i, j, k); break;
+        case 4: m = tfsf_bits<T, 4, 2> (a, i, j, k); break;
+        default: m = tfsf_bits<T, 5, 2> (a, i, j, k); break;
       }
     }
     gst (out + q, m);
@@ -1541,6 +1651,12 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void h_half_3d<T, 0> (StepArgs<T>);                                  \
   template __global__ void h_half_3d<T, 1> (StepArgs<T>);                                  \
   template __global__ void h_half_3d<T, 2> (StepArgs<T>);                                  \
+  template __global__ void e_half_tez<T, 0> (StepArgs<T>);                                 \
+  template __global__ void e_half_tez<T, 1> (StepArgs<T>);                                 \
+  template __global__ void e_half_tez<T, 2> (StepArgs<T>);                                 \
+  template __global__ void h_half_tez<T, 0> (StepArgs<T>);                                 \
+  template __global__ void h_half_tez<T, 1> (StepArgs<T>);                                 \
+  template __global__ void h_half_tez<T, 2> (StepArgs<T>);                                 \
   template __global__ void e_half_tmz<T, 0> (StepArgs<T>);                                 \
   template __global__ void e_half_tmz<T, 1> (StepArgs<T>);                                 \
   template __global__ void e_half_tmz<T, 2> (StepArgs<T>);                                 \
@@ -1568,6 +1684,9 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void scattered_kernel<T, 3, 3> (StepArgs<T>, const T *, T *, int);  \
   template __global__ void scattered_kernel<T, 4, 3> (StepArgs<T>, const T *, T *, int);  \
   template __global__ void scattered_kernel<T, 5, 3> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 0, 2> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 1, 2> (StepArgs<T>, const T *, T *, int);  \
+  template __global__ void scattered_kernel<T, 5, 2> (StepArgs<T>, const T *, T *, int);  \
   template __global__ void scattered_kernel<T, 2, 2> (StepArgs<T>, const T *, T *, int);  \
   template __global__ void scattered_kernel<T, 3, 2> (StepArgs<T>, const T *, T *, int);  \
   template __global__ void scattered_kernel<T, 4, 2> (StepArgs<T>, const T *, T *, int);

@@ -89,6 +89,16 @@ Scalars compute_scalars (const yee_config &cfg)
     s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a1) * std::sin (a2);
     s.zero_inc[2] = cfg.tfsf_left[2] - 2.5 * std::cos (a1);
   }
+  else if (cfg.scheme == YEE_SCHEME_TEZ)
+  {
+    // YeeGridLayout.h:1939-2024 and YeeGridLayout.cpp:2424-2440 (TEz); Hz = -valH, i.e. valH * -1
+    s.proj[YEE_EX] = std::sin (a2);
+    s.proj[YEE_EY] = -std::cos (a2);
+    s.proj[YEE_HZ] = -1.0;
+    s.zero_inc[0] = cfg.tfsf_left[0] - 2.5 * std::cos (a2);
+    s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a2);
+    s.zero_inc[2] = 0;
+  }
   else
   {
     // YeeGridLayout.h:2194-2277 and YeeGridLayout.cpp:2583-2599 (TMz)

@@ -22,7 +22,7 @@
  *     message of the last failure on the calling thread is yee_last_error().
  *   - Host pointers are borrowed for the duration of the call and copied.
  *   - Grids are dense [x][y][z] arrays with z fastest (the reference's storage order,
- *     Source/Grid/Grid.h:585-730); TMz grids are [x][y].
+ *     Source/Grid/Grid.h:585-730); 2D (TMz, TEz) grids are [x][y].
  *   - A handle owns its device memory and one HIP stream; a handle is not thread-safe,
  *     distinct handles may be used concurrently.
  *   - Fields start at zero, as in the reference (field loading is commented out in
@@ -115,7 +115,7 @@ void yee_destroy (yee_handle *h);
 
 /*
  * Sets one material grid (full [x][y][z] FP64 array of size[0]*size[1]*size[2] values,
- * or [x][y] for TMz).  Defaults: eps = mu = 1, omega/gamma = 0.  Omega values are in
+ * or [x][y] in 2D).  Defaults: eps = mu = 1, omega/gamma = 0.  Omega values are in
  * rad/s (the reference scales --omegape-sphere by 2*pi*f before storing, Scheme.inc.h:3091).
  * Replaces the material part of Scheme::initGrids (Source/Scheme/Scheme.inc.h:2962-3165).
  */
@@ -170,7 +170,7 @@ int yee_time (yee_handle *h, int32_t *t);
 
 /*
  * Copies the newest time layer of a component to host memory as FP64
- * ([x][y][z], or [x][y] for TMz).  Components absent from the scheme return YEE_EINVAL.
+ * ([x][y][z], or [x][y] in 2D).  Components absent from the scheme return YEE_EINVAL.
  * D/B (and D1/B1) hold the reference's values inside the PML slabs; in the interior the
  * engine updates E/H directly, which equals the reference's D->E stage with sigma = 0
  * up to rounding, and D/B read back as 0 there.

@@ -247,3 +247,29 @@ def test_cli_binary_and_material_dumps(binary, dtype, precision, engine_lib, tmp
     out = subprocess.run([exe] + (base + " --load-eps-from-file short.dat").split(), capture_output=True, text=True,
                          timeout=120, cwd=again)
     assert out.returncode != 0 and "short.dat" in out.stderr
+
+
+def test_cli_tez(engine_lib, tmp_path):
+    """--2d-tez: the CLI runs the TEz scheme (Ex, Ey, Hz) and dumps exactly those components."""
+    from fdtd3d_amd.config import parse_args
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    args = ("--2d-tez --size x:30,y:26 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:6 "
+            "--tfsf-size-right x:6,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 0 --time-steps 40")
+    say("[gpu] cli tez")
+    out = subprocess.run([exe] + (args + " --save-res --save-as-txt").split(), capture_output=True, text=True,
+                         timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    eng = Engine(parse_args(args.split()))
+    eng.prepare()
+    eng.step(40)
+    eng.sync()
+    for name in ("Ex", "Ey", "Hz"):
+        xy, val = _read_txt(tmp_path / f"previous-1_[timestep=40]_[pid=0]_[name={name}].txt")
+        assert xy.shape == (30 * 26, 2)
+        assert np.abs(val).max() > 0, name
+        assert np.allclose(val, eng.field(name).ravel(), rtol=1e-14, atol=1e-300), name
+    for name in ("Ez", "Hx", "Hy"):
+        assert not (tmp_path / f"previous-1_[timestep=40]_[pid=0]_[name={name}].txt").exists()
+    eng.close()

@@ -78,6 +78,13 @@ CASES = [
     # TMz with a resonant Drude square inside PML + TF/SF (the single-run TMz half-step kernels, MODE 2)
     ("tmz_drude", "--2d-tmz --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:9,y:8 "
      "--tfsf-size-right x:9,y:8 --angle-teta 90 --angle-phi 30 --angle-psi 90 --use-metamaterials --time-steps 300"),
+    # TEz: the smallest grid, PML everywhere, a resonant Drude square with PML + TF/SF
+    ("min_tez", "--2d-tez --size x:3,y:3 --point-source x:1,y:1 --point-source-hz --point-source-ex "
+     "--angle-psi 0 --time-steps 9"),
+    ("all_pml_tez", "--2d-tez --size x:10,y:8 --use-pml --pml-size x:5,y:4 --point-source x:5,y:4 "
+     "--point-source-hz --angle-psi 0 --time-steps 25"),
+    ("tez_drude", "--2d-tez --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:9,y:8 "
+     "--tfsf-size-right x:9,y:8 --angle-teta 90 --angle-phi 30 --angle-psi 0 --use-metamaterials --time-steps 300"),
     # a point source off the grid is ignored
     ("source_off_grid", "--3d --size x:12,y:10,z:14 --use-pml --pml-size x:2,y:2,z:2 --point-source x:30,y:4,z:5 "
      "--point-source-ez --time-steps 15"),
@@ -85,7 +92,7 @@ CASES = [
 
 
 def _materials(name, cfg):
-    if name == "tmz_drude":
+    if name in ("tmz_drude", "tez_drude"):
         w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
         om = np.zeros(cfg.shape)
         ga = np.zeros(cfg.shape)

@@ -73,6 +73,8 @@ MASK_CASES = [
     "--angle-teta 30 --angle-phi 45 --angle-psi 90 --time-steps 1",
     "--2d-tmz --size x:23,y:29 --use-tfsf --tfsf-size-left x:5,y:6 --tfsf-size-right x:4,y:7 "
     "--angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 1",
+    "--2d-tez --size x:23,y:29 --use-tfsf --tfsf-size-left x:5,y:6 --tfsf-size-right x:4,y:7 "
+    "--angle-teta 90 --angle-phi 30 --angle-psi 0 --time-steps 1",
 ]
 
 
@@ -152,6 +154,8 @@ SCATTERED_CASES = [
     "--tfsf-size-right x:9,y:8,z:7 --angle-teta 30 --angle-phi 40 --angle-psi 20 --time-steps 60",
     "--2d-tmz --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:10,y:9 "
     "--tfsf-size-right x:9,y:10 --angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 80",
+    "--2d-tez --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:10,y:9 "
+    "--tfsf-size-right x:9,y:10 --angle-teta 90 --angle-phi 50 --angle-psi 0 --time-steps 80",
 ]
 
 
