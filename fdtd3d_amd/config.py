@@ -16,9 +16,16 @@ import struct
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
+SCHEME_TEX = 7
+SCHEME_TEY = 8
 SCHEME_TEZ = 9
+SCHEME_TMX = 10
+SCHEME_TMY = 11
 SCHEME_TMZ = 12
 SCHEME_DIM3 = 13
+# the option of each 2D scheme (Settings.inc.h) and the axis that scheme lacks
+SCHEMES_2D = {"--2d-tex": (SCHEME_TEX, 0), "--2d-tey": (SCHEME_TEY, 1), "--2d-tez": (SCHEME_TEZ, 2),
+              "--2d-tmx": (SCHEME_TMX, 0), "--2d-tmy": (SCHEME_TMY, 1), "--2d-tmz": (SCHEME_TMZ, 2)}
 
 USE_PML = 0x1
 USE_TFSF = 0x2
@@ -109,7 +116,27 @@ class Config:
 
     @property
     def shape(self):
-        return tuple(self.size[:3]) if self.dim == 3 else tuple(self.size[:2])
+        if self.dim == 3:
+            return tuple(self.size[:3])
+        return tuple(self.size[a] for a in range(3) if a != self.missing_axis)
+
+    @property
+    def missing_axis(self) -> int:
+        """The axis a 2D scheme lacks (-1 in 3D)."""
+        for scheme, axis in SCHEMES_2D.values():
+            if scheme == self.scheme:
+                return axis
+        return -1
+
+    @property
+    def components(self):
+        """Field components of the scheme (InternalScheme component flags): TE<a> has H along the
+        missing axis a and E across it, TM<a> the reverse."""
+        names = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
+        if self.dim == 3:
+            return names
+        m, te = self.missing_axis, self.scheme <= SCHEME_TEZ
+        return [n for c, n in enumerate(names) if ((c >= 3) == te) == ((c % 3) == m)]
 
     def to_c(self) -> YeeConfigC:
         c = YeeConfigC()
@@ -175,7 +202,6 @@ _IGNORED_FLAGS = {
 # options of the reference whose feature is outside this engine: rejected loudly
 _UNSUPPORTED = {
     "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
-    "--2d-tex", "--2d-tey", "--2d-tmx", "--2d-tmy",
     "--use-double-material-precision", "--eps-normed", "--mu-normed",
     "--current-source-jx", "--current-source-jy", "--current-source-jz",
     "--current-source-mx", "--current-source-my", "--current-source-mz",
@@ -216,10 +242,8 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             args = args[: i + 1] + rest + args[i + 1:]
         elif a == "--3d":
             cfg.scheme = SCHEME_DIM3
-        elif a == "--2d-tmz":
-            cfg.scheme = SCHEME_TMZ
-        elif a == "--2d-tez":
-            cfg.scheme = SCHEME_TEZ
+        elif a in SCHEMES_2D:
+            cfg.scheme = SCHEMES_2D[a][0]
         elif a == "--size":
             cfg.size = _parse_coord(val(), 100)
         elif a == "--same-size":

@@ -40,7 +40,11 @@ extern "C" {
 #define YEE_ABI_VERSION 1
 
 /* SchemeType values of the reference (Source/Settings/Settings.h:47-62). */
+#define YEE_SCHEME_TEX 7
+#define YEE_SCHEME_TEY 8
 #define YEE_SCHEME_TEZ 9
+#define YEE_SCHEME_TMX 10
+#define YEE_SCHEME_TMY 11
 #define YEE_SCHEME_TMZ 12
 #define YEE_SCHEME_DIM3 13
 

@@ -124,8 +124,10 @@ def sphere_map(cfg: Config, value: float, outside: float, center, radius: int) -
             else:
                 out[idx] = L.oracle_sphere_accurate_3d(mx, my, mz, cx, cy, cz, r, value, outside, cfg.sphere_accuracy)
         else:
+            # the two axes of the 2D scheme (the sphere centre is given in 3D coordinates)
+            ca, cb = [c for a, c in enumerate((cx, cy, cz)) if a != cfg.missing_axis]
             i, j = idx
-            out[idx] = L.oracle_sphere_accurate_2d(i + 0.5, j + 0.5, cx, cy, r, value, outside, cfg.sphere_accuracy)
+            out[idx] = L.oracle_sphere_accurate_2d(i + 0.5, j + 0.5, ca, cb, r, value, outside, cfg.sphere_accuracy)
     return out
 
 

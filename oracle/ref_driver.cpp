@@ -107,30 +107,39 @@ class RefRun
   Layout *layout;
   IS *is;
   int dim;
+  int miss;   /* the axis a 2D scheme lacks (-1 in 3D): TEx/TMx 0, TEy/TMy 1, TEz/TMz 2 */
+
+  /* slot of physical axis a among the scheme's coordinates */
+  int slot (int a) const
+  {
+    return (miss >= 0 && a > miss) ? a - 1 : a;
+  }
+  FPValue fpSlot (const TCFP &c, int sl) const;
+  grid_coord tcSlot (const TC &c, int sl) const;
 
   TC coordOf (grid_coord i, grid_coord j, grid_coord k) const
   {
     return TC::initAxesCoordinate (i, j, k, ct1, ct2, ct3);
   }
 
+  /* extent along physical axis a (1 along the missing axis) */
   grid_coord sz (int axis) const
   {
-    if (axis == 0) return size.get1 ();
-    if (axis == 1) return size.get2 ();
-    return sizeZ ();
+    return axis == miss ? 1 : tcSlot (size, slot (axis));
   }
-
-  grid_coord sizeZ () const;
 
   template <class F>
   void forAll (grid_coord lo, grid_coord hiOff, F f)
   {
-    grid_coord nx = sz (0), ny = sz (1), nz = sz (2);
-    grid_coord zlo = dim == 3 ? lo : 0;
-    grid_coord zhi = dim == 3 ? nz - hiOff : 1;
-    for (grid_coord i = lo; i < nx - hiOff; ++i)
-      for (grid_coord j = lo; j < ny - hiOff; ++j)
-        for (grid_coord k = zlo; k < zhi; ++k)
+    grid_coord l[3], h[3];
+    for (int a = 0; a < 3; ++a)
+    {
+      l[a] = a == miss ? 0 : lo;
+      h[a] = a == miss ? 1 : sz (a) - hiOff;
+    }
+    for (grid_coord i = l[0]; i < h[0]; ++i)
+      for (grid_coord j = l[1]; j < h[1]; ++j)
+        for (grid_coord k = l[2]; k < h[2]; ++k)
           f (coordOf (i, j, k));
   }
 
@@ -182,29 +191,31 @@ class RefRun
     FPValue dx = SOLVER_SETTINGS.getGridStep ();
     TC pml = layout->getLeftBorderPML ();
     /* x: fractional distance, right threshold N - P (SchemeHelper.h:592-630) */
+    if (miss != 0)
     {
       Grid<TC> *s = is->getSigmaX ();
-      FPValue P = FPValue (pml.get1 ());
+      FPValue P = FPValue (tcSlot (pml, slot (0)));
       FPValue boundary = P * dx;
       FPValue n = FPValue (sz (0));
       forAll (0, 0, [&] (TC p) {
-        FPValue x = layout->getEpsCoordFP (p).get1 ();
+        FPValue x = fpSlot (layout->getEpsCoordFP (p), slot (0));
         FPValue val = 0;
         if (x < P) val = sigmaValue (P - x, boundary, dx);
         else if (x >= n - P) val = sigmaValue (x - (n - P), boundary, dx);
         s->setFieldValue (val, p, 0);
       });
     }
-    /* y (and z in 3D): truncated integer distance, right threshold N + 0.5 - P (SchemeHelper.h:632-718) */
-    for (int axis = 1; axis < dim; ++axis)
+    /* y, z: truncated integer distance, right threshold N + 0.5 - P (SchemeHelper.h:632-718) */
+    for (int axis = 1; axis < 3; ++axis)
     {
+      if (axis == miss || (miss < 0 && axis >= dim)) continue;
       Grid<TC> *s = axis == 1 ? is->getSigmaY () : is->getSigmaZ ();
-      FPValue P = FPValue (axis == 1 ? pml.get2 () : pmlZ ());
+      FPValue P = FPValue (tcSlot (pml, slot (axis)));
       FPValue boundary = P * dx;
       FPValue n = (FPValue) sz (axis) + FPValue (0.5);
       forAll (0, 0, [&] (TC p) {
         TCFP fp = layout->getEpsCoordFP (p);
-        FPValue y = axis == 1 ? fp.get2 () : fpZ (fp);
+        FPValue y = fpSlot (fp, slot (axis));
         FPValue val = 0;
         if (y < P)
         {
@@ -221,8 +232,6 @@ class RefRun
     }
   }
 
-  grid_coord pmlZ () const;
-  FPValue fpZ (const TCFP &) const;
 
   template <uint8_t grid_type, bool usePML, bool useMeta>
   void fieldStep (time_step t)
@@ -315,11 +324,12 @@ class RefRun
 
 public:
 
-  RefRun (TC s, TC pml, TC tfL, TC tfR, int d)
-    : size (s), dim (d)
+  RefRun (TC s, TC pml, TC tfL, TC tfR, int d, int missing = -1)
+    : size (s), dim (d), miss (d == 3 ? -1 : missing)
   {
-    ct1 = CoordinateType::X;
-    ct2 = CoordinateType::Y;
+    ct1 = miss == 0 ? CoordinateType::Y : CoordinateType::X;
+    ct2 = miss == 2 ? CoordinateType::Y : CoordinateType::Z;
+    if (d == 3) ct2 = CoordinateType::Y;
     ct3 = d == 3 ? CoordinateType::Z : CoordinateType::NONE;
     layout = new Layout (s, pml, tfL, tfR,
                          SOLVER_SETTINGS.getIncidentWaveAngle1 () * PhysicsConst::Pi / 180.0,
@@ -445,12 +455,9 @@ public:
     putRecord ("Mu", dumpGrid (is->getMu (), 0));
     if (pml)
     {
-      putRecord ("SigmaX", dumpGrid (is->getSigmaX (), 0));
-      putRecord ("SigmaY", dumpGrid (is->getSigmaY (), 0));
-      if (dim == 3)
-      {
-        putRecord ("SigmaZ", dumpGrid (is->getSigmaZ (), 0));
-      }
+      if (miss != 0) putRecord ("SigmaX", dumpGrid (is->getSigmaX (), 0));
+      if (miss != 1) putRecord ("SigmaY", dumpGrid (is->getSigmaY (), 0));
+      if (dim == 3 || (miss >= 0 && miss != 2)) putRecord ("SigmaZ", dumpGrid (is->getSigmaZ (), 0));
     }
     if (meta)
     {
@@ -478,56 +485,48 @@ public:
   }
 };
 
+#define REF_SLOTS_3D(MODE)                                                                                   \
+  template <>                                                                                                \
+  FPValue RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate3DTemplate>::fpSlot (const GridCoordinateFP3D &c, int sl) const \
+  {                                                                                                          \
+    return sl == 0 ? c.get1 () : (sl == 1 ? c.get2 () : c.get3 ());                                          \
+  }                                                                                                          \
+  template <>                                                                                                \
+  grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate3DTemplate>::tcSlot (const GridCoordinate3D &c, int sl) const \
+  {                                                                                                          \
+    return sl == 0 ? c.get1 () : (sl == 1 ? c.get2 () : c.get3 ());                                          \
+  }
+#define REF_SLOTS_2D(MODE)                                                                                   \
+  template <>                                                                                                \
+  FPValue RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate>::fpSlot (const GridCoordinateFP2D &c, int sl) const \
+  {                                                                                                          \
+    return sl == 0 ? c.get1 () : c.get2 ();                                                                  \
+  }                                                                                                          \
+  template <>                                                                                                \
+  grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate>::tcSlot (const GridCoordinate2D &c, int sl) const \
+  {                                                                                                          \
+    return sl == 0 ? c.get1 () : c.get2 ();                                                                  \
+  }
 #ifdef MODE_DIM3
-template <>
-grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate>::sizeZ () const
-{
-  return size.get3 ();
-}
-template <>
-grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate>::pmlZ () const
-{
-  return layout->getLeftBorderPML ().get3 ();
-}
-template <>
-FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate>::fpZ (const GridCoordinateFP3D &c) const
-{
-  return c.get3 ();
-}
+REF_SLOTS_3D (Dim3)
 #endif
-#ifdef MODE_TMZ
-template <>
-grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate>::sizeZ () const
-{
-  return 1;
-}
-template <>
-grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate>::pmlZ () const
-{
-  return 0;
-}
-template <>
-FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate>::fpZ (const GridCoordinateFP2D &) const
-{
-  return 0;
-}
+#ifdef MODE_TEX
+REF_SLOTS_2D (Dim2_TEx)
+#endif
+#ifdef MODE_TEY
+REF_SLOTS_2D (Dim2_TEy)
 #endif
 #ifdef MODE_TEZ
-template <>
-grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate>::sizeZ () const
-{
-  return 1;
-}
-template <>
-grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate>::pmlZ () const
-{
-  return 0;
-}
-template <>
-FPValue RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate>::fpZ (const GridCoordinateFP2D &) const
-{
-  return 0;
-}
+REF_SLOTS_2D (Dim2_TEz)
+#endif
+#ifdef MODE_TMX
+REF_SLOTS_2D (Dim2_TMx)
+#endif
+#ifdef MODE_TMY
+REF_SLOTS_2D (Dim2_TMy)
+#endif
+#ifdef MODE_TMZ
+REF_SLOTS_2D (Dim2_TMz)
 #endif
 
 } /* namespace */
@@ -567,33 +566,37 @@ int main (int argc, char **argv)
   }
   else
 #endif
-#ifdef MODE_TMZ
-  if (st == SchemeType::Dim2_TMz)
-  {
-    GridCoordinate2D s = GRID_COORDINATE_2D (SOLVER_SETTINGS.getSizeX (), SOLVER_SETTINGS.getSizeY (), CoordinateType::X, CoordinateType::Y);
-    GridCoordinate2D p = GRID_COORDINATE_2D (SOLVER_SETTINGS.getPMLSizeX (), SOLVER_SETTINGS.getPMLSizeY (), CoordinateType::X, CoordinateType::Y);
-    GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeftX (), SOLVER_SETTINGS.getTFSFSizeLeftY (), CoordinateType::X, CoordinateType::Y);
-    GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), CoordinateType::X, CoordinateType::Y);
-    RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TMz), GridCoordinate2DTemplate> run (s, p, l, r, 2);
-    run.initMaterials ();
-    secs = run.run ();
-    run.dump ();
-  }
+#define REF_RUN_2D(MODE, MISSING, GA, GB, CA, CB)                                                           \
+  if (st == SchemeType::MODE)                                                                                \
+  {                                                                                                          \
+    GridCoordinate2D s = GRID_COORDINATE_2D (SOLVER_SETTINGS.getSize##GA (), SOLVER_SETTINGS.getSize##GB (), CA, CB); \
+    GridCoordinate2D p = GRID_COORDINATE_2D (SOLVER_SETTINGS.getPMLSize##GA (), SOLVER_SETTINGS.getPMLSize##GB (), CA, CB); \
+    GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeft##GA (), SOLVER_SETTINGS.getTFSFSizeLeft##GB (), CA, CB); \
+    GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRight##GA (), SOLVER_SETTINGS.getTFSFSizeRight##GB (), CA, CB); \
+    RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate> run (s, p, l, r, 2, MISSING); \
+    run.initMaterials ();                                                                                    \
+    secs = run.run ();                                                                                       \
+    run.dump ();                                                                                             \
+  }                                                                                                          \
   else
+  /* the planes of main.cpp:817-1000 */
+#ifdef MODE_TEX
+  REF_RUN_2D (Dim2_TEx, 0, Y, Z, CoordinateType::Y, CoordinateType::Z)
+#endif
+#ifdef MODE_TEY
+  REF_RUN_2D (Dim2_TEy, 1, X, Z, CoordinateType::X, CoordinateType::Z)
 #endif
 #ifdef MODE_TEZ
-  if (st == SchemeType::Dim2_TEz)
-  {
-    GridCoordinate2D s = GRID_COORDINATE_2D (SOLVER_SETTINGS.getSizeX (), SOLVER_SETTINGS.getSizeY (), CoordinateType::X, CoordinateType::Y);
-    GridCoordinate2D p = GRID_COORDINATE_2D (SOLVER_SETTINGS.getPMLSizeX (), SOLVER_SETTINGS.getPMLSizeY (), CoordinateType::X, CoordinateType::Y);
-    GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeftX (), SOLVER_SETTINGS.getTFSFSizeLeftY (), CoordinateType::X, CoordinateType::Y);
-    GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), CoordinateType::X, CoordinateType::Y);
-    RefRun<static_cast<SchemeType_t> (SchemeType::Dim2_TEz), GridCoordinate2DTemplate> run (s, p, l, r, 2);
-    run.initMaterials ();
-    secs = run.run ();
-    run.dump ();
-  }
-  else
+  REF_RUN_2D (Dim2_TEz, 2, X, Y, CoordinateType::X, CoordinateType::Y)
+#endif
+#ifdef MODE_TMX
+  REF_RUN_2D (Dim2_TMx, 0, Y, Z, CoordinateType::Y, CoordinateType::Z)
+#endif
+#ifdef MODE_TMY
+  REF_RUN_2D (Dim2_TMy, 1, X, Z, CoordinateType::X, CoordinateType::Z)
+#endif
+#ifdef MODE_TMZ
+  REF_RUN_2D (Dim2_TMz, 2, X, Y, CoordinateType::X, CoordinateType::Y)
 #endif
   {
     fprintf (stderr, "scheme not driven by this build\n");

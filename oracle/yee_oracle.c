@@ -59,7 +59,9 @@ struct oracle
 {
   yee_config cfg;
   int dim;                 /* 3 or 2 */
-  int64_t n[3];            /* n[2] == 1 for TMz */
+  int act[3];              /* axes of the scheme: a 2D scheme has one axis of extent 1, which it lacks */
+  int64_t lo[3], hi[3];    /* updated range per axis: [1, n-1), or [0, 1) along the missing axis */
+  int64_t n[3];            /* n[a] == 1 along the missing axis of a 2D scheme */
   int64_t ncell;
   int present[6];
   int use_pml, use_tfsf, use_meta;
@@ -102,7 +104,7 @@ static void o_real_coord (const oracle *o, int comp, int64_t i, int64_t j, int64
   int64_t p[3] = { i, j, k };
   for (int a = 0; a < 3; ++a)
   {
-    out[a] = a < o->dim ? (double) p[a] + o_min_coord (comp, a) : 0.0;
+    out[a] = o->act[a] ? (double) p[a] + o_min_coord (comp, a) : 0.0;
   }
 }
 
@@ -111,7 +113,7 @@ static void o_eps_index (const oracle *o, const double real[3], int64_t out[3])
 {
   for (int a = 0; a < 3; ++a)
   {
-    out[a] = a < o->dim ? (int64_t) (real[a] - 0.5) : 0;
+    out[a] = o->act[a] ? (int64_t) (real[a] - 0.5) : 0;
   }
 }
 
@@ -136,7 +138,7 @@ static int o_material_cells (const oracle *o, int comp, int64_t i, int64_t j, in
   {
     double lo[3] = { real[0], real[1], real[2] };
     double hi[3] = { real[0], real[1], real[2] };
-    if (c->axis < o->dim)
+    if (o->act[c->axis])
     {
       lo[c->axis] -= 0.5;
       hi[c->axis] += 0.5;
@@ -152,8 +154,8 @@ static int o_material_cells (const oracle *o, int comp, int64_t i, int64_t j, in
   for (int q = 0; q < 4; ++q)
   {
     double r[3] = { real[0], real[1], real[2] };
-    if (b < o->dim) r[b] += sb[q];
-    if (cc < o->dim) r[cc] += sc[q];
+    if (o->act[b]) r[b] += sb[q];
+    if (o->act[cc]) r[cc] += sc[q];
     o_eps_index (o, r, cells[q]);
   }
   return 4;
@@ -229,7 +231,7 @@ static int o_tfsf_pred (const oracle *o, int comp, int dir, const double real[3]
   const comp_desc *c = &CD[comp];
   int b = dir / 2;
   int low = (dir % 2) == 0;
-  if (b == c->axis || b >= o->dim)
+  if (b == c->axis || !o->act[b])
   {
     return 0;
   }
@@ -238,14 +240,14 @@ static int o_tfsf_pred (const oracle *o, int comp, int dir, const double real[3]
   if (!c->is_h)
   {
     cond_b = low ? o_border_eq (real[b], o->bl[b], -5) : o_border_eq (real[b], o->br[b], 5);
-    if (c->axis < o->dim) cond_a = o_border_range (real[c->axis], o->bl[c->axis], -1, o->br[c->axis], 1);
-    if (third < o->dim) cond_t = o_border_range (real[third], o->bl[third], 4, o->br[third], -4);
+    if (o->act[c->axis]) cond_a = o_border_range (real[c->axis], o->bl[c->axis], -1, o->br[c->axis], 1);
+    if (o->act[third]) cond_t = o_border_range (real[third], o->bl[third], 4, o->br[third], -4);
   }
   else
   {
     cond_b = low ? o_border_eq (real[b], o->bl[b], 0) : o_border_eq (real[b], o->br[b], 0);
-    if (c->axis < o->dim) cond_a = o_border_range (real[c->axis], o->bl[c->axis], 4, o->br[c->axis], -4);
-    if (third < o->dim) cond_t = o_border_range (real[third], o->bl[third], -1, o->br[third], 1);
+    if (o->act[c->axis]) cond_a = o_border_range (real[c->axis], o->bl[c->axis], 4, o->br[c->axis], -4);
+    if (o->act[third]) cond_t = o_border_range (real[third], o->bl[third], -1, o->br[third], 1);
   }
   return cond_b && cond_a && cond_t;
 }
@@ -255,6 +257,20 @@ static int o_tfsf_pred (const oracle *o, int comp, int dir, const double real[3]
 static double o_project (const oracle *o, int comp, double v)
 {
   double a1 = o->a1, a2 = o->a2, a3 = o->a3;
+  const double s1 = sin (a1), c1 = cos (a1);
+  switch (o->cfg.scheme)
+  {
+    case YEE_SCHEME_TEX:   /* YeeGridLayout.h:1761-1846 */
+      return comp == YEE_EY ? v * (-c1) : comp == YEE_EZ ? v * s1 : comp == YEE_HX ? v : 0;
+    case YEE_SCHEME_TEY:   /* :1847-1932 */
+      return comp == YEE_EX ? v * (-c1) : comp == YEE_EZ ? v * s1 : comp == YEE_HY ? -v : 0;
+    case YEE_SCHEME_TMX:   /* :2025-2104 */
+      return comp == YEE_EX ? v : comp == YEE_HY ? v * c1 : comp == YEE_HZ ? -v * s1 : 0;
+    case YEE_SCHEME_TMY:   /* :2105-2190 */
+      return comp == YEE_EY ? -v : comp == YEE_HX ? v * c1 : comp == YEE_HZ ? -v * s1 : 0;
+    default:
+      break;
+  }
   if (o->dim == 2 && o->cfg.scheme == YEE_SCHEME_TEZ)
   {
     switch (comp)
@@ -298,14 +314,22 @@ static double o_incident (const oracle *o, const double real[3], double ddiff, c
   double d;
   double x = real[0] - o->zero_inc[0];
   double y = real[1] - o->zero_inc[1];
+  double z = real[2] - o->zero_inc[2];
   if (o->dim == 3)
   {
-    double z = real[2] - o->zero_inc[2];
     d = x * sin (o->a1) * cos (o->a2) + y * sin (o->a1) * sin (o->a2) + z * cos (o->a1) - ddiff;
+  }
+  else if (!o->act[0])
+  {
+    d = y * sin (o->a1) + z * cos (o->a1) - ddiff;   /* Dim2_TEx / TMx (InternalScheme.cpp:210-222) */
+  }
+  else if (!o->act[1])
+  {
+    d = x * sin (o->a1) + z * cos (o->a1) - ddiff;   /* Dim2_TEy / TMy (:227-239) */
   }
   else
   {
-    d = x * cos (o->a2) + y * sin (o->a2) - ddiff;
+    d = x * cos (o->a2) + y * sin (o->a2) - ddiff;   /* Dim2_TEz / TMz (:244-256) */
   }
   double c1 = (double) ((int64_t) d);
   double c2 = c1 + 1;
@@ -489,17 +513,26 @@ double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, doubl
 
 oracle *oracle_create (const yee_config *cfg)
 {
-  if (!cfg || cfg->layout != 0
-      || (cfg->scheme != YEE_SCHEME_DIM3 && cfg->scheme != YEE_SCHEME_TMZ && cfg->scheme != YEE_SCHEME_TEZ))
+  if (!cfg || cfg->layout != 0 || cfg->scheme < YEE_SCHEME_TEX || cfg->scheme > YEE_SCHEME_DIM3)
   {
     return NULL;
   }
   oracle *o = (oracle *) calloc (1, sizeof (oracle));
   o->cfg = *cfg;
   o->dim = cfg->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
-  o->n[0] = cfg->size[0];
-  o->n[1] = cfg->size[1];
-  o->n[2] = o->dim == 3 ? cfg->size[2] : 1;
+  /* 2D schemes lack one axis (main.cpp:817-1000: TEx/TMx on (y, z), TEy/TMy on (x, z), TEz/TMz on
+   * (x, y)); the grids keep the order of the remaining two, which a 3D index with extent 1 along
+   * the missing axis reproduces */
+  const int missing = cfg->scheme == YEE_SCHEME_DIM3 ? -1
+                      : (cfg->scheme == YEE_SCHEME_TEX || cfg->scheme == YEE_SCHEME_TMX) ? 0
+                      : (cfg->scheme == YEE_SCHEME_TEY || cfg->scheme == YEE_SCHEME_TMY) ? 1 : 2;
+  for (int a = 0; a < 3; ++a)
+  {
+    o->act[a] = a != missing;
+    o->n[a] = o->act[a] ? cfg->size[a] : 1;
+    o->lo[a] = o->act[a] ? 1 : 0;
+    o->hi[a] = o->act[a] ? o->n[a] - 1 : 1;
+  }
   o->ncell = o->n[0] * o->n[1] * o->n[2];
   o->use_pml = (cfg->flags & YEE_USE_PML) != 0;
   o->use_tfsf = (cfg->flags & YEE_USE_TFSF) != 0;
@@ -513,9 +546,14 @@ oracle *oracle_create (const yee_config *cfg)
   for (int c = 0; c < 6; ++c)
   {
     /* InternalScheme component flags (InternalScheme.inc.h:3243-3272): TMz Ez, Hx, Hy; TEz Ex, Ey, Hz */
+    /* TE<a>: H along the missing axis a and the two E components across it; TM<a> the reverse */
     if (o->dim == 3) o->present[c] = 1;
-    else if (cfg->scheme == YEE_SCHEME_TMZ) o->present[c] = (c == YEE_EZ || c == YEE_HX || c == YEE_HY);
-    else o->present[c] = (c == YEE_EX || c == YEE_EY || c == YEE_HZ);
+    else
+    {
+      const int te = cfg->scheme <= YEE_SCHEME_TEZ;
+      const int along = (c % 3) == missing;
+      o->present[c] = (c >= 3) == te ? along : !along;
+    }
   }
   for (int m = 0; m < 6; ++m)
   {
@@ -559,6 +597,14 @@ oracle *oracle_create (const yee_config *cfg)
     o->zero_inc[1] = cfg->tfsf_left[1] - 2.5 * sin (o->a1) * sin (o->a2);
     o->zero_inc[2] = cfg->tfsf_left[2] - 2.5 * cos (o->a1);
   }
+  else if (!o->act[0] || !o->act[1])
+  {
+    /* TEx, TMx (y, z) and TEy, TMy (x, z): YeeGridLayout.cpp:2332-2333, 2385-2386, 2491-2492, 2544-2545 */
+    const int first = o->act[0] ? 0 : 1;
+    o->zero_inc[first] = cfg->tfsf_left[first] - 2.5 * sin (o->a1);
+    o->zero_inc[2] = cfg->tfsf_left[2] - 2.5 * cos (o->a1);
+    o->zero_inc[o->act[0] ? 1 : 0] = 0;
+  }
   else
   {
     o->zero_inc[0] = cfg->tfsf_left[0] - 2.5 * cos (o->a2);
@@ -568,7 +614,7 @@ oracle *oracle_create (const yee_config *cfg)
   if (o->use_tfsf)
   {
     /* allocateGridsInc: 500 * size.x cells, 2 layers (InternalScheme.specific.h:99-107) */
-    o->inc_len = 500 * o->n[0];
+    o->inc_len = 500 * (o->act[0] ? o->n[0] : o->n[1]);   /* size.get1 () of the scheme's grid */
     for (int l = 0; l < 2; ++l)
     {
       o->einc[l] = (double *) calloc (o->inc_len, sizeof (double));
@@ -611,8 +657,9 @@ int oracle_prepare (oracle *o)
   if (!o) return -1;
   if (o->use_pml)
   {
-    for (int a = 0; a < o->dim; ++a)
+    for (int a = 0; a < 3; ++a)
     {
+      if (!o->act[a]) continue;
       o->sigma[a] = (double *) calloc (o->ncell, sizeof (double));
       double P = (double) o->cfg.pml[a];
       double boundary = P * o->dx;
@@ -710,14 +757,14 @@ static void o_tfsf (const oracle *o, int comp, int64_t i, int64_t j, int64_t k,
   if (u11 || u12)
   {
     double r[3];
-    o_real_coord (o, c->opp1, i + p1[0], j + p1[1], k + (o->dim == 3 ? p1[2] : 0), r);
+    o_real_coord (o, c->opp1, i + o->act[0] * p1[0], j + o->act[1] * p1[1], k + o->act[2] * p1[2], r);
     diff1 = o_project (o, c->opp1, o_incident (o, r, ddiff, inc));
     if (c->is_h) diff1 *= -1.0;
   }
   if (u21 || u22)
   {
     double r[3];
-    o_real_coord (o, c->opp2, i + p2[0], j + p2[1], k + (o->dim == 3 ? p2[2] : 0), r);
+    o_real_coord (o, c->opp2, i + o->act[0] * p2[0], j + o->act[1] * p2[1], k + o->act[2] * p2[2], r);
     diff2 = o_project (o, c->opp2, o_incident (o, r, ddiff, inc));
     if (c->is_h) diff2 *= -1.0;
   }
@@ -735,7 +782,7 @@ static void o_tfsf (const oracle *o, int comp, int64_t i, int64_t j, int64_t k,
 
 static double o_sig (const oracle *o, int axis, int comp, int64_t i, int64_t j, int64_t k)
 {
-  if (axis >= o->dim) return 0;   /* SigmaZ is absent in TMz: material reads as 0 */
+  if (!o->act[axis]) return 0;   /* the sigma of the missing axis of a 2D scheme reads as 0 */
   return o_get_material (o, comp, i, j, k, o->sigma[axis]);
 }
 
@@ -749,18 +796,17 @@ static void o_field_step (oracle *o, int comp, int32_t t)
   const double *omg = c->is_h ? o->mat[YEE_MAT_OMEGAPM] : o->mat[YEE_MAT_OMEGAPE];
   const double *gam = c->is_h ? o->mat[YEE_MAT_GAMMAM] : o->mat[YEE_MAT_GAMMAE];
   const double dt = o->dt, dx = o->dx;
-  int64_t k0 = o->dim == 3 ? 1 : 0, k1 = o->dim == 3 ? o->n[2] - 1 : 1;
   int o1 = c->opp1, o2 = c->opp2;
-  int dz = o->dim == 3;
+  const int dx_ = o->act[0], dy_ = o->act[1], dz = o->act[2];   /* offsets along a missing axis drop */
 
   if (t > 0)
   {
     double *out = o->use_pml ? o->d[comp][0] : o->f[comp][0];
     const double *prev = o->use_pml ? o->d[comp][1] : o->f[comp][1];
     /* sweep 1: calculateFieldStepIteration (InternalScheme.inc.h:2351-2417) */
-    for (int64_t i = 1; i < o->n[0] - 1; ++i)
-      for (int64_t j = 1; j < o->n[1] - 1; ++j)
-        for (int64_t k = k0; k < k1; ++k)
+    for (int64_t i = o->lo[0]; i < o->hi[0]; ++i)
+      for (int64_t j = o->lo[1]; j < o->hi[1]; ++j)
+        for (int64_t k = o->lo[2]; k < o->hi[2]; ++k)
         {
           int64_t q = o_idx (o, i, j, k);
           double val = prev[q];
@@ -779,10 +825,10 @@ static void o_field_step (oracle *o, int comp, int32_t t)
             ca = 1.0;
             cb = dt / (material * modifier * dx);
           }
-          double v11 = o_read1 (o, o1, i + c->d11[0], j + c->d11[1], k + dz * c->d11[2]);
-          double v12 = o_read1 (o, o1, i + c->d12[0], j + c->d12[1], k + dz * c->d12[2]);
-          double v21 = o_read1 (o, o2, i + c->d21[0], j + c->d21[1], k + dz * c->d21[2]);
-          double v22 = o_read1 (o, o2, i + c->d22[0], j + c->d22[1], k + dz * c->d22[2]);
+          double v11 = o_read1 (o, o1, i + dx_ * c->d11[0], j + dy_ * c->d11[1], k + dz * c->d11[2]);
+          double v12 = o_read1 (o, o1, i + dx_ * c->d12[0], j + dy_ * c->d12[1], k + dz * c->d12[2]);
+          double v21 = o_read1 (o, o2, i + dx_ * c->d21[0], j + dy_ * c->d21[1], k + dz * c->d21[2]);
+          double v22 = o_read1 (o, o2, i + dx_ * c->d22[0], j + dy_ * c->d22[1], k + dz * c->d22[2]);
           if (o->use_tfsf)
           {
             o_tfsf (o, comp, i, j, k, &v11, &v12, &v21, &v22);
@@ -799,9 +845,9 @@ static void o_field_step (oracle *o, int comp, int32_t t)
         /* sweep 2: calculateFieldStepIterationPMLMetamaterials (InternalScheme.inc.h:2452-2549) */
         double *d0 = o->d[comp][0], *dp = o->d[comp][1], *dpp = o->d[comp][2];
         double *m0 = o->d1[comp][0], *mp = o->d1[comp][1], *mpp = o->d1[comp][2];
-        for (int64_t i = 1; i < o->n[0] - 1; ++i)
-          for (int64_t j = 1; j < o->n[1] - 1; ++j)
-            for (int64_t k = k0; k < k1; ++k)
+        for (int64_t i = o->lo[0]; i < o->hi[0]; ++i)
+          for (int64_t j = o->lo[1]; j < o->hi[1]; ++j)
+            for (int64_t k = o->lo[2]; k < o->hi[2]; ++k)
             {
               int64_t q = o_idx (o, i, j, k);
               double m1, m2;
@@ -822,9 +868,9 @@ static void o_field_step (oracle *o, int comp, int32_t t)
       const double *prv = o->use_meta ? o->d1[comp][1] : o->d[comp][1];
       double *e0 = o->f[comp][0];
       const double *e1 = o->f[comp][1];
-      for (int64_t i = 1; i < o->n[0] - 1; ++i)
-        for (int64_t j = 1; j < o->n[1] - 1; ++j)
-          for (int64_t k = k0; k < k1; ++k)
+      for (int64_t i = o->lo[0]; i < o->hi[0]; ++i)
+        for (int64_t j = o->lo[1]; j < o->hi[1]; ++j)
+          for (int64_t k = o->lo[2]; k < o->hi[2]; ++k)
           {
             int64_t q = o_idx (o, i, j, k);
             double material1 = o_get_material (o, comp, i, j, k, mgrid);
@@ -845,7 +891,8 @@ static void o_field_step (oracle *o, int comp, int32_t t)
   /* performPointSourceCalc (InternalScheme.inc.h:2818-2895), also at t == 0 */
   if (o->cfg.point_source & (1u << comp))
   {
-    int64_t p[3] = { o->cfg.point_source_pos[0], o->cfg.point_source_pos[1], o->dim == 3 ? o->cfg.point_source_pos[2] : 0 };
+    int64_t p[3];
+    for (int a = 0; a < 3; ++a) p[a] = o->act[a] ? o->cfg.point_source_pos[a] : 0;
     if (p[0] >= 0 && p[0] < o->n[0] && p[1] >= 0 && p[1] < o->n[1] && p[2] >= 0 && p[2] < o->n[2])
     {
       double ts = c->is_h ? t + 1.0 : t + 0.5;
@@ -967,9 +1014,9 @@ int oracle_get_scattered (oracle *o, int32_t comp, double *out)
         double real[3];
         o_real_coord (o, comp, i, j, k, real);
         int skip = 0;
-        for (int a = 0; a < o->dim; ++a)
+        for (int a = 0; a < 3; ++a)
         {
-          if (real[a] < o->bl[a] || real[a] > o->br[a]) skip = 1;
+          if (o->act[a] && (real[a] < o->bl[a] || real[a] > o->br[a])) skip = 1;
         }
         const int64_t q = o_idx (o, i, j, k);
         const double v = o->f[comp][1][q];

@@ -69,6 +69,23 @@ def cases(tmp):
     out["tez_pml_pointsource"] = (
         "--2d-tez --size x:30,y:26 --use-pml --pml-size x:4,y:5 --point-source x:13,y:11 --point-source-hz "
         "--point-source-ex --angle-psi 0 --time-steps 50").split()
+    # the (y, z) and (x, z) planes: TMx / TEx assert phi = 90, TMy / TEy phi = 0
+    out["tmx_pml_tfsf_sphere"] = (
+        "--2d-tmx --size x:7,y:30,z:26 --use-pml --pml-size y:4,z:4 --use-tfsf --tfsf-size-left y:7,z:6 "
+        "--tfsf-size-right y:6,z:8 --angle-teta 35 --angle-phi 90 --angle-psi 0 --eps-sphere 3 "
+        "--eps-sphere-center x:3,y:15,z:13 --eps-sphere-radius 5 --time-steps 50").split()
+    out["tex_pml_tfsf_pointsource"] = (
+        "--2d-tex --size x:5,y:28,z:24 --use-pml --pml-size y:3,z:4 --use-tfsf --tfsf-size-left y:7,z:6 "
+        "--tfsf-size-right y:6,z:7 --angle-teta 55 --angle-phi 90 --angle-psi 90 --point-source x:0,y:12,z:10 "
+        "--point-source-hx --point-source-ez --time-steps 50").split()
+    out["tmy_pml_tfsf_pointsource"] = (
+        "--2d-tmy --size x:27,y:7,z:25 --use-pml --pml-size x:4,z:3 --use-tfsf --tfsf-size-left x:7,z:6 "
+        "--tfsf-size-right x:6,z:7 --angle-teta 40 --angle-phi 0 --angle-psi 0 --point-source x:12,y:0,z:11 "
+        "--point-source-ey --time-steps 50").split()
+    out["tey_pml_tfsf_sphere"] = (
+        "--2d-tey --size x:25,y:6,z:29 --use-pml --pml-size x:3,z:4 --use-tfsf --tfsf-size-left x:6,z:7 "
+        "--tfsf-size-right x:7,z:6 --angle-teta 25 --angle-phi 0 --angle-psi 90 --eps-sphere 2.5 "
+        "--eps-sphere-center x:12,y:2,z:14 --eps-sphere-radius 5 --time-steps 50").split()
     return out
 
 

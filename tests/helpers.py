@@ -56,7 +56,7 @@ def rel_l2(a, b):
 
 
 def field_names(cfg, aux=False):
-    base = FIELDS_3D if cfg.dim == 3 else (FIELDS_TEZ if cfg.scheme == 9 else FIELDS_TMZ)
+    base = list(cfg.components)
     names = list(base)
     if aux and cfg.use_pml:
         names += [("D" if n[0] == "E" else "B") + n[1] for n in base]

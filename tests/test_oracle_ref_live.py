@@ -39,6 +39,20 @@ CASES = [
     "--2d-tez --size x:26,y:31 --use-pml --pml-size x:3,y:5 --point-source x:11,y:14 --point-source-ex "
     "--point-source-ey --point-source-hz --mu-sphere 2 --mu-sphere-center x:13,y:15 --mu-sphere-radius 4 "
     "--angle-psi 0 --time-steps 45",
+    # TMx, TEx on the (y, z) plane (the layouts assert phi = 90 and psi = 0 / 90), TMy, TEy on
+    # (x, z) (phi = 0): oblique TF/SF, PML, spheres, point sources
+    "--2d-tmx --size x:9,y:26,z:30 --use-pml --pml-size y:4,z:3 --use-tfsf --tfsf-size-left y:7,z:8 "
+    "--tfsf-size-right y:8,z:7 --angle-teta 35 --angle-phi 90 --angle-psi 0 --eps-sphere 3 "
+    "--eps-sphere-center x:4,y:13,z:15 --eps-sphere-radius 5 --time-steps 50",
+    "--2d-tex --size x:5,y:28,z:24 --use-pml --pml-size y:3,z:4 --use-tfsf --tfsf-size-left y:7,z:6 "
+    "--tfsf-size-right y:6,z:7 --angle-teta 55 --angle-phi 90 --angle-psi 90 --point-source x:0,y:12,z:10 "
+    "--point-source-hx --point-source-ez --time-steps 50",
+    "--2d-tmy --size x:27,y:7,z:25 --use-pml --pml-size x:4,z:3 --use-tfsf --tfsf-size-left x:7,z:6 "
+    "--tfsf-size-right x:6,z:7 --angle-teta 40 --angle-phi 0 --angle-psi 0 --mu-sphere 2 "
+    "--mu-sphere-center x:13,y:3,z:12 --mu-sphere-radius 4 --time-steps 50",
+    "--2d-tey --size x:25,y:6,z:29 --use-pml --pml-size x:3,z:4 --use-tfsf --tfsf-size-left x:6,z:7 "
+    "--tfsf-size-right x:7,z:6 --angle-teta 25 --angle-phi 0 --angle-psi 90 --point-source x:11,y:0,z:13 "
+    "--point-source-hy --time-steps 50",
     # TEz at phi = 0 without PML
     "--2d-tez --size x:24,y:20 --use-tfsf --tfsf-size-left x:5,y:5 --tfsf-size-right x:5,y:5 --angle-teta 90 "
     "--angle-phi 0 --angle-psi 0 --time-steps 40",
