@@ -123,6 +123,21 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
     a.run_lo[ax] = 1;
     a.run_hi[ax] = (int) h->n[ax] - 1;
   }
+  if (h->miss >= 0)
+  {
+    // a 2D scheme on the 3D kernels: its one cell along the missing axis is updated, a
+    // neighbour along that axis is the cell itself (stride 0), and the PML and TF/SF tests
+    // never fire along it
+    const int m = h->miss;
+    a.run_lo[m] = 0;
+    a.run_hi[m] = 1;
+    if (m == 0) a.sx = 0;
+    else a.sy = 0;
+    a.slab_lo[m] = -1;
+    a.slab_hi[m] = 1 << 30;
+    a.tl[m] = -(1 << 28);
+    a.tr[m] = 1 << 28;
+  }
   a.eps = (const T *) h->mat[YEE_MAT_EPS];
   a.mu = (const T *) h->mat[YEE_MAT_MU];
   a.ope = (const T *) h->mat[YEE_MAT_OMEGAPE];
@@ -274,7 +289,7 @@ int launch_sources (yee_handle *h, bool e_half, int32_t t, int buf, bool skip_bo
   const int c0 = e_half ? 0 : 3;
   for (int c = c0; c < c0 + 3; ++c)
   {
-    if (!(h->cfg.point_source & (1u << c)) || !h->present[c]) continue;
+    if (!(h->cfg.point_source & (1u << c)) || !h->exposed[c]) continue;
     const int64_t *p = h->cfg.point_source_pos;
     const int64_t pz = h->dim == 3 ? p[2] : 0;
     // getFieldValueOrNullByAbsolutePos: positions outside the grid are ignored
@@ -550,13 +565,15 @@ int upload (yee_handle *h, void **dst, const double *src, size_t n)
 int validate (const yee_config *c)
 {
   if (!c) return fail (YEE_EINVAL, "null config");
-  if (c->scheme != YEE_SCHEME_DIM3 && c->scheme != YEE_SCHEME_TMZ && c->scheme != YEE_SCHEME_TEZ)
-    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13), YEE_SCHEME_TMZ (12) or YEE_SCHEME_TEZ (9)");
+  if (c->scheme < YEE_SCHEME_TEX || c->scheme > YEE_SCHEME_DIM3)
+    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13) or a 2D scheme TEx .. TMz (7 .. 12)");
   if (c->layout != 0) return fail (YEE_EINVAL, "only the E_CENTERED layout (0) is supported");
   if (c->precision != 4 && c->precision != 8) return fail (YEE_EINVAL, "precision must be 4 or 8");
   const int dim = c->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
-  for (int a = 0; a < dim; ++a)
+  const int miss = missing_axis (*c);
+  for (int a = 0; a < 3; ++a)
   {
+    if (a == miss) continue;
     if (c->size[a] < 3 || c->size[a] > (1 << 20)) return fail (YEE_EINVAL, "size out of range");
     // the PML sizes matter only with PML on (a 3-cell grid keeps the default size 2 otherwise)
     if ((c->flags & YEE_USE_PML) && (c->pml[a] < 0 || 2 * c->pml[a] > c->size[a]))
@@ -565,7 +582,9 @@ int validate (const yee_config *c)
         && (c->tfsf_left[a] < 1 || c->tfsf_right[a] < 1 || c->tfsf_left[a] + c->tfsf_right[a] >= c->size[a]))
       return fail (YEE_EINVAL, "TF/SF box out of range");
   }
-  int64_t cells = c->size[0] * c->size[1] * (dim == 3 ? c->size[2] : 1);
+  int64_t cells = 1;
+  for (int a = 0; a < 3; ++a) cells *= a == miss ? 1 : c->size[a];
+  (void) dim;
   if (cells > (int64_t) 1 << 34) return fail (YEE_EINVAL, "grid too large");
   if (!(c->dx > 0) || !(c->wavelength > 0) || !(c->courant > 0))
     return fail (YEE_EINVAL, "dx, wavelength and courant factor must be positive");
@@ -746,17 +765,33 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (rc) return rc;
   yee_handle *h = new yee_handle ();
   h->cfg = *cfg;
-  h->dim = cfg->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
-  h->n[0] = cfg->size[0];
-  h->n[1] = cfg->size[1];
-  h->n[2] = h->dim == 3 ? cfg->size[2] : 1;
+  // 2D schemes on the (y, z) and (x, z) planes run on the 3D kernels with the missing axis of
+  // extent 1 (neighbours along it are the cell itself, the offsets the reference drops); TEz
+  // and TMz have kernels of their own
+  const int miss = missing_axis (*cfg);
+  h->miss = miss == 2 ? -1 : miss;
+  h->dim = (cfg->scheme == YEE_SCHEME_DIM3 || h->miss >= 0) ? 3 : 2;
+  for (int a = 0; a < 3; ++a) h->n[a] = a == miss ? 1 : cfg->size[a];
+  if (miss >= 0)
+  {
+    h->cfg.pml[miss] = 0;                  // no PML and no TF/SF border along the missing axis
+    h->cfg.point_source_pos[miss] = 0;     // the source position drops that coordinate
+  }
   h->ncell = h->n[0] * h->n[1] * h->n[2];
   h->wsize = cfg->precision == 4 ? 4 : 8;
   h->mode = h->use_meta () ? 2 : (h->use_pml () ? 1 : 0);
   for (int c = 0; c < 6; ++c)
-    // InternalScheme component flags (InternalScheme.inc.h:3243-3272)
-    h->present[c] = h->dim == 3 || (cfg->scheme == YEE_SCHEME_TMZ ? (c == YEE_EZ || c == YEE_HX || c == YEE_HY)
-                                                                : (c == YEE_EX || c == YEE_EY || c == YEE_HZ));
+  {
+    // InternalScheme component flags (InternalScheme.inc.h:3243-3272): TE<a> has H along the
+    // missing axis a and E across it, TM<a> the reverse
+    const bool te = cfg->scheme <= YEE_SCHEME_TEZ, along = (c % 3) == miss;
+    h->exposed[c] = miss < 0 || ((c >= 3) == te ? along : !along);
+    // the 3D kernels update all six components; the absent ones stay exactly 0 (their curl
+    // operands cancel or are absent, their incident projections are 0)
+    h->present[c] = h->dim == 3 || h->exposed[c];
+  }
+  for (int c = 0; c < 6; ++c)
+    if (!h->exposed[c]) h->cfg.point_source &= ~(1u << c);
   auto bail = [&] (int code) {
     yee_destroy (h);
     return code;
@@ -772,7 +807,7 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   const char *kern = std::getenv ("YEE_KERNEL");
   const bool want_fused = !(kern && std::strcmp (kern, "split") == 0);
   h->fused = false;
-  if (h->dim == 3 && h->n[2] <= 1024 && want_fused)
+  if (h->dim == 3 && h->miss < 0 && h->n[2] <= 1024 && want_fused)
   {
     bool ok = true;
     for (int ax = 0; ax < 3; ++ax)
@@ -838,7 +873,7 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (h->use_tfsf ())
   {
     // allocateGridsInc: 500 * size.x cells (InternalScheme.specific.h:99-107)
-    h->inc_len = 500 * h->n[0];
+    h->inc_len = 500 * h->n[miss == 0 ? 1 : 0];   // size.get1 () of the scheme's grid
     if ((rc = alloc_zero (h, &h->einc, h->inc_len * h->wsize))) return bail (rc);
     if ((rc = alloc_zero (h, &h->hinc[0], h->inc_len * h->wsize))) return bail (rc);
     if ((rc = alloc_zero (h, &h->hinc[1], h->inc_len * h->wsize))) return bail (rc);
@@ -1027,7 +1062,7 @@ int yee_get_field_raw (yee_handle *h, int32_t comp, void *out)
   if (!h || !out) return fail (YEE_EINVAL, "null argument");
   if (comp < 0 || comp > 17) return fail (YEE_EINVAL, "unknown component");
   const int base = comp % 6;
-  if (!h->present[base]) return fail (YEE_EINVAL, "component not present in this scheme");
+  if (!h->exposed[base]) return fail (YEE_EINVAL, "component not present in this scheme");
   void *src = comp < 6 ? h->f[h->cur][base] : (comp < 12 ? h->d[h->cur][base] : h->d1[h->cur][base]);
   if (!src) return fail (YEE_EINVAL, "component not allocated (enable PML / metamaterials)");
   HIP_TRY (hipStreamSynchronize (h->stream));
@@ -1116,7 +1151,7 @@ extern "C" {
 int yee_get_scattered_field (yee_handle *h, int32_t comp, double *out)
 {
   if (!h || !out) return fail (YEE_EINVAL, "null argument");
-  if (comp < 0 || comp > 5 || !h->present[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
+  if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
   // the reference interpolates EInc / HInc, which exist only with TF/SF
   if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
   if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
@@ -1149,7 +1184,7 @@ int yee_get_mask (yee_handle *h, int32_t kind, int32_t comp, uint8_t *out)
   if (!h || !out) return fail (YEE_EINVAL, "null argument");
   if (kind == 0)
   {
-    if (comp < 0 || comp > 5 || !h->present[comp]) return fail (YEE_EINVAL, "bad component");
+    if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "bad component");
     uint8_t *dev = nullptr;
     HIP_TRY (hipMalloc (&dev, (size_t) h->ncell));
     const int grid = (int) std::min<int64_t> (4096, (h->ncell + 255) / 256);

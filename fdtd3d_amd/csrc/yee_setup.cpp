@@ -89,6 +89,48 @@ Scalars compute_scalars (const yee_config &cfg)
     s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a1) * std::sin (a2);
     s.zero_inc[2] = cfg.tfsf_left[2] - 2.5 * std::cos (a1);
   }
+  else if (missing_axis (cfg) != 2)
+  {
+    // TEx / TMx on (y, z) and TEy / TMy on (x, z), run on the 3D kernels with the missing axis
+    // of extent 1.  Their incident wave d = a sin(theta) + z cos(theta) (InternalScheme.cpp:
+    // 210-239, 261-288) is the 3D expression x S1 C2 + y S1 S2 + z C1 with the factors of the
+    // missing axis set to 0 and the other to 1 (exact: x * S1 * 1.0 == x * S1, + 0.0 is exact).
+    const int miss = missing_axis (cfg);
+    s.S1 = std::sin (a1);
+    s.C1 = std::cos (a1);
+    s.C2 = miss == 0 ? 0.0 : 1.0;
+    s.S2 = miss == 0 ? 1.0 : 0.0;
+    const int first = miss == 0 ? 1 : 0;
+    // YeeGridLayout.cpp:2332-2333, 2385-2386, 2491-2492, 2544-2545
+    s.zero_inc[first] = cfg.tfsf_left[first] - 2.5 * std::sin (a1);
+    s.zero_inc[2] = cfg.tfsf_left[2] - 2.5 * std::cos (a1);
+    s.zero_inc[miss] = 0;
+    for (int c = 0; c < 6; ++c) s.proj[c] = 0.0;
+    // YeeGridLayout.h:1761-1932 (TEx, TEy) and 2025-2190 (TMx, TMy)
+    switch (cfg.scheme)
+    {
+      case YEE_SCHEME_TEX:
+        s.proj[YEE_EY] = -std::cos (a1);
+        s.proj[YEE_EZ] = std::sin (a1);
+        s.proj[YEE_HX] = 1.0;
+        break;
+      case YEE_SCHEME_TEY:
+        s.proj[YEE_EX] = -std::cos (a1);
+        s.proj[YEE_EZ] = std::sin (a1);
+        s.proj[YEE_HY] = -1.0;
+        break;
+      case YEE_SCHEME_TMX:
+        s.proj[YEE_EX] = 1.0;
+        s.proj[YEE_HY] = std::cos (a1);
+        s.proj[YEE_HZ] = -std::sin (a1);
+        break;
+      default:   // TMy
+        s.proj[YEE_EY] = -1.0;
+        s.proj[YEE_HX] = std::cos (a1);
+        s.proj[YEE_HZ] = -std::sin (a1);
+        break;
+    }
+  }
   else if (cfg.scheme == YEE_SCHEME_TEZ)
   {
     // YeeGridLayout.h:1939-2024 and YeeGridLayout.cpp:2424-2440 (TEz); Hz = -valH, i.e. valH * -1
@@ -122,6 +164,17 @@ static double graded_sigma (double dist, double boundary, double dx)
   const double hi = (dist + 1) * dx;
   const double lo = dist * dx;
   return factor * (std::pow (hi, (double) (m + 1)) - std::pow (lo, (double) (m + 1)));
+}
+
+int missing_axis (const yee_config &cfg)
+{
+  switch (cfg.scheme)
+  {
+    case YEE_SCHEME_TEX: case YEE_SCHEME_TMX: return 0;
+    case YEE_SCHEME_TEY: case YEE_SCHEME_TMY: return 1;
+    case YEE_SCHEME_TEZ: case YEE_SCHEME_TMZ: return 2;
+    default: return -1;
+  }
 }
 
 std::vector<double> sigma_profile (const yee_config &cfg, int axis)
@@ -169,18 +222,19 @@ double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p,
 
 std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
 {
-  const int dim = cfg.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  const int miss = missing_axis (cfg);
   std::vector<double> tab ((size_t) 6 * kNumRoles * tab_len, 0.0);
   if (!(cfg.flags & YEE_USE_PML))
   {
     return tab;
   }
   std::vector<double> prof[3];
-  for (int a = 0; a < dim; ++a) prof[a] = sigma_profile (cfg, a);
+  for (int a = 0; a < 3; ++a)
+    if (a != miss) prof[a] = sigma_profile (cfg, a);
   const Scalars s = compute_scalars (cfg);
   const double e0 = eps0 (), dt = s.dt, dx = cfg.dx;
   auto at = [&] (int comp, int axis, int64_t p) -> double {
-    if (axis >= dim) return 0.0;   // SigmaZ is absent in TMz and reads as 0
+    if (axis == miss) return 0.0;   // a 2D scheme has no sigma along its missing axis: reads as 0
     return sigma_at (prof[axis], comp, axis, p, cfg.size[axis]);
   };
   for (int comp = 0; comp < 6; ++comp)
@@ -189,20 +243,20 @@ std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
     double *T = &tab[(size_t) comp * kNumRoles * tab_len];
     for (int64_t p = 0; p < tab_len; ++p)
     {
-      if (c.sig_curl < dim ? p < cfg.size[c.sig_curl] : p == 0)
+      if (c.sig_curl != miss ? p < cfg.size[c.sig_curl] : p == 0)
       {
         const double m = at (comp, c.sig_curl, p);
         const double dd = (2 * e0 * 1.0 + m * dt);
         T[kCAC * tab_len + p] = (2 * e0 * 1.0 - m * dt) / dd;
         T[kCBC * tab_len + p] = (2 * e0 * dt / dx) / dd;
       }
-      if (c.sig4 < dim ? p < cfg.size[c.sig4] : p == 0)
+      if (c.sig4 != miss ? p < cfg.size[c.sig4] : p == 0)
       {
         const double m = at (comp, c.sig4, p);
         T[kP4 * tab_len + p] = 2 * e0 * 1.0 + m * dt;
         T[kM4 * tab_len + p] = 2 * e0 * 1.0 - m * dt;
       }
-      if (c.sig5 < dim ? p < cfg.size[c.sig5] : p == 0)
+      if (c.sig5 != miss ? p < cfg.size[c.sig5] : p == 0)
       {
         const double m = at (comp, c.sig5, p);
         const double dd = (2 * e0 * 1.0 + m * dt);

@@ -51,6 +51,10 @@ struct Scalars
 
 Scalars compute_scalars (const yee_config &cfg);
 
+// The axis a 2D scheme lacks (main.cpp:817-1000: TEx/TMx on (y, z), TEy/TMy on (x, z),
+// TEz/TMz on (x, y)); -1 for 3D.
+int missing_axis (const yee_config &cfg);
+
 double wave_number (double delta, double lambda, double courant, double nlambda, double a1, double a2);
 
 // Sigma of eps-cell p along `axis` (SchemeHelper::initSigmaX/Y/Z, SchemeHelper.h:592-718).

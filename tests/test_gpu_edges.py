@@ -85,6 +85,13 @@ CASES = [
      "--point-source-hz --angle-psi 0 --time-steps 25"),
     ("tez_drude", "--2d-tez --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-tfsf --tfsf-size-left x:9,y:8 "
      "--tfsf-size-right x:9,y:8 --angle-teta 90 --angle-phi 30 --angle-psi 0 --use-metamaterials --time-steps 300"),
+    # TMx / TEy (the y-z and x-z planes, run on the 3D kernels): smallest grids, a Drude square
+    ("min_tmx", "--2d-tmx --size x:4,y:3,z:3 --point-source x:2,y:1,z:1 --point-source-ex --point-source-hy "
+     "--angle-phi 90 --angle-psi 0 --time-steps 9"),
+    ("min_tey", "--2d-tey --size x:3,y:5,z:3 --point-source x:1,y:3,z:1 --point-source-hy --angle-phi 0 "
+     "--time-steps 9"),
+    ("tmy_drude", "--2d-tmy --size x:40,y:4,z:36 --use-pml --pml-size x:6,z:6 --use-tfsf --tfsf-size-left x:9,z:8 "
+     "--tfsf-size-right x:9,z:8 --angle-teta 30 --angle-phi 0 --angle-psi 0 --use-metamaterials --time-steps 300"),
     # a point source off the grid is ignored
     ("source_off_grid", "--3d --size x:12,y:10,z:14 --use-pml --pml-size x:2,y:2,z:2 --point-source x:30,y:4,z:5 "
      "--point-source-ez --time-steps 15"),
@@ -92,7 +99,7 @@ CASES = [
 
 
 def _materials(name, cfg):
-    if name in ("tmz_drude", "tez_drude"):
+    if name in ("tmz_drude", "tez_drude", "tmy_drude"):
         w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
         om = np.zeros(cfg.shape)
         ga = np.zeros(cfg.shape)

@@ -75,6 +75,10 @@ MASK_CASES = [
     "--angle-teta 90 --angle-phi 30 --angle-psi 90 --time-steps 1",
     "--2d-tez --size x:23,y:29 --use-tfsf --tfsf-size-left x:5,y:6 --tfsf-size-right x:4,y:7 "
     "--angle-teta 90 --angle-phi 30 --angle-psi 0 --time-steps 1",
+    "--2d-tmx --size x:9,y:23,z:27 --use-tfsf --tfsf-size-left y:5,z:6 --tfsf-size-right y:4,z:7 "
+    "--angle-teta 35 --angle-phi 90 --angle-psi 0 --time-steps 1",
+    "--2d-tey --size x:26,y:9,z:21 --use-tfsf --tfsf-size-left x:5,z:6 --tfsf-size-right x:6,z:4 "
+    "--angle-teta 50 --angle-phi 0 --angle-psi 90 --time-steps 1",
 ]
 
 
