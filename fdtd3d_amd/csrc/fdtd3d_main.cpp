@@ -126,8 +126,7 @@ const std::set<std::string> kIgnoredFlags = {
   "--ntff-to-stdout",
 };
 const std::set<std::string> kUnsupported = {
-  "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy", "--2d-tex", "--2d-tey",
-  "--2d-tmx", "--2d-tmy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
+  "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
   "--current-source-jx", "--current-source-jy", "--current-source-jz", "--current-source-mx",
   "--current-source-my", "--current-source-mz",
   // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
@@ -177,6 +176,10 @@ Options parse (int argc, char **argv)
     else if (a == "--3d") c.scheme = YEE_SCHEME_DIM3;
     else if (a == "--2d-tmz") c.scheme = YEE_SCHEME_TMZ;
     else if (a == "--2d-tez") c.scheme = YEE_SCHEME_TEZ;
+    else if (a == "--2d-tmx") c.scheme = YEE_SCHEME_TMX;
+    else if (a == "--2d-tmy") c.scheme = YEE_SCHEME_TMY;
+    else if (a == "--2d-tex") c.scheme = YEE_SCHEME_TEX;
+    else if (a == "--2d-tey") c.scheme = YEE_SCHEME_TEY;
     else if (a == "--size") parse_coord (val (), 100, c.size);
     else if (a == "--same-size") copy3 (c.size);
     else if (a == "--pml-size") parse_coord (val (), 2, c.pml);
@@ -256,7 +259,7 @@ Options parse (int argc, char **argv)
     if (done) continue;
     if (kIgnoredWithArg.count (a)) (void) val ();
     else if (kIgnoredFlags.count (a)) continue;
-    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (3D, TMz and TEz, E_CENTERED, real fields)");
+    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (3D and 2D, E_CENTERED, real fields)");
     else die ("unknown option " + a);
   }
   return o;
@@ -292,18 +295,52 @@ int min_coord2 (int comp, int axis)
   return m[comp][axis];
 }
 
-// Dump range [start, end) of a component (Scheme::getStartCoord / getEndCoord, Scheme.inc.h:4647-4790;
-// the middle plane along the orthogonal axis in 3D, SchemeHelper.h:860-990).  comp -1: a material
-// grid, whose minimum coordinate is taken as 0 there.
+// The scheme's axes in the order of its coordinates: x, y, z in 3D, the two axes a 2D scheme has
+// (main.cpp:817-1000: TEx/TMx y, z; TEy/TMy x, z; TEz/TMz x, y).
+std::vector<int> scheme_axes (const yee_config &c)
+{
+  switch (c.scheme)
+  {
+    case YEE_SCHEME_TEX: case YEE_SCHEME_TMX: return { 1, 2 };
+    case YEE_SCHEME_TEY: case YEE_SCHEME_TMY: return { 0, 2 };
+    case YEE_SCHEME_TEZ: case YEE_SCHEME_TMZ: return { 0, 1 };
+    default: return { 0, 1, 2 };
+  }
+}
+
+// Whether a 2D scheme has component comp: TE<a> has H along the missing axis a and E across it,
+// TM<a> the reverse (InternalScheme.inc.h:3243-3272).
+bool scheme_has (const yee_config &c, int comp)
+{
+  if (c.scheme == YEE_SCHEME_DIM3) return true;
+  const std::vector<int> ax = scheme_axes (c);
+  const int miss = 3 - ax[0] - ax[1];
+  const bool te = c.scheme <= YEE_SCHEME_TEZ, along = (comp % 3) == miss;
+  return (comp >= 3) == te ? along : !along;
+}
+
+// Grid extent per scheme coordinate (1 past the scheme's dimension).
+void scheme_size (const yee_config &c, int64_t size[3])
+{
+  const std::vector<int> ax = scheme_axes (c);
+  for (int s = 0; s < 3; ++s) size[s] = s < (int) ax.size () ? c.size[ax[s]] : 1;
+}
+
+// Dump range [start, end) of a component, per scheme coordinate (Scheme::getStartCoord /
+// getEndCoord, Scheme.inc.h:4647-4790; the middle plane along the orthogonal axis in 3D,
+// SchemeHelper.h:860-990).  comp -1: a material grid, whose minimum coordinate is taken as 0
+// there.
 void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int64_t start[3], int64_t end[3])
 {
   const SaveOptions &sv = o.save;
+  const std::vector<int> ax = scheme_axes (o.cfg);
   if (sv.manual)
   {
-    for (int a = 0; a < 3; ++a)
+    // TC::initAxesCoordinate keeps the scheme's coordinates of the x/y/z options
+    for (int s = 0; s < 3; ++s)
     {
-      start[a] = sv.start[a];
-      end[a] = sv.end[a];
+      start[s] = s < dim ? sv.start[ax[s]] : 0;
+      end[s] = s < dim ? sv.end[ax[s]] : 1;
     }
     return;
   }
@@ -315,12 +352,12 @@ void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int
   if (sv.no_pml && (o.cfg.flags & YEE_USE_PML))
   {
     // convertCoord (border - min) truncates the half-integer coordinate; start adds 1
-    for (int a = 0; a < dim; ++a)
+    for (int s = 0; s < dim; ++s)
     {
-      const int64_t L = o.cfg.pml[a], R = size[a] - o.cfg.pml[a];
-      const int m2 = comp < 0 ? 0 : min_coord2 (comp, a);
-      start[a] = (2 * L - m2) / 2 + 1;
-      end[a] = (2 * R - m2) / 2;
+      const int64_t L = o.cfg.pml[ax[s]], R = size[s] - o.cfg.pml[ax[s]];
+      const int m2 = comp < 0 ? 0 : min_coord2 (comp, ax[s]);
+      start[s] = (2 * L - m2) / 2 + 1;
+      end[s] = (2 * R - m2) / 2;
     }
   }
   if (dim == 3)
@@ -383,7 +420,8 @@ void save_materials (const Options &o, yee_handle *h)
 {
   const yee_config &c = o.cfg;
   const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
-  const int64_t size[3] = { c.size[0], c.size[1], dim == 3 ? c.size[2] : 1 };
+  int64_t size[3];
+  scheme_size (c, size);
   std::vector<double> v ((size_t) (size[0] * size[1] * size[2]));
   static const char *names[6] = { "Eps", "Mu", "OmegaPE", "OmegaPM", "GammaE", "GammaM" };
   const int count = (c.flags & YEE_USE_METAMATERIALS) ? 6 : 2;
@@ -407,21 +445,20 @@ void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
 {
   const yee_config &c = o.cfg;
   const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
-  const int64_t size[3] = { c.size[0], c.size[1], dim == 3 ? c.size[2] : 1 };
+  int64_t size[3];
+  scheme_size (c, size);
   const int64_t n = size[0] * size[1] * size[2];
   static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
   std::vector<std::vector<double>> fields (6);
   for (int comp = 0; comp < 6; ++comp)
   {
-    if (dim == 2 && (c.scheme == YEE_SCHEME_TMZ ? (comp != YEE_EZ && comp != YEE_HX && comp != YEE_HY)
-                                                : (comp != YEE_EX && comp != YEE_EY && comp != YEE_HZ)))
-      continue;
+    if (dim == 2 && !scheme_has (c, comp)) continue;
     fields[comp].resize ((size_t) n);
     if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, fields[comp].data ()));
     else check (yee_get_field (h, comp, fields[comp].data ()));
   }
   std::vector<double> e, hv;
-  const int64_t len = 500 * c.size[0];
+  const int64_t len = 500 * size[0];   // allocateGridsInc: 500 * size.get1 ()
   const bool inc = (o.save.einc || o.save.hinc) && (c.flags & YEE_USE_TFSF);
   if (inc)
   {
@@ -466,8 +503,9 @@ int main (int argc, char **argv)
 {
   Options o = parse (argc, argv);
   yee_config &c = o.cfg;
-  if (c.scheme != YEE_SCHEME_DIM3) c.size[2] = 1;
-  const int64_t n = c.size[0] * c.size[1] * (c.scheme == YEE_SCHEME_DIM3 ? c.size[2] : 1);
+  int64_t ssize[3];
+  scheme_size (c, ssize);
+  const int64_t n = ssize[0] * ssize[1] * ssize[2];
 
   char info[256];
   check (yee_device_info (info, sizeof info));

@@ -928,7 +928,14 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
             // eps-point coordinates (i + 0.5, j + 0.5, k + 0.5)
             const double mx = i + 0.5, my = j + 0.5, mz = k + 0.5;
             double val;
-            if (h->dim == 3)
+            if (h->miss >= 0)
+            {
+              // a 2D scheme on the (y, z) or (x, z) plane: the 2D sphere on its two coordinates
+              const double ma = h->miss == 0 ? my : mx, ca = h->miss == 0 ? cy : cx;
+              val = stair ? sphere_stair_2d (ma, mz, ca, cz, radius, value, outside)
+                          : sphere_accurate_2d (ma, mz, ca, cz, radius, value, outside, accuracy);
+            }
+            else if (h->dim == 3)
               val = stair ? sphere_stair_3d (mx, my, mz, cx, cy, cz, radius, value, outside)
                           : sphere_accurate_3d (mx, my, mz, cx, cy, cz, radius, value, outside, accuracy);
             else

@@ -82,7 +82,9 @@ enum yee_material
  */
 typedef struct yee_config
 {
-  int32_t scheme;             /* YEE_SCHEME_DIM3 (--3d), YEE_SCHEME_TMZ (--2d-tmz) or YEE_SCHEME_TEZ (--2d-tez) */
+  int32_t scheme;             /* YEE_SCHEME_DIM3 (--3d) or a 2D scheme YEE_SCHEME_TEX .. TMZ (--2d-tex .. --2d-tmz);
+                                 2D grids are [a][b] over the scheme's two axes: TEx/TMx (y, z), TEy/TMy (x, z),
+                                 TEz/TMz (x, y) (main.cpp:817-1000); the third size is ignored */
   int32_t layout;             /* --layout-type: only 0 (E_CENTERED) is supported */
   int32_t precision;          /* 4 = FP32 fields, 8 = FP64 fields (reference: -DFLOAT_VALUES / -DDOUBLE_VALUES) */
   int32_t time_steps;         /* --time-steps T; step t = 0 performs no field update (Scheme.inc.h:1032) */

@@ -273,3 +273,39 @@ def test_cli_tez(engine_lib, tmp_path):
     for name in ("Ez", "Hx", "Hy"):
         assert not (tmp_path / f"previous-1_[timestep=40]_[pid=0]_[name={name}].txt").exists()
     eng.close()
+
+
+@pytest.mark.parametrize("args", [
+    "--2d-tmx --size x:5,y:26,z:22 --use-pml --pml-size y:4,z:3 --use-tfsf --tfsf-size-left y:7,z:6 "
+    "--tfsf-size-right y:6,z:7 --angle-teta 35 --angle-phi 90 --angle-psi 0 --eps-sphere 3 "
+    "--eps-sphere-center x:2,y:13,z:11 --eps-sphere-radius 4 --time-steps 40",
+    "--2d-tey --size x:24,y:6,z:20 --use-pml --pml-size x:3,z:4 --point-source x:11,y:3,z:9 "
+    "--point-source-hy --point-source-ex --angle-phi 0 --time-steps 40",
+])
+def test_cli_2d_planes(args, engine_lib, tmp_path):
+    """2D schemes on the (y, z) and (x, z) planes: the CLI dumps exactly the scheme's components
+    in its own two coordinates (TXTDumper prints the scheme's coordinates)."""
+    from fdtd3d_amd.config import parse_args
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    cfg = parse_args(args.split())
+    say(f"[gpu] cli 2d plane {args.split()[0]}")
+    out = subprocess.run([exe] + (args + " --save-res --save-as-txt").split(), capture_output=True, text=True,
+                         timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    eng = Engine(cfg)
+    if cfg.eps_sphere != 1.0:
+        eng.set_material_sphere("eps", cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius)
+    eng.prepare()
+    eng.step(cfg.time_steps)
+    eng.sync()
+    na, nb = cfg.shape
+    for name in cfg.components:
+        ab, val = _read_txt(tmp_path / f"previous-1_[timestep=40]_[pid=0]_[name={name}].txt")
+        assert ab.shape == (na * nb, 2) and ab[-1].tolist() == [na - 1, nb - 1]
+        assert np.abs(val).max() > 0, name
+        assert np.allclose(val, eng.field(name).ravel(), rtol=1e-14, atol=1e-300), name
+    for name in set(["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]) - set(cfg.components):
+        assert not (tmp_path / f"previous-1_[timestep=40]_[pid=0]_[name={name}].txt").exists(), name
+    eng.close()
