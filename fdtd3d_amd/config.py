@@ -58,6 +58,8 @@ class YeeConfigC(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("point_source", ctypes.c_uint32),
         ("point_source_pos", ctypes.c_int64 * 3),
+        ("current_source", ctypes.c_uint32),
+        ("current_source_pos", ctypes.c_int64 * 3),
     ]
 
 
@@ -87,6 +89,9 @@ class Config:
     use_metamaterials: bool = False
     point_source: List[str] = field(default_factory=list)   # component names, e.g. ["Ez"]
     point_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
+    # --current-source-j<x|y|z> / -m<x|y|z> as component names ("Ex" for Jx, "Hx" for Mx), --current-source
+    current_source: List[str] = field(default_factory=list)
+    current_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
     # material options (Settings.inc.h:236-266)
     eps_file: str = ""
     mu_file: str = ""
@@ -151,6 +156,11 @@ class Config:
             c.tfsf_right[a] = int(self.tfsf_right[a])
             c.angle_deg[a] = float(self.angle_deg[a])
             c.point_source_pos[a] = int(self.point_source_pos[a])
+        c.current_source = 0
+        for name in self.current_source:
+            c.current_source |= 1 << COMP_INDEX[name]
+        for a in range(3):
+            c.current_source_pos[a] = int(self.current_source_pos[a])
         c.dx = self.dx
         c.wavelength = self.wavelength
         c.courant = self.courant
@@ -187,7 +197,7 @@ _IGNORED_WITH_ARG = {
     "--cuda-buffer-size", "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end",
     "--ntff-filename", "--ntff-step-angle", "--interm-ntff-step", "--interm-save-step",
     "--save-start-coord", "--save-end-coord", "--rebalance-step", "--stored-steps",
-    "--current-source", "--save-cmd-to-file",
+    "--save-cmd-to-file",
 }
 _IGNORED_FLAGS = {
     "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
@@ -203,8 +213,6 @@ _IGNORED_FLAGS = {
 _UNSUPPORTED = {
     "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
     "--use-double-material-precision", "--eps-normed", "--mu-normed",
-    "--current-source-jx", "--current-source-jy", "--current-source-jz",
-    "--current-source-mx", "--current-source-my", "--current-source-mz",
     # the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
     "--use-ntff",
 }
@@ -242,6 +250,10 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             args = args[: i + 1] + rest + args[i + 1:]
         elif a == "--3d":
             cfg.scheme = SCHEME_DIM3
+        elif a == "--current-source":
+            cfg.current_source_pos = _parse_coord(val(), 0)
+        elif a.startswith("--current-source-") and a[-2:] in ("jx", "jy", "jz", "mx", "my", "mz"):
+            cfg.current_source.append(("E" if a[-2] == "j" else "H") + a[-1])
         elif a in SCHEMES_2D:
             cfg.scheme = SCHEMES_2D[a][0]
         elif a == "--size":

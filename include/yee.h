@@ -99,6 +99,8 @@ typedef struct yee_config
   uint32_t flags;             /* YEE_USE_* */
   uint32_t point_source;      /* bit (1 << yee_component) for each --point-source-<comp> */
   int64_t point_source_pos[3];/* --point-source x:,y:,z: */
+  uint32_t current_source;    /* bit (1 << yee_component) for each --current-source-j<x|y|z> (E) / -m<x|y|z> (H) */
+  int64_t current_source_pos[3]; /* --current-source x:,y:,z: (a cell the step updates) */
 } yee_config;
 
 typedef struct yee_handle yee_handle;

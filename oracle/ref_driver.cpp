@@ -266,6 +266,25 @@ class RefRun
         is->template calculateFieldStepIteration<grid_type, false> (timestep, p, grid->getTotalPosition (p),
           d11, d12, d21, d22, grid, coordFP, opp1, opp2, rhs, Ca, Cb, usePML, gridType, mg, mgt, modifier);
       });
+      /* current source after the curl sweep (Scheme.inc.h:1171-1260) */
+      bool cur = false;
+      switch (grid_type)
+      {
+        case (static_cast<uint8_t> (GridType::EX)): cur = SOLVER_SETTINGS.getDoUseCurrentSourceJx (); break;
+        case (static_cast<uint8_t> (GridType::EY)): cur = SOLVER_SETTINGS.getDoUseCurrentSourceJy (); break;
+        case (static_cast<uint8_t> (GridType::EZ)): cur = SOLVER_SETTINGS.getDoUseCurrentSourceJz (); break;
+        case (static_cast<uint8_t> (GridType::HX)): cur = SOLVER_SETTINGS.getDoUseCurrentSourceMx (); break;
+        case (static_cast<uint8_t> (GridType::HY)): cur = SOLVER_SETTINGS.getDoUseCurrentSourceMy (); break;
+        case (static_cast<uint8_t> (GridType::HZ)): cur = SOLVER_SETTINGS.getDoUseCurrentSourceMz (); break;
+        default: break;
+      }
+      if (cur)
+      {
+        FieldValue current = sin (is->getGridTimeStep () * timestep * FPValue (2) * PhysicsConst::Pi * is->getSourceFrequency ());
+        current = current * PhysicsConst::Mu0 / is->getGridTimeStep () / FPValue (2.0);
+        is->template calculateFieldStepIterationCurrent<grid_type, false> (current, grid, Ca, Cb, usePML, gridType, mg, mgt,
+                                                                           modifier);
+      }
       if (usePML)
       {
         if (useMeta)

@@ -838,6 +838,41 @@ static void o_field_step (oracle *o, int comp, int32_t t)
           double tmp = v12 - v11 - v22 + v21 - rhs * dx;
           out[q] = val * ca + tmp * cb;
         }
+    /*
+     * Current source (Scheme::calculateFieldStep, Scheme.inc.h:1171-1260) after the curl sweep:
+     * current = sin (dt * timestep * 2 pi f) * mu0 / dt / 2 with timestep t (E) or t + 1/2 (H),
+     * subtracted as current * dx * Cb at the source cell, Cb from computeCaCb there
+     * (calculateFieldStepIterationCurrent, InternalScheme.inc.h:2419-2450).
+     */
+    if (o->cfg.current_source & (1u << comp))
+    {
+      int64_t p[3];
+      int inside = 1;
+      for (int a = 0; a < 3; ++a)
+      {
+        p[a] = o->act[a] ? o->cfg.current_source_pos[a] : 0;
+        inside = inside && p[a] >= 0 && p[a] < o->n[a];
+      }
+      if (inside)
+      {
+        const double timestep = c->is_h ? (double) t + 0.5 : (double) t;
+        double current = sin (dt * timestep * 2 * O_PI * o->freq);
+        current = current * o_mu0 () / dt / 2.0;
+        double cb;
+        if (o->use_pml)
+        {
+          double material = o_sig (o, c->sig_curl, comp, p[0], p[1], p[2]);
+          double dd = (2 * eps0 * 1.0 + material * dt);
+          cb = (2 * eps0 * dt / dx) / dd;
+        }
+        else
+        {
+          double material = o_get_material (o, comp, p[0], p[1], p[2], mgrid);
+          cb = dt / (material * modifier * dx);
+        }
+        out[o_idx (o, p[0], p[1], p[2])] -= current * dx * cb;
+      }
+    }
     if (o->use_pml)
     {
       if (o->use_meta)

@@ -53,6 +53,17 @@ CASES = [
     "--2d-tey --size x:25,y:6,z:29 --use-pml --pml-size x:3,z:4 --use-tfsf --tfsf-size-left x:6,z:7 "
     "--tfsf-size-right x:7,z:6 --angle-teta 25 --angle-phi 0 --angle-psi 90 --point-source x:11,y:0,z:13 "
     "--point-source-hy --time-steps 50",
+    # current sources J / M (Scheme.inc.h:1171-1260): with PML (subtracted from D), without PML,
+    # in TMz and on a y-z plane
+    "--3d --size x:16,y:14,z:18 --use-pml --pml-size x:3,y:3,z:3 --current-source x:8,y:7,z:9 "
+    "--current-source-jz --current-source-my --eps-sphere 2 --eps-sphere-center x:8,y:7,z:9 "
+    "--eps-sphere-radius 3 --time-steps 40",
+    "--3d --size x:15,y:13,z:12 --current-source x:7,y:6,z:5 --current-source-jx --current-source-mz "
+    "--time-steps 30",
+    "--2d-tmz --size x:30,y:26 --use-pml --pml-size x:4,y:4 --current-source x:14,y:12 --current-source-jz "
+    "--current-source-mx --time-steps 50",
+    "--2d-tmx --size x:4,y:24,z:26 --use-pml --pml-size y:3,z:3 --current-source x:0,y:11,z:12 "
+    "--current-source-jx --current-source-mz --angle-phi 90 --angle-psi 0 --time-steps 40",
     # TEz at phi = 0 without PML
     "--2d-tez --size x:24,y:20 --use-tfsf --tfsf-size-left x:5,y:5 --tfsf-size-right x:5,y:5 --angle-teta 90 "
     "--angle-phi 0 --angle-psi 0 --time-steps 40",
