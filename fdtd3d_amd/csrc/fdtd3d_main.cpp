@@ -117,7 +117,7 @@ std::vector<std::string> read_cmd_file (const std::string &path)
 const std::set<std::string> kIgnoredWithArg = {
   "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size", "--cuda-buffer-size",
   "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end", "--ntff-filename", "--ntff-step-angle",
-  "--interm-ntff-step", "--rebalance-step", "--stored-steps", "--current-source", "--save-cmd-to-file",
+  "--interm-ntff-step", "--rebalance-step", "--stored-steps", "--save-cmd-to-file",
 };
 const std::set<std::string> kIgnoredFlags = {
   "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
@@ -126,9 +126,8 @@ const std::set<std::string> kIgnoredFlags = {
   "--ntff-to-stdout",
 };
 const std::set<std::string> kUnsupported = {
-  "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy", "--use-double-material-precision", "--eps-normed", "--mu-normed",
-  "--current-source-jx", "--current-source-jy", "--current-source-jz", "--current-source-mx",
-  "--current-source-my", "--current-source-mz",
+  "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
+  "--use-double-material-precision", "--eps-normed", "--mu-normed",
   // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
   "--use-ntff",
 };
@@ -174,6 +173,10 @@ Options parse (int argc, char **argv)
       args.insert (args.begin () + (long) i + 1, rest.begin (), rest.end ());
     }
     else if (a == "--3d") c.scheme = YEE_SCHEME_DIM3;
+    else if (a == "--current-source") parse_coord (val (), 0, c.current_source_pos);
+    else if (a.rfind ("--current-source-", 0) == 0 && a.size () == 19 && (a[17] == 'j' || a[17] == 'm')
+             && a[18] >= 'x' && a[18] <= 'z')
+      c.current_source |= 1u << ((a[17] == 'j' ? 0 : 3) + (a[18] - 'x'));   // Jx..Jz on Ex..Ez, Mx..Mz on Hx..Hz
     else if (a == "--2d-tmz") c.scheme = YEE_SCHEME_TMZ;
     else if (a == "--2d-tez") c.scheme = YEE_SCHEME_TEZ;
     else if (a == "--2d-tmx") c.scheme = YEE_SCHEME_TMX;

@@ -547,6 +547,8 @@ int yee_batch_create (const yee_config *cfgs, int32_t nsim, yee_batch **out)
     if (c.precision != prec || (prec != 4 && prec != 8)) return batch_fail (YEE_EINVAL, "one precision (4 or 8) per batch");
     if (c.flags & (YEE_USE_TFSF | YEE_USE_METAMATERIALS))
       return batch_fail (YEE_ENOTSUP, "batched runs support PML and point sources (no TF/SF, no metamaterials)");
+    if (c.current_source)
+      return batch_fail (YEE_ENOTSUP, "batched runs take point sources, not current sources");
     for (int a = 0; a < 2; ++a)
     {
       if (c.size[a] < 3 || c.size[a] > (1 << 16)) return batch_fail (YEE_EINVAL, "size out of range");

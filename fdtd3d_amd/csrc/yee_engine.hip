@@ -123,6 +123,15 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
     a.run_lo[ax] = 1;
     a.run_hi[ax] = (int) h->n[ax] - 1;
   }
+  // current source J / M (Scheme.inc.h:1171-1260): the current of this half step, with the
+  // reference's operations, at timestep t (E) or t + 1/2 (H); applied by the half-step kernels
+  a.cur_mask = h->cfg.current_source & 0x3fu;
+  for (int ax = 0; ax < 3; ++ax) a.cur_pos[ax] = (int) h->cfg.current_source_pos[ax];
+  {
+    const double ts = role == kRoleH ? (double) h->t + 0.5 : (double) h->t;
+    double current = std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+    a.cur_val = current * mu0 () / h->sc.dt / 2.0;
+  }
   if (h->miss >= 0)
   {
     // a 2D scheme on the 3D kernels: its one cell along the missing axis is updated, a
@@ -278,7 +287,7 @@ bool fused_owns_host (const yee_handle *h, const int64_t p[3])
 
 bool source_in_fused_box (const yee_handle *h)
 {
-  return fused_owns_host (h, h->cfg.point_source_pos);
+  return (h->cfg.point_source & 0x3fu) && fused_owns_host (h, h->cfg.point_source_pos);
 }
 
 // Point sources as separate one-cell launches into buffer `buf`; with the fused step the
@@ -390,7 +399,7 @@ int fused_step (yee_handle *h, int32_t t, int prev)
 {
   int rc;
   const int dst = 1 - h->cur;
-  int64_t lo[16][3], hi[16][3];
+  int64_t lo[20][3], hi[20][3];
   int nslab = slab_boxes (h, lo, hi);
   nslab += face_boxes (h, lo + nslab, hi + nslab);
   if (h->mode == 2)
@@ -414,6 +423,20 @@ int fused_step (yee_handle *h, int32_t t, int prev)
     {
       lo[nslab][ax] = h->cfg.point_source_pos[ax];
       hi[nslab][ax] = h->cfg.point_source_pos[ax] + 1;
+    }
+    ++nslab;
+  }
+  const bool cur_on_src = src_box && h->cfg.current_source_pos[0] == h->cfg.point_source_pos[0]
+                          && h->cfg.current_source_pos[1] == h->cfg.point_source_pos[1]
+                          && h->cfg.current_source_pos[2] == h->cfg.point_source_pos[2];
+  if (h->cfg.current_source && !cur_on_src && fused_owns_host (h, h->cfg.current_source_pos))
+  {
+    // the current-source cell: the half-step kernels subtract the current after its curl
+    // update (the fused kernel leaves it out, kAxCur)
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      lo[nslab][ax] = h->cfg.current_source_pos[ax];
+      hi[nslab][ax] = h->cfg.current_source_pos[ax] + 1;
     }
     ++nslab;
   }
@@ -591,6 +614,10 @@ int validate (const yee_config *c)
   if ((c->flags & YEE_USE_METAMATERIALS) && !(c->flags & YEE_USE_PML))
     return fail (YEE_EINVAL, "metamaterials require PML (reference: Scheme.inc.h:889-892)");
   if (c->time_steps < 0) return fail (YEE_EINVAL, "negative time steps");
+  if (c->current_source & 0x3fu)
+    for (int a = 0; a < 3; ++a)
+      if (a != miss && (c->current_source_pos[a] < 1 || c->current_source_pos[a] > c->size[a] - 2))
+        return fail (YEE_EINVAL, "the current source must lie on a cell the step updates ([1, N-1) on each axis)");
   return YEE_OK;
 }
 
@@ -775,7 +802,8 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (miss >= 0)
   {
     h->cfg.pml[miss] = 0;                  // no PML and no TF/SF border along the missing axis
-    h->cfg.point_source_pos[miss] = 0;     // the source position drops that coordinate
+    h->cfg.point_source_pos[miss] = 0;     // the source positions drop that coordinate
+    h->cfg.current_source_pos[miss] = 0;
   }
   h->ncell = h->n[0] * h->n[1] * h->n[2];
   h->wsize = cfg->precision == 4 ? 4 : 8;
@@ -791,7 +819,11 @@ int yee_create (const yee_config *cfg, yee_handle **out)
     h->present[c] = h->dim == 3 || h->exposed[c];
   }
   for (int c = 0; c < 6; ++c)
-    if (!h->exposed[c]) h->cfg.point_source &= ~(1u << c);
+    if (!h->exposed[c])
+    {
+      h->cfg.point_source &= ~(1u << c);
+      h->cfg.current_source &= ~(1u << c);   // the reference updates no absent component
+    }
   auto bail = [&] (int code) {
     yee_destroy (h);
     return code;

@@ -75,6 +75,9 @@ struct StepArgs
   uint32_t src_mask;           // bit c: hard source on component c at flat index src_q
   int64_t src_q;
   int src_pos[3];              // the same position as (x, y, z)
+  uint32_t cur_mask;           // bit c: current source J (E) / M (H) on component c at cur_pos
+  int cur_pos[3];
+  double cur_val;              // the current of this half step: sin (dt ts 2 pi f) mu0 / dt / 2
   // TF/SF corrections of the cells on the z faces of the TF/SF box (planes k = L-1, R-1, R),
   // precomputed per step: [part (E, H)][face][x - (L-1)][y - (L-1)][comp (Ex/Ey or Hx/Hy)][slot lo/hi]
   const double *zcorr;

@@ -59,7 +59,8 @@ __device__ __forceinline__ PmlCoef<T> pml_coef (const StepArgs<T> &a, const int 
 //   store : write the auxiliary D/B (D1/B1) state of this cell
 template <typename T, int MODE, int COMP>
 __device__ __forceinline__ T cell_value_pc (const StepArgs<T> &a, int64_t q, T f_old, const Old<T> &old, T tmp, T mat,
-                                            bool special, T om, T ga, bool store, const PmlCoef<T> &pc)
+                                            bool special, T om, T ga, bool store, const PmlCoef<T> &pc,
+                                            T jdx = (T) 0)
 {
   const T d_old = old.d;
   const T modc = COMP >= 3 ? a.mu0 : a.eps0;
@@ -67,7 +68,9 @@ __device__ __forceinline__ T cell_value_pc (const StepArgs<T> &a, int64_t q, T f
   {
     // computeCaCb, non-PML branch: Ca = 1, Cb = dt / (material * modifier * dx)
     const T cb = direct_cb (a.dt, mat, modc, a.dx);
-    return f_old * (T) 1.0 + tmp * cb;
+    // a current source subtracts current * dx * Cb after the curl update (0 elsewhere: exact)
+    const T f_new = f_old * (T) 1.0 + tmp * cb;
+    return f_new - jdx * cb;
   }
   const T cac = pc.cac, cbc = pc.cbc, p4 = pc.p4, m4 = pc.m4, ca5 = pc.ca5, dd5 = pc.dd5;
   if constexpr (MODE == 2 && sizeof (T) == 4)
@@ -82,7 +85,7 @@ __device__ __forceinline__ T cell_value_pc (const StepArgs<T> &a, int64_t q, T f
     //   f_new   = f ca5 + dD1_new p4 / dd5 + D1 (p4 - m4) / dd5
     // equal to the reference's expressions in real arithmetic.  dp / d1p hold dD / dD1 here
     // (values one step older in the FP64 state).  FP64 keeps the reference's operations.
-    const T dn = d_old * (cac - (T) 1) + tmp * cbc;
+    const T dn = d_old * (cac - (T) 1) + tmp * cbc - jdx * cbc;
     const T dd_old = old.dp, m_old = old.m, dm_old = old.mp;
     const T M = modc, dt = a.dt;
     const T wdt2 = (dt * om) * (dt * om);
@@ -102,8 +105,9 @@ __device__ __forceinline__ T cell_value_pc (const StepArgs<T> &a, int64_t q, T f
     }
     return f_new;
   }
-  // curl update of D/B with the PML Ca/Cb (computeCaCb, PML branch)
-  const T d_new = d_old * cac + tmp * cbc;
+  // curl update of D/B with the PML Ca/Cb (computeCaCb, PML branch), then the current source
+  const T d_cur = d_old * cac + tmp * cbc;
+  const T d_new = d_cur - jdx * cbc;
   T f_new;
   if (MODE == 2)
   {
@@ -144,7 +148,10 @@ __device__ __forceinline__ T cell_value (const StepArgs<T> &a, int64_t q, const 
 {
   PmlCoef<T> pc {};
   if (MODE != 0 && special) pc = pml_coef<T, COMP> (a, p);
-  return cell_value_pc<T, MODE, COMP> (a, q, f_old, old, tmp, mat, special, om, ga, store, pc);
+  // current source J / M of this component at this cell (Scheme.inc.h:1171-1260): current * dx
+  const bool cur = ((a.cur_mask >> COMP) & 1u) && p[0] == a.cur_pos[0] && p[1] == a.cur_pos[1] && p[2] == a.cur_pos[2];
+  const T jdx = cur ? (T) a.cur_val * a.dx : (T) 0;
+  return cell_value_pc<T, MODE, COMP> (a, q, f_old, old, tmp, mat, special, om, ga, store, pc, jdx);
 }
 
 // In-place update of one component (half-step kernels).  The old values are loaded by the
@@ -348,7 +355,7 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
 // Ownership bits of one coordinate of a cell (axis_bits): inside the run box, inside
 // [L-1, R] of the TF/SF box (with TF/SF on), on a face plane L-1, R-1 or R of the TF/SF box
 // (x and y only), inside the Drude box, on the point source (with a source on).
-constexpr uint32_t kAxRun = 1, kAxFRange = 2, kAxFPlane = 4, kAxDrude = 8, kAxSrc = 16;
+constexpr uint32_t kAxRun = 1, kAxFRange = 2, kAxFPlane = 4, kAxDrude = 8, kAxSrc = 16, kAxCur = 32;
 
 // Thread geometry of the fused sweep.  A cell is updated by the fused kernel when it lies in
 // the run box, is not next to an x or y face of the TF/SF box, not in the Drude box and not the
@@ -381,6 +388,7 @@ __device__ __forceinline__ uint32_t axis_bits (const StepArgs<T> &a, int ax, int
   b |= (ax < 2 && (p == a.tl[ax] - 1 || p == a.tr[ax] - 1 || p == a.tr[ax])) ? kAxFPlane : 0u;
   b |= (p >= a.box_lo[ax] && p < a.box_hi[ax]) ? kAxDrude : 0u;
   b |= (a.src_mask && p == a.src_pos[ax]) ? kAxSrc : 0u;
+  b |= (a.cur_mask && p == a.cur_pos[ax]) ? kAxCur : 0u;
   return b;
 }
 
@@ -394,7 +402,7 @@ __device__ __forceinline__ uint32_t own_mask (uint32_t xb, uint32_t yb)
   // (kAxFPlane shifted onto kAxFRange), kAxDrude when inside the Drude box on x and y
   const uint32_t u = xb & yb;
   const uint32_t face = u & ((xb | yb) >> 1) & kAxFRange;
-  return (0u - (u & kAxRun)) & (kAxRun | face | (u & (kAxDrude | kAxSrc)));
+  return (0u - (u & kAxRun)) & (kAxRun | face | (u & (kAxDrude | kAxSrc | kAxCur)));
 }
 
 __device__ __forceinline__ bool lane_owns (uint32_t zb, uint32_t m)

@@ -101,7 +101,17 @@ INVALID_CONFIGS = [
     (dict(size=[10, 10, 10], courant=-0.5), b"positive"),
     (dict(size=[10, 10, 10], use_metamaterials=True), b"PML"),
     (dict(size=[10, 10, 10], time_steps=-1), b"time steps"),
+    (dict(size=[10, 10, 10], current_source=["Ez"], current_source_pos=[0, 5, 5]), b"current source"),
+    (dict(size=[10, 10, 10], current_source=["Hx"], current_source_pos=[5, 9, 5]), b"current source"),
 ]
+
+
+def test_parse_current_sources():
+    cfg = parse_args("--3d --size x:12 --same-size --current-source x:5,y:6,z:7 --current-source-jz "
+                     "--current-source-mx".split())
+    assert cfg.current_source == ["Ez", "Hx"] and cfg.current_source_pos == [5, 6, 7]
+    c = cfg.to_c()
+    assert c.current_source == (1 << 2) | (1 << 3) and list(c.current_source_pos) == [5, 6, 7]
 
 
 @pytest.mark.parametrize("kw,msg", INVALID_CONFIGS)

@@ -187,6 +187,60 @@ def test_scattered_field_vs_oracle(args, precision, engine_lib):
     eng.close()
 
 
+CURRENT_CASES = [
+    # J / M current sources (Scheme.inc.h:1171-1260) with PML (subtracted from D) and TF/SF
+    "--3d --size x:26,y:24,z:28 --use-pml --pml-size x:4 --same-size-pml --use-tfsf --tfsf-size-left x:7 "
+    "--tfsf-size-right x:7 --same-size-tfsf --angle-teta 30 --angle-phi 45 --angle-psi 90 "
+    "--current-source x:13,y:12,z:14 --current-source-jz --current-source-my --point-source x:13,y:12,z:14 "
+    "--point-source-ex --time-steps 50",
+    # no PML: the direct update, a current on a TF/SF face cell (FP32 bar 2e-5: see CURRENT_FP32_TOL)
+    "--3d --size x:20,y:18,z:22 --use-tfsf --tfsf-size-left x:5 --tfsf-size-right x:5 --same-size-tfsf "
+    "--angle-teta 0 --angle-phi 0 --angle-psi 0 --current-source x:4,y:9,z:11 --current-source-jx "
+    "--current-source-mz --time-steps 40",
+    # in a PML slab, Drude metamaterials on
+    "--3d --size x:22,y:20,z:24 --use-pml --pml-size x:4 --same-size-pml --use-metamaterials "
+    "--current-source x:2,y:10,z:12 --current-source-jy --current-source-mx --time-steps 40",
+    "--2d-tmz --size x:36,y:30 --use-pml --pml-size x:5,y:5 --current-source x:17,y:14 --current-source-jz "
+    "--current-source-mx --time-steps 60",
+    "--2d-tey --size x:30,y:5,z:28 --use-pml --pml-size x:4,z:4 --current-source x:14,y:0,z:13 "
+    "--current-source-jx --current-source-my --angle-phi 0 --time-steps 60",
+]
+
+
+# FP32 bars of the current-source cases.  Next to a J source without PML, E reaches ~7e4 V/m
+# beside H ~ 1 A/m, and the curl of E that updates H cancels most of it: the reference's own
+# FP32 build (refdrv_f32_fast) deviates from FP64 by 1.24e-5 in H on case 1
+# (tests/test_oracle_ref_live.py::test_reference_fp32_deviation_next_to_current_source), so its bar
+# is 2e-5; the others keep 1e-5.
+CURRENT_FP32_TOL = {1: 2e-5}
+
+
+@pytest.mark.parametrize("case", range(len(CURRENT_CASES)))
+@pytest.mark.parametrize("precision", [8, 4])
+def test_current_sources_vs_oracle(case, precision, engine_lib):
+    args = CURRENT_CASES[case]
+    cfg = parse_args(args.split())
+    o = run_oracle(cfg)
+    cfg.precision = precision
+    say(f"[gpu] current sources precision {precision}: {args[:50]}")
+    eng = run_engine(cfg)
+    names = field_names(cfg)
+    assert max(float(np.abs(o.field(f)).max()) for f in names) > 0
+    if precision == 8:
+        for f in names:
+            got, want = eng.field(f), o.field(f)
+            if cfg.use_pml:
+                assert rel_l2(got, want) <= TOL_FP64, f"{f}: {rel_l2(got, want)}"
+            else:
+                assert np.array_equal(got, want), f"{f}: max diff {np.abs(got - want).max()}"
+    else:
+        for group in ("E", "H"):
+            ns = [n for n in names if n[0] == group]
+            r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
+            assert r <= CURRENT_FP32_TOL.get(case, TOL_FP32), f"{group}: {r}"
+    eng.close()
+
+
 def test_special_mask_covers_every_nonzero_sigma(engine_lib):
     cfg = parse_args("--3d --size x:20,y:18,z:22 --use-pml --pml-size x:4,y:3,z:5 --time-steps 1".split())
     say("[gpu] special mask")
@@ -221,6 +275,11 @@ SPLIT_VS_FUSED = [
     # point source inside a PML slab, nz not a multiple of 64
     ("--3d --size x:30,y:26,z:90 --use-pml --pml-size x:4,y:3,z:7 --point-source x:2,y:13,z:45 "
      "--point-source-hz --point-source-ex --time-steps 50"),
+    # current sources J / M inside the fused box and inside a PML slab
+    ("--3d --size x:40,y:36,z:44 --use-pml --pml-size x:5,y:4,z:6 --current-source x:20,y:17,z:22 "
+     "--current-source-jz --current-source-mx --time-steps 40"),
+    ("--3d --size x:30,y:26,z:40 --use-pml --pml-size x:4,y:3,z:6 --current-source x:15,y:2,z:20 "
+     "--current-source-jx --current-source-my --time-steps 40"),
     # rows of 10 waves (the 12-wave instantiation of the fused kernel), PML + TF/SF
     ("--3d --size x:14,y:12,z:620 --use-pml --pml-size x:3,y:2,z:5 --use-tfsf --tfsf-size-left x:5,y:4,z:6 "
      "--tfsf-size-right x:5,y:4,z:7 --angle-teta 20 --angle-phi 70 --angle-psi 10 --time-steps 30"),

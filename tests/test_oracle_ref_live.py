@@ -84,3 +84,25 @@ def test_oracle_matches_reference_driver(args, tmp_path):
     o.step(cfg.time_steps)
     for name in field_names(cfg, aux=True):
         assert np.array_equal(o.field(name).ravel(), rec[name]), name
+
+
+def test_reference_fp32_deviation_next_to_current_source(tmp_path):
+    """The reference's own FP32 build (FLOAT_VALUES) against the FP64 oracle on a J / M source
+    without PML: H deviates by more than 1e-5 (E ~ 7e4 V/m next to the J source cancels in
+    the curl that updates H), the FP32 format's limit on this case rather than the engine's."""
+    from oracle.binding import ref_driver_path
+    if not os.path.exists(ref_driver_path("f32_fast")):
+        pytest.skip("oracle/_ref/refdrv_f32_fast not built")
+    argv = ("--3d --size x:20,y:18,z:22 --use-tfsf --tfsf-size-left x:5 --tfsf-size-right x:5 --same-size-tfsf "
+            "--angle-teta 0 --angle-phi 0 --angle-psi 0 --current-source x:4,y:9,z:11 --current-source-jx "
+            "--current-source-mz --time-steps 40").split()
+    cfg = parse_args(argv)
+    rec, _ = run_ref_driver(argv, str(tmp_path / "r32.bin"), kind="f32_fast")
+    o = Oracle(cfg)
+    o.prepare()
+    o.step(cfg.time_steps)
+    names = [n for n in field_names(cfg) if n[0] == "H"]
+    num = sum(float(np.sum((rec[n].reshape(cfg.shape) - o.field(n)) ** 2)) for n in names)
+    den = sum(float(np.sum(o.field(n) ** 2)) for n in names)
+    r = np.sqrt(num / den)
+    assert 1e-5 < r < 2e-5, r
