@@ -166,6 +166,11 @@ static double graded_sigma (double dist, double boundary, double dx)
   return factor * (std::pow (hi, (double) (m + 1)) - std::pow (lo, (double) (m + 1)));
 }
 
+int first_active_axis (const yee_config &cfg)
+{
+  return missing_axis (cfg) == 0 ? 1 : 0;
+}
+
 int missing_axis (const yee_config &cfg)
 {
   switch (cfg.scheme)
@@ -186,16 +191,18 @@ std::vector<double> sigma_profile (const yee_config &cfg, int axis)
   for (int64_t p = 0; p < n; ++p)
   {
     const double pos = (double) p + 0.5;   // eps-grid coordinate
-    if (axis == 0)
+    if (axis == first_active_axis (cfg))
     {
-      // x: fractional distance, right edge at N - P
+      // the scheme's first coordinate (initSigmaX, Scheme.cpp:141-420; y for TEx / TMx):
+      // fractional distance, right edge at N - P
       const double edge = (double) n;
       if (pos < P) prof[p] = graded_sigma (P - pos, boundary, cfg.dx);
       else if (pos >= edge - P) prof[p] = graded_sigma (pos - (edge - P), boundary, cfg.dx);
     }
     else
     {
-      // y, z: distance truncated to an integer, right edge at N + 0.5 - P
+      // its other coordinates (initSigmaY / Z): distance truncated to an integer, right edge
+      // at N + 0.5 - P
       const double edge = (double) n + 0.5;
       if (pos < P) prof[p] = graded_sigma ((double) (int64_t) (P - pos), boundary, cfg.dx);
       else if (pos >= edge - P) prof[p] = graded_sigma ((double) (int64_t) (pos - (edge - P)), boundary, cfg.dx);

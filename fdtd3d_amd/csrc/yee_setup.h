@@ -55,6 +55,10 @@ Scalars compute_scalars (const yee_config &cfg);
 // TEz/TMz on (x, y)); -1 for 3D.
 int missing_axis (const yee_config &cfg);
 
+// The axis of the scheme's first grid coordinate (y for TEx / TMx, x otherwise): the one whose
+// PML sigma follows initSigmaX.
+int first_active_axis (const yee_config &cfg);
+
 double wave_number (double delta, double lambda, double courant, double nlambda, double a1, double a2);
 
 // Sigma of eps-cell p along `axis` (SchemeHelper::initSigmaX/Y/Z, SchemeHelper.h:592-718).

@@ -190,48 +190,34 @@ class RefRun
   {
     FPValue dx = SOLVER_SETTINGS.getGridStep ();
     TC pml = layout->getLeftBorderPML ();
-    /* x: fractional distance, right threshold N - P (SchemeHelper.h:592-630) */
-    if (miss != 0)
-    {
-      Grid<TC> *s = is->getSigmaX ();
-      FPValue P = FPValue (tcSlot (pml, slot (0)));
-      FPValue boundary = P * dx;
-      FPValue n = FPValue (sz (0));
-      forAll (0, 0, [&] (TC p) {
-        FPValue x = fpSlot (layout->getEpsCoordFP (p), slot (0));
-        FPValue val = 0;
-        if (x < P) val = sigmaValue (P - x, boundary, dx);
-        else if (x >= n - P) val = sigmaValue (x - (n - P), boundary, dx);
-        s->setFieldValue (val, p, 0);
-      });
-    }
-    /* y, z: truncated integer distance, right threshold N + 0.5 - P (SchemeHelper.h:632-718) */
-    for (int axis = 1; axis < 3; ++axis)
+    /* Scheme::initSigmas (Scheme.cpp:141-420) calls initSigmaX on the scheme's first coordinate
+     * and initSigmaY / initSigmaZ on its second / third, whatever physical axis they are (TEx /
+     * TMx: initSigmaX for y).  initSigmaX: fractional distance, right threshold N - P
+     * (SchemeHelper.h:592-630); initSigmaY / Z: truncated integer distance, right threshold
+     * N + 0.5 - P (SchemeHelper.h:632-718). */
+    for (int axis = 0; axis < 3; ++axis)
     {
       if (axis == miss || (miss < 0 && axis >= dim)) continue;
-      Grid<TC> *s = axis == 1 ? is->getSigmaY () : is->getSigmaZ ();
+      Grid<TC> *s = axis == 0 ? is->getSigmaX () : (axis == 1 ? is->getSigmaY () : is->getSigmaZ ());
       FPValue P = FPValue (tcSlot (pml, slot (axis)));
       FPValue boundary = P * dx;
-      FPValue n = (FPValue) sz (axis) + FPValue (0.5);
+      const bool first = slot (axis) == 0;
+      FPValue n = first ? FPValue (sz (axis)) : (FPValue) sz (axis) + FPValue (0.5);
       forAll (0, 0, [&] (TC p) {
-        TCFP fp = layout->getEpsCoordFP (p);
-        FPValue y = fpSlot (fp, slot (axis));
+        FPValue y = fpSlot (layout->getEpsCoordFP (p), slot (axis));
         FPValue val = 0;
         if (y < P)
         {
-          grid_coord d = (grid_coord) (P - y);
-          val = sigmaValue (d, boundary, dx);
+          val = first ? sigmaValue (P - y, boundary, dx) : sigmaValue ((grid_coord) (P - y), boundary, dx);
         }
         else if (y >= n - P)
         {
-          grid_coord d = (grid_coord) (y - (n - P));
-          val = sigmaValue (d, boundary, dx);
+          val = first ? sigmaValue (y - (n - P), boundary, dx) : sigmaValue ((grid_coord) (y - (n - P)), boundary, dx);
         }
         s->setFieldValue (val, p, 0);
       });
     }
   }
-
 
   template <uint8_t grid_type, bool usePML, bool useMeta>
   void fieldStep (time_step t)

@@ -657,6 +657,8 @@ int oracle_prepare (oracle *o)
   if (!o) return -1;
   if (o->use_pml)
   {
+    int first = 0;
+    while (first < 2 && !o->act[first]) ++first;
     for (int a = 0; a < 3; ++a)
     {
       if (!o->act[a]) continue;
@@ -670,9 +672,11 @@ int oracle_prepare (oracle *o)
             int64_t p[3] = { i, j, k };
             double pos = (double) p[a] + 0.5;   /* getEpsCoordFP */
             double val = 0;
-            if (a == 0)
+            /* initSigmaX on the scheme's first coordinate (Scheme.cpp:141-420: y for TEx / TMx),
+             * initSigmaY / Z on the others */
+            if (a == first)
             {
-              double size = (double) o->n[0];
+              double size = (double) o->n[a];
               if (pos < P) val = o_sigma_value (P - pos, boundary, o->dx);
               else if (pos >= size - P) val = o_sigma_value (pos - (size - P), boundary, o->dx);
             }

@@ -125,6 +125,25 @@ def test_sigma_profiles_closed_form():
     assert sx[N - P] > 0 and sy[N - P] > 0 and sy[N - P] < sx[N - P]
 
 
+def test_sigma_follows_the_scheme_coordinate_slot():
+    """Scheme::initSigmas (Scheme.cpp:141-420) applies initSigmaX to a scheme's first grid
+    coordinate whatever physical axis it is: on the (y, z) plane of TMx / TEx the y profile is
+    the fractional-distance one that x has in 3D, z keeps the truncated one."""
+    P, N = 5, 24
+    d3 = Oracle(Config(scheme=SCHEME_DIM3, size=[N, N, N], pml=[P, P, P], use_pml=True, time_steps=1))
+    d3.prepare()
+    sx3, sy3 = d3.material(6)[:, 0, 0], d3.material(7)[0, :, 0]
+    for name in ("--2d-tmx", "--2d-tex"):
+        cfg = parse_args(f"{name} --size y:{N},z:{N} --use-pml --pml-size y:{P},z:{P}".split())
+        o = Oracle(cfg)
+        o.prepare()
+        sy = o.material(7).reshape(N, N)[:, 0]
+        sz = o.material(8).reshape(N, N)[0, :]
+        np.testing.assert_array_equal(sy, sx3)
+        np.testing.assert_array_equal(sz, sy3)
+        assert not np.array_equal(sy, sz)
+
+
 def test_tmz_point_source_symmetry():
     n = 25
     c = n // 2
