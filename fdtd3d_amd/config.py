@@ -16,6 +16,12 @@ import struct
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
+SCHEME_EXHY = 1
+SCHEME_EXHZ = 2
+SCHEME_EYHX = 3
+SCHEME_EYHZ = 4
+SCHEME_EZHX = 5
+SCHEME_EZHY = 6
 SCHEME_TEX = 7
 SCHEME_TEY = 8
 SCHEME_TEZ = 9
@@ -24,6 +30,10 @@ SCHEME_TMY = 11
 SCHEME_TMZ = 12
 SCHEME_DIM3 = 13
 # the option of each 2D scheme (Settings.inc.h) and the axis that scheme lacks
+# 1D schemes: (scheme, the axis of the grid, its components) (main.cpp:629-800)
+SCHEMES_1D = {"--1d-exhy": (SCHEME_EXHY, 2, ("Ex", "Hy")), "--1d-exhz": (SCHEME_EXHZ, 1, ("Ex", "Hz")),
+              "--1d-eyhx": (SCHEME_EYHX, 2, ("Ey", "Hx")), "--1d-eyhz": (SCHEME_EYHZ, 0, ("Ey", "Hz")),
+              "--1d-ezhx": (SCHEME_EZHX, 1, ("Ez", "Hx")), "--1d-ezhy": (SCHEME_EZHY, 0, ("Ez", "Hy"))}
 SCHEMES_2D = {"--2d-tex": (SCHEME_TEX, 0), "--2d-tey": (SCHEME_TEY, 1), "--2d-tez": (SCHEME_TEZ, 2),
               "--2d-tmx": (SCHEME_TMX, 0), "--2d-tmy": (SCHEME_TMY, 1), "--2d-tmz": (SCHEME_TMZ, 2)}
 
@@ -117,17 +127,24 @@ class Config:
 
     @property
     def dim(self) -> int:
-        return 3 if self.scheme == SCHEME_DIM3 else 2
+        return 3 if self.scheme == SCHEME_DIM3 else (2 if self.scheme >= SCHEME_TEX else 1)
+
+    @property
+    def active_axes(self):
+        """The axes of the scheme's grid (its order: x before y before z)."""
+        if self.dim == 1:
+            return [v[1] for v in SCHEMES_1D.values() if v[0] == self.scheme]
+        return [a for a in range(3) if a != self.missing_axis]
 
     @property
     def shape(self):
         if self.dim == 3:
             return tuple(self.size[:3])
-        return tuple(self.size[a] for a in range(3) if a != self.missing_axis)
+        return tuple(self.size[a] for a in self.active_axes)
 
     @property
     def missing_axis(self) -> int:
-        """The axis a 2D scheme lacks (-1 in 3D)."""
+        """The axis a 2D scheme lacks (-1 in 3D and 1D)."""
         for scheme, axis in SCHEMES_2D.values():
             if scheme == self.scheme:
                 return axis
@@ -140,6 +157,8 @@ class Config:
         names = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
         if self.dim == 3:
             return names
+        if self.dim == 1:
+            return [n for n in names if any(v[0] == self.scheme and n in v[2] for v in SCHEMES_1D.values())]
         m, te = self.missing_axis, self.scheme <= SCHEME_TEZ
         return [n for c, n in enumerate(names) if ((c >= 3) == te) == ((c % 3) == m)]
 
@@ -211,7 +230,7 @@ _IGNORED_FLAGS = {
 }
 # options of the reference whose feature is outside this engine: rejected loudly
 _UNSUPPORTED = {
-    "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
+
     "--use-double-material-precision", "--eps-normed", "--mu-normed",
     # the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
     "--use-ntff",
@@ -256,6 +275,8 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             cfg.current_source.append(("E" if a[-2] == "j" else "H") + a[-1])
         elif a in SCHEMES_2D:
             cfg.scheme = SCHEMES_2D[a][0]
+        elif a in SCHEMES_1D:
+            cfg.scheme = SCHEMES_1D[a][0]
         elif a == "--size":
             cfg.size = _parse_coord(val(), 100)
         elif a == "--same-size":

@@ -126,7 +126,6 @@ const std::set<std::string> kIgnoredFlags = {
   "--ntff-to-stdout",
 };
 const std::set<std::string> kUnsupported = {
-  "--1d-exhy", "--1d-exhz", "--1d-eyhx", "--1d-eyhz", "--1d-ezhx", "--1d-ezhy",
   "--use-double-material-precision", "--eps-normed", "--mu-normed",
   // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
   "--use-ntff",
@@ -177,6 +176,12 @@ Options parse (int argc, char **argv)
     else if (a.rfind ("--current-source-", 0) == 0 && a.size () == 19 && (a[17] == 'j' || a[17] == 'm')
              && a[18] >= 'x' && a[18] <= 'z')
       c.current_source |= 1u << ((a[17] == 'j' ? 0 : 3) + (a[18] - 'x'));   // Jx..Jz on Ex..Ez, Mx..Mz on Hx..Hz
+    else if (a == "--1d-exhy") c.scheme = YEE_SCHEME_EXHY;
+    else if (a == "--1d-exhz") c.scheme = YEE_SCHEME_EXHZ;
+    else if (a == "--1d-eyhx") c.scheme = YEE_SCHEME_EYHX;
+    else if (a == "--1d-eyhz") c.scheme = YEE_SCHEME_EYHZ;
+    else if (a == "--1d-ezhx") c.scheme = YEE_SCHEME_EZHX;
+    else if (a == "--1d-ezhy") c.scheme = YEE_SCHEME_EZHY;
     else if (a == "--2d-tmz") c.scheme = YEE_SCHEME_TMZ;
     else if (a == "--2d-tez") c.scheme = YEE_SCHEME_TEZ;
     else if (a == "--2d-tmx") c.scheme = YEE_SCHEME_TMX;
@@ -262,7 +267,7 @@ Options parse (int argc, char **argv)
     if (done) continue;
     if (kIgnoredWithArg.count (a)) (void) val ();
     else if (kIgnoredFlags.count (a)) continue;
-    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (3D and 2D, E_CENTERED, real fields)");
+    else if (kUnsupported.count (a)) die ("option " + a + " is outside the Yee engine (E_CENTERED, real fields)");
     else die ("unknown option " + a);
   }
   return o;
@@ -298,12 +303,22 @@ int min_coord2 (int comp, int axis)
   return m[comp][axis];
 }
 
+// Dimension of the scheme's grid.
+int scheme_dim (const yee_config &c)
+{
+  return c.scheme == YEE_SCHEME_DIM3 ? 3 : (c.scheme >= YEE_SCHEME_TEX ? 2 : 1);
+}
+
 // The scheme's axes in the order of its coordinates: x, y, z in 3D, the two axes a 2D scheme has
-// (main.cpp:817-1000: TEx/TMx y, z; TEy/TMy x, z; TEz/TMz x, y).
+// (main.cpp:817-1000: TEx/TMx y, z; TEy/TMy x, z; TEz/TMz x, y), the one of a 1D scheme
+// (main.cpp:629-800: ExHy/EyHx z, ExHz/EzHx y, EyHz/EzHy x).
 std::vector<int> scheme_axes (const yee_config &c)
 {
   switch (c.scheme)
   {
+    case YEE_SCHEME_EXHY: case YEE_SCHEME_EYHX: return { 2 };
+    case YEE_SCHEME_EXHZ: case YEE_SCHEME_EZHX: return { 1 };
+    case YEE_SCHEME_EYHZ: case YEE_SCHEME_EZHY: return { 0 };
     case YEE_SCHEME_TEX: case YEE_SCHEME_TMX: return { 1, 2 };
     case YEE_SCHEME_TEY: case YEE_SCHEME_TMY: return { 0, 2 };
     case YEE_SCHEME_TEZ: case YEE_SCHEME_TMZ: return { 0, 1 };
@@ -316,6 +331,13 @@ std::vector<int> scheme_axes (const yee_config &c)
 bool scheme_has (const yee_config &c, int comp)
 {
   if (c.scheme == YEE_SCHEME_DIM3) return true;
+  if (c.scheme < YEE_SCHEME_TEX)
+  {
+    // a 1D scheme: the two components it names (InternalScheme.inc.h:3243-3260)
+    static const int k1d[7][2] = { { -1, -1 }, { YEE_EX, YEE_HY }, { YEE_EX, YEE_HZ }, { YEE_EY, YEE_HX },
+                                   { YEE_EY, YEE_HZ }, { YEE_EZ, YEE_HX }, { YEE_EZ, YEE_HY } };
+    return comp == k1d[c.scheme][0] || comp == k1d[c.scheme][1];
+  }
   const std::vector<int> ax = scheme_axes (c);
   const int miss = 3 - ax[0] - ax[1];
   const bool te = c.scheme <= YEE_SCHEME_TEZ, along = (comp % 3) == miss;
@@ -422,7 +444,7 @@ std::vector<bool> dump_formats (const SaveOptions &sv)
 void save_materials (const Options &o, yee_handle *h)
 {
   const yee_config &c = o.cfg;
-  const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  const int dim = scheme_dim (c);
   int64_t size[3];
   scheme_size (c, size);
   std::vector<double> v ((size_t) (size[0] * size[1] * size[2]));
@@ -447,7 +469,7 @@ void save_materials (const Options &o, yee_handle *h)
 void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
 {
   const yee_config &c = o.cfg;
-  const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  const int dim = scheme_dim (c);
   int64_t size[3];
   scheme_size (c, size);
   const int64_t n = size[0] * size[1] * size[2];
@@ -455,7 +477,7 @@ void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
   std::vector<std::vector<double>> fields (6);
   for (int comp = 0; comp < 6; ++comp)
   {
-    if (dim == 2 && !scheme_has (c, comp)) continue;
+    if (dim < 3 && !scheme_has (c, comp)) continue;
     fields[comp].resize ((size_t) n);
     if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, fields[comp].data ()));
     else check (yee_get_field (h, comp, fields[comp].data ()));

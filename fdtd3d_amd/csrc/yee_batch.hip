@@ -155,6 +155,7 @@ StepArgs<T> sim_args (const yee_batch *b, int s, int parity)
   const yee_config &c = b->cfg[s];
   const bool pml = (c.flags & YEE_USE_PML) != 0;
   StepArgs<T> a {};
+  a.sz = 1;
   a.nx = (int) c.size[0];
   a.ny = (int) c.size[1];
   a.nz = 1;

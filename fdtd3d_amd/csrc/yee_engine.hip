@@ -132,16 +132,18 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
     double current = std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
     a.cur_val = current * mu0 () / h->sc.dt / 2.0;
   }
-  if (h->miss >= 0)
+  a.sz = 1;
+  for (int m = 0; m < 3; ++m)
   {
-    // a 2D scheme on the 3D kernels: its one cell along the missing axis is updated, a
+    if (!h->lack[m]) continue;
+    // a 1D / 2D scheme on the 3D kernels: its one cell along a lacked axis is updated, a
     // neighbour along that axis is the cell itself (stride 0), and the PML and TF/SF tests
     // never fire along it
-    const int m = h->miss;
     a.run_lo[m] = 0;
     a.run_hi[m] = 1;
     if (m == 0) a.sx = 0;
-    else a.sy = 0;
+    else if (m == 1) a.sy = 0;
+    else a.sz = 0;
     a.slab_lo[m] = -1;
     a.slab_hi[m] = 1 << 30;
     a.tl[m] = -(1 << 28);
@@ -588,15 +590,14 @@ int upload (yee_handle *h, void **dst, const double *src, size_t n)
 int validate (const yee_config *c)
 {
   if (!c) return fail (YEE_EINVAL, "null config");
-  if (c->scheme < YEE_SCHEME_TEX || c->scheme > YEE_SCHEME_DIM3)
-    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13) or a 2D scheme TEx .. TMz (7 .. 12)");
+  if (c->scheme < YEE_SCHEME_EXHY || c->scheme > YEE_SCHEME_DIM3)
+    return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13), a 2D scheme TEx .. TMz (7 .. 12) or a 1D "
+                             "scheme ExHy .. EzHy (1 .. 6)");
   if (c->layout != 0) return fail (YEE_EINVAL, "only the E_CENTERED layout (0) is supported");
   if (c->precision != 4 && c->precision != 8) return fail (YEE_EINVAL, "precision must be 4 or 8");
-  const int dim = c->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
-  const int miss = missing_axis (*c);
   for (int a = 0; a < 3; ++a)
   {
-    if (a == miss) continue;
+    if (!axis_active (*c, a)) continue;
     if (c->size[a] < 3 || c->size[a] > (1 << 20)) return fail (YEE_EINVAL, "size out of range");
     // the PML sizes matter only with PML on (a 3-cell grid keeps the default size 2 otherwise)
     if ((c->flags & YEE_USE_PML) && (c->pml[a] < 0 || 2 * c->pml[a] > c->size[a]))
@@ -606,8 +607,7 @@ int validate (const yee_config *c)
       return fail (YEE_EINVAL, "TF/SF box out of range");
   }
   int64_t cells = 1;
-  for (int a = 0; a < 3; ++a) cells *= a == miss ? 1 : c->size[a];
-  (void) dim;
+  for (int a = 0; a < 3; ++a) cells *= axis_active (*c, a) ? c->size[a] : 1;
   if (cells > (int64_t) 1 << 34) return fail (YEE_EINVAL, "grid too large");
   if (!(c->dx > 0) || !(c->wavelength > 0) || !(c->courant > 0))
     return fail (YEE_EINVAL, "dx, wavelength and courant factor must be positive");
@@ -616,7 +616,7 @@ int validate (const yee_config *c)
   if (c->time_steps < 0) return fail (YEE_EINVAL, "negative time steps");
   if (c->current_source & 0x3fu)
     for (int a = 0; a < 3; ++a)
-      if (a != miss && (c->current_source_pos[a] < 1 || c->current_source_pos[a] > c->size[a] - 2))
+      if (axis_active (*c, a) && (c->current_source_pos[a] < 1 || c->current_source_pos[a] > c->size[a] - 2))
         return fail (YEE_EINVAL, "the current source must lie on a cell the step updates ([1, N-1) on each axis)");
   return YEE_OK;
 }
@@ -795,15 +795,18 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   // 2D schemes on the (y, z) and (x, z) planes run on the 3D kernels with the missing axis of
   // extent 1 (neighbours along it are the cell itself, the offsets the reference drops); TEz
   // and TMz have kernels of their own
-  const int miss = missing_axis (*cfg);
+  // 1D schemes run on the 3D kernels with both other axes of extent 1
+  const int miss = missing_axis (*cfg), line = line_axis (*cfg);
   h->miss = miss == 2 ? -1 : miss;
-  h->dim = (cfg->scheme == YEE_SCHEME_DIM3 || h->miss >= 0) ? 3 : 2;
-  for (int a = 0; a < 3; ++a) h->n[a] = a == miss ? 1 : cfg->size[a];
-  if (miss >= 0)
+  h->dim = (cfg->scheme == YEE_SCHEME_DIM3 || h->miss >= 0 || line >= 0) ? 3 : 2;
+  for (int a = 0; a < 3; ++a)
   {
-    h->cfg.pml[miss] = 0;                  // no PML and no TF/SF border along the missing axis
-    h->cfg.point_source_pos[miss] = 0;     // the source positions drop that coordinate
-    h->cfg.current_source_pos[miss] = 0;
+    h->n[a] = axis_active (*cfg, a) ? cfg->size[a] : 1;
+    if (axis_active (*cfg, a)) continue;
+    h->lack[a] = a != 2 || line >= 0;      // TEz / TMz keep kernels of their own
+    h->cfg.pml[a] = 0;                     // no PML and no TF/SF border along a lacked axis
+    h->cfg.point_source_pos[a] = 0;        // the source positions drop that coordinate
+    h->cfg.current_source_pos[a] = 0;
   }
   h->ncell = h->n[0] * h->n[1] * h->n[2];
   h->wsize = cfg->precision == 4 ? 4 : 8;
@@ -812,8 +815,12 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   {
     // InternalScheme component flags (InternalScheme.inc.h:3243-3272): TE<a> has H along the
     // missing axis a and E across it, TM<a> the reverse
+    // (a 1D scheme: the two components it names, InternalScheme.inc.h:3243-3260)
+    static const int k1d[7][2] = { { -1, -1 }, { YEE_EX, YEE_HY }, { YEE_EX, YEE_HZ }, { YEE_EY, YEE_HX },
+                                   { YEE_EY, YEE_HZ }, { YEE_EZ, YEE_HX }, { YEE_EZ, YEE_HY } };
     const bool te = cfg->scheme <= YEE_SCHEME_TEZ, along = (c % 3) == miss;
-    h->exposed[c] = miss < 0 || ((c >= 3) == te ? along : !along);
+    if (line >= 0) h->exposed[c] = c == k1d[cfg->scheme][0] || c == k1d[cfg->scheme][1];
+    else h->exposed[c] = miss < 0 || ((c >= 3) == te ? along : !along);
     // the 3D kernels update all six components; the absent ones stay exactly 0 (their curl
     // operands cancel or are absent, their incident projections are 0)
     h->present[c] = h->dim == 3 || h->exposed[c];
@@ -839,7 +846,7 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   const char *kern = std::getenv ("YEE_KERNEL");
   const bool want_fused = !(kern && std::strcmp (kern, "split") == 0);
   h->fused = false;
-  if (h->dim == 3 && h->miss < 0 && h->n[2] <= 1024 && want_fused)
+  if (h->dim == 3 && h->miss < 0 && line < 0 && h->n[2] <= 1024 && want_fused)
   {
     bool ok = true;
     for (int ax = 0; ax < 3; ++ax)
@@ -905,7 +912,7 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   if (h->use_tfsf ())
   {
     // allocateGridsInc: 500 * size.x cells (InternalScheme.specific.h:99-107)
-    h->inc_len = 500 * h->n[miss == 0 ? 1 : 0];   // size.get1 () of the scheme's grid
+    h->inc_len = 500 * h->n[first_active_axis (*cfg)];   // size.get1 () of the scheme's grid
     if ((rc = alloc_zero (h, &h->einc, h->inc_len * h->wsize))) return bail (rc);
     if ((rc = alloc_zero (h, &h->hinc[0], h->inc_len * h->wsize))) return bail (rc);
     if ((rc = alloc_zero (h, &h->hinc[1], h->inc_len * h->wsize))) return bail (rc);
@@ -945,6 +952,7 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
   if (!h || !center) return fail (YEE_EINVAL, "null argument");
   if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
   if (!stair && accuracy <= 0) return fail (YEE_EINVAL, "sphere accuracy must be positive");
+  if (line_axis (h->cfg) >= 0) return fail (YEE_EINVAL, "spheres on a 1D grid are not supported");
   std::vector<double> v ((size_t) h->ncell);
   const double cx = (double) center[0] + 0.5, cy = (double) center[1] + 0.5, cz = (double) center[2] + 0.5;
   const int64_t nx = h->n[0], ny = h->n[1], nz = h->n[2];

@@ -37,7 +37,8 @@ struct yee_handle
   int mode = 0;   // 0 plain, 1 PML, 2 PML + Drude
   bool present[6] = {};   // components held and updated (all six on the 3D kernels)
   bool exposed[6] = {};   // components of the scheme (what the API reads and sources drive)
-  int miss = -1;          // missing axis of a 2D scheme run on the 3D kernels (TEx/TMx 0, TEy/TMy 1)
+  bool lack[3] = { false, false, false };   // axes a 1D / 2D scheme run on the 3D kernels lacks (extent 1)
+  int miss = -1;          // the single lacked axis of such a 2D scheme (TEx/TMx 0, TEy/TMy 1), else -1
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = { nullptr, nullptr };
 

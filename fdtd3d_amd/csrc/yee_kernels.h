@@ -30,6 +30,7 @@ struct StepArgs
   int nx, ny, nz;
   int64_t sx;   // ny * nz
   int sy;       // nz
+  int sz;       // 1 (0 along an axis the scheme lacks, as sx / sy then are)
 
   // cells that run the reference's two-stage D/B update: the PML slabs
   // (p < slab_lo[a] or p >= slab_hi[a]) and, with metamaterials, the Drude box

@@ -195,6 +195,7 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
 {
   const int64_t sx = a.sx;
   const int sy = a.sy;
+  const int sz = a.sz;   // 1, or 0 when the scheme lacks z (a 1D line along x or y)
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
   const int p[3] = { i, j, k };
   const bool special = is_special (a, p, 3, MODE);
@@ -205,8 +206,8 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
   load_old<T, MODE, 2> (a, q, special, fo2, do2);
   const T *Hx = a.f[3], *Hy = a.f[4], *Hz = a.f[5];
   const T hx = gld (Hx + q), hy = gld (Hy + q), hz = gld (Hz + q);
-  T vx[4] = { gld (Hz + q - sy), hz, gld (Hy + q - 1), hy };
-  T vy[4] = { gld (Hx + q - 1), hx, gld (Hz + q - sx), hz };
+  T vx[4] = { gld (Hz + q - sy), hz, gld (Hy + q - sz), hy };
+  T vy[4] = { gld (Hx + q - sz), hx, gld (Hz + q - sx), hz };
   T vz[4] = { gld (Hy + q - sx), hy, gld (Hx + q - sy), hx };
   T ex_m = a.eps_u, ey_m = a.eps_u, ez_m = a.eps_u;
   if (a.eps)
@@ -215,7 +216,7 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
     const T e = gld (a.eps + q);
     ex_m = avg2 (e, gld (a.eps + q + sx));
     ey_m = avg2 (e, gld (a.eps + q + sy));
-    ez_m = avg2 (e, gld (a.eps + q + 1));
+    ez_m = avg2 (e, gld (a.eps + q + sz));
   }
   if (a.tfsf && tfsf_near<T, 3> (a, i, j, k))
   {
@@ -231,13 +232,13 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
     {
       const T *o = a.ope;
       oxm = avg2 (gld (o + q), gld (o + q + sx)); oym = avg2 (gld (o + q), gld (o + q + sy));
-      ozm = avg2 (gld (o + q), gld (o + q + 1));
+      ozm = avg2 (gld (o + q), gld (o + q + sz));
     }
     if (a.ge)
     {
       const T *g = a.ge;
       gxm = avg2 (gld (g + q), gld (g + q + sx)); gym = avg2 (gld (g + q), gld (g + q + sy));
-      gzm = avg2 (gld (g + q), gld (g + q + 1));
+      gzm = avg2 (gld (g + q), gld (g + q + sz));
     }
   }
   update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl (vx), ex_m, special, oxm, gxm);
@@ -254,6 +255,7 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
 {
   const int64_t sx = a.sx;
   const int sy = a.sy;
+  const int sz = a.sz;   // 1, or 0 when the scheme lacks z (a 1D line along x or y)
   const int64_t q = (int64_t) i * sx + (int64_t) j * sy + k;
   const int p[3] = { i, j, k };
   const bool special = is_special (a, p, 3, MODE);
@@ -264,8 +266,8 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
   load_old<T, MODE, 5> (a, q, special, fo5, do5);
   const T *Ex = a.f[0], *Ey = a.f[1], *Ez = a.f[2];
   const T ex = gld (Ex + q), ey = gld (Ey + q), ez = gld (Ez + q);
-  T vx[4] = { ey, gld (Ey + q + 1), ez, gld (Ez + q + sy) };
-  T vy[4] = { ez, gld (Ez + q + sx), ex, gld (Ex + q + 1) };
+  T vx[4] = { ey, gld (Ey + q + sz), ez, gld (Ez + q + sy) };
+  T vy[4] = { ez, gld (Ez + q + sx), ex, gld (Ex + q + sz) };
   T vz[4] = { ex, gld (Ex + q + sy), ey, gld (Ey + q + sx) };
   if (a.tfsf && tfsf_near<T, 3> (a, i, j, k))
   {
@@ -278,8 +280,8 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
   if (a.mu)
   {
     const T *m = a.mu;
-    hx_m = avg4 (gld (m + q), gld (m + q + 1), gld (m + q + sy), gld (m + q + sy + 1));
-    hy_m = avg4 (gld (m + q), gld (m + q + 1), gld (m + q + sx), gld (m + q + sx + 1));
+    hx_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sy), gld (m + q + sy + sz));
+    hy_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sx), gld (m + q + sx + sz));
     hz_m = avg4 (gld (m + q), gld (m + q + sy), gld (m + q + sx), gld (m + q + sx + sy));
   }
   T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
@@ -288,14 +290,14 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
     const T *o = a.opm, *g = a.gm;
     if (o)
     {
-      oxm = sum4 (gld (o + q), gld (o + q + 1), gld (o + q + sy), gld (o + q + sy + 1));
-      oym = sum4 (gld (o + q), gld (o + q + 1), gld (o + q + sx), gld (o + q + sx + 1));
+      oxm = sum4 (gld (o + q), gld (o + q + sz), gld (o + q + sy), gld (o + q + sy + sz));
+      oym = sum4 (gld (o + q), gld (o + q + sz), gld (o + q + sx), gld (o + q + sx + sz));
       ozm = sum4 (gld (o + q), gld (o + q + sy), gld (o + q + sx), gld (o + q + sx + sy));
     }
     if (g)
     {
-      gxm = sum4 (gld (g + q), gld (g + q + 1), gld (g + q + sy), gld (g + q + sy + 1));
-      gym = sum4 (gld (g + q), gld (g + q + 1), gld (g + q + sx), gld (g + q + sx + 1));
+      gxm = sum4 (gld (g + q), gld (g + q + sz), gld (g + q + sy), gld (g + q + sy + sz));
+      gym = sum4 (gld (g + q), gld (g + q + sz), gld (g + q + sx), gld (g + q + sx + sz));
       gzm = sum4 (gld (g + q), gld (g + q + sy), gld (g + q + sx), gld (g + q + sx + sy));
     }
   }

@@ -89,6 +89,30 @@ Scalars compute_scalars (const yee_config &cfg)
     s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a1) * std::sin (a2);
     s.zero_inc[2] = cfg.tfsf_left[2] - 2.5 * std::cos (a1);
   }
+  else if (line_axis (cfg) >= 0)
+  {
+    // the 1D schemes, run on the 3D kernels with the two other axes of extent 1: their incident
+    // wave d is the coordinate along the line (InternalScheme.cpp:110-200), the 3D expression
+    // x S1 C2 + y S1 S2 + z C1 with factor 1 on the line's axis and 0 on the others (exact)
+    const int line = line_axis (cfg);
+    s.S1 = line == 2 ? 0.0 : 1.0;
+    s.C1 = line == 2 ? 1.0 : 0.0;
+    s.C2 = line == 0 ? 1.0 : 0.0;
+    s.S2 = line == 1 ? 1.0 : 0.0;
+    // leftBorderTotalField - 2.5 along the line (YeeGridLayout.cpp:2062-2263)
+    for (int a = 0; a < 3; ++a) s.zero_inc[a] = a == line ? cfg.tfsf_left[a] - 2.5 : 0.0;
+    for (int c = 0; c < 6; ++c) s.proj[c] = 0.0;
+    // YeeGridLayout.h:1247-1760: -valE / -valH written as val * -1 (the same bits)
+    switch (cfg.scheme)
+    {
+      case YEE_SCHEME_EXHY: s.proj[YEE_EX] = -1.0; s.proj[YEE_HY] = -1.0; break;
+      case YEE_SCHEME_EXHZ: s.proj[YEE_EX] = 1.0; s.proj[YEE_HZ] = -1.0; break;
+      case YEE_SCHEME_EYHX: s.proj[YEE_EY] = -1.0; s.proj[YEE_HX] = 1.0; break;
+      case YEE_SCHEME_EYHZ: s.proj[YEE_EY] = -1.0; s.proj[YEE_HZ] = -1.0; break;
+      case YEE_SCHEME_EZHX: s.proj[YEE_EZ] = 1.0; s.proj[YEE_HX] = 1.0; break;
+      default: s.proj[YEE_EZ] = 1.0; s.proj[YEE_HY] = -1.0; break;   // EzHy
+    }
+  }
   else if (missing_axis (cfg) != 2)
   {
     // TEx / TMx on (y, z) and TEy / TMy on (x, z), run on the 3D kernels with the missing axis
@@ -166,9 +190,22 @@ static double graded_sigma (double dist, double boundary, double dx)
   return factor * (std::pow (hi, (double) (m + 1)) - std::pow (lo, (double) (m + 1)));
 }
 
+int line_axis (const yee_config &cfg)
+{
+  static const int kAxis[7] = { -1, 2, 1, 2, 0, 1, 0 };
+  return cfg.scheme >= YEE_SCHEME_EXHY && cfg.scheme <= YEE_SCHEME_EZHY ? kAxis[cfg.scheme] : -1;
+}
+
+bool axis_active (const yee_config &cfg, int axis)
+{
+  const int line = line_axis (cfg);
+  return line >= 0 ? axis == line : axis != missing_axis (cfg);
+}
+
 int first_active_axis (const yee_config &cfg)
 {
-  return missing_axis (cfg) == 0 ? 1 : 0;
+  const int line = line_axis (cfg);
+  return line >= 0 ? line : (missing_axis (cfg) == 0 ? 1 : 0);
 }
 
 int missing_axis (const yee_config &cfg)
@@ -229,7 +266,6 @@ double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p,
 
 std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
 {
-  const int miss = missing_axis (cfg);
   std::vector<double> tab ((size_t) 6 * kNumRoles * tab_len, 0.0);
   if (!(cfg.flags & YEE_USE_PML))
   {
@@ -237,11 +273,11 @@ std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
   }
   std::vector<double> prof[3];
   for (int a = 0; a < 3; ++a)
-    if (a != miss) prof[a] = sigma_profile (cfg, a);
+    if (axis_active (cfg, a)) prof[a] = sigma_profile (cfg, a);
   const Scalars s = compute_scalars (cfg);
   const double e0 = eps0 (), dt = s.dt, dx = cfg.dx;
   auto at = [&] (int comp, int axis, int64_t p) -> double {
-    if (axis == miss) return 0.0;   // a 2D scheme has no sigma along its missing axis: reads as 0
+    if (!axis_active (cfg, axis)) return 0.0;   // no sigma along an axis the scheme lacks: reads as 0
     return sigma_at (prof[axis], comp, axis, p, cfg.size[axis]);
   };
   for (int comp = 0; comp < 6; ++comp)
@@ -250,20 +286,20 @@ std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
     double *T = &tab[(size_t) comp * kNumRoles * tab_len];
     for (int64_t p = 0; p < tab_len; ++p)
     {
-      if (c.sig_curl != miss ? p < cfg.size[c.sig_curl] : p == 0)
+      if (axis_active (cfg, c.sig_curl) ? p < cfg.size[c.sig_curl] : p == 0)
       {
         const double m = at (comp, c.sig_curl, p);
         const double dd = (2 * e0 * 1.0 + m * dt);
         T[kCAC * tab_len + p] = (2 * e0 * 1.0 - m * dt) / dd;
         T[kCBC * tab_len + p] = (2 * e0 * dt / dx) / dd;
       }
-      if (c.sig4 != miss ? p < cfg.size[c.sig4] : p == 0)
+      if (axis_active (cfg, c.sig4) ? p < cfg.size[c.sig4] : p == 0)
       {
         const double m = at (comp, c.sig4, p);
         T[kP4 * tab_len + p] = 2 * e0 * 1.0 + m * dt;
         T[kM4 * tab_len + p] = 2 * e0 * 1.0 - m * dt;
       }
-      if (c.sig5 != miss ? p < cfg.size[c.sig5] : p == 0)
+      if (axis_active (cfg, c.sig5) ? p < cfg.size[c.sig5] : p == 0)
       {
         const double m = at (comp, c.sig5, p);
         const double dd = (2 * e0 * 1.0 + m * dt);

@@ -55,8 +55,15 @@ Scalars compute_scalars (const yee_config &cfg);
 // TEz/TMz on (x, y)); -1 for 3D.
 int missing_axis (const yee_config &cfg);
 
-// The axis of the scheme's first grid coordinate (y for TEx / TMx, x otherwise): the one whose
-// PML sigma follows initSigmaX.
+// The axis of a 1D scheme's grid (main.cpp:629-800: ExHy / EyHx z, ExHz / EzHx y, EyHz / EzHy x);
+// -1 for the other schemes.
+int line_axis (const yee_config &cfg);
+
+// Whether the scheme's grid extends along `axis` (3D: every axis; 2D: two; 1D: one).
+bool axis_active (const yee_config &cfg, int axis);
+
+// The axis of the scheme's first grid coordinate (y for TEx / TMx, the line's axis for 1D, x
+// otherwise): the one whose PML sigma follows initSigmaX.
 int first_active_axis (const yee_config &cfg);
 
 double wave_number (double delta, double lambda, double courant, double nlambda, double a1, double a2);

@@ -40,6 +40,12 @@ extern "C" {
 #define YEE_ABI_VERSION 1
 
 /* SchemeType values of the reference (Source/Settings/Settings.h:47-62). */
+#define YEE_SCHEME_EXHY 1   /* 1D schemes: the grid runs along z (ExHy, EyHx), y (ExHz, EzHx) or x (EyHz, EzHy) */
+#define YEE_SCHEME_EXHZ 2
+#define YEE_SCHEME_EYHX 3
+#define YEE_SCHEME_EYHZ 4
+#define YEE_SCHEME_EZHX 5
+#define YEE_SCHEME_EZHY 6
 #define YEE_SCHEME_TEX 7
 #define YEE_SCHEME_TEY 8
 #define YEE_SCHEME_TEZ 9
@@ -84,7 +90,9 @@ typedef struct yee_config
 {
   int32_t scheme;             /* YEE_SCHEME_DIM3 (--3d) or a 2D scheme YEE_SCHEME_TEX .. TMZ (--2d-tex .. --2d-tmz);
                                  2D grids are [a][b] over the scheme's two axes: TEx/TMx (y, z), TEy/TMy (x, z),
-                                 TEz/TMz (x, y) (main.cpp:817-1000); the third size is ignored */
+                                 TEz/TMz (x, y) (main.cpp:817-1000); the third size is ignored.  1D schemes
+                                 YEE_SCHEME_EXHY .. EZHY (--1d-exhy .. --1d-ezhy) take the size of their one
+                                 axis (main.cpp:629-800) and ignore the other two */
   int32_t layout;             /* --layout-type: only 0 (E_CENTERED) is supported */
   int32_t precision;          /* 4 = FP32 fields, 8 = FP64 fields (reference: -DFLOAT_VALUES / -DDOUBLE_VALUES) */
   int32_t time_steps;         /* --time-steps T; step t = 0 performs no field update (Scheme.inc.h:1032) */

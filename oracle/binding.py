@@ -123,6 +123,8 @@ def sphere_map(cfg: Config, value: float, outside: float, center, radius: int) -
                 out[idx] = L.oracle_sphere_stair_3d(mx, my, mz, cx, cy, cz, r, value, outside)
             else:
                 out[idx] = L.oracle_sphere_accurate_3d(mx, my, mz, cx, cy, cz, r, value, outside, cfg.sphere_accuracy)
+        elif cfg.dim == 1:
+            raise ValueError("spheres on a 1D grid are not driven")
         else:
             # the two axes of the 2D scheme (the sphere centre is given in 3D coordinates)
             ca, cb = [c for a, c in enumerate((cx, cy, cz)) if a != cfg.missing_axis]

@@ -108,10 +108,18 @@ class RefRun
   IS *is;
   int dim;
   int miss;   /* the axis a 2D scheme lacks (-1 in 3D): TEx/TMx 0, TEy/TMy 1, TEz/TMz 2 */
+  int axis1;  /* the axis of a 1D scheme (-1 otherwise) */
+
+  /* an axis the scheme lacks */
+  bool missing (int a) const
+  {
+    return dim == 1 ? a != axis1 : a == miss;
+  }
 
   /* slot of physical axis a among the scheme's coordinates */
   int slot (int a) const
   {
+    if (dim == 1) return 0;
     return (miss >= 0 && a > miss) ? a - 1 : a;
   }
   FPValue fpSlot (const TCFP &c, int sl) const;
@@ -125,7 +133,7 @@ class RefRun
   /* extent along physical axis a (1 along the missing axis) */
   grid_coord sz (int axis) const
   {
-    return axis == miss ? 1 : tcSlot (size, slot (axis));
+    return missing (axis) ? 1 : tcSlot (size, slot (axis));
   }
 
   template <class F>
@@ -134,8 +142,8 @@ class RefRun
     grid_coord l[3], h[3];
     for (int a = 0; a < 3; ++a)
     {
-      l[a] = a == miss ? 0 : lo;
-      h[a] = a == miss ? 1 : sz (a) - hiOff;
+      l[a] = missing (a) ? 0 : lo;
+      h[a] = missing (a) ? 1 : sz (a) - hiOff;
     }
     for (grid_coord i = l[0]; i < h[0]; ++i)
       for (grid_coord j = l[1]; j < h[1]; ++j)
@@ -197,7 +205,7 @@ class RefRun
      * N + 0.5 - P (SchemeHelper.h:632-718). */
     for (int axis = 0; axis < 3; ++axis)
     {
-      if (axis == miss || (miss < 0 && axis >= dim)) continue;
+      if (missing (axis)) continue;
       Grid<TC> *s = axis == 0 ? is->getSigmaX () : (axis == 1 ? is->getSigmaY () : is->getSigmaZ ());
       FPValue P = FPValue (tcSlot (pml, slot (axis)));
       FPValue boundary = P * dx;
@@ -329,13 +337,20 @@ class RefRun
 
 public:
 
-  RefRun (TC s, TC pml, TC tfL, TC tfR, int d, int missing = -1)
-    : size (s), dim (d), miss (d == 3 ? -1 : missing)
+  /* d = 3; d = 2 with the missing axis; d = 1 with the scheme's axis */
+  RefRun (TC s, TC pml, TC tfL, TC tfR, int d, int axis = -1)
+    : size (s), dim (d), miss (d == 2 ? axis : -1), axis1 (d == 1 ? axis : -1)
   {
+    static const CoordinateType kCt[3] = { CoordinateType::X, CoordinateType::Y, CoordinateType::Z };
     ct1 = miss == 0 ? CoordinateType::Y : CoordinateType::X;
     ct2 = miss == 2 ? CoordinateType::Y : CoordinateType::Z;
     if (d == 3) ct2 = CoordinateType::Y;
     ct3 = d == 3 ? CoordinateType::Z : CoordinateType::NONE;
+    if (d == 1)
+    {
+      ct1 = kCt[axis1];
+      ct2 = CoordinateType::NONE;
+    }
     layout = new Layout (s, pml, tfL, tfR,
                          SOLVER_SETTINGS.getIncidentWaveAngle1 () * PhysicsConst::Pi / 180.0,
                          SOLVER_SETTINGS.getIncidentWaveAngle2 () * PhysicsConst::Pi / 180.0,
@@ -460,9 +475,9 @@ public:
     putRecord ("Mu", dumpGrid (is->getMu (), 0));
     if (pml)
     {
-      if (miss != 0) putRecord ("SigmaX", dumpGrid (is->getSigmaX (), 0));
-      if (miss != 1) putRecord ("SigmaY", dumpGrid (is->getSigmaY (), 0));
-      if (dim == 3 || (miss >= 0 && miss != 2)) putRecord ("SigmaZ", dumpGrid (is->getSigmaZ (), 0));
+      if (!missing (0)) putRecord ("SigmaX", dumpGrid (is->getSigmaX (), 0));
+      if (!missing (1)) putRecord ("SigmaY", dumpGrid (is->getSigmaY (), 0));
+      if (!missing (2)) putRecord ("SigmaZ", dumpGrid (is->getSigmaZ (), 0));
     }
     if (meta)
     {
@@ -512,8 +527,37 @@ public:
   {                                                                                                          \
     return sl == 0 ? c.get1 () : c.get2 ();                                                                  \
   }
+#define REF_SLOTS_1D(MODE)                                                                                   \
+  template <>                                                                                                \
+  FPValue RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate>::fpSlot (const GridCoordinateFP1D &c, int) const \
+  {                                                                                                          \
+    return c.get1 ();                                                                                        \
+  }                                                                                                          \
+  template <>                                                                                                \
+  grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate>::tcSlot (const GridCoordinate1D &c, int) const \
+  {                                                                                                          \
+    return c.get1 ();                                                                                        \
+  }
 #ifdef MODE_DIM3
 REF_SLOTS_3D (Dim3)
+#endif
+#ifdef MODE_EX_HY
+REF_SLOTS_1D (Dim1_ExHy)
+#endif
+#ifdef MODE_EX_HZ
+REF_SLOTS_1D (Dim1_ExHz)
+#endif
+#ifdef MODE_EY_HX
+REF_SLOTS_1D (Dim1_EyHx)
+#endif
+#ifdef MODE_EY_HZ
+REF_SLOTS_1D (Dim1_EyHz)
+#endif
+#ifdef MODE_EZ_HX
+REF_SLOTS_1D (Dim1_EzHx)
+#endif
+#ifdef MODE_EZ_HY
+REF_SLOTS_1D (Dim1_EzHy)
 #endif
 #ifdef MODE_TEX
 REF_SLOTS_2D (Dim2_TEx)
@@ -584,6 +628,38 @@ int main (int argc, char **argv)
     run.dump ();                                                                                             \
   }                                                                                                          \
   else
+  /* the lines of main.cpp:629-800 */
+#define REF_RUN_1D(MODE, AXIS, GA, CA)                                                                      \
+  if (st == SchemeType::MODE)                                                                                \
+  {                                                                                                          \
+    GridCoordinate1D s = GRID_COORDINATE_1D (SOLVER_SETTINGS.getSize##GA (), CA);                            \
+    GridCoordinate1D p = GRID_COORDINATE_1D (SOLVER_SETTINGS.getPMLSize##GA (), CA);                         \
+    GridCoordinate1D l = GRID_COORDINATE_1D (SOLVER_SETTINGS.getTFSFSizeLeft##GA (), CA);                    \
+    GridCoordinate1D r = GRID_COORDINATE_1D (SOLVER_SETTINGS.getTFSFSizeRight##GA (), CA);                   \
+    RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate> run (s, p, l, r, 1, AXIS); \
+    run.initMaterials ();                                                                                    \
+    secs = run.run ();                                                                                       \
+    run.dump ();                                                                                             \
+  }                                                                                                          \
+  else
+#ifdef MODE_EX_HY
+  REF_RUN_1D (Dim1_ExHy, 2, Z, CoordinateType::Z)
+#endif
+#ifdef MODE_EX_HZ
+  REF_RUN_1D (Dim1_ExHz, 1, Y, CoordinateType::Y)
+#endif
+#ifdef MODE_EY_HX
+  REF_RUN_1D (Dim1_EyHx, 2, Z, CoordinateType::Z)
+#endif
+#ifdef MODE_EY_HZ
+  REF_RUN_1D (Dim1_EyHz, 0, X, CoordinateType::X)
+#endif
+#ifdef MODE_EZ_HX
+  REF_RUN_1D (Dim1_EzHx, 1, Y, CoordinateType::Y)
+#endif
+#ifdef MODE_EZ_HY
+  REF_RUN_1D (Dim1_EzHy, 0, X, CoordinateType::X)
+#endif
   /* the planes of main.cpp:817-1000 */
 #ifdef MODE_TEX
   REF_RUN_2D (Dim2_TEx, 0, Y, Z, CoordinateType::Y, CoordinateType::Z)

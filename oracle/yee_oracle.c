@@ -260,6 +260,13 @@ static double o_project (const oracle *o, int comp, double v)
   const double s1 = sin (a1), c1 = cos (a1);
   switch (o->cfg.scheme)
   {
+    /* the 1D schemes (YeeGridLayout.h:1247-1760) */
+    case YEE_SCHEME_EXHY: return comp == YEE_EX ? -v : comp == YEE_HY ? -v : 0;
+    case YEE_SCHEME_EXHZ: return comp == YEE_EX ? v : comp == YEE_HZ ? -v : 0;
+    case YEE_SCHEME_EYHX: return comp == YEE_EY ? -v : comp == YEE_HX ? v : 0;
+    case YEE_SCHEME_EYHZ: return comp == YEE_EY ? -v : comp == YEE_HZ ? -v : 0;
+    case YEE_SCHEME_EZHX: return comp == YEE_EZ ? v : comp == YEE_HX ? v : 0;
+    case YEE_SCHEME_EZHY: return comp == YEE_EZ ? v : comp == YEE_HY ? -v : 0;
     case YEE_SCHEME_TEX:   /* YeeGridLayout.h:1761-1846 */
       return comp == YEE_EY ? v * (-c1) : comp == YEE_EZ ? v * s1 : comp == YEE_HX ? v : 0;
     case YEE_SCHEME_TEY:   /* :1847-1932 */
@@ -315,7 +322,12 @@ static double o_incident (const oracle *o, const double real[3], double ddiff, c
   double x = real[0] - o->zero_inc[0];
   double y = real[1] - o->zero_inc[1];
   double z = real[2] - o->zero_inc[2];
-  if (o->dim == 3)
+  if (o->dim == 1)
+  {
+    /* the 1D schemes: the coordinate along the grid (InternalScheme.cpp:110-200) */
+    d = (o->act[0] ? x : o->act[1] ? y : z) - ddiff;
+  }
+  else if (o->dim == 3)
   {
     d = x * sin (o->a1) * cos (o->a2) + y * sin (o->a1) * sin (o->a2) + z * cos (o->a1) - ddiff;
   }
@@ -513,22 +525,25 @@ double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, doubl
 
 oracle *oracle_create (const yee_config *cfg)
 {
-  if (!cfg || cfg->layout != 0 || cfg->scheme < YEE_SCHEME_TEX || cfg->scheme > YEE_SCHEME_DIM3)
+  if (!cfg || cfg->layout != 0 || cfg->scheme < YEE_SCHEME_EXHY || cfg->scheme > YEE_SCHEME_DIM3)
   {
     return NULL;
   }
   oracle *o = (oracle *) calloc (1, sizeof (oracle));
   o->cfg = *cfg;
-  o->dim = cfg->scheme == YEE_SCHEME_DIM3 ? 3 : 2;
+  o->dim = cfg->scheme == YEE_SCHEME_DIM3 ? 3 : cfg->scheme >= YEE_SCHEME_TEX ? 2 : 1;
   /* 2D schemes lack one axis (main.cpp:817-1000: TEx/TMx on (y, z), TEy/TMy on (x, z), TEz/TMz on
    * (x, y)); the grids keep the order of the remaining two, which a 3D index with extent 1 along
    * the missing axis reproduces */
   const int missing = cfg->scheme == YEE_SCHEME_DIM3 ? -1
                       : (cfg->scheme == YEE_SCHEME_TEX || cfg->scheme == YEE_SCHEME_TMX) ? 0
                       : (cfg->scheme == YEE_SCHEME_TEY || cfg->scheme == YEE_SCHEME_TMY) ? 1 : 2;
+  /* 1D schemes run along the axis across both of their components (main.cpp:629-800: ExHy and
+   * EyHx along z, ExHz and EzHx along y, EyHz and EzHy along x) */
+  static const int k1dAxis[7] = { -1, 2, 1, 2, 0, 1, 0 };
   for (int a = 0; a < 3; ++a)
   {
-    o->act[a] = a != missing;
+    o->act[a] = o->dim == 1 ? a == k1dAxis[cfg->scheme] : a != missing;
     o->n[a] = o->act[a] ? cfg->size[a] : 1;
     o->lo[a] = o->act[a] ? 1 : 0;
     o->hi[a] = o->act[a] ? o->n[a] - 1 : 1;
@@ -547,7 +562,11 @@ oracle *oracle_create (const yee_config *cfg)
   {
     /* InternalScheme component flags (InternalScheme.inc.h:3243-3272): TMz Ez, Hx, Hy; TEz Ex, Ey, Hz */
     /* TE<a>: H along the missing axis a and the two E components across it; TM<a> the reverse */
+    /* 1D: the two components the scheme names (InternalScheme.inc.h:3243-3260) */
+    static const int k1dComps[7][2] = { { -1, -1 }, { YEE_EX, YEE_HY }, { YEE_EX, YEE_HZ }, { YEE_EY, YEE_HX },
+                                        { YEE_EY, YEE_HZ }, { YEE_EZ, YEE_HX }, { YEE_EZ, YEE_HY } };
     if (o->dim == 3) o->present[c] = 1;
+    else if (o->dim == 1) o->present[c] = c == k1dComps[cfg->scheme][0] || c == k1dComps[cfg->scheme][1];
     else
     {
       const int te = cfg->scheme <= YEE_SCHEME_TEZ;
@@ -591,7 +610,12 @@ oracle *oracle_create (const yee_config *cfg)
     o->bl[a] = 0.0 + (double) cfg->tfsf_left[a];
     o->br[a] = 0.0 + (double) (cfg->size[a] - cfg->tfsf_right[a]);
   }
-  if (o->dim == 3)
+  if (o->dim == 1)
+  {
+    /* the 1D layouts: leftBorderTotalField - 2.5 along the scheme's axis (YeeGridLayout.cpp:2062-2263) */
+    for (int a = 0; a < 3; ++a) o->zero_inc[a] = o->act[a] ? cfg->tfsf_left[a] - 2.5 : 0;
+  }
+  else if (o->dim == 3)
   {
     o->zero_inc[0] = cfg->tfsf_left[0] - 2.5 * sin (o->a1) * cos (o->a2);
     o->zero_inc[1] = cfg->tfsf_left[1] - 2.5 * sin (o->a1) * sin (o->a2);
@@ -614,7 +638,7 @@ oracle *oracle_create (const yee_config *cfg)
   if (o->use_tfsf)
   {
     /* allocateGridsInc: 500 * size.x cells, 2 layers (InternalScheme.specific.h:99-107) */
-    o->inc_len = 500 * (o->act[0] ? o->n[0] : o->n[1]);   /* size.get1 () of the scheme's grid */
+    o->inc_len = 500 * (o->act[0] ? o->n[0] : o->act[1] ? o->n[1] : o->n[2]);   /* size.get1 () of the scheme's grid */
     for (int l = 0; l < 2; ++l)
     {
       o->einc[l] = (double *) calloc (o->inc_len, sizeof (double));

@@ -86,6 +86,23 @@ def cases(tmp):
         "--2d-tey --size x:25,y:6,z:29 --use-pml --pml-size x:3,z:4 --use-tfsf --tfsf-size-left x:6,z:7 "
         "--tfsf-size-right x:7,z:6 --angle-teta 25 --angle-phi 0 --angle-psi 90 --eps-sphere 2.5 "
         "--eps-sphere-center x:12,y:2,z:14 --eps-sphere-radius 5 --time-steps 50").split()
+    # 1D lines (main.cpp:629-800): ExHy along z with a Drude layer inside PML + TF/SF; EzHy along
+    # x with a point source and J / M currents
+    n1 = 120
+    om1, ga1 = np.zeros(n1), np.zeros(n1)
+    om1[70:85], ga1[70:85] = 1.5 * W0, 0.01 * W0
+    p_om1, p_ga1 = os.path.join(tmp, "omega1d.dat"), os.path.join(tmp, "gamma1d.dat")
+    om1.tofile(p_om1)
+    ga1.tofile(p_ga1)
+    out["exhy_pml_tfsf_drude"] = (
+        f"--1d-exhy --size z:{n1} --use-pml --pml-size z:10 --use-tfsf --tfsf-size-left z:20 "
+        f"--tfsf-size-right z:20 --angle-teta 0 --angle-phi 0 --angle-psi 90 --use-metamaterials "
+        f"--load-omegape-from-file {p_om1} --load-omegapm-from-file {p_om1} --load-gammae-from-file {p_ga1} "
+        f"--load-gammam-from-file {p_ga1} --time-steps 300").split()
+    out["ezhy_pml_sources"] = (
+        "--1d-ezhy --size x:90 --use-pml --pml-size x:8 --point-source x:30 --point-source-ez "
+        "--current-source x:55 --current-source-jz --current-source-my --angle-teta 90 --angle-phi 0 "
+        "--angle-psi 90 --time-steps 200").split()
     return out
 
 

@@ -37,7 +37,7 @@ def test_error_reporting_without_gpu():
     from fdtd3d_amd.config import Config
     L = load_library()
     h = ctypes.c_void_p()
-    bad = Config(scheme=5).to_c()
+    bad = Config(scheme=14).to_c()
     assert L.yee_create(ctypes.byref(bad), ctypes.byref(h)) < 0
     assert b"scheme" in L.yee_last_error()
     assert L.yee_create(None, ctypes.byref(h)) < 0
@@ -62,8 +62,6 @@ def test_parse_rejects_unknown_and_unsupported():
     with pytest.raises(OptionError):
         parse_args(["--no-such-option"])
     with pytest.raises(OptionError):
-        parse_args(["--1d-exhy"])
-    with pytest.raises(OptionError):
         parse_args(["--3d", "--use-metamaterials"])
     with pytest.raises(OptionError):
         parse_args(["--3d", "--use-ntff"])   # the far field needs complex fields
@@ -81,7 +79,7 @@ def test_cli_rejects_options_outside_the_path():
     if not os.path.exists(exe):
         pytest.skip(f"{exe} not built")
     import subprocess
-    for argv in (["--1d-exhy"], ["--no-such-flag"], ["--size"], ["--use-ntff"]):
+    for argv in (["--no-such-flag"], ["--size"], ["--use-ntff"]):
         out = subprocess.run([exe] + argv, capture_output=True, text=True, timeout=60)
         assert out.returncode != 0, argv
         assert out.stderr.startswith("fdtd3d: "), out.stderr
@@ -102,8 +100,19 @@ INVALID_CONFIGS = [
     (dict(size=[10, 10, 10], use_metamaterials=True), b"PML"),
     (dict(size=[10, 10, 10], time_steps=-1), b"time steps"),
     (dict(size=[10, 10, 10], current_source=["Ez"], current_source_pos=[0, 5, 5]), b"current source"),
+    (dict(scheme=1, size=[10, 10, 2]), b"size"),                      # ExHy: the line along z
+    (dict(scheme=4, size=[20, 1, 1], pml=[11, 0, 0], use_pml=True), b"pml"),   # EyHz along x
+    (dict(scheme=0, size=[10, 10, 10]), b"scheme"),
     (dict(size=[10, 10, 10], current_source=["Hx"], current_source_pos=[5, 9, 5]), b"current source"),
 ]
+
+
+def test_parse_1d_schemes():
+    from fdtd3d_amd.config import SCHEME_EXHY, SCHEME_EZHY
+    c = parse_args("--1d-exhy --size z:80 --use-pml --pml-size z:6".split())
+    assert c.scheme == SCHEME_EXHY and c.dim == 1 and c.shape == (80,) and c.components == ["Ex", "Hy"]
+    c = parse_args("--1d-ezhy --size x:40".split())
+    assert c.scheme == SCHEME_EZHY and c.active_axes == [0] and c.shape == (40,) and c.components == ["Ez", "Hy"]
 
 
 def test_parse_current_sources():

@@ -92,6 +92,29 @@ CASES = [
      "--time-steps 9"),
     ("tmy_drude", "--2d-tmy --size x:40,y:4,z:36 --use-pml --pml-size x:6,z:6 --use-tfsf --tfsf-size-left x:9,z:8 "
      "--tfsf-size-right x:9,z:8 --angle-teta 30 --angle-phi 0 --angle-psi 0 --use-metamaterials --time-steps 300"),
+    # the six 1D lines (the 3D half-step kernels with two axes of extent 1; z of extent 1 for the
+    # lines along x and y), at the angles each layout asserts
+    ("line_exhy", "--1d-exhy --size z:70 --use-pml --pml-size z:8 --use-tfsf --tfsf-size-left z:15 "
+     "--tfsf-size-right z:15 --angle-teta 0 --angle-phi 0 --angle-psi 90 --time-steps 90"),
+    ("line_exhz", "--1d-exhz --size y:64 --use-pml --pml-size y:7 --use-tfsf --tfsf-size-left y:14 "
+     "--tfsf-size-right y:16 --angle-teta 90 --angle-phi 90 --angle-psi 0 --point-source y:40 --point-source-hz "
+     "--time-steps 90"),
+    ("line_eyhx", "--1d-eyhx --size z:66 --use-tfsf --tfsf-size-left z:12 --tfsf-size-right z:12 --angle-teta 0 "
+     "--angle-phi 90 --angle-psi 90 --current-source z:30 --current-source-jy --time-steps 80"),
+    ("line_eyhz", "--1d-eyhz --size x:72 --use-pml --pml-size x:9 --use-tfsf --tfsf-size-left x:15 "
+     "--tfsf-size-right x:15 --angle-teta 90 --angle-phi 0 --angle-psi 0 --time-steps 90"),
+    ("line_ezhx", "--1d-ezhx --size y:60 --use-pml --pml-size y:6 --point-source y:25 --point-source-ez "
+     "--point-source-hx --angle-teta 90 --angle-phi 90 --angle-psi 90 --time-steps 80"),
+    ("line_ezhy", "--1d-ezhy --size x:64 --use-pml --pml-size x:8 --use-tfsf --tfsf-size-left x:16 "
+     "--tfsf-size-right x:14 --angle-teta 90 --angle-phi 0 --angle-psi 90 --current-source x:32 "
+     "--current-source-mz --time-steps 90"),
+    # a resonant Drude layer on a line along x; the smallest line; a long line along z
+    ("line_eyhz_drude", "--1d-eyhz --size x:120 --use-pml --pml-size x:10 --use-tfsf --tfsf-size-left x:20 "
+     "--tfsf-size-right x:20 --angle-teta 90 --angle-phi 0 --angle-psi 0 --use-metamaterials --time-steps 300"),
+    ("min_line", "--1d-ezhx --size y:3 --point-source y:1 --point-source-ez --angle-teta 90 --angle-phi 90 "
+     "--angle-psi 90 --time-steps 9"),
+    ("long_line", "--1d-exhy --size z:3000 --use-pml --pml-size z:40 --use-tfsf --tfsf-size-left z:100 "
+     "--tfsf-size-right z:100 --angle-teta 0 --angle-phi 0 --angle-psi 90 --time-steps 400"),
     # a point source off the grid is ignored
     ("source_off_grid", "--3d --size x:12,y:10,z:14 --use-pml --pml-size x:2,y:2,z:2 --point-source x:30,y:4,z:5 "
      "--point-source-ez --time-steps 15"),
@@ -105,6 +128,13 @@ def _materials(name, cfg):
         ga = np.zeros(cfg.shape)
         om[15:25, 13:23] = 1.5 * w0
         ga[15:25, 13:23] = 0.01 * w0
+        return {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
+    if name == "line_eyhz_drude":
+        w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
+        om = np.zeros(cfg.shape)
+        ga = np.zeros(cfg.shape)
+        om[70:85] = 1.5 * w0
+        ga[70:85] = 0.01 * w0
         return {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
     if name != "rows_1025":
         return {}

@@ -64,6 +64,19 @@ CASES = [
     "--current-source-mx --time-steps 50",
     "--2d-tmx --size x:4,y:24,z:26 --use-pml --pml-size y:3,z:3 --current-source x:0,y:11,z:12 "
     "--current-source-jx --current-source-mz --angle-phi 90 --angle-psi 0 --time-steps 40",
+    # the six 1D lines, each at the angles its layout asserts (YeeGridLayout.cpp:2065-2290)
+    "--1d-exhy --size z:70 --use-pml --pml-size z:8 --use-tfsf --tfsf-size-left z:15 --tfsf-size-right z:15 "
+    "--angle-teta 0 --angle-phi 0 --angle-psi 90 --time-steps 90",
+    "--1d-exhz --size y:64 --use-pml --pml-size y:7 --use-tfsf --tfsf-size-left y:14 --tfsf-size-right y:16 "
+    "--angle-teta 90 --angle-phi 90 --angle-psi 0 --point-source y:40 --point-source-hz --time-steps 90",
+    "--1d-eyhx --size z:66 --use-tfsf --tfsf-size-left z:12 --tfsf-size-right z:12 --angle-teta 0 "
+    "--angle-phi 90 --angle-psi 90 --current-source z:30 --current-source-jy --time-steps 80",
+    "--1d-eyhz --size x:72 --use-pml --pml-size x:9 --use-tfsf --tfsf-size-left x:15 --tfsf-size-right x:15 "
+    "--angle-teta 90 --angle-phi 0 --angle-psi 0 --time-steps 90",
+    "--1d-ezhx --size y:60 --use-pml --pml-size y:6 --point-source y:25 --point-source-ez --point-source-hx "
+    "--angle-teta 90 --angle-phi 90 --angle-psi 90 --time-steps 80",
+    "--1d-ezhy --size x:64 --use-pml --pml-size x:8 --use-tfsf --tfsf-size-left x:16 --tfsf-size-right x:14 "
+    "--angle-teta 90 --angle-phi 0 --angle-psi 90 --current-source x:32 --current-source-mz --time-steps 90",
     # TEz at phi = 0 without PML
     "--2d-tez --size x:24,y:20 --use-tfsf --tfsf-size-left x:5,y:5 --tfsf-size-right x:5,y:5 --angle-teta 90 "
     "--angle-phi 0 --angle-psi 0 --time-steps 40",
