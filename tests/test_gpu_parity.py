@@ -241,6 +241,37 @@ def test_current_sources_vs_oracle(case, precision, engine_lib):
     eng.close()
 
 
+@pytest.mark.parametrize("args,box", [
+    ("--2d-tmz --size x:40,y:36 --use-pml --pml-size x:6,y:6 --use-metamaterials --current-source x:20,y:18 "
+     "--current-source-jz --current-source-mx --time-steps 150", (slice(15, 25), slice(13, 23))),
+    ("--3d --size x:30,y:28,z:32 --use-pml --pml-size x:4,y:4,z:4 --use-metamaterials --current-source x:15,y:14,z:16 "
+     "--current-source-jx --current-source-mz --time-steps 80", (slice(12, 19), slice(11, 18), slice(13, 20))),
+])
+@pytest.mark.parametrize("precision", [8, 4])
+def test_current_source_inside_drude_box(args, box, precision, engine_lib):
+    """A J / M source inside a resonant Drude region: the current enters D / B before the ADE
+    stage (FP32: the increment form of the ADE carries it as -jdx*cbc in dD)."""
+    cfg = parse_args(args.split())
+    w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
+    om, ga = np.zeros(cfg.shape), np.zeros(cfg.shape)
+    om[box], ga[box] = 1.5 * w0, 0.01 * w0
+    mats = {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
+    o = run_oracle(cfg, mats)
+    cfg.precision = precision
+    say(f"[gpu] current source in Drude box precision {precision}: {args[:40]}")
+    eng = run_engine(cfg, mats)
+    names = field_names(cfg)
+    if precision == 8:
+        for f in names:
+            assert rel_l2(eng.field(f), o.field(f)) <= TOL_FP64, f
+    else:
+        for group in ("E", "H"):
+            ns = [n for n in names if n[0] == group]
+            r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
+            assert r <= TOL_FP32, f"{group}: {r}"
+    eng.close()
+
+
 def test_special_mask_covers_every_nonzero_sigma(engine_lib):
     cfg = parse_args("--3d --size x:20,y:18,z:22 --use-pml --pml-size x:4,y:3,z:5 --time-steps 1".split())
     say("[gpu] special mask")
