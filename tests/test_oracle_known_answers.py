@@ -179,3 +179,75 @@ def test_point_source_closed_form(sigma, steps):
         if sigma > 0:
             want *= math.exp(-((ts - t0) / sigma) ** 2)
         assert o.field(name)[11, 17] == want, name
+
+
+def _unit_test_1d_vectors():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "unit_test_1d_vectors.json")
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def unit_test_1d_config(mode, v):
+    """The 1D grid of the reference unit test (unit-test-internalscheme.cpp:33-38, test1D_<mode>):
+    20 cells along the mode's axis, TF/SF 7 cells from each end, dx 0.002, lambda 0.2 (exact
+    doubles, not CLI-rounded), 200 steps."""
+    ax = v["axis"]
+    t, p, s = v["angles"]
+    cfg = parse_args(f"--1d-{mode.lower()} --size {ax}:20 --use-tfsf --tfsf-size-left {ax}:7 "
+                     f"--tfsf-size-right {ax}:7 --angle-teta {t:g} --angle-phi {p:g} --angle-psi {s:g} "
+                     "--time-steps 200".split())
+    cfg.dx = 0.002
+    cfg.wavelength = 0.2
+    return cfg
+
+
+@pytest.mark.parametrize("mode", ["ExHy", "ExHz", "EyHx", "EyHz", "EzHx", "EzHy"])
+def test_unit_test_1d_golden_vectors(mode):
+    """The reference's hand-written 1D golden vectors (unit-test-internalscheme.cpp:1033-1600,
+    fixture written by tests/golden/extract_unit_test_vectors.py).  They come from a complex-field
+    build whose plane-wave source is sin(arg) + i cos(arg) (InternalScheme.inc.h:3309-3313); every
+    update coefficient is real, so the real part is the real-field run with the sin source: it must
+    match bit for bit.  The imaginary part is the run with the source shifted by pi/2 (sin(arg +
+    pi/2) rounds differently from cos(arg)): within the unit test's own bar, 1e-7
+    (FPEXACT_ACCURACY, Kernels/FieldValue.h:150-160)."""
+    v = _unit_test_1d_vectors()[mode]
+    cfg = unit_test_1d_config(mode, v)
+    re = Oracle(cfg)
+    re.prepare()
+    re.step(200)
+    np.testing.assert_array_equal(re.field(v["component"]).ravel(), np.array(v["re"]))
+    im = Oracle(cfg)
+    im.set_source_phase(math.pi / 2)
+    im.prepare()
+    im.step(200)
+    np.testing.assert_allclose(im.field(v["component"]).ravel(), np.array(v["im"]), rtol=0, atol=1e-7)
+    # the TF/SF region carries a unit-amplitude wave, zero outside (test1D, :866-877)
+    amp = np.hypot(re.field(v["component"]).ravel(), im.field(v["component"]).ravel())
+    inside = amp > 0
+    assert inside.sum() == 6 and np.all(np.abs(amp[inside] - 1) < math.sqrt(0.000018))
+
+
+def test_sigma_x_closed_form_telescoping():
+    """initSigmaX (SchemeHelper.h:597-634): the eps-point coordinate i + 1/2 against a fractional
+    threshold, distance P - (i + 1/2) on the left and i + 1/2 - (N - P) on the right.  The cell
+    values are the reference's polynomial grading sigma_max (x/L)^6 integrated over one cell
+    (initSigma, :574-592), so over a slab they telescope: sum sigma dx = sigma_max dx
+    ((P + 1/2)^7 - (1/2)^7) / (7 P^6), with the x-cells [0, P) and [N - P, N) nonzero, mirror
+    images of each other.  A whole-cell distance (P - i) would give (P + 1)^7 - 1, a threshold of
+    N + 1/2 - P (the y / z rule) P^7: both fail here."""
+    for P, N, dx in ((6, 30, 0.0005), (16, 64, 0.000625), (3, 11, 0.002)):
+        cfg = Config(scheme=SCHEME_DIM3, size=[N, N, N], pml=[P, P, P], dx=dx, use_pml=True, time_steps=1)
+        o = Oracle(cfg)
+        o.prepare()
+        eps0 = 1 / (4 * math.pi * 1e-7 * 299792458.0 ** 2)
+        mu0 = 4 * math.pi * 1e-7
+        smax = -math.log(1e-16) * 7 / (2 * math.sqrt(mu0 / eps0) * (P * dx))
+        sx = o.material(6)[:, 0, 0]
+        want = smax * dx * ((P + 0.5) ** 7 - 0.5 ** 7) / (7 * P ** 6)
+        assert math.isclose(sx[:P].sum() * dx, want, rel_tol=1e-12)
+        assert math.isclose(sx[N - P:].sum() * dx, want, rel_tol=1e-12)
+        assert np.all(sx[:P] > 0) and np.all(sx[N - P:] > 0) and np.all(sx[P:N - P] == 0)
+        np.testing.assert_allclose(sx, sx[::-1], rtol=1e-13, atol=0)
+        assert np.all(np.diff(sx[:P]) < 0)
