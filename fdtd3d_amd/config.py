@@ -225,13 +225,18 @@ _IGNORED_FLAGS = {
     "--save-as-txt", "--save-tfsf-e-incident", "--save-tfsf-h-incident", "--manual-save-coords",
     "--save-no-pml", "--palette-gray", "--palette-rgb", "--orth-axis-x", "--orth-axis-y",
     "--orth-axis-z", "--dynamic-grid", "--check-disabling-cond", "--check-enabling-cond",
-    "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
-    "--ntff-to-stdout",
+    "--ntff-reverse", "--ntff-to-stdout",
+    # precomputed coefficient grids (Scheme.inc.h:3309-3791): the same expressions as the
+    # per-cell path the engine computes, so the results are the same with or without them
+    "--use-ca-cb", "--use-ca-cb-pml",
 }
 # options of the reference whose feature is outside this engine: rejected loudly
 _UNSUPPORTED = {
 
     "--use-double-material-precision", "--eps-normed", "--mu-normed",
+    # the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
+    # (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
+    "--use-ca-cb-pml-metamaterials",
     # the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
     "--use-ntff",
 }

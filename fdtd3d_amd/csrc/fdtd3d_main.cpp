@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <set>
 #include <string>
 #include <type_traits>
@@ -122,11 +123,16 @@ const std::set<std::string> kIgnoredWithArg = {
 const std::set<std::string> kIgnoredFlags = {
   "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
   "--palette-gray", "--palette-rgb", "--dynamic-grid", "--check-disabling-cond",
-  "--check-enabling-cond", "--use-ca-cb", "--use-ca-cb-pml", "--use-ca-cb-pml-metamaterials", "--ntff-reverse",
-  "--ntff-to-stdout",
+  "--check-enabling-cond", "--ntff-reverse", "--ntff-to-stdout",
+  // precomputed coefficient grids (Scheme.inc.h:3309-3791): the same expressions as the per-cell
+  // path the engine computes, so the results are the same with or without them
+  "--use-ca-cb", "--use-ca-cb-pml",
 };
 const std::set<std::string> kUnsupported = {
   "--use-double-material-precision", "--eps-normed", "--mu-normed",
+  // the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
+  // (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
+  "--use-ca-cb-pml-metamaterials",
   // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
   "--use-ntff",
 };
@@ -293,6 +299,51 @@ std::vector<double> load_dat (const std::string &path, int64_t n)
   if (in.gcount () != (std::streamsize) (n * sizeof (FileValue)))
     die (path + ": expected " + std::to_string (n) + " values of " + std::to_string (sizeof (FileValue)) + " bytes");
   return std::vector<double> (raw.begin (), raw.end ());
+}
+
+// TXTLoader::loadFromFile (Source/File-Management/Loader/TXTLoader.h:55-104, TXTLoader.cpp:25-78):
+// one line per cell in [x][y][z] order (z fastest), the cell's dim indices then its value; the
+// indices must name the cell the line is read for; values are read with std::stof (STOF,
+// Helpers/PAssert.h:170), so they are float-rounded in either build; nothing may follow the last
+// line.
+std::vector<double> load_txt (const std::string &path, int dim, const int64_t size[3])
+{
+  std::ifstream in (path);
+  if (!in) die ("cannot open " + path);
+  const int64_t n = size[0] * size[1] * size[2];
+  std::vector<double> v ((size_t) n);
+  std::string line;
+  int64_t q = 0;
+  for (int64_t x = 0; x < size[0]; ++x)
+    for (int64_t y = 0; y < size[1]; ++y)
+      for (int64_t z = 0; z < size[2]; ++z, ++q)
+      {
+        if (!std::getline (in, line)) die (path + ": expected " + std::to_string (n) + " lines");
+        std::istringstream ss (line);
+        std::vector<std::string> tok;
+        std::string w;
+        while (ss >> w) tok.push_back (w);
+        if ((int) tok.size () < dim + 1) die (path + ": line " + std::to_string (q + 1) + ": expected indices and a value");
+        const int64_t want[3] = { x, y, z };
+        for (int d = 0; d < dim; ++d)
+          if (std::stoll (tok[d]) != want[d]) die (path + ": line " + std::to_string (q + 1) + ": indices do not match");
+        v[(size_t) q] = (double) std::stof (tok[dim]);
+      }
+  in.peek ();
+  if (!in.eof ()) die (path + ": more lines than cells");
+  return v;
+}
+
+// Material file by extension, as GridFileManager::getFileType (File-Management/Commons.cpp:45-67)
+// chooses the loader: .dat raw values, .txt text lines; .bmp images are not read.
+std::vector<double> load_material (const std::string &path, int dim, const int64_t size[3])
+{
+  const size_t dot = path.find_last_of ('.');
+  const std::string ext = dot == std::string::npos ? "" : path.substr (dot + 1);
+  if (ext == "txt") return load_txt (path, dim, size);
+  if (ext == "dat") return load_dat (path, size[0] * size[1] * size[2]);
+  die (path + ": material files are read as .dat or .txt (the reference's BMP loader is not built)");
+  return {};
 }
 
 // Minimum coordinate of a component in the E_CENTERED layout, doubled (YeeGridLayout.cpp:1937-2043):
@@ -550,7 +601,7 @@ int main (int argc, char **argv)
     if (m >= 2 && !meta) continue;
     if (!o.file[m].empty ())
     {
-      std::vector<double> v = load_dat (o.file[m], n);
+      std::vector<double> v = load_material (o.file[m], scheme_dim (c), ssize);
       check (yee_set_material (h, m, v.data ()));
     }
     if (m < 4)

@@ -21,7 +21,13 @@
  *   - the step ordering of Scheme::performNStepsForBlock (Scheme/Scheme.inc.h:28-473)
  *     and the three sweeps of Scheme::calculateFieldStep (Scheme/Scheme.inc.h:876-1344);
  *   - the material set-up of Scheme::initGrids (Scheme/Scheme.inc.h:2962-3165);
- *   - the PML sigma profiles of SchemeHelper::initSigma{,X,Y,Z} (Scheme/SchemeHelper.h:574-718).
+ *   - the PML sigma profiles of SchemeHelper::initSigma{,X,Y,Z} (Scheme/SchemeHelper.h:574-718);
+ *   - with --use-ca-cb / --use-ca-cb-pml, the coefficient grids Scheme::initGrids precomputes
+ *     (Scheme/Scheme.inc.h:3309-3548, 3549-3791), which the reference's own
+ *     usePrecomputedGrids = true iterations then read.  --use-ca-cb-pml-metamaterials is
+ *     refused: the reference's binding (InternalScheme.inc.h:1902-2295) never sets the a1 / a2
+ *     grids (CA1 / CA2 stay null, Scheme.inc.h:932) and the precomputed Drude iteration
+ *     dereferences them (InternalScheme.inc.h:2501-2502).
  *
  * Output: one binary file (path from env YEE_REF_OUT, default "ref_out.bin") with
  * records  [char name[16]][int64 count][double values[count]]  holding the latest
@@ -227,6 +233,75 @@ class RefRun
     }
   }
 
+  /* The coefficient grids of --use-ca-cb / --use-ca-cb-pml for component grid_type, as
+   * Scheme::initGrids fills them (Scheme.inc.h:3309-3548 curl Ca / Cb, 3549-3791 PML D -> E
+   * Ca / Cb / Cc), over the cells the sweeps update; the material grids come from the
+   * reference's own binding (calculateFieldStepInit). */
+  template <uint8_t grid_type, bool usePML, bool useMeta>
+  void fillCoefficients ()
+  {
+    Grid<TC> *grid = NULLPTR;
+    GridType gridType = GridType::NONE;
+    Grid<TC> *mg = NULLPTR, *mg1 = NULLPTR, *mg2 = NULLPTR, *mg3 = NULLPTR, *mg4 = NULLPTR, *mg5 = NULLPTR;
+    GridType mgt = GridType::NONE, mgt1 = GridType::NONE, mgt2 = GridType::NONE, mgt3 = GridType::NONE,
+             mgt4 = GridType::NONE, mgt5 = GridType::NONE;
+    Grid<TC> *opp1 = NULLPTR, *opp2 = NULLPTR, *pml1 = NULLPTR, *pml2 = NULLPTR;
+    GridType pmlType1 = GridType::NONE, pmlType2 = GridType::NONE;
+    Grid<TC> *Ca = NULLPTR, *Cb = NULLPTR, *CB0 = NULLPTR, *CB1 = NULLPTR, *CB2 = NULLPTR, *CA1 = NULLPTR,
+             *CA2 = NULLPTR, *CaPML = NULLPTR, *CbPML = NULLPTR, *CcPML = NULLPTR;
+    SourceCallBack rhs = NULLPTR, border = NULLPTR, exact = NULLPTR;
+    FPValue modifier = 0;
+    is->template calculateFieldStepInit<grid_type, usePML, useMeta> (&grid, &gridType,
+      &mg, &mgt, &mg1, &mgt1, &mg2, &mgt2, &mg3, &mgt3, &mg4, &mgt4, &mg5, &mgt5,
+      &opp1, &opp2, &pml1, &pmlType1, &pml2, &pmlType2, &rhs, &border, &exact, &modifier,
+      &Ca, &Cb, &CB0, &CB1, &CB2, &CA1, &CA2, &CaPML, &CbPML, &CcPML);
+    const FPValue dt = is->getGridTimeStep (), dx = is->getGridStep ();
+    const FPValue eps0 = PhysicsConst::Eps0, k_mod = FPValue (1);
+    forAll (1, 1, [&] (TC p) {
+      TC posAbs = grid->getTotalPosition (p);
+      if (Ca != NULLPTR)
+      {
+        FPValue ca, cb;
+        if (usePML)
+        {
+          FPValue material = mg ? is->getMaterial (posAbs, gridType, mg, mgt) : 0;
+          FPValue dd = (2 * eps0 * k_mod + material * dt);
+          ca = (2 * eps0 * k_mod - material * dt) / dd;
+          cb = (2 * eps0 * dt / dx) / dd;
+        }
+        else
+        {
+          FPValue material = is->getMaterial (posAbs, gridType, mg, mgt);
+          ca = FPValue (1);
+          cb = dt / (material * modifier * dx);   /* modifier: PhysicsConst::Eps0 (E) or Mu0 (H) */
+        }
+        Ca->setFieldValue (FIELDVALUE (ca, FPValue (0)), p, 0);
+        Cb->setFieldValue (FIELDVALUE (cb, FPValue (0)), p, 0);
+      }
+      if (usePML && CaPML != NULLPTR)
+      {
+        FPValue material1 = is->getMaterial (posAbs, pmlType1, mg1, mgt1);
+        FPValue material4 = mg4 ? is->getMaterial (posAbs, pmlType1, mg4, mgt4) : 0;
+        FPValue material5 = mg5 ? is->getMaterial (posAbs, pmlType1, mg5, mgt5) : 0;
+        FPValue mod = useMeta ? FPValue (1) : material1 * modifier;
+        FPValue dd = (2 * eps0 * k_mod + material5 * dt);
+        CaPML->setFieldValue (FIELDVALUE ((2 * eps0 * k_mod - material5 * dt) / dd, FPValue (0)), p, 0);
+        CbPML->setFieldValue (FIELDVALUE (((2 * eps0 * k_mod + material4 * dt) / (mod)) / dd, FPValue (0)), p, 0);
+        CcPML->setFieldValue (FIELDVALUE (((2 * eps0 * k_mod - material4 * dt) / (mod)) / dd, FPValue (0)), p, 0);
+      }
+    });
+  }
+
+  template <uint8_t grid_type>
+  void fillCoefficientsDispatch ()
+  {
+    bool pml = SOLVER_SETTINGS.getDoUsePML ();
+    bool meta = SOLVER_SETTINGS.getDoUseMetamaterials ();
+    if (pml && meta) fillCoefficients<grid_type, true, true> ();
+    else if (pml) fillCoefficients<grid_type, true, false> ();
+    else fillCoefficients<grid_type, false, false> ();
+  }
+
   template <uint8_t grid_type, bool usePML, bool useMeta>
   void fieldStep (time_step t)
   {
@@ -257,8 +332,12 @@ class RefRun
       FPValue timestep = isE ? FPValue (t) : FPValue (t) + FPValue (0.5);
       TCFP coordFP;
       forAll (1, 1, [&] (TC p) {
-        is->template calculateFieldStepIteration<grid_type, false> (timestep, p, grid->getTotalPosition (p),
-          d11, d12, d21, d22, grid, coordFP, opp1, opp2, rhs, Ca, Cb, usePML, gridType, mg, mgt, modifier);
+        if (Ca != NULLPTR)
+          is->template calculateFieldStepIteration<grid_type, true> (timestep, p, grid->getTotalPosition (p),
+            d11, d12, d21, d22, grid, coordFP, opp1, opp2, rhs, Ca, Cb, usePML, gridType, mg, mgt, modifier);
+        else
+          is->template calculateFieldStepIteration<grid_type, false> (timestep, p, grid->getTotalPosition (p),
+            d11, d12, d21, d22, grid, coordFP, opp1, opp2, rhs, Ca, Cb, usePML, gridType, mg, mgt, modifier);
       });
       /* current source after the curl sweep (Scheme.inc.h:1171-1260) */
       bool cur = false;
@@ -276,8 +355,12 @@ class RefRun
       {
         FieldValue current = sin (is->getGridTimeStep () * timestep * FPValue (2) * PhysicsConst::Pi * is->getSourceFrequency ());
         current = current * PhysicsConst::Mu0 / is->getGridTimeStep () / FPValue (2.0);
-        is->template calculateFieldStepIterationCurrent<grid_type, false> (current, grid, Ca, Cb, usePML, gridType, mg, mgt,
-                                                                           modifier);
+        if (Ca != NULLPTR)
+          is->template calculateFieldStepIterationCurrent<grid_type, true> (current, grid, Ca, Cb, usePML, gridType, mg,
+                                                                            mgt, modifier);
+        else
+          is->template calculateFieldStepIterationCurrent<grid_type, false> (current, grid, Ca, Cb, usePML, gridType, mg,
+                                                                             mgt, modifier);
       }
       if (usePML)
       {
@@ -289,8 +372,12 @@ class RefRun
           });
         }
         forAll (1, 1, [&] (TC p) {
-          is->template calculateFieldStepIterationPML<useMeta, false> (t, p, grid, pml1, pml2, CaPML, CbPML, CcPML,
-            pmlType1, mg1, mgt1, mg4, mgt4, mg5, mgt5, modifier);
+          if (CaPML != NULLPTR)
+            is->template calculateFieldStepIterationPML<useMeta, true> (t, p, grid, pml1, pml2, CaPML, CbPML, CcPML,
+              pmlType1, mg1, mgt1, mg4, mgt4, mg5, mgt5, modifier);
+          else
+            is->template calculateFieldStepIterationPML<useMeta, false> (t, p, grid, pml1, pml2, CaPML, CbPML, CcPML,
+              pmlType1, mg1, mgt1, mg4, mgt4, mg5, mgt5, modifier);
         });
       }
     }
@@ -406,6 +493,16 @@ public:
     if (SOLVER_SETTINGS.getDoUsePML ())
     {
       initSigmas ();
+    }
+    if (SOLVER_SETTINGS.getDoUseCaCbGrids () || SOLVER_SETTINGS.getDoUseCaCbPMLGrids ()
+        || SOLVER_SETTINGS.getDoUseCaCbPMLMetaGrids ())
+    {
+      if (is->getDoNeedEx ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::EX)> ();
+      if (is->getDoNeedEy ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::EY)> ();
+      if (is->getDoNeedEz ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::EZ)> ();
+      if (is->getDoNeedHx ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::HX)> ();
+      if (is->getDoNeedHy ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::HY)> ();
+      if (is->getDoNeedHz ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::HZ)> ();
     }
   }
 
@@ -587,6 +684,11 @@ int main (int argc, char **argv)
   if (SOLVER_SETTINGS.getLayoutType () != E_CENTERED)
   {
     fprintf (stderr, "only --layout-type 0 (E_CENTERED) is driven\n");
+    return 3;
+  }
+  if (SOLVER_SETTINGS.getDoUseCaCbPMLMetaGrids ())
+  {
+    fprintf (stderr, "--use-ca-cb-pml-metamaterials: the reference never binds its a1 / a2 grids\n");
     return 3;
   }
   const char *outName = getenv ("YEE_REF_OUT");

@@ -65,6 +65,18 @@ def test_parse_rejects_unknown_and_unsupported():
         parse_args(["--3d", "--use-metamaterials"])
     with pytest.raises(OptionError):
         parse_args(["--3d", "--use-ntff"])   # the far field needs complex fields
+    with pytest.raises(OptionError):
+        # the reference's precomputed Drude path dereferences a1 / a2 grids it never binds
+        parse_args("--3d --use-pml --use-metamaterials --use-ca-cb-pml-metamaterials".split())
+
+
+def test_parse_accepts_precomputed_coefficient_flags():
+    """--use-ca-cb / --use-ca-cb-pml select the reference's precomputed coefficient grids, whose
+    results equal the per-cell path (tests/test_oracle_ref_live.py): accepted, same config."""
+    base = parse_args("--3d --size x:20 --same-size --use-pml --pml-size x:4 --same-size-pml".split())
+    cacb = parse_args("--3d --size x:20 --same-size --use-pml --pml-size x:4 --same-size-pml --use-ca-cb "
+                      "--use-ca-cb-pml".split())
+    assert base == cacb
 
 
 def test_cmd_file(tmp_path):
@@ -79,7 +91,8 @@ def test_cli_rejects_options_outside_the_path():
     if not os.path.exists(exe):
         pytest.skip(f"{exe} not built")
     import subprocess
-    for argv in (["--no-such-flag"], ["--size"], ["--use-ntff"]):
+    for argv in (["--no-such-flag"], ["--size"], ["--use-ntff"],
+                 ["--use-pml", "--use-metamaterials", "--use-ca-cb-pml-metamaterials"]):
         out = subprocess.run([exe] + argv, capture_output=True, text=True, timeout=60)
         assert out.returncode != 0, argv
         assert out.stderr.startswith("fdtd3d: "), out.stderr

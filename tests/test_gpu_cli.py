@@ -341,3 +341,43 @@ def test_cli_1d_lines(args, engine_lib, tmp_path):
         assert rel_l2(rows[:, 1], o.field(name)) <= 1e-12, name
     for name in set(["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]) - set(cfg.components):
         assert not (tmp_path / f"previous-1_[timestep=40]_[pid=0]_[name={name}].txt").exists(), name
+
+
+def test_cli_txt_material_input(engine_lib, tmp_path):
+    """A .txt material file (TXTLoader: one 'x y z value' line per cell, z fastest, values read
+    with std::stof) gives the same run as the .dat file of the same values; lines whose indices
+    do not name their cell, and files with a line too many, are refused."""
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    shape = (16, 14, 18)
+    base = ("--3d --size x:16,y:14,z:18 --use-pml --pml-size x:3 --same-size-pml --use-tfsf "
+            "--tfsf-size-left x:5 --tfsf-size-right x:5 --same-size-tfsf --angle-teta 30 --angle-phi 45 "
+            "--angle-psi 90 --time-steps 25 --save-res --save-as-dat --manual-save-coords "
+            "--save-start-coord x:0,y:0,z:0 --save-end-coord x:16,y:14,z:18")
+    eps = np.random.default_rng(11).uniform(1.0, 6.0, size=shape).astype(np.float32)
+    _write_dat(tmp_path / "eps.dat", eps.astype(np.float64), np.float64)
+    lines = [f"{x} {y} {z} {float(eps[x, y, z]):.9g}" for x in range(shape[0]) for y in range(shape[1])
+             for z in range(shape[2])]
+    (tmp_path / "eps.txt").write_text("\n".join(lines) + "\n")
+    say("[gpu] cli txt material input")
+    for name in ("dat", "txt"):
+        d = tmp_path / name
+        d.mkdir()
+        out = subprocess.run([exe] + (base + f" --load-eps-from-file ../eps.{name}").split(), capture_output=True,
+                             text=True, timeout=120, cwd=d)
+        assert out.returncode == 0, out.stderr
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        f = f"previous-1_[timestep=25]_[pid=0]_[name={comp}].dat"
+        a = np.fromfile(tmp_path / "dat" / f, dtype=np.float64)
+        b = np.fromfile(tmp_path / "txt" / f, dtype=np.float64)
+        assert np.abs(a).max() > 0
+        np.testing.assert_array_equal(a, b, err_msg=comp)
+    bad = list(lines)
+    bad[5] = "0 0 6 2.0"   # names cell (0, 0, 6) on the line of (0, 0, 5)
+    (tmp_path / "bad.txt").write_text("\n".join(bad) + "\n")
+    (tmp_path / "long.txt").write_text("\n".join(lines + ["0 0 0 1.0"]) + "\n")
+    for f in ("bad.txt", "long.txt"):
+        out = subprocess.run([exe] + (base + f" --load-eps-from-file {f}").split(), capture_output=True, text=True,
+                             timeout=120, cwd=tmp_path)
+        assert out.returncode != 0 and f in out.stderr, (f, out.stderr)

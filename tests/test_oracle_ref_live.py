@@ -119,3 +119,48 @@ def test_reference_fp32_deviation_next_to_current_source(tmp_path):
     den = sum(float(np.sum(o.field(n) ** 2)) for n in names)
     r = np.sqrt(num / den)
     assert 1e-5 < r < 2e-5, r
+
+
+CA_CB_CASES = [
+    # plain (Ca / Cb grids of the curl update), eps and mu spheres, oblique TF/SF, current source
+    "--3d --size x:16,y:14,z:15 --use-tfsf --tfsf-size-left x:4,y:4,z:4 --tfsf-size-right x:4,y:4,z:4 "
+    "--angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 3 --eps-sphere-center x:8,y:7,z:7 "
+    "--eps-sphere-radius 4 --mu-sphere 2 --mu-sphere-center x:7,y:7,z:8 --mu-sphere-radius 3 "
+    "--current-source x:6,y:6,z:6 --current-source-jy --time-steps 30",
+    # PML: curl grids with sigma, D -> E grids (Ca / Cb / Cc), point source
+    "--3d --size x:18,y:16,z:17 --use-pml --pml-size x:4,y:3,z:4 --use-tfsf --tfsf-size-left x:6,y:5,z:6 "
+    "--tfsf-size-right x:6,y:5,z:6 --angle-teta 60 --angle-phi 20 --angle-psi 30 --eps-sphere 2.5 "
+    "--eps-sphere-center x:9,y:8,z:8 --eps-sphere-radius 3 --point-source x:8,y:8,z:8 --point-source-hz "
+    "--time-steps 30",
+    # PML + Drude terms (the Drude coefficients stay per cell: the reference's
+    # --use-ca-cb-pml-metamaterials dereferences a1 / a2 grids its binding never sets)
+    "--3d --size x:16,y:16,z:16 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:5 "
+    "--tfsf-size-right x:5 --same-size-tfsf --angle-teta 0 --angle-phi 0 --angle-psi 0 --use-metamaterials "
+    "--omegape-sphere 1.5 --omegape-sphere-center x:8,y:8,z:8 --omegape-sphere-radius 3 --omegapm-sphere 1.5 "
+    "--omegapm-sphere-center x:8,y:8,z:8 --omegapm-sphere-radius 3 --time-steps 30",
+    # TMz with PML
+    "--2d-tmz --size x:30,y:26 --use-pml --pml-size x:5,y:3 --point-source x:12,y:9 --point-source-ez "
+    "--time-steps 40",
+]
+
+
+@pytest.mark.parametrize("args", CA_CB_CASES)
+def test_reference_precomputed_coefficients_equal_default_path(args, tmp_path):
+    """--use-ca-cb / --use-ca-cb-pml make the reference read coefficient grids precomputed on
+    the host (Scheme.inc.h:3309-3791) instead of computing them per cell (computeCaCb,
+    InternalScheme.inc.h:2297-2349, and the per-cell branch of calculateFieldStepIterationPML):
+    the same FPValue expressions, so every field is bit-identical, in the FP64 and the FP32
+    build.  The engine computes the per-cell path, so it honours these flags with the results it
+    gives without them."""
+    argv = args.split()
+    flags = ["--use-ca-cb"]
+    if "--use-pml" in argv:
+        flags.append("--use-ca-cb-pml")
+    kinds = ["f64"] + (["f32_fast"] if os.path.exists(ref_driver_path("f32_fast")) else [])
+    for kind in kinds:
+        base, _ = run_ref_driver(argv, str(tmp_path / f"base_{kind}.bin"), kind=kind)
+        cacb, _ = run_ref_driver(argv + flags, str(tmp_path / f"cacb_{kind}.bin"), kind=kind)
+        names = [n for n in base if n[0] in "EHDB" and n not in ("EInc", "HInc")]
+        assert names
+        for n in names:
+            assert np.array_equal(base[n], cacb[n]), (kind, n)
