@@ -243,6 +243,66 @@ def run_cfg5(args) -> dict:
     return out
 
 
+def adjoint_ratio(fd_check: bool = False) -> dict:
+    """(forward + backward) / plain forward of the 256^3 FP32 adjoint workload, checkpoint every
+    50 steps, from HIP events on the engine stream."""
+    from fdtd3d_amd.engine import Engine
+    c, spec, design = cfg5_adjoint(precision=4)
+    eps = np.ones(c.shape, dtype=np.float32)
+    lo, hi = spec["design_lo"], spec["design_hi"]
+    eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = design
+    e = Engine(c)
+    e.set_material("eps", eps)
+    e.prepare()
+    e.step(c.time_steps)
+    e.sync()
+    plain_ms = e.last_step_ms() * c.time_steps
+    e.close()
+    e = Engine(c)
+    e.set_material("eps", eps)
+    e.prepare()
+    e.adjoint_setup(spec["design_lo"], spec["design_hi"], spec["monitor_z"], spec["monitor_lo"], spec["monitor_hi"],
+                    spec["window"], checkpoint_every=50)
+    e.forward(design)
+    e.backward(1.0)
+    tm = e.adjoint_timing()
+    e.close()
+    return {"value": round((tm["forward_ms"] + tm["backward_ms"]) / plain_ms, 3), "target": "<= 3",
+            "plain_forward_ms": round(plain_ms, 1), "forward_ms": round(tm["forward_ms"], 1),
+            "backward_ms": round(tm["backward_ms"], 1),
+            "workload": "256^3 FP32, PML 16, TF/SF lambda 32 cells, 800 steps, design 64^3, checkpoint every 50 "
+                        "steps (gradient checks: tests/test_gpu_z_fullsize.py)"}
+
+
+def default_extras(args) -> dict:
+    """The metric's second figure (adjoint/forward time ratio) and the batch speed-up, on their
+    own workloads, appended to the default line."""
+    out = {}
+    try:
+        t0 = time.perf_counter()
+        out["adjoint_ratio"] = adjoint_ratio()
+        print(f"[bench] adjoint ratio {out['adjoint_ratio']} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+              flush=True)
+    except Exception as exc:
+        out["adjoint_ratio"] = {"error": str(exc)[:200]}
+    try:
+        t0 = time.perf_counter()
+
+        class A:
+            batch_steps = 2000
+            batch_only = False
+        r = run_cfg4(A)
+        out["batch_speedup"] = {"value": round(r["t_seq"] / r["t_batch"], 2), "target": ">= 3",
+                                "batched_s": round(r["t_batch"], 3), "sequential_s": round(r["t_seq"], 3),
+                                "mcell_updates_per_s": round(r["updates"] / r["t_batch"] / 1e6, 1),
+                                "workload": "64 TMz sims, edges log-uniform [256, 2048], PML 12, 2000 steps, FP32"}
+        print(f"[bench] batch speed-up {out['batch_speedup']['value']} ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+    except Exception as exc:
+        out["batch_speedup"] = {"error": str(exc)[:200]}
+    return out
+
+
 def precision_check(args) -> dict:
     """FP32 against FP64 on the same inputs (the FP32-rounded materials), both on the GPU, after
     --check-steps time steps: relative L2 of every field component.  The FP64 engine is itself
@@ -341,6 +401,20 @@ def pmc_traffic(workload: str):
     return None
 
 
+def host_cpu() -> dict:
+    """CPU model and logical core count of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(sample_n: int = 96, sample_steps: int = 31) -> dict:
     """Reference arithmetic on one core, config-2 features at sample_n^3 (T-1 updates)."""
     from oracle.binding import ref_driver_path, run_ref_driver
@@ -359,7 +433,9 @@ def cpu_baseline(sample_n: int = 96, sample_steps: int = 31) -> dict:
                     f"--load-eps-from-file {epsf} --time-steps {sample_steps}").split()
             _, secs = run_ref_driver(argv, os.path.join(tmp, "out.bin"), kind="f32_fast", timeout=600)
         return {"value": upd / secs / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "reference",
-                "sample": sample + "; reference InternalScheme per-cell code (default path), g++ -O3"}
+                "sample": sample + "; the reference's own InternalScheme per-cell code (default path, compiled from "
+                          "its sources with g++ -O3) under the step loop of oracle/ref_driver.cpp, not the "
+                          "reference CLI's 'Total time'", **host_cpu()}
     from fdtd3d_amd.config import Config as _C
     from oracle.binding import Oracle
     c = cfg2(n=sample_n, precision=8, steps=sample_steps)
@@ -370,7 +446,7 @@ def cpu_baseline(sample_n: int = 96, sample_steps: int = 31) -> dict:
     o.step(sample_steps)
     secs = time.perf_counter() - t0
     return {"value": upd / secs / 1e6, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": sample + "; C oracle FP64 (reference driver not built)"}
+            "sample": sample + "; C oracle FP64 (reference driver not built)", **host_cpu()}
 
 
 def copy_probe(mib: int) -> None:
@@ -401,6 +477,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true",
                     help="cfg4-batch: time the batched run only (for profiling; no speed-up figure)")
     ap.add_argument("--fd-voxels", type=int, default=16, help="finite-difference voxels of the cfg5-adjoint workload")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: skip the adjoint/forward ratio and the batch speed-up measurements")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the run, copy a buffer of this many MiB device-to-device (a plain copy "
                          "of known size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -517,6 +595,10 @@ def main():
         except Exception as exc:  # the GPU figure stands on its own
             cpu = {"error": str(exc)[:200]}
 
+    extras = {}
+    if args.workload == "cfg2" and not args.no_extras:
+        extras = default_extras(args)
+
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -525,7 +607,7 @@ def main():
         "config": {"workload": args.workload, "description": wl["desc"], "grid": list(cfg.shape),
                    "pml": cfg.pml[0], "tfsf_inset": cfg.tfsf_left[0], "time_steps_of_run": cfg.time_steps,
                    "cells": cells},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(line), flush=True)
 

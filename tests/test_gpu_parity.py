@@ -321,7 +321,7 @@ SPLIT_VS_FUSED = [
 @pytest.mark.parametrize("precision", [8, 4])
 def test_fused_step_matches_half_steps(args, precision, engine_lib, monkeypatch):
     """The fused E+H kernel (PML-free box) and the half-step kernels compute the same
-    expressions in the same order: results must be bit-identical."""
+    expressions in the same order: results, D / B included, must be bit-identical."""
     cfg = parse_args(args.split())
     cfg.precision = precision
     rng = np.random.default_rng(7)
