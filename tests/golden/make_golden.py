@@ -6,6 +6,14 @@ newest time layer of every field, the material and sigma grids, the head of the
 incident 1D grids and the run scalars, as compressed .npz files.
 
     python tests/golden/make_golden.py            # regenerate all fixtures
+    python tests/golden/make_golden.py cfg1_sphere80   # one fixture
+
+cfg1_sphere80 is the 80^3 FP64 configuration in full (accurate eps = 4 sphere of radius 20,
+PML 10, TF/SF inset 20, 200 steps).  Its fields are too large to commit whole, so it stores a
+summary: per component the L2 norm, sum and largest magnitude, the full mid-z plane (z = 40), and
+4096 cells drawn with a fixed seed; plus the eps grid (mostly 1 and 4: it compresses to a few
+kB).  It runs the reference driver's -O3 build (no asserts; the same arithmetic, built with the
+same -ffp-contract=off), about two minutes.
 """
 from __future__ import annotations
 
@@ -106,8 +114,50 @@ def cases(tmp):
     return out
 
 
+CFG1_ARGV = ("--3d --size x:80 --same-size --dx 0.0005 --wavelength 0.02 --angle-teta 0 --angle-phi 0 "
+             "--angle-psi 0 --use-tfsf --tfsf-size-left x:20 --tfsf-size-right x:20 --same-size-tfsf --use-pml "
+             "--pml-size x:10 --same-size-pml --courant-factor 0.5 --time-steps 200 --eps-sphere 4 "
+             "--eps-sphere-center x:40,y:40,z:40 --eps-sphere-radius 20")
+SUMMARY_SEED = 0xC0F1
+SUMMARY_CELLS = 4096
+
+
+def summary_payload(rec: dict, shape, argv: str) -> dict:
+    """Per component: L2, sum, max |v|, the mid-z plane and SUMMARY_CELLS seeded cells."""
+    rng = np.random.default_rng(SUMMARY_SEED)
+    flat = rng.choice(int(np.prod(shape)), size=SUMMARY_CELLS, replace=False)
+    out = {"argv": np.array(argv), "cells": flat.astype(np.int64), "Eps": rec["Eps"],
+           "scalars": rec["scalars"]}
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        v = rec[name]
+        out[name + "_l2"] = np.array(np.sqrt(np.sum(v * v)))
+        out[name + "_sum"] = np.array(np.sum(v))
+        out[name + "_max"] = np.array(np.abs(v).max())
+        out[name + "_midz"] = v.reshape(shape)[:, :, shape[2] // 2].copy()
+        out[name + "_cells"] = v[flat]
+    return out
+
+
+def write_cfg1(bin_path: str = ""):
+    """cfg1_sphere80 from the reference driver (or from an existing driver output file)."""
+    from oracle.binding import read_ref_output
+    with tempfile.TemporaryDirectory() as tmp:
+        if bin_path:
+            rec = read_ref_output(bin_path)
+        else:
+            rec, _ = run_ref_driver(CFG1_ARGV.split(), os.path.join(tmp, "cfg1.bin"), kind="f64_fast", timeout=3600)
+    payload = summary_payload(rec, (80, 80, 80), CFG1_ARGV)
+    np.savez_compressed(os.path.join(HERE, "cfg1_sphere80.npz"), **payload)
+    print("cfg1_sphere80", sorted(payload))
+
+
 def main():
-    only = set(sys.argv[1:])   # optional: the fixture names to (re)generate
+    only = set(a for a in sys.argv[1:] if not a.startswith("--"))   # optional: the fixture names to (re)generate
+    if not only or "cfg1_sphere80" in only:
+        write_cfg1(next((a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--from=")), ""))
+        only.discard("cfg1_sphere80")
+        if not only and len(sys.argv) > 1:
+            return
     with tempfile.TemporaryDirectory() as tmp:
         for name, argv in cases(tmp).items():
             if only and name not in only:

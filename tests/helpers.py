@@ -16,8 +16,39 @@ FIELDS_TMZ = ["Ez", "Hx", "Hy"]
 FIELDS_TEZ = ["Ex", "Ey", "Hz"]
 
 
+# fixtures that hold a summary of a run too large to store whole (see tests/golden/make_golden.py)
+SUMMARY_FIXTURES = ("cfg1_sphere80",)
+
+
 def golden_names():
-    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """Fixtures that hold every field of their run in full."""
+    return sorted(n for n in (os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+                  if n not in SUMMARY_FIXTURES)
+
+
+def load_summary(name):
+    """(config, fixture dict) of a summary fixture: per component the L2 norm, sum, largest
+    magnitude, mid-z plane and a seeded sample of cells ("cells": flat indices), and the eps grid."""
+    data = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    return parse_args(shlex.split(str(data["argv"]))), data
+
+
+def summary_errors(fields: dict, data: dict, shape) -> dict:
+    """Relative errors of fields (name -> array of the grid's shape) against a summary fixture:
+    L2 norm, sum (relative to the L2 norm times sqrt(cells)), largest magnitude, mid-z plane
+    (relative L2) and the sampled cells (relative to the largest magnitude)."""
+    out = {}
+    n = float(np.prod(shape))
+    for c, v in fields.items():
+        v = np.asarray(v, dtype=np.float64).reshape(shape)
+        l2 = float(data[c + "_l2"])
+        e = {"l2": abs(float(np.sqrt(np.sum(v * v))) - l2) / l2,
+             "sum": abs(float(np.sum(v)) - float(data[c + "_sum"])) / (l2 * np.sqrt(n)),
+             "max": abs(float(np.abs(v).max()) - float(data[c + "_max"])) / float(data[c + "_max"]),
+             "midz": rel_l2(v[:, :, shape[2] // 2], data[c + "_midz"]),
+             "cells": float(np.abs(v.ravel()[data["cells"]] - data[c + "_cells"]).max()) / float(data[c + "_max"])}
+        out[c] = e
+    return out
 
 
 def load_golden(name):
