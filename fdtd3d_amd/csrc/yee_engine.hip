@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -31,7 +32,7 @@ template <typename T, int MODE> __global__ void h_half_tez (StepArgs<T>);
 template <typename T> __global__ void incident_step (T *, const T *, T *, int64_t, int, double, double, double);
 template <typename T> __global__ void point_source (T *, int64_t, double);
 template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, uint8_t *);
-template <typename T, int COMP, int DIM> __global__ void scattered_kernel (StepArgs<T>, const T *, T *, int);
+template <typename T, int COMP, int DIM> __global__ void scattered_kernel (StepArgs<T>, const T *, double *, int, int64_t, int64_t);
 template <typename T, int BY, int MAT, int MAXW> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 
@@ -835,6 +836,7 @@ int yee_create (const yee_config *cfg, yee_handle **out)
     yee_destroy (h);
     return code;
   };
+  if (hipGetDevice (&h->device) != hipSuccess) return bail (fail (YEE_EDEVICE, "hipGetDevice failed"));
   hipError_t e = hipStreamCreateWithFlags (&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail (fail (YEE_EDEVICE, std::string ("hipStreamCreate: ") + hipGetErrorString (e)));
   if (hipEventCreate (&h->ev[0]) != hipSuccess || hipEventCreate (&h->ev[1]) != hipSuccess)
@@ -1159,35 +1161,76 @@ int yee_get_material (yee_handle *h, int32_t which, double *out)
 
 namespace {
 
+// The scattered field of cells [q0, q0 + count) into dev (FP64), enqueued on the handle's stream.
 template <typename T>
-int scattered (yee_handle *h, int comp, void *dev)
+int scattered (yee_handle *h, int comp, double *dev, int64_t q0, int64_t count)
 {
   const StepArgs<T> a = make_args<T> (h, 0);
   const T *f = (const T *) h->f[h->cur][comp];
-  T *o = (T *) dev;
   const int hsel = (h->t + 1) & 1;   // buffer of the newest HInc (yee_get_incident)
-  const dim3 grid ((unsigned) ((h->ncell + 255) / 256)), block (256);
+  const dim3 grid ((unsigned) ((count + 255) / 256)), block (256);
   if (h->dim == 3)
     switch (comp)
     {
-      case 0: hipLaunchKernelGGL ((scattered_kernel<T, 0, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 1: hipLaunchKernelGGL ((scattered_kernel<T, 1, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 4: hipLaunchKernelGGL ((scattered_kernel<T, 4, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      default: hipLaunchKernelGGL ((scattered_kernel<T, 5, 3>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 0: hipLaunchKernelGGL ((scattered_kernel<T, 0, 3>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 1: hipLaunchKernelGGL ((scattered_kernel<T, 1, 3>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 3>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 3>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 4: hipLaunchKernelGGL ((scattered_kernel<T, 4, 3>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      default: hipLaunchKernelGGL ((scattered_kernel<T, 5, 3>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
     }
   else
     switch (comp)
     {
-      case 0: hipLaunchKernelGGL ((scattered_kernel<T, 0, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 1: hipLaunchKernelGGL ((scattered_kernel<T, 1, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      case 4: hipLaunchKernelGGL ((scattered_kernel<T, 4, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
-      default: hipLaunchKernelGGL ((scattered_kernel<T, 5, 2>), grid, block, 0, h->stream, a, f, o, hsel); break;
+      case 0: hipLaunchKernelGGL ((scattered_kernel<T, 0, 2>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 1: hipLaunchKernelGGL ((scattered_kernel<T, 1, 2>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 2: hipLaunchKernelGGL ((scattered_kernel<T, 2, 2>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 3: hipLaunchKernelGGL ((scattered_kernel<T, 3, 2>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      case 4: hipLaunchKernelGGL ((scattered_kernel<T, 4, 2>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
+      default: hipLaunchKernelGGL ((scattered_kernel<T, 5, 2>), grid, block, 0, h->stream, a, f, dev, hsel, q0, count); break;
     }
   HIP_TRY (hipGetLastError ());
+  return YEE_OK;
+}
+
+int scattered_checks (yee_handle *h, int32_t comp, const double *out)
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
+  // the reference interpolates EInc / HInc, which exist only with TF/SF
+  if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
+  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  return YEE_OK;
+}
+
+// Staging array of yee_get_scattered_field_noalloc: a fixed device array, so that the entry
+// point allocates nothing.  Each device that runs the library holds its own copy until the
+// process ends; host code reaches it only through the address hipGetSymbolAddress returns on
+// that device (the array's name is a host-side stand-in, not a device address).  Every call on
+// a device shares its copy: calls that overlap on one GPU race.
+constexpr int64_t kScratchValues = (int64_t) 4 << 20;   // 4 Mi FP64 values, 32 MiB
+static __device__ __align__ (256) double scratch[kScratchValues];
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_address (double **out)
+{
+  int dev = 0;
+  HIP_TRY (hipGetDevice (&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail (YEE_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock (g_scratch_mu);
+  if (!g_scratch_addr[dev])
+  {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress (&p, HIP_SYMBOL (scratch));
+    if (e != hipSuccess || !p)
+      return fail (YEE_EDEVICE, std::string ("hipGetSymbolAddress (scratch): ") + hipGetErrorString (e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = (double *) g_scratch_addr[dev];
   return YEE_OK;
 }
 
@@ -1197,32 +1240,40 @@ extern "C" {
 
 int yee_get_scattered_field (yee_handle *h, int32_t comp, double *out)
 {
-  if (!h || !out) return fail (YEE_EINVAL, "null argument");
-  if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
-  // the reference interpolates EInc / HInc, which exist only with TF/SF
-  if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
-  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  int rc = scattered_checks (h, comp, out);
+  if (rc) return rc;
   HIP_TRY (hipStreamSynchronize (h->stream));
-  void *dev = nullptr;
-  HIP_TRY (hipMalloc (&dev, (size_t) h->ncell * h->wsize));
-  int rc = h->fp32 () ? scattered<float> (h, comp, dev) : scattered<double> (h, comp, dev);
+  double *dev = nullptr;
+  HIP_TRY (hipMalloc (&dev, (size_t) h->ncell * sizeof (double)));
+  rc = h->fp32 () ? scattered<float> (h, comp, dev, 0, h->ncell) : scattered<double> (h, comp, dev, 0, h->ncell);
   hipError_t e = rc ? hipSuccess : hipStreamSynchronize (h->stream);
-  if (!rc && e == hipSuccess)
-  {
-    if (h->fp32 ())
-    {
-      std::vector<float> tmp ((size_t) h->ncell);
-      e = hipMemcpy (tmp.data (), dev, (size_t) h->ncell * 4, hipMemcpyDeviceToHost);
-      for (size_t q = 0; q < tmp.size (); ++q) out[q] = tmp[q];
-    }
-    else
-    {
-      e = hipMemcpy (out, dev, (size_t) h->ncell * 8, hipMemcpyDeviceToHost);
-    }
-  }
+  if (!rc && e == hipSuccess) e = hipMemcpy (out, dev, (size_t) h->ncell * sizeof (double), hipMemcpyDeviceToHost);
   (void) hipFree (dev);
   if (rc) return rc;
   if (e != hipSuccess) return fail (YEE_EDEVICE, std::string ("scattered field: ") + hipGetErrorString (e));
+  return YEE_OK;
+}
+
+int yee_get_scattered_field_noalloc (yee_handle *h, int32_t comp, double *out)
+{
+  int rc = scattered_checks (h, comp, out);
+  if (rc) return rc;
+  int dev = 0;
+  HIP_TRY (hipGetDevice (&dev));
+  if (dev != h->device) return fail (YEE_EINVAL, "call on the device the handle was created on");
+  double *s = nullptr;
+  if ((rc = scratch_address (&s))) return rc;
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  // the grid in chunks of the staging array: each chunk is written whole by the kernel before
+  // it is copied out, so nothing left from an earlier call is read
+  for (int64_t q0 = 0; q0 < h->ncell; q0 += kScratchValues)
+  {
+    const int64_t cnt = std::min<int64_t> (kScratchValues, h->ncell - q0);
+    rc = h->fp32 () ? scattered<float> (h, comp, s, q0, cnt) : scattered<double> (h, comp, s, q0, cnt);
+    if (rc) return rc;
+    HIP_TRY (hipMemcpyAsync (out + q0, s, (size_t) cnt * sizeof (double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY (hipStreamSynchronize (h->stream));
+  }
   return YEE_OK;
 }
 

@@ -39,6 +39,7 @@ struct yee_handle
   bool exposed[6] = {};   // components of the scheme (what the API reads and sources drive)
   bool lack[3] = { false, false, false };   // axes a 1D / 2D scheme run on the 3D kernels lacks (extent 1)
   int miss = -1;          // the single lacked axis of such a 2D scheme (TEx/TMx 0, TEy/TMy 1), else -1
+  int device = 0;                // the device current at yee_create (its stream and memory live there)
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = { nullptr, nullptr };
 

@@ -1593,13 +1593,15 @@ __global__ void point_source (T *f, int64_t q, double value)
 // box [L, R] (SchemeHelper::doSkipMakeScattered, SchemeHelper.h:905-911: points outside keep
 // the total field).  E takes EInc, H the newest HInc buffer (hsel), interpolated as the TF/SF
 // correction does.
+// Cells [q0, q0 + count) of the flat grid; out[q - q0] receives the value in the field
+// precision, widened to FP64.
 template <typename T, int COMP, int DIM>
-__global__ void scattered_kernel (StepArgs<T> a, const T *f, T *out, int hsel)
+__global__ void scattered_kernel (StepArgs<T> a, const T *f, double *out, int hsel, int64_t q0, int64_t count)
 {
   const int nzz = DIM == 3 ? a.nz : 1;
-  const int64_t ncell = (int64_t) a.nx * a.ny * nzz;
-  const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncell) return;
+  const int64_t u = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= count) return;
+  const int64_t q = q0 + u;
   const int k = (int) (q % nzz);
   const int j = (int) ((q / nzz) % a.ny);
   const int i = (int) (q / ((int64_t) nzz * a.ny));
@@ -1612,7 +1614,7 @@ __global__ void scattered_kernel (StepArgs<T> a, const T *f, T *out, int hsel)
     const T *inc = COMP >= 3 ? a.tg->hinc[hsel] : a.tg->einc;
     v -= incident_at<T, COMP, DIM> (a, i, j, k, inc, COMP >= 3 ? 0.5 : 0.0);
   }
-  out[q] = (T) v;
+  out[u] = (double) (T) v;
 }
 
 template <typename T>
@@ -1688,18 +1690,18 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *); \
-  template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 1, 3> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 2, 3> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 3, 3> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 4, 3> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 5, 3> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 0, 2> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 1, 2> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 5, 2> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 2, 2> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 3, 2> (StepArgs<T>, const T *, T *, int);  \
-  template __global__ void scattered_kernel<T, 4, 2> (StepArgs<T>, const T *, T *, int);
+  template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 1, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 2, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 3, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 4, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 5, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 0, 2> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 1, 2> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 5, 2> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 2, 2> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 3, 2> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
+  template __global__ void scattered_kernel<T, 4, 2> (StepArgs<T>, const T *, double *, int, int64_t, int64_t);
 
 YEE_INST (float)
 YEE_INST (double)

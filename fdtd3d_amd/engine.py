@@ -28,6 +28,7 @@ EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
+    "yee_get_scattered_field_noalloc",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
     "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
@@ -74,6 +75,7 @@ def load_library():
     L.yee_get_field.argtypes = [vp, i32, vp]
     L.yee_get_field_raw.argtypes = [vp, i32, vp]
     L.yee_get_scattered_field.argtypes = [vp, i32, vp]
+    L.yee_get_scattered_field_noalloc.argtypes = [vp, i32, vp]
     L.yee_get_material.argtypes = [vp, i32, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]

@@ -209,6 +209,16 @@ int yee_get_material (yee_handle *h, int32_t which, double *out);
 int yee_get_scattered_field (yee_handle *h, int32_t comp, double *out);
 
 /*
+ * Same as yee_get_scattered_field, allocating nothing: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that sample the field every few steps).
+ * No counterpart in the reference.  The field is staged through a fixed 32 MiB device array
+ * (4 Mi FP64 values, one per device, held until the process ends) in chunks, so any grid size
+ * works; the call must be made on the device the handle was created on (YEE_EINVAL otherwise).
+ * Calls that overlap on one GPU share that array and race.
+ */
+int yee_get_scattered_field_noalloc (yee_handle *h, int32_t comp, double *out);
+
+/*
  * Mask export for bit-exact checks of the set-up tables.
  *   kind 0: TF/SF correction mask of component `comp`: bit d set when the reference's
  *           doNeedTFSFUpdate<comp>Border(pos, dir d) is true, d in LEFT=0, RIGHT=1, DOWN=2,
