@@ -94,3 +94,27 @@ def field_names(cfg, aux=False):
     if aux and cfg.use_metamaterials:
         names += [("D1" if n[0] == "E" else "B1") + n[1] for n in base]
     return names
+
+
+def check_fp32(label, names, eng_field, ref_field, tol=1e-5):
+    """FP32 bar: the E and H vectors, and each component with >= 10 % of its vector's squared
+    norm, within tol relative L2 of the FP64 reference; every component's error printed."""
+    for group in ("E", "H"):
+        ns = [n for n in names if n[0] == group]
+        if not ns:
+            continue
+        got = {n: np.asarray(eng_field(n), np.float64) for n in ns}
+        ref = {n: np.asarray(ref_field(n), np.float64) for n in ns}
+        num = {n: float(np.sum((got[n] - ref[n]) ** 2)) for n in ns}
+        den = {n: float(np.sum(ref[n] ** 2)) for n in ns}
+        total = sum(den.values())
+        vec = (sum(num.values()) / total) ** 0.5 if total > 0 else sum(num.values()) ** 0.5
+        parts = []
+        for n in ns:
+            share = den[n] / total if total > 0 else 0.0
+            own = (num[n] / den[n]) ** 0.5 if den[n] > 0 else 0.0
+            parts.append(f"{n} {own:.1e} (share {share:.2f})")
+            if share >= 0.1:
+                assert own <= tol, f"{label}/{n}: {own} (norm share {share})"
+        print(f"[gpu] {label} FP32 {group} vector {vec:.2e}: " + ", ".join(parts), flush=True)
+        assert vec <= tol, f"{label}/{group}: {vec}"

@@ -14,7 +14,7 @@ import pytest
 from fdtd3d_amd.config import parse_args
 from fdtd3d_amd.engine import Engine, run_engine
 from oracle.binding import run_oracle
-from tests.helpers import field_names, rel_l2
+from tests.helpers import check_fp32, field_names, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -27,19 +27,11 @@ def say(msg):
     sys.stdout.flush()
 
 
-def _vec_rel(got, want):
-    num = sum(float(np.sum((g - w) ** 2)) for g, w in zip(got, want))
-    den = sum(float(np.sum(w ** 2)) for w in want)
-    return np.sqrt(num / den) if den > 0 else np.sqrt(num)
-
-
 def _check(cfg, eng, o, exact):
     if cfg.precision == 4:
-        # the E and H vectors (a component can hold only FP64 rounding residue, ~1e-19)
-        for group in ("E", "H"):
-            ns = [n for n in field_names(cfg) if n[0] == group]
-            r = _vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
-            assert r <= TOL_FP32, (group, r)
+        # the E and H vectors, and each component holding >= 10 % of its vector (a component can
+        # hold only FP64 rounding residue, ~1e-19)
+        check_fp32("edge case", field_names(cfg), eng.field, o.field, TOL_FP32)
         return
     for f in field_names(cfg):
         got, want = eng.field(f), o.field(f)

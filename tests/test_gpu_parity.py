@@ -4,7 +4,10 @@ Bars (per component, after the last step):
   - FP64, runs without PML: bit-identical to the reference (same operations, no FMA);
   - FP64 with PML / Drude: relative L2 <= 1e-12 (the engine updates E/H directly outside
     the PML slabs, which equals the reference's D->E stage with sigma = 0 up to rounding);
-  - FP32: relative L2 of the E and H vectors <= 1e-5 against the FP64 oracle;
+  - FP32: relative L2 of the E and H vectors <= 1e-5 against the FP64 oracle, and of every
+    component holding at least 10 % of its vector's squared norm against its own norm (the
+    smaller components are scattered / cross-polarised fields, on which the FP32 rounding of
+    the large ones weighs more);
   - TF/SF and PML masks: bit-exact.
 """
 import sys
@@ -15,7 +18,7 @@ import pytest
 from fdtd3d_amd.config import parse_args
 from fdtd3d_amd.engine import Engine, run_engine
 from oracle.binding import Oracle, run_oracle, sphere_map
-from tests.helpers import field_names, golden_names, load_golden, rel_l2
+from tests.helpers import check_fp32, field_names, golden_names, load_golden, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -32,6 +35,8 @@ def vec_rel(eng_fields, ref_fields):
     num = sum(np.sum((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2) for a, b in zip(eng_fields, ref_fields))
     den = sum(np.sum(np.asarray(b, np.float64) ** 2) for b in ref_fields)
     return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
+
+
 
 
 @pytest.mark.parametrize("name", golden_names())
@@ -60,11 +65,7 @@ def test_engine_fp32_vs_fp64_oracle(name, engine_lib):
     cfg.precision = 4
     say(f"[gpu] fixture {name} fp32")
     eng = run_engine(cfg, mats)
-    names = field_names(cfg)
-    e_names = [n for n in names if n[0] == "E"]
-    h_names = [n for n in names if n[0] == "H"]
-    assert vec_rel([eng.field(n) for n in e_names], [data[n].reshape(cfg.shape) for n in e_names]) <= TOL_FP32
-    assert vec_rel([eng.field(n) for n in h_names], [data[n].reshape(cfg.shape) for n in h_names]) <= TOL_FP32
+    check_fp32(name, field_names(cfg), eng.field, lambda n: data[n].reshape(cfg.shape))
     eng.close()
 
 
@@ -109,6 +110,11 @@ LARGER = [
     # TMz point source with PML and random eps blobs
     ("tmz_blobs", "--2d-tmz --size x:96,y:80 --use-pml --pml-size x:12,y:12 --point-source x:40,y:33 "
      "--point-source-ez --time-steps 200"),
+    # rows of 200 cells: the fused kernel's box holds 187 z cells of each row, three waves with
+    # the z+1 neighbours of each wave's last lane passed through LDS; oblique TF/SF, block eps
+    ("fused_rows_3_waves", "--3d --size x:24,y:22,z:200 --use-pml --pml-size x:6 --same-size-pml --use-tfsf "
+     "--tfsf-size-left x:8,y:8,z:40 --tfsf-size-right x:8,y:8,z:40 --angle-teta 30 --angle-phi 45 "
+     "--angle-psi 90 --time-steps 120"),
 ]
 
 
@@ -117,7 +123,7 @@ def _materials(name, cfg):
     mats = {}
     if cfg.eps_sphere != 1.0:
         mats["eps"] = sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius)
-    if name == "cfg2_small":
+    if name in ("cfg2_small", "fused_rows_3_waves"):
         nb = [(s + 7) // 8 for s in cfg.shape]
         blocks = rng.uniform(1.0, 12.0, size=nb)
         mats["eps"] = np.kron(blocks, np.ones((8, 8, 8)))[: cfg.shape[0], : cfg.shape[1], : cfg.shape[2]]
@@ -146,10 +152,7 @@ def test_engine_vs_oracle_larger(name, args, precision, engine_lib):
             r = rel_l2(eng.field(f), o.field(f))
             assert r <= TOL_FP64, f"{name}/{f}: {r}"
     else:
-        for group in ("E", "H"):
-            ns = [n for n in names if n[0] == group]
-            r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
-            assert r <= TOL_FP32, f"{name}/{group}: {r}"
+        check_fp32(name, names, eng.field, o.field)
     eng.close()
 
 
@@ -265,10 +268,7 @@ def test_current_source_inside_drude_box(args, box, precision, engine_lib):
         for f in names:
             assert rel_l2(eng.field(f), o.field(f)) <= TOL_FP64, f
     else:
-        for group in ("E", "H"):
-            ns = [n for n in names if n[0] == group]
-            r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
-            assert r <= TOL_FP32, f"{group}: {r}"
+        check_fp32("current source in Drude box", names, eng.field, o.field)
     eng.close()
 
 
@@ -383,10 +383,5 @@ def test_fp32_drude_resonance_long_run(engine_lib):
     cfg.precision = 4
     say("[gpu] fp32 drude resonance 500 steps")
     eng = run_engine(cfg, mats)
-    names = field_names(cfg)
-    for group in ("E", "H"):
-        ns = [n for n in names if n[0] == group]
-        r = vec_rel([eng.field(n) for n in ns], [o.field(n) for n in ns])
-        say(f"[gpu] {group}: {r:.2e}")
-        assert r <= TOL_FP32, (group, r)
+    check_fp32("drude resonance 500 steps", field_names(cfg), eng.field, o.field)
     eng.close()

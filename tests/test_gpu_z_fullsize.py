@@ -135,3 +135,100 @@ def test_fullsize_adjoint_gradient_vs_central_differences(engine_lib):
         say(f"[gpu] voxel {v}: adjoint {grad[v]:.9e} fd {fd:.9e} rel {rel:.2e}")
         assert rel <= 1e-4, (v, grad[v], fd)
     e.close()
+
+
+def _cfg3_run(precision, kernel, steps, monkeypatch):
+    monkeypatch.setenv("YEE_KERNEL", kernel)
+    cfg = bench.cfg3(steps=steps)
+    cfg.precision = precision
+    eng = Engine(cfg)
+    for name, arr in bench.drude_materials(cfg).items():
+        eng.set_material(name, arr)
+    eng.prepare()
+    eng.step(steps)
+    eng.sync()
+    return eng
+
+
+def test_fullsize_cfg3_drude(engine_lib, monkeypatch):
+    """The 512^3 Drude workload at its full 2000 steps (central 128^3 slab, omega_p = 1.5 w0,
+    gamma = 0.01 w0 for E and H, PML 16, TF/SF at normal incidence, psi = 0: the incident field
+    is Ey / Hx, the other four components are scattered by the slab only):
+      - the fused kernel with the Drude box on the half-step kernels equals the all-half-step
+        run bit for bit (FP32);
+      - FP32 against FP64 on the same inputs: the E and H vectors within 1e-5 relative L2, and
+        every component within 1e-5 relative to the L2 norm of its vector (E or H).  The
+        per-component error relative to the component's own norm is printed: the scattered
+        components are far smaller than the incident ones, so FP32 rounding of the incident field
+        weighs more on them."""
+    steps = 2000
+    say(f"[gpu] cfg3 512^3 Drude FP32, {steps} steps: half-step kernels")
+    eng = _cfg3_run(4, "split", steps, monkeypatch)
+    split = {c: eng.field_raw(c) for c in COMPS}
+    eng.close()
+    say(f"[gpu] cfg3 512^3 Drude FP32, {steps} steps: fused kernel")
+    eng = _cfg3_run(4, "fused", steps, monkeypatch)
+    f32 = {c: eng.field_raw(c) for c in COMPS}
+    eng.close()
+    for c in COMPS:
+        assert np.abs(split[c]).max() > 0, c
+        assert np.array_equal(split[c], f32[c]), c
+    del split
+    say(f"[gpu] cfg3 512^3 Drude FP64, {steps} steps")
+    eng = _cfg3_run(8, "fused", steps, monkeypatch)
+    f64 = {c: eng.field_raw(c) for c in COMPS}
+    eng.close()
+    for group in (COMPS[:3], COMPS[3:]):
+        num = {c: float(np.sum((f32[c].astype(np.float64) - f64[c]) ** 2)) for c in group}
+        den = {c: float(np.sum(f64[c] ** 2)) for c in group}
+        vec = (sum(num.values()) / sum(den.values())) ** 0.5
+        say(f"[gpu] cfg3 FP32 vs FP64, {group[0][0]} vector: {vec:.2e}")
+        assert vec <= 1e-5, (group, vec)
+        for c in group:
+            own = (num[c] / den[c]) ** 0.5
+            of_vec = (num[c] / sum(den.values())) ** 0.5
+            share = den[c] / sum(den.values())
+            say(f"[gpu] cfg3 {c}: {own:.2e} of its norm, {of_vec:.2e} of the vector's (norm share {share:.1e})")
+            assert of_vec <= 1e-5, (c, of_vec)
+
+
+def test_fullsize_adjoint_fp32_gradient_vs_fp64_central_differences(engine_lib):
+    """The 256^3 adjoint workload in FP32 (the configuration's precision): dJ/deps of the FP32
+    discrete adjoint on 16 seeded design voxels against 4th-order central differences of the FP64
+    forward run (h = 1e-2 eps), within 1e-4 relative."""
+    c32, spec, design = bench.cfg5_adjoint(precision=4)
+    say(f"[gpu] full size adjoint {c32.size} FP32, {c32.time_steps} steps, 16 voxels")
+    eps = np.ones(c32.shape)
+    lo, hi = spec["design_lo"], spec["design_hi"]
+    eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = design
+    args = (spec["design_lo"], spec["design_hi"], spec["monitor_z"], spec["monitor_lo"], spec["monitor_hi"],
+            spec["window"])
+    e = Engine(c32)
+    e.set_material("eps", eps.astype(np.float32))
+    e.prepare()
+    e.adjoint_setup(*args, checkpoint_every=50)
+    e.forward(design)
+    grad32 = e.backward(1.0)
+    e.close()
+    c64, _, _ = bench.cfg5_adjoint(precision=8)
+    e = Engine(c64)
+    e.set_material("eps", eps)
+    e.prepare()
+    e.adjoint_setup(*args, checkpoint_every=50)
+    rng = np.random.default_rng(0xF32)
+    worst = 0.0
+    for _ in range(16):
+        v = tuple(int(rng.integers(0, design.shape[a])) for a in range(3))
+        h = 1e-2 * design[v]
+        js = []
+        for m in (2, 1, -1, -2):
+            d = design.copy()
+            d[v] += m * h
+            js.append(e.forward(d))
+        fd = (-js[0] + 8 * js[1] - 8 * js[2] + js[3]) / (12 * h)
+        rel = abs(grad32[v] - fd) / abs(fd)
+        worst = max(worst, rel)
+        say(f"[gpu] voxel {v}: FP32 adjoint {grad32[v]:.7e} FP64 fd {fd:.7e} rel {rel:.2e}")
+        assert rel <= 1e-4, (v, grad32[v], fd)
+    say(f"[gpu] FP32 adjoint worst of 16 voxels: {worst:.2e}")
+    e.close()
