@@ -40,6 +40,7 @@ SCHEMES_2D = {"--2d-tex": (SCHEME_TEX, 0), "--2d-tey": (SCHEME_TEY, 1), "--2d-te
 USE_PML = 0x1
 USE_TFSF = 0x2
 USE_METAMATERIALS = 0x4
+COMPLEX_FIELDS = 0x8
 
 COMPONENTS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz",
               "Dx", "Dy", "Dz", "Bx", "By", "Bz",
@@ -97,6 +98,8 @@ class Config:
     use_pml: bool = False
     use_tfsf: bool = False
     use_metamaterials: bool = False
+    # complex field values (the reference's -DCOMPLEX_FIELD_VALUES build; no command-line flag)
+    complex_fields: bool = False
     point_source: List[str] = field(default_factory=list)   # component names, e.g. ["Ez"]
     point_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
     # --current-source-j<x|y|z> / -m<x|y|z> as component names ("Ex" for Jx, "Hx" for Mx), --current-source
@@ -184,7 +187,8 @@ class Config:
         c.wavelength = self.wavelength
         c.courant = self.courant
         c.flags = ((USE_PML if self.use_pml else 0) | (USE_TFSF if self.use_tfsf else 0)
-                   | (USE_METAMATERIALS if self.use_metamaterials else 0))
+                   | (USE_METAMATERIALS if self.use_metamaterials else 0)
+                   | (COMPLEX_FIELDS if self.complex_fields else 0))
         mask = 0
         for name in self.point_source:
             mask |= 1 << COMP_INDEX[name]

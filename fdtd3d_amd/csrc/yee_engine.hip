@@ -130,7 +130,11 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   for (int ax = 0; ax < 3; ++ax) a.cur_pos[ax] = (int) h->cfg.current_source_pos[ax];
   {
     const double ts = role == kRoleH ? (double) h->t + 0.5 : (double) h->t;
-    double current = std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+    double current;
+    if (h->src_kind == 0) current = std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+    // the complex build's current is (cos, -sin) of dt (ts - 1) 2 pi f (Scheme.inc.h:1230-1235)
+    else if (h->src_kind == 1) current = std::cos (h->sc.dt * (ts - 1) * 2 * kPi * h->sc.freq);
+    else current = -std::sin (h->sc.dt * (ts - 1) * 2 * kPi * h->sc.freq);
     a.cur_val = current * mu0 () / h->sc.dt / 2.0;
   }
   a.sz = 1;
@@ -251,7 +255,9 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
 double source_value (const yee_handle *h, bool e_half, int32_t t)
 {
   const double ts = e_half ? t + 0.5 : t + 1.0;
-  double v = std::sin (h->sc.dt * ts * 2 * kPi * h->sc.freq);
+  // the complex build's point source is (sin, cos) of the same argument (InternalScheme.inc.h:2888-2893)
+  const double arg = h->sc.dt * ts * 2 * kPi * h->sc.freq;
+  double v = h->src_kind == 2 ? std::cos (arg) : std::sin (arg);
   if (h->src_sigma > 0)
   {
     const double u = (ts - h->src_t0) / h->src_sigma;
@@ -511,7 +517,8 @@ int step_once (yee_handle *h)
     const double arg = h->sc.dt * t * 2 * kPi * h->sc.freq;
     ProfScope ps (h, 0);
     hipLaunchKernelGGL ((incident_step<T>), dim3 (1), dim3 (1024), 0, h->stream, (T *) h->einc,
-                       (const T *) h->hinc[prev], (T *) h->hinc[cur], h->inc_len, (int) t, modE, modH, std::sin (arg));
+                       (const T *) h->hinc[prev], (T *) h->hinc[cur], h->inc_len, (int) t, modE, modH,
+                       h->src_kind == 2 ? std::cos (arg) : std::sin (arg));   // complex build: (sin, cos)
     HIP_TRY (hipGetLastError ());
   }
   int rc;
@@ -921,6 +928,19 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   }
   if (hipStreamSynchronize (h->stream) != hipSuccess) return bail (fail (YEE_EDEVICE, "stream sync failed"));
   h->sc = compute_scalars (h->cfg);
+  if (h->complex ())
+  {
+    // complex field values: this handle is the real part; the imaginary part is a second real
+    // handle of the same configuration whose sources are the imaginary parts of the complex
+    // build's sources
+    h->src_kind = 1;
+    yee_config ci = *cfg;
+    ci.flags &= ~YEE_COMPLEX_FIELDS;
+    yee_handle *im = nullptr;
+    if ((rc = yee_create (&ci, &im))) return bail (rc);
+    im->src_kind = 2;
+    h->imag = im;
+  }
   *out = h;
   return YEE_OK;
 }
@@ -928,6 +948,7 @@ int yee_create (const yee_config *cfg, yee_handle **out)
 void yee_destroy (yee_handle *h)
 {
   if (!h) return;
+  if (h->imag) yee_destroy (h->imag);
   if (h->stream) (void) hipStreamSynchronize (h->stream);
   adjoint_free (h);
   for (void *p : h->grids) (void) hipFree (p);
@@ -945,7 +966,9 @@ void yee_destroy (yee_handle *h)
 
 int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values)
 {
-  return set_material_impl (h, which, values);
+  int rc = set_material_impl (h, which, values);
+  if (!rc && h->imag) rc = set_material_impl (h->imag, which, values);
+  return rc;
 }
 
 int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double outside, const int64_t center[3],
@@ -987,17 +1010,26 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
           }
     });
   for (std::thread &th : pool) th.join ();
-  return set_material_impl (h, which, v.data ());
+  int rc = set_material_impl (h, which, v.data ());
+  if (!rc && h->imag) rc = set_material_impl (h->imag, which, v.data ());
+  return rc;
 }
 
 int yee_set_material (yee_handle *h, int32_t which, const double *values)
 {
-  return set_material_impl (h, which, values);
+  int rc = set_material_impl (h, which, values);
+  if (!rc && h->imag) rc = set_material_impl (h->imag, which, values);
+  return rc;
 }
 
 int yee_prepare (yee_handle *h)
 {
   if (!h) return fail (YEE_EINVAL, "null handle");
+  if (h->imag)
+  {
+    int rc = yee_prepare (h->imag);
+    if (rc) return rc;
+  }
   if (h->prepared) return YEE_OK;
   // the Drude ADE divides by A = 4 M eps + ...: eps / mu must be nonzero (checked in set_material)
   if (h->mode == 2 && !h->box_any)
@@ -1031,10 +1063,19 @@ int yee_step (yee_handle *h, int32_t n)
   if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
   if (n < 0) return fail (YEE_EINVAL, "negative step count");
   HIP_TRY (hipEventRecord (h->ev[0], h->stream));
+  if (h->imag) HIP_TRY (hipStreamWaitEvent (h->imag->stream, h->ev[0], 0));
   for (int32_t s = 0; s < n; ++s)
   {
     int rc = h->fp32 () ? step_once<float> (h) : step_once<double> (h);
     if (rc) return rc;
+    // complex fields: the imaginary part on its own stream, alongside
+    if (h->imag && (rc = h->fp32 () ? step_once<float> (h->imag) : step_once<double> (h->imag))) return rc;
+  }
+  if (h->imag)
+  {
+    // the step time covers both parts
+    HIP_TRY (hipEventRecord (h->imag->ev[1], h->imag->stream));
+    HIP_TRY (hipStreamWaitEvent (h->stream, h->imag->ev[1], 0));
   }
   HIP_TRY (hipEventRecord (h->ev[1], h->stream));
   HIP_TRY (hipEventSynchronize (h->ev[1]));
@@ -1082,6 +1123,11 @@ int yee_set_point_source_gaussian (yee_handle *h, double t0, double sigma)
   if (!std::isfinite (t0) || !std::isfinite (sigma)) return fail (YEE_EINVAL, "t0 and sigma must be finite");
   h->src_t0 = t0;
   h->src_sigma = sigma;
+  if (h->imag)
+  {
+    h->imag->src_t0 = t0;
+    h->imag->src_sigma = sigma;
+  }
   return YEE_OK;
 }
 
@@ -1089,6 +1135,15 @@ int yee_sync (yee_handle *h)
 {
   if (!h) return fail (YEE_EINVAL, "null handle");
   HIP_TRY (hipStreamSynchronize (h->stream));
+  if (h->imag) HIP_TRY (hipStreamSynchronize (h->imag->stream));
+  return YEE_OK;
+}
+
+int yee_get_imag_part (yee_handle *h, yee_handle **imag)
+{
+  if (!h || !imag) return fail (YEE_EINVAL, "null argument");
+  if (!h->imag) return fail (YEE_EINVAL, "the handle holds real field values (create it with YEE_COMPLEX_FIELDS)");
+  *imag = h->imag;
   return YEE_OK;
 }
 

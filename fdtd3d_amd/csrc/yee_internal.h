@@ -80,6 +80,13 @@ struct yee_handle
   bool use_meta () const { return (cfg.flags & YEE_USE_METAMATERIALS) != 0; }
 
   yee::AdjointState *adj = nullptr;   // forward/backward pair state (yee_adjoint_setup)
+
+  // complex field values: the part this handle's sources drive (0 the real build, 1 the real
+  // part, 2 the imaginary part of the complex build), and the imaginary part's handle, which a
+  // complex handle (the real part) owns
+  int src_kind = 0;
+  yee_handle *imag = nullptr;
+  bool complex () const { return (cfg.flags & YEE_COMPLEX_FIELDS) != 0; }
 };
 
 namespace yee {

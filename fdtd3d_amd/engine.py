@@ -28,7 +28,7 @@ EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
-    "yee_get_scattered_field_noalloc",
+    "yee_get_scattered_field_noalloc", "yee_get_imag_part",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
     "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
@@ -76,6 +76,7 @@ def load_library():
     L.yee_get_field_raw.argtypes = [vp, i32, vp]
     L.yee_get_scattered_field.argtypes = [vp, i32, vp]
     L.yee_get_scattered_field_noalloc.argtypes = [vp, i32, vp]
+    L.yee_get_imag_part.argtypes = [vp, ctypes.POINTER(vp)]
     L.yee_get_material.argtypes = [vp, i32, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
@@ -120,6 +121,37 @@ def device_info() -> str:
     buf = ctypes.create_string_buffer(256)
     _check(load_library().yee_device_info(buf, 256))
     return buf.value.decode()
+
+
+class ImagPart:
+    """Read access to the imaginary part of a complex-field Engine (a handle it owns)."""
+
+    def __init__(self, owner: "Engine", h):
+        self._owner = owner   # keeps the owning handle alive
+        self.h = h
+        self.shape = owner.shape
+        self.dtype = owner.dtype
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(load_library().yee_get_field(self.h, COMP_INDEX[name], out.ctypes.data))
+        return out
+
+    def field_raw(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(load_library().yee_get_field_raw(self.h, COMP_INDEX[name], out.ctypes.data))
+        return out
+
+    def scattered(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(load_library().yee_get_scattered_field(self.h, COMP_INDEX[name], out.ctypes.data))
+        return out
+
+    def incident(self, n: int):
+        e = np.empty(n, dtype=np.float64)
+        hh = np.empty(n, dtype=np.float64)
+        _check(load_library().yee_get_incident(self.h, e.ctypes.data, hh.ctypes.data, int(n)))
+        return e, hh
 
 
 class Engine:
@@ -234,6 +266,12 @@ class Engine:
         out = np.empty(self.shape, dtype=np.float64)
         _check(load_library().yee_get_field(self.h, COMP_INDEX[name], out.ctypes.data))
         return out
+
+    def imag(self) -> "ImagPart":
+        """The imaginary part of a complex-field run (the real part is this handle)."""
+        h = ctypes.c_void_p()
+        _check(load_library().yee_get_imag_part(self.h, ctypes.byref(h)))
+        return ImagPart(self, h)
 
     def scattered(self, name: str) -> np.ndarray:
         """Scattered field: the newest field minus the incident wave inside the TF/SF box."""

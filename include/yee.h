@@ -58,6 +58,8 @@ extern "C" {
 #define YEE_USE_PML 0x1u            /* --use-pml */
 #define YEE_USE_TFSF 0x2u           /* --use-tfsf */
 #define YEE_USE_METAMATERIALS 0x4u  /* --use-metamaterials (requires --use-pml, Scheme.inc.h:889-892) */
+#define YEE_COMPLEX_FIELDS 0x8u     /* complex field values (the reference's -DCOMPLEX_FIELD_VALUES build,
+                                       Kernels/FieldValue.h:68-72); see yee_get_imag_part */
 
 /* Field components (GridType order of the reference). */
 enum yee_component
@@ -177,6 +179,19 @@ int yee_step (yee_handle *h, int32_t n);
  * next step on.
  */
 int yee_set_point_source_gaussian (yee_handle *h, double t0, double sigma);
+
+/*
+ * Complex field values (YEE_COMPLEX_FIELDS): the reference's update multiplies every field by
+ * real coefficients (computeCaCb and the PML / Drude coefficients are FIELDVALUE (c, 0)), so its
+ * real and imaginary parts evolve independently and meet only in the sources, which the complex
+ * build makes complex: the incident wave (sin, cos) of dt t 2 pi f (InternalScheme.inc.h:3310-3316),
+ * the point source (sin, cos) (:2888-2893), the current source (cos, -sin) of dt (ts - 1) 2 pi f
+ * (Scheme.inc.h:1230-1235).  A complex handle holds two real parts, each with its part of every
+ * source; calls on the handle act on both.  yee_get_field & co. read the real part; this call
+ * returns the handle of the imaginary part (owned by h: read from it, never destroy or step it
+ * by itself).  YEE_EINVAL for a real-field handle.
+ */
+int yee_get_imag_part (yee_handle *h, yee_handle **imag);
 
 /* Waits for all enqueued work of the handle. */
 int yee_sync (yee_handle *h);

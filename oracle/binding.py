@@ -6,6 +6,7 @@ module; the product never does.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 import subprocess
 
@@ -46,6 +47,7 @@ def lib():
         L.oracle_incident_len.restype = ctypes.c_int64
         L.oracle_incident_len.argtypes = [vp]
         L.oracle_set_source_phase.argtypes = [vp, ctypes.c_double]
+        L.oracle_set_source_kind.argtypes = [vp, ctypes.c_int]
         L.oracle_set_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.oracle_adjoint.argtypes = [vp, ctypes.POINTER(OracleAdjSpec), ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_double), vp]
@@ -155,6 +157,11 @@ class Oracle:
     def set_source_phase(self, phase: float) -> None:
         lib().oracle_set_source_phase(self.h, float(phase))
 
+    def set_source_kind(self, kind: int) -> None:
+        """1 / 2: the real / imaginary part of the complex build's sources; 0: the real build."""
+        if lib().oracle_set_source_kind(self.h, int(kind)) != 0:
+            raise ValueError(f"source kind {kind}")
+
     def set_source_gaussian(self, t0: float, sigma: float) -> None:
         lib().oracle_set_source_gaussian(self.h, float(t0), float(sigma))
 
@@ -208,13 +215,22 @@ class Oracle:
         return out
 
 
-def run_oracle(cfg: Config, materials=None, steps=None) -> Oracle:
+def run_oracle(cfg: Config, materials=None, steps=None, source_kind: int = 0) -> Oracle:
     o = Oracle(cfg)
+    if source_kind:
+        o.set_source_kind(source_kind)
     for name, arr in (materials or {}).items():
         o.set_material(name, arr)
     o.prepare()
     o.step(cfg.time_steps if steps is None else steps)
     return o
+
+
+def run_oracle_complex(cfg: Config, materials=None, steps=None):
+    """The reference's complex-field build: its real and imaginary parts, each a real run driven
+    by its part of the complex sources (the update multiplies fields by real coefficients only)."""
+    real = dataclasses.replace(cfg, complex_fields=False)
+    return (run_oracle(real, materials, steps, source_kind=1), run_oracle(real, materials, steps, source_kind=2))
 
 
 def read_ref_output(path: str):

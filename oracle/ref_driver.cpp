@@ -80,7 +80,18 @@ FPValue sigmaValue (FPValue dist, FPValue boundary, FPValue gridStep)
   return boundaryFactor * (pow (x1, (exponent + 1)) - pow (x2, (exponent + 1)));
 }
 
-bool readRaw (const std::string &name, std::vector<FieldValue> &dst, size_t n)
+/* Real / imaginary part of a field value (the complex build, -DCOMPLEX_FIELD_VALUES, dumps both). */
+double partOf (const FieldValue &v, int part)
+{
+#ifdef COMPLEX_FIELD_VALUES
+  return part ? (double) v.imag () : (double) v.real ();
+#else
+  return part ? 0.0 : (double) v;
+#endif
+}
+
+/* Material files hold FPValue values (real materials in either build). */
+bool readRaw (const std::string &name, std::vector<FPValue> &dst, size_t n)
 {
   FILE *f = fopen (name.c_str (), "rb");
   if (!f)
@@ -89,7 +100,7 @@ bool readRaw (const std::string &name, std::vector<FieldValue> &dst, size_t n)
     return false;
   }
   dst.resize (n);
-  size_t got = fread (dst.data (), sizeof (FieldValue), n, f);
+  size_t got = fread (dst.data (), sizeof (FPValue), n, f);
   fclose (f);
   if (got != n)
   {
@@ -157,10 +168,10 @@ class RefRun
           f (coordOf (i, j, k));
   }
 
-  std::vector<double> dumpGrid (Grid<TC> *g, int layer)
+  std::vector<double> dumpGrid (Grid<TC> *g, int layer, int part = 0)
   {
     std::vector<double> v;
-    forAll (0, 0, [&] (TC p) { v.push_back ((double) *g->getFieldValue (p, layer)); });
+    forAll (0, 0, [&] (TC p) { v.push_back (partOf (*g->getFieldValue (p, layer), part)); });
     return v;
   }
 
@@ -171,13 +182,13 @@ class RefRun
       return;
     }
     size_t n = (size_t) sz (0) * sz (1) * sz (2);
-    std::vector<FieldValue> raw;
+    std::vector<FPValue> raw;
     if (!readRaw (file, raw, n))
     {
       exit (2);
     }
     size_t idx = 0;
-    forAll (0, 0, [&] (TC p) { g->setFieldValue (raw[idx++], p, 0); });
+    forAll (0, 0, [&] (TC p) { g->setFieldValue (FIELDVALUE (raw[idx++], FPValue (0)), p, 0); });
   }
 
   /* Sphere of `value` (outside `outside`) on the material grid; follows Scheme::initGrids (Scheme.inc.h:2983-3012). */
@@ -353,7 +364,14 @@ class RefRun
       }
       if (cur)
       {
+#ifdef COMPLEX_FIELD_VALUES
+        /* the complex build's current (Scheme.inc.h:1230-1235) */
+        FieldValue current = FieldValue (
+          cos (is->getGridTimeStep () * (timestep - 1) * FPValue (2) * PhysicsConst::Pi * is->getSourceFrequency ()),
+          -sin (is->getGridTimeStep () * (timestep - 1) * FPValue (2) * PhysicsConst::Pi * is->getSourceFrequency ()));
+#else
         FieldValue current = sin (is->getGridTimeStep () * timestep * FPValue (2) * PhysicsConst::Pi * is->getSourceFrequency ());
+#endif
         current = current * PhysicsConst::Mu0 / is->getGridTimeStep () / FPValue (2.0);
         if (Ca != NULLPTR)
           is->template calculateFieldStepIterationCurrent<grid_type, true> (current, grid, Ca, Cb, usePML, gridType, mg,
@@ -548,7 +566,17 @@ public:
     bool pml = SOLVER_SETTINGS.getDoUsePML ();
     bool meta = SOLVER_SETTINGS.getDoUseMetamaterials ();
     /* after the final shift the newest values sit in time layer 1 (Grid.h:1304-1319) */
+#ifdef COMPLEX_FIELD_VALUES
+    /* the imaginary parts as "<name>.im" */
+#define DUMP_IF(need, name, getter)                                                                         \
+  if (need)                                                                                                  \
+  {                                                                                                          \
+    putRecord (name, dumpGrid (is->getter (), 1));                                                           \
+    putRecord (std::string (name).append (".im").c_str (), dumpGrid (is->getter (), 1, 1));                    \
+  }
+#else
 #define DUMP_IF(need, name, getter) if (need) putRecord (name, dumpGrid (is->getter (), 1));
+#endif
     DUMP_IF (is->getDoNeedEx (), "Ex", getEx)
     DUMP_IF (is->getDoNeedEy (), "Ey", getEy)
     DUMP_IF (is->getDoNeedEz (), "Ez", getEz)
@@ -585,15 +613,21 @@ public:
     }
     if (SOLVER_SETTINGS.getDoUseTFSF ())
     {
-      std::vector<double> e, h;
+      std::vector<double> e, h, ei, hi;
       for (grid_coord i = 0; i < is->getEInc ()->getSize ().get1 (); ++i)
       {
         GridCoordinate1D p = GRID_COORDINATE_1D (i, CoordinateType::X);
-        e.push_back ((double) *is->getEInc ()->getFieldValue (p, 1));
-        h.push_back ((double) *is->getHInc ()->getFieldValue (p, 1));
+        e.push_back (partOf (*is->getEInc ()->getFieldValue (p, 1), 0));
+        h.push_back (partOf (*is->getHInc ()->getFieldValue (p, 1), 0));
+        ei.push_back (partOf (*is->getEInc ()->getFieldValue (p, 1), 1));
+        hi.push_back (partOf (*is->getHInc ()->getFieldValue (p, 1), 1));
       }
       putRecord ("EInc", e);
       putRecord ("HInc", h);
+#ifdef COMPLEX_FIELD_VALUES
+      putRecord ("EInc.im", ei);
+      putRecord ("HInc.im", hi);
+#endif
     }
     std::vector<double> scal;
     scal.push_back ((double) is->getGridTimeStep ());

@@ -80,6 +80,7 @@ struct oracle
   int32_t t;
   int prepared;
   double src_phase;        /* test hook: added to every source argument (0 = reference) */
+  int src_kind;            /* 0 the real build; 1 / 2 the real / imaginary part of the complex build's sources */
   double src_t0, src_sigma; /* Gaussian envelope of the point source in time steps (sigma <= 0: none) */
 };
 
@@ -735,7 +736,8 @@ static void o_inc_e (oracle *o, int32_t t)
     e0[i] = e1[i] + (h1[i - 1] - h1[i]) * mod;
   }
   double arg = o->dt * t * 2 * O_PI * o->freq;
-  e0[0] = sin (arg + o->src_phase);
+  /* the complex build sets EInc[0] = (sin (arg), cos (arg)) (InternalScheme.inc.h:3310-3316) */
+  e0[0] = o->src_kind == 2 ? cos (arg + o->src_phase) : sin (arg + o->src_phase);
 }
 
 /* performPlaneWaveHSteps (Source/Scheme/InternalScheme.inc.h:3320-3357); the last cell is skipped. */
@@ -884,7 +886,11 @@ static void o_field_step (oracle *o, int comp, int32_t t)
       if (inside)
       {
         const double timestep = c->is_h ? (double) t + 0.5 : (double) t;
-        double current = sin (dt * timestep * 2 * O_PI * o->freq);
+        /* the complex build's current is (cos, -sin) of dt (timestep - 1) 2 pi f
+         * (Scheme.inc.h:1230-1235) */
+        double current = o->src_kind == 0 ? sin (dt * timestep * 2 * O_PI * o->freq)
+                         : (o->src_kind == 1 ? cos (dt * (timestep - 1) * 2 * O_PI * o->freq)
+                                             : -sin (dt * (timestep - 1) * 2 * O_PI * o->freq));
         current = current * o_mu0 () / dt / 2.0;
         double cb;
         if (o->use_pml)
@@ -959,7 +965,9 @@ static void o_field_step (oracle *o, int comp, int32_t t)
     if (p[0] >= 0 && p[0] < o->n[0] && p[1] >= 0 && p[1] < o->n[1] && p[2] >= 0 && p[2] < o->n[2])
     {
       double ts = c->is_h ? t + 1.0 : t + 0.5;
-      double v = sin (o->dt * ts * 2 * O_PI * o->freq + o->src_phase);
+      /* the complex build's point source is (sin, cos) of the same argument (:2888-2893) */
+      double v = o->src_kind == 2 ? cos (o->dt * ts * 2 * O_PI * o->freq + o->src_phase)
+                                  : sin (o->dt * ts * 2 * O_PI * o->freq + o->src_phase);
       /* Gaussian envelope (no reference counterpart; oracle_set_source_gaussian) */
       if (o->src_sigma > 0) v *= exp (-((ts - o->src_t0) / o->src_sigma) * ((ts - o->src_t0) / o->src_sigma));
       o->f[comp][0][o_idx (o, p[0], p[1], p[2])] = v;
@@ -1098,6 +1106,18 @@ int oracle_set_source_phase (oracle *o, double phase)
 {
   if (!o) return -1;
   o->src_phase = phase;
+  return 0;
+}
+
+/*
+ * Complex field values: the reference's complex build (-DCOMPLEX_FIELD_VALUES) multiplies fields
+ * by real coefficients only, so its real and imaginary parts are two real runs that differ in
+ * their sources.  kind 1 gives the real part, kind 2 the imaginary part, kind 0 the real build.
+ */
+int oracle_set_source_kind (oracle *o, int kind)
+{
+  if (!o || kind < 0 || kind > 2) return -1;
+  o->src_kind = kind;
   return 0;
 }
 

@@ -43,6 +43,8 @@ int oracle_get_scalars (oracle *o, double out[3]);
 int64_t oracle_incident_len (oracle *o);
 /* Test hook: phase added to every source argument (pi/2 gives the cos-source run). */
 int oracle_set_source_phase (oracle *o, double phase);
+/* Sources of the complex build: 1 its real part, 2 its imaginary part, 0 the real build. */
+int oracle_set_source_kind (oracle *o, int kind);
 int oracle_set_source_gaussian (oracle *o, double t0, double sigma);
 
 /* FP64 adjoint of the reference-form update (see yee_oracle.c): runs T steps from a freshly

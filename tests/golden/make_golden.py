@@ -114,6 +114,22 @@ def cases(tmp):
     return out
 
 
+def complex_cases():
+    """name -> argv of the fixtures written by the complex-field build (refdrv_c64): the real parts
+    under the component names, the imaginary parts as "<name>.im"."""
+    return {
+        "c3d_pml_tfsf_sources": (
+            "--3d --size x:16,y:15,z:17 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:5,y:5,z:5 "
+            "--tfsf-size-right x:5,y:5,z:5 --angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 3 "
+            "--eps-sphere-center x:8,y:7,z:8 --eps-sphere-radius 3 --point-source x:7,y:7,z:9 --point-source-hy "
+            "--current-source x:9,y:8,z:8 --current-source-jx --current-source-mz --time-steps 30").split(),
+        "ctmz_pml_tfsf_current": (
+            "--2d-tmz --size x:30,y:26 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:7 "
+            "--tfsf-size-right x:7,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 90 --current-source x:14,y:12 "
+            "--current-source-jz --current-source-mx --point-source x:10,y:9 --point-source-ez --time-steps 50").split(),
+    }
+
+
 CFG1_ARGV = ("--3d --size x:80 --same-size --dx 0.0005 --wavelength 0.02 --angle-teta 0 --angle-phi 0 "
              "--angle-psi 0 --use-tfsf --tfsf-size-left x:20 --tfsf-size-right x:20 --same-size-tfsf --use-pml "
              "--pml-size x:10 --same-size-pml --courant-factor 0.5 --time-steps 200 --eps-sphere 4 "
@@ -159,6 +175,17 @@ def main():
         if not only and len(sys.argv) > 1:
             return
     with tempfile.TemporaryDirectory() as tmp:
+        for name, argv in complex_cases().items():
+            if only and name not in only:
+                continue
+            rec, _ = run_ref_driver(argv, os.path.join(tmp, name + ".bin"), kind="c64")
+            payload = {"argv": np.array(" ".join(argv)), "complex": np.array(True)}
+            for key, val in rec.items():
+                if key.startswith("EInc") or key.startswith("HInc"):
+                    val = val[:256]
+                payload[key] = val
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **payload)
+            print(name, sorted(payload))
         for name, argv in cases(tmp).items():
             if only and name not in only:
                 continue

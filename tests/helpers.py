@@ -67,6 +67,7 @@ def load_golden(name):
             continue
         clean.append(a)
     cfg = parse_args(clean)
+    cfg.complex_fields = bool(data.get("complex", False))   # written by the complex-field build
     shape = cfg.shape
     mats = {}
     if not np.all(data["Eps"] == 1.0):

@@ -45,17 +45,20 @@ def test_engine_fp64_vs_reference_fixture(name, engine_lib):
     say(f"[gpu] fixture {name} fp64")
     eng = run_engine(cfg, mats)
     exact = not cfg.use_pml
-    for f in field_names(cfg):
-        got = eng.field(f).ravel()
-        ref = data[f]
-        if exact:
-            assert np.array_equal(got, ref), f"{name}/{f}: max diff {np.abs(got - ref).max()}"
-        else:
-            assert rel_l2(got, ref) <= TOL_FP64, f"{name}/{f}: rel {rel_l2(got, ref)}"
-    if cfg.use_tfsf:
-        e, h = eng.incident(256)
-        np.testing.assert_array_equal(e, data["EInc"])
-        np.testing.assert_array_equal(h, data["HInc"])
+    # a fixture of the complex-field build holds the imaginary parts as "<name>.im"
+    parts = [(eng, "")] + ([(eng.imag(), ".im")] if cfg.complex_fields else [])
+    for part, sfx in parts:
+        for f in field_names(cfg):
+            got = part.field(f).ravel()
+            ref = data[f + sfx]
+            if exact:
+                assert np.array_equal(got, ref), f"{name}/{f}{sfx}: max diff {np.abs(got - ref).max()}"
+            else:
+                assert rel_l2(got, ref) <= TOL_FP64, f"{name}/{f}{sfx}: rel {rel_l2(got, ref)}"
+        if cfg.use_tfsf:
+            e, h = part.incident(256)
+            np.testing.assert_array_equal(e, data["EInc" + sfx])
+            np.testing.assert_array_equal(h, data["HInc" + sfx])
     eng.close()
 
 
@@ -66,6 +69,9 @@ def test_engine_fp32_vs_fp64_oracle(name, engine_lib):
     say(f"[gpu] fixture {name} fp32")
     eng = run_engine(cfg, mats)
     check_fp32(name, field_names(cfg), eng.field, lambda n: data[n].reshape(cfg.shape))
+    if cfg.complex_fields:
+        im = eng.imag()
+        check_fp32(name + " (imaginary part)", field_names(cfg), im.field, lambda n: data[n + ".im"].reshape(cfg.shape))
     eng.close()
 
 
@@ -384,4 +390,53 @@ def test_fp32_drude_resonance_long_run(engine_lib):
     say("[gpu] fp32 drude resonance 500 steps")
     eng = run_engine(cfg, mats)
     check_fp32("drude resonance 500 steps", field_names(cfg), eng.field, o.field)
+    eng.close()
+
+
+COMPLEX_CASES = [
+    # oblique TF/SF, eps blocks, point and current sources, PML; three-wave fused rows
+    "--3d --size x:24,y:22,z:150 --use-pml --pml-size x:5 --same-size-pml --use-tfsf --tfsf-size-left x:7,y:7,z:30 "
+    "--tfsf-size-right x:7,y:7,z:30 --angle-teta 30 --angle-phi 45 --angle-psi 90 --point-source x:12,y:11,z:70 "
+    "--point-source-hy --current-source x:10,y:12,z:80 --current-source-jx --current-source-mz --time-steps 80",
+    # Drude cube inside PML + TF/SF
+    "--3d --size x:24,y:24,z:24 --use-pml --pml-size x:4 --same-size-pml --use-tfsf --tfsf-size-left x:7 "
+    "--tfsf-size-right x:7 --same-size-tfsf --angle-teta 0 --angle-phi 0 --angle-psi 0 --use-metamaterials "
+    "--time-steps 80",
+    # TMz with TF/SF and sources
+    "--2d-tmz --size x:60,y:52 --use-pml --pml-size x:8,y:8 --use-tfsf --tfsf-size-left x:14,y:13 "
+    "--tfsf-size-right x:13,y:14 --angle-teta 90 --angle-phi 30 --angle-psi 90 --current-source x:30,y:26 "
+    "--current-source-jz --point-source x:20,y:18 --point-source-ez --time-steps 100",
+]
+
+
+@pytest.mark.parametrize("args", COMPLEX_CASES)
+@pytest.mark.parametrize("precision", [8, 4])
+def test_complex_fields_vs_oracle(args, precision, engine_lib):
+    """Complex field values (YEE_COMPLEX_FIELDS): both parts against the oracle's runs with the
+    complex build's sources (pinned bit for bit to the reference's complex build)."""
+    import dataclasses
+    from oracle.binding import run_oracle_complex
+    cfg = dataclasses.replace(parse_args(args.split()), complex_fields=True)
+    mats = {}
+    if "--use-metamaterials" in args:
+        w0 = 2 * np.pi * 299792458.0 / cfg.wavelength
+        om, ga = np.zeros(cfg.shape), np.zeros(cfg.shape)
+        om[9:15, 9:15, 9:15], ga[9:15, 9:15, 9:15] = 1.5 * w0, 0.01 * w0
+        mats = {"omegape": om, "omegapm": om, "gammae": ga, "gammam": ga}
+    elif cfg.dim == 3:
+        rng = np.random.default_rng(5)
+        nb = [(s + 7) // 8 for s in cfg.shape]
+        mats["eps"] = np.kron(rng.uniform(1.0, 6.0, size=nb), np.ones((8, 8, 8)))[: cfg.shape[0], : cfg.shape[1], : cfg.shape[2]]
+    re_, im_ = run_oracle_complex(cfg, mats)
+    cfg.precision = precision
+    say(f"[gpu] complex fields precision {precision}: {args[:50]}")
+    eng = run_engine(cfg, mats)
+    im = eng.imag()
+    for label, part, ref in (("real", eng, re_), ("imag", im, im_)):
+        if precision == 8:
+            for f in field_names(cfg):
+                r = rel_l2(part.field(f), ref.field(f))
+                assert r <= TOL_FP64, f"{label} {f}: {r}"
+        else:
+            check_fp32(f"complex {label}", field_names(cfg), part.field, ref.field)
     eng.close()

@@ -209,20 +209,31 @@ def test_unit_test_1d_golden_vectors(mode):
     fixture written by tests/golden/extract_unit_test_vectors.py).  They come from a complex-field
     build whose plane-wave source is sin(arg) + i cos(arg) (InternalScheme.inc.h:3309-3313); every
     update coefficient is real, so the real part is the real-field run with the sin source: it must
-    match bit for bit.  The imaginary part is the run with the source shifted by pi/2 (sin(arg +
-    pi/2) rounds differently from cos(arg)): within the unit test's own bar, 1e-7
-    (FPEXACT_ACCURACY, Kernels/FieldValue.h:150-160)."""
+    match bit for bit.  The imaginary part is the run with the cos source (the complex build's
+    sources, oracle source kind 2): bit for bit as well.  The run with the sin source shifted by
+    pi/2 (sin(arg + pi/2) rounds differently from cos(arg)) stays within the unit test's own bar,
+    1e-7 (FPEXACT_ACCURACY, Kernels/FieldValue.h:150-160)."""
     v = _unit_test_1d_vectors()[mode]
     cfg = unit_test_1d_config(mode, v)
     re = Oracle(cfg)
     re.prepare()
     re.step(200)
     np.testing.assert_array_equal(re.field(v["component"]).ravel(), np.array(v["re"]))
+    re1 = Oracle(cfg)
+    re1.set_source_kind(1)
+    re1.prepare()
+    re1.step(200)
+    np.testing.assert_array_equal(re1.field(v["component"]).ravel(), np.array(v["re"]))
     im = Oracle(cfg)
-    im.set_source_phase(math.pi / 2)
+    im.set_source_kind(2)
     im.prepare()
     im.step(200)
-    np.testing.assert_allclose(im.field(v["component"]).ravel(), np.array(v["im"]), rtol=0, atol=1e-7)
+    np.testing.assert_array_equal(im.field(v["component"]).ravel(), np.array(v["im"]))
+    shifted = Oracle(cfg)
+    shifted.set_source_phase(math.pi / 2)
+    shifted.prepare()
+    shifted.step(200)
+    np.testing.assert_allclose(shifted.field(v["component"]).ravel(), np.array(v["im"]), rtol=0, atol=1e-7)
     # the TF/SF region carries a unit-amplitude wave, zero outside (test1D, :866-877)
     amp = np.hypot(re.field(v["component"]).ravel(), im.field(v["component"]).ravel())
     inside = amp > 0

@@ -164,3 +164,49 @@ def test_reference_precomputed_coefficients_equal_default_path(args, tmp_path):
         assert names
         for n in names:
             assert np.array_equal(base[n], cacb[n]), (kind, n)
+
+
+COMPLEX_CASES = [
+    # PML + oblique TF/SF + eps sphere (complex incident wave)
+    "--3d --size x:16,y:15,z:17 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:5,y:5,z:5 "
+    "--tfsf-size-right x:5,y:5,z:5 --angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 3 "
+    "--eps-sphere-center x:8,y:7,z:8 --eps-sphere-radius 3 --time-steps 30",
+    # point and current sources with PML (complex point source; the current of the complex build)
+    "--3d --size x:16,y:14,z:18 --use-pml --pml-size x:3,y:3,z:3 --point-source x:6,y:7,z:9 --point-source-ez "
+    "--current-source x:9,y:7,z:8 --current-source-jx --current-source-my --time-steps 35",
+    # Drude sphere with TF/SF
+    "--3d --size x:16,y:16,z:16 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:5 "
+    "--tfsf-size-right x:5 --same-size-tfsf --angle-teta 0 --angle-phi 0 --angle-psi 0 --use-metamaterials "
+    "--omegape-sphere 1.5 --omegape-sphere-center x:8,y:8,z:8 --omegape-sphere-radius 3 --omegapm-sphere 1.5 "
+    "--omegapm-sphere-center x:8,y:8,z:8 --omegapm-sphere-radius 3 --time-steps 30",
+    # TMz and a 1D line
+    "--2d-tmz --size x:30,y:26 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:7 "
+    "--tfsf-size-right x:7,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 90 --current-source x:14,y:12 "
+    "--current-source-jz --time-steps 50",
+    "--1d-exhy --size z:70 --use-pml --pml-size z:8 --use-tfsf --tfsf-size-left z:15 --tfsf-size-right z:15 "
+    "--angle-teta 0 --angle-phi 0 --angle-psi 90 --point-source z:50 --point-source-ex --time-steps 90",
+]
+
+
+@pytest.mark.skipif(not os.path.exists(ref_driver_path("c64")), reason="complex reference driver not built")
+@pytest.mark.parametrize("args", COMPLEX_CASES)
+def test_oracle_complex_parts_match_reference_complex_build(args, tmp_path):
+    """The reference built with -DCOMPLEX_FIELD_VALUES against the oracle's two real runs with
+    the complex build's sources (real part: kind 1, imaginary part: kind 2), bit for bit."""
+    from oracle.binding import run_oracle_complex
+    cfg = parse_args(args.split())
+    rec, _ = run_ref_driver(args.split(), str(tmp_path / "c.bin"), kind="c64")
+    mats = {}
+    if cfg.eps_sphere != 1.0:
+        mats["eps"] = sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius)
+    for key, name in (("OmegaPE", "omegape"), ("OmegaPM", "omegapm"), ("GammaE", "gammae"), ("GammaM", "gammam")):
+        if key in rec:
+            mats[name] = rec[key].reshape(cfg.shape)
+    re_, im_ = run_oracle_complex(cfg, mats)
+    names = field_names(cfg, aux=True)
+    assert names
+    for n in names:
+        assert np.array_equal(re_.field(n).ravel(), rec[n]), (n, "real")
+        assert np.array_equal(im_.field(n).ravel(), rec[n + ".im"]), (n, "imag")
+        if n[0] in "EH" and len(n) == 2:
+            assert np.abs(rec[n + ".im"]).max() > 0 or np.abs(rec[n]).max() == 0, n
