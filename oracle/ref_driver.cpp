@@ -29,6 +29,10 @@
  *     grids (CA1 / CA2 stay null, Scheme.inc.h:932) and the precomputed Drude iteration
  *     dereferences them (InternalScheme.inc.h:2501-2502).
  *
+ * With env YEE_REF_TXT set to a directory, every E / H component of a 3D run is also written
+ * there through the reference's own TXTDumper (File-Management/Dumper/TXTDumper.h:55-92), whole
+ * grid, newest layer, as <dir>/<name>.txt.
+ *
  * Output: one binary file (path from env YEE_REF_OUT, default "ref_out.bin") with
  * records  [char name[16]][int64 count][double values[count]]  holding the latest
  * time layer of every state grid, the material grids and the incident 1D grids.
@@ -44,8 +48,12 @@
 #include <vector>
 #include <sys/time.h>
 
+#include <iomanip>
+#include <limits>
+
 #include "InternalScheme.h"
 #include "Approximation.h"
+#include "TXTDumper.h"   /* expects <iomanip> and <limits> first, as its own TXTDumper.cpp includes them */
 
 namespace
 {
@@ -561,6 +569,29 @@ public:
     return (double) (tv2.tv_usec - tv1.tv_usec) / 1000000 + (double) (tv2.tv_sec - tv1.tv_sec);
   }
 
+  /* The E / H components through the reference's TXTDumper (3D runs). */
+  void dumpText (const char *dir)
+  {
+    if (dim != 3)
+    {
+      return;
+    }
+    TC zero = TC::initAxesCoordinate (0, 0, 0, ct1, ct2, ct3);
+    Grid<TC> *grids[6] = { is->getEx (), is->getEy (), is->getEz (), is->getHx (), is->getHy (), is->getHz () };
+    const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
+    for (int c = 0; c < 6; ++c)
+    {
+      if (!grids[c])
+      {
+        continue;
+      }
+      /* the dumper names its file after the layer it writes: entry 1 for the newest layer */
+      std::vector<std::string> files (2, std::string (dir) + "/" + names[c] + ".txt");
+      TXTDumper<TC> dumper;
+      dumper.dumpGrid (grids[c], zero, grids[c]->getSize (), SOLVER_SETTINGS.getNumTimeSteps (), 1, files);
+    }
+  }
+
   void dump ()
   {
     bool pml = SOLVER_SETTINGS.getDoUsePML ();
@@ -748,6 +779,7 @@ int main (int argc, char **argv)
     run.initMaterials ();
     secs = run.run ();
     run.dump ();
+    if (const char *txt = getenv ("YEE_REF_TXT")) run.dumpText (txt);
   }
   else
 #endif

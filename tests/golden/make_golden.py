@@ -130,6 +130,32 @@ def complex_cases():
     }
 
 
+# Text dumps written by the reference's own TXTDumper (the reference driver with YEE_REF_TXT),
+# FP64 without PML, so the engine's fields are bit-identical to the reference's and its CLI's
+# --save-as-txt dumps must match these files byte for byte.
+TXT_DUMP_ARGV = ("--3d --size x:10,y:9,z:11 --use-tfsf --tfsf-size-left x:3 --tfsf-size-right x:3 --same-size-tfsf "
+                 "--angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 3 --eps-sphere-center x:5,y:4,z:5 "
+                 "--eps-sphere-radius 3 --point-source x:4,y:4,z:6 --point-source-hz --time-steps 20")
+TXT_DUMP_COMPONENTS = ("Ex", "Hz")
+
+
+def write_txt_dumps():
+    out_dir = os.path.join(HERE, "txt_dumps_3d")
+    os.makedirs(out_dir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        os.environ["YEE_REF_TXT"] = tmp
+        try:
+            run_ref_driver(TXT_DUMP_ARGV.split(), os.path.join(tmp, "t.bin"))
+        finally:
+            del os.environ["YEE_REF_TXT"]
+        for c in TXT_DUMP_COMPONENTS:
+            with open(os.path.join(tmp, c + ".txt"), "rb") as src, open(os.path.join(out_dir, c + ".txt"), "wb") as dst:
+                dst.write(src.read())
+    with open(os.path.join(out_dir, "argv.txt"), "w") as fh:
+        fh.write(TXT_DUMP_ARGV + "\n")
+    print("txt_dumps_3d", TXT_DUMP_COMPONENTS)
+
+
 CFG1_ARGV = ("--3d --size x:80 --same-size --dx 0.0005 --wavelength 0.02 --angle-teta 0 --angle-phi 0 "
              "--angle-psi 0 --use-tfsf --tfsf-size-left x:20 --tfsf-size-right x:20 --same-size-tfsf --use-pml "
              "--pml-size x:10 --same-size-pml --courant-factor 0.5 --time-steps 200 --eps-sphere 4 "
@@ -169,6 +195,11 @@ def write_cfg1(bin_path: str = ""):
 
 def main():
     only = set(a for a in sys.argv[1:] if not a.startswith("--"))   # optional: the fixture names to (re)generate
+    if not only or "txt_dumps_3d" in only:
+        write_txt_dumps()
+        only.discard("txt_dumps_3d")
+        if not only and len(sys.argv) > 1:
+            return
     if not only or "cfg1_sphere80" in only:
         write_cfg1(next((a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--from=")), ""))
         only.discard("cfg1_sphere80")

@@ -381,3 +381,28 @@ def test_cli_txt_material_input(engine_lib, tmp_path):
         out = subprocess.run([exe] + (base + f" --load-eps-from-file {f}").split(), capture_output=True, text=True,
                              timeout=120, cwd=tmp_path)
         assert out.returncode != 0 and f in out.stderr, (f, out.stderr)
+
+
+def test_cli_text_dump_matches_reference_txtdumper_byte_for_byte(engine_lib, tmp_path):
+    """--save-res --save-as-txt of the whole grid against the files the reference's own TXTDumper
+    wrote for the same run (tests/golden/txt_dumps_3d, FP64 without PML: the fields are
+    bit-identical, so the text must be too: coordinates, 15 significant digits, line order)."""
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "txt_dumps_3d")
+    with open(os.path.join(gold, "argv.txt")) as fh:
+        argv = fh.read().split()
+    steps = int(argv[argv.index("--time-steps") + 1])
+    say("[gpu] cli text dump vs the reference's TXTDumper")
+    out = subprocess.run([exe] + argv + ["--save-res", "--save-as-txt", "--manual-save-coords", "--save-start-coord",
+                                         "x:0,y:0,z:0", "--save-end-coord", "x:10,y:9,z:11"],
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    for comp in ("Ex", "Hz"):
+        with open(os.path.join(gold, comp + ".txt"), "rb") as fh:
+            want = fh.read()
+        with open(tmp_path / f"previous-1_[timestep={steps}]_[pid=0]_[name={comp}].txt", "rb") as fh:
+            got = fh.read()
+        assert len(want) > 1000
+        assert got == want, (comp, got[:200], want[:200])
