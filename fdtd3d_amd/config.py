@@ -41,6 +41,7 @@ USE_PML = 0x1
 USE_TFSF = 0x2
 USE_METAMATERIALS = 0x4
 COMPLEX_FIELDS = 0x8
+DOUBLE_MATERIAL_PRECISION = 0x10
 
 COMPONENTS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz",
               "Dx", "Dy", "Dz", "Bx", "By", "Bz",
@@ -100,6 +101,8 @@ class Config:
     use_metamaterials: bool = False
     # complex field values (the reference's -DCOMPLEX_FIELD_VALUES build; no command-line flag)
     complex_fields: bool = False
+    # --use-double-material-precision: material grids of twice the field grid's extent per axis
+    double_material: bool = False
     point_source: List[str] = field(default_factory=list)   # component names, e.g. ["Ez"]
     point_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
     # --current-source-j<x|y|z> / -m<x|y|z> as component names ("Ex" for Jx, "Hx" for Mx), --current-source
@@ -146,6 +149,12 @@ class Config:
         return tuple(self.size[a] for a in self.active_axes)
 
     @property
+    def material_shape(self):
+        """Shape of the material grids: the field grid's, doubled along every axis with
+        --use-double-material-precision (YeeGridLayout sizeEps, YeeGridLayout.cpp:1972)."""
+        return tuple(n * 2 for n in self.shape) if self.double_material else self.shape
+
+    @property
     def missing_axis(self) -> int:
         """The axis a 2D scheme lacks (-1 in 3D and 1D)."""
         for scheme, axis in SCHEMES_2D.values():
@@ -188,7 +197,8 @@ class Config:
         c.courant = self.courant
         c.flags = ((USE_PML if self.use_pml else 0) | (USE_TFSF if self.use_tfsf else 0)
                    | (USE_METAMATERIALS if self.use_metamaterials else 0)
-                   | (COMPLEX_FIELDS if self.complex_fields else 0))
+                   | (COMPLEX_FIELDS if self.complex_fields else 0)
+                   | (DOUBLE_MATERIAL_PRECISION if self.double_material else 0))
         mask = 0
         for name in self.point_source:
             mask |= 1 << COMP_INDEX[name]
@@ -236,8 +246,7 @@ _IGNORED_FLAGS = {
 }
 # options of the reference whose feature is outside this engine: rejected loudly
 _UNSUPPORTED = {
-
-    "--use-double-material-precision", "--eps-normed", "--mu-normed",
+    "--eps-normed", "--mu-normed",
     # the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
     # (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
     "--use-ca-cb-pml-metamaterials",
@@ -368,6 +377,8 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             cfg.sphere_accuracy = int(val())
         elif a == "--stair-sphere-approx":
             cfg.stair_sphere = True
+        elif a == "--use-double-material-precision":
+            cfg.double_material = True
         elif a in _IGNORED_WITH_ARG:
             val()
         elif a in _IGNORED_FLAGS:
@@ -379,6 +390,9 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
         i += 1
     if cfg.use_metamaterials and not cfg.use_pml:
         raise OptionError("--use-metamaterials requires --use-pml (reference: Scheme.inc.h:889-892)")
+    if cfg.use_metamaterials and cfg.double_material:
+        raise OptionError("--use-double-material-precision excludes --use-metamaterials (the reference's "
+                          "8-cell Drude average is UNREACHABLE, Approximation.cpp:196)")
     if cfg.layout != 0:
         raise OptionError("only --layout-type 0 (E_CENTERED) is supported")
     return cfg

@@ -129,7 +129,7 @@ const std::set<std::string> kIgnoredFlags = {
   "--use-ca-cb", "--use-ca-cb-pml",
 };
 const std::set<std::string> kUnsupported = {
-  "--use-double-material-precision", "--eps-normed", "--mu-normed",
+  "--eps-normed", "--mu-normed",
   // the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
   // (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
   "--use-ca-cb-pml-metamaterials",
@@ -219,6 +219,7 @@ Options parse (int argc, char **argv)
     else if (a == "--point-source") parse_coord (val (), 0, c.point_source_pos);
     else if (a == "--sphere-accuracy") o.accuracy = std::stoi (val ());
     else if (a == "--stair-sphere-approx") o.stair = true;
+    else if (a == "--use-double-material-precision") o.cfg.flags |= YEE_DOUBLE_MATERIAL_PRECISION;
     else if (a == "--save-res") o.save.res = true;
     else if (a == "--save-interm-res") o.save.interm = true;
     else if (a == "--interm-save-step") o.save.interm_step = std::stoll (val ());
@@ -402,6 +403,16 @@ void scheme_size (const yee_config &c, int64_t size[3])
   for (int s = 0; s < 3; ++s) size[s] = s < (int) ax.size () ? c.size[ax[s]] : 1;
 }
 
+// Material grid extent per scheme coordinate: the field grid's, twice it with
+// --use-double-material-precision (YeeGridLayout sizeEps, YeeGridLayout.cpp:1972).
+void material_size (const yee_config &c, int64_t size[3])
+{
+  scheme_size (c, size);
+  const int dim = c.scheme == YEE_SCHEME_DIM3 ? 3 : (c.scheme >= YEE_SCHEME_TEX ? 2 : 1);
+  if (c.flags & YEE_DOUBLE_MATERIAL_PRECISION)
+    for (int s = 0; s < dim; ++s) size[s] *= 2;
+}
+
 // Dump range [start, end) of a component, per scheme coordinate (Scheme::getStartCoord /
 // getEndCoord, Scheme.inc.h:4647-4790; the middle plane along the orthogonal axis in 3D,
 // SchemeHelper.h:860-990).  comp -1: a material grid, whose minimum coordinate is taken as 0
@@ -497,7 +508,7 @@ void save_materials (const Options &o, yee_handle *h)
   const yee_config &c = o.cfg;
   const int dim = scheme_dim (c);
   int64_t size[3];
-  scheme_size (c, size);
+  material_size (c, size);
   std::vector<double> v ((size_t) (size[0] * size[1] * size[2]));
   static const char *names[6] = { "Eps", "Mu", "OmegaPE", "OmegaPM", "GammaE", "GammaM" };
   const int count = (c.flags & YEE_USE_METAMATERIALS) ? 6 : 2;
@@ -601,7 +612,9 @@ int main (int argc, char **argv)
     if (m >= 2 && !meta) continue;
     if (!o.file[m].empty ())
     {
-      std::vector<double> v = load_material (o.file[m], scheme_dim (c), ssize);
+      int64_t msize[3];
+      material_size (c, msize);
+      std::vector<double> v = load_material (o.file[m], scheme_dim (c), msize);
       check (yee_set_material (h, m, v.data ()));
     }
     if (m < 4)

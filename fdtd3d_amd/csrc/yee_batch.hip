@@ -546,8 +546,9 @@ int yee_batch_create (const yee_config *cfgs, int32_t nsim, yee_batch **out)
     if (c.scheme != YEE_SCHEME_TMZ) return batch_fail (YEE_EINVAL, "batched runs are TMz (YEE_SCHEME_TMZ)");
     if (c.layout != 0) return batch_fail (YEE_EINVAL, "only the E_CENTERED layout is supported");
     if (c.precision != prec || (prec != 4 && prec != 8)) return batch_fail (YEE_EINVAL, "one precision (4 or 8) per batch");
-    if (c.flags & (YEE_USE_TFSF | YEE_USE_METAMATERIALS))
-      return batch_fail (YEE_ENOTSUP, "batched runs support PML and point sources (no TF/SF, no metamaterials)");
+    if (c.flags & (YEE_USE_TFSF | YEE_USE_METAMATERIALS | YEE_COMPLEX_FIELDS | YEE_DOUBLE_MATERIAL_PRECISION))
+      return batch_fail (YEE_ENOTSUP, "batched runs support PML and point sources (no TF/SF, metamaterials, complex "
+                                      "fields or double material precision)");
     if (c.current_source)
       return batch_fail (YEE_ENOTSUP, "batched runs take point sources, not current sources");
     for (int a = 0; a < 2; ++a)

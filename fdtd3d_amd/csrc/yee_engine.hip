@@ -35,6 +35,8 @@ template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, u
 template <typename T, int COMP, int DIM> __global__ void scattered_kernel (StepArgs<T>, const T *, double *, int, int64_t, int64_t);
 template <typename T, int BY, int MAT, int MAXW> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
+template <typename T>
+__global__ void fine_average (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *);
 
 static thread_local std::string g_error;
 
@@ -162,6 +164,12 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   a.gm = (const T *) h->mat[YEE_MAT_GAMMAM];
   a.eps_u = (T) h->mat_u[YEE_MAT_EPS];
   a.mu_u = (T) h->mat_u[YEE_MAT_MU];
+  if (h->fine ())
+  {
+    // eps / mu live on the fine grid: the kernels read the per-component averages instead
+    a.eps = a.mu = nullptr;
+    for (int c = 0; c < 6; ++c) a.matc[c] = (const T *) h->matc[c];
+  }
   a.dt = (T) h->sc.dt;
   a.dx = (T) h->cfg.dx;
   a.eps0 = (T) eps0 ();
@@ -551,15 +559,20 @@ int step_once (yee_handle *h)
 constexpr int64_t kPadFront = 64;
 int64_t pad_back (const yee_handle *h) { return 4 * h->n[2] + 64; }
 
-int alloc_grid (yee_handle *h, void **view)
+int alloc_cells (yee_handle *h, void **view, int64_t n)
 {
-  const size_t elems = (size_t) (kPadFront + h->ncell + pad_back (h));
+  const size_t elems = (size_t) (kPadFront + n + pad_back (h));
   void *raw = nullptr;
   HIP_TRY (hipMalloc (&raw, elems * h->wsize));
   h->grids.push_back (raw);
   HIP_TRY (hipMemsetAsync (raw, 0, elems * h->wsize, h->stream));
   *view = (char *) raw + kPadFront * h->wsize;
   return YEE_OK;
+}
+
+int alloc_grid (yee_handle *h, void **view)
+{
+  return alloc_cells (h, view, h->ncell);
 }
 
 int alloc_zero (yee_handle *h, void **p, size_t bytes)
@@ -621,6 +634,9 @@ int validate (const yee_config *c)
     return fail (YEE_EINVAL, "dx, wavelength and courant factor must be positive");
   if ((c->flags & YEE_USE_METAMATERIALS) && !(c->flags & YEE_USE_PML))
     return fail (YEE_EINVAL, "metamaterials require PML (reference: Scheme.inc.h:889-892)");
+  if ((c->flags & YEE_USE_METAMATERIALS) && (c->flags & YEE_DOUBLE_MATERIAL_PRECISION))
+    return fail (YEE_EINVAL, "double material precision excludes metamaterials (the reference's 8-cell Drude "
+                             "average is UNREACHABLE, Approximation.cpp:196)");
   if (c->time_steps < 0) return fail (YEE_EINVAL, "negative time steps");
   if (c->current_source & 0x3fu)
     for (int a = 0; a < 3; ++a)
@@ -634,7 +650,7 @@ int upload_as (yee_handle *h, void **dst, const S *src, size_t n)
 {
   if (!*dst)
   {
-    int rc = alloc_grid (h, dst);
+    int rc = alloc_cells (h, dst, (int64_t) n);
     if (rc) return rc;
     HIP_TRY (hipStreamSynchronize (h->stream));
   }
@@ -661,7 +677,7 @@ int set_material_impl (yee_handle *h, int32_t which, const S *values)
   if (!h || !values) return fail (YEE_EINVAL, "null argument");
   if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
   if (h->prepared) return fail (YEE_ESTATE, "materials must be set before yee_prepare");
-  const size_t n = (size_t) h->ncell;
+  const size_t n = (size_t) h->mcell;   // the fine grid with double material precision
   bool uniform = true;
   bool nonzero = false;
   int64_t lo[3] = { h->n[0], h->n[1], h->n[2] }, hi[3] = { -1, -1, -1 };
@@ -674,9 +690,9 @@ int set_material_impl (yee_handle *h, int32_t which, const S *values)
       nonzero = true;
       if (which >= 2)
       {
-        const int64_t k = (int64_t) (q % h->n[2]);
-        const int64_t j = (int64_t) ((q / h->n[2]) % h->n[1]);
-        const int64_t i = (int64_t) (q / ((size_t) h->n[2] * h->n[1]));
+        const int64_t k = (int64_t) (q % h->mn[2]);
+        const int64_t j = (int64_t) ((q / h->mn[2]) % h->mn[1]);
+        const int64_t i = (int64_t) (q / ((size_t) h->mn[2] * h->mn[1]));
         const int64_t p[3] = { i, j, k };
         for (int a = 0; a < 3; ++a)
         {
@@ -710,6 +726,36 @@ int set_material_impl (yee_handle *h, int32_t which, const S *values)
     return YEE_OK;
   }
   return upload_as (h, &h->mat[which], values, n);
+}
+
+// Double material precision: the 8-cell averages of eps (E components) and mu (H components)
+// per field cell, computed on the device from the fine grids (a uniform material needs none:
+// every average of a constant is the constant).
+template <typename T>
+int fine_averages (yee_handle *h)
+{
+  int act = 0;
+  for (int a = 0; a < 3; ++a)
+    if (axis_active (h->cfg, a)) act |= 1 << a;
+  for (int g = 0; g < 2; ++g)
+  {
+    if (!h->mat[g]) continue;
+    T *out[3] = { nullptr, nullptr, nullptr };
+    for (int r = 0; r < 3; ++r)
+    {
+      const int c = 3 * g + r;
+      if (!h->present[c]) continue;
+      int rc = alloc_grid (h, &h->matc[c]);
+      if (rc) return rc;
+      out[r] = (T *) h->matc[c];
+    }
+    const int64_t blocks = std::min<int64_t> ((h->ncell + 255) / 256, 65536);
+    hipLaunchKernelGGL ((fine_average<T>), dim3 ((unsigned) blocks), dim3 (256), 0, h->stream,
+                        (const T *) h->mat[g], h->mn[0], h->mn[1], h->mn[2], (int) h->n[0], (int) h->n[1],
+                        (int) h->n[2], act, 3 * g, out[0], out[1], out[2]);
+    HIP_TRY (hipGetLastError ());
+  }
+  return YEE_OK;
 }
 
 template <typename T>
@@ -817,6 +863,8 @@ int yee_create (const yee_config *cfg, yee_handle **out)
     h->cfg.current_source_pos[a] = 0;
   }
   h->ncell = h->n[0] * h->n[1] * h->n[2];
+  for (int a = 0; a < 3; ++a) h->mn[a] = (h->fine () && axis_active (*cfg, a)) ? 2 * h->n[a] : h->n[a];
+  h->mcell = h->mn[0] * h->mn[1] * h->mn[2];
   h->wsize = cfg->precision == 4 ? 4 : 8;
   h->mode = h->use_meta () ? 2 : (h->use_pml () ? 1 : 0);
   for (int c = 0; c < 6; ++c)
@@ -855,7 +903,8 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   const char *kern = std::getenv ("YEE_KERNEL");
   const bool want_fused = !(kern && std::strcmp (kern, "split") == 0);
   h->fused = false;
-  if (h->dim == 3 && h->miss < 0 && line < 0 && h->n[2] <= 1024 && want_fused)
+  // (double material precision keeps the half-step kernels: they read per-component averages)
+  if (h->dim == 3 && h->miss < 0 && line < 0 && h->n[2] <= 1024 && want_fused && !h->fine ())
   {
     bool ok = true;
     for (int ax = 0; ax < 3; ++ax)
@@ -978,9 +1027,13 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
   if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
   if (!stair && accuracy <= 0) return fail (YEE_EINVAL, "sphere accuracy must be positive");
   if (line_axis (h->cfg) >= 0) return fail (YEE_EINVAL, "spheres on a 1D grid are not supported");
-  std::vector<double> v ((size_t) h->ncell);
-  const double cx = (double) center[0] + 0.5, cy = (double) center[1] + 0.5, cz = (double) center[2] + 0.5;
-  const int64_t nx = h->n[0], ny = h->n[1], nz = h->n[2];
+  std::vector<double> v ((size_t) h->mcell);
+  // double material precision: the fine grid, centre 2 c + 0.5 and radius 2 r (Scheme.inc.h:2992-3009)
+  const double mod = h->fine () ? 2.0 : 1.0;
+  const double cx = (double) center[0] * mod + 0.5, cy = (double) center[1] * mod + 0.5,
+               cz = (double) center[2] * mod + 0.5;
+  radius *= mod;
+  const int64_t nx = h->mn[0], ny = h->mn[1], nz = h->mn[2];
   // the accurate approximation integrates accuracy^2 columns per cell: split x planes over threads
   const unsigned nthr = std::max (1u, std::min (16u, std::thread::hardware_concurrency ()));
   std::vector<std::thread> pool;
@@ -1043,6 +1096,11 @@ int yee_prepare (yee_handle *h)
   {
     // incident-wave geometry of the TF/SF correction, read by the corrected cells only
     int rc = h->fp32 () ? upload_tfsf_geom<float> (h) : upload_tfsf_geom<double> (h);
+    if (rc) return rc;
+  }
+  if (h->fine ())
+  {
+    int rc = h->fp32 () ? fine_averages<float> (h) : fine_averages<double> (h);
     if (rc) return rc;
   }
   if (h->use_pml ())
@@ -1185,11 +1243,18 @@ int yee_get_field (yee_handle *h, int32_t comp, double *out)
   return YEE_OK;
 }
 
+int yee_material_size (yee_handle *h, int64_t out[3])
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  for (int a = 0; a < 3; ++a) out[a] = h->mn[a];
+  return YEE_OK;
+}
+
 int yee_get_material (yee_handle *h, int32_t which, double *out)
 {
   if (!h || !out) return fail (YEE_EINVAL, "null argument");
   if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
-  const size_t n = (size_t) h->ncell;
+  const size_t n = (size_t) h->mcell;
   HIP_TRY (hipStreamSynchronize (h->stream));
   if (!h->mat[which])
   {

@@ -57,6 +57,12 @@ struct yee_handle
 
   void *mat[6] = {};
   double mat_u[6] = { 1, 1, 0, 0, 0, 0 };
+  // material grid extent (twice the field grid along the scheme's axes with double material
+  // precision) and, in that mode, the 8-cell averages of eps (E components) and mu (H
+  // components) per field cell, computed at yee_prepare (null for a uniform material)
+  int64_t mn[3] = { 1, 1, 1 };
+  int64_t mcell = 0;
+  void *matc[6] = {};
   bool mat_nonzero[6] = {};
   int64_t box_lo[3] = { 0, 0, 0 }, box_hi[3] = { 0, 0, 0 };   // Drude-material bounding box (eps cells)
   bool box_any = false;
@@ -78,6 +84,7 @@ struct yee_handle
   bool use_pml () const { return (cfg.flags & YEE_USE_PML) != 0; }
   bool use_tfsf () const { return (cfg.flags & YEE_USE_TFSF) != 0; }
   bool use_meta () const { return (cfg.flags & YEE_USE_METAMATERIALS) != 0; }
+  bool fine () const { return (cfg.flags & YEE_DOUBLE_MATERIAL_PRECISION) != 0; }
 
   yee::AdjointState *adj = nullptr;   // forward/backward pair state (yee_adjoint_setup)
 

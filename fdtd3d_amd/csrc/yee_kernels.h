@@ -53,6 +53,9 @@ struct StepArgs
   // materials: per-cell arrays, or a uniform value when the pointer is null
   const T *eps, *mu, *ope, *opm, *ge, *gm;
   T eps_u, mu_u;
+  // double material precision: the averaged eps (E) / mu (H) of each component per field cell,
+  // read in place of the 2- / 4-cell averages of eps / mu when non-null
+  const T *matc[6];
 
   // scalars in the field precision
   T dt, dx, eps0, mu0;

@@ -210,7 +210,14 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
   T vy[4] = { gld (Hx + q - sz), hx, gld (Hz + q - sx), hz };
   T vz[4] = { gld (Hy + q - sx), hy, gld (Hx + q - sy), hx };
   T ex_m = a.eps_u, ey_m = a.eps_u, ez_m = a.eps_u;
-  if (a.eps)
+  if (a.matc[0])
+  {
+    // double material precision: the 8-cell averages, precomputed per component
+    ex_m = gld (a.matc[0] + q);
+    ey_m = gld (a.matc[1] + q);
+    ez_m = gld (a.matc[2] + q);
+  }
+  else if (a.eps)
   {
     // 2-cell averages along each component's axis (getMaterial, E_CENTERED)
     const T e = gld (a.eps + q);
@@ -277,7 +284,13 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
   }
   // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
   T hx_m = a.mu_u, hy_m = a.mu_u, hz_m = a.mu_u;
-  if (a.mu)
+  if (a.matc[3])
+  {
+    hx_m = gld (a.matc[3] + q);
+    hy_m = gld (a.matc[4] + q);
+    hz_m = gld (a.matc[5] + q);
+  }
+  else if (a.mu)
   {
     const T *m = a.mu;
     hx_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sy), gld (m + q + sy + sz));
@@ -943,7 +956,7 @@ __device__ __forceinline__ void tmz_e_cell (const StepArgs<T> &a, int i, int j)
   if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0)) tfsf_apply<T, 2, 2> (a, i, j, 0, v);
   // Ez sits on the eps point: the 2-cell average reads the same cell twice
   const T e = a.eps ? a.eps[q] : a.eps_u;
-  const T mat = avg2 (e, e);
+  const T mat = a.matc[2] ? a.matc[2][q] : avg2 (e, e);
   T om = 0, ga = 0;
   if (MODE == 2 && special)
   {
@@ -977,7 +990,12 @@ __device__ __forceinline__ void tmz_h_cell (const StepArgs<T> &a, int i, int j)
     tfsf_apply<T, 4, 2> (a, i, j, 0, vy);
   }
   T mx, my;
-  if (a.mu)
+  if (a.matc[3])
+  {
+    mx = a.matc[3][q];
+    my = a.matc[4][q];
+  }
+  else if (a.mu)
   {
     const T *m = a.mu;
     mx = avg4 (m[q], m[q], m[q + sy], m[q + sy]);
@@ -1034,7 +1052,12 @@ __device__ __forceinline__ void tez_e_cell (const StepArgs<T> &a, int i, int j)
   }
   // 2-cell averages along each component's axis
   T ex_m = a.eps_u, ey_m = a.eps_u;
-  if (a.eps)
+  if (a.matc[0])
+  {
+    ex_m = a.matc[0][q];
+    ey_m = a.matc[1][q];
+  }
+  else if (a.eps)
   {
     const T e = a.eps[q];
     ex_m = avg2 (e, a.eps[q + sx]);
@@ -1075,7 +1098,11 @@ __device__ __forceinline__ void tez_h_cell (const StepArgs<T> &a, int i, int j)
   if (a.tfsf && tfsf_near<T, 2> (a, i, j, 0)) tfsf_apply<T, 5, 2> (a, i, j, 0, vz);
   // 4-cell transverse average
   T hz_m = a.mu_u;
-  if (a.mu)
+  if (a.matc[5])
+  {
+    hz_m = a.matc[5][q];
+  }
+  else if (a.mu)
   {
     const T *m = a.mu;
     hz_m = avg4 (m[q], m[q + sy], m[q + sx], m[q + sx + sy]);
@@ -1587,6 +1614,68 @@ __global__ void point_source (T *f, int64_t q, double value)
   if (threadIdx.x == 0 && blockIdx.x == 0) f[q] = (T) value;
 }
 
+// Double material precision (--use-double-material-precision): the material grid has twice the
+// field grid's extent along each axis of the scheme, and every field point averages the 8 fine
+// cells around it (InternalScheme::getMaterial, InternalScheme.inc.h:2924-2950, with the cells
+// of YeeGridLayout::initMaterialCoordinates, YeeGridLayout.h:866-1037, and the pairwise
+// Approximation::approximateMaterial, Approximation.cpp:36-54).  Along an axis where the point
+// sits at an integer coordinate (an E component's own axis, an H component's other axes) the
+// fine cells are 2p+1, 2p+2, else 2p, 2p+1; an axis the scheme lacks contributes cell 0 twice.
+// The pairs nest innermost-first along the component's (inner, mid, outer) axes: Ex (y, x, z),
+// Ey (x, y, z), Ez (y, x, z), Hx (x, z, y), Hy (y, z, x), Hz (z, y, x), the order in which the
+// reference lists the 8 cells.  One thread per field cell writes the averages of the three
+// components cbase .. cbase + 2 (eps for E, mu for H) that have an output array; indices are
+// clamped to the fine grid (the clamped values reach only cells the sweeps never update).
+template <typename T>
+__device__ __forceinline__ T fine_avg8 (const T *m, const int64_t mn[3], int act, int c, const int p[3])
+{
+  int64_t lo[3], hi[3];
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    const bool integer = c >= 3 ? ax != c - 3 : ax == c;
+    if (!((act >> ax) & 1))
+    {
+      lo[ax] = hi[ax] = 0;
+      continue;
+    }
+    const int64_t top = mn[ax] - 1, l = (int64_t) 2 * p[ax] + (integer ? 1 : 0);
+    lo[ax] = l < top ? l : top;
+    hi[ax] = l + 1 < top ? l + 1 : top;
+  }
+  int ai, am, ao;
+  switch (c)
+  {
+    case 0: ai = 1; am = 0; ao = 2; break;
+    case 1: ai = 0; am = 1; ao = 2; break;
+    case 2: ai = 1; am = 0; ao = 2; break;
+    case 3: ai = 0; am = 2; ao = 1; break;
+    case 4: ai = 1; am = 2; ao = 0; break;
+    default: ai = 2; am = 1; ao = 0; break;
+  }
+  const int64_t st[3] = { mn[1] * mn[2], mn[2], 1 };
+  auto v = [&] (int si, int sm, int so) -> T {
+    return m[(si ? hi[ai] : lo[ai]) * st[ai] + (sm ? hi[am] : lo[am]) * st[am] + (so ? hi[ao] : lo[ao]) * st[ao]];
+  };
+  const T q0 = avg2 (avg2 (v (0, 0, 0), v (1, 0, 0)), avg2 (v (0, 1, 0), v (1, 1, 0)));
+  const T q1 = avg2 (avg2 (v (0, 0, 1), v (1, 0, 1)), avg2 (v (0, 1, 1), v (1, 1, 1)));
+  return avg2 (q0, q1);
+}
+
+template <typename T>
+__global__ void fine_average (const T *m, int64_t mn0, int64_t mn1, int64_t mn2, int nx, int ny, int nz, int act,
+                              int cbase, T *o0, T *o1, T *o2)
+{
+  const int64_t n = (int64_t) nx * ny * nz;
+  const int64_t mn[3] = { mn0, mn1, mn2 };
+  for (int64_t q = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; q < n; q += (int64_t) gridDim.x * blockDim.x)
+  {
+    const int p[3] = { (int) (q / ((int64_t) ny * nz)), (int) ((q / nz) % ny), (int) (q % nz) };
+    if (o0) gst (o0 + q, fine_avg8 (m, mn, act, cbase, p));
+    if (o1) gst (o1 + q, fine_avg8 (m, mn, act, cbase + 1, p));
+    if (o2) gst (o2 + q, fine_avg8 (m, mn, act, cbase + 2, p));
+  }
+}
+
 // TF/SF predicate mask of one component for every cell (set-up check export).
 // Scattered field of one component (Scheme::makeGridScattered, Scheme.inc.h:4113-4228): the
 // newest field minus the incident plane wave at the component's grid points inside the TF/SF
@@ -1690,6 +1779,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *); \
+  template __global__ void fine_average<T> (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *); \
   template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
   template __global__ void scattered_kernel<T, 1, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
   template __global__ void scattered_kernel<T, 2, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \

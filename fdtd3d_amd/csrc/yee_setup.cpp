@@ -221,9 +221,12 @@ int missing_axis (const yee_config &cfg)
 
 std::vector<double> sigma_profile (const yee_config &cfg, int axis)
 {
-  const int64_t n = cfg.size[axis];
+  // double material precision: the sigma grid is the fine material grid, 2 N cells with a PML
+  // of 2 P cells and boundary 2 P dx (SchemeHelper.h:598, 640, 682)
+  const int64_t mod = (cfg.flags & YEE_DOUBLE_MATERIAL_PRECISION) ? 2 : 1;
+  const int64_t n = cfg.size[axis] * mod;
   std::vector<double> prof (n, 0.0);
-  const double P = (double) cfg.pml[axis];
+  const double P = (double) (cfg.pml[axis] * mod);
   const double boundary = P * cfg.dx;
   for (int64_t p = 0; p < n; ++p)
   {
@@ -256,6 +259,15 @@ double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p,
 {
   const CompDesc &c = kComp[comp];
   const bool integer_coord = c.is_h ? (axis != c.axis) : (axis == c.axis);
+  if ((int64_t) prof.size () == 2 * n)
+  {
+    // fine sigma grid (double material precision): the 8-cell average of a profile that varies
+    // along one axis only reduces exactly to the mean of its two fine cells on that axis,
+    // 2p+1 and 2p+2 at an integer coordinate, 2p and 2p+1 at a half-integer one
+    const int64_t f = 2 * p + (integer_coord ? 1 : 0);
+    const double hi = f + 1 < 2 * n ? prof[f + 1] : 0.0;
+    return (prof[f] + hi) / 2.0;
+  }
   if (!integer_coord)
   {
     return prof[p];

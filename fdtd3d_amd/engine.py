@@ -28,7 +28,7 @@ EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
-    "yee_get_scattered_field_noalloc", "yee_get_imag_part",
+    "yee_get_scattered_field_noalloc", "yee_get_imag_part", "yee_material_size",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
     "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
@@ -78,6 +78,7 @@ def load_library():
     L.yee_get_scattered_field_noalloc.argtypes = [vp, i32, vp]
     L.yee_get_imag_part.argtypes = [vp, ctypes.POINTER(vp)]
     L.yee_get_material.argtypes = [vp, i32, vp]
+    L.yee_material_size.argtypes = [vp, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.yee_get_scalars.argtypes = [vp, vp]
@@ -166,6 +167,10 @@ class Engine:
         self.h = h
         self.shape = cfg.shape
         self.dtype = np.float32 if cfg.precision == 4 else np.float64
+        # material grids: the field grid's shape, or the fine grid of double material precision
+        ms = (ctypes.c_int64 * 3)()
+        _check(L.yee_material_size(h, ms))
+        self.material_shape = tuple(int(ms[a]) for a in range(3) if cfg.dim == 3 or a in cfg.active_axes)
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -185,8 +190,8 @@ class Engine:
         self.close()
 
     def set_material(self, name: str, values: np.ndarray) -> None:
-        if int(np.size(values)) != int(np.prod(self.shape)):
-            raise ValueError(f"material {name}: expected {self.shape}, got {np.shape(values)}")
+        if int(np.size(values)) != int(np.prod(self.material_shape)):
+            raise ValueError(f"material {name}: expected {self.material_shape}, got {np.shape(values)}")
         if isinstance(values, np.ndarray) and values.dtype == np.float32:
             arr = np.ascontiguousarray(values)
             _check(load_library().yee_set_material_f32(self.h, MAT_INDEX[name], arr.ctypes.data))
@@ -281,7 +286,7 @@ class Engine:
 
     def material(self, name: str) -> np.ndarray:
         """Material grid as the engine holds it (relative eps / mu; omega_p, gamma in rad/s)."""
-        out = np.empty(self.shape, dtype=np.float64)
+        out = np.empty(self.material_shape, dtype=np.float64)
         _check(load_library().yee_get_material(self.h, MAT_INDEX[name], out.ctypes.data))
         return out
 

@@ -60,6 +60,15 @@ extern "C" {
 #define YEE_USE_METAMATERIALS 0x4u  /* --use-metamaterials (requires --use-pml, Scheme.inc.h:889-892) */
 #define YEE_COMPLEX_FIELDS 0x8u     /* complex field values (the reference's -DCOMPLEX_FIELD_VALUES build,
                                        Kernels/FieldValue.h:68-72); see yee_get_imag_part */
+#define YEE_DOUBLE_MATERIAL_PRECISION 0x10u
+                                    /* --use-double-material-precision: material grids (eps, mu) of twice
+                                       the field grid's extent along each axis of the scheme, each field
+                                       point averaging the 8 material cells around it
+                                       (YeeGridLayout::initMaterialCoordinates, Layout/YeeGridLayout.h:866-1037;
+                                       InternalScheme::getMaterial, Scheme/InternalScheme.inc.h:2924-2950);
+                                       PML sigma profiles on the same fine grid (SchemeHelper.h:598, 640, 682).
+                                       Excludes metamaterials: the reference's 8-cell Drude average is
+                                       UNREACHABLE (Layout/Approximation.cpp:196) */
 
 /* Field components (GridType order of the reference). */
 enum yee_component
@@ -133,11 +142,19 @@ void yee_destroy (yee_handle *h);
 
 /*
  * Sets one material grid (full [x][y][z] FP64 array of size[0]*size[1]*size[2] values,
- * or [x][y] in 2D).  Defaults: eps = mu = 1, omega/gamma = 0.  Omega values are in
+ * or [x][y] in 2D; the fine grid of yee_material_size with YEE_DOUBLE_MATERIAL_PRECISION).  Defaults: eps = mu = 1, omega/gamma = 0.  Omega values are in
  * rad/s (the reference scales --omegape-sphere by 2*pi*f before storing, Scheme.inc.h:3091).
  * Replaces the material part of Scheme::initGrids (Source/Scheme/Scheme.inc.h:2962-3165).
  */
 int yee_set_material (yee_handle *h, int32_t which, const double *values);
+
+/*
+ * Extent of the material grids along x, y, z: the field grid's size, or twice it along each axis
+ * of the scheme with YEE_DOUBLE_MATERIAL_PRECISION (1 along an axis the scheme lacks).  Replaces
+ * YeeGridLayout::getEpsSize (Source/Layout/YeeGridLayout.h:144, sizeEps of YeeGridLayout.cpp:1972).
+ * yee_set_material* and yee_get_material take [x][y][z] arrays of this size.
+ */
+int yee_material_size (yee_handle *h, int64_t out[3]);
 
 /* Same as yee_set_material from an FP32 array (the reference's -DFLOAT_VALUES grids). */
 int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values);
@@ -150,7 +167,9 @@ int yee_set_material_f32 (yee_handle *h, int32_t which, const float *values);
  * approximateSphereStair (:730-764) of `value` inside and `outside` elsewhere.  `center` is in
  * cell indices (the reference adds 0.5, the eps-point offset); `radius` in cells.  The value
  * replaces the whole grid, as in the reference (a sphere overrides a loaded file).  Omega values
- * are in rad/s: the reference passes --omegape-sphere * 2*pi*f here.
+ * are in rad/s: the reference passes --omegape-sphere * 2*pi*f here.  With
+ * YEE_DOUBLE_MATERIAL_PRECISION the sphere is drawn on the fine grid with centre 2 center + 0.5
+ * and radius 2 radius (Scheme.inc.h:2992-3009).
  */
 int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double outside, const int64_t center[3],
                              double radius, int32_t accuracy, int32_t stair);

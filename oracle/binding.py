@@ -110,12 +110,14 @@ def oracle_objective(cfg: Config, eps: np.ndarray, monitor_z: int, monitor_lo, m
 
 
 def sphere_map(cfg: Config, value: float, outside: float, center, radius: int) -> np.ndarray:
-    """Material map of a sphere as Scheme::initGrids builds it (Scheme.inc.h:2983-3012)."""
+    """Material map of a sphere as Scheme::initGrids builds it (Scheme.inc.h:2983-3012); with
+    --use-double-material-precision on the fine grid, centre 2 c + 0.5 and radius 2 r."""
     L = lib()
-    shape = cfg.shape
+    shape = cfg.material_shape
+    mod = 2.0 if cfg.double_material else 1.0
     out = np.empty(shape, dtype=np.float64)
-    cx, cy, cz = (float(center[0]) + 0.5, float(center[1]) + 0.5, float(center[2]) + 0.5)
-    r = float(radius)
+    cx, cy, cz = (float(center[0]) * mod + 0.5, float(center[1]) * mod + 0.5, float(center[2]) * mod + 0.5)
+    r = float(radius) * mod
     it = np.ndindex(*shape)
     for idx in it:
         if cfg.dim == 3:
@@ -145,6 +147,7 @@ class Oracle:
         if not self.h:
             raise ValueError("oracle_create rejected the configuration")
         self.shape = cfg.shape
+        self.material_shape = cfg.material_shape
 
     def close(self):
         if self.h:
@@ -166,7 +169,7 @@ class Oracle:
         lib().oracle_set_source_gaussian(self.h, float(t0), float(sigma))
 
     def set_material(self, name: str, values: np.ndarray) -> None:
-        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.shape)
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.material_shape)
         rc = lib().oracle_set_material(self.h, MAT_INDEX[name], arr.ctypes.data)
         if rc:
             raise ValueError(name)
@@ -185,7 +188,7 @@ class Oracle:
         return out
 
     def material(self, which: int) -> np.ndarray:
-        out = np.empty(self.shape, dtype=np.float64)
+        out = np.empty(self.material_shape, dtype=np.float64)
         if lib().oracle_get_material(self.h, which, out.ctypes.data):
             raise KeyError(which)
         return out

@@ -161,6 +161,30 @@ class RefRun
     return missing (axis) ? 1 : tcSlot (size, slot (axis));
   }
 
+  /* material grid extent along physical axis a: sizeEps, twice the field grid with
+   * --use-double-material-precision (YeeGridLayout.cpp:1972) */
+  grid_coord msz (int axis) const
+  {
+    return missing (axis) ? 1 : sz (axis) * (layout->getIsDoubleMaterialPrecision () ? 2 : 1);
+  }
+
+  /* every cell of a material grid */
+  template <class F>
+  void forAllMat (F f)
+  {
+    for (grid_coord i = 0; i < msz (0); ++i)
+      for (grid_coord j = 0; j < msz (1); ++j)
+        for (grid_coord k = 0; k < msz (2); ++k)
+          f (coordOf (i, j, k));
+  }
+
+  std::vector<double> dumpMat (Grid<TC> *g)
+  {
+    std::vector<double> v;
+    forAllMat ([&] (TC p) { v.push_back (partOf (*g->getFieldValue (p, 0), 0)); });
+    return v;
+  }
+
   template <class F>
   void forAll (grid_coord lo, grid_coord hiOff, F f)
   {
@@ -189,31 +213,33 @@ class RefRun
     {
       return;
     }
-    size_t n = (size_t) sz (0) * sz (1) * sz (2);
+    size_t n = (size_t) msz (0) * msz (1) * msz (2);
     std::vector<FPValue> raw;
     if (!readRaw (file, raw, n))
     {
       exit (2);
     }
     size_t idx = 0;
-    forAll (0, 0, [&] (TC p) { g->setFieldValue (FIELDVALUE (raw[idx++], FPValue (0)), p, 0); });
+    forAllMat ([&] (TC p) { g->setFieldValue (FIELDVALUE (raw[idx++], FPValue (0)), p, 0); });
   }
 
-  /* Sphere of `value` (outside `outside`) on the material grid; follows Scheme::initGrids (Scheme.inc.h:2983-3012). */
+  /* Sphere of `value` (outside `outside`) on the material grid; follows Scheme::initGrids
+   * (Scheme.inc.h:2983-3012), modifier 2 with double material precision. */
   void sphere (Grid<TC> *g, FPValue value, FPValue outside, grid_coord cx, grid_coord cy, grid_coord cz, int radius)
   {
-    TCFP center = TCFP::initAxesCoordinate ((FPValue) cx, (FPValue) cy, (FPValue) cz, ct1, ct2, ct3) * FPValue (1)
+    const FPValue modifier = layout->getIsDoubleMaterialPrecision () ? 2 : 1;
+    TCFP center = TCFP::initAxesCoordinate ((FPValue) cx, (FPValue) cy, (FPValue) cz, ct1, ct2, ct3) * modifier
                   + TCFP::initAxesCoordinate (0.5, 0.5, 0.5, ct1, ct2, ct3);
-    forAll (0, 0, [&] (TC p) {
+    forAllMat ([&] (TC p) {
       TCFP posAbs = layout->getEpsCoordFP (p);
       FieldValue v;
       if (SOLVER_SETTINGS.getDoUseStairApproximation ())
       {
-        v = Approximation::approximateSphereStair (posAbs, center, radius * FPValue (1), value, outside);
+        v = Approximation::approximateSphereStair (posAbs, center, radius * modifier, value, outside);
       }
       else
       {
-        v = Approximation::approximateSphereAccurate (posAbs, center, radius * FPValue (1), value, outside);
+        v = Approximation::approximateSphereAccurate (posAbs, center, radius * modifier, value, outside);
       }
       g->setFieldValue (v, p, 0);
     });
@@ -232,11 +258,13 @@ class RefRun
     {
       if (missing (axis)) continue;
       Grid<TC> *s = axis == 0 ? is->getSigmaX () : (axis == 1 ? is->getSigmaY () : is->getSigmaZ ());
-      FPValue P = FPValue (tcSlot (pml, slot (axis)));
+      /* the sigma grid is a material grid: fine, with PML size 2 P, under double material
+       * precision (SchemeHelper.h:598, 640, 682) */
+      FPValue P = FPValue (tcSlot (pml, slot (axis)) * (layout->getIsDoubleMaterialPrecision () ? 2 : 1));
       FPValue boundary = P * dx;
       const bool first = slot (axis) == 0;
-      FPValue n = first ? FPValue (sz (axis)) : (FPValue) sz (axis) + FPValue (0.5);
-      forAll (0, 0, [&] (TC p) {
+      FPValue n = first ? FPValue (msz (axis)) : (FPValue) msz (axis) + FPValue (0.5);
+      forAllMat ([&] (TC p) {
         FPValue y = fpSlot (layout->getEpsCoordFP (p), slot (axis));
         FPValue val = 0;
         if (y < P)
@@ -627,13 +655,13 @@ public:
     DUMP_IF (meta && is->getDoNeedHy (), "B1y", getB1y)
     DUMP_IF (meta && is->getDoNeedHz (), "B1z", getB1z)
 #undef DUMP_IF
-    putRecord ("Eps", dumpGrid (is->getEps (), 0));
-    putRecord ("Mu", dumpGrid (is->getMu (), 0));
+    putRecord ("Eps", dumpMat (is->getEps ()));
+    putRecord ("Mu", dumpMat (is->getMu ()));
     if (pml)
     {
-      if (!missing (0)) putRecord ("SigmaX", dumpGrid (is->getSigmaX (), 0));
-      if (!missing (1)) putRecord ("SigmaY", dumpGrid (is->getSigmaY (), 0));
-      if (!missing (2)) putRecord ("SigmaZ", dumpGrid (is->getSigmaZ (), 0));
+      if (!missing (0)) putRecord ("SigmaX", dumpMat (is->getSigmaX ()));
+      if (!missing (1)) putRecord ("SigmaY", dumpMat (is->getSigmaY ()));
+      if (!missing (2)) putRecord ("SigmaZ", dumpMat (is->getSigmaZ ()));
     }
     if (meta)
     {
@@ -754,6 +782,11 @@ int main (int argc, char **argv)
   if (SOLVER_SETTINGS.getDoUseCaCbPMLMetaGrids ())
   {
     fprintf (stderr, "--use-ca-cb-pml-metamaterials: the reference never binds its a1 / a2 grids\n");
+    return 3;
+  }
+  if (SOLVER_SETTINGS.getDoUseDoubleMaterialPrecision () && SOLVER_SETTINGS.getDoUseMetamaterials ())
+  {
+    fprintf (stderr, "--use-double-material-precision with metamaterials: the 8-cell Drude average is UNREACHABLE\n");
     return 3;
   }
   const char *outName = getenv ("YEE_REF_OUT");

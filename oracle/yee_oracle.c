@@ -82,6 +82,8 @@ struct oracle
   double src_phase;        /* test hook: added to every source argument (0 = reference) */
   int src_kind;            /* 0 the real build; 1 / 2 the real / imaginary part of the complex build's sources */
   double src_t0, src_sigma; /* Gaussian envelope of the point source in time steps (sigma <= 0: none) */
+  int fine;                /* --use-double-material-precision: material and sigma grids of extent mn */
+  int64_t mn[3], mcell;
 };
 
 static int64_t o_idx (const oracle *o, int64_t i, int64_t j, int64_t k)
@@ -120,7 +122,7 @@ static void o_eps_index (const oracle *o, const double real[3], int64_t out[3])
 
 static double o_mval (const oracle *o, const double *grid, const int64_t p[3])
 {
-  return grid[o_idx (o, p[0], p[1], p[2])];
+  return grid[(p[0] * o->mn[1] + p[1]) * o->mn[2] + p[2]];
 }
 
 /*
@@ -162,12 +164,81 @@ static int o_material_cells (const oracle *o, int comp, int64_t i, int64_t j, in
   return 4;
 }
 
-/* InternalScheme::getMaterial + Approximation::approximateMaterial (Approximation.cpp:36-47). */
+/*
+ * The 8 fine material cells of a field point with double material precision
+ * (YeeGridLayout::initMaterialCoordinates, Source/Layout/YeeGridLayout.h:866-1037, E_CENTERED,
+ * isDoubleMaterialPrecision branches), in the reference's order 11, 12, 21, 22, 31, 32, 41, 42.
+ * Each cell is getEpsCoord (realCoord + shift) * 2 + offset.  E components: the shift is -0.5
+ * (lo) or +0.5 (hi) along their own axis; H components: one of four transverse shifts c1..c4
+ * = (-,-), (-,+), (+,-), (+,+) on their two other axes in (x, y, z) order.  Shifts and
+ * offsets along an axis the scheme lacks are dropped (TC::initAxesCoordinate), so the cell
+ * index there is 0.
+ */
+static const int kFineBase[6][8] = {
+  { 0, 0, 1, 1, 0, 0, 1, 1 },   /* Ex: lo, lo, hi, hi, lo, lo, hi, hi */
+  { 0, 0, 1, 1, 0, 0, 1, 1 },   /* Ey */
+  { 0, 0, 0, 0, 1, 1, 1, 1 },   /* Ez */
+  { 0, 0, 1, 1, 2, 2, 3, 3 },   /* Hx: c1, c1, c2, c2, c3, c3, c4, c4 */
+  { 0, 0, 1, 1, 2, 2, 3, 3 },   /* Hy */
+  { 0, 0, 1, 1, 2, 2, 3, 3 },   /* Hz */
+};
+static const int kFineOff[6][8][3] = {
+  { { 1, 0, 0 }, { 1, 1, 0 }, { 0, 0, 0 }, { 0, 1, 0 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 0, 1 }, { 0, 1, 1 } },
+  { { 0, 1, 0 }, { 1, 1, 0 }, { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 1 }, { 1, 1, 1 }, { 0, 0, 1 }, { 1, 0, 1 } },
+  { { 0, 0, 1 }, { 0, 1, 1 }, { 1, 0, 1 }, { 1, 1, 1 }, { 0, 0, 0 }, { 0, 1, 0 }, { 1, 0, 0 }, { 1, 1, 0 } },
+  { { 0, 1, 1 }, { 1, 1, 1 }, { 0, 1, 0 }, { 1, 1, 0 }, { 0, 0, 1 }, { 1, 0, 1 }, { 0, 0, 0 }, { 1, 0, 0 } },
+  { { 1, 0, 1 }, { 1, 1, 1 }, { 1, 0, 0 }, { 1, 1, 0 }, { 0, 0, 1 }, { 0, 1, 1 }, { 0, 0, 0 }, { 0, 1, 0 } },
+  { { 1, 1, 0 }, { 1, 1, 1 }, { 1, 0, 0 }, { 1, 0, 1 }, { 0, 1, 0 }, { 0, 1, 1 }, { 0, 0, 0 }, { 0, 0, 1 } },
+};
+
+static void o_material_cells_fine (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, int64_t cells[8][3])
+{
+  const comp_desc *c = &CD[comp];
+  double real[3];
+  o_real_coord (o, comp, i, j, k, real);
+  int64_t base[4][3];
+  if (!c->is_h)
+  {
+    for (int s = 0; s < 2; ++s)
+    {
+      double r[3] = { real[0], real[1], real[2] };
+      if (o->act[c->axis]) r[c->axis] += s ? 0.5 : -0.5;
+      o_eps_index (o, r, base[s]);
+    }
+  }
+  else
+  {
+    const int b = c->axis == 0 ? 1 : 0;
+    const int cc = c->axis == 2 ? 1 : 2;
+    for (int s = 0; s < 4; ++s)
+    {
+      double r[3] = { real[0], real[1], real[2] };
+      if (o->act[b]) r[b] += (s & 2) ? 0.5 : -0.5;
+      if (o->act[cc]) r[cc] += (s & 1) ? 0.5 : -0.5;
+      o_eps_index (o, r, base[s]);
+    }
+  }
+  for (int q = 0; q < 8; ++q)
+    for (int a = 0; a < 3; ++a)
+      cells[q][a] = o->act[a] ? base[kFineBase[comp][q]][a] * 2 + kFineOff[comp][q][a] : 0;
+}
+
+/* InternalScheme::getMaterial + Approximation::approximateMaterial (Approximation.cpp:36-54). */
 static double o_get_material (const oracle *o, int comp, int64_t i, int64_t j, int64_t k, const double *grid)
 {
   if (!grid)
   {
     return 0;
+  }
+  if (o->fine)
+  {
+    int64_t f[8][3];
+    o_material_cells_fine (o, comp, i, j, k, f);
+    double v[8];
+    for (int q = 0; q < 8; ++q) v[q] = o_mval (o, grid, f[q]);
+    double a = ((v[0] + v[1]) / 2.0 + (v[2] + v[3]) / 2.0) / 2.0;
+    double b = ((v[4] + v[5]) / 2.0 + (v[6] + v[7]) / 2.0) / 2.0;
+    return (a + b) / 2.0;
   }
   int64_t cells[4][3];
   int nc = o_material_cells (o, comp, i, j, k, cells);
@@ -550,6 +621,10 @@ oracle *oracle_create (const yee_config *cfg)
     o->hi[a] = o->act[a] ? o->n[a] - 1 : 1;
   }
   o->ncell = o->n[0] * o->n[1] * o->n[2];
+  /* --use-double-material-precision: sizeEps = 2 size along the scheme's axes (YeeGridLayout.cpp:1972) */
+  o->fine = (cfg->flags & YEE_DOUBLE_MATERIAL_PRECISION) != 0;
+  for (int a = 0; a < 3; ++a) o->mn[a] = (o->fine && o->act[a]) ? 2 * o->n[a] : o->n[a];
+  o->mcell = o->mn[0] * o->mn[1] * o->mn[2];
   o->use_pml = (cfg->flags & YEE_USE_PML) != 0;
   o->use_tfsf = (cfg->flags & YEE_USE_TFSF) != 0;
   o->use_meta = (cfg->flags & YEE_USE_METAMATERIALS) != 0;
@@ -557,6 +632,11 @@ oracle *oracle_create (const yee_config *cfg)
   {
     free (o);
     return NULL;   /* Scheme.inc.h:889-892: metamaterials without PML are unreachable */
+  }
+  if (o->use_meta && o->fine)
+  {
+    free (o);
+    return NULL;   /* the 8-cell Drude average is UNREACHABLE (Approximation.cpp:196) */
   }
   o->layers = (o->use_meta && o->use_pml) ? 3 : 2;
   for (int c = 0; c < 6; ++c)
@@ -577,9 +657,9 @@ oracle *oracle_create (const yee_config *cfg)
   }
   for (int m = 0; m < 6; ++m)
   {
-    o->mat[m] = (double *) malloc (sizeof (double) * o->ncell);
+    o->mat[m] = (double *) malloc (sizeof (double) * o->mcell);
     double v = m < 2 ? 1.0 : 0.0;
-    for (int64_t q = 0; q < o->ncell; ++q) o->mat[m][q] = v;
+    for (int64_t q = 0; q < o->mcell; ++q) o->mat[m][q] = v;
   }
   for (int c = 0; c < 6; ++c)
   {
@@ -672,7 +752,7 @@ void oracle_destroy (oracle *o)
 int oracle_set_material (oracle *o, int32_t which, const double *values)
 {
   if (!o || !values || which < 0 || which > 5) return -1;
-  memcpy (o->mat[which], values, sizeof (double) * o->ncell);
+  memcpy (o->mat[which], values, sizeof (double) * o->mcell);
   return 0;
 }
 
@@ -687,12 +767,13 @@ int oracle_prepare (oracle *o)
     for (int a = 0; a < 3; ++a)
     {
       if (!o->act[a]) continue;
-      o->sigma[a] = (double *) calloc (o->ncell, sizeof (double));
-      double P = (double) o->cfg.pml[a];
+      /* double material precision: the fine grid, PML size 2 P (SchemeHelper.h:598, 640, 682) */
+      o->sigma[a] = (double *) calloc (o->mcell, sizeof (double));
+      double P = (double) (o->cfg.pml[a] * (o->fine ? 2 : 1));
       double boundary = P * o->dx;
-      for (int64_t i = 0; i < o->n[0]; ++i)
-        for (int64_t j = 0; j < o->n[1]; ++j)
-          for (int64_t k = 0; k < o->n[2]; ++k)
+      for (int64_t i = 0; i < o->mn[0]; ++i)
+        for (int64_t j = 0; j < o->mn[1]; ++j)
+          for (int64_t k = 0; k < o->mn[2]; ++k)
           {
             int64_t p[3] = { i, j, k };
             double pos = (double) p[a] + 0.5;   /* getEpsCoordFP */
@@ -701,17 +782,17 @@ int oracle_prepare (oracle *o)
              * initSigmaY / Z on the others */
             if (a == first)
             {
-              double size = (double) o->n[a];
+              double size = (double) o->mn[a];
               if (pos < P) val = o_sigma_value (P - pos, boundary, o->dx);
               else if (pos >= size - P) val = o_sigma_value (pos - (size - P), boundary, o->dx);
             }
             else
             {
-              double size = (double) o->n[a] + 0.5;
+              double size = (double) o->mn[a] + 0.5;
               if (pos < P) val = o_sigma_value ((double) (int64_t) (P - pos), boundary, o->dx);
               else if (pos >= size - P) val = o_sigma_value ((double) (int64_t) (pos - (size - P)), boundary, o->dx);
             }
-            o->sigma[a][o_idx (o, i, j, k)] = val;
+            o->sigma[a][(i * o->mn[1] + j) * o->mn[2] + k] = val;
           }
     }
   }
@@ -1027,12 +1108,12 @@ int oracle_get_material (oracle *o, int32_t which, double *out)
   if (!o || !out) return -1;
   if (which >= 0 && which < 6)
   {
-    memcpy (out, o->mat[which], sizeof (double) * o->ncell);
+    memcpy (out, o->mat[which], sizeof (double) * o->mcell);
     return 0;
   }
   if (which >= 6 && which < 9 && o->sigma[which - 6])
   {
-    memcpy (out, o->sigma[which - 6], sizeof (double) * o->ncell);
+    memcpy (out, o->sigma[which - 6], sizeof (double) * o->mcell);
     return 0;
   }
   return -1;
@@ -1221,7 +1302,8 @@ static void o_scatter (const oracle *o, int comp, int64_t i, int64_t j, int64_t 
 
 int oracle_adjoint (oracle *o, const oracle_adj_spec *sp, int32_t T, double *J_out, double *grad)
 {
-  if (!o || !sp || !J_out || !grad || !o->prepared || o->t != 0 || o->dim != 3 || o->use_meta || T < 2) return -1;
+  if (!o || !sp || !J_out || !grad || !o->prepared || o->t != 0 || o->dim != 3 || o->use_meta || o->fine || T < 2)
+    return -1;
   if (sp->window < 1 || sp->window > T - 1) return -1;
   const int64_t nc = o->ncell;
   const int nstate = o->use_pml ? 12 : 6;

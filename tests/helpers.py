@@ -119,3 +119,40 @@ def check_fp32(label, names, eng_field, ref_field, tol=1e-5):
                 assert own <= tol, f"{label}/{n}: {own} (norm share {share})"
         print(f"[gpu] {label} FP32 {group} vector {vec:.2e}: " + ", ".join(parts), flush=True)
         assert vec <= tol, f"{label}/{group}: {vec}"
+
+
+DOUBLE_MATERIAL_CASES = [
+    # random eps and mu grids on the fine grid (pins the order of the nested 8-cell averages),
+    # PML and oblique TF/SF
+    ("--3d --size x:13,y:12,z:14 --use-pml --pml-size x:3,y:2,z:3 --use-tfsf --tfsf-size-left x:4,y:4,z:4 "
+     "--tfsf-size-right x:4,y:4,z:5 --angle-teta 30 --angle-phi 45 --angle-psi 90 --point-source x:6,y:6,z:7 "
+     "--point-source-hz --time-steps 25", ("eps", "mu")),
+    # accurate eps sphere and stair mu sphere on the fine grid
+    ("--3d --size x:14,y:13,z:12 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:4 "
+     "--tfsf-size-right x:4 --same-size-tfsf --angle-teta 60 --angle-phi 20 --angle-psi 30 --eps-sphere 3 "
+     "--eps-sphere-center x:7,y:6,z:6 --eps-sphere-radius 3 --sphere-accuracy 20 --time-steps 25", ()),
+    ("--3d --size x:12,y:14,z:13 --use-tfsf --tfsf-size-left x:3 --tfsf-size-right x:3 --same-size-tfsf "
+     "--angle-teta 90 --angle-phi 45 --angle-psi 0 --mu-sphere 2 --mu-sphere-center x:6,y:7,z:6 "
+     "--mu-sphere-radius 3 --stair-sphere-approx --current-source x:5,y:6,z:6 --current-source-jz --time-steps 25",
+     ()),
+    # the 2D schemes: TMz / TEz with kernels of their own, TMx / TEy on the 3D kernels
+    ("--2d-tmz --size x:26,y:24 --use-pml --pml-size x:4,y:3 --use-tfsf --tfsf-size-left x:7,y:6 "
+     "--tfsf-size-right x:6,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 90 --point-source x:11,y:10 "
+     "--point-source-ez --time-steps 40", ("eps", "mu")),
+    ("--2d-tez --size x:24,y:26 --use-pml --pml-size x:3,y:4 --use-tfsf --tfsf-size-left x:6,y:7 "
+     "--tfsf-size-right x:7,y:6 --angle-teta 90 --angle-phi 30 --angle-psi 0 --eps-sphere 3 "
+     "--eps-sphere-center x:12,y:13 --eps-sphere-radius 4 --sphere-accuracy 20 --time-steps 40", ("mu",)),
+    ("--2d-tmx --size x:6,y:24,z:22 --use-pml --pml-size y:3,z:3 --use-tfsf --tfsf-size-left y:6,z:6 "
+     "--tfsf-size-right y:6,z:6 --angle-teta 35 --angle-phi 90 --angle-psi 0 --time-steps 40", ("eps", "mu")),
+    ("--2d-tey --size x:23,y:5,z:25 --use-pml --pml-size x:3,z:4 --point-source x:11,y:0,z:12 "
+     "--point-source-hy --time-steps 40", ("eps", "mu")),
+    # a 1D line
+    ("--1d-ezhy --size x:60 --use-pml --pml-size x:7 --use-tfsf --tfsf-size-left x:15 --tfsf-size-right x:14 "
+     "--angle-teta 90 --angle-phi 0 --angle-psi 90 --time-steps 70", ("eps", "mu")),
+]
+
+
+def random_material(cfg, seed, lo=1.0, hi=4.0):
+    """A random material grid of the configuration's material shape."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, size=cfg.material_shape)

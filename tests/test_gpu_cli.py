@@ -406,3 +406,47 @@ def test_cli_text_dump_matches_reference_txtdumper_byte_for_byte(engine_lib, tmp
             got = fh.read()
         assert len(want) > 1000
         assert got == want, (comp, got[:200], want[:200])
+
+
+def test_cli_double_material_precision(engine_lib, tmp_path):
+    """--use-double-material-precision on the command line: material files hold the fine grid
+    (twice the field grid along each axis), spheres are drawn on it, --save-materials dumps it
+    (the middle plane of the fine grid, Scheme::initGrids takes the material grid's own size),
+    and the fields equal the engine's run of the same configuration."""
+    from fdtd3d_amd.config import parse_args
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    base = ("--3d --size x:14,y:12,z:16 --use-pml --pml-size x:3 --same-size-pml --use-tfsf --tfsf-size-left x:4 "
+            "--tfsf-size-right x:4 --same-size-tfsf --angle-teta 30 --angle-phi 45 --angle-psi 90 --mu-sphere 2 "
+            "--mu-sphere-center x:7,y:6,z:8 --mu-sphere-radius 3 --sphere-accuracy 10 --time-steps 30 "
+            "--use-double-material-precision")
+    cfg = parse_args(base.split())
+    fine = cfg.material_shape
+    assert fine == (28, 24, 32)
+    eps = np.random.default_rng(5).uniform(1.0, 5.0, size=fine)
+    _write_dat(tmp_path / "eps.dat", eps, np.float64)
+    say("[gpu] cli double material precision")
+    out = subprocess.run([exe] + (base + " --load-eps-from-file eps.dat --save-materials --save-res --save-as-dat").split(),
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    m_eps = np.fromfile(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Eps].dat", dtype=np.float64)
+    np.testing.assert_array_equal(m_eps, eps[:, :, fine[2] // 2].ravel())
+    eng = Engine(cfg)
+    eng.set_material("eps", eps)
+    eng.set_material_sphere("mu", cfg.mu_sphere, 1.0, cfg.mu_sphere_center, cfg.mu_sphere_radius,
+                            accuracy=cfg.sphere_accuracy)
+    m_mu = np.fromfile(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Mu].dat", dtype=np.float64)
+    np.testing.assert_array_equal(m_mu, eng.material("mu")[:, :, fine[2] // 2].ravel())
+    eng.prepare()
+    eng.step(cfg.time_steps)
+    eng.sync()
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        got = np.fromfile(tmp_path / f"previous-1_[timestep=30]_[pid=0]_[name={name}].dat", dtype=np.float64)
+        np.testing.assert_array_equal(got, eng.field(name)[:, :, 8].ravel(), err_msg=name)
+    eng.close()
+    # a file of the field grid's size is one eighth of what the fine grid needs
+    _write_dat(tmp_path / "short.dat", eps[::2, ::2, ::2], np.float64)
+    out = subprocess.run([exe] + (base + " --load-eps-from-file short.dat").split(), capture_output=True, text=True,
+                         timeout=120, cwd=tmp_path)
+    assert out.returncode != 0

@@ -10,7 +10,7 @@ import pytest
 
 from fdtd3d_amd.config import parse_args
 from oracle.binding import Oracle, ref_driver_path, run_ref_driver, sphere_map
-from tests.helpers import field_names
+from tests.helpers import DOUBLE_MATERIAL_CASES, field_names, random_material
 
 pytestmark = pytest.mark.skipif(not os.path.exists(ref_driver_path("f64")), reason="reference driver not built")
 
@@ -210,3 +210,55 @@ def test_oracle_complex_parts_match_reference_complex_build(args, tmp_path):
         assert np.array_equal(im_.field(n).ravel(), rec[n + ".im"]), (n, "imag")
         if n[0] in "EH" and len(n) == 2:
             assert np.abs(rec[n + ".im"]).max() > 0 or np.abs(rec[n]).max() == 0, n
+
+
+@pytest.mark.parametrize("args,files", DOUBLE_MATERIAL_CASES)
+def test_oracle_double_material_precision_matches_reference_driver(args, files, tmp_path):
+    """--use-double-material-precision: material and sigma grids of twice the field grid's
+    extent, each field point averaging 8 fine cells in the reference's nested order, spheres
+    and PML profiles drawn on the fine grid.  The oracle's fields (and the material / sigma
+    grids) equal the reference driver's bit for bit; random material grids make every
+    reordering of the nested averages visible."""
+    argv = (args + " --use-double-material-precision").split()
+    cfg = parse_args(argv)
+    assert cfg.double_material
+    mats = {}
+    for s, name in enumerate(files):
+        mats[name] = random_material(cfg, 11 + s)
+        path = tmp_path / f"{name}.raw"
+        mats[name].astype(np.float64).tofile(path)
+        argv += [f"--load-{name}-from-file", str(path)]
+    rec, _ = run_ref_driver(argv, str(tmp_path / "r.bin"))
+    o = Oracle(cfg)
+    if cfg.eps_sphere != 1.0:
+        mats["eps"] = sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius)
+    if cfg.mu_sphere != 1.0:
+        mats["mu"] = sphere_map(cfg, cfg.mu_sphere, 1.0, cfg.mu_sphere_center, cfg.mu_sphere_radius)
+    for name, arr in mats.items():
+        o.set_material(name, arr)
+    o.prepare()
+    assert np.array_equal(o.material(0).ravel(), rec["Eps"])
+    assert np.array_equal(o.material(1).ravel(), rec["Mu"])
+    if cfg.use_pml:
+        for a, key in enumerate(("SigmaX", "SigmaY", "SigmaZ")):
+            if key in rec:
+                assert np.array_equal(o.material(6 + a).ravel(), rec[key]), key
+    o.step(cfg.time_steps)
+    names = field_names(cfg, aux=True)
+    for name in names:
+        assert np.array_equal(o.field(name).ravel(), rec[name]), name
+    assert max(float(np.abs(rec[n]).max()) for n in names) > 0
+
+
+def test_double_material_precision_differs_from_the_coarse_average(tmp_path):
+    """The fine grid changes the answer: a sphere drawn on it (centre 2 c + 0.5, radius 2 r) and
+    averaged over 8 cells gives other fields than the coarse 2- / 4-cell averages, in the
+    reference as in the oracle."""
+    args = ("--3d --size x:12,y:12,z:12 --use-tfsf --tfsf-size-left x:3 --tfsf-size-right x:3 --same-size-tfsf "
+            "--eps-sphere 4 --eps-sphere-center x:6,y:6,z:6 --eps-sphere-radius 3 --sphere-accuracy 10 "
+            "--time-steps 20").split()
+    coarse, _ = run_ref_driver(args, str(tmp_path / "c.bin"))
+    fine, _ = run_ref_driver(args + ["--use-double-material-precision"], str(tmp_path / "f.bin"))
+    assert fine["Eps"].size == 8 * coarse["Eps"].size
+    d = np.linalg.norm(fine["Ey"] - coarse["Ey"]) / np.linalg.norm(coarse["Ey"])
+    assert 1e-3 < d < 0.5, d
