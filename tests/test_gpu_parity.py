@@ -325,16 +325,19 @@ SPLIT_VS_FUSED = [
 
 @pytest.mark.parametrize("args", SPLIT_VS_FUSED)
 @pytest.mark.parametrize("precision", [8, 4])
-def test_fused_step_matches_half_steps(args, precision, engine_lib, monkeypatch):
+@pytest.mark.parametrize("block", [4, 1])
+def test_fused_step_matches_half_steps(args, precision, block, engine_lib, monkeypatch):
     """The fused E+H kernel (PML-free box) and the half-step kernels compute the same
-    expressions in the same order: results, D / B included, must be bit-identical."""
+    expressions in the same order: results, D / B included, must be bit-identical.  eps is
+    constant on 4^3 blocks or random per cell (block 1: every z+1 / y+1 / x+1 average of the
+    kernel's eps exchange differs from its neighbour's)."""
     cfg = parse_args(args.split())
     cfg.precision = precision
     rng = np.random.default_rng(7)
-    nb = [(s + 3) // 4 for s in cfg.shape]
-    eps = np.kron(rng.uniform(1.0, 6.0, size=nb), np.ones((4, 4, 4)))[: cfg.shape[0], : cfg.shape[1], : cfg.shape[2]]
+    nb = [(s + block - 1) // block for s in cfg.shape]
+    eps = np.kron(rng.uniform(1.0, 6.0, size=nb), np.ones((block,) * 3))[: cfg.shape[0], : cfg.shape[1], : cfg.shape[2]]
     mats = {"eps": eps}
-    say(f"[gpu] split vs fused precision {precision}: {args[:60]}")
+    say(f"[gpu] split vs fused precision {precision} eps block {block}: {args[:60]}")
     monkeypatch.setenv("YEE_KERNEL", "split")
     a = run_engine(cfg, mats)
     monkeypatch.setenv("YEE_KERNEL", "fused")
