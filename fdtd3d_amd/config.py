@@ -12,6 +12,7 @@ the reference build.
 from __future__ import annotations
 
 import ctypes
+import re
 import struct
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -42,6 +43,8 @@ USE_TFSF = 0x2
 USE_METAMATERIALS = 0x4
 COMPLEX_FIELDS = 0x8
 DOUBLE_MATERIAL_PRECISION = 0x10
+# analytic test problems (include/yee.h YEE_FN_*), in the priority order of Scheme::initCallBacks
+TEST_FUNCTIONS = {"polinom1": 1, "polinom2": 2, "polinom3": 3, "sin1": 4}
 
 COMPONENTS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz",
               "Dx", "Dy", "Dz", "Bx", "By", "Bz",
@@ -103,6 +106,17 @@ class Config:
     complex_fields: bool = False
     # --use-double-material-precision: material grids of twice the field grid's extent per axis
     double_material: bool = False
+    # analytic test problems (Settings.inc.h:134-178): kinds (TEST_FUNCTIONS values, 0 = none) of
+    # the start values, border values, right side and exact solution; the norm box
+    start_fn: int = 0
+    border_fn: int = 0
+    rhs_fn: int = 0
+    exact_fn: int = 0
+    norm_start: List[int] = field(default_factory=lambda: [0, 0, 0])
+    norm_end: List[int] = field(default_factory=lambda: [0, 0, 0])
+    # --eps-normed / --mu-normed: eps = 1 / eps0 / mu = 1 / mu0 everywhere (Scheme.inc.h:3016-3025, 3066-3075)
+    eps_normed: bool = False
+    mu_normed: bool = False
     point_source: List[str] = field(default_factory=list)   # component names, e.g. ["Ez"]
     point_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
     # --current-source-j<x|y|z> / -m<x|y|z> as component names ("Ex" for Jx, "Hx" for Mx), --current-source
@@ -227,7 +241,7 @@ def _parse_coord(text: str, default: int) -> List[int]:
 # options that take one argument and are accepted but do not affect the Yee path
 _IGNORED_WITH_ARG = {
     "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size",
-    "--cuda-buffer-size", "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end",
+    "--cuda-buffer-size", "--cuda-gpus", "--num-cuda-threads",
     "--ntff-filename", "--ntff-step-angle", "--interm-ntff-step", "--interm-save-step",
     "--save-start-coord", "--save-end-coord", "--rebalance-step", "--stored-steps",
     "--save-cmd-to-file",
@@ -246,13 +260,18 @@ _IGNORED_FLAGS = {
 }
 # options of the reference whose feature is outside this engine: rejected loudly
 _UNSUPPORTED = {
-    "--eps-normed", "--mu-normed",
     # the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
     # (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
     "--use-ca-cb-pml-metamaterials",
     # the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
     "--use-ntff",
 }
+
+
+# --use-<kind>-start-values / -border-condition / -right-side, --calc-<kind>-diff-norm
+# (Settings.inc.h:134-153; sin1 has no right side)
+_TEST_FN = re.compile(r"^--(use|calc)-(polinom1|polinom2|polinom3|sin1)-"
+                      r"(start-values|border-condition|right-side|diff-norm)$")
 
 
 def _read_cmd_file(path: str) -> List[str]:
@@ -379,6 +398,23 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             cfg.stair_sphere = True
         elif a == "--use-double-material-precision":
             cfg.double_material = True
+        elif _TEST_FN.match(a) and ((_TEST_FN.match(a).group(1) == "calc") == (_TEST_FN.match(a).group(3) == "diff-norm")) \
+                and a != "--use-sin1-right-side":
+            m = _TEST_FN.match(a)
+            kind = TEST_FUNCTIONS[m.group(2)]
+            slot = {"start-values": "start_fn", "border-condition": "border_fn", "right-side": "rhs_fn",
+                    "diff-norm": "exact_fn"}[m.group(3)]
+            # Scheme::initCallBacks takes the first kind of polinom1, polinom2, polinom3, sin1
+            cur = getattr(cfg, slot)
+            setattr(cfg, slot, kind if cur == 0 else min(cur, kind))
+        elif a == "--norm-start":
+            cfg.norm_start = _parse_coord(val(), 0)
+        elif a == "--norm-end":
+            cfg.norm_end = _parse_coord(val(), 0)
+        elif a == "--eps-normed":
+            cfg.eps_normed = True
+        elif a == "--mu-normed":
+            cfg.mu_normed = True
         elif a in _IGNORED_WITH_ARG:
             val()
         elif a in _IGNORED_FLAGS:

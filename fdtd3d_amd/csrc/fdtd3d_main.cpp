@@ -18,6 +18,7 @@
 // Built twice: fdtd3d (FP64 fields, the reference's default value type) and fdtd3d-f32.
 #include <algorithm>
 #include <chrono>
+#include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -64,7 +65,31 @@ struct Options
   Sphere sphere[4] = { { 1.0 }, { 1.0 }, { 0.0 }, { 0.0 } };   // eps, mu, omegape, omegapm
   int accuracy = 100;
   bool stair = false;
+  // analytic test problem: YEE_FN_* kinds of the start values, border values, right side and
+  // exact solution (Settings.inc.h:134-153), the norm box (--norm-start / --norm-end), and
+  // --eps-normed / --mu-normed
+  int fn[4] = { 0, 0, 0, 0 };
+  int64_t norm_start[3] = { 0, 0, 0 }, norm_end[3] = { 0, 0, 0 };
+  bool eps_normed = false, mu_normed = false;
 };
+
+// --use-<kind>-start-values / -border-condition / -right-side, --calc-<kind>-diff-norm: the slot
+// (0 start, 1 border, 2 right side, 3 exact) and kind, or -1 for another option.  sin1 has no
+// right side.
+int test_flag (const std::string &a, int &kind)
+{
+  static const char *kinds[4] = { "polinom1", "polinom2", "polinom3", "sin1" };
+  for (int k = 0; k < 4; ++k)
+  {
+    const std::string n = kinds[k];
+    kind = k + 1;
+    if (a == "--use-" + n + "-start-values") return 0;
+    if (a == "--use-" + n + "-border-condition") return 1;
+    if (k < 3 && a == "--use-" + n + "-right-side") return 2;
+    if (a == "--calc-" + n + "-diff-norm") return 3;
+  }
+  return -1;
+}
 
 [[noreturn]] void die (const std::string &msg)
 {
@@ -117,7 +142,7 @@ std::vector<std::string> read_cmd_file (const std::string &path)
 
 const std::set<std::string> kIgnoredWithArg = {
   "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size", "--cuda-buffer-size",
-  "--cuda-gpus", "--num-cuda-threads", "--norm-start", "--norm-end", "--ntff-filename", "--ntff-step-angle",
+  "--cuda-gpus", "--num-cuda-threads", "--ntff-filename", "--ntff-step-angle",
   "--interm-ntff-step", "--rebalance-step", "--stored-steps", "--save-cmd-to-file",
 };
 const std::set<std::string> kIgnoredFlags = {
@@ -129,7 +154,6 @@ const std::set<std::string> kIgnoredFlags = {
   "--use-ca-cb", "--use-ca-cb-pml",
 };
 const std::set<std::string> kUnsupported = {
-  "--eps-normed", "--mu-normed",
   // the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
   // (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
   "--use-ca-cb-pml-metamaterials",
@@ -220,6 +244,15 @@ Options parse (int argc, char **argv)
     else if (a == "--sphere-accuracy") o.accuracy = std::stoi (val ());
     else if (a == "--stair-sphere-approx") o.stair = true;
     else if (a == "--use-double-material-precision") o.cfg.flags |= YEE_DOUBLE_MATERIAL_PRECISION;
+    else if (a == "--norm-start") parse_coord (val (), 0, o.norm_start);
+    else if (a == "--norm-end") parse_coord (val (), 0, o.norm_end);
+    else if (a == "--eps-normed") o.eps_normed = true;
+    else if (a == "--mu-normed") o.mu_normed = true;
+    else if (int kind = 0, slot = test_flag (a, kind); slot >= 0)
+    {
+      // Scheme::initCallBacks takes the first of polinom1, polinom2, polinom3, sin1
+      o.fn[slot] = o.fn[slot] == 0 ? kind : std::min (o.fn[slot], kind);
+    }
     else if (a == "--save-res") o.save.res = true;
     else if (a == "--save-interm-res") o.save.interm = true;
     else if (a == "--interm-save-step") o.save.interm_step = std::stoll (val ());
@@ -600,6 +633,8 @@ int main (int argc, char **argv)
 
   yee_handle *h = nullptr;
   check (yee_create (&c, &h));
+  if (o.fn[0] || o.fn[1] || o.fn[2] || o.fn[3])
+    check (yee_set_test_functions (h, o.fn[0], o.fn[1], o.fn[2], o.fn[3], o.norm_start, o.norm_end));
   double sc[3];
   check (yee_get_scalars (h, sc));
   const double two_pi_f = 2 * 3.14159265358979323846 * sc[1];
@@ -628,6 +663,16 @@ int main (int argc, char **argv)
       }
     }
   }
+  // --eps-normed / --mu-normed: 1 / eps0, 1 / mu0 everywhere, after files and spheres
+  // (Scheme.inc.h:3016-3025, 3066-3075)
+  {
+    const double cl = 299792458.0, mu0 = 4 * 3.14159265358979323846 * 0.0000001, eps0 = 1 / (mu0 * (cl * cl));
+    int64_t msize[3];
+    material_size (c, msize);
+    const size_t mc = (size_t) (msize[0] * msize[1] * msize[2]);
+    if (o.eps_normed) check (yee_set_material (h, YEE_MAT_EPS, std::vector<double> (mc, 1.0 / eps0).data ()));
+    if (o.mu_normed) check (yee_set_material (h, YEE_MAT_MU, std::vector<double> (mc, 1.0 / mu0).data ()));
+  }
   check (yee_prepare (h));
 
   const SaveOptions &sv = o.save;
@@ -645,7 +690,25 @@ int main (int argc, char **argv)
   if (dumping && ((sv.res && sv.scat_res) || (sv.interm && sv.scat_interm)) && !(c.flags & YEE_USE_TFSF))
     die ("scattered-field dumps need --use-tfsf");
   const auto t0 = std::chrono::steady_clock::now ();
-  if (dumping && sv.interm)
+  if (o.fn[3])
+  {
+    // one step at a time: after each, the "DIFF NORM" line of every component with an exact
+    // solution (Scheme.inc.h:1543-1549), E before H as the reference computes them
+    static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
+    for (int64_t done = 0; done < c.time_steps; ++done)
+    {
+      check (yee_step (h, 1));
+      for (int comp = 0; comp < 6; ++comp)
+      {
+        double nv[4];
+        if (yee_get_diff_norm (h, comp, nv) != YEE_OK) continue;
+        std::printf ("-> DIFF NORM %s. Timestep %" PRId64 ". Value = ( %.20f ) = ( %.20f %% )\n", names[comp], done, nv[0],
+                     nv[1]);
+      }
+      if (dumping && sv.interm && (done + 1) % sv.interm_step == 0) save_grids (o, h, done + 1, sv.scat_interm);
+    }
+  }
+  else if (dumping && sv.interm)
   {
     // SchemeHelper::performNSteps3D (SchemeHelper.h:100-118): after step t, save when
     // t / step < (t + 1) / step, as time step t + 1

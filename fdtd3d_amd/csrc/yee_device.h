@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/yee.h"
 #include "yee_kernels.h"
 
 namespace yee {
@@ -336,6 +337,89 @@ template <typename T>
 __device__ __forceinline__ T curl (const T v[4])
 {
   return v[1] - v[0] - v[3] + v[2];
+}
+
+// Analytic test problems of the reference's real build (Source/Scheme/CallBack.cpp:28-143;
+// SQR (a) = a * a, the constants of PhysicsConst.h:30-32), evaluated in FP64 with the
+// reference's operations: field values and the right-side currents J (E components) / M (H
+// components).  Components a kind does not define read as 0 (Scheme::initCallBacks sets none).
+__device__ __forceinline__ double test_field (int kind, int comp, double x, double y, double z, double t)
+{
+  const double c = 299792458.0, mu0 = 4 * 3.14159265358979323846 * 0.0000001, e0 = 1 / (mu0 * (c * c));
+  if (kind == YEE_FN_POLINOM2)
+  {
+    const int a = comp % 3;
+    const double u = a == 0 ? y : x, v = a == 2 ? y : z;
+    const double e = (c * c) * t * ((u * u) + (v * v));
+    return comp < 3 ? e : e0 * e;
+  }
+  if (comp != YEE_EZ && comp != YEE_HY) return 0.0;
+  if (kind == YEE_FN_POLINOM1)
+  {
+    const double ez = (c * c) * (x * x) * t;
+    return comp == YEE_EZ ? ez : e0 * ez;
+  }
+  if (kind == YEE_FN_POLINOM3)
+  {
+    const double ez = (c * c) * ((x * t) * (x * t));
+    return comp == YEE_EZ ? ez : e0 * ez;
+  }
+  if (kind == YEE_FN_SIN1) return comp == YEE_EZ ? c * sin (t - x) : -c * sin (t - x);
+  return 0.0;
+}
+
+__device__ __forceinline__ double test_rhs (int kind, int comp, double x, double y, double z, double t)
+{
+  const double c = 299792458.0, mu0 = 4 * 3.14159265358979323846 * 0.0000001, e0 = 1 / (mu0 * (c * c));
+  if (kind == YEE_FN_POLINOM2)
+  {
+    const double p[3] = { x, y, z };
+    const int a = comp % 3;
+    const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+    if (comp < 3)
+    {
+      // Jx: 2 t (z - y) + y^2 + z^2, Jy: 2 t (x - z) + x^2 + z^2, Jz: 2 t (y - x) + x^2 + y^2
+      const double d = a == 0 ? (p[2] - p[1]) : (a == 1 ? (p[0] - p[2]) : (p[1] - p[0]));
+      return -(c * c) * e0 * (2 * t * d + (p[u] * p[u]) + (p[v] * p[v]));
+    }
+    // Mx: 2 t (y - z), My: 2 t (z - x), Mz: 2 t (x - y)
+    const double d = a == 0 ? (p[1] - p[2]) : (a == 1 ? (p[2] - p[0]) : (p[0] - p[1]));
+    return -(c * c) * (2 * t * d + e0 * mu0 * ((p[u] * p[u]) + (p[v] * p[v])));
+  }
+  if (comp != YEE_EZ && comp != YEE_HY) return 0.0;
+  if (kind == YEE_FN_POLINOM1)
+  {
+    if (comp == YEE_EZ) return -(c * c) * e0 * (-2 * x * t + (x * x));
+    return -(c * c) * (-2 * x * t + (x * x) * e0 * mu0);
+  }
+  if (kind == YEE_FN_POLINOM3)
+  {
+    if (comp == YEE_EZ) return -(c * c) * 2 * e0 * x * t * (x - t);
+    return -(c * c) * 2 * x * t * (e0 * mu0 * x - t);
+  }
+  return 0.0;
+}
+
+// The real coordinate (m) of component COMP at cell p: (p + the component's minimum coordinate,
+// E_CENTERED: 1 along an E component's own axis and 0.5 across it, the reverse for H) * fn_dx.
+template <typename T>
+__device__ __forceinline__ void test_coords (const StepArgs<T> &a, int comp, const int p[3], double r[3])
+{
+  const int axis = comp % 3;
+  const bool h = comp >= 3;
+  for (int d = 0; d < 3; ++d) r[d] = ((double) p[d] + ((d == axis) != h ? 1.0 : 0.5)) * a.fn_dx[d];
+}
+
+// The curl of the half-step kernels with the right side of an analytic test problem
+// subtracted last (calcField, InternalScheme.inc.h:765-772: o12 - o11 - o22 + o21 - J dx).
+template <typename T, int COMP>
+__device__ __forceinline__ T curl_rhs (const StepArgs<T> &a, const T v[4], const int p[3])
+{
+  const T cv = curl (v);
+  if (!a.rhs_kind) return cv;
+  double r[3];
+  test_coords (a, COMP, p, r);
+  return cv - (T) test_rhs (a.rhs_kind, COMP, r[0], r[1], r[2], a.rhs_time) * a.dx;
 }
 
 template <typename T>

@@ -35,6 +35,9 @@ template <typename T> __global__ void tfsf_mask_kernel (StepArgs<T>, int, int, u
 template <typename T, int COMP, int DIM> __global__ void scattered_kernel (StepArgs<T>, const T *, double *, int, int64_t, int64_t);
 template <typename T, int BY, int MAT, int MAXW> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
+template <typename T> __global__ void test_fill (StepArgs<T>, T *, int, int, double, int);
+template <typename T>
+__global__ void test_norm (StepArgs<T>, const T *, int, int, double, int, int, int, int, int, int, double *);
 template <typename T>
 __global__ void fine_average (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *);
 
@@ -183,6 +186,11 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   a.zc_n[1] = (int) h->zc_n[1];
   // the E half-step of step t reads HInc after step t-1, kept in buffer (t+1) & 1
   a.hinc_sel = parity;
+  // analytic test problem: the right side of this half step, at time step t (E) or t + 1/2 (H)
+  // (Scheme.inc.h:1062-1098)
+  a.rhs_kind = h->fn_rhs;
+  a.rhs_time = (role == kRoleH ? (double) h->t + 0.5 : (double) h->t) * h->sc.dt;
+  for (int ax = 0; ax < 3; ++ax) a.fn_dx[ax] = axis_active (h->cfg, ax) ? h->cfg.dx : 0.0;
   return a;
 }
 
@@ -327,6 +335,15 @@ int launch_sources (yee_handle *h, bool e_half, int32_t t, int buf, bool skip_bo
     HIP_TRY (hipGetLastError ());
   }
   return YEE_OK;
+}
+
+// Components an analytic test problem kind defines (Scheme::initCallBacks, Scheme.inc.h:
+// 2150-2430): polinom2 all six, polinom1 / polinom3 / sin1 Ez and Hy.
+bool test_defines (int kind, int comp)
+{
+  if (kind == YEE_FN_POLINOM2) return true;
+  if (kind == YEE_FN_POLINOM1 || kind == YEE_FN_POLINOM3 || kind == YEE_FN_SIN1) return comp == YEE_EZ || comp == YEE_HY;
+  return false;
 }
 
 // Records an event pair around the next launch when kernel timing is on.
@@ -512,6 +529,56 @@ int fused_step (yee_handle *h, int32_t t, int prev)
   return launch_sources<T> (h, false, t, dst, false);
 }
 
+// Analytic test problem, after the half step's update of every component and before its point
+// sources (Scheme::calculateFieldStep, Scheme.inc.h:1346-1550, then performPointSourceCalc):
+// border values, then the diff norm of each component with an exact solution, at time step
+// t + 1/2 (E) or t + 1 (H), also at t == 0.
+constexpr int kNormBlocks = 1024;
+
+template <typename T>
+int test_hooks (yee_handle *h, bool e_half)
+{
+  const double time = (e_half ? (double) h->t + 0.5 : (double) h->t + 1.0) * h->sc.dt;
+  StepArgs<T> a = make_args<T> (h, 0);
+  const int64_t blocks = std::max<int64_t> (1, std::min<int64_t> ((h->ncell + 255) / 256, 65536));
+  for (int c = e_half ? 0 : 3; c < (e_half ? 3 : 6); ++c)
+  {
+    if (!h->present[c]) continue;
+    T *f = (T *) h->f[h->cur][c];
+    if (h->fn_border && test_defines (h->fn_border, c))
+    {
+      hipLaunchKernelGGL ((test_fill<T>), dim3 ((unsigned) blocks), dim3 (256), 0, h->stream, a, f, c, h->fn_border, time, 1);
+      HIP_TRY (hipGetLastError ());
+    }
+    if (h->fn_exact && test_defines (h->fn_exact, c))
+    {
+      hipLaunchKernelGGL ((test_norm<T>), dim3 (kNormBlocks), dim3 (256), 0, h->stream, a, (const T *) f, c, h->fn_exact,
+                         time, (int) h->norm_lo[0], (int) h->norm_lo[1], (int) h->norm_lo[2], (int) h->norm_hi[0],
+                         (int) h->norm_hi[1], (int) h->norm_hi[2], h->norm_part + (size_t) c * 2 * kNormBlocks);
+      HIP_TRY (hipGetLastError ());
+      h->norm_valid[c] = true;
+    }
+  }
+  return YEE_OK;
+}
+
+// Start values of an analytic test problem (E at dt / 2, H at dt), at yee_prepare.
+template <typename T>
+int test_start (yee_handle *h)
+{
+  StepArgs<T> a = make_args<T> (h, 0);
+  const int64_t blocks = std::max<int64_t> (1, std::min<int64_t> ((h->ncell + 255) / 256, 65536));
+  for (int c = 0; c < 6; ++c)
+  {
+    if (!h->present[c] || !test_defines (h->fn_start, c)) continue;
+    const double time = c >= 3 ? h->sc.dt : 0.5 * h->sc.dt;
+    hipLaunchKernelGGL ((test_fill<T>), dim3 ((unsigned) blocks), dim3 (256), 0, h->stream, a, (T *) h->f[h->cur][c], c,
+                       h->fn_start, time, 0);
+    HIP_TRY (hipGetLastError ());
+  }
+  return YEE_OK;
+}
+
 template <typename T>
 int step_once (yee_handle *h)
 {
@@ -542,12 +609,14 @@ int step_once (yee_handle *h)
     ProfScope ps (h, 1);
     if ((rc = launch_half<T> (h, true, prev))) return rc;
   }
+  if (h->fn_any () && (rc = test_hooks<T> (h, true))) return rc;
   if ((rc = launch_sources<T> (h, true, t, h->cur, false))) return rc;
   if (t > 0)
   {
     ProfScope ps (h, 2);
     if ((rc = launch_half<T> (h, false, prev))) return rc;
   }
+  if (h->fn_any () && (rc = test_hooks<T> (h, false))) return rc;
   if ((rc = launch_sources<T> (h, false, t, h->cur, false))) return rc;
   h->t++;
   return YEE_OK;
@@ -1009,6 +1078,7 @@ void yee_destroy (yee_handle *h)
   if (h->hinc[1]) (void) hipFree (h->hinc[1]);
   for (hipEvent_t e : h->ev)
     if (e) (void) hipEventDestroy (e);
+  if (h->norm_part) (void) hipFree (h->norm_part);
   if (h->stream) (void) hipStreamDestroy (h->stream);
   delete h;
 }
@@ -1108,6 +1178,11 @@ int yee_prepare (yee_handle *h)
     h->tab_len = (int) std::max (h->n[0], std::max (h->n[1], h->n[2]));
     std::vector<double> tab = pml_tables (h->cfg, h->tab_len);
     int rc = upload (h, &h->pml, tab.data (), tab.size ());
+    if (rc) return rc;
+  }
+  if (h->fn_start)
+  {
+    int rc = h->fp32 () ? test_start<float> (h) : test_start<double> (h);
     if (rc) return rc;
   }
   HIP_TRY (hipStreamSynchronize (h->stream));
@@ -1247,6 +1322,75 @@ int yee_material_size (yee_handle *h, int64_t out[3])
 {
   if (!h || !out) return fail (YEE_EINVAL, "null argument");
   for (int a = 0; a < 3; ++a) out[a] = h->mn[a];
+  return YEE_OK;
+}
+
+int yee_set_test_functions (yee_handle *h, int32_t start_values, int32_t border, int32_t right_side, int32_t exact,
+                            const int64_t norm_start[3], const int64_t norm_end[3])
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  if (h->prepared) return fail (YEE_ESTATE, "test functions must be set before yee_prepare");
+  const int32_t kinds[4] = { start_values, border, right_side, exact };
+  for (int32_t k : kinds)
+    if (k < YEE_FN_NONE || k > YEE_FN_SIN1) return fail (YEE_EINVAL, "unknown test function kind");
+  if (right_side == YEE_FN_SIN1) return fail (YEE_EINVAL, "sin1 has no right side (CallBack.cpp)");
+  const bool any = start_values || border || right_side || exact;
+  if (any && (h->use_pml () || h->complex () || h->use_meta ()))
+    return fail (YEE_ENOTSUP, "test functions run without PML, metamaterials and complex fields (the reference's "
+                              "border and norm then act on its D / E grids)");
+  if (start_values && !border)
+    return fail (YEE_ENOTSUP, "start values need border values (the reference's two time layers otherwise alternate "
+                              "the boundary cells between the start values and zero)");
+  int64_t lo[3], hi[3];
+  for (int a = 0; a < 3; ++a)
+  {
+    const bool act = axis_active (h->cfg, a);
+    lo[a] = act ? 1 : 0;
+    hi[a] = act ? h->n[a] - 1 : 1;
+    if (act && norm_start && norm_start[a] != 0) lo[a] = norm_start[a];
+    if (act && norm_end && norm_end[a] != 0) hi[a] = norm_end[a];
+    if (lo[a] < 0 || hi[a] > h->n[a] || lo[a] >= hi[a]) return fail (YEE_EINVAL, "norm box out of the grid");
+  }
+  if (exact && !h->norm_part)
+  {
+    void *p = nullptr;
+    HIP_TRY (hipMalloc (&p, (size_t) 6 * 2 * kNormBlocks * sizeof (double)));
+    h->norm_part = (double *) p;
+  }
+  h->fn_start = start_values;
+  h->fn_border = border;
+  h->fn_rhs = right_side;
+  h->fn_exact = exact;
+  for (int a = 0; a < 3; ++a)
+  {
+    h->norm_lo[a] = lo[a];
+    h->norm_hi[a] = hi[a];
+  }
+  // the fused kernel has no right side: the half-step kernels update the whole grid in place
+  if (any) h->fused = false;
+  return YEE_OK;
+}
+
+int yee_get_diff_norm (yee_handle *h, int32_t comp, double out[4])
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (comp < 0 || comp > 5) return fail (YEE_EINVAL, "component must be E or H (0 .. 5)");
+  if (!h->norm_valid[comp]) return fail (YEE_ESTATE, "no diff norm of this component yet (exact solution and a step)");
+  std::vector<double> part ((size_t) 2 * kNormBlocks);
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  HIP_TRY (hipMemcpy (part.data (), h->norm_part + (size_t) comp * 2 * kNormBlocks, part.size () * sizeof (double),
+                      hipMemcpyDeviceToHost));
+  double sum = 0.0, mx = 0.0;
+  for (int b = 0; b < kNormBlocks; ++b)
+  {
+    sum += part[(size_t) 2 * b];
+    mx = std::max (mx, part[(size_t) 2 * b + 1]);
+  }
+  const double norm = std::sqrt (sum / (double) h->ncell);
+  out[0] = norm;
+  out[1] = norm * 100.0 / mx;
+  out[2] = sum;
+  out[3] = mx;
   return YEE_OK;
 }
 

@@ -93,6 +93,15 @@ struct yee_handle
   // complex handle (the real part) owns
   int src_kind = 0;
   yee_handle *imag = nullptr;
+
+  // analytic test problem (YEE_FN_* kinds; yee_set_test_functions): start values, border values,
+  // right side, exact solution of the diff norm, its box, and per component the workgroup
+  // partial sums of the last step's norm (test_norm)
+  int fn_start = 0, fn_border = 0, fn_rhs = 0, fn_exact = 0;
+  int64_t norm_lo[3] = { 0, 0, 0 }, norm_hi[3] = { 0, 0, 0 };
+  double *norm_part = nullptr;
+  bool norm_valid[6] = {};
+  bool fn_any () const { return fn_start || fn_border || fn_rhs || fn_exact; }
   bool complex () const { return (cfg.flags & YEE_COMPLEX_FIELDS) != 0; }
 };
 

@@ -86,6 +86,12 @@ struct StepArgs
   // precomputed per step: [part (E, H)][face][x - (L-1)][y - (L-1)][comp (Ex/Ey or Hx/Hy)][slot lo/hi]
   const double *zcorr;
   int zc_n[2];                 // extents of the face tables along x and y
+  // analytic test problem (YEE_FN_* kind, 0 = none): the right side J / M subtracted from the
+  // curl of the half-step kernels, evaluated at time rhs_time (s) at (p + min coordinate) * fn_dx
+  // (fn_dx = dx along the scheme's axes, 0 along an axis it lacks)
+  int rhs_kind;
+  double rhs_time;
+  double fn_dx[3];
 };
 
 }  // namespace yee

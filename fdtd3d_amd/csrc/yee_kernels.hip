@@ -248,9 +248,9 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
       gzm = avg2 (gld (g + q), gld (g + q + sz));
     }
   }
-  update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl (vx), ex_m, special, oxm, gxm);
-  update_cell<T, MODE, 1> (a, q, p, fo1, do1, curl (vy), ey_m, special, oym, gym);
-  update_cell<T, MODE, 2> (a, q, p, fo2, do2, curl (vz), ez_m, special, ozm, gzm);
+  update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl_rhs<T, 0> (a, vx, p), ex_m, special, oxm, gxm);
+  update_cell<T, MODE, 1> (a, q, p, fo1, do1, curl_rhs<T, 1> (a, vy, p), ey_m, special, oym, gym);
+  update_cell<T, MODE, 2> (a, q, p, fo2, do2, curl_rhs<T, 2> (a, vz, p), ez_m, special, ozm, gzm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -314,9 +314,9 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
       gzm = sum4 (gld (g + q), gld (g + q + sy), gld (g + q + sx), gld (g + q + sx + sy));
     }
   }
-  update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), hx_m, special, oxm, gxm);
-  update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), hy_m, special, oym, gym);
-  update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl (vz), hz_m, special, ozm, gzm);
+  update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl_rhs<T, 3> (a, vx, p), hx_m, special, oxm, gxm);
+  update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl_rhs<T, 4> (a, vy, p), hy_m, special, oym, gym);
+  update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl_rhs<T, 5> (a, vz, p), hz_m, special, ozm, gzm);
 }
 
 // Drude runs (MODE 2): a cell outside the Drude box has omega_p = gamma = 0 in every average,
@@ -964,7 +964,7 @@ __device__ __forceinline__ void tmz_e_cell (const StepArgs<T> &a, int i, int j)
     om = avg2 (o, o);
     ga = avg2 (g, g);
   }
-  update_cell<T, MODE, 2> (a, q, p, fo, dold, curl (v), mat, special, om, ga);
+  update_cell<T, MODE, 2> (a, q, p, fo, dold, curl_rhs<T, 2> (a, v, p), mat, special, om, ga);
 }
 
 template <typename T, int MODE>
@@ -1020,8 +1020,8 @@ __device__ __forceinline__ void tmz_h_cell (const StepArgs<T> &a, int i, int j)
       gmy = sum4 (g[q], g[q], g[q + sx], g[q + sx]);
     }
   }
-  update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl (vx), mx, special, omx, gmx);
-  update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl (vy), my, special, omy, gmy);
+  update_cell<T, MODE, 3> (a, q, p, fo3, do3, curl_rhs<T, 3> (a, vx, p), mx, special, omx, gmx);
+  update_cell<T, MODE, 4> (a, q, p, fo4, do4, curl_rhs<T, 4> (a, vy, p), my, special, omy, gmy);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1077,8 +1077,8 @@ __device__ __forceinline__ void tez_e_cell (const StepArgs<T> &a, int i, int j)
       gym = avg2 (a.ge[q], a.ge[q + sy]);
     }
   }
-  update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl (vx), ex_m, special, oxm, gxm);
-  update_cell<T, MODE, 1> (a, q, p, fo1, do1, curl (vy), ey_m, special, oym, gym);
+  update_cell<T, MODE, 0> (a, q, p, fo0, do0, curl_rhs<T, 0> (a, vx, p), ex_m, special, oxm, gxm);
+  update_cell<T, MODE, 1> (a, q, p, fo1, do1, curl_rhs<T, 1> (a, vy, p), ey_m, special, oym, gym);
 }
 
 template <typename T, int MODE>
@@ -1114,7 +1114,7 @@ __device__ __forceinline__ void tez_h_cell (const StepArgs<T> &a, int i, int j)
     if (o) ozm = sum4 (o[q], o[q + sy], o[q + sx], o[q + sx + sy]);
     if (g) gzm = sum4 (g[q], g[q + sy], g[q + sx], g[q + sx + sy]);
   }
-  update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl (vz), hz_m, special, ozm, gzm);
+  update_cell<T, MODE, 5> (a, q, p, fo5, do5, curl_rhs<T, 5> (a, vz, p), hz_m, special, ozm, gzm);
 }
 
 template <typename T, int MODE>
@@ -1676,6 +1676,72 @@ __global__ void fine_average (const T *m, int64_t mn0, int64_t mn1, int64_t mn2,
   }
 }
 
+// Analytic test problems: component `comp` of every cell (border 0: the start values of
+// Scheme::initGridWithInitialVals, Scheme.inc.h:2863-2958) or of every cell with an index 0 or
+// N-1 along an axis of the scheme (border 1: calculateFieldStepIterationBorder,
+// InternalScheme.inc.h:2646-2705, with doSkipBorderFunc :539-555) set to the kind's value at
+// time `time` (s).  Grid-stride over the cells; the border pass skips the interior.
+template <typename T>
+__global__ void test_fill (StepArgs<T> a, T *f, int comp, int kind, double time, int border)
+{
+  const int64_t n = (int64_t) a.nx * a.ny * a.nz;
+  for (int64_t q = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; q < n; q += (int64_t) gridDim.x * blockDim.x)
+  {
+    const int p[3] = { (int) (q / ((int64_t) a.ny * a.nz)), (int) ((q / a.nz) % a.ny), (int) (q % a.nz) };
+    if (border)
+    {
+      const int ext[3] = { a.nx, a.ny, a.nz };
+      bool edge = false;
+      for (int d = 0; d < 3; ++d) edge = edge || (a.fn_dx[d] != 0.0 && (p[d] == 0 || p[d] == ext[d] - 1));
+      if (!edge) continue;
+    }
+    double r[3];
+    test_coords (a, comp, p, r);
+    gst (f + q, (T) test_field (kind, comp, r[0], r[1], r[2], time));
+  }
+}
+
+// Difference norm against the exact solution (calculateFieldStepIterationExact,
+// InternalScheme.inc.h:2707-2816) over the box [lo, hi): per workgroup the sum of
+// (exact - field)^2 and the largest |exact| of its grid-stride share, written to part[2 b],
+// part[2 b + 1] (the host adds the workgroups' sums in order).
+template <typename T>
+__global__ __launch_bounds__ (256) void test_norm (StepArgs<T> a, const T *f, int comp, int kind, double time,
+                                                   int lo0, int lo1, int lo2, int hi0, int hi1, int hi2, double *part)
+{
+  __shared__ double ss[256], sm[256];
+  const int64_t e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
+  const int64_t n = e0 * e1 * e2;
+  double sum = 0.0, mx = 0.0;
+  for (int64_t u = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; u < n; u += (int64_t) gridDim.x * blockDim.x)
+  {
+    const int p[3] = { lo0 + (int) (u / (e1 * e2)), lo1 + (int) ((u / e2) % e1), lo2 + (int) (u % e2) };
+    double r[3];
+    test_coords (a, comp, p, r);
+    const double exact = test_field (kind, comp, r[0], r[1], r[2], time);
+    const double d = exact - (double) gld (f + (int64_t) p[0] * a.sx + (int64_t) p[1] * a.sy + (int64_t) p[2] * a.sz);
+    sum += d * d;
+    mx = fmax (mx, fabs (exact));
+  }
+  ss[threadIdx.x] = sum;
+  sm[threadIdx.x] = mx;
+  __syncthreads ();
+  for (int w = 128; w > 0; w >>= 1)
+  {
+    if ((int) threadIdx.x < w)
+    {
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+      sm[threadIdx.x] = fmax (sm[threadIdx.x], sm[threadIdx.x + w]);
+    }
+    __syncthreads ();
+  }
+  if (threadIdx.x == 0)
+  {
+    gst (part + 2 * blockIdx.x, ss[0]);
+    gst (part + 2 * blockIdx.x + 1, sm[0]);
+  }
+}
+
 // TF/SF predicate mask of one component for every cell (set-up check export).
 // Scattered field of one component (Scheme::makeGridScattered, Scheme.inc.h:4113-4228): the
 // newest field minus the incident plane wave at the component's grid points inside the TF/SF
@@ -1779,6 +1845,8 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void tfsf_ztable<T> (StepArgs<T>, double *);                         \
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *); \
+  template __global__ void test_fill<T> (StepArgs<T>, T *, int, int, double, int); \
+  template __global__ void test_norm<T> (StepArgs<T>, const T *, int, int, double, int, int, int, int, int, int, double *); \
   template __global__ void fine_average<T> (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *); \
   template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
   template __global__ void scattered_kernel<T, 1, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \

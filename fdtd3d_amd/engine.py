@@ -28,12 +28,18 @@ EXPORTED = [
     "yee_last_error", "yee_device_info", "yee_create", "yee_destroy", "yee_set_material",
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
-    "yee_get_scattered_field_noalloc", "yee_get_imag_part", "yee_material_size",
+    "yee_get_scattered_field_noalloc", "yee_get_imag_part", "yee_material_size", "yee_set_test_functions",
+    "yee_get_diff_norm",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
     "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
     "yee_batch_sync", "yee_batch_time", "yee_batch_last_step_ms", "yee_batch_tuning", "yee_batch_get_field",
 ]
+
+
+# PhysicsConst.h:27-32
+MU0 = 4 * 3.14159265358979323846 * 0.0000001
+EPS0 = 1 / (MU0 * (299792458.0 * 299792458.0))
 
 
 class AdjointSpecC(ctypes.Structure):
@@ -79,6 +85,8 @@ def load_library():
     L.yee_get_imag_part.argtypes = [vp, ctypes.POINTER(vp)]
     L.yee_get_material.argtypes = [vp, i32, vp]
     L.yee_material_size.argtypes = [vp, vp]
+    L.yee_set_test_functions.argtypes = [vp, i32, i32, i32, i32, vp, vp]
+    L.yee_get_diff_norm.argtypes = [vp, i32, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.yee_get_scalars.argtypes = [vp, vp]
@@ -171,6 +179,11 @@ class Engine:
         ms = (ctypes.c_int64 * 3)()
         _check(L.yee_material_size(h, ms))
         self.material_shape = tuple(int(ms[a]) for a in range(3) if cfg.dim == 3 or a in cfg.active_axes)
+        if cfg.start_fn or cfg.border_fn or cfg.rhs_fn or cfg.exact_fn:
+            # analytic test problem of the reference's --use-<kind>-* / --calc-<kind>-diff-norm
+            ns = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_start])
+            ne = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_end])
+            _check(L.yee_set_test_functions(h, cfg.start_fn, cfg.border_fn, cfg.rhs_fn, cfg.exact_fn, ns, ne))
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -211,7 +224,18 @@ class Engine:
         _check(load_library().yee_set_point_source_gaussian(self.h, float(t0), float(sigma)))
 
     def prepare(self) -> None:
+        # --eps-normed / --mu-normed replace the grids by 1 / eps0, 1 / mu0 (Scheme.inc.h:3016-3025, 3066-3075)
+        for on, name, v in ((self.cfg.eps_normed, "eps", 1.0 / EPS0), (self.cfg.mu_normed, "mu", 1.0 / MU0)):
+            if on:
+                self.set_material(name, np.full(self.material_shape, v))
         _check(load_library().yee_prepare(self.h))
+
+    def diff_norm(self, name: str):
+        """(norm, percent, sum of squares, max |exact|) of a component against the exact solution
+        at the last step (the reference's "DIFF NORM" line)."""
+        out = (ctypes.c_double * 4)()
+        _check(load_library().yee_get_diff_norm(self.h, COMP_INDEX[name], out))
+        return tuple(out)
 
     def step(self, n: int) -> None:
         _check(load_library().yee_step(self.h, int(n)))

@@ -70,6 +70,15 @@ extern "C" {
                                        Excludes metamaterials: the reference's 8-cell Drude average is
                                        UNREACHABLE (Layout/Approximation.cpp:196) */
 
+/* Analytic test problems of the reference's real build (Source/Scheme/CallBack.cpp:28-143), the
+   kinds of --use-<kind>-start-values / --use-<kind>-border-condition / --use-<kind>-right-side /
+   --calc-<kind>-diff-norm (Settings.inc.h:134-153); see yee_set_test_functions. */
+#define YEE_FN_NONE 0
+#define YEE_FN_POLINOM1 1   /* Ez = c^2 x^2 t, Hy = eps0 Ez; right side Jz, My */
+#define YEE_FN_POLINOM2 2   /* Ex = c^2 t (y^2 + z^2) and its cyclic twins, H = eps0 E; right side J, M */
+#define YEE_FN_POLINOM3 3   /* Ez = c^2 (x t)^2, Hy = eps0 Ez; right side Jz, My */
+#define YEE_FN_SIN1 4       /* Ez = c sin (t - x), Hy = -c sin (t - x); no right side */
+
 /* Field components (GridType order of the reference). */
 enum yee_component
 {
@@ -211,6 +220,36 @@ int yee_set_point_source_gaussian (yee_handle *h, double t0, double sigma);
  * by itself).  YEE_EINVAL for a real-field handle.
  */
 int yee_get_imag_part (yee_handle *h, yee_handle **imag);
+
+/*
+ * Analytic test problem (YEE_FN_* kinds, YEE_FN_NONE for none), replacing the callbacks
+ * Scheme::initCallBacks installs for --use-<kind>-start-values / -border-condition / -right-side
+ * and --calc-<kind>-diff-norm (Source/Scheme/Scheme.inc.h:2150-2430):
+ *   start_values: every cell of the kind's components at E: dt / 2, H: dt
+ *                 (Scheme::initGridWithInitialVals, Scheme.inc.h:2863-2958);
+ *   border:       every step, after a component's update, its cells with an index 0 or N-1
+ *                 (calculateFieldStepIterationBorder, InternalScheme.inc.h:2646-2705);
+ *   right_side:   J (E) / M (H) subtracted in the curl update, calcField (InternalScheme.inc.h:765-772);
+ *   exact:        every step the difference norm of the kind's components over the box
+ *                 [norm_start, norm_end) (0 entries: [1, N-1); --norm-start / --norm-end), read
+ *                 with yee_get_diff_norm (calculateFieldStepIterationExact, InternalScheme.inc.h:
+ *                 2707-2816; Scheme.inc.h:1377-1550).
+ * Before yee_prepare.  Runs on the half-step kernels.  YEE_ENOTSUP with PML, metamaterials or
+ * complex fields (the reference's border and norm then act on its D grids), and for start values
+ * without border values (its two time layers then alternate the boundary cells between the start
+ * values and zero).
+ */
+int yee_set_test_functions (yee_handle *h, int32_t start_values, int32_t border, int32_t right_side, int32_t exact,
+                            const int64_t norm_start[3], const int64_t norm_end[3]);
+
+/*
+ * Difference norm of component comp against the exact solution at the last step, as
+ * Scheme::calculateFieldStep prints it ("DIFF NORM", Scheme.inc.h:1543-1549): out[0] =
+ * sqrt (sum of (exact - field)^2 over the norm box / cells of the grid), out[1] = out[0] * 100 /
+ * max |exact| (percent), out[2] = the sum, out[3] = max |exact|.  Taken after the component's
+ * update and border values and before its point source, as the reference does.
+ */
+int yee_get_diff_norm (yee_handle *h, int32_t comp, double out[4]);
 
 /* Waits for all enqueued work of the handle. */
 int yee_sync (yee_handle *h);

@@ -49,6 +49,8 @@ def lib():
         L.oracle_set_source_phase.argtypes = [vp, ctypes.c_double]
         L.oracle_set_source_kind.argtypes = [vp, ctypes.c_int]
         L.oracle_set_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.oracle_set_test_functions.argtypes = [vp] + [ctypes.c_int] * 4 + [vp, vp]
+        L.oracle_get_diff_norm.argtypes = [vp, ctypes.c_int32, vp]
         L.oracle_adjoint.argtypes = [vp, ctypes.POINTER(OracleAdjSpec), ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_double), vp]
         d = ctypes.c_double
@@ -148,6 +150,11 @@ class Oracle:
             raise ValueError("oracle_create rejected the configuration")
         self.shape = cfg.shape
         self.material_shape = cfg.material_shape
+        if cfg.start_fn or cfg.border_fn or cfg.rhs_fn or cfg.exact_fn:
+            ns = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_start])
+            ne = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_end])
+            if lib().oracle_set_test_functions(self.h, cfg.start_fn, cfg.border_fn, cfg.rhs_fn, cfg.exact_fn, ns, ne):
+                raise ValueError("oracle_set_test_functions rejected the configuration")
 
     def close(self):
         if self.h:
@@ -175,7 +182,20 @@ class Oracle:
             raise ValueError(name)
 
     def prepare(self) -> None:
+        # --eps-normed / --mu-normed (Scheme.inc.h:3016-3025, 3066-3075)
+        mu0 = 4 * 3.14159265358979323846 * 0.0000001
+        eps0 = 1 / (mu0 * (299792458.0 * 299792458.0))
+        for on, name, v in ((self.cfg.eps_normed, "eps", 1.0 / eps0), (self.cfg.mu_normed, "mu", 1.0 / mu0)):
+            if on:
+                self.set_material(name, np.full(self.material_shape, v))
         lib().oracle_prepare(self.h)
+
+    def diff_norm(self, name: str):
+        """(norm, percent, sum of squares, max |exact|) at the last step."""
+        out = (ctypes.c_double * 4)()
+        if lib().oracle_get_diff_norm(self.h, COMP_INDEX[name], out):
+            raise KeyError(name)
+        return tuple(out)
 
     def step(self, n: int) -> None:
         if lib().oracle_step(self.h, int(n)):

@@ -53,6 +53,7 @@
 
 #include "InternalScheme.h"
 #include "Approximation.h"
+#include "CallBack.h"
 #include "TXTDumper.h"   /* expects <iomanip> and <limits> first, as its own TXTDumper.cpp includes them */
 
 namespace
@@ -133,6 +134,8 @@ class RefRun
   IS *is;
   int dim;
   int miss;   /* the axis a 2D scheme lacks (-1 in 3D): TEx/TMx 0, TEy/TMy 1, TEz/TMz 2 */
+  /* diff norm and percent of each step per component (index 0 .. 5: Ex .. Hz) */
+  std::vector<double> norms[6];
   int axis1;  /* the axis of a 1D scheme (-1 otherwise) */
 
   /* an axis the scheme lacks */
@@ -377,8 +380,10 @@ class RefRun
       bool isE = grid_type == static_cast<uint8_t> (GridType::EX) || grid_type == static_cast<uint8_t> (GridType::EY)
                  || grid_type == static_cast<uint8_t> (GridType::EZ);
       FPValue timestep = isE ? FPValue (t) : FPValue (t) + FPValue (0.5);
-      TCFP coordFP;
       forAll (1, 1, [&] (TC p) {
+        /* the right side is read at the component's point (Scheme.inc.h:1108-1148) */
+        TCFP coordFP;
+        if (rhs != NULLPTR) coordFP = compCoordFP<grid_type> (grid->getTotalPosition (p));
         if (Ca != NULLPTR)
           is->template calculateFieldStepIteration<grid_type, true> (timestep, p, grid->getTotalPosition (p),
             d11, d12, d21, d22, grid, coordFP, opp1, opp2, rhs, Ca, Cb, usePML, gridType, mg, mgt, modifier);
@@ -434,6 +439,40 @@ class RefRun
               pmlType1, mg1, mgt1, mg4, mgt4, mg5, mgt5, modifier);
         });
       }
+    }
+
+    /* border values and the diff norm of an analytic test problem, also at t == 0
+     * (Scheme::calculateFieldStep, Scheme.inc.h:1346-1550) */
+    if (border != NULLPTR)
+    {
+      forAll (0, 0, [&] (TC p) { is->template calculateFieldStepIterationBorder<grid_type> (t, grid->getTotalPosition (p), grid, border); });
+    }
+    if (exact != NULLPTR)
+    {
+      FPValue normRe = 0, normIm = 0, normMod = 0, maxRe = 0, maxIm = 0, maxMod = 0;
+      grid_coord lo[3], hi[3];
+      const grid_coord ns[3] = { SOLVER_SETTINGS.getExactSolutionCompareStartX (), SOLVER_SETTINGS.getExactSolutionCompareStartY (),
+                                 SOLVER_SETTINGS.getExactSolutionCompareStartZ () };
+      const grid_coord ne[3] = { SOLVER_SETTINGS.getExactSolutionCompareEndX (), SOLVER_SETTINGS.getExactSolutionCompareEndY (),
+                                 SOLVER_SETTINGS.getExactSolutionCompareEndZ () };
+      for (int a = 0; a < 3; ++a)
+      {
+        lo[a] = missing (a) ? 0 : (ns[a] != 0 ? ns[a] : 1);
+        hi[a] = missing (a) ? 1 : (ne[a] != 0 ? ne[a] : sz (a) - 1);
+      }
+      Grid<TC> *normGrid = usePML ? pml2 : grid;
+      for (grid_coord i = lo[0]; i < hi[0]; ++i)
+        for (grid_coord j = lo[1]; j < hi[1]; ++j)
+          for (grid_coord k = lo[2]; k < hi[2]; ++k)
+            is->template calculateFieldStepIterationExact<grid_type> (t, normGrid->getTotalPosition (coordOf (i, j, k)),
+              normGrid, exact, normRe, normIm, normMod, maxRe, maxIm, maxMod);
+      normRe = sqrt (normRe / normGrid->getTotalSize ().calculateTotalCoord ());
+      const int ci = grid_type == static_cast<uint8_t> (GridType::EX) ? 0 : grid_type == static_cast<uint8_t> (GridType::EY) ? 1
+                     : grid_type == static_cast<uint8_t> (GridType::EZ) ? 2 : grid_type == static_cast<uint8_t> (GridType::HX) ? 3
+                     : grid_type == static_cast<uint8_t> (GridType::HY) ? 4 : 5;
+      std::vector<double> &rec = norms[ci];
+      rec.push_back ((double) normRe);
+      rec.push_back ((double) (normRe * 100.0 / maxRe));
     }
 
     bool src = false;
@@ -502,8 +541,147 @@ public:
     is->initScheme (SOLVER_SETTINGS.getGridStep (), SOLVER_SETTINGS.getSourceWaveLength ());
   }
 
+  /* The real build's CPU branch of Scheme::initCallBacks (Scheme.inc.h:2150-2430): the analytic
+   * test problem's callbacks on the reference's own InternalScheme, in its priority order. */
+  void initCallBacks ()
+  {
+#ifndef COMPLEX_FIELD_VALUES
+    if (SOLVER_SETTINGS.getDoUsePolinom1BorderCondition ())
+    {
+      is->setCallbackEzBorder (CallBack::polinom1_ez);
+      is->setCallbackHyBorder (CallBack::polinom1_hy);
+    }
+    else if (SOLVER_SETTINGS.getDoUsePolinom2BorderCondition ())
+    {
+      is->setCallbackExBorder (CallBack::polinom2_ex);
+      is->setCallbackEyBorder (CallBack::polinom2_ey);
+      is->setCallbackEzBorder (CallBack::polinom2_ez);
+      is->setCallbackHxBorder (CallBack::polinom2_hx);
+      is->setCallbackHyBorder (CallBack::polinom2_hy);
+      is->setCallbackHzBorder (CallBack::polinom2_hz);
+    }
+    else if (SOLVER_SETTINGS.getDoUsePolinom3BorderCondition ())
+    {
+      is->setCallbackEzBorder (CallBack::polinom3_ez);
+      is->setCallbackHyBorder (CallBack::polinom3_hy);
+    }
+    else if (SOLVER_SETTINGS.getDoUseSin1BorderCondition ())
+    {
+      is->setCallbackEzBorder (CallBack::sin1_ez);
+      is->setCallbackHyBorder (CallBack::sin1_hy);
+    }
+    if (SOLVER_SETTINGS.getDoUsePolinom1StartValues ())
+    {
+      is->setCallbackEzInitial (CallBack::polinom1_ez);
+      is->setCallbackHyInitial (CallBack::polinom1_hy);
+    }
+    else if (SOLVER_SETTINGS.getDoUsePolinom2StartValues ())
+    {
+      is->setCallbackExInitial (CallBack::polinom2_ex);
+      is->setCallbackEyInitial (CallBack::polinom2_ey);
+      is->setCallbackEzInitial (CallBack::polinom2_ez);
+      is->setCallbackHxInitial (CallBack::polinom2_hx);
+      is->setCallbackHyInitial (CallBack::polinom2_hy);
+      is->setCallbackHzInitial (CallBack::polinom2_hz);
+    }
+    else if (SOLVER_SETTINGS.getDoUsePolinom3StartValues ())
+    {
+      is->setCallbackEzInitial (CallBack::polinom3_ez);
+      is->setCallbackHyInitial (CallBack::polinom3_hy);
+    }
+    else if (SOLVER_SETTINGS.getDoUseSin1StartValues ())
+    {
+      is->setCallbackEzInitial (CallBack::sin1_ez);
+      is->setCallbackHyInitial (CallBack::sin1_hy);
+    }
+    if (SOLVER_SETTINGS.getDoUsePolinom1RightSide ())
+    {
+      is->setCallbackJz (CallBack::polinom1_jz);
+      is->setCallbackMy (CallBack::polinom1_my);
+    }
+    else if (SOLVER_SETTINGS.getDoUsePolinom2RightSide ())
+    {
+      is->setCallbackJx (CallBack::polinom2_jx);
+      is->setCallbackJy (CallBack::polinom2_jy);
+      is->setCallbackJz (CallBack::polinom2_jz);
+      is->setCallbackMx (CallBack::polinom2_mx);
+      is->setCallbackMy (CallBack::polinom2_my);
+      is->setCallbackMz (CallBack::polinom2_mz);
+    }
+    else if (SOLVER_SETTINGS.getDoUsePolinom3RightSide ())
+    {
+      is->setCallbackJz (CallBack::polinom3_jz);
+      is->setCallbackMy (CallBack::polinom3_my);
+    }
+    if (SOLVER_SETTINGS.getDoCalculatePolinom1DiffNorm ())
+    {
+      is->setCallbackEzExact (CallBack::polinom1_ez);
+      is->setCallbackHyExact (CallBack::polinom1_hy);
+    }
+    else if (SOLVER_SETTINGS.getDoCalculatePolinom2DiffNorm ())
+    {
+      is->setCallbackExExact (CallBack::polinom2_ex);
+      is->setCallbackEyExact (CallBack::polinom2_ey);
+      is->setCallbackEzExact (CallBack::polinom2_ez);
+      is->setCallbackHxExact (CallBack::polinom2_hx);
+      is->setCallbackHyExact (CallBack::polinom2_hy);
+      is->setCallbackHzExact (CallBack::polinom2_hz);
+    }
+    else if (SOLVER_SETTINGS.getDoCalculatePolinom3DiffNorm ())
+    {
+      is->setCallbackEzExact (CallBack::polinom3_ez);
+      is->setCallbackHyExact (CallBack::polinom3_hy);
+    }
+    else if (SOLVER_SETTINGS.getDoCalculateSin1DiffNorm ())
+    {
+      is->setCallbackEzExact (CallBack::sin1_ez);
+      is->setCallbackHyExact (CallBack::sin1_hy);
+    }
+#endif
+  }
+
+  /* Real coordinate of a component's point (YeeGridLayout::get<C>CoordFP). */
+  template <uint8_t grid_type>
+  TCFP compCoordFP (TC posAbs)
+  {
+    switch (grid_type)
+    {
+      case (static_cast<uint8_t> (GridType::EX)): return layout->getExCoordFP (posAbs);
+      case (static_cast<uint8_t> (GridType::EY)): return layout->getEyCoordFP (posAbs);
+      case (static_cast<uint8_t> (GridType::EZ)): return layout->getEzCoordFP (posAbs);
+      case (static_cast<uint8_t> (GridType::HX)): return layout->getHxCoordFP (posAbs);
+      case (static_cast<uint8_t> (GridType::HY)): return layout->getHyCoordFP (posAbs);
+      default: return layout->getHzCoordFP (posAbs);
+    }
+  }
+
+  /* Scheme::initGridWithInitialVals (Scheme.inc.h:2863-2958) for one component: every cell of
+   * its grid at `timestep` (s), called as Scheme::initGrids does (:3255-3279). */
+  template <uint8_t grid_type>
+  void initialValues (Grid<TC> *g, SourceCallBack cb, FPValue timestep)
+  {
+    if (!g || cb == NULLPTR) return;
+    forAll (0, 0, [&] (TC p) {
+      TCFP realCoord = compCoordFP<grid_type> (g->getTotalPosition (p));
+      g->setFieldValue (cb (expandTo3D (realCoord * is->getGridStep (), ct1, ct2, ct3), timestep), p, 0);
+    });
+  }
+
+  void initValues ()
+  {
+    const FPValue dt = is->getGridTimeStep ();
+    if (is->getDoNeedEx ()) initialValues<static_cast<uint8_t> (GridType::EX)> (is->getEx (), is->getCallbackExInitial (), 0.5 * dt);
+    if (is->getDoNeedEy ()) initialValues<static_cast<uint8_t> (GridType::EY)> (is->getEy (), is->getCallbackEyInitial (), 0.5 * dt);
+    if (is->getDoNeedEz ()) initialValues<static_cast<uint8_t> (GridType::EZ)> (is->getEz (), is->getCallbackEzInitial (), 0.5 * dt);
+    if (is->getDoNeedHx ()) initialValues<static_cast<uint8_t> (GridType::HX)> (is->getHx (), is->getCallbackHxInitial (), dt);
+    if (is->getDoNeedHy ()) initialValues<static_cast<uint8_t> (GridType::HY)> (is->getHy (), is->getCallbackHyInitial (), dt);
+    if (is->getDoNeedHz ()) initialValues<static_cast<uint8_t> (GridType::HZ)> (is->getHz (), is->getCallbackHzInitial (), dt);
+  }
+
   void initMaterials ()
   {
+    /* Scheme::initScheme: the callbacks, then Scheme::initGrids (materials, then the start values) */
+    initCallBacks ();
     is->getEps ()->initialize (FieldValueHelpers::getFieldValueRealOnly (1.0));
     loadMaterial (is->getEps (), SOLVER_SETTINGS.getEpsFileName ());
     if (SOLVER_SETTINGS.getEpsSphere () != 1)
@@ -512,6 +690,11 @@ public:
               SOLVER_SETTINGS.getEpsSphereCenterY (), SOLVER_SETTINGS.getEpsSphereCenterZ (),
               SOLVER_SETTINGS.getEpsSphereRadius ());
     }
+    if (SOLVER_SETTINGS.getUseEpsAllNorm ())
+    {
+      /* Scheme.inc.h:3016-3025 */
+      forAllMat ([&] (TC p) { is->getEps ()->setFieldValue (FieldValueHelpers::getFieldValueRealOnly (FPValue (1.0) / PhysicsConst::Eps0), p, 0); });
+    }
     is->getMu ()->initialize (FieldValueHelpers::getFieldValueRealOnly (1.0));
     loadMaterial (is->getMu (), SOLVER_SETTINGS.getMuFileName ());
     if (SOLVER_SETTINGS.getMuSphere () != 1)
@@ -519,6 +702,11 @@ public:
       sphere (is->getMu (), SOLVER_SETTINGS.getMuSphere (), 1.0, SOLVER_SETTINGS.getMuSphereCenterX (),
               SOLVER_SETTINGS.getMuSphereCenterY (), SOLVER_SETTINGS.getMuSphereCenterZ (),
               SOLVER_SETTINGS.getMuSphereRadius ());
+    }
+    if (SOLVER_SETTINGS.getUseMuAllNorm ())
+    {
+      /* Scheme.inc.h:3066-3075 */
+      forAllMat ([&] (TC p) { is->getMu ()->setFieldValue (FieldValueHelpers::getFieldValueRealOnly (FPValue (1.0) / PhysicsConst::Mu0), p, 0); });
     }
     if (SOLVER_SETTINGS.getDoUseMetamaterials ())
     {
@@ -558,6 +746,7 @@ public:
       if (is->getDoNeedHy ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::HY)> ();
       if (is->getDoNeedHz ()) fillCoefficientsDispatch<static_cast<uint8_t> (GridType::HZ)> ();
     }
+    initValues ();
   }
 
   double run ()
@@ -657,6 +846,11 @@ public:
 #undef DUMP_IF
     putRecord ("Eps", dumpMat (is->getEps ()));
     putRecord ("Mu", dumpMat (is->getMu ()));
+    {
+      static const char *kNormName[6] = { "NormEx", "NormEy", "NormEz", "NormHx", "NormHy", "NormHz" };
+      for (int c = 0; c < 6; ++c)
+        if (!norms[c].empty ()) putRecord (kNormName[c], norms[c]);
+    }
     if (pml)
     {
       if (!missing (0)) putRecord ("SigmaX", dumpMat (is->getSigmaX ()));

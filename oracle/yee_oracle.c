@@ -24,6 +24,87 @@
 static double o_mu0 (void) { return 4 * O_PI * 0.0000001; }
 static double o_eps0 (void) { return 1 / (o_mu0 () * (O_C * O_C)); }
 
+/*
+ * Analytic test problems, the real build's callbacks (Source/Scheme/CallBack.cpp:28-143; SQR (a)
+ * is a * a, PhysicsConst.h:30-32): field values at (x, y, z, t) and the right-side currents J
+ * (E components) / M (H components).  Which components each kind defines follows
+ * Scheme::initCallBacks (Scheme.inc.h:2150-2430): polinom1, polinom3, sin1 Ez and Hy,
+ * polinom2 all six; right sides polinom1 / polinom3 Jz and My, polinom2 all six, sin1 none.
+ */
+static int o_fn_has (int kind, int comp)
+{
+  if (kind == YEE_FN_POLINOM2) return 1;
+  if (kind == YEE_FN_POLINOM1 || kind == YEE_FN_POLINOM3 || kind == YEE_FN_SIN1) return comp == YEE_EZ || comp == YEE_HY;
+  return 0;
+}
+
+static int o_rhs_has (int kind, int comp)
+{
+  if (kind == YEE_FN_POLINOM2) return 1;
+  if (kind == YEE_FN_POLINOM1 || kind == YEE_FN_POLINOM3) return comp == YEE_EZ || comp == YEE_HY;
+  return 0;
+}
+
+static double o_fn_value (int kind, int comp, double x, double y, double z, double t)
+{
+  const double c = O_C, e0 = o_eps0 ();
+  switch (kind)
+  {
+    case YEE_FN_POLINOM1:
+    {
+      const double ez = (c * c) * (x * x) * t;
+      return comp == YEE_EZ ? ez : e0 * ez;
+    }
+    case YEE_FN_POLINOM2:
+    {
+      const int a = comp % 3;
+      const double u = a == 0 ? y : x, v = a == 2 ? y : z;   /* Ex: y, z; Ey: x, z; Ez: x, y */
+      const double e = (c * c) * t * ((u * u) + (v * v));
+      return comp < 3 ? e : e0 * e;
+    }
+    case YEE_FN_POLINOM3:
+    {
+      const double ez = (c * c) * ((x * t) * (x * t));
+      return comp == YEE_EZ ? ez : e0 * ez;
+    }
+    case YEE_FN_SIN1:
+      return comp == YEE_EZ ? c * sin (t - x) : -c * sin (t - x);
+    default:
+      return 0;
+  }
+}
+
+static double o_rhs_value (int kind, int comp, double x, double y, double z, double t)
+{
+  const double c = O_C, e0 = o_eps0 (), m0 = o_mu0 ();
+  switch (kind)
+  {
+    case YEE_FN_POLINOM1:
+      if (comp == YEE_EZ) return -(c * c) * e0 * (-2 * x * t + (x * x));
+      return -(c * c) * (-2 * x * t + (x * x) * e0 * m0);
+    case YEE_FN_POLINOM2:
+    {
+      /* Jx: 2 t (z - y) + y^2 + z^2; Jy: 2 t (x - z) + x^2 + z^2; Jz: 2 t (y - x) + x^2 + y^2 */
+      const double p[3] = { x, y, z };
+      const int a = comp % 3;
+      const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+      if (comp < 3)
+      {
+        const double d = a == 0 ? (p[2] - p[1]) : (a == 1 ? (p[0] - p[2]) : (p[1] - p[0]));
+        return -(c * c) * e0 * (2 * t * d + (p[u] * p[u]) + (p[v] * p[v]));
+      }
+      /* Mx: 2 t (y - z); My: 2 t (z - x); Mz: 2 t (x - y) */
+      const double d = a == 0 ? (p[1] - p[2]) : (a == 1 ? (p[2] - p[0]) : (p[0] - p[1]));
+      return -(c * c) * (2 * t * d + e0 * m0 * ((p[u] * p[u]) + (p[v] * p[v])));
+    }
+    case YEE_FN_POLINOM3:
+      if (comp == YEE_EZ) return -(c * c) * 2 * e0 * x * t * (x - t);
+      return -(c * c) * 2 * x * t * (e0 * m0 * x - t);
+    default:
+      return 0;
+  }
+}
+
 enum { LEFT = 0, RIGHT = 1, DOWN = 2, UP = 3, BACK = 4, FRONT = 5 };
 
 /*
@@ -84,6 +165,11 @@ struct oracle
   double src_t0, src_sigma; /* Gaussian envelope of the point source in time steps (sigma <= 0: none) */
   int fine;                /* --use-double-material-precision: material and sigma grids of extent mn */
   int64_t mn[3], mcell;
+  /* analytic test problems (YEE_FN_*): start values, border values, right side, exact solution */
+  int fn_start, fn_border, fn_rhs, fn_exact;
+  int64_t norm_lo[3], norm_hi[3];
+  double norm[6][4];       /* last diff norm per component: norm, percent, sum of squares, max |exact| */
+  int norm_valid[6];
 };
 
 static int64_t o_idx (const oracle *o, int64_t i, int64_t j, int64_t k)
@@ -796,7 +882,55 @@ int oracle_prepare (oracle *o)
           }
     }
   }
+  /* start values of an analytic test problem on every cell, E at dt / 2 and H at dt
+   * (Scheme::initGridWithInitialVals, Scheme.inc.h:2863-2958, called at :3255-3279) */
+  if (o->fn_start)
+    for (int c = 0; c < 6; ++c)
+    {
+      if (!o->present[c] || !o_fn_has (o->fn_start, c)) continue;
+      const double time = CD[c].is_h ? o->dt : 0.5 * o->dt;
+      for (int64_t i = 0; i < o->n[0]; ++i)
+        for (int64_t j = 0; j < o->n[1]; ++j)
+          for (int64_t k = 0; k < o->n[2]; ++k)
+          {
+            double r[3];
+            o_real_coord (o, c, i, j, k, r);
+            o->f[c][0][o_idx (o, i, j, k)] = o_fn_value (o->fn_start, c, r[0] * o->dx, r[1] * o->dx, r[2] * o->dx, time);
+          }
+    }
   o->prepared = 1;
+  return 0;
+}
+
+/* Analytic test problems (YEE_FN_* kinds; 0 = none) and the norm box: [1, N-1) on each axis of the
+ * scheme, an axis with a nonzero start / end taking it (--norm-start / --norm-end,
+ * Scheme.inc.h:1394-1420).  Rejects PML (the border and norm then act on D / E grids this
+ * restatement does not follow) and start values without border values (the reference then
+ * alternates its boundary cells between the start values and zero with its two time layers). */
+int oracle_set_test_functions (oracle *o, int start, int border, int rhs, int exact, const int64_t ns[3],
+                               const int64_t ne[3])
+{
+  if (!o || o->prepared) return -1;
+  if ((start || border || rhs || exact) && o->use_pml) return -1;
+  if (start && !border) return -1;
+  o->fn_start = start;
+  o->fn_border = border;
+  o->fn_rhs = rhs;
+  o->fn_exact = exact;
+  for (int a = 0; a < 3; ++a)
+  {
+    o->norm_lo[a] = o->act[a] ? 1 : 0;
+    o->norm_hi[a] = o->act[a] ? o->n[a] - 1 : 1;
+    if (o->act[a] && ns && ns[a] != 0) o->norm_lo[a] = ns[a];
+    if (o->act[a] && ne && ne[a] != 0) o->norm_hi[a] = ne[a];
+  }
+  return 0;
+}
+
+int oracle_get_diff_norm (oracle *o, int32_t comp, double out[4])
+{
+  if (!o || !out || comp < 0 || comp > 5 || !o->norm_valid[comp]) return -1;
+  for (int q = 0; q < 4; ++q) out[q] = o->norm[comp][q];
   return 0;
 }
 
@@ -944,8 +1078,17 @@ static void o_field_step (oracle *o, int comp, int32_t t)
           {
             o_tfsf (o, comp, i, j, k, &v11, &v12, &v21, &v22);
           }
-          /* calcField (InternalScheme.inc.h:765-772), right side J = 0 */
+          /* calcField (InternalScheme.inc.h:765-772); the right side J / M of an analytic test
+           * problem at the component's point, at time step t (E) or t + 1/2 (H)
+           * (Scheme.inc.h:1062-1098, InternalScheme.inc.h:2410-2413), else 0 */
           double rhs = 0.0;
+          if (o->fn_rhs && o_rhs_has (o->fn_rhs, comp))
+          {
+            double r[3];
+            o_real_coord (o, comp, i, j, k, r);
+            const double timestep = c->is_h ? (double) t + 0.5 : (double) t;
+            rhs = o_rhs_value (o->fn_rhs, comp, r[0] * dx, r[1] * dx, r[2] * dx, timestep * dt);
+          }
           double tmp = v12 - v11 - v22 + v21 - rhs * dx;
           out[q] = val * ca + tmp * cb;
         }
@@ -1036,6 +1179,53 @@ static void o_field_step (oracle *o, int comp, int32_t t)
             e0[q] = e1[q] * ca + cur[q] * cb - prv[q] * cc;
           }
     }
+  }
+
+  /* border values of an analytic test problem on every cell with an index 0 or N-1, also at
+   * t == 0 (calculateFieldStepIterationBorder, InternalScheme.inc.h:2646-2705; doSkipBorderFunc
+   * :539-555), at time step t + 1/2 (E) or t + 1 (H) */
+  if (o->fn_border && o_fn_has (o->fn_border, comp))
+  {
+    const double time = (c->is_h ? (double) t + 1.0 : (double) t + 0.5) * dt;
+    for (int64_t i = 0; i < o->n[0]; ++i)
+      for (int64_t j = 0; j < o->n[1]; ++j)
+        for (int64_t k = 0; k < o->n[2]; ++k)
+        {
+          const int64_t p[3] = { i, j, k };
+          int edge = 0;
+          for (int a = 0; a < 3; ++a) edge = edge || (o->act[a] && (p[a] == 0 || p[a] == o->n[a] - 1));
+          if (!edge) continue;
+          double r[3];
+          o_real_coord (o, comp, i, j, k, r);
+          o->f[comp][0][o_idx (o, i, j, k)] = o_fn_value (o->fn_border, comp, r[0] * dx, r[1] * dx, r[2] * dx, time);
+        }
+  }
+  /* difference norm against the exact solution over the norm box, in the iteration order of the
+   * reference (calculateFieldStepIterationExact, InternalScheme.inc.h:2707-2816; Scheme.inc.h:
+   * 1377-1550): sqrt (sum of squares / cells of the grid) and its percentage of max |exact| */
+  if (o->fn_exact && o_fn_has (o->fn_exact, comp))
+  {
+    const double time = (c->is_h ? (double) t + 1.0 : (double) t + 0.5) * dt;
+    double sum = 0.0, mx = 0.0;
+    const double *f0 = o->f[comp][0];
+    for (int64_t i = o->norm_lo[0]; i < o->norm_hi[0]; ++i)
+      for (int64_t j = o->norm_lo[1]; j < o->norm_hi[1]; ++j)
+        for (int64_t k = o->norm_lo[2]; k < o->norm_hi[2]; ++k)
+        {
+          double r[3];
+          o_real_coord (o, comp, i, j, k, r);
+          const double exact = o_fn_value (o->fn_exact, comp, r[0] * dx, r[1] * dx, r[2] * dx, time);
+          const double diff = exact - f0[o_idx (o, i, j, k)];
+          sum += diff * diff;
+          const double ea = fabs (exact);
+          if (mx < ea) mx = ea;
+        }
+    const double norm = sqrt (sum / (double) o->ncell);
+    o->norm[comp][0] = norm;
+    o->norm[comp][1] = norm * 100.0 / mx;
+    o->norm[comp][2] = sum;
+    o->norm[comp][3] = mx;
+    o->norm_valid[comp] = 1;
   }
 
   /* performPointSourceCalc (InternalScheme.inc.h:2818-2895), also at t == 0 */

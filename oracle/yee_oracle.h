@@ -46,6 +46,12 @@ int oracle_set_source_phase (oracle *o, double phase);
 /* Sources of the complex build: 1 its real part, 2 its imaginary part, 0 the real build. */
 int oracle_set_source_kind (oracle *o, int kind);
 int oracle_set_source_gaussian (oracle *o, double t0, double sigma);
+/* Analytic test problems (YEE_FN_* kinds for start values, border values, right side and the
+ * exact solution of the diff norm) and the norm box (0 entries: [1, N-1)); before prepare. */
+int oracle_set_test_functions (oracle *o, int start, int border, int rhs, int exact, const int64_t ns[3],
+                               const int64_t ne[3]);
+/* Diff norm of a component at the last step: norm, percent of max |exact|, sum of squares, max. */
+int oracle_get_diff_norm (oracle *o, int32_t comp, double out[4]);
 
 /* FP64 adjoint of the reference-form update (see yee_oracle.c): runs T steps from a freshly
  * prepared oracle, returns J = sum over the last `window` steps of sum of Ez^2 on the monitor

@@ -156,3 +156,41 @@ def random_material(cfg, seed, lo=1.0, hi=4.0):
     """A random material grid of the configuration's material shape."""
     rng = np.random.default_rng(seed)
     return rng.uniform(lo, hi, size=cfg.material_shape)
+
+
+# Analytic test problems of the reference's test suite (Tests/suite t2-t5: polinom1 in 3D, TMz and
+# TEy, polinom2 in 3D, polinom3 in 3D, TMz and TEy, sin1 with eps / mu normed), small sizes, plus a
+# norm box, a 1D line and a TF/SF run with a right side only.
+_FN = "--use-{k}-border-condition --use-{k}-start-values --use-{k}-right-side --calc-{k}-diff-norm"
+TEST_FN_CASES = [
+    "--3d --size x:12 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom1") + " --time-steps 21",
+    "--2d-tmz --size x:14 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom1") + " --time-steps 21",
+    "--2d-tey --size x:14 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom1") + " --time-steps 21",
+    "--3d --size x:12 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom2") + " --time-steps 21",
+    "--3d --size x:12 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom3") + " --time-steps 21",
+    "--2d-tmz --size x:14 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom3") + " --time-steps 21",
+    "--3d --size x:10 --same-size --angle-phi 0 --dx 0.001 --wavelength 0.02 --use-sin1-border-condition "
+    "--use-sin1-start-values --calc-sin1-diff-norm --eps-normed --mu-normed --courant-factor 149896229 --time-steps 21",
+    "--3d --size x:13,y:11,z:12 --angle-phi 0 --dx 0.001 --wavelength 0.02 " + _FN.format(k="polinom2")
+    + " --norm-start x:3,z:2 --norm-end y:8 --time-steps 15",
+    "--1d-ezhy --size x:40 --angle-teta 90 --angle-phi 0 --angle-psi 90 --dx 0.001 --wavelength 0.02 "
+    + _FN.format(k="polinom1") + " --time-steps 30",
+    "--3d --size x:16 --same-size --use-tfsf --tfsf-size-left x:4 --tfsf-size-right x:4 --same-size-tfsf "
+    "--angle-teta 30 --angle-phi 45 --angle-psi 90 --use-polinom2-right-side --time-steps 20",
+]
+
+
+def run_norms(eng, cfg):
+    """Steps `eng` (engine or oracle) through cfg.time_steps one at a time; returns the per-step
+    (norm, percent) of every component with an exact solution and the largest |exact|."""
+    out = {}
+    for _ in range(cfg.time_steps):
+        eng.step(1)
+        for n in cfg.components:
+            try:
+                v = eng.diff_norm(n)
+            except Exception:
+                continue
+            out.setdefault(n, []).append(v)
+    return {n: np.array(v) for n, v in out.items()}
+

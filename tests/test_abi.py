@@ -148,3 +148,19 @@ def test_create_rejects_invalid_configs(kw, msg):
     assert L.yee_create(ctypes.byref(c), ctypes.byref(h)) == -1
     assert msg in L.yee_last_error(), L.yee_last_error()
     assert not h.value
+
+
+def test_parse_test_function_options():
+    """--use-<kind>-* / --calc-<kind>-diff-norm take the first kind in the reference's priority
+    (polinom1, polinom2, polinom3, sin1); sin1 has no right side; --norm-start / --norm-end and
+    --eps-normed / --mu-normed are read."""
+    from fdtd3d_amd.config import OptionError, parse_args
+    cfg = parse_args("--3d --use-polinom3-border-condition --use-polinom2-border-condition --use-sin1-start-values "
+                     "--use-polinom1-right-side --calc-sin1-diff-norm --norm-start x:2,z:3 --norm-end y:7 "
+                     "--eps-normed --mu-normed".split())
+    assert (cfg.border_fn, cfg.start_fn, cfg.rhs_fn, cfg.exact_fn) == (2, 4, 1, 4)
+    assert cfg.norm_start == [2, 0, 3] and cfg.norm_end == [0, 7, 0]
+    assert cfg.eps_normed and cfg.mu_normed
+    for bad in ("--use-sin1-right-side", "--use-polinom1-diff-norm", "--calc-polinom1-border-condition"):
+        with pytest.raises(OptionError):
+            parse_args(["--3d", bad])

@@ -10,7 +10,7 @@ import pytest
 
 from fdtd3d_amd.config import parse_args
 from oracle.binding import Oracle, ref_driver_path, run_ref_driver, sphere_map
-from tests.helpers import DOUBLE_MATERIAL_CASES, field_names, random_material
+from tests.helpers import DOUBLE_MATERIAL_CASES, TEST_FN_CASES, field_names, random_material
 
 pytestmark = pytest.mark.skipif(not os.path.exists(ref_driver_path("f64")), reason="reference driver not built")
 
@@ -262,3 +262,22 @@ def test_double_material_precision_differs_from_the_coarse_average(tmp_path):
     assert fine["Eps"].size == 8 * coarse["Eps"].size
     d = np.linalg.norm(fine["Ey"] - coarse["Ey"]) / np.linalg.norm(coarse["Ey"])
     assert 1e-3 < d < 0.5, d
+
+
+@pytest.mark.parametrize("args", TEST_FN_CASES)
+def test_oracle_test_functions_match_reference_driver(args, tmp_path):
+    """Analytic test problems (start values, border values, right side, diff norm, eps / mu
+    normed): the oracle's fields and its per-step diff norms and percentages equal the reference
+    driver's (the reference's own callbacks, calculateFieldStepIterationBorder / Exact) bit for bit."""
+    from tests.helpers import run_norms
+    argv = args.split()
+    cfg = parse_args(argv)
+    rec, _ = run_ref_driver(argv, str(tmp_path / "r.bin"))
+    o = Oracle(cfg)
+    o.prepare()
+    norms = run_norms(o, cfg)
+    for name in field_names(cfg):
+        assert np.array_equal(o.field(name).ravel(), rec[name]), name
+    assert sorted("Norm" + n for n in norms) == sorted(k for k in rec if k.startswith("Norm"))
+    for n, v in norms.items():
+        assert np.array_equal(v[:, :2].ravel(), rec["Norm" + n]), n

@@ -1,0 +1,69 @@
+"""The reference test suite's analytic problems (Tests/suite t2-t5) against the oracle, with the
+suite's own pass criteria (its run.sh files), E_CENTERED layout:
+
+  t2.1 / t2.2 / t2.3  polinom1 in 3D / TMz / TEy: start values, border values, right side; the
+                      largest "DIFF NORM Ez" percentage over the run <= 1e-4 % (20 cells, dx 1 mm,
+                      101 steps; 40 cells, dx 0.5 mm, 201 steps)
+  t3                  polinom2 in 3D, every E component, same bar
+  t4.1 / t4.2 / t4.3  polinom3 in 3D / TMz / TEy, same bar
+  t5                  sin1 with eps / mu normed and Courant factor 149896229: the largest absolute
+                      Ez norm falls by more than 3.3 per halving of dx (10 / 20 / 40 cells over
+                      T + 1, 2 T + 1, 4 T + 1 steps, T = 40 and 80): second-order convergence
+
+The polynomial solutions are reproduced by the scheme exactly, up to rounding, so a wrong
+coordinate, time level, sign or operand order of a start value, border value or right side
+breaks these bars by orders of magnitude.
+"""
+import numpy as np
+import pytest
+
+from fdtd3d_amd.config import parse_args
+from oracle.binding import Oracle
+from tests.helpers import run_norms
+
+_FN = "--use-{k}-border-condition --use-{k}-start-values --use-{k}-right-side --calc-{k}-diff-norm"
+
+
+def _max_percent(scheme, kind, size, dx, steps, comps):
+    args = (f"{scheme} --size x:{size} --same-size --angle-phi 0 --dx {dx} --wavelength 0.02 "
+            + _FN.format(k=kind) + f" --time-steps {steps}")
+    cfg = parse_args(args.split())
+    o = Oracle(cfg)
+    o.prepare()
+    norms = run_norms(o, cfg)
+    o.close()
+    return max(float(norms[c][:, 1].max()) for c in comps)
+
+
+@pytest.mark.parametrize("scheme,kind,comps", [
+    ("--3d", "polinom1", ["Ez"]),          # t2.1
+    ("--2d-tmz", "polinom1", ["Ez"]),      # t2.2
+    ("--2d-tey", "polinom1", ["Ez"]),      # t2.3
+    ("--3d", "polinom2", ["Ex", "Ey", "Ez"]),   # t3
+    ("--3d", "polinom3", ["Ez"]),          # t4.1
+    ("--2d-tmz", "polinom3", ["Ez"]),      # t4.2
+    ("--2d-tey", "polinom3", ["Ez"]),      # t4.3
+])
+@pytest.mark.parametrize("size,dx,steps", [(20, "0.001", 101), (40, "0.0005", 201)])
+def test_polynomial_solution_reproduced(scheme, kind, comps, size, dx, steps):
+    assert _max_percent(scheme, kind, size, dx, steps, comps) <= 1e-4
+
+
+def _sin1_max(size, dx, steps):
+    args = (f"--3d --size x:{size} --same-size --angle-phi 0 --dx {dx} --wavelength 0.02 --use-sin1-border-condition "
+            f"--use-sin1-start-values --calc-sin1-diff-norm --eps-normed --mu-normed --courant-factor 149896229 "
+            f"--time-steps {steps}")
+    cfg = parse_args(args.split())
+    o = Oracle(cfg)
+    o.prepare()
+    norms = run_norms(o, cfg)
+    o.close()
+    return float(norms["Ez"][:, 0].max())
+
+
+@pytest.mark.parametrize("T", [40, 80])
+def test_sin1_second_order_convergence(T):
+    d1 = _sin1_max(10, "0.001", T + 1)
+    d2 = _sin1_max(20, "0.0005", 2 * T + 1)
+    d3 = _sin1_max(40, "0.00025", 4 * T + 1)
+    assert d1 / d2 > 3.3 and d2 / d3 > 3.3, (d1, d2, d3)
