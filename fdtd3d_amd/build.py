@@ -6,6 +6,7 @@ Outputs (git-ignored, shipped with the tree):
     fdtd3d_amd/libyee.so        C ABI of include/yee.h (HIP kernels + host set-up)
     fdtd3d_amd/bin/fdtd3d       CLI front end, FP64 fields (reference default value type)
     fdtd3d_amd/bin/fdtd3d-f32   CLI front end, FP32 fields
+    fdtd3d_amd/bin/fdtd3d-complex  CLI front end, FP64 complex fields (far field with --use-ntff)
 """
 from __future__ import annotations
 
@@ -74,9 +75,9 @@ def build_cli() -> None:
     src = os.path.join(CSRC, "fdtd3d_main.cpp")
     if not os.path.exists(src):
         return
-    for name, prec in (("fdtd3d", "8"), ("fdtd3d-f32", "4")):
+    for name, prec, cplx in (("fdtd3d", "8", "0"), ("fdtd3d-f32", "4", "0"), ("fdtd3d-complex", "8", "1")):
         out = os.path.join(BIN, name)
-        _run(["g++", "-O2", "-std=c++17", f"-DYEE_CLI_PRECISION={prec}", src, "-o", out,
+        _run(["g++", "-O2", "-std=c++17", f"-DYEE_CLI_PRECISION={prec}", f"-DYEE_CLI_COMPLEX={cplx}", src, "-o", out,
               f"-L{HERE}", "-lyee", "-Wl,-rpath,$ORIGIN/..", "-lm"])
 
 

@@ -117,6 +117,16 @@ class Config:
     # --eps-normed / --mu-normed: eps = 1 / eps0 / mu = 1 / mu0 everywhere (Scheme.inc.h:3016-3025, 3066-3075)
     eps_normed: bool = False
     mu_normed: bool = False
+    # near-to-far-field transform (Settings.inc.h:85-88, 124, 181-187): --use-ntff needs complex
+    # field values (Scheme::Pointing_scat asserts the complex build, Scheme.inc.h:4096-4098)
+    use_ntff: bool = False
+    ntff_size: List[int] = field(default_factory=lambda: [3, 3, 3])
+    ntff_diff: int = 1
+    ntff_step_angle: float = 10.0
+    ntff_filename: str = "ntff-res"
+    ntff_to_stdout: bool = False
+    ntff_reverse: bool = False
+    interm_ntff_step: int = 100
     point_source: List[str] = field(default_factory=list)   # component names, e.g. ["Ez"]
     point_source_pos: List[int] = field(default_factory=lambda: [0, 0, 0])
     # --current-source-j<x|y|z> / -m<x|y|z> as component names ("Ex" for Jx, "Hx" for Mx), --current-source
@@ -240,20 +250,18 @@ def _parse_coord(text: str, default: int) -> List[int]:
 
 # options that take one argument and are accepted but do not affect the Yee path
 _IGNORED_WITH_ARG = {
-    "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size",
-    "--cuda-buffer-size", "--cuda-gpus", "--num-cuda-threads",
-    "--ntff-filename", "--ntff-step-angle", "--interm-ntff-step", "--interm-save-step",
+    "--log-level", "--buffer-size", "--topology-size",
+    "--cuda-buffer-size", "--cuda-gpus", "--num-cuda-threads", "--interm-save-step",
     "--save-start-coord", "--save-end-coord", "--rebalance-step", "--stored-steps",
     "--save-cmd-to-file",
 }
 _IGNORED_FLAGS = {
-    "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
+    "--same-size-topology", "--manual-topology", "--use-cuda",
     "--save-res", "--save-res-per-process", "--save-materials", "--save-interm-res",
     "--save-scattered-field-res", "--save-scattered--field-interm", "--save-as-bmp", "--save-as-dat",
     "--save-as-txt", "--save-tfsf-e-incident", "--save-tfsf-h-incident", "--manual-save-coords",
     "--save-no-pml", "--palette-gray", "--palette-rgb", "--orth-axis-x", "--orth-axis-y",
     "--orth-axis-z", "--dynamic-grid", "--check-disabling-cond", "--check-enabling-cond",
-    "--ntff-reverse", "--ntff-to-stdout",
     # precomputed coefficient grids (Scheme.inc.h:3309-3791): the same expressions as the
     # per-cell path the engine computes, so the results are the same with or without them
     "--use-ca-cb", "--use-ca-cb-pml",
@@ -263,8 +271,6 @@ _UNSUPPORTED = {
     # the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
     # (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
     "--use-ca-cb-pml-metamaterials",
-    # the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
-    "--use-ntff",
 }
 
 
@@ -411,6 +417,24 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             cfg.norm_start = _parse_coord(val(), 0)
         elif a == "--norm-end":
             cfg.norm_end = _parse_coord(val(), 0)
+        elif a == "--use-ntff":
+            cfg.use_ntff = True
+        elif a == "--ntff-size":
+            cfg.ntff_size = _parse_coord(val(), 3)
+        elif a == "--same-size-ntff":
+            cfg.ntff_size = [cfg.ntff_size[0]] * 3
+        elif a == "--ntff-diff":
+            cfg.ntff_diff = int(val())
+        elif a == "--ntff-step-angle":
+            cfg.ntff_step_angle = stof(val())
+        elif a == "--ntff-filename":
+            cfg.ntff_filename = val()
+        elif a == "--ntff-to-stdout":
+            cfg.ntff_to_stdout = True
+        elif a == "--ntff-reverse":
+            cfg.ntff_reverse = True
+        elif a == "--interm-ntff-step":
+            cfg.interm_ntff_step = int(val())
         elif a == "--eps-normed":
             cfg.eps_normed = True
         elif a == "--mu-normed":

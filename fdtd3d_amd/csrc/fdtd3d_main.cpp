@@ -15,7 +15,10 @@
 // options that do not change the fields or the dumps) or rejected with an error (other
 // schemes and physics, image dumps).
 //
-// Built twice: fdtd3d (FP64 fields, the reference's default value type) and fdtd3d-f32.
+// Built three times: fdtd3d (FP64 fields, the reference's default value type), fdtd3d-f32 and
+// fdtd3d-complex (FP64 complex fields, the reference's -DCOMPLEX_FIELD_VALUES=ON build: text
+// dumps carry the real and imaginary parts, binary dumps both doubles, and --use-ntff writes the
+// far-field diagrams of Scheme::saveNTFF).
 #include <algorithm>
 #include <chrono>
 #include <cinttypes>
@@ -33,6 +36,9 @@
 
 #ifndef YEE_CLI_PRECISION
 #define YEE_CLI_PRECISION 8
+#endif
+#ifndef YEE_CLI_COMPLEX
+#define YEE_CLI_COMPLEX 0
 #endif
 
 namespace {
@@ -71,6 +77,12 @@ struct Options
   int fn[4] = { 0, 0, 0, 0 };
   int64_t norm_start[3] = { 0, 0, 0 }, norm_end[3] = { 0, 0, 0 };
   bool eps_normed = false, mu_normed = false;
+  // near-to-far-field transform (Settings.inc.h:85-88, 124, 181-187)
+  bool ntff = false, ntff_reverse = false, ntff_stdout = false;
+  int64_t ntff_size[3] = { 3, 3, 3 };
+  int64_t ntff_diff = 1, ntff_step = 100;
+  double ntff_angle_step = 10.0;
+  std::string ntff_file = "ntff-res";
 };
 
 // --use-<kind>-start-values / -border-condition / -right-side, --calc-<kind>-diff-norm: the slot
@@ -141,14 +153,13 @@ std::vector<std::string> read_cmd_file (const std::string &path)
 }
 
 const std::set<std::string> kIgnoredWithArg = {
-  "--log-level", "--ntff-size", "--ntff-diff", "--buffer-size", "--topology-size", "--cuda-buffer-size",
-  "--cuda-gpus", "--num-cuda-threads", "--ntff-filename", "--ntff-step-angle",
-  "--interm-ntff-step", "--rebalance-step", "--stored-steps", "--save-cmd-to-file",
+  "--log-level", "--buffer-size", "--topology-size", "--cuda-buffer-size",
+  "--cuda-gpus", "--num-cuda-threads", "--rebalance-step", "--stored-steps", "--save-cmd-to-file",
 };
 const std::set<std::string> kIgnoredFlags = {
-  "--same-size-ntff", "--same-size-topology", "--manual-topology", "--use-cuda",
+  "--same-size-topology", "--manual-topology", "--use-cuda",
   "--palette-gray", "--palette-rgb", "--dynamic-grid", "--check-disabling-cond",
-  "--check-enabling-cond", "--ntff-reverse", "--ntff-to-stdout",
+  "--check-enabling-cond",
   // precomputed coefficient grids (Scheme.inc.h:3309-3791): the same expressions as the per-cell
   // path the engine computes, so the results are the same with or without them
   "--use-ca-cb", "--use-ca-cb-pml",
@@ -157,8 +168,6 @@ const std::set<std::string> kUnsupported = {
   // the reference's precomputed Drude path reads a1 / a2 grids its binding never sets
   // (InternalScheme.inc.h:1902-2295, :2501-2502; Scheme.inc.h:932)
   "--use-ca-cb-pml-metamaterials",
-  // the far field (Scheme::Pointing_scat, Scheme.inc.h:4035-4096) needs complex fields
-  "--use-ntff",
 };
 
 Options parse (int argc, char **argv)
@@ -169,6 +178,7 @@ Options parse (int argc, char **argv)
   c.scheme = YEE_SCHEME_DIM3;
   c.layout = 0;
   c.precision = YEE_CLI_PRECISION;
+  if (YEE_CLI_COMPLEX) c.flags |= YEE_COMPLEX_FIELDS;
   c.time_steps = 100;
   for (int a = 0; a < 3; ++a)
   {
@@ -272,6 +282,15 @@ Options parse (int argc, char **argv)
     else if (a == "--save-as-dat") o.save.dat = true;
     else if (a == "--save-materials") o.save.materials = true;
     else if (a == "--save-as-bmp") o.save.refused.push_back (a);
+    else if (a == "--use-ntff") o.ntff = true;
+    else if (a == "--ntff-reverse") o.ntff_reverse = true;
+    else if (a == "--ntff-to-stdout") o.ntff_stdout = true;
+    else if (a == "--ntff-size") parse_coord (val (), 3, o.ntff_size);
+    else if (a == "--same-size-ntff") copy3 (o.ntff_size);
+    else if (a == "--ntff-diff") o.ntff_diff = std::stoll (val ());
+    else if (a == "--ntff-step-angle") o.ntff_angle_step = fp_option (val ());
+    else if (a == "--ntff-filename") o.ntff_file = val ();
+    else if (a == "--interm-ntff-step") o.ntff_step = std::stoll (val ());
     else done = false;
     if (done) continue;
     for (int k = 0; k < 6 && !done; ++k)
@@ -496,7 +515,7 @@ void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int
 //   (DATDumper.h:62, 129), so it writes no usable file; here the binary dump takes the name its
 //   text counterpart has, with the .dat extension.
 void write_dump (bool binary, const std::string &name, int layer, int64_t t, int dim, const int64_t size[3],
-                 const int64_t start[3], const int64_t end[3], const double *v)
+                 const int64_t start[3], const int64_t end[3], const double *v, const double *im = nullptr)
 {
   int64_t total = 1;
   for (int a = 0; a < dim; ++a) total *= size[a];
@@ -511,15 +530,21 @@ void write_dump (bool binary, const std::string &name, int layer, int64_t t, int
     for (int64_t y = (dim >= 2 ? start[1] : 0); y < (dim >= 2 ? end[1] : 1); ++y)
       for (int64_t z = (dim == 3 ? start[2] : 0); z < (dim == 3 ? end[2] : 1); ++z)
       {
-        const double val = v[(x * (dim >= 2 ? size[1] : 1) + y) * nz + z];
+        const int64_t idx = (x * (dim >= 2 ? size[1] : 1) + y) * nz + z;
+        const double val = v[idx];
         if (binary)
         {
-          const FileValue fv = (FileValue) val;
-          if (std::fwrite (&fv, sizeof fv, 1, f) != 1) die ("cannot write " + path);
+          // a complex FieldValue is its real and its imaginary part
+          const FileValue fv[2] = { (FileValue) val, (FileValue) (im ? im[idx] : 0.0) };
+          if (std::fwrite (fv, sizeof (FileValue), im ? 2 : 1, f) != (im ? 2u : 1u)) die ("cannot write " + path);
+          continue;
         }
-        else if (dim == 3) std::fprintf (f, "%lld %lld %lld %.15g\n", (long long) x, (long long) y, (long long) z, val);
-        else if (dim == 2) std::fprintf (f, "%lld %lld %.15g\n", (long long) x, (long long) y, val);
-        else std::fprintf (f, "%lld %.15g\n", (long long) x, val);
+        if (dim == 3) std::fprintf (f, "%lld %lld %lld ", (long long) x, (long long) y, (long long) z);
+        else if (dim == 2) std::fprintf (f, "%lld %lld ", (long long) x, (long long) y);
+        else std::fprintf (f, "%lld ", (long long) x);
+        // TXTDumper::writeToFile: "real imag" in the complex build
+        if (im) std::fprintf (f, "%.15g %.15g\n", val, im[idx]);
+        else std::fprintf (f, "%.15g\n", val);
       }
   if (std::fclose (f) != 0) die ("cannot write " + path);
   std::printf ("Saved. \n");
@@ -543,6 +568,8 @@ void save_materials (const Options &o, yee_handle *h)
   int64_t size[3];
   material_size (c, size);
   std::vector<double> v ((size_t) (size[0] * size[1] * size[2]));
+  // the complex build holds materials as complex values with a zero imaginary part
+  const std::vector<double> zero (YEE_CLI_COMPLEX ? v.size () : 0, 0.0);
   static const char *names[6] = { "Eps", "Mu", "OmegaPE", "OmegaPM", "GammaE", "GammaM" };
   const int count = (c.flags & YEE_USE_METAMATERIALS) ? 6 : 2;
   for (bool binary : dump_formats (o.save))
@@ -553,7 +580,7 @@ void save_materials (const Options &o, yee_handle *h)
       dump_range (o, -1, dim, size, start, end);
       for (int a = 0; a < dim; ++a)
         if (start[a] < 0 || end[a] > size[a] || start[a] >= end[a]) die ("dump coordinates out of the grid");
-      write_dump (binary, names[m], 0, 0, dim, size, start, end, v.data ());
+      write_dump (binary, names[m], 0, 0, dim, size, start, end, v.data (), YEE_CLI_COMPLEX ? zero.data () : nullptr);
     }
 }
 
@@ -569,28 +596,37 @@ void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
   scheme_size (c, size);
   const int64_t n = size[0] * size[1] * size[2];
   static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
-  std::vector<std::vector<double>> fields (6);
+  // real part, and with complex fields the imaginary part (its own handle)
+  yee_handle *parts[2] = { h, nullptr };
+  if (YEE_CLI_COMPLEX) check (yee_get_imag_part (h, &parts[1]));
+  const int nparts = YEE_CLI_COMPLEX ? 2 : 1;
+  std::vector<std::vector<double>> fields[2] = { std::vector<std::vector<double>> (6), std::vector<std::vector<double>> (6) };
   for (int comp = 0; comp < 6; ++comp)
   {
     if (dim < 3 && !scheme_has (c, comp)) continue;
-    fields[comp].resize ((size_t) n);
-    if (scattered && !o.save.per_process) check (yee_get_scattered_field (h, comp, fields[comp].data ()));
-    else check (yee_get_field (h, comp, fields[comp].data ()));
+    for (int p = 0; p < nparts; ++p)
+    {
+      fields[p][comp].resize ((size_t) n);
+      if (scattered && !o.save.per_process) check (yee_get_scattered_field (parts[p], comp, fields[p][comp].data ()));
+      else check (yee_get_field (parts[p], comp, fields[p][comp].data ()));
+    }
   }
-  std::vector<double> e, hv;
+  std::vector<double> e[2], hv[2];
   const int64_t len = 500 * size[0];   // allocateGridsInc: 500 * size.get1 ()
   const bool inc = (o.save.einc || o.save.hinc) && (c.flags & YEE_USE_TFSF);
   if (inc)
-  {
-    e.resize ((size_t) len);
-    hv.resize ((size_t) len);
-    check (yee_get_incident (h, e.data (), hv.data (), len));
-  }
+    for (int p = 0; p < nparts; ++p)
+    {
+      e[p].resize ((size_t) len);
+      hv[p].resize ((size_t) len);
+      check (yee_get_incident (parts[p], e[p].data (), hv[p].data (), len));
+    }
+  auto imag = [&] (const std::vector<double> &v) { return v.empty () ? nullptr : v.data (); };
   for (bool binary : dump_formats (o.save))
   {
     for (int comp = 0; comp < 6; ++comp)
     {
-      if (fields[comp].empty ()) continue;
+      if (fields[0][comp].empty ()) continue;
       int64_t start[3], end[3];
       if (o.save.per_process)
       {
@@ -606,14 +642,60 @@ void save_grids (const Options &o, yee_handle *h, int64_t t, bool scattered)
       }
       for (int a = 0; a < dim; ++a)
         if (start[a] < 0 || end[a] > size[a] || start[a] >= end[a]) die ("dump coordinates out of the grid");
-      write_dump (binary, names[comp], 1, t, dim, size, start, end, fields[comp].data ());
+      write_dump (binary, names[comp], 1, t, dim, size, start, end, fields[0][comp].data (), imag (fields[1][comp]));
     }
     if (inc)
     {
       const int64_t s1[3] = { len, 1, 1 }, b[3] = { 0, 0, 0 };
-      if (o.save.einc) write_dump (binary, "EInc", 1, t, 1, s1, b, s1, e.data ());
-      if (o.save.hinc) write_dump (binary, "HInc", 1, t, 1, s1, b, s1, hv.data ());
+      if (o.save.einc) write_dump (binary, "EInc", 1, t, 1, s1, b, s1, e[0].data (), imag (e[1]));
+      if (o.save.hinc) write_dump (binary, "HInc", 1, t, 1, s1, b, s1, hv[0].data (), imag (hv[1]));
     }
+  }
+}
+
+// Scheme::saveNTFF (Scheme.inc.h:4535-4640) at time step t: for each box step s of --ntff-diff,
+// the box [ntff-size + s, size - ntff-size + 1 - s] and one diagram, forward (phi from 0 while
+// phi <= 2 pi + pi / 180, by --ntff-step-angle degrees, added up in FPValue as the reference's loop
+// does) or, with --ntff-reverse, the single angle phi of the incident wave; every angle at the
+// incident wave's teta, in the file <--ntff-filename>_[timestep=t]_[step=s].txt or on stdout,
+// values printed as std::ostream does by default (%g).
+void save_ntff (const Options &o, yee_handle *h, int64_t t)
+{
+  const yee_config &c = o.cfg;
+  const double pi = 3.14159265358979323846;
+  const double ang[3] = { c.angle_deg[0] * pi / 180.0, c.angle_deg[1] * pi / 180.0, c.angle_deg[2] * pi / 180.0 };
+  std::vector<double> phi;
+  if (o.ntff_reverse)
+    phi.push_back (ang[1]);
+  else
+  {
+    const double end = 2 * pi + pi / 180, step = pi * o.ntff_angle_step / 180;
+    if (!(step > 0)) die ("--ntff-step-angle must be positive");
+    for (double a = 0.0; a <= end; a += step) phi.push_back (a);
+  }
+  std::vector<double> val (phi.size ());
+  for (int64_t s = 0; s < o.ntff_diff; ++s)
+  {
+    int64_t left[3], right[3];
+    for (int a = 0; a < 3; ++a)
+    {
+      left[a] = o.ntff_size[a] + s;
+      right[a] = c.size[a] - o.ntff_size[a] + 1 - s;
+    }
+    check (yee_ntff (h, ang[0], phi.data (), (int32_t) phi.size (), left, right, val.data ()));
+    FILE *f = stdout;
+    std::string path;
+    if (!o.ntff_stdout)
+    {
+      path = o.ntff_file + "_[timestep=" + std::to_string (t) + "]_[step=" + std::to_string (s) + "].txt";
+      f = std::fopen (path.c_str (), "w");
+      if (!f) die ("cannot write " + path);
+    }
+    std::fprintf (f, "==== NTFF (step=%lld) ====\n%s\n\n", (long long) s, o.ntff_reverse ? "Reverse diagram" : "Forward diagram");
+    for (size_t i = 0; i < phi.size (); ++i)
+      std::fprintf (f, "timestep = %lld, incident wave angle=(%g,%g,%g,), angle NTFF = %g, NTFF value = %g\n", (long long) t,
+                    ang[0], ang[1], ang[2], phi[i], val[i]);
+    if (f != stdout && std::fclose (f) != 0) die ("cannot write " + path);
   }
 }
 
@@ -689,41 +771,51 @@ int main (int argc, char **argv)
     die ("--save-scattered-field-res cannot be combined with --save-res-per-process");
   if (dumping && ((sv.res && sv.scat_res) || (sv.interm && sv.scat_interm)) && !(c.flags & YEE_USE_TFSF))
     die ("scattered-field dumps need --use-tfsf");
-  const auto t0 = std::chrono::steady_clock::now ();
-  if (o.fn[3])
+  if (o.ntff)
   {
-    // one step at a time: after each, the "DIFF NORM" line of every component with an exact
-    // solution (Scheme.inc.h:1543-1549), E before H as the reference computes them
-    static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
-    for (int64_t done = 0; done < c.time_steps; ++done)
-    {
-      check (yee_step (h, 1));
-      for (int comp = 0; comp < 6; ++comp)
-      {
-        double nv[4];
-        if (yee_get_diff_norm (h, comp, nv) != YEE_OK) continue;
-        std::printf ("-> DIFF NORM %s. Timestep %" PRId64 ". Value = ( %.20f ) = ( %.20f %% )\n", names[comp], done, nv[0],
-                     nv[1]);
-      }
-      if (dumping && sv.interm && (done + 1) % sv.interm_step == 0) save_grids (o, h, done + 1, sv.scat_interm);
-    }
+    // Scheme::Pointing_scat asserts the complex build (Scheme.inc.h:4096-4098)
+    if (!YEE_CLI_COMPLEX)
+      die ("--use-ntff: Solver is not compiled with support of complex values (use fdtd3d-complex, the "
+           "-DCOMPLEX_FIELD_VALUES=ON build)");
+    if (o.ntff_step <= 0) die ("--interm-ntff-step must be positive");
   }
-  else if (dumping && sv.interm)
+  const auto t0 = std::chrono::steady_clock::now ();
   {
-    // SchemeHelper::performNSteps3D (SchemeHelper.h:100-118): after step t, save when
-    // t / step < (t + 1) / step, as time step t + 1
+    // SchemeHelper::performNSteps3D (SchemeHelper.h:100-118): after step t, the NTFF and then the
+    // intermediate grids are saved when t / step < (t + 1) / step, as time step t + 1; with an
+    // exact solution every step prints the "DIFF NORM" line of every component that has one
+    // (Scheme.inc.h:1543-1549), E before H as the reference computes them.  Steps between
+    // these events run as one yee_step.
+    static const char *names[6] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz" };
+    const bool interm = dumping && sv.interm;
     int64_t done = 0;
     while (done < c.time_steps)
     {
-      const int64_t next = std::min<int64_t> (c.time_steps, (done / sv.interm_step + 1) * sv.interm_step);
-      check (yee_step (h, (int32_t) (next - done)));
+      int64_t next = o.fn[3] ? done + 1 : c.time_steps;
+      if (interm) next = std::min<int64_t> (next, (done / sv.interm_step + 1) * sv.interm_step);
+      if (o.ntff) next = std::min<int64_t> (next, (done / o.ntff_step + 1) * o.ntff_step);
+      const bool ntff_here = o.ntff && next % o.ntff_step == 0;
+      if (ntff_here)
+      {
+        // the N sums average H over the last two steps: keep it one step before
+        if (next - 1 > done) check (yee_step (h, (int32_t) (next - 1 - done)));
+        check (yee_ntff_snapshot (h));
+        check (yee_step (h, 1));
+      }
+      else
+        check (yee_step (h, (int32_t) (next - done)));
       done = next;
-      if (done % sv.interm_step == 0) save_grids (o, h, done, sv.scat_interm);
+      if (o.fn[3])
+        for (int comp = 0; comp < 6; ++comp)
+        {
+          double nv[4];
+          if (yee_get_diff_norm (h, comp, nv) != YEE_OK) continue;
+          std::printf ("-> DIFF NORM %s. Timestep %" PRId64 ". Value = ( %.20f ) = ( %.20f %% )\n", names[comp], done - 1,
+                       nv[0], nv[1]);
+        }
+      if (ntff_here) save_ntff (o, h, done);
+      if (interm && done % sv.interm_step == 0) save_grids (o, h, done, sv.scat_interm);
     }
-  }
-  else
-  {
-    check (yee_step (h, c.time_steps));
   }
   if (dumping && sv.res) save_grids (o, h, c.time_steps, sv.scat_res);
   check (yee_sync (h));

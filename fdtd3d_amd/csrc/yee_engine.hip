@@ -36,6 +36,7 @@ template <typename T, int COMP, int DIM> __global__ void scattered_kernel (StepA
 template <typename T, int BY, int MAT, int MAXW> __global__ void step_3d_fused (StepArgs<T>);
 template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 template <typename T> __global__ void test_fill (StepArgs<T>, T *, int, int, double, int);
+template <typename T> __global__ void ntff_faces (NtffArgs<T>);
 template <typename T>
 __global__ void test_norm (StepArgs<T>, const T *, int, int, double, int, int, int, int, int, int, double *);
 template <typename T>
@@ -846,6 +847,57 @@ int upload_tfsf_geom (yee_handle *h)
   return YEE_OK;
 }
 
+int ntff_snapshot_part (yee_handle *h)
+{
+  for (int c = 0; c < 3; ++c)
+  {
+    if (!h->ntff_h[c])
+    {
+      int rc = alloc_grid (h, &h->ntff_h[c]);
+      if (rc) return rc;
+    }
+    HIP_TRY (hipMemcpyAsync (h->ntff_h[c], h->f[h->cur][3 + c], (size_t) h->ncell * h->wsize, hipMemcpyDeviceToDevice,
+                             h->stream));
+  }
+  h->ntff_t = h->t;
+  return YEE_OK;
+}
+
+int ntff_check (const yee_handle *h)
+{
+  if (!h->imag) return fail (YEE_ENOTSUP, "the near-to-far-field transform needs complex field values (YEE_COMPLEX_FIELDS)");
+  if (h->dim != 3 || h->miss >= 0)
+    return fail (YEE_ENOTSUP, "the near-to-far-field transform runs on 3D schemes only (the reference's 1D / 2D "
+                              "variants are unreachable)");
+  for (int a = 0; a < 3; ++a)
+    if (h->n[a] % 2) return fail (YEE_ENOTSUP, "the near-to-far-field transform needs even grid sizes");
+  return YEE_OK;
+}
+
+template <typename T>
+NtffArgs<T> ntff_args (const yee_handle *h, double teta, const int64_t left[3], const int64_t right[3])
+{
+  NtffArgs<T> a {};
+  for (int c = 0; c < 3; ++c)
+  {
+    a.e_re[c] = (const T *) h->f[h->cur][c];
+    a.e_im[c] = (const T *) h->imag->f[h->imag->cur][c];
+    a.h_re[c] = (const T *) h->f[h->cur][3 + c];
+    a.h_im[c] = (const T *) h->imag->f[h->imag->cur][3 + c];
+    a.hp_re[c] = (const T *) h->ntff_h[c];
+    a.hp_im[c] = (const T *) h->imag->ntff_h[c];
+    a.n[c] = (int) h->n[c];
+    a.left[c] = (int) left[c];
+    a.right[c] = (int) right[c];
+  }
+  a.sx = h->n[1] * h->n[2];
+  a.sy = (int) h->n[2];
+  a.dx = h->cfg.dx;
+  a.k = 2 * kPi / h->cfg.wavelength;
+  a.teta = teta;
+  return a;
+}
+
 }  // namespace
 
 namespace yee {
@@ -1079,6 +1131,7 @@ void yee_destroy (yee_handle *h)
   for (hipEvent_t e : h->ev)
     if (e) (void) hipEventDestroy (e);
   if (h->norm_part) (void) hipFree (h->norm_part);
+  if (h->ntff_buf) (void) hipFree (h->ntff_buf);
   if (h->stream) (void) hipStreamDestroy (h->stream);
   delete h;
 }
@@ -1391,6 +1444,86 @@ int yee_get_diff_norm (yee_handle *h, int32_t comp, double out[4])
   out[1] = norm * 100.0 / mx;
   out[2] = sum;
   out[3] = mx;
+  return YEE_OK;
+}
+
+int yee_ntff_snapshot (yee_handle *h)
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  int rc = ntff_check (h);
+  if (rc) return rc;
+  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  if ((rc = ntff_snapshot_part (h)) || (rc = ntff_snapshot_part (h->imag))) return rc;
+  return YEE_OK;
+}
+
+int yee_ntff (yee_handle *h, double teta, const double *phi, int32_t nphi, const int64_t left[3], const int64_t right[3],
+              double *out)
+{
+  if (!h || !phi || !left || !right || !out) return fail (YEE_EINVAL, "null argument");
+  int rc = ntff_check (h);
+  if (rc) return rc;
+  if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
+  if (nphi <= 0 || nphi > 65536) return fail (YEE_EINVAL, "angle count must be in 1 .. 65536");
+  for (int a = 0; a < 3; ++a)
+    if (left[a] < 2 || right[a] > h->n[a] - 2 || right[a] <= left[a])
+      return fail (YEE_EINVAL, "the box must satisfy 2 <= left < right <= size - 2 along every axis");
+  if (h->t < 1 || h->ntff_t != h->t - 1 || h->imag->ntff_t != h->t - 1)
+    return fail (YEE_ESTATE, "yee_ntff_snapshot must be taken one step before (the N sums average H over the "
+                             "newest two steps)");
+  const int64_t need = (int64_t) nphi * (1 + 12 * 4);
+  if (need > h->ntff_cap)
+  {
+    if (h->ntff_buf) HIP_TRY (hipFree (h->ntff_buf));
+    h->ntff_buf = nullptr;
+    h->ntff_cap = 0;
+    HIP_TRY (hipMalloc ((void **) &h->ntff_buf, (size_t) need * sizeof (double)));
+    h->ntff_cap = need;
+  }
+  // both parts' state is final once both streams drained
+  HIP_TRY (hipStreamSynchronize (h->imag->stream));
+  HIP_TRY (hipMemcpyAsync (h->ntff_buf, phi, (size_t) nphi * sizeof (double), hipMemcpyHostToDevice, h->stream));
+  double *part = h->ntff_buf + nphi;
+  const dim3 grid ((unsigned) nphi * 12);
+  if (h->fp32 ())
+  {
+    NtffArgs<float> a = ntff_args<float> (h, teta, left, right);
+    a.phis = h->ntff_buf;
+    a.part = part;
+    hipLaunchKernelGGL ((ntff_faces<float>), grid, dim3 (256), 0, h->stream, a);
+  }
+  else
+  {
+    NtffArgs<double> a = ntff_args<double> (h, teta, left, right);
+    a.phis = h->ntff_buf;
+    a.part = part;
+    hipLaunchKernelGGL ((ntff_faces<double>), grid, dim3 (256), 0, h->stream, a);
+  }
+  HIP_TRY (hipGetLastError ());
+  std::vector<double> host ((size_t) nphi * 48);
+  HIP_TRY (hipMemcpyAsync (host.data (), part, host.size () * sizeof (double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  const double e0 = eps0 (), m0 = mu0 ();
+  const double n0 = std::sqrt (m0 / e0);
+  const double k = 2 * kPi / h->cfg.wavelength;
+  for (int32_t i = 0; i < nphi; ++i)
+  {
+    // teta / phi sums of N and L, faces in the order ntffN / ntffL add them (x, y, z; left, right)
+    double nt[2] = { 0, 0 }, np[2] = { 0, 0 }, lt[2] = { 0, 0 }, lp[2] = { 0, 0 };
+    for (int ax = 0; ax < 3; ++ax)
+      for (int kind = 0; kind < 2; ++kind)
+        for (int side = 0; side < 2; ++side)
+        {
+          const double *q = host.data () + ((size_t) i * 12 + kind * 6 + ax * 2 + side) * 4;
+          double *t = kind ? lt : nt, *f = kind ? lp : np;
+          t[0] += q[0]; t[1] += q[1]; f[0] += q[2]; f[1] += q[3];
+        }
+    // Scheme::Pointing_scat (Scheme.inc.h:4085-4100) over Pointing_inc (:4103-4109)
+    const double f1r = lp[0] + nt[0] * n0, f1i = lp[1] + nt[1] * n0;
+    const double f2r = lt[0] - np[0] * n0, f2i = lt[1] - np[1] * n0;
+    const double scat = (k * k) / (8 * kPi * n0) * ((f1r * f1r + f1i * f1i) + (f2r * f2r + f2i * f2i));
+    out[i] = scat / (std::sqrt (e0 / m0) / 2);
+  }
   return YEE_OK;
 }
 

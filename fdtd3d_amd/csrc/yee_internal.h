@@ -102,6 +102,14 @@ struct yee_handle
   double *norm_part = nullptr;
   bool norm_valid[6] = {};
   bool fn_any () const { return fn_start || fn_border || fn_rhs || fn_exact; }
+
+  // near-to-far-field transform: H of the step before the newest (yee_ntff_snapshot; each part's
+  // handle holds its own), the step it was taken at, and the device buffer of the angles and
+  // the per-face sums of yee_ntff
+  void *ntff_h[3] = {};
+  int32_t ntff_t = -1;
+  double *ntff_buf = nullptr;
+  int64_t ntff_cap = 0;
   bool complex () const { return (cfg.flags & YEE_COMPLEX_FIELDS) != 0; }
 };
 

@@ -94,4 +94,19 @@ struct StepArgs
   double fn_dx[3];
 };
 
+// Near-to-far-field transform (yee_ntff): the complex fields' two parts, the snapshot of H before
+// the last step, the box and the angles.
+template <typename T>
+struct NtffArgs
+{
+  const T *e_re[3], *e_im[3], *h_re[3], *h_im[3], *hp_re[3], *hp_im[3];
+  int n[3];
+  int64_t sx;
+  int sy;
+  int left[3], right[3];
+  double dx, k, teta;
+  const double *phis;
+  double *part;   // [angle][12 face passes][4]: teta sum (re, im), phi sum (re, im)
+};
+
 }  // namespace yee

@@ -1742,6 +1742,142 @@ __global__ __launch_bounds__ (256) void test_norm (StepArgs<T> a, const T *f, in
   }
 }
 
+// Near-to-far-field transform of the complex build (Scheme::Pointing_scat, Scheme.inc.h:4036-4101,
+// over the face sums of SchemeHelper::ntffN3D_x/y/z and ntffL3D_x/y/z, SchemeHelper.h:1030-2148,
+// E_CENTERED).  Workgroup (angle a, pass p): pass p / 6 = N (surface currents from H averaged
+// over its last two time layers) or L (from E averaged across the face), axis (p % 6) / 2, side
+// p % 2 (left / right plane of the box); the workgroup's threads stride over the face cells
+// [left + 1/2, right - 1/2] of the two in-plane axes and reduce the complex teta and phi sums.
+struct Cplx
+{
+  double re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ Cplx ntff_at (const T *re, const T *im, const NtffArgs<T> &a, int c, const double p[3])
+{
+  // minimum coordinates of the component (E_CENTERED, YeeGridLayout.cpp:1962-1969)
+  const int axis = c % 3;
+  const bool h = c >= 3;
+  int64_t q[3];
+  for (int d = 0; d < 3; ++d) q[d] = (int64_t) (p[d] - (((d == axis) != h) ? 1.0 : 0.5));
+  const int64_t idx = q[0] * a.sx + q[1] * a.sy + q[2];
+  return Cplx { (double) gld (re + idx), (double) gld (im + idx) };
+}
+
+template <typename T>
+__global__ __launch_bounds__ (256) void ntff_faces (NtffArgs<T> a)
+{
+  __shared__ double red[4][256];
+  const int angle = (int) blockIdx.x / 12, pass = (int) blockIdx.x % 12;
+  const int kind = pass / 6, ax = (pass % 6) / 2, side = pass % 2;
+  const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+  const int p0 = side ? a.right[ax] : a.left[ax];
+  const double sign = side ? 1.0 : -1.0;   // x0 == right ? 1 : -1 (left == right cannot occur)
+  const double teta = a.teta, phi = a.phis[angle];
+  const double st = sin (teta), ct = cos (teta), sp = sin (phi), cp = cos (phi);
+  const double diff0[3] = { (double) (a.n[0] / 2), (double) (a.n[1] / 2), (double) (a.n[2] / 2) };
+  // the two components of each face (N: H, L: E) and the extra -1 of the teta / phi sums
+  const int c1 = kind == 0 ? (ax == 2 ? 4 : 5) : (ax == 0 ? 1 : 0);
+  const int c2 = kind == 0 ? (ax == 0 ? 4 : 3) : (ax == 2 ? 1 : 2);
+  const double ts = kind == 0 ? (ax == 0 ? -1.0 : 1.0) : (ax == 1 ? -1.0 : 1.0);
+  const double fs = kind == 0 ? (ax == 2 ? 1.0 : -1.0) : (ax == 2 ? -1.0 : 1.0);
+  const int nu = a.right[u] - a.left[u], nv = a.right[v] - a.left[v];
+  double sum[4] = { 0, 0, 0, 0 };
+  for (int w = (int) threadIdx.x; w < nu * nv; w += (int) blockDim.x)
+  {
+    double base[3];
+    base[ax] = (double) p0;
+    base[u] = a.left[u] + 0.5 + (double) (w / nv);
+    base[v] = a.left[v] + 0.5 + (double) (w % nv);
+    double ps[4][3];
+    for (int q = 0; q < 4; ++q)
+      for (int d = 0; d < 3; ++d) ps[q][d] = base[d];
+    ps[0][u] -= 0.5; ps[1][u] += 0.5; ps[2][v] -= 0.5; ps[3][v] += 0.5;
+    Cplx f[4];
+    for (int q = 0; q < 4; ++q)
+    {
+      const int c = q < 2 ? c1 : c2;
+      if (kind == 0)
+      {
+        // H averaged over the newest two time layers
+        const Cplx x = ntff_at (a.h_re[c - 3], a.h_im[c - 3], a, c, ps[q]);
+        const Cplx y = ntff_at (a.hp_re[c - 3], a.hp_im[c - 3], a, c, ps[q]);
+        f[q] = Cplx { (x.re + y.re) / 2, (x.im + y.im) / 2 };
+      }
+      else
+      {
+        // E averaged across the face
+        double lo[3] = { ps[q][0], ps[q][1], ps[q][2] }, hi[3] = { ps[q][0], ps[q][1], ps[q][2] };
+        lo[ax] -= 0.5;
+        hi[ax] += 0.5;
+        const Cplx x = ntff_at (a.e_re[c], a.e_im[c], a, c, lo);
+        const Cplx y = ntff_at (a.e_re[c], a.e_im[c], a, c, hi);
+        f[q] = Cplx { (x.re + y.re) / 2, (x.im + y.im) / 2 };
+      }
+    }
+    const Cplx A { (f[0].re + f[1].re) / 2, (f[0].im + f[1].im) / 2 };
+    const Cplx B { (f[2].re + f[3].re) / 2, (f[2].im + f[3].im) / 2 };
+    double arg = (base[0] - diff0[0]) * st * cp + (base[1] - diff0[1]) * st * sp + (base[2] - diff0[2]) * ct;
+    arg *= a.dx;
+    const double er = cos (a.k * arg), ei = sin (a.k * arg);
+    double tr, ti, fr, fi;   // the teta and phi integrands before the phase
+    if (ax == 0)
+    {
+      if (kind == 0)
+      {
+        tr = A.re * ct * sp + B.re * st; ti = A.im * ct * sp + B.im * st;
+        fr = A.re * cp; fi = A.im * cp;
+      }
+      else
+      {
+        tr = B.re * ct * sp + A.re * st; ti = B.im * ct * sp + A.im * st;
+        fr = B.re * cp; fi = B.im * cp;
+      }
+    }
+    else if (ax == 1)
+    {
+      if (kind == 0)
+      {
+        tr = A.re * ct * cp + B.re * st; ti = A.im * ct * cp + B.im * st;
+        fr = A.re * sp; fi = A.im * sp;
+      }
+      else
+      {
+        tr = B.re * ct * cp + A.re * st; ti = B.im * ct * cp + A.im * st;
+        fr = B.re * sp; fi = B.im * sp;
+      }
+    }
+    else
+    {
+      if (kind == 0)
+      {
+        tr = -A.re * ct * cp + B.re * ct * sp; ti = -A.im * ct * cp + B.im * ct * sp;
+        fr = A.re * sp + B.re * cp; fi = A.im * sp + B.im * cp;
+      }
+      else
+      {
+        tr = B.re * ct * cp - A.re * ct * sp; ti = B.im * ct * cp - A.im * ct * sp;
+        fr = B.re * sp + A.re * cp; fi = B.im * sp + A.im * cp;
+      }
+    }
+    const double s2 = a.dx * a.dx;
+    sum[0] += (tr * er - ti * ei) * s2 * ts * sign;
+    sum[1] += (tr * ei + ti * er) * s2 * ts * sign;
+    sum[2] += (fr * er - fi * ei) * s2 * fs * sign;
+    sum[3] += (fr * ei + fi * er) * s2 * fs * sign;
+  }
+  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = sum[q];
+  __syncthreads ();
+  for (int w = 128; w > 0; w >>= 1)
+  {
+    if ((int) threadIdx.x < w)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads ();
+  }
+  if (threadIdx.x < 4) gst (a.part + ((size_t) angle * 12 + pass) * 4 + threadIdx.x, red[threadIdx.x][0]);
+}
+
 // TF/SF predicate mask of one component for every cell (set-up check export).
 // Scattered field of one component (Scheme::makeGridScattered, Scheme.inc.h:4113-4228): the
 // newest field minus the incident plane wave at the component's grid points inside the TF/SF
@@ -1846,6 +1982,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void point_source<T> (T *, int64_t, double);                         \
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *); \
   template __global__ void test_fill<T> (StepArgs<T>, T *, int, int, double, int); \
+  template __global__ void ntff_faces<T> (NtffArgs<T>); \
   template __global__ void test_norm<T> (StepArgs<T>, const T *, int, int, double, int, int, int, int, int, int, double *); \
   template __global__ void fine_average<T> (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *); \
   template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \

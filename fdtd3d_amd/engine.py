@@ -14,6 +14,7 @@ The binding refuses to run without the native library: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from typing import Dict, Optional
 
@@ -29,7 +30,7 @@ EXPORTED = [
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
     "yee_get_scattered_field_noalloc", "yee_get_imag_part", "yee_material_size", "yee_set_test_functions",
-    "yee_get_diff_norm",
+    "yee_get_diff_norm", "yee_ntff_snapshot", "yee_ntff",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
     "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
@@ -87,6 +88,8 @@ def load_library():
     L.yee_material_size.argtypes = [vp, vp]
     L.yee_set_test_functions.argtypes = [vp, i32, i32, i32, i32, vp, vp]
     L.yee_get_diff_norm.argtypes = [vp, i32, vp]
+    L.yee_ntff_snapshot.argtypes = [vp]
+    L.yee_ntff.argtypes = [vp, ctypes.c_double, vp, i32, vp, vp, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
     L.yee_get_scalars.argtypes = [vp, vp]
@@ -236,6 +239,21 @@ class Engine:
         out = (ctypes.c_double * 4)()
         _check(load_library().yee_get_diff_norm(self.h, COMP_INDEX[name], out))
         return tuple(out)
+
+    def ntff_snapshot(self) -> None:
+        """Keeps H of both parts for the near-to-far-field transform one step later."""
+        _check(load_library().yee_ntff_snapshot(self.h))
+
+    def ntff(self, teta: float, phis, left, right) -> np.ndarray:
+        """Pointing_scat / Pointing_inc at elevation teta and each azimuth of phis (radians) over
+        the box [left, right] (Scheme::saveNTFF, Scheme.inc.h:4535-4640); needs ntff_snapshot
+        one step before."""
+        phi = np.ascontiguousarray(np.atleast_1d(np.asarray(phis, dtype=np.float64)))
+        lo = (ctypes.c_int64 * 3)(*[int(v) for v in left])
+        hi = (ctypes.c_int64 * 3)(*[int(v) for v in right])
+        out = np.zeros(phi.size)
+        _check(load_library().yee_ntff(self.h, float(teta), phi.ctypes.data, int(phi.size), lo, hi, out.ctypes.data))
+        return out
 
     def step(self, n: int) -> None:
         _check(load_library().yee_step(self.h, int(n)))
@@ -426,3 +444,34 @@ def run_engine(cfg: Config, materials: Optional[Dict[str, np.ndarray]] = None, s
     eng.step(cfg.time_steps if steps is None else steps)
     eng.sync()
     return eng
+
+
+def ntff_angles(cfg: Config) -> np.ndarray:
+    """Azimuths (radians) of one NTFF diagram (Scheme::saveNTFF, Scheme.inc.h:4561-4605): the
+    incident wave's phi with --ntff-reverse, else 0, step, 2 step, ... while <= 2 pi + pi / 180,
+    step = pi * --ntff-step-angle / 180, added up one by one as the reference's loop does."""
+    if cfg.ntff_reverse:
+        return np.array([cfg.angle_deg[1] * math.pi / 180.0])
+    end, step = 2 * math.pi + math.pi / 180, math.pi * cfg.ntff_step_angle / 180
+    if not step > 0:
+        raise ValueError("--ntff-step-angle must be positive")
+    out, a = [], 0.0
+    while a <= end:
+        out.append(a)
+        a += step
+    return np.array(out)
+
+
+def ntff_diagrams(eng: "Engine", cfg: Config):
+    """The diagrams Scheme::saveNTFF writes at the engine's current step, one per box step s of
+    --ntff-diff: (s, left, right, phis, values), box [ntff_size + s, size - ntff_size + 1 - s],
+    elevation the incident wave's teta.  The engine must have taken ntff_snapshot one step before."""
+    teta = cfg.angle_deg[0] * math.pi / 180.0
+    phis = ntff_angles(cfg)
+    out = []
+    for s in range(cfg.ntff_diff):
+        left = [cfg.ntff_size[a] + s for a in range(3)]
+        right = [cfg.size[a] - cfg.ntff_size[a] + 1 - s for a in range(3)]
+        out.append((s, left, right, phis, eng.ntff(teta, phis, left, right)))
+    return out
+

@@ -251,6 +251,22 @@ int yee_set_test_functions (yee_handle *h, int32_t start_values, int32_t border,
  */
 int yee_get_diff_norm (yee_handle *h, int32_t comp, double out[4]);
 
+/*
+ * Near-to-far-field transform (--use-ntff) of a complex 3D handle at its newest step, as
+ * Scheme::saveNTFF computes one diagram (Scheme.inc.h:4535-4640): for each of the nphi angles
+ * phi[i] (radians) at the elevation teta (radians; the reference passes the incident wave's
+ * teta), out[i] = Pointing_scat / Pointing_inc (Scheme.inc.h:4036-4110) over the box
+ * [left, right] (the reference's leftNTFF / rightNTFF: size - left + 1 on its right side), whose
+ * six faces sum the surface currents N (from H averaged over the newest two steps) and L (from E
+ * averaged across the face), SchemeHelper::ntffN3D / ntffL3D (SchemeHelper.h:1030-2148).
+ * yee_ntff_snapshot keeps the H of the step before: call it, step once, then call yee_ntff
+ * (YEE_ESTATE otherwise).  YEE_ENOTSUP without YEE_COMPLEX_FIELDS, for 1D / 2D schemes (the
+ * reference's are unreachable) and odd sizes; YEE_EINVAL unless 2 <= left < right <= size - 2.
+ */
+int yee_ntff_snapshot (yee_handle *h);
+int yee_ntff (yee_handle *h, double teta, const double *phi, int32_t nphi, const int64_t left[3],
+              const int64_t right[3], double *out);
+
 /* Waits for all enqueued work of the handle. */
 int yee_sync (yee_handle *h);
 

@@ -51,6 +51,9 @@ def lib():
         L.oracle_set_source_gaussian.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.oracle_set_test_functions.argtypes = [vp] + [ctypes.c_int] * 4 + [vp, vp]
         L.oracle_get_diff_norm.argtypes = [vp, ctypes.c_int32, vp]
+        L.oracle_ntff.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, vp, vp, vp]
+        L.oracle_scale_fields.argtypes = [vp, ctypes.c_double]
+        L.oracle_scale_fields.restype = None
         L.oracle_adjoint.argtypes = [vp, ctypes.POINTER(OracleAdjSpec), ctypes.c_int32,
                                      ctypes.POINTER(ctypes.c_double), vp]
         d = ctypes.c_double
@@ -197,6 +200,10 @@ class Oracle:
             raise KeyError(name)
         return tuple(out)
 
+    def scale_fields(self, s: float) -> None:
+        """Multiplies every E / H value of every time layer by s."""
+        lib().oracle_scale_fields(self.h, float(s))
+
     def step(self, n: int) -> None:
         if lib().oracle_step(self.h, int(n)):
             raise RuntimeError("oracle_step")
@@ -287,3 +294,15 @@ def run_ref_driver(args, out_path: str, kind: str = "f64", timeout: float = 600)
         if line.startswith("Total time ="):
             secs = float(line.split("=")[1].split()[0])
     return read_ref_output(out_path), secs
+
+
+def oracle_ntff(re: "Oracle", im: "Oracle", teta: float, phi: float, left, right):
+    """(Pointing_scat / Pointing_inc, Pointing_scat) of the complex build at (teta, phi) from the
+    real / imaginary part oracles (run_oracle_complex) over the box faces left, right."""
+    lo = (ctypes.c_int64 * 3)(*[int(v) for v in left])
+    hi = (ctypes.c_int64 * 3)(*[int(v) for v in right])
+    out = (ctypes.c_double * 2)()
+    if lib().oracle_ntff(re.h, im.h, float(teta), float(phi), lo, hi, out):
+        raise ValueError("oracle_ntff rejected the box")
+    return out[0], out[1]
+

@@ -927,6 +927,183 @@ int oracle_set_test_functions (oracle *o, int start, int border, int rhs, int ex
   return 0;
 }
 
+/*
+ * Near-to-far-field transform of the complex build (Scheme::Pointing_scat, Scheme.inc.h:4036-4101,
+ * over the faces of SchemeHelper::ntffN3D / ntffL3D, SchemeHelper.h:1030-2148, E_CENTERED): the
+ * real and imaginary parts come from two oracles (re, im) stepped in lock step.  N sums the
+ * surface currents from H averaged over its newest two time layers, L from E averaged across the
+ * face; both over the cells [left + 1/2, right - 1/2] of each face of the box, with the phase
+ * exp (i k r.n) of the cell's offset from the grid centre.  Returns Pointing_scat / Pointing_inc.
+ */
+typedef struct { double re, im; } o_cplx;
+
+static o_cplx o_cadd (o_cplx a, o_cplx b) { o_cplx r = { a.re + b.re, a.im + b.im }; return r; }
+static o_cplx o_cscale (o_cplx a, double s) { o_cplx r = { a.re * s, a.im * s }; return r; }
+static o_cplx o_cmul (o_cplx a, o_cplx b) { o_cplx r = { a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re }; return r; }
+
+/* minimum coordinates, E_CENTERED (YeeGridLayout.cpp:1962-1969) */
+static const double kMinCoord[6][3] = { { 1.0, 0.5, 0.5 }, { 0.5, 1.0, 0.5 }, { 0.5, 0.5, 1.0 },
+                                        { 0.5, 1.0, 1.0 }, { 1.0, 0.5, 1.0 }, { 1.0, 1.0, 0.5 } };
+
+/* value of component c at real position p (integer after subtracting the minimum coordinate) of
+ * layer `layer` as a complex number */
+static o_cplx o_at (const oracle *re, const oracle *im, int c, const double p[3], int layer)
+{
+  int64_t q[3];
+  for (int a = 0; a < 3; ++a) q[a] = (int64_t) (p[a] - kMinCoord[c][a]);
+  const int64_t idx = o_idx (re, q[0], q[1], q[2]);
+  o_cplx v = { re->f[c][layer][idx], im->f[c][layer][idx] };
+  return v;
+}
+
+/* N (kind 0) or L (kind 1) of face axis `ax` at plane p0 */
+static void o_ntff_face (const oracle *re, const oracle *im, int kind, int ax, int64_t p0, double teta, double phi,
+                         const int64_t left[3], const int64_t right[3], o_cplx *sum_teta, o_cplx *sum_phi)
+{
+  const double dx = re->dx;
+  const double diff0[3] = { (double) (re->n[0] / 2), (double) (re->n[1] / 2), (double) (re->n[2] / 2) };
+  const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;   /* the two in-plane axes, in order */
+  const double sign = p0 == right[ax] ? 1.0 : -1.0;
+  const double k = 2 * O_PI / re->lambda;
+  /* components: N: H (Hz/Hy on x, Hz/Hx on y, Hy/Hx on z); L: E likewise */
+  static const int kComp[2][3][2] = { { { YEE_HZ, YEE_HY }, { YEE_HZ, YEE_HX }, { YEE_HY, YEE_HX } },
+                                      { { YEE_EY, YEE_EZ }, { YEE_EX, YEE_EZ }, { YEE_EX, YEE_EY } } };
+  const int c1 = kComp[kind][ax][0], c2 = kComp[kind][ax][1];
+  o_cplx st = { 0, 0 }, sp = { 0, 0 };
+  for (double cu = left[u] + 0.5; cu <= right[u] - 0.5; ++cu)
+    for (double cv = left[v] + 0.5; cv <= right[v] - 0.5; ++cv)
+    {
+      double base[3];
+      base[ax] = (double) p0;
+      base[u] = cu;
+      base[v] = cv;
+      /* pos1 / pos2: -/+ 1/2 along u (first component), pos3 / pos4: -/+ 1/2 along v (second) */
+      double p1[3] = { base[0], base[1], base[2] }, p2[3] = { base[0], base[1], base[2] };
+      double p3[3] = { base[0], base[1], base[2] }, p4[3] = { base[0], base[1], base[2] };
+      p1[u] -= 0.5; p2[u] += 0.5; p3[v] -= 0.5; p4[v] += 0.5;
+      o_cplx a1, a2, b1, b2;
+      if (kind == 0)
+      {
+        /* H averaged over the newest two time layers (CurrentAfterShift: layer 1;
+         * PreviousAfterShift: layer 2 of three, else layer 0; Grid.h:1184-1208) */
+        const int pl = re->layers > 2 ? 2 : 0;
+        a1 = o_cscale (o_cadd (o_at (re, im, c1, p1, 1), o_at (re, im, c1, p1, pl)), 1.0 / 2);
+        a2 = o_cscale (o_cadd (o_at (re, im, c1, p2, 1), o_at (re, im, c1, p2, pl)), 1.0 / 2);
+        b1 = o_cscale (o_cadd (o_at (re, im, c2, p3, 1), o_at (re, im, c2, p3, pl)), 1.0 / 2);
+        b2 = o_cscale (o_cadd (o_at (re, im, c2, p4, 1), o_at (re, im, c2, p4, pl)), 1.0 / 2);
+        /* the reference divides by 2 (FPValue (2)); / 2 and * 1/2 are the same in binary FP */
+      }
+      else
+      {
+        /* E averaged across the face (-/+ 1/2 along the face normal) */
+        double q[4][2][3];
+        const double *ps[4] = { p1, p2, p3, p4 };
+        for (int w = 0; w < 4; ++w)
+          for (int s = 0; s < 2; ++s)
+          {
+            for (int a = 0; a < 3; ++a) q[w][s][a] = ps[w][a];
+            q[w][s][ax] += s ? 0.5 : -0.5;
+          }
+        a1 = o_cscale (o_cadd (o_at (re, im, c1, q[0][0], 1), o_at (re, im, c1, q[0][1], 1)), 1.0 / 2);
+        a2 = o_cscale (o_cadd (o_at (re, im, c1, q[1][0], 1), o_at (re, im, c1, q[1][1], 1)), 1.0 / 2);
+        b1 = o_cscale (o_cadd (o_at (re, im, c2, q[2][0], 1), o_at (re, im, c2, q[2][1], 1)), 1.0 / 2);
+        b2 = o_cscale (o_cadd (o_at (re, im, c2, q[3][0], 1), o_at (re, im, c2, q[3][1], 1)), 1.0 / 2);
+      }
+      const double r[3] = { base[0] - diff0[0], base[1] - diff0[1], base[2] - diff0[2] };
+      double arg = r[0] * sin (teta) * cos (phi) + r[1] * sin (teta) * sin (phi) + r[2] * cos (teta);
+      arg *= dx;
+      const o_cplx ex = { cos (k * arg), sin (k * arg) };
+      const o_cplx A = o_cscale (o_cadd (a1, a2), 1.0 / 2), B = o_cscale (o_cadd (b1, b2), 1.0 / 2);
+      o_cplx t, f;
+      double ts, fs;   /* extra factor -1 of the teta / phi sums */
+      if (ax == 0)
+      {
+        /* N_x: A = Hz, B = Hy; L_x: A = Ey, B = Ez */
+        if (kind == 0)
+        {
+          t = o_cadd (o_cscale (o_cscale (A, cos (teta)), sin (phi)), o_cscale (B, sin (teta)));
+          f = o_cscale (A, cos (phi));
+          ts = -1; fs = -1;
+        }
+        else
+        {
+          t = o_cadd (o_cscale (o_cscale (B, cos (teta)), sin (phi)), o_cscale (A, sin (teta)));
+          f = o_cscale (B, cos (phi));
+          ts = 1; fs = 1;
+        }
+      }
+      else if (ax == 1)
+      {
+        /* N_y: A = Hz, B = Hx; L_y: A = Ex, B = Ez */
+        if (kind == 0)
+        {
+          t = o_cadd (o_cscale (o_cscale (A, cos (teta)), cos (phi)), o_cscale (B, sin (teta)));
+          f = o_cscale (A, sin (phi));
+          ts = 1; fs = -1;
+        }
+        else
+        {
+          t = o_cadd (o_cscale (o_cscale (B, cos (teta)), cos (phi)), o_cscale (A, sin (teta)));
+          f = o_cscale (B, sin (phi));
+          ts = -1; fs = 1;
+        }
+      }
+      else
+      {
+        /* N_z: A = Hy, B = Hx; L_z: A = Ex, B = Ey */
+        if (kind == 0)
+        {
+          t = o_cadd (o_cscale (o_cscale (o_cscale (A, -1.0), cos (teta)), cos (phi)), o_cscale (o_cscale (B, cos (teta)), sin (phi)));
+          f = o_cadd (o_cscale (A, sin (phi)), o_cscale (B, cos (phi)));
+          ts = 1; fs = 1;
+        }
+        else
+        {
+          t = o_cadd (o_cscale (o_cscale (B, cos (teta)), cos (phi)), o_cscale (o_cscale (o_cscale (A, cos (teta)), sin (phi)), -1.0));
+          f = o_cadd (o_cscale (B, sin (phi)), o_cscale (A, cos (phi)));
+          ts = 1; fs = -1;
+        }
+      }
+      st = o_cadd (st, o_cscale (o_cscale (o_cscale (o_cmul (t, ex), dx * dx), ts), sign));
+      sp = o_cadd (sp, o_cscale (o_cscale (o_cscale (o_cmul (f, ex), dx * dx), fs), sign));
+    }
+  *sum_teta = o_cadd (*sum_teta, st);
+  *sum_phi = o_cadd (*sum_phi, sp);
+}
+
+int oracle_ntff (const oracle *re, const oracle *im, double teta, double phi, const int64_t left[3],
+                 const int64_t right[3], double *out)
+{
+  if (!re || !im || !out || re->dim != 3 || re->layers < 2) return -1;
+  for (int a = 0; a < 3; ++a)
+    if (re->n[a] % 2 || left[a] < 2 || right[a] > re->n[a] - 2 || right[a] <= left[a]) return -1;
+  o_cplx nt = { 0, 0 }, np = { 0, 0 }, lt = { 0, 0 }, lp = { 0, 0 };
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    o_ntff_face (re, im, 0, ax, left[ax], teta, phi, left, right, &nt, &np);
+    o_ntff_face (re, im, 0, ax, right[ax], teta, phi, left, right, &nt, &np);
+    o_ntff_face (re, im, 1, ax, left[ax], teta, phi, left, right, &lt, &lp);
+    o_ntff_face (re, im, 1, ax, right[ax], teta, phi, left, right, &lt, &lp);
+  }
+  const double n0 = sqrt (o_mu0 () / o_eps0 ());
+  const double k = 2 * O_PI / re->lambda;
+  const o_cplx first = o_cadd (lp, o_cscale (nt, n0));
+  const o_cplx second = o_cadd (lt, o_cscale (np, -n0));
+  const double a1 = first.re * first.re + first.im * first.im, a2 = second.re * second.re + second.im * second.im;
+  const double scat = (k * k) / (8 * O_PI * n0) * (a1 + a2);
+  out[0] = scat / (sqrt (o_eps0 () / o_mu0 ()) / 2);   /* Pointing_inc (Scheme.inc.h:4103-4109) */
+  out[1] = scat;
+  return 0;
+}
+
+void oracle_scale_fields (oracle *o, double s)
+{
+  for (int c = 0; c < 6; ++c)
+    for (int l = 0; l < o->layers; ++l)
+      if (o->f[c][l])
+        for (int64_t i = 0; i < o->ncell; ++i) o->f[c][l][i] *= s;
+}
+
 int oracle_get_diff_norm (oracle *o, int32_t comp, double out[4])
 {
   if (!o || !out || comp < 0 || comp > 5 || !o->norm_valid[comp]) return -1;

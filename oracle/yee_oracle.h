@@ -50,6 +50,13 @@ int oracle_set_source_gaussian (oracle *o, double t0, double sigma);
  * exact solution of the diff norm) and the norm box (0 entries: [1, N-1)); before prepare. */
 int oracle_set_test_functions (oracle *o, int start, int border, int rhs, int exact, const int64_t ns[3],
                                const int64_t ne[3]);
+/* Near-to-far-field value of the complex build at (teta, phi): the real and imaginary parts are two
+ * oracles stepped together; out[0] = Pointing_scat / Pointing_inc, out[1] = Pointing_scat. */
+int oracle_ntff (const oracle *re, const oracle *im, double teta, double phi, const int64_t left[3],
+                 const int64_t right[3], double *out);
+/* Multiplies every E / H value of every time layer by s (a test of the far field conjugates the
+ * complex fields by scaling the imaginary part's oracle by -1). */
+void oracle_scale_fields (oracle *o, double s);
 /* Diff norm of a component at the last step: norm, percent of max |exact|, sum of squares, max. */
 int oracle_get_diff_norm (oracle *o, int32_t comp, double out[4]);
 

@@ -64,10 +64,34 @@ def test_parse_rejects_unknown_and_unsupported():
     with pytest.raises(OptionError):
         parse_args(["--3d", "--use-metamaterials"])
     with pytest.raises(OptionError):
-        parse_args(["--3d", "--use-ntff"])   # the far field needs complex fields
-    with pytest.raises(OptionError):
         # the reference's precomputed Drude path dereferences a1 / a2 grids it never binds
         parse_args("--3d --use-pml --use-metamaterials --use-ca-cb-pml-metamaterials".split())
+
+
+def test_parse_ntff_options():
+    """--use-ntff and its settings (Settings.inc.h:85-88, 124, 181-187): defaults, --same-size-ntff,
+    and the diagram's angles as Scheme::saveNTFF's loop adds them up."""
+    import math
+    from fdtd3d_amd.engine import ntff_angles
+    d = parse_args(["--3d", "--use-ntff"])
+    assert d.use_ntff and d.ntff_size == [3, 3, 3] and d.ntff_diff == 1 and d.ntff_step_angle == 10.0
+    assert d.ntff_filename == "ntff-res" and d.interm_ntff_step == 100 and not d.ntff_reverse and not d.ntff_to_stdout
+    c = parse_args("--3d --use-ntff --ntff-size x:5,y:4 --same-size-ntff --ntff-diff 3 --ntff-step-angle 7.5 "
+                   "--ntff-filename far --ntff-to-stdout --ntff-reverse --interm-ntff-step 20".split())
+    assert c.ntff_size == [5, 5, 5] and c.ntff_diff == 3 and c.ntff_step_angle == 7.5
+    assert c.ntff_filename == "far" and c.ntff_to_stdout and c.ntff_reverse and c.interm_ntff_step == 20
+    assert list(ntff_angles(c)) == [c.angle_deg[1] * math.pi / 180.0]
+    fwd = ntff_angles(d)
+    assert len(fwd) == 37 and fwd[0] == 0.0 and abs(fwd[-1] - 2 * math.pi) < 1e-12
+    step = math.pi * 10.0 / 180
+    acc = 0.0
+    for a in fwd:
+        assert a == acc
+        acc += step
+    # 1 degree: 361 additions of pi / 180 stay 4.6e-14 below 2 pi + pi / 180, so the loop also
+    # prints that angle (362 lines)
+    one = ntff_angles(parse_args("--3d --ntff-step-angle 1".split()))
+    assert len(one) == 362 and one[-1] < 2 * math.pi + math.pi / 180
 
 
 def test_parse_accepts_precomputed_coefficient_flags():
