@@ -45,6 +45,8 @@ COMPLEX_FIELDS = 0x8
 DOUBLE_MATERIAL_PRECISION = 0x10
 # analytic test problems (include/yee.h YEE_FN_*), in the priority order of Scheme::initCallBacks
 TEST_FUNCTIONS = {"polinom1": 1, "polinom2": 2, "polinom3": 3, "sin1": 4}
+# exact solutions of a 1D pair (--calc-exp<n>-<pair>-diff-norm): YEE_FN_EXP1 .. EXP3
+EXP_FUNCTIONS = {"exp1": 5, "exp2": 6, "exp3": 7}
 
 COMPONENTS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz",
               "Dx", "Dy", "Dz", "Bx", "By", "Bz",
@@ -119,6 +121,10 @@ class Config:
     mu_normed: bool = False
     # near-to-far-field transform (Settings.inc.h:85-88, 124, 181-187): --use-ntff needs complex
     # field values (Scheme::Pointing_scat asserts the complex build, Scheme.inc.h:4096-4098)
+    # --calc-exp<n>-<pair>-diff-norm: exp kind (EXP_FUNCTIONS value, 0 = none) and the pair's
+    # 1D scheme value (SCHEME_EXHY .. SCHEME_EZHY)
+    exp_fn: int = 0
+    exp_pair: int = 0
     use_ntff: bool = False
     ntff_size: List[int] = field(default_factory=lambda: [3, 3, 3])
     ntff_diff: int = 1
@@ -158,6 +164,13 @@ class Config:
     @property
     def dim(self) -> int:
         return 3 if self.scheme == SCHEME_DIM3 else (2 if self.scheme >= SCHEME_TEX else 1)
+
+    @property
+    def exp_border(self) -> float:
+        """The material border coordinate of the exp solutions: the eps sphere centre minus its
+        radius along the pair's line axis (CallBack.cpp:297-760)."""
+        axis = [v[1] for v in SCHEMES_1D.values() if v[0] == self.exp_pair][0]
+        return float(self.eps_sphere_center[axis] - self.eps_sphere_radius)
 
     @property
     def active_axes(self):
@@ -276,6 +289,7 @@ _UNSUPPORTED = {
 
 # --use-<kind>-start-values / -border-condition / -right-side, --calc-<kind>-diff-norm
 # (Settings.inc.h:134-153; sin1 has no right side)
+_EXP_FN = re.compile(r"^--calc-(exp1|exp2|exp3)-(exhy|exhz|eyhx|eyhz|ezhx|ezhy)-diff-norm$")
 _TEST_FN = re.compile(r"^--(use|calc)-(polinom1|polinom2|polinom3|sin1)-"
                       r"(start-values|border-condition|right-side|diff-norm)$")
 
@@ -417,6 +431,14 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
             cfg.norm_start = _parse_coord(val(), 0)
         elif a == "--norm-end":
             cfg.norm_end = _parse_coord(val(), 0)
+        elif _EXP_FN.match(a):
+            m = _EXP_FN.match(a)
+            kind, pair = EXP_FUNCTIONS[m.group(1)], SCHEMES_1D["--1d-" + m.group(2)][0]
+            if cfg.exp_fn and cfg.exp_pair != pair:
+                raise OptionError("exact solutions of two different 1D pairs are not supported")
+            # Scheme::initCallBacks takes exp1 before exp2 before exp3 within a pair
+            cfg.exp_fn = kind if cfg.exp_fn == 0 else min(cfg.exp_fn, kind)
+            cfg.exp_pair = pair
         elif a == "--use-ntff":
             cfg.use_ntff = True
         elif a == "--ntff-size":

@@ -77,6 +77,8 @@ struct Options
   int fn[4] = { 0, 0, 0, 0 };
   int64_t norm_start[3] = { 0, 0, 0 }, norm_end[3] = { 0, 0, 0 };
   bool eps_normed = false, mu_normed = false;
+  // --calc-exp<n>-<pair>-diff-norm: YEE_FN_EXP1 .. EXP3 and the pair's YEE_SCHEME_EXHY .. EZHY
+  int exp_kind = 0, exp_pair = 0;
   // near-to-far-field transform (Settings.inc.h:85-88, 124, 181-187)
   bool ntff = false, ntff_reverse = false, ntff_stdout = false;
   int64_t ntff_size[3] = { 3, 3, 3 };
@@ -258,6 +260,22 @@ Options parse (int argc, char **argv)
     else if (a == "--norm-end") parse_coord (val (), 0, o.norm_end);
     else if (a == "--eps-normed") o.eps_normed = true;
     else if (a == "--mu-normed") o.mu_normed = true;
+    else if (a.rfind ("--calc-exp", 0) == 0 && a.size () == 26 && a.compare (16, 10, "-diff-norm") == 0
+             && a[10] >= '1' && a[10] <= '3' && a[11] == '-')
+    {
+      // --calc-exp<n>-<pair>-diff-norm (Settings.inc.h:153-175)
+      static const char *pairs[6] = { "exhy", "exhz", "eyhx", "eyhz", "ezhx", "ezhy" };
+      const std::string pr = a.substr (12, 4);
+      int pair = -1;
+      for (int q = 0; q < 6; ++q)
+        if (pr == pairs[q]) pair = YEE_SCHEME_EXHY + q;
+      if (pair < 0) die ("unknown option " + a);
+      if (o.exp_kind && o.exp_pair != pair) die ("exact solutions of two different 1D pairs are not supported");
+      const int kind = YEE_FN_EXP1 + (a[10] - '1');
+      // Scheme::initCallBacks takes exp1 before exp2 before exp3 within a pair
+      o.exp_kind = o.exp_kind ? std::min (o.exp_kind, kind) : kind;
+      o.exp_pair = pair;
+    }
     else if (int kind = 0, slot = test_flag (a, kind); slot >= 0)
     {
       // Scheme::initCallBacks takes the first of polinom1, polinom2, polinom3, sin1
@@ -717,6 +735,14 @@ int main (int argc, char **argv)
   check (yee_create (&c, &h));
   if (o.fn[0] || o.fn[1] || o.fn[2] || o.fn[3])
     check (yee_set_test_functions (h, o.fn[0], o.fn[1], o.fn[2], o.fn[3], o.norm_start, o.norm_end));
+  if (o.exp_kind)
+  {
+    // the dielectric border of the exp solutions: eps sphere centre - radius along the pair's line
+    static const int kAxis[6] = { 2, 1, 2, 0, 1, 0 };
+    const int ax = kAxis[o.exp_pair - YEE_SCHEME_EXHY];
+    const double border = (double) (o.sphere[0].center[ax] - o.sphere[0].radius);
+    check (yee_set_exact_exp (h, o.exp_kind, o.exp_pair, border, o.norm_start, o.norm_end));
+  }
   double sc[3];
   check (yee_get_scalars (h, sc));
   const double two_pi_f = 2 * 3.14159265358979323846 * sc[1];
@@ -791,7 +817,8 @@ int main (int argc, char **argv)
     int64_t done = 0;
     while (done < c.time_steps)
     {
-      int64_t next = o.fn[3] ? done + 1 : c.time_steps;
+      const bool exact = o.fn[3] || o.exp_kind;
+      int64_t next = exact ? done + 1 : c.time_steps;
       if (interm) next = std::min<int64_t> (next, (done / sv.interm_step + 1) * sv.interm_step);
       if (o.ntff) next = std::min<int64_t> (next, (done / o.ntff_step + 1) * o.ntff_step);
       const bool ntff_here = o.ntff && next % o.ntff_step == 0;
@@ -805,9 +832,19 @@ int main (int argc, char **argv)
       else
         check (yee_step (h, (int32_t) (next - done)));
       done = next;
-      if (o.fn[3])
+      if (exact)
         for (int comp = 0; comp < 6; ++comp)
         {
+          if (YEE_CLI_COMPLEX)
+          {
+            // the complex build's line (Scheme.inc.h:1537-1542)
+            double nv[9];
+            if (yee_get_diff_norm_complex (h, comp, nv) != YEE_OK) continue;
+            std::printf ("-> DIFF NORM %s. Timestep %" PRId64 ". Value = ( %.20f , %.20f ) = ( %.20f %% , %.20f %% ), "
+                         "module = %.20f = ( %.20f %% )\n",
+                         names[comp], done - 1, nv[0], nv[1], nv[3], nv[4], nv[2], nv[5]);
+            continue;
+          }
           double nv[4];
           if (yee_get_diff_norm (h, comp, nv) != YEE_OK) continue;
           std::printf ("-> DIFF NORM %s. Timestep %" PRId64 ". Value = ( %.20f ) = ( %.20f %% )\n", names[comp], done - 1,

@@ -400,6 +400,62 @@ __device__ __forceinline__ double test_rhs (int kind, int comp, double x, double
   return 0.0;
 }
 
+// Exact solutions of the 1D plane wave meeting a dielectric half-space (Tests/suite t7.x;
+// CallBack.cpp:280-760): kind YEE_FN_EXP1 i e^{i arg}, EXP2 -i/3 e^{i arg}, EXP3 2i/3 e^{i arg},
+// arg = k (coord - d dx) - w (t - t0) with the host-computed k and phase origin d; H = E times
+// (k / (mu0 w)).  The complex operations of the reference's CComplex (i * exponent (i arg) =
+// (0 c - 1 s, 1 c + 0 s)), then the sign of the pair's wrapper.  Per pair (ExHy, ExHz, EyHx,
+// EyHz, EzHx, EzHy): the line axis, the E and H components and their signs.
+__device__ __forceinline__ int exp_axis (int pair)
+{
+  return pair == 0 || pair == 2 ? 2 : (pair == 1 || pair == 4 ? 1 : 0);
+}
+
+__device__ __forceinline__ bool exp_defines (int pair, int comp)
+{
+  const int e = pair < 2 ? YEE_EX : (pair < 4 ? YEE_EY : YEE_EZ);
+  const int h = pair == 0 || pair == 5 ? YEE_HY : (pair == 1 || pair == 3 ? YEE_HZ : YEE_HX);
+  return comp == e || comp == h;
+}
+
+template <typename T>
+__device__ __forceinline__ void test_exp (const StepArgs<T> &a, int kind, int comp, const double r[3], double t,
+                                          double &re, double &im)
+{
+  const int axis = exp_axis (a.exp_pair);
+  const double arg = a.exp_k * (r[axis] - a.exp_d * a.fn_dx[axis]) - a.exp_w * (t - a.exp_t0);
+  const double c = cos (arg), s = sin (arg);
+  if (kind == YEE_FN_EXP1)
+  {
+    re = 0.0 * c - 1.0 * s;
+    im = 1.0 * c + 0.0 * s;
+  }
+  else if (kind == YEE_FN_EXP2)
+  {
+    // (-i) (c, s) times 1, over 3
+    re = ((-0.0 * c - (-1.0) * s) * 1.0) / 3.0;
+    im = ((-1.0 * c + -0.0 * s) * 1.0) / 3.0;
+  }
+  else
+  {
+    re = ((0.0 * c - 1.0 * s) * 2.0) / 3.0;
+    im = ((1.0 * c + 0.0 * s) * 2.0) / 3.0;
+  }
+  if (comp >= 3)
+  {
+    re = re * a.exp_m;
+    im = im * a.exp_m;
+  }
+  // signs of the wrappers: ExHy (-, -), ExHz (+, -), EyHx (-, +), EyHz (-, -), EzHx (+, +), EzHy (+, -)
+  const int p = a.exp_pair;
+  const bool neg = comp >= 3 ? (p == 0 || p == 1 || p == 3 || p == 5) : (p == 0 || p == 2 || p == 3);
+  if (neg)
+  {
+    re = -re;
+    im = -im;
+  }
+}
+
 // The real coordinate (m) of component COMP at cell p: (p + the component's minimum coordinate,
 // E_CENTERED: 1 along an E component's own axis and 0.5 across it, the reverse for H) * fn_dx.
 template <typename T>

@@ -38,7 +38,7 @@ template <typename T> __global__ void tfsf_ztable (StepArgs<T>, double *);
 template <typename T> __global__ void test_fill (StepArgs<T>, T *, int, int, double, int);
 template <typename T> __global__ void ntff_faces (NtffArgs<T>);
 template <typename T>
-__global__ void test_norm (StepArgs<T>, const T *, int, int, double, int, int, int, int, int, int, double *);
+__global__ void test_norm (StepArgs<T>, const T *, const T *, int, int, double, int, int, int, int, int, int, double *);
 template <typename T>
 __global__ void fine_average (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *);
 
@@ -192,6 +192,13 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   a.rhs_kind = h->fn_rhs;
   a.rhs_time = (role == kRoleH ? (double) h->t + 0.5 : (double) h->t) * h->sc.dt;
   for (int ax = 0; ax < 3; ++ax) a.fn_dx[ax] = axis_active (h->cfg, ax) ? h->cfg.dx : 0.0;
+  a.exp_pair = h->exp_pair;
+  a.exp_part = h->src_kind == 2 ? 1 : 0;
+  a.exp_k = h->exp_k;
+  a.exp_d = h->exp_d;
+  a.exp_w = h->exp_w;
+  a.exp_t0 = h->exp_t0;
+  a.exp_m = h->exp_m;
   return a;
 }
 
@@ -342,9 +349,19 @@ int launch_sources (yee_handle *h, bool e_half, int32_t t, int buf, bool skip_bo
 // 2150-2430): polinom2 all six, polinom1 / polinom3 / sin1 Ez and Hy.
 bool test_defines (int kind, int comp)
 {
+  if (kind >= YEE_FN_EXP1) return false;   // per pair: exp_has
   if (kind == YEE_FN_POLINOM2) return true;
   if (kind == YEE_FN_POLINOM1 || kind == YEE_FN_POLINOM3 || kind == YEE_FN_SIN1) return comp == YEE_EZ || comp == YEE_HY;
   return false;
+}
+
+// Components of an exp exact solution: the E and H component of the handle's pair
+// (Scheme.inc.h:2437-2760).
+bool exp_has (const yee_handle *h, int comp)
+{
+  static const int kE[6] = { YEE_EX, YEE_EX, YEE_EY, YEE_EY, YEE_EZ, YEE_EZ };
+  static const int kH[6] = { YEE_HY, YEE_HZ, YEE_HX, YEE_HZ, YEE_HX, YEE_HY };
+  return h->fn_exact >= YEE_FN_EXP1 && (comp == kE[h->exp_pair] || comp == kH[h->exp_pair]);
 }
 
 // Records an event pair around the next launch when kernel timing is on.
@@ -551,9 +568,10 @@ int test_hooks (yee_handle *h, bool e_half)
       hipLaunchKernelGGL ((test_fill<T>), dim3 ((unsigned) blocks), dim3 (256), 0, h->stream, a, f, c, h->fn_border, time, 1);
       HIP_TRY (hipGetLastError ());
     }
-    if (h->fn_exact && test_defines (h->fn_exact, c))
+    if (h->fn_exact && (test_defines (h->fn_exact, c) || exp_has (h, c)))
     {
-      hipLaunchKernelGGL ((test_norm<T>), dim3 (kNormBlocks), dim3 (256), 0, h->stream, a, (const T *) f, c, h->fn_exact,
+      hipLaunchKernelGGL ((test_norm<T>), dim3 (kNormBlocks), dim3 (256), 0, h->stream, a, (const T *) f,
+                         (const T *) nullptr, c, h->fn_exact,
                          time, (int) h->norm_lo[0], (int) h->norm_lo[1], (int) h->norm_lo[2], (int) h->norm_hi[0],
                          (int) h->norm_hi[1], (int) h->norm_hi[2], h->norm_part + (size_t) c * 2 * kNormBlocks);
       HIP_TRY (hipGetLastError ());
@@ -693,8 +711,12 @@ int validate (const yee_config *c)
     // the PML sizes matter only with PML on (a 3-cell grid keeps the default size 2 otherwise)
     if ((c->flags & YEE_USE_PML) && (c->pml[a] < 0 || 2 * c->pml[a] > c->size[a]))
       return fail (YEE_EINVAL, "pml size out of range");
+    // a 1D line may put its right TF/SF border on the grid's end (right size 0: the scattered
+    // field only left of the left border, as Tests/suite t7.x do); its cells N - 1 and N are
+    // never updated, so no right-border correction fires
+    const int64_t min_right = line_axis (*c) >= 0 ? 0 : 1;
     if ((c->flags & YEE_USE_TFSF)
-        && (c->tfsf_left[a] < 1 || c->tfsf_right[a] < 1 || c->tfsf_left[a] + c->tfsf_right[a] >= c->size[a]))
+        && (c->tfsf_left[a] < 1 || c->tfsf_right[a] < min_right || c->tfsf_left[a] + c->tfsf_right[a] >= c->size[a]))
       return fail (YEE_EINVAL, "TF/SF box out of range");
   }
   int64_t cells = 1;
@@ -896,6 +918,58 @@ NtffArgs<T> ntff_args (const yee_handle *h, double teta, const int64_t left[3], 
   a.k = 2 * kPi / h->cfg.wavelength;
   a.teta = teta;
   return a;
+}
+
+// yee_set_exact_exp on one part's handle.
+int set_exact_exp_part (yee_handle *h, int32_t kind, int32_t pair, double border, const int64_t ns[3], const int64_t ne[3])
+{
+  int rc = yee_set_test_functions (h, 0, 0, 0, 0, ns, ne);
+  if (rc) return rc;
+  if (!h->norm_part)
+  {
+    void *p = nullptr;
+    HIP_TRY (hipMalloc (&p, (size_t) 12 * 2 * kNormBlocks * sizeof (double)));
+    h->norm_part = (double *) p;
+  }
+  static const int kAxis[6] = { 2, 1, 2, 0, 1, 0 };
+  const int pr = pair - YEE_SCHEME_EXHY;
+  // CallBack::exp1_e / exp2_e / exp3_e and exp*_h (CallBack.cpp:535-760), constants with the
+  // reference's operations
+  const double delta = h->cfg.dx, lambda = h->cfg.wavelength, courant = h->cfg.courant;
+  const double nl = lambda / delta;
+  const double k = wave_number (delta, lambda, courant, nl, h->sc.a1, h->sc.a2);
+  const double f = kC / lambda;
+  const double w = 2 * kPi * f;
+  const double step = delta * courant / kC;
+  const double c0 = (double) h->cfg.tfsf_left[kAxis[pr]] - 2.5;
+  double kk, d, hk;
+  if (kind == YEE_FN_EXP1)
+  {
+    kk = k;
+    d = c0;
+    hk = k;
+  }
+  else if (kind == YEE_FN_EXP2)
+  {
+    kk = -k;
+    d = 2 * border - c0;
+    hk = -k;
+  }
+  else
+  {
+    kk = 2 * k;
+    d = (border + c0) / 2.0;
+    hk = kk;
+  }
+  h->fn_exact = kind;
+  h->exp_pair = pr;
+  h->exp_k = kk;
+  h->exp_d = d;
+  h->exp_w = w;
+  h->exp_t0 = 0.5 * step;
+  h->exp_m = hk / (mu0 () * w);
+  h->fused = false;
+  return YEE_OK;
 }
 
 }  // namespace
@@ -1149,8 +1223,8 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
   if (!h || !center) return fail (YEE_EINVAL, "null argument");
   if (which < 0 || which > 5) return fail (YEE_EINVAL, "unknown material");
   if (!stair && accuracy <= 0) return fail (YEE_EINVAL, "sphere accuracy must be positive");
-  if (line_axis (h->cfg) >= 0) return fail (YEE_EINVAL, "spheres on a 1D grid are not supported");
   std::vector<double> v ((size_t) h->mcell);
+  const int line = line_axis (h->cfg);
   // double material precision: the fine grid, centre 2 c + 0.5 and radius 2 r (Scheme.inc.h:2992-3009)
   const double mod = h->fine () ? 2.0 : 1.0;
   const double cx = (double) center[0] * mod + 0.5, cy = (double) center[1] * mod + 0.5,
@@ -1169,7 +1243,13 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
             // eps-point coordinates (i + 0.5, j + 0.5, k + 0.5)
             const double mx = i + 0.5, my = j + 0.5, mz = k + 0.5;
             double val;
-            if (h->miss >= 0)
+            if (line >= 0)
+            {
+              // a 1D line: the segment on its one coordinate
+              const double m = line == 0 ? mx : (line == 1 ? my : mz), c = line == 0 ? cx : (line == 1 ? cy : cz);
+              val = stair ? sphere_stair_1d (m, c, radius, value, outside) : sphere_accurate_1d (m, c, radius, value, outside);
+            }
+            else if (h->miss >= 0)
             {
               // a 2D scheme on the (y, z) or (x, z) plane: the 2D sphere on its two coordinates
               const double ma = h->miss == 0 ? my : mx, ca = h->miss == 0 ? cy : cx;
@@ -1388,9 +1468,12 @@ int yee_set_test_functions (yee_handle *h, int32_t start_values, int32_t border,
     if (k < YEE_FN_NONE || k > YEE_FN_SIN1) return fail (YEE_EINVAL, "unknown test function kind");
   if (right_side == YEE_FN_SIN1) return fail (YEE_EINVAL, "sin1 has no right side (CallBack.cpp)");
   const bool any = start_values || border || right_side || exact;
-  if (any && (h->use_pml () || h->complex () || h->use_meta ()))
-    return fail (YEE_ENOTSUP, "test functions run without PML, metamaterials and complex fields (the reference's "
-                              "border and norm then act on its D / E grids)");
+  if ((start_values || border || right_side) && (h->use_pml () || h->complex () || h->use_meta ()))
+    return fail (YEE_ENOTSUP, "start values, border values and right sides run without PML, metamaterials and complex "
+                              "fields (the reference's border values then go to its D grids)");
+  if (exact && (h->complex () || h->use_meta ()))
+    return fail (YEE_ENOTSUP, "the polinom / sin1 exact solutions are real-build callbacks (UNREACHABLE in the "
+                              "complex build); metamaterials are not supported with them");
   if (start_values && !border)
     return fail (YEE_ENOTSUP, "start values need border values (the reference's two time layers otherwise alternate "
                               "the boundary cells between the start values and zero)");
@@ -1407,7 +1490,7 @@ int yee_set_test_functions (yee_handle *h, int32_t start_values, int32_t border,
   if (exact && !h->norm_part)
   {
     void *p = nullptr;
-    HIP_TRY (hipMalloc (&p, (size_t) 6 * 2 * kNormBlocks * sizeof (double)));
+    HIP_TRY (hipMalloc (&p, (size_t) 12 * 2 * kNormBlocks * sizeof (double)));
     h->norm_part = (double *) p;
   }
   h->fn_start = start_values;
@@ -1524,6 +1607,72 @@ int yee_ntff (yee_handle *h, double teta, const double *phi, int32_t nphi, const
     const double scat = (k * k) / (8 * kPi * n0) * ((f1r * f1r + f1i * f1i) + (f2r * f2r + f2i * f2i));
     out[i] = scat / (std::sqrt (e0 / m0) / 2);
   }
+  return YEE_OK;
+}
+
+int yee_set_exact_exp (yee_handle *h, int32_t kind, int32_t pair, double border, const int64_t norm_start[3],
+                       const int64_t norm_end[3])
+{
+  if (!h) return fail (YEE_EINVAL, "null handle");
+  if (h->prepared) return fail (YEE_ESTATE, "test functions must be set before yee_prepare");
+  if (kind < YEE_FN_EXP1 || kind > YEE_FN_EXP3) return fail (YEE_EINVAL, "kind must be YEE_FN_EXP1 .. YEE_FN_EXP3");
+  if (pair < YEE_SCHEME_EXHY || pair > YEE_SCHEME_EZHY) return fail (YEE_EINVAL, "pair must be YEE_SCHEME_EXHY .. EZHY");
+  if (!std::isfinite (border)) return fail (YEE_EINVAL, "border coordinate must be finite");
+  if (h->use_meta ()) return fail (YEE_ENOTSUP, "exact solutions with metamaterials are not supported");
+  if (h->imag && h->cfg.point_source)
+    return fail (YEE_ENOTSUP, "the complex module norm is taken after the step: no point sources with it");
+  int rc = set_exact_exp_part (h, kind, pair, border, norm_start, norm_end);
+  if (!rc && h->imag) rc = set_exact_exp_part (h->imag, kind, pair, border, norm_start, norm_end);
+  return rc;
+}
+
+int yee_get_diff_norm_complex (yee_handle *h, int32_t comp, double out[9])
+{
+  if (!h || !out) return fail (YEE_EINVAL, "null argument");
+  if (!h->imag) return fail (YEE_EINVAL, "the handle holds real field values");
+  if (comp < 0 || comp > 5) return fail (YEE_EINVAL, "component must be E or H (0 .. 5)");
+  if (!exp_has (h, comp) || !h->norm_valid[comp] || !h->imag->norm_valid[comp])
+    return fail (YEE_ESTATE, "no diff norm of this component yet (exp exact solution and a step)");
+  if (h->t != h->imag->t) return fail (YEE_ESTATE, "the two parts are at different steps");
+  double re[4], im[4];
+  int rc = yee_get_diff_norm (h, comp, re);
+  if (!rc) rc = yee_get_diff_norm (h->imag, comp, im);
+  if (rc) return rc;
+  // the module norm from both parts at the last step (calculateFieldStepIterationExact, complex)
+  HIP_TRY (hipStreamSynchronize (h->imag->stream));
+  const int64_t t = h->t - 1;
+  const double time = (comp >= 3 ? (double) t + 1.0 : (double) t + 0.5) * h->sc.dt;
+  double *mpart = h->norm_part + (size_t) (6 + comp) * 2 * kNormBlocks;
+  if (h->fp32 ())
+    hipLaunchKernelGGL ((test_norm<float>), dim3 (kNormBlocks), dim3 (256), 0, h->stream, make_args<float> (h, 0),
+                       (const float *) h->f[h->cur][comp], (const float *) h->imag->f[h->imag->cur][comp], comp,
+                       h->fn_exact, time, (int) h->norm_lo[0], (int) h->norm_lo[1], (int) h->norm_lo[2],
+                       (int) h->norm_hi[0], (int) h->norm_hi[1], (int) h->norm_hi[2], mpart);
+  else
+    hipLaunchKernelGGL ((test_norm<double>), dim3 (kNormBlocks), dim3 (256), 0, h->stream, make_args<double> (h, 0),
+                       (const double *) h->f[h->cur][comp], (const double *) h->imag->f[h->imag->cur][comp], comp,
+                       h->fn_exact, time, (int) h->norm_lo[0], (int) h->norm_lo[1], (int) h->norm_lo[2],
+                       (int) h->norm_hi[0], (int) h->norm_hi[1], (int) h->norm_hi[2], mpart);
+  HIP_TRY (hipGetLastError ());
+  std::vector<double> part ((size_t) 2 * kNormBlocks);
+  HIP_TRY (hipMemcpyAsync (part.data (), mpart, part.size () * sizeof (double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY (hipStreamSynchronize (h->stream));
+  double sum = 0.0, mx = 0.0;
+  for (int b = 0; b < kNormBlocks; ++b)
+  {
+    sum += part[(size_t) 2 * b];
+    mx = std::max (mx, part[(size_t) 2 * b + 1]);
+  }
+  const double mod = std::sqrt (sum / (double) h->ncell);
+  out[0] = re[0];
+  out[1] = im[0];
+  out[2] = mod;
+  out[3] = re[1];
+  out[4] = im[1];
+  out[5] = mod * 100.0 / mx;
+  out[6] = re[3];
+  out[7] = im[3];
+  out[8] = mx;
   return YEE_OK;
 }
 

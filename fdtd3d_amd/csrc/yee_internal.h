@@ -102,6 +102,10 @@ struct yee_handle
   double *norm_part = nullptr;
   bool norm_valid[6] = {};
   bool fn_any () const { return fn_start || fn_border || fn_rhs || fn_exact; }
+  // exact solution exp1 .. exp3 of a 1D pair (yee_set_exact_exp): pair 0 (ExHy) .. 5 (EzHy) and
+  // the host-computed constants of its phase and H factor (StepArgs exp_*)
+  int exp_pair = 0;
+  double exp_k = 0, exp_d = 0, exp_w = 0, exp_t0 = 0, exp_m = 0;
 
   // near-to-far-field transform: H of the step before the newest (yee_ntff_snapshot; each part's
   // handle holds its own), the step it was taken at, and the device buffer of the angles and

@@ -92,6 +92,11 @@ struct StepArgs
   int rhs_kind;
   double rhs_time;
   double fn_dx[3];
+  // exact solutions exp1 .. exp3 of a 1D pair (yee_set_exact_exp): the pair (0 ExHy .. 5 EzHy),
+  // the wave number of the phase (k, -k or 2k), the phase origin in cells, w, t0, the H factor
+  // (k, -k or 2k) / (mu0 w), and the part this handle holds (0 real, 1 imaginary)
+  int exp_pair, exp_part;
+  double exp_k, exp_d, exp_w, exp_t0, exp_m;
 };
 
 // Near-to-far-field transform (yee_ntff): the complex fields' two parts, the snapshot of H before

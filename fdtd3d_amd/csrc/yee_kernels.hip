@@ -1704,10 +1704,14 @@ __global__ void test_fill (StepArgs<T> a, T *f, int comp, int kind, double time,
 // Difference norm against the exact solution (calculateFieldStepIterationExact,
 // InternalScheme.inc.h:2707-2816) over the box [lo, hi): per workgroup the sum of
 // (exact - field)^2 and the largest |exact| of its grid-stride share, written to part[2 b],
-// part[2 b + 1] (the host adds the workgroups' sums in order).
+// part[2 b + 1] (the host adds the workgroups' sums in order).  The exact value is the part
+// this handle holds (exp_part) of an exp solution, else the real analytic value.  With fim set
+// (complex fields: f the real part, fim the imaginary part) the module norm instead:
+// (|exact| - |field|)^2 and the largest |exact|.
 template <typename T>
-__global__ __launch_bounds__ (256) void test_norm (StepArgs<T> a, const T *f, int comp, int kind, double time,
-                                                   int lo0, int lo1, int lo2, int hi0, int hi1, int hi2, double *part)
+__global__ __launch_bounds__ (256) void test_norm (StepArgs<T> a, const T *f, const T *fim, int comp, int kind,
+                                                   double time, int lo0, int lo1, int lo2, int hi0, int hi1, int hi2,
+                                                   double *part)
 {
   __shared__ double ss[256], sm[256];
   const int64_t e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
@@ -1718,8 +1722,24 @@ __global__ __launch_bounds__ (256) void test_norm (StepArgs<T> a, const T *f, in
     const int p[3] = { lo0 + (int) (u / (e1 * e2)), lo1 + (int) ((u / e2) % e1), lo2 + (int) (u % e2) };
     double r[3];
     test_coords (a, comp, p, r);
-    const double exact = test_field (kind, comp, r[0], r[1], r[2], time);
-    const double d = exact - (double) gld (f + (int64_t) p[0] * a.sx + (int64_t) p[1] * a.sy + (int64_t) p[2] * a.sz);
+    double er, ei = 0.0;
+    if (kind >= YEE_FN_EXP1)
+      test_exp (a, kind, comp, r, time, er, ei);
+    else
+      er = test_field (kind, comp, r[0], r[1], r[2], time);
+    const int64_t q = (int64_t) p[0] * a.sx + (int64_t) p[1] * a.sy + (int64_t) p[2] * a.sz;
+    double exact, d;
+    if (fim)
+    {
+      const double fr = (double) gld (f + q), fi = (double) gld (fim + q);
+      exact = sqrt (er * er + ei * ei);
+      d = exact - sqrt (fr * fr + fi * fi);
+    }
+    else
+    {
+      exact = a.exp_part ? ei : er;
+      d = exact - (double) gld (f + q);
+    }
     sum += d * d;
     mx = fmax (mx, fabs (exact));
   }
@@ -1983,7 +2003,7 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void tfsf_mask_kernel<T> (StepArgs<T>, int, int, uint8_t *); \
   template __global__ void test_fill<T> (StepArgs<T>, T *, int, int, double, int); \
   template __global__ void ntff_faces<T> (NtffArgs<T>); \
-  template __global__ void test_norm<T> (StepArgs<T>, const T *, int, int, double, int, int, int, int, int, int, double *); \
+  template __global__ void test_norm<T> (StepArgs<T>, const T *, const T *, int, int, double, int, int, int, int, int, int, double *); \
   template __global__ void fine_average<T> (const T *, int64_t, int64_t, int64_t, int, int, int, int, int, T *, T *, T *); \
   template __global__ void scattered_kernel<T, 0, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \
   template __global__ void scattered_kernel<T, 1, 3> (StepArgs<T>, const T *, double *, int, int64_t, int64_t); \

@@ -402,4 +402,33 @@ double sphere_stair_2d (double mx, double my, double cx, double cy, double radiu
   return eps;
 }
 
+// Approximation::approximateSphereAccurate on a 1D grid (Approximation.cpp:390-448): the cell
+// [m - 1/2, m + 1/2] against [c - r, c + r].  The reference tests "start >= left || end <= right"
+// for a cell inside, so every cell that overlaps the segment takes the full value (the partial
+// branches after it are reached only by cells wider than the segment).
+double sphere_accurate_1d (double m, double c, double radius, double eps, double outside)
+{
+  const double start = m - 0.5, end = m + 0.5;
+  const double left = c - radius, right = c + radius;
+  double vol = 0;
+  if (end <= left || start >= right)
+    vol = 0;
+  else if (start >= left || end <= right)
+    vol = 1;
+  else if (start < left)
+    vol = end - left;
+  else if (end > right)
+    vol = right - start;
+  if (vol > 1) vol = 1;
+  return eps * vol + outside * (1 - vol);
+}
+
+// Approximation::approximateSphereStair on a 1D grid (Approximation.cpp:678-699): both cell ends
+// within the radius.
+double sphere_stair_1d (double m, double c, double radius, double eps, double outside)
+{
+  const double d1 = m - 0.5 - c, d2 = m + 0.5 - c;
+  return (std::sqrt (d1 * d1) <= radius && std::sqrt (d2 * d2) <= radius) ? eps : outside;
+}
+
 }  // namespace yee

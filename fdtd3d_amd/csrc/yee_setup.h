@@ -90,5 +90,8 @@ double sphere_accurate_2d (double mx, double my, double cx, double cy, double ra
 double sphere_stair_3d (double mx, double my, double mz, double cx, double cy, double cz,
                         double radius, double eps, double outside);
 double sphere_stair_2d (double mx, double my, double cx, double cy, double radius, double eps, double outside);
+// The 1D segment of a "sphere" on a 1D line (Approximation.cpp:390-448, 678-699).
+double sphere_accurate_1d (double m, double c, double radius, double eps, double outside);
+double sphere_stair_1d (double m, double c, double radius, double eps, double outside);
 
 }  // namespace yee

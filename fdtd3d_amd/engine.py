@@ -30,7 +30,7 @@ EXPORTED = [
     "yee_prepare", "yee_step", "yee_sync", "yee_time", "yee_get_field", "yee_get_field_raw", "yee_get_scattered_field", "yee_get_material",
     "yee_get_mask", "yee_get_incident", "yee_get_scalars", "yee_last_step_ms", "yee_profile_steps",
     "yee_get_scattered_field_noalloc", "yee_get_imag_part", "yee_material_size", "yee_set_test_functions",
-    "yee_get_diff_norm", "yee_ntff_snapshot", "yee_ntff",
+    "yee_get_diff_norm", "yee_ntff_snapshot", "yee_ntff", "yee_set_exact_exp", "yee_get_diff_norm_complex",
     "yee_set_material_f32", "yee_set_material_sphere", "yee_set_point_source_gaussian",
     "yee_batch_last_error", "yee_batch_create", "yee_batch_destroy", "yee_batch_set_material",
     "yee_batch_set_material_f32", "yee_batch_set_point_source_gaussian", "yee_batch_prepare", "yee_batch_step",
@@ -89,6 +89,8 @@ def load_library():
     L.yee_set_test_functions.argtypes = [vp, i32, i32, i32, i32, vp, vp]
     L.yee_get_diff_norm.argtypes = [vp, i32, vp]
     L.yee_ntff_snapshot.argtypes = [vp]
+    L.yee_set_exact_exp.argtypes = [vp, i32, i32, ctypes.c_double, vp, vp]
+    L.yee_get_diff_norm_complex.argtypes = [vp, i32, vp]
     L.yee_ntff.argtypes = [vp, ctypes.c_double, vp, i32, vp, vp, vp]
     L.yee_get_mask.argtypes = [vp, i32, i32, vp]
     L.yee_get_incident.argtypes = [vp, vp, vp, ctypes.c_int64]
@@ -187,6 +189,11 @@ class Engine:
             ns = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_start])
             ne = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_end])
             _check(L.yee_set_test_functions(h, cfg.start_fn, cfg.border_fn, cfg.rhs_fn, cfg.exact_fn, ns, ne))
+        if cfg.exp_fn:
+            # --calc-exp<n>-<pair>-diff-norm
+            ns = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_start])
+            ne = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_end])
+            _check(L.yee_set_exact_exp(h, cfg.exp_fn, cfg.exp_pair, cfg.exp_border, ns, ne))
 
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -238,6 +245,13 @@ class Engine:
         at the last step (the reference's "DIFF NORM" line)."""
         out = (ctypes.c_double * 4)()
         _check(load_library().yee_get_diff_norm(self.h, COMP_INDEX[name], out))
+        return tuple(out)
+
+    def diff_norm_complex(self, name: str):
+        """The complex build's diff norm of a component at the last step: (norm re, norm im,
+        norm of the module, their percentages, max |Re|, max |Im|, max |exact|)."""
+        out = (ctypes.c_double * 9)()
+        _check(load_library().yee_get_diff_norm_complex(self.h, COMP_INDEX[name], out))
         return tuple(out)
 
     def ntff_snapshot(self) -> None:

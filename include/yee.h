@@ -78,6 +78,10 @@ extern "C" {
 #define YEE_FN_POLINOM2 2   /* Ex = c^2 t (y^2 + z^2) and its cyclic twins, H = eps0 E; right side J, M */
 #define YEE_FN_POLINOM3 3   /* Ez = c^2 (x t)^2, Hy = eps0 Ez; right side Jz, My */
 #define YEE_FN_SIN1 4       /* Ez = c sin (t - x), Hy = -c sin (t - x); no right side */
+/* Exact solutions of a 1D plane wave meeting a dielectric half-space (yee_set_exact_exp): */
+#define YEE_FN_EXP1 5       /* incident i e^{i (k (z - z0) - w (t - t0))} */
+#define YEE_FN_EXP2 6       /* reflected -i/3 e^{i (-k (z - (2 zb - z0)) - w (t - t0))} */
+#define YEE_FN_EXP3 7       /* transmitted 2i/3 e^{i (2k (z - (zb + z0)/2) - w (t - t0))} */
 
 /* Field components (GridType order of the reference). */
 enum yee_component
@@ -250,6 +254,32 @@ int yee_set_test_functions (yee_handle *h, int32_t start_values, int32_t border,
  * update and border values and before its point source, as the reference does.
  */
 int yee_get_diff_norm (yee_handle *h, int32_t comp, double out[4]);
+
+/*
+ * Exact solution exp1 / exp2 / exp3 (YEE_FN_EXP1 .. EXP3) of the 1D pair `pair`
+ * (YEE_SCHEME_EXHY .. EZHY: its E and H component, the line axis) for the reference's
+ * --calc-exp<n>-<pair>-diff-norm (Scheme.inc.h:2437-2760; CallBack.cpp:280-760): E1 the
+ * incident wave, E2 the wave reflected and E3 the wave transmitted by a dielectric of eps = 4
+ * starting at `border` (the reference's eps sphere centre - radius along the line), each
+ * H = E k / (mu0 w) (-k for E2, 2k for E3), z0 = the TF/SF left size along the line - 2.5,
+ * k the numerical wave number, t0 = dt / 2, and the signs of the reference's per-pair wrappers.
+ * Every step the diff norm of the pair's components over the norm box (as yee_set_test_functions),
+ * read with yee_get_diff_norm (the real part; a complex handle's imaginary part handle gives
+ * the imaginary part) or yee_get_diff_norm_complex.  Runs with PML (the norm reads E, as the
+ * reference's normGrid = gridPML2) and complex fields; YEE_ENOTSUP with metamaterials and, for a
+ * complex handle, point sources (its module norm is taken after the step).  Before yee_prepare.
+ */
+int yee_set_exact_exp (yee_handle *h, int32_t kind, int32_t pair, double border, const int64_t norm_start[3],
+                       const int64_t norm_end[3]);
+
+/*
+ * The complex build's diff norm of component comp at the last step (Scheme.inc.h:1537-1542):
+ * out[0..2] = the norms of the real part, the imaginary part and the module
+ * (sqrt (sum / cells of the grid), the module's sum over (|exact| - |field|)^2),
+ * out[3..5] their percentages of max |Re exact|, max |Im exact|, max |exact|, out[6..8] those
+ * maxima.  A complex handle with an exp exact solution (yee_set_exact_exp).
+ */
+int yee_get_diff_norm_complex (yee_handle *h, int32_t comp, double out[9]);
 
 /*
  * Near-to-far-field transform (--use-ntff) of a complex 3D handle at its newest step, as

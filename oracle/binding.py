@@ -52,6 +52,8 @@ def lib():
         L.oracle_set_test_functions.argtypes = [vp] + [ctypes.c_int] * 4 + [vp, vp]
         L.oracle_get_diff_norm.argtypes = [vp, ctypes.c_int32, vp]
         L.oracle_ntff.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, vp, vp, vp]
+        L.oracle_set_exact_exp.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp]
+        L.oracle_exp_mod_norm.argtypes = [vp, vp, ctypes.c_int32, vp]
         L.oracle_scale_fields.argtypes = [vp, ctypes.c_double]
         L.oracle_scale_fields.restype = None
         L.oracle_adjoint.argtypes = [vp, ctypes.POINTER(OracleAdjSpec), ctypes.c_int32,
@@ -61,6 +63,10 @@ def lib():
         L.oracle_sphere_accurate_3d.argtypes = [d] * 9 + [ctypes.c_int]
         L.oracle_sphere_accurate_2d.restype = d
         L.oracle_sphere_accurate_2d.argtypes = [d] * 7 + [ctypes.c_int]
+        L.oracle_sphere_accurate_1d.restype = d
+        L.oracle_sphere_accurate_1d.argtypes = [d] * 5
+        L.oracle_sphere_stair_1d.restype = d
+        L.oracle_sphere_stair_1d.argtypes = [d] * 5
         L.oracle_sphere_stair_3d.restype = d
         L.oracle_sphere_stair_3d.argtypes = [d] * 9
         L.oracle_wave_number.restype = d
@@ -133,7 +139,11 @@ def sphere_map(cfg: Config, value: float, outside: float, center, radius: int) -
             else:
                 out[idx] = L.oracle_sphere_accurate_3d(mx, my, mz, cx, cy, cz, r, value, outside, cfg.sphere_accuracy)
         elif cfg.dim == 1:
-            raise ValueError("spheres on a 1D grid are not driven")
+            # the segment on the line's one coordinate
+            c = (cx, cy, cz)[cfg.active_axes[0]]
+            m = idx[0] + 0.5
+            out[idx] = (L.oracle_sphere_stair_1d(m, c, r, value, outside) if cfg.stair_sphere
+                        else L.oracle_sphere_accurate_1d(m, c, r, value, outside))
         else:
             # the two axes of the 2D scheme (the sphere centre is given in 3D coordinates)
             ca, cb = [c for a, c in enumerate((cx, cy, cz)) if a != cfg.missing_axis]
@@ -158,6 +168,11 @@ class Oracle:
             ne = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_end])
             if lib().oracle_set_test_functions(self.h, cfg.start_fn, cfg.border_fn, cfg.rhs_fn, cfg.exact_fn, ns, ne):
                 raise ValueError("oracle_set_test_functions rejected the configuration")
+        if cfg.exp_fn:
+            ns = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_start])
+            ne = (ctypes.c_int64 * 3)(*[int(v) for v in cfg.norm_end])
+            if lib().oracle_set_exact_exp(self.h, cfg.exp_fn, cfg.exp_pair, cfg.exp_border, ns, ne):
+                raise ValueError("oracle_set_exact_exp rejected the configuration")
 
     def close(self):
         if self.h:
@@ -294,6 +309,15 @@ def run_ref_driver(args, out_path: str, kind: str = "f64", timeout: float = 600)
         if line.startswith("Total time ="):
             secs = float(line.split("=")[1].split()[0])
     return read_ref_output(out_path), secs
+
+
+def oracle_exp_mod_norm(re: "Oracle", im: "Oracle", name: str):
+    """(norm, percent) of the module of the complex build's exp exact solution at the last step,
+    from the real and imaginary part oracles."""
+    out = (ctypes.c_double * 2)()
+    if lib().oracle_exp_mod_norm(re.h, im.h, COMP_INDEX[name], out):
+        raise ValueError("oracle_exp_mod_norm rejected the call")
+    return out[0], out[1]
 
 
 def oracle_ntff(re: "Oracle", im: "Oracle", teta: float, phi: float, left, right):

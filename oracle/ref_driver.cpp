@@ -473,6 +473,15 @@ class RefRun
       std::vector<double> &rec = norms[ci];
       rec.push_back ((double) normRe);
       rec.push_back ((double) (normRe * 100.0 / maxRe));
+#ifdef COMPLEX_FIELD_VALUES
+      /* the complex build's line (Scheme.inc.h:1537-1542): imaginary part and module too */
+      normIm = sqrt (normIm / normGrid->getTotalSize ().calculateTotalCoord ());
+      normMod = sqrt (normMod / normGrid->getTotalSize ().calculateTotalCoord ());
+      rec.push_back ((double) normIm);
+      rec.push_back ((double) (normIm * 100.0 / maxIm));
+      rec.push_back ((double) normMod);
+      rec.push_back ((double) (normMod * 100.0 / maxMod));
+#endif
     }
 
     bool src = false;
@@ -638,6 +647,50 @@ public:
       is->setCallbackHyExact (CallBack::sin1_hy);
     }
 #endif
+    /* the exp exact solutions of the 1D pairs, both builds (Scheme.inc.h:2437-2760): one chain
+     * per pair, exp1 before exp2 before exp3 */
+    expChains ();
+  }
+
+  /* One chain per pair of Scheme::initCallBacks' exp part. */
+  void expChains ()
+  {
+    if (SOLVER_SETTINGS.getDoCalculateExp1ExHyDiffNorm ())
+    { is->setCallbackExExact (CallBack::exp1_ex_exhy); is->setCallbackHyExact (CallBack::exp1_hy_exhy); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp2ExHyDiffNorm ())
+    { is->setCallbackExExact (CallBack::exp2_ex_exhy); is->setCallbackHyExact (CallBack::exp2_hy_exhy); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp3ExHyDiffNorm ())
+    { is->setCallbackExExact (CallBack::exp3_ex_exhy); is->setCallbackHyExact (CallBack::exp3_hy_exhy); }
+    if (SOLVER_SETTINGS.getDoCalculateExp1ExHzDiffNorm ())
+    { is->setCallbackExExact (CallBack::exp1_ex_exhz); is->setCallbackHzExact (CallBack::exp1_hz_exhz); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp2ExHzDiffNorm ())
+    { is->setCallbackExExact (CallBack::exp2_ex_exhz); is->setCallbackHzExact (CallBack::exp2_hz_exhz); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp3ExHzDiffNorm ())
+    { is->setCallbackExExact (CallBack::exp3_ex_exhz); is->setCallbackHzExact (CallBack::exp3_hz_exhz); }
+    if (SOLVER_SETTINGS.getDoCalculateExp1EyHxDiffNorm ())
+    { is->setCallbackEyExact (CallBack::exp1_ey_eyhx); is->setCallbackHxExact (CallBack::exp1_hx_eyhx); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp2EyHxDiffNorm ())
+    { is->setCallbackEyExact (CallBack::exp2_ey_eyhx); is->setCallbackHxExact (CallBack::exp2_hx_eyhx); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp3EyHxDiffNorm ())
+    { is->setCallbackEyExact (CallBack::exp3_ey_eyhx); is->setCallbackHxExact (CallBack::exp3_hx_eyhx); }
+    if (SOLVER_SETTINGS.getDoCalculateExp1EyHzDiffNorm ())
+    { is->setCallbackEyExact (CallBack::exp1_ey_eyhz); is->setCallbackHzExact (CallBack::exp1_hz_eyhz); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp2EyHzDiffNorm ())
+    { is->setCallbackEyExact (CallBack::exp2_ey_eyhz); is->setCallbackHzExact (CallBack::exp2_hz_eyhz); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp3EyHzDiffNorm ())
+    { is->setCallbackEyExact (CallBack::exp3_ey_eyhz); is->setCallbackHzExact (CallBack::exp3_hz_eyhz); }
+    if (SOLVER_SETTINGS.getDoCalculateExp1EzHxDiffNorm ())
+    { is->setCallbackEzExact (CallBack::exp1_ez_ezhx); is->setCallbackHxExact (CallBack::exp1_hx_ezhx); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp2EzHxDiffNorm ())
+    { is->setCallbackEzExact (CallBack::exp2_ez_ezhx); is->setCallbackHxExact (CallBack::exp2_hx_ezhx); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp3EzHxDiffNorm ())
+    { is->setCallbackEzExact (CallBack::exp3_ez_ezhx); is->setCallbackHxExact (CallBack::exp3_hx_ezhx); }
+    if (SOLVER_SETTINGS.getDoCalculateExp1EzHyDiffNorm ())
+    { is->setCallbackEzExact (CallBack::exp1_ez_ezhy); is->setCallbackHyExact (CallBack::exp1_hy_ezhy); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp2EzHyDiffNorm ())
+    { is->setCallbackEzExact (CallBack::exp2_ez_ezhy); is->setCallbackHyExact (CallBack::exp2_hy_ezhy); }
+    else if (SOLVER_SETTINGS.getDoCalculateExp3EzHyDiffNorm ())
+    { is->setCallbackEzExact (CallBack::exp3_ez_ezhy); is->setCallbackHyExact (CallBack::exp3_hy_ezhy); }
   }
 
   /* Real coordinate of a component's point (YeeGridLayout::get<C>CoordFP). */

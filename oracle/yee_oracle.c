@@ -167,6 +167,10 @@ struct oracle
   int64_t mn[3], mcell;
   /* analytic test problems (YEE_FN_*): start values, border values, right side, exact solution */
   int fn_start, fn_border, fn_rhs, fn_exact;
+  /* exact solutions exp1 / exp2 / exp3 of a 1D scheme's pair (YEE_FN_EXP*): the pair, the
+   * material border coordinate (eps sphere centre - radius along the line) */
+  int exp_pair;
+  double exp_border;
   int64_t norm_lo[3], norm_hi[3];
   double norm[6][4];       /* last diff norm per component: norm, percent, sum of squares, max |exact| */
   int norm_valid[6];
@@ -623,6 +627,31 @@ double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, do
 }
 
 /* Approximation::approximateSphereAccurate, 2D (Source/Layout/Approximation.cpp:471-561). */
+/* approximateSphereAccurate, 1D (Approximation.cpp:390-448): its inside test is
+ * "start >= left || end <= right", so a cell overlapping the segment is filled completely. */
+double oracle_sphere_accurate_1d (double m, double c, double radius, double eps, double outside)
+{
+  const double start = m - 0.5, end = m + 0.5, left = c - radius, right = c + radius;
+  double volume = 0;
+  if (end <= left || start >= right)
+    ;
+  else if (start >= left || end <= right)
+    volume = 1;
+  else if (start < left)
+    volume = end - left;
+  else if (end > right)
+    volume = right - start;
+  if (volume > 1) volume = 1;
+  return eps * volume + outside * (1 - volume);
+}
+
+/* approximateSphereStair, 1D (Approximation.cpp:678-699): both ends within the radius. */
+double oracle_sphere_stair_1d (double m, double c, double radius, double eps, double outside)
+{
+  const double a = m - 0.5 - c, b = m + 0.5 - c;
+  return (sqrt (a * a) <= radius && sqrt (b * b) <= radius) ? eps : outside;
+}
+
 double oracle_sphere_accurate_2d (double mx, double my, double cx, double cy,
                                   double radius, double eps, double outside, int accuracy)
 {
@@ -902,6 +931,106 @@ int oracle_prepare (oracle *o)
   return 0;
 }
 
+/*
+ * Exact solutions of the 1D plane wave meeting a dielectric half-space (Tests/suite t7.x;
+ * CallBack.cpp:280-760): incident E1 = i e^{i (k (z - z0 dx) - w (t - t0))}, reflected
+ * E2 = -i/3 e^{i (-k (z - (2 zb - z0) dx) - w (t - t0))}, transmitted
+ * E3 = 2i/3 e^{i (2k (z - (zb + z0)/2 dx) - w (t - t0))}, H = E k / (mu0 w) (-k for E2, 2k for
+ * E3); k the numerical wave number (approximateWaveNumber), w = 2 pi c / lambda,
+ * t0 = dt / 2, z0 = the TF/SF left size along the line - 2.5.  Per pair (ExHy, ExHz, EyHx, EyHz,
+ * EzHx, EzHy) the line axis and the signs of E and H of the reference's wrappers.  The complex
+ * value with the reference's complex operations (i * exponent (i arg) = (0 c - 1 s, 0 s + 1 c));
+ * the real build returns its real part with the same operations.
+ */
+static __attribute__ ((noinline)) double o_sin_only (double x)
+{
+  return sin (x);
+}
+
+static __attribute__ ((noinline)) void o_sin_cos (double x, double *s, double *c)
+{
+  *c = cos (x);
+  *s = sin (x);
+}
+
+static const int kExpAxis[6] = { 2, 1, 2, 0, 1, 0 };
+static const int kExpE[6] = { YEE_EX, YEE_EX, YEE_EY, YEE_EY, YEE_EZ, YEE_EZ };
+static const int kExpH[6] = { YEE_HY, YEE_HZ, YEE_HX, YEE_HZ, YEE_HX, YEE_HY };
+static const double kExpSignE[6] = { -1, 1, -1, -1, 1, 1 };
+static const double kExpSignH[6] = { -1, -1, 1, -1, 1, -1 };
+
+static int o_exp_has (const oracle *o, int comp)
+{
+  return o->fn_exact >= YEE_FN_EXP1 && (comp == kExpE[o->exp_pair] || comp == kExpH[o->exp_pair]);
+}
+
+static void o_exp_value (const oracle *o, int comp, double coord, double t, double *re, double *im)
+{
+  const int pair = o->exp_pair, n = o->fn_exact - YEE_FN_EXP1 + 1;
+  const double lambda = o->lambda, courant = o->courant, delta = o->dx;
+  const double nl = lambda / delta;
+  double k = oracle_wave_number (delta, lambda, courant, nl, o->a1, o->a2);
+  const double f = O_C / lambda;
+  const double w = 2 * O_PI * f;
+  const double step = delta * courant / O_C;
+  const double t0 = 0.5 * step;
+  const double c0 = (double) o->cfg.tfsf_left[kExpAxis[pair]] - 2.5;
+  const double cb = o->exp_border;
+  double arg, hk;
+  if (n == 1)
+  {
+    arg = k * (coord - c0 * delta) - w * (t - t0);
+    hk = k;
+  }
+  else if (n == 2)
+  {
+    const double diff = 2 * cb - c0;
+    arg = -k * (coord - diff * delta) - w * (t - t0);
+    hk = -k;
+  }
+  else
+  {
+    k = 2 * k;
+    const double diff = (cb + c0) / 2.0;
+    arg = k * (coord - diff * delta) - w * (t - t0);
+    hk = k;
+  }
+  /* the real build evaluates sin (arg) alone; the complex build CComplex (cos, sin) (Complex.h:139),
+   * whose two calls the compiler merges into one sincos, which differs from sin in the last bit
+   * for rare arguments: each build's form is kept apart */
+  double s, c = 0.0;
+  if (o->src_kind)
+    o_sin_cos (arg, &s, &c);
+  else
+    s = o_sin_only (arg);
+  double r, i;
+  if (n == 1)
+  {
+    r = o->src_kind ? 0.0 * c - 1.0 * s : -s;
+    i = 1.0 * c + 0.0 * s;
+  }
+  else if (n == 2)
+  {
+    /* (-i) (c, s) = (-0 c - (-1) s, (-1) c + -0 s), times 1, over 3 */
+    r = o->src_kind ? ((-0.0 * c - (-1.0) * s) * 1.0) / 3.0 : s * 1.0 / 3.0;
+    i = ((-1.0 * c + -0.0 * s) * 1.0) / 3.0;
+  }
+  else
+  {
+    r = o->src_kind ? ((0.0 * c - 1.0 * s) * 2.0) / 3.0 : -s * 2.0 / 3.0;
+    i = ((1.0 * c + 0.0 * s) * 2.0) / 3.0;
+  }
+  if (comp >= 3)
+  {
+    const double m = hk / (o_mu0 () * w);
+    r = r * m;
+    i = i * m;
+  }
+  const double sg = comp >= 3 ? kExpSignH[pair] : kExpSignE[pair];
+  *re = sg < 0 ? -r : r;
+  *im = sg < 0 ? -i : i;
+}
+
 /* Analytic test problems (YEE_FN_* kinds; 0 = none) and the norm box: [1, N-1) on each axis of the
  * scheme, an axis with a nonzero start / end taking it (--norm-start / --norm-end,
  * Scheme.inc.h:1394-1420).  Rejects PML (the border and norm then act on D / E grids this
@@ -911,7 +1040,9 @@ int oracle_set_test_functions (oracle *o, int start, int border, int rhs, int ex
                                const int64_t ne[3])
 {
   if (!o || o->prepared) return -1;
-  if ((start || border || rhs || exact) && o->use_pml) return -1;
+  if (start > YEE_FN_SIN1 || border > YEE_FN_SIN1 || rhs > YEE_FN_SIN1 || exact > YEE_FN_SIN1) return -1;
+  /* with PML the border values go to D; the norm reads E (normGrid = gridPML2) */
+  if ((start || border || rhs) && o->use_pml) return -1;
   if (start && !border) return -1;
   o->fn_start = start;
   o->fn_border = border;
@@ -1102,6 +1233,49 @@ void oracle_scale_fields (oracle *o, double s)
     for (int l = 0; l < o->layers; ++l)
       if (o->f[c][l])
         for (int64_t i = 0; i < o->ncell; ++i) o->f[c][l][i] *= s;
+}
+
+/* Exact solution exp<kind> of a 1D pair (kind YEE_FN_EXP1 .. EXP3, pair YEE_SCHEME_EXHY .. EZHY)
+ * for --calc-exp<n>-<pair>-diff-norm (Scheme.inc.h:2437-2760), border = the material border
+ * coordinate (eps sphere centre - radius along the line), and the norm box. */
+int oracle_set_exact_exp (oracle *o, int kind, int pair, double border, const int64_t ns[3], const int64_t ne[3])
+{
+  if (!o || o->prepared || kind < YEE_FN_EXP1 || kind > YEE_FN_EXP3 || pair < YEE_SCHEME_EXHY || pair > YEE_SCHEME_EZHY)
+    return -1;
+  if (oracle_set_test_functions (o, 0, 0, 0, 0, ns, ne)) return -1;
+  o->fn_exact = kind;
+  o->exp_pair = pair - YEE_SCHEME_EXHY;
+  o->exp_border = border;
+  return 0;
+}
+
+/* Module norm of the complex build at the last step: sqrt (sum (|exact| - |field|)^2 / cells) and
+ * its percentage of max |exact|, from the real and imaginary part oracles at the same step. */
+int oracle_exp_mod_norm (const oracle *re, const oracle *im, int32_t comp, double out[2])
+{
+  if (!re || !im || !out || comp < 0 || comp > 5 || !o_exp_has (re, comp) || re->t != im->t || re->t < 1) return -1;
+  const int64_t t = re->t - 1;
+  const double time = (comp >= 3 ? (double) t + 1.0 : (double) t + 0.5) * re->dt;
+  const int axis = kExpAxis[re->exp_pair];
+  double sum = 0.0, mx = 0.0;
+  for (int64_t i = re->norm_lo[0]; i < re->norm_hi[0]; ++i)
+    for (int64_t j = re->norm_lo[1]; j < re->norm_hi[1]; ++j)
+      for (int64_t k = re->norm_lo[2]; k < re->norm_hi[2]; ++k)
+      {
+        double r[3], er, ei;
+        o_real_coord (re, comp, i, j, k, r);
+        o_exp_value (re, comp, r[axis] * re->dx, time, &er, &ei);
+        const int64_t q = o_idx (re, i, j, k);
+        /* after the step's shift the newest values are layer 1 (CurrentAfterShift) */
+        const double fr = re->f[comp][1][q], fi = im->f[comp][1][q];
+        const double me = sqrt (er * er + ei * ei), mf = sqrt (fr * fr + fi * fi);
+        sum += (me - mf) * (me - mf);
+        if (mx < me) mx = me;
+      }
+  const double norm = sqrt (sum / (double) re->ncell);
+  out[0] = norm;
+  out[1] = norm * 100.0 / mx;
+  return 0;
 }
 
 int oracle_get_diff_norm (oracle *o, int32_t comp, double out[4])
@@ -1380,18 +1554,29 @@ static void o_field_step (oracle *o, int comp, int32_t t)
   /* difference norm against the exact solution over the norm box, in the iteration order of the
    * reference (calculateFieldStepIterationExact, InternalScheme.inc.h:2707-2816; Scheme.inc.h:
    * 1377-1550): sqrt (sum of squares / cells of the grid) and its percentage of max |exact| */
-  if (o->fn_exact && o_fn_has (o->fn_exact, comp))
+  if (o->fn_exact && (o_fn_has (o->fn_exact, comp) || o_exp_has (o, comp)))
   {
     const double time = (c->is_h ? (double) t + 1.0 : (double) t + 0.5) * dt;
     double sum = 0.0, mx = 0.0;
     const double *f0 = o->f[comp][0];
+    const int exp = o->fn_exact >= YEE_FN_EXP1;
     for (int64_t i = o->norm_lo[0]; i < o->norm_hi[0]; ++i)
       for (int64_t j = o->norm_lo[1]; j < o->norm_hi[1]; ++j)
         for (int64_t k = o->norm_lo[2]; k < o->norm_hi[2]; ++k)
         {
           double r[3];
           o_real_coord (o, comp, i, j, k, r);
-          const double exact = o_fn_value (o->fn_exact, comp, r[0] * dx, r[1] * dx, r[2] * dx, time);
+          double exact;
+          if (exp)
+          {
+            /* the part this oracle carries: the real part, or the imaginary part of the complex
+             * build's imaginary-part run */
+            double er, ei;
+            o_exp_value (o, comp, r[kExpAxis[o->exp_pair]] * dx, time, &er, &ei);
+            exact = o->src_kind == 2 ? ei : er;
+          }
+          else
+            exact = o_fn_value (o->fn_exact, comp, r[0] * dx, r[1] * dx, r[2] * dx, time);
           const double diff = exact - f0[o_idx (o, i, j, k)];
           sum += diff * diff;
           const double ea = fabs (exact);

@@ -57,6 +57,12 @@ int oracle_ntff (const oracle *re, const oracle *im, double teta, double phi, co
 /* Multiplies every E / H value of every time layer by s (a test of the far field conjugates the
  * complex fields by scaling the imaginary part's oracle by -1). */
 void oracle_scale_fields (oracle *o, double s);
+/* Exact solution exp1 / exp2 / exp3 (YEE_FN_EXP1 .. EXP3) of a 1D pair (YEE_SCHEME_EXHY .. EZHY),
+ * border = eps sphere centre - radius along the line; before prepare. */
+int oracle_set_exact_exp (oracle *o, int kind, int pair, double border, const int64_t ns[3], const int64_t ne[3]);
+/* Module norm (out[0]) and its percentage (out[1]) of the complex build from the real and
+ * imaginary part oracles at the same step. */
+int oracle_exp_mod_norm (const oracle *re, const oracle *im, int32_t comp, double out[2]);
 /* Diff norm of a component at the last step: norm, percent of max |exact|, sum of squares, max. */
 int oracle_get_diff_norm (oracle *o, int32_t comp, double out[4]);
 
@@ -74,6 +80,9 @@ int oracle_adjoint (oracle *o, const oracle_adj_spec *sp, int32_t T, double *J, 
 /* Material generators, restated from Approximation.cpp (used to build parity inputs). */
 double oracle_sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,
                                   double radius, double eps, double outside, int accuracy);
+/* 1D segments (Approximation.cpp:390-448, 678-699): cell [m - 1/2, m + 1/2], segment [c - r, c + r]. */
+double oracle_sphere_accurate_1d (double m, double c, double radius, double eps, double outside);
+double oracle_sphere_stair_1d (double m, double c, double radius, double eps, double outside);
 double oracle_sphere_accurate_2d (double mx, double my, double cx, double cy,
                                   double radius, double eps, double outside, int accuracy);
 double oracle_sphere_stair_3d (double mx, double my, double mz, double cx, double cy, double cz,

@@ -194,3 +194,54 @@ def run_norms(eng, cfg):
             out.setdefault(n, []).append(v)
     return {n: np.array(v) for n, v in out.items()}
 
+
+
+# Exact solutions exp1 / exp2 / exp3 of the 1D pairs (Tests/suite t7.1 - t7.6, scaled down by 5:
+# 120 cells, PML 10, TF/SF border 30 on the left only, a dielectric from 60 on): per pair the line
+# axis and the incident angles of its suite script.
+EXP_PAIRS = {
+    "exhy": ("z", "--angle-teta 0 --angle-phi 0 --angle-psi 90"),
+    "exhz": ("y", "--angle-teta 90 --angle-phi 90 --angle-psi 0"),
+    "eyhx": ("z", "--angle-teta 0 --angle-phi 90 --angle-psi 90"),
+    "eyhz": ("x", "--angle-teta 90 --angle-phi 0 --angle-psi 0"),
+    "ezhx": ("y", "--angle-teta 90 --angle-phi 90 --angle-psi 90"),
+    "ezhy": ("x", "--angle-teta 90 --angle-phi 0 --angle-psi 90"),
+}
+
+
+def exp_case(pair, kind, size=120, pml=10, tfsf=30, center=90, radius=30, steps=200, dx="0.0001", wavelength="0.004",
+             eps=None, norm=None):
+    """Command line of a suite t7 run: exp1 in vacuum (eps 1) over the total-field region, exp2
+    (reflected) in the scattered-field region left of the border, exp3 (transmitted) inside the
+    eps = 4 dielectric."""
+    a, angles = EXP_PAIRS[pair]
+    if eps is None:
+        eps = 1 if kind == "exp1" else 4
+    if norm is None:
+        b = center - radius
+        norm = {"exp1": (tfsf + 1, size - pml - 1), "exp2": (pml + 1, tfsf - 1), "exp3": (b + 1, size - pml - 1)}[kind]
+    return (f"--1d-{pair} --size {a}:{size} {angles} --dx {dx} --wavelength {wavelength} --courant-factor 0.5 "
+            f"--pml-size {a}:{pml} --use-pml --use-tfsf --tfsf-size-left {a}:{tfsf} --tfsf-size-right {a}:0 "
+            f"--eps-sphere {eps} --eps-sphere-center {a}:{center} --eps-sphere-radius {radius} "
+            f"--norm-start {a}:{norm[0]} --norm-end {a}:{norm[1]} --calc-{kind}-{pair}-diff-norm --time-steps {steps}")
+
+
+EXP_CASES = [exp_case(p, k) for p in EXP_PAIRS for k in ("exp1", "exp2", "exp3")]
+
+
+def run_norms_complex(re_, im_, cfg, mod_norm):
+    """Steps the real and imaginary part runs (engine parts or oracles) in lock step; per step the
+    complex build's line of every component with an exp solution: (norm re, percent re, norm im,
+    percent im, norm of the module, percent).  mod_norm (re_, im_, name) gives the module's."""
+    out = {}
+    for _ in range(cfg.time_steps):
+        re_.step(1)
+        im_.step(1)
+        for n in cfg.components:
+            try:
+                r, i = re_.diff_norm(n), im_.diff_norm(n)
+            except Exception:
+                continue
+            m = mod_norm(re_, im_, n)
+            out.setdefault(n, []).append((r[0], r[1], i[0], i[1], m[0], m[1]))
+    return {n: np.array(v) for n, v in out.items()}

@@ -68,6 +68,19 @@ def test_parse_rejects_unknown_and_unsupported():
         parse_args("--3d --use-pml --use-metamaterials --use-ca-cb-pml-metamaterials".split())
 
 
+def test_parse_exp_solution_options():
+    """--calc-exp<n>-<pair>-diff-norm (Settings.inc.h:153-175): kind and pair, exp1 before exp2
+    before exp3 within a pair (Scheme::initCallBacks), two pairs refused; the dielectric border
+    is the eps sphere centre minus its radius along the pair's line."""
+    from fdtd3d_amd.config import EXP_FUNCTIONS, SCHEMES_1D
+    c = parse_args("--1d-eyhz --size x:60 --eps-sphere 4 --eps-sphere-center x:40 --eps-sphere-radius 15 "
+                   "--calc-exp3-eyhz-diff-norm --calc-exp2-eyhz-diff-norm".split())
+    assert c.exp_fn == EXP_FUNCTIONS["exp2"] == 6 and c.exp_pair == SCHEMES_1D["--1d-eyhz"][0]
+    assert c.exp_border == 25.0
+    with pytest.raises(OptionError):
+        parse_args("--1d-exhy --calc-exp1-exhy-diff-norm --calc-exp1-ezhy-diff-norm".split())
+
+
 def test_parse_ntff_options():
     """--use-ntff and its settings (Settings.inc.h:85-88, 124, 181-187): defaults, --same-size-ntff,
     and the diagram's angles as Scheme::saveNTFF's loop adds them up."""

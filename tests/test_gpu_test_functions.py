@@ -9,6 +9,8 @@
   resolve; the reference's suite runs them in its default FP64 build.
 - The reference suite's pass criteria (Tests/suite t2-t5) on the engine, FP64 and FP32.
 - The command line prints the reference's "DIFF NORM" lines, which the suite's awk reads.
+- The exp solutions of the 1D pairs (Tests/suite t7.x): engine against the oracle, real and
+  complex handles; the suite's bars at full size on every pair; the complex build's line.
 """
 import os
 import subprocess
@@ -20,7 +22,7 @@ import pytest
 from fdtd3d_amd.config import parse_args
 from fdtd3d_amd.engine import Engine
 from oracle.binding import Oracle
-from tests.helpers import TEST_FN_CASES, check_fp32, field_names, rel_l2, run_norms
+from tests.helpers import EXP_CASES, TEST_FN_CASES, check_fp32, field_names, rel_l2, run_norms
 
 pytestmark = pytest.mark.gpu
 
@@ -134,3 +136,122 @@ def test_cli_diff_norm_lines(engine_lib):
     eng.close()
     vals = np.array([[float(ln.split()[9]), float(ln.split()[13])] for ln in ez])
     assert np.allclose(vals, norms["Ez"][:, :2], rtol=1e-12, atol=1e-20)   # %.20f keeps 20 decimals
+
+
+def _exp_engine(cfg):
+    eng = Engine(cfg)
+    eng.set_material_sphere("eps", cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius,
+                            accuracy=cfg.sphere_accuracy, stair=cfg.stair_sphere)
+    eng.prepare()
+    return eng
+
+
+def _exp_oracles(cfg, kinds):
+    from oracle.binding import sphere_map
+    parts = []
+    for kind in kinds:
+        o = Oracle(cfg)
+        if kind:
+            o.set_source_kind(kind)
+        o.set_material("eps", sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius))
+        o.prepare()
+        parts.append(o)
+    return parts
+
+
+@pytest.mark.parametrize("args", EXP_CASES)
+def test_exp_solutions_vs_oracle(args, engine_lib):
+    """--calc-exp<n>-<pair>-diff-norm on the engine against the oracle (pinned bit for bit to the
+    reference's real and complex builds): FP64, real and complex handles; fields within 1e-12
+    (PML), every step's norms and percentages within 1e-8 relative (the device's sin / cos and
+    its order of the sums; the norm is a difference ~1e-3 of the fields)."""
+    import dataclasses
+    from oracle.binding import oracle_exp_mod_norm
+    from tests.helpers import run_norms_complex
+    cfg = parse_args(args.split())
+    say(f"[gpu] exp solution: {args[:12]} {args.split('--calc-')[1][:10]}")
+    # real build
+    (o,) = _exp_oracles(cfg, [0])
+    ref = run_norms(o, cfg)
+    eng = _exp_engine(cfg)
+    got = run_norms(eng, cfg)
+    assert sorted(got) == sorted(ref) and len(ref) == 2
+    for n in field_names(cfg):
+        assert rel_l2(eng.field(n), o.field(n)) <= 1e-12, n
+    for n in ref:
+        assert np.allclose(got[n][:, :2], ref[n][:, :2], rtol=1e-8, atol=0), n
+    eng.close()
+    o.close()
+    # complex build
+    ccfg = dataclasses.replace(cfg, complex_fields=True)
+    re_, im_ = _exp_oracles(cfg, [1, 2])
+    cref = run_norms_complex(re_, im_, cfg, oracle_exp_mod_norm)
+    eng = _exp_engine(ccfg)
+    cgot = {}
+    for _ in range(cfg.time_steps):
+        eng.step(1)
+        for n in cref:
+            v = eng.diff_norm_complex(n)
+            cgot.setdefault(n, []).append((v[0], v[3], v[1], v[4], v[2], v[5]))
+    for n in field_names(cfg):
+        assert rel_l2(eng.field(n), re_.field(n)) <= 1e-12, n
+        assert rel_l2(eng.imag().field(n), im_.field(n)) <= 1e-12, n
+    for n in cref:
+        assert np.allclose(np.array(cgot[n]), cref[n], rtol=1e-8, atol=0), n
+    eng.close()
+
+
+@pytest.mark.parametrize("pair", ["exhy", "exhz", "eyhx", "eyhz", "ezhx", "ezhy"])
+def test_suite_t7_plane_wave_and_dielectric(pair, engine_lib):
+    """Tests/suite t7.1 - t7.6 (one 1D pair each, complex build, 600 cells, dz 0.1 mm, lambda
+    2 cm, PML 50, TF/SF border at 150 on the left only, the dielectric from 300, 10000 steps):
+    exp1 in vacuum over [155, 545), exp3 inside eps = 4 over [305, 545), exp2 reflected over
+    [55, 145); at the last step the real, imaginary and module percentages of both components
+    stay under 0.12 / 0.12 / 0.17 %."""
+    import dataclasses
+    from tests.helpers import exp_case
+    for kind, norm, eps, bar in (("exp1", (155, 545), 1, 0.12), ("exp3", (305, 545), 4, 0.12),
+                                 ("exp2", (55, 145), 4, 0.17)):
+        args = exp_case(pair, kind, size=600, pml=50, tfsf=150, center=450, radius=150, steps=10000, dx="0.0001",
+                        wavelength="0.02", eps=eps, norm=norm)
+        cfg = dataclasses.replace(parse_args(args.split()), complex_fields=True)
+        eng = _exp_engine(cfg)
+        eng.step(cfg.time_steps)
+        worst = 0.0
+        for n in cfg.components:
+            v = eng.diff_norm_complex(n)
+            worst = max(worst, v[3], v[4], v[5])
+        eng.close()
+        say(f"[gpu] t7 {pair} {kind}: largest percentage {worst:.4f} (bar {bar})")
+        assert 0 < worst <= bar, (pair, kind, worst)
+
+
+def test_cli_complex_diff_norm_lines(engine_lib):
+    """fdtd3d-complex prints the complex build's line, "-> DIFF NORM Ex. Timestep t. Value = ( re ,
+    im ) = ( re % , im % ), module = m = ( m % )", whose fields 16, 19 and 27 the suite's awk
+    reads; the numbers are the engine's.  The real binary prints the real line for the same run."""
+    import dataclasses
+    from tests.helpers import exp_case
+    exe = os.path.join(BIN, "fdtd3d-complex")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    args = exp_case("exhy", "exp3", steps=120).split() + ["--log-level", "0"]
+    say("[gpu] cli complex diff norm lines")
+    out = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    ex = [ln for ln in out.stdout.splitlines() if "DIFF NORM Ex" in ln]
+    hy = [ln for ln in out.stdout.splitlines() if "DIFF NORM Hy" in ln]
+    assert len(ex) == 120 and len(hy) == 120
+    last = ex[-1].split()
+    assert ex[-1].startswith("-> DIFF NORM Ex. Timestep 119. Value = ( ") and last[20:22] == ["),", "module"]
+    cfg = dataclasses.replace(parse_args(args), complex_fields=True)
+    eng = _exp_engine(cfg)
+    eng.step(cfg.time_steps)
+    v = eng.diff_norm_complex("Ex")
+    eng.close()
+    printed = [float(last[i]) for i in (9, 11, 15, 18, 23, 26)]
+    assert np.allclose(printed, [v[0], v[1], v[3], v[4], v[2], v[5]], rtol=1e-12, atol=1e-20)
+    real = subprocess.run([os.path.join(BIN, "fdtd3d")] + args, capture_output=True, text=True, timeout=120)
+    assert real.returncode == 0, real.stderr
+    rl = [ln for ln in real.stdout.splitlines() if "DIFF NORM Ex" in ln]
+    assert len(rl) == 120 and rl[-1].startswith("-> DIFF NORM Ex. Timestep 119. Value = ( ") and rl[-1].endswith(" % )")

@@ -10,7 +10,7 @@ import pytest
 
 from fdtd3d_amd.config import parse_args
 from oracle.binding import Oracle, ref_driver_path, run_ref_driver, sphere_map
-from tests.helpers import DOUBLE_MATERIAL_CASES, TEST_FN_CASES, field_names, random_material
+from tests.helpers import DOUBLE_MATERIAL_CASES, EXP_CASES, TEST_FN_CASES, field_names, random_material
 
 pytestmark = pytest.mark.skipif(not os.path.exists(ref_driver_path("f64")), reason="reference driver not built")
 
@@ -281,3 +281,58 @@ def test_oracle_test_functions_match_reference_driver(args, tmp_path):
     assert sorted("Norm" + n for n in norms) == sorted(k for k in rec if k.startswith("Norm"))
     for n, v in norms.items():
         assert np.array_equal(v[:, :2].ravel(), rec["Norm" + n]), n
+
+
+def _exp_oracles(cfg, kinds):
+    parts = []
+    for kind in kinds:
+        o = Oracle(cfg)
+        if kind:
+            o.set_source_kind(kind)
+        o.set_material("eps", sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius))
+        o.prepare()
+        parts.append(o)
+    return parts
+
+
+@pytest.mark.parametrize("args", EXP_CASES)
+def test_oracle_exp_solutions_match_reference_driver(args, tmp_path):
+    """--calc-exp<n>-<pair>-diff-norm in the real build (a 1D wave meeting an eps = 4 half-space
+    drawn as a 1D sphere, PML, a left-only TF/SF border): the fields and every step's norm and
+    percentage equal the driver's, which runs the reference's own CallBack exp functions and
+    calculateFieldStepIterationExact (on E with PML, normGrid = gridPML2), bit for bit."""
+    from tests.helpers import run_norms
+    argv = args.split()
+    cfg = parse_args(argv)
+    rec, _ = run_ref_driver(argv, str(tmp_path / "r.bin"))
+    (o,) = _exp_oracles(cfg, [0])
+    assert np.array_equal(o.material(0).ravel(), rec["Eps"])
+    norms = run_norms(o, cfg)
+    for name in field_names(cfg):
+        assert np.array_equal(o.field(name).ravel(), rec[name]), name
+    assert sorted("Norm" + n for n in norms) == sorted(k for k in rec if k.startswith("Norm")) and len(norms) == 2
+    for n, v in norms.items():
+        assert np.array_equal(v[:, :2].ravel(), rec["Norm" + n]), n
+
+
+@pytest.mark.skipif(not os.path.exists(ref_driver_path("c64")), reason="complex reference driver not built")
+@pytest.mark.parametrize("args", EXP_CASES)
+def test_oracle_exp_solutions_match_reference_complex_build(args, tmp_path):
+    """The same in the complex build (the suite's own build for t7.x): both parts' fields and,
+    per step, the norms and percentages of the real part, the imaginary part and the module equal
+    the driver's bit for bit (the complex build's CComplex (cos, sin), which the compiler merges
+    into sincos, against the real build's lone sin: they differ in the last bit for rare
+    arguments, and each build is followed)."""
+    from oracle.binding import oracle_exp_mod_norm
+    from tests.helpers import run_norms_complex
+    argv = args.split()
+    cfg = parse_args(argv)
+    rec, _ = run_ref_driver(argv, str(tmp_path / "c.bin"), kind="c64")
+    re_, im_ = _exp_oracles(cfg, [1, 2])
+    norms = run_norms_complex(re_, im_, cfg, oracle_exp_mod_norm)
+    for name in field_names(cfg):
+        assert np.array_equal(re_.field(name).ravel(), rec[name]), name
+        assert np.array_equal(im_.field(name).ravel(), rec[name + ".im"]), name
+    assert len(norms) == 2
+    for n, v in norms.items():
+        assert np.array_equal(v, rec["Norm" + n].reshape(-1, 6)), n

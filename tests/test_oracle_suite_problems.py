@@ -9,6 +9,9 @@ suite's own pass criteria (its run.sh files), E_CENTERED layout:
   t5                  sin1 with eps / mu normed and Courant factor 149896229: the largest absolute
                       Ez norm falls by more than 3.3 per halving of dx (10 / 20 / 40 cells over
                       T + 1, 2 T + 1, 4 T + 1 steps, T = 40 and 80): second-order convergence
+  t7.1                ExHy, complex build: the incident (exp1), transmitted (exp3) and reflected
+                      (exp2) waves of a plane wave meeting eps = 4 (real, imaginary and module
+                      percentages at the last step under 0.12 / 0.12 / 0.17 %)
 
 The polynomial solutions are reproduced by the scheme exactly, up to rounding, so a wrong
 coordinate, time level, sign or operand order of a start value, border value or right side
@@ -67,3 +70,35 @@ def test_sin1_second_order_convergence(T):
     d2 = _sin1_max(20, "0.0005", 2 * T + 1)
     d3 = _sin1_max(40, "0.00025", 4 * T + 1)
     assert d1 / d2 > 3.3 and d2 / d3 > 3.3, (d1, d2, d3)
+
+
+@pytest.mark.parametrize("kind,norm,eps,bar", [
+    ("exp1", (155, 545), 1, 0.12),   # t7.1: the wave passes through vacuum
+    ("exp3", (305, 545), 4, 0.12),   # transmitted into eps = 4
+    ("exp2", (55, 145), 4, 0.17),    # reflected back into the scattered-field region
+])
+def test_t71_plane_wave_and_dielectric(kind, norm, eps, bar):
+    """Tests/suite t7.1 (ExHy, complex build, 600 cells, dz 0.1 mm, lambda 2 cm, PML 50, TF/SF
+    border at 150 on the left only, the dielectric from 300, 10000 steps): at the last step the
+    percentages of the real part, the imaginary part and the module stay under the script's bar
+    for Ex and Hy."""
+    from oracle.binding import oracle_exp_mod_norm
+    from oracle.binding import sphere_map
+    from tests.helpers import exp_case
+    args = exp_case("exhy", kind, size=600, pml=50, tfsf=150, center=450, radius=150, steps=10000, dx="0.0001",
+                    wavelength="0.02", eps=eps, norm=norm)
+    cfg = parse_args(args.split())
+    parts = []
+    for k in (1, 2):
+        o = Oracle(cfg)
+        o.set_source_kind(k)
+        o.set_material("eps", sphere_map(cfg, cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius))
+        o.prepare()
+        o.step(cfg.time_steps)
+        parts.append(o)
+    worst = 0.0
+    for n in ("Ex", "Hy"):
+        worst = max(worst, parts[0].diff_norm(n)[1], parts[1].diff_norm(n)[1], oracle_exp_mod_norm(parts[0], parts[1], n)[1])
+    for o in parts:
+        o.close()
+    assert 0 < worst <= bar, worst
