@@ -450,3 +450,49 @@ def test_cli_double_material_precision(engine_lib, tmp_path):
     out = subprocess.run([exe] + (base + " --load-eps-from-file short.dat").split(), capture_output=True, text=True,
                          timeout=120, cwd=tmp_path)
     assert out.returncode != 0
+
+
+def test_cli_complex_dumps(engine_lib, tmp_path):
+    """fdtd3d-complex (the reference's -DCOMPLEX_FIELD_VALUES build): text dumps carry "x y z re im"
+    (TXTDumper.h:78-82), binary dumps both doubles of each FieldValue, the materials an imaginary
+    part of 0; the values are the engine's two parts."""
+    import dataclasses
+    from fdtd3d_amd.config import parse_args
+    exe = os.path.join(BIN, "fdtd3d-complex")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    base = ("--3d --size x:16,y:14,z:18 --use-pml --pml-size x:3 --same-size-pml --use-tfsf --tfsf-size-left x:5 "
+            "--tfsf-size-right x:5 --same-size-tfsf --angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 3 "
+            "--eps-sphere-center x:8,y:7,z:9 --eps-sphere-radius 3 --time-steps 30")
+    args = base + " --save-res --save-as-txt --save-as-dat --save-materials --save-tfsf-e-incident"
+    say("[gpu] cli complex dumps")
+    out = subprocess.run([exe] + args.split(), capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    cfg = dataclasses.replace(parse_args(base.split()), complex_fields=True)
+    eng = Engine(cfg)
+    eng.set_material_sphere("eps", cfg.eps_sphere, 1.0, cfg.eps_sphere_center, cfg.eps_sphere_radius,
+                            accuracy=cfg.sphere_accuracy, stair=cfg.stair_sphere)
+    eng.prepare()
+    eng.step(cfg.time_steps)
+    eng.sync()
+    im = eng.imag()
+    for name in ("Ex", "Hz"):
+        rows = np.loadtxt(tmp_path / f"previous-1_[timestep=30]_[pid=0]_[name={name}].txt")
+        assert rows.shape == (16 * 14, 5) and (rows[:, 2] == 9).all()
+        assert np.allclose(rows[:, 3], eng.field(name)[:, :, 9].ravel(), rtol=1e-14, atol=1e-300), name
+        assert np.allclose(rows[:, 4], im.field(name)[:, :, 9].ravel(), rtol=1e-14, atol=1e-300), name
+        assert np.abs(rows[:, 4]).max() > 0
+        raw = np.fromfile(tmp_path / f"previous-1_[timestep=30]_[pid=0]_[name={name}].dat", dtype=np.float64)
+        assert raw.size == 2 * 16 * 14
+        assert np.array_equal(raw[0::2], eng.field(name)[:, :, 9].ravel()), name
+        assert np.array_equal(raw[1::2], im.field(name)[:, :, 9].ravel()), name
+    eps = np.loadtxt(tmp_path / "previous-0_[timestep=0]_[pid=0]_[name=Eps].txt")
+    assert eps.shape[1] == 5 and (eps[:, 4] == 0).all()
+    assert np.allclose(eps[:, 3], eng.material("eps")[:, :, 9].ravel(), rtol=1e-14, atol=0)
+    inc = np.loadtxt(tmp_path / "previous-1_[timestep=30]_[pid=0]_[name=EInc].txt")
+    e_re, _ = eng.incident()
+    e_im, _ = im.incident(500 * 16)
+    assert inc.shape == (500 * 16, 3)
+    assert np.allclose(inc[:, 1], e_re[: 500 * 16], rtol=1e-14, atol=1e-300)
+    assert np.allclose(inc[:, 2], e_im[: 500 * 16], rtol=1e-14, atol=1e-300)
+    eng.close()
