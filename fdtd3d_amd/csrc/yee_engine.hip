@@ -415,9 +415,11 @@ int slab_boxes (const yee_handle *h, int64_t lo[6][3], int64_t hi[6][3])
 }
 
 // The cells next to the x and y faces of the TF/SF box (tfsf_near_host): for x and y the planes
-// L-1, R-1 and R, spanning [L-1, R] on the other axes, clipped to the interior (the z faces
-// are corrected inside the fused kernel).  Boxes may overlap; a cell updated twice from the
-// old buffer gets the same value both times.
+// L-1, R-1 and R, spanning [L-1, R] on the other axes, clipped to the fused kernel's box (the
+// z faces are corrected inside the fused kernel; the face cells outside its box lie in the slab
+// boxes).  The face boxes overlap one another only inside the fused box, where no cell has PML
+// or Drude state: a cell updated twice reads the old E / H buffer both times and gets the same
+// value, while the in-place D / B of the slab cells are updated exactly once.
 int face_boxes (const yee_handle *h, int64_t lo[9][3], int64_t hi[9][3])
 {
   if (!h->use_tfsf ()) return 0;
@@ -433,8 +435,8 @@ int face_boxes (const yee_handle *h, int64_t lo[9][3], int64_t hi[9][3])
       {
         const int64_t Lb = h->cfg.tfsf_left[b], Rb = h->n[b] - h->cfg.tfsf_right[b];
         const int64_t l = b == ax ? pl : Lb - 1, u = b == ax ? pl + 1 : Rb + 1;
-        lo[m][b] = std::max<int64_t> (l, 1);
-        hi[m][b] = std::min<int64_t> (u, h->n[b] - 1);
+        lo[m][b] = std::max<int64_t> (l, h->fbox_lo[b]);
+        hi[m][b] = std::min<int64_t> (u, h->fbox_hi[b]);
         ok = ok && hi[m][b] > lo[m][b];
       }
       if (ok) ++m;
@@ -1109,9 +1111,10 @@ int yee_create (const yee_config *cfg, yee_handle **out)
       h->fbox_hi[ax] = std::min<int64_t> (h->n[ax] - 1, h->n[ax] - P - 1);
       ok = ok && h->fbox_hi[ax] > h->fbox_lo[ax];
     }
-    // two buffers of E/H (6), D/B (6 with PML) and the Drude history (18 more)
-    const int arrays = 6 + (h->mode >= 1 ? 6 : 0) + (h->mode == 2 ? 18 : 0);
-    const double bytes = 2.0 * arrays * (double) h->ncell * (double) h->wsize;
+    // two buffers of E/H (6), one of D/B (6 with PML), two of D/B and the Drude history with
+    // metamaterials (24)
+    const int arrays = 12 + (h->mode == 1 ? 6 : 0) + (h->mode == 2 ? 48 : 0);
+    const double bytes = (double) arrays * (double) h->ncell * (double) h->wsize;
     h->fused = ok && bytes < 200e9;
   }
   if (h->fused)
@@ -1148,12 +1151,26 @@ int yee_create (const yee_config *cfg, yee_handle **out)
       h->zc_n[ax] = (h->n[ax] - cfg->tfsf_right[ax]) - cfg->tfsf_left[ax] + 2;
     if ((rc = alloc_zero (h, &h->zcorr, (size_t) (6 * h->zc_n[0] * h->zc_n[1] * 4) * sizeof (double)))) return bail (rc);
   }
+  // E / H are double-buffered for the fused kernel (it recomputes neighbour cells from old
+  // values); the PML D / B are touched only by the half-step kernels, which read every old value
+  // of a cell before they store it and update each slab cell in exactly one launch (slab boxes
+  // are disjoint, face boxes are clipped to the fused box), so they update in place: the second
+  // buffer's views alias the first's.  The Drude box overlaps slab and face boxes, so Drude
+  // runs keep two buffers.
   const int nbuf = h->fused ? 2 : 1;
   for (int bsel = 0; bsel < nbuf; ++bsel)
     for (int c = 0; c < 6; ++c)
     {
       if (!h->present[c]) continue;
       if ((rc = alloc_grid (h, &h->f[bsel][c]))) return bail (rc);
+      if (bsel == 1 && h->mode == 1)
+      {
+        h->d[1][c] = h->d[0][c];
+        h->dp[1][c] = h->dp[0][c];
+        h->d1[1][c] = h->d1[0][c];
+        h->d1p[1][c] = h->d1p[0][c];
+        continue;
+      }
       if (h->mode >= 1 && (rc = alloc_grid (h, &h->d[bsel][c]))) return bail (rc);
       if (h->mode == 2)
       {
