@@ -1568,6 +1568,12 @@ int yee_ntff (yee_handle *h, double teta, const double *phi, int32_t nphi, const
   for (int a = 0; a < 3; ++a)
     if (left[a] < 2 || right[a] > h->n[a] - 2 || right[a] <= left[a])
       return fail (YEE_EINVAL, "the box must satisfy 2 <= left < right <= size - 2 along every axis");
+  for (int a = 0; a < 3; ++a)
+  {
+    // a face's cell count is a 32-bit loop bound of the face kernel
+    const int64_t u = right[(a + 1) % 3] - left[(a + 1) % 3], v = right[(a + 2) % 3] - left[(a + 2) % 3];
+    if (u * v >= ((int64_t) 1 << 31)) return fail (YEE_EINVAL, "box face too large");
+  }
   if (h->t < 1 || h->ntff_t != h->t - 1 || h->imag->ntff_t != h->t - 1)
     return fail (YEE_ESTATE, "yee_ntff_snapshot must be taken one step before (the N sums average H over the "
                              "newest two steps)");
