@@ -475,6 +475,9 @@ def parse_args(argv: Sequence[str], precision: int = 8) -> Config:
     if cfg.use_metamaterials and cfg.double_material:
         raise OptionError("--use-double-material-precision excludes --use-metamaterials (the reference's "
                           "8-cell Drude average is UNREACHABLE, Approximation.cpp:196)")
-    if cfg.layout != 0:
-        raise OptionError("only --layout-type 0 (E_CENTERED) is supported")
+    if cfg.layout not in (0, 1):
+        raise OptionError("--layout-type must be 0 (E_CENTERED) or 1 (H_CENTERED)")
+    if cfg.layout == 1 and (cfg.use_metamaterials or cfg.double_material):
+        raise OptionError("the H_CENTERED layout (--layout-type 1) runs without --use-metamaterials and "
+                          "--use-double-material-precision")
     return cfg

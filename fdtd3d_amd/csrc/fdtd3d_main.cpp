@@ -417,12 +417,13 @@ std::vector<double> load_material (const std::string &path, int dim, const int64
   return {};
 }
 
-// Minimum coordinate of a component in the E_CENTERED layout, doubled (YeeGridLayout.cpp:1937-2043):
-// Ex (1, .5, .5), Ey (.5, 1, .5), Ez (.5, .5, 1), Hx (.5, 1, 1), Hy (1, .5, 1), Hz (1, 1, .5).
-int min_coord2 (int comp, int axis)
+// Minimum coordinate of a component, doubled (YeeGridLayout.cpp:1937-2043): E_CENTERED
+// Ex (1, .5, .5), Ey (.5, 1, .5), Ez (.5, .5, 1), Hx (.5, 1, 1), Hy (1, .5, 1), Hz (1, 1, .5);
+// H_CENTERED swaps the E and H rows.
+int min_coord2 (int comp, int axis, int layout)
 {
   static const int m[6][3] = { { 2, 1, 1 }, { 1, 2, 1 }, { 1, 1, 2 }, { 1, 2, 2 }, { 2, 1, 2 }, { 2, 2, 1 } };
-  return m[comp][axis];
+  return m[layout == 1 ? (comp + 3) % 6 : comp][axis];
 }
 
 // Dimension of the scheme's grid.
@@ -512,7 +513,7 @@ void dump_range (const Options &o, int comp, int dim, const int64_t size[3], int
     for (int s = 0; s < dim; ++s)
     {
       const int64_t L = o.cfg.pml[ax[s]], R = size[s] - o.cfg.pml[ax[s]];
-      const int m2 = comp < 0 ? 0 : min_coord2 (comp, ax[s]);
+      const int m2 = comp < 0 ? 0 : min_coord2 (comp, ax[s], o.cfg.layout);
       start[s] = (2 * L - m2) / 2 + 1;
       end[s] = (2 * R - m2) / 2;
     }

@@ -569,9 +569,10 @@ int yee_adjoint_setup (yee_handle *h, const yee_adjoint_spec *sp)
 {
   if (!h || !sp) return fail (YEE_EINVAL, "null argument");
   if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
-  if (h->cfg.scheme != YEE_SCHEME_DIM3 || h->mode == 2 || h->cfg.current_source || h->imag || h->fine () || h->fn_any ())
-    return fail (YEE_ENOTSUP, "the adjoint covers 3D runs of real fields without metamaterials, current sources, "
-                              "double material precision or test functions");
+  if (h->cfg.scheme != YEE_SCHEME_DIM3 || h->mode == 2 || h->cfg.current_source || h->imag || h->fine () || h->fn_any ()
+      || h->cfg.layout != 0)
+    return fail (YEE_ENOTSUP, "the adjoint covers 3D E_CENTERED runs of real fields without metamaterials, current "
+                              "sources, double material precision or test functions");
   const int T = h->cfg.time_steps;
   if (T < 2) return fail (YEE_EINVAL, "the adjoint needs --time-steps >= 2");
   if (sp->window < 1 || sp->window > T - 1) return fail (YEE_EINVAL, "window must be in [1, time_steps - 1]");

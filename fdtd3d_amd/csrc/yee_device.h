@@ -115,9 +115,21 @@ __device__ __forceinline__ float direct_cb (float dt, float m, float m0, float d
   return dt * __builtin_amdgcn_rcpf (m * m0 * dx);
 }
 
-// E_CENTERED component tables (the host CompDesc plus the circuit offsets of
-// YeeGridLayout.cpp:1937-1961).  Every lookup below is resolved at compile time: runtime
-// indexing of small arrays would push them to scratch memory.
+// Component tables (the host CompDesc plus the circuit offsets of YeeGridLayout.cpp:1937-1961,
+// E_CENTERED, and :1991-2015, H_CENTERED).  Every lookup below is resolved at compile time:
+// runtime indexing of small arrays would push them to scratch memory.
+//
+// LAY selects the Yee layout: 0 = E_CENTERED, 1 = H_CENTERED.  H_CENTERED puts each E
+// component where E_CENTERED puts the H component of the same axis and the reverse
+// (YeeGridLayout.cpp:1962-1967, 2016-2021); every position-dependent rule (stencil offsets,
+// TF/SF predicates and corrected slot, material averages, sigma positions) follows the
+// position, i.e. the component's role: "H-like" for H under E_CENTERED and E under H_CENTERED.
+template <int COMP, int LAY>
+constexpr bool role_h ()
+{
+  return (COMP >= 3) != (LAY != 0);
+}
+
 struct Tab
 {
   // slot directions (LEFT=0, RIGHT=1, DOWN=2, UP=3, BACK=4, FRONT=5) of d11, d12, d21, d22
@@ -139,7 +151,22 @@ struct Tab
     { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, 0 }, { 0, 0, 1 } },
     { { 0, 0, 0 }, { 0, 1, 0 }, { 0, 0, 0 }, { 1, 0, 0 } },
   };
+  // the same under H_CENTERED: E reads H at {0, +1}, H reads E at {-1, 0}
+  static constexpr int offh[6][4][3] = {
+    { { 0, 0, 0 }, { 0, 1, 0 }, { 0, 0, 0 }, { 0, 0, 1 } },
+    { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 0, 0 }, { 1, 0, 0 } },
+    { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, 0 }, { 0, 1, 0 } },
+    { { 0, 0, -1 }, { 0, 0, 0 }, { 0, -1, 0 }, { 0, 0, 0 } },
+    { { -1, 0, 0 }, { 0, 0, 0 }, { 0, 0, -1 }, { 0, 0, 0 } },
+    { { 0, -1, 0 }, { 0, 0, 0 }, { -1, 0, 0 }, { 0, 0, 0 } },
+  };
 };
+
+template <int COMP, int SLOT, int AX, int LAY>
+constexpr int slot_off ()
+{
+  return LAY ? Tab::offh[COMP][SLOT][AX] : Tab::off[COMP][SLOT][AX];
+}
 
 // sigma axes (curl, sigma4, sigma5) per component
 template <int COMP> struct SigAxes;
@@ -157,48 +184,50 @@ __device__ __forceinline__ int pick (int i, int j, int k)
 }
 
 // Twice the real coordinate of component COMP along axis AX at integer index p.
-template <int COMP, int AX>
+template <int COMP, int AX, int LAY = 0>
 __device__ __forceinline__ int twice_coord (int p)
 {
-  constexpr bool h = COMP >= 3;
+  constexpr bool h = role_h<COMP, LAY> ();
   constexpr int own = COMP % 3;
   return 2 * p + (h ? (AX == own ? 1 : 2) : (AX == own ? 2 : 1));
 }
 
 // doNeedTFSFUpdate<comp>Border(pos, dir) on doubled integer coordinates (YeeGridLayout.cpp:272-1740;
 // the FP tests c == b +- 0.5, c in (L - 0.1, R + 0.1), c in (L + 0.4, R - 0.4) are exact on half-integers).
-template <typename T, int COMP, int DIR, int DIM>
+// The H_CENTERED predicates (YeeGridLayout.cpp:44-1175) are those of the component role: an
+// H_CENTERED E component tests what an E_CENTERED H component at the same point tests.
+template <typename T, int COMP, int DIR, int DIM, int LAY = 0>
 __device__ __forceinline__ bool tfsf_pred (const StepArgs<T> &a, int i, int j, int k)
 {
   constexpr int b = DIR >> 1;
   constexpr bool low = (DIR & 1) == 0;
   constexpr int ax = COMP % 3;
-  constexpr bool h = COMP >= 3;
+  constexpr bool h = role_h<COMP, LAY> ();
   constexpr int t = 3 - b - ax;
   if (b == ax || b >= DIM) return false;
-  const int cb2 = twice_coord<COMP, b> (pick<b> (i, j, k));
+  const int cb2 = twice_coord<COMP, b, LAY> (pick<b> (i, j, k));
   bool ok = h ? (low ? cb2 == 2 * a.tl[b] : cb2 == 2 * a.tr[b])
               : (low ? cb2 == 2 * a.tl[b] - 1 : cb2 == 2 * a.tr[b] + 1);
   if (ax < DIM)
   {
-    const int c2 = twice_coord<COMP, ax> (pick<ax> (i, j, k));
+    const int c2 = twice_coord<COMP, ax, LAY> (pick<ax> (i, j, k));
     ok = ok && (h ? (c2 >= 2 * a.tl[ax] + 1 && c2 <= 2 * a.tr[ax] - 1) : (c2 >= 2 * a.tl[ax] && c2 <= 2 * a.tr[ax]));
   }
   if (t < DIM)
   {
-    const int c2 = twice_coord<COMP, t> (pick<t> (i, j, k));
+    const int c2 = twice_coord<COMP, t, LAY> (pick<t> (i, j, k));
     ok = ok && (h ? (c2 >= 2 * a.tl[t] && c2 <= 2 * a.tr[t]) : (c2 >= 2 * a.tl[t] + 1 && c2 <= 2 * a.tr[t] - 1));
   }
   return ok;
 }
 
 // Runtime-direction form (mask export only).
-template <typename T, int COMP, int DIM>
+template <typename T, int COMP, int DIM, int LAY = 0>
 __device__ uint8_t tfsf_bits (const StepArgs<T> &a, int i, int j, int k)
 {
-  return (uint8_t) ((tfsf_pred<T, COMP, 0, DIM> (a, i, j, k) ? 1 : 0) | (tfsf_pred<T, COMP, 1, DIM> (a, i, j, k) ? 2 : 0)
-                    | (tfsf_pred<T, COMP, 2, DIM> (a, i, j, k) ? 4 : 0) | (tfsf_pred<T, COMP, 3, DIM> (a, i, j, k) ? 8 : 0)
-                    | (tfsf_pred<T, COMP, 4, DIM> (a, i, j, k) ? 16 : 0) | (tfsf_pred<T, COMP, 5, DIM> (a, i, j, k) ? 32 : 0));
+  return (uint8_t) ((tfsf_pred<T, COMP, 0, DIM, LAY> (a, i, j, k) ? 1 : 0) | (tfsf_pred<T, COMP, 1, DIM, LAY> (a, i, j, k) ? 2 : 0)
+                    | (tfsf_pred<T, COMP, 2, DIM, LAY> (a, i, j, k) ? 4 : 0) | (tfsf_pred<T, COMP, 3, DIM, LAY> (a, i, j, k) ? 8 : 0)
+                    | (tfsf_pred<T, COMP, 4, DIM, LAY> (a, i, j, k) ? 16 : 0) | (tfsf_pred<T, COMP, 5, DIM, LAY> (a, i, j, k) ? 32 : 0));
 }
 
 // A cell can carry a TF/SF correction only on the planes L-1, R-1, R of some axis while
@@ -219,16 +248,16 @@ __device__ __forceinline__ bool tfsf_near (const StepArgs<T> &a, int i, int j, i
 
 // Incident value of component OPP at integer position (i, j, k): approximateIncidentWave +
 // get<opp>FromIncident (InternalScheme.cpp:375-407, InternalScheme.inc.h:500-515).
-template <typename T, int OPP, int DIM>
+template <typename T, int OPP, int DIM, int LAY = 0>
 __device__ double incident_at (const StepArgs<T> &a, int i, int j, int k, const T *inc, double ddiff)
 {
   const TfsfGeom<T> &tg = *a.tg;
-  const double x = 0.5 * (double) twice_coord<OPP, 0> (i) - tg.zinc[0];
-  const double y = 0.5 * (double) twice_coord<OPP, 1> (j) - tg.zinc[1];
+  const double x = 0.5 * (double) twice_coord<OPP, 0, LAY> (i) - tg.zinc[0];
+  const double y = 0.5 * (double) twice_coord<OPP, 1, LAY> (j) - tg.zinc[1];
   double d;
   if (DIM == 3)
   {
-    const double z = 0.5 * (double) twice_coord<OPP, 2> (k) - tg.zinc[2];
+    const double z = 0.5 * (double) twice_coord<OPP, 2, LAY> (k) - tg.zinc[2];
     d = x * tg.S1 * tg.C2 + y * tg.S1 * tg.S2 + z * tg.C1 - ddiff;
   }
   else
@@ -276,37 +305,42 @@ __device__ __forceinline__ bool tfsf_pair_diff (const StepArgs<T> &a, int i, int
   return true;
 }
 
-template <typename T, int COMP, int PAIR, int DIM>
+// The incident array and its half-step shift follow the physical type (E reads HInc, H reads
+// EInc); the corrected slot and the sign follow the role (isRevertVals and the layout's sign
+// flip, InternalScheme.inc.h:1405-1835): an H-like component corrects the slot on the border
+// side with the sign flipped, an E-like one the slot opposite to it.
+template <typename T, int COMP, int PAIR, int DIM, int LAY = 0>
 __device__ __forceinline__ void tfsf_pair (const StepArgs<T> &a, int i, int j, int k, T &v_lo, T &v_hi)
 {
   constexpr bool h = COMP >= 3;
+  constexpr bool rh = role_h<COMP, LAY> ();
   constexpr int OPP = Tab::opp[COMP][PAIR];
-  const bool lo = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR], DIM> (a, i, j, k);
-  const bool hi = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR + 1], DIM> (a, i, j, k);
+  const bool lo = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR], DIM, LAY> (a, i, j, k);
+  const bool hi = tfsf_pred<T, COMP, Tab::dir[COMP][2 * PAIR + 1], DIM, LAY> (a, i, j, k);
   if (!lo && !hi) return;
   const T *inc = h ? a.tg->einc : a.tg->hinc[a.hinc_sel];
   const double ddiff = h ? 0.0 : 0.5;
   // slot 2*PAIR is "lo" (d11 / d21), slot 2*PAIR+1 is "hi" (d12 / d22)
-  const bool use_hi_slot = h ? !lo : lo;
+  const bool use_hi_slot = rh ? !lo : lo;
   constexpr int s_lo = 2 * PAIR, s_hi = 2 * PAIR + 1;
   double diff;
   if (use_hi_slot)
-    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_hi][0], j + Tab::off[COMP][s_hi][1],
-                                     k + (DIM == 3 ? Tab::off[COMP][s_hi][2] : 0), inc, ddiff);
+    diff = incident_at<T, OPP, DIM, LAY> (a, i + slot_off<COMP, s_hi, 0, LAY> (), j + slot_off<COMP, s_hi, 1, LAY> (),
+                                          k + (DIM == 3 ? slot_off<COMP, s_hi, 2, LAY> () : 0), inc, ddiff);
   else
-    diff = incident_at<T, OPP, DIM> (a, i + Tab::off[COMP][s_lo][0], j + Tab::off[COMP][s_lo][1],
-                                     k + (DIM == 3 ? Tab::off[COMP][s_lo][2] : 0), inc, ddiff);
-  if (h) diff *= -1.0;
+    diff = incident_at<T, OPP, DIM, LAY> (a, i + slot_off<COMP, s_lo, 0, LAY> (), j + slot_off<COMP, s_lo, 1, LAY> (),
+                                          k + (DIM == 3 ? slot_off<COMP, s_lo, 2, LAY> () : 0), inc, ddiff);
+  if (rh) diff *= -1.0;
   if (use_hi_slot) v_hi = (T) ((double) v_hi - diff);
   else v_lo = (T) ((double) v_lo - diff);
 }
 
 // calculateTFSF for one component at cell (i, j, k): corrects the operands v11, v12, v21, v22.
-template <typename T, int COMP, int DIM>
+template <typename T, int COMP, int DIM, int LAY = 0>
 __device__ __forceinline__ void tfsf_apply (const StepArgs<T> &a, int i, int j, int k, T v[4])
 {
-  tfsf_pair<T, COMP, 0, DIM> (a, i, j, k, v[0], v[1]);
-  tfsf_pair<T, COMP, 1, DIM> (a, i, j, k, v[2], v[3]);
+  tfsf_pair<T, COMP, 0, DIM, LAY> (a, i, j, k, v[0], v[1]);
+  tfsf_pair<T, COMP, 1, DIM, LAY> (a, i, j, k, v[2], v[3]);
 }
 
 // Inside the Drude box: the field cells whose 2- or 4-cell averages reach a nonzero Drude cell.
@@ -457,12 +491,13 @@ __device__ __forceinline__ void test_exp (const StepArgs<T> &a, int kind, int co
 }
 
 // The real coordinate (m) of component COMP at cell p: (p + the component's minimum coordinate,
-// E_CENTERED: 1 along an E component's own axis and 0.5 across it, the reverse for H) * fn_dx.
+// E_CENTERED: 1 along an E component's own axis and 0.5 across it, the reverse for H; H_CENTERED
+// the reverse of both) * fn_dx.
 template <typename T>
 __device__ __forceinline__ void test_coords (const StepArgs<T> &a, int comp, const int p[3], double r[3])
 {
   const int axis = comp % 3;
-  const bool h = comp >= 3;
+  const bool h = (comp >= 3) != (a.layout != 0);
   for (int d = 0; d < 3; ++d) r[d] = ((double) p[d] + ((d == axis) != h ? 1.0 : 0.5)) * a.fn_dx[d];
 }
 

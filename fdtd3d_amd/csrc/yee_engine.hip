@@ -24,6 +24,8 @@
 namespace yee {
 
 template <typename T, int MODE> __global__ void e_half_3d (StepArgs<T>);
+template <typename T, int MODE> __global__ void e_half_3d_hc (StepArgs<T>);
+template <typename T, int MODE> __global__ void h_half_3d_hc (StepArgs<T>);
 template <typename T, int MODE> __global__ void h_half_3d (StepArgs<T>);
 template <typename T, int MODE> __global__ void e_half_tmz (StepArgs<T>);
 template <typename T, int MODE> __global__ void h_half_tmz (StepArgs<T>);
@@ -199,6 +201,7 @@ StepArgs<T> make_args (const yee_handle *h, int parity, ArgRole role = kRoleE)
   a.exp_w = h->exp_w;
   a.exp_t0 = h->exp_t0;
   a.exp_m = h->exp_m;
+  a.layout = h->cfg.layout;
   return a;
 }
 
@@ -232,7 +235,18 @@ int launch_half (yee_handle *h, bool e_half, int parity, const int64_t *lo = nul
       for (int ax = 0; ax < 3; ++ax) meets = meets && a.run_lo[ax] < a.box_hi[ax] && a.box_lo[ax] < a.run_hi[ax];
       if (!meets) mode = 1;
     }
-    switch (mode * 2 + (e_half ? 0 : 1))
+    if (h->cfg.layout == 1)
+    {
+      // H_CENTERED: plain or PML runs only (validate refuses the rest)
+      switch (mode * 2 + (e_half ? 0 : 1))
+      {
+        case 0: hipLaunchKernelGGL ((e_half_3d_hc<T, 0>), grid, block, 0, h->stream, a); break;
+        case 1: hipLaunchKernelGGL ((h_half_3d_hc<T, 0>), grid, block, 0, h->stream, a); break;
+        case 2: hipLaunchKernelGGL ((e_half_3d_hc<T, 1>), grid, block, 0, h->stream, a); break;
+        default: hipLaunchKernelGGL ((h_half_3d_hc<T, 1>), grid, block, 0, h->stream, a); break;
+      }
+    }
+    else switch (mode * 2 + (e_half ? 0 : 1))
     {
       case 0: hipLaunchKernelGGL ((e_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
       case 1: hipLaunchKernelGGL ((h_half_3d<T, 0>), grid, block, 0, h->stream, a); break;
@@ -704,7 +718,9 @@ int validate (const yee_config *c)
   if (c->scheme < YEE_SCHEME_EXHY || c->scheme > YEE_SCHEME_DIM3)
     return fail (YEE_EINVAL, "scheme must be YEE_SCHEME_DIM3 (13), a 2D scheme TEx .. TMz (7 .. 12) or a 1D "
                              "scheme ExHy .. EzHy (1 .. 6)");
-  if (c->layout != 0) return fail (YEE_EINVAL, "only the E_CENTERED layout (0) is supported");
+  if (c->layout != 0 && c->layout != 1) return fail (YEE_EINVAL, "layout must be 0 (E_CENTERED) or 1 (H_CENTERED)");
+  if (c->layout == 1 && (c->flags & (YEE_USE_METAMATERIALS | YEE_DOUBLE_MATERIAL_PRECISION)))
+    return fail (YEE_EINVAL, "the H_CENTERED layout runs without metamaterials and double material precision");
   if (c->precision != 4 && c->precision != 8) return fail (YEE_EINVAL, "precision must be 4 or 8");
   for (int a = 0; a < 3; ++a)
   {
@@ -890,6 +906,7 @@ int ntff_snapshot_part (yee_handle *h)
 int ntff_check (const yee_handle *h)
 {
   if (!h->imag) return fail (YEE_ENOTSUP, "the near-to-far-field transform needs complex field values (YEE_COMPLEX_FIELDS)");
+  if (h->cfg.layout != 0) return fail (YEE_ENOTSUP, "the near-to-far-field transform runs on the E_CENTERED layout only");
   if (h->dim != 3 || h->miss >= 0)
     return fail (YEE_ENOTSUP, "the near-to-far-field transform runs on 3D schemes only (the reference's 1D / 2D "
                               "variants are unreachable)");
@@ -1047,14 +1064,16 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   // extent 1 (neighbours along it are the cell itself, the offsets the reference drops); TEz
   // and TMz have kernels of their own
   // 1D schemes run on the 3D kernels with both other axes of extent 1
+  // (H_CENTERED runs TEz / TMz on the 3D kernels too: the 2D kernels are E_CENTERED only)
   const int miss = missing_axis (*cfg), line = line_axis (*cfg);
-  h->miss = miss == 2 ? -1 : miss;
+  const bool hc = cfg->layout == 1;
+  h->miss = (miss == 2 && !hc) ? -1 : miss;
   h->dim = (cfg->scheme == YEE_SCHEME_DIM3 || h->miss >= 0 || line >= 0) ? 3 : 2;
   for (int a = 0; a < 3; ++a)
   {
     h->n[a] = axis_active (*cfg, a) ? cfg->size[a] : 1;
     if (axis_active (*cfg, a)) continue;
-    h->lack[a] = a != 2 || line >= 0;      // TEz / TMz keep kernels of their own
+    h->lack[a] = a != 2 || line >= 0 || hc;   // TEz / TMz keep kernels of their own (E_CENTERED)
     h->cfg.pml[a] = 0;                     // no PML and no TF/SF border along a lacked axis
     h->cfg.point_source_pos[a] = 0;        // the source positions drop that coordinate
     h->cfg.current_source_pos[a] = 0;
@@ -1101,7 +1120,8 @@ int yee_create (const yee_config *cfg, yee_handle **out)
   const bool want_fused = !(kern && std::strcmp (kern, "split") == 0);
   h->fused = false;
   // (double material precision keeps the half-step kernels: they read per-component averages)
-  if (h->dim == 3 && h->miss < 0 && line < 0 && h->n[2] <= 1024 && want_fused && !h->fine ())
+  // (the fused kernel is E_CENTERED: H_CENTERED runs take the half-step kernels)
+  if (h->dim == 3 && h->miss < 0 && line < 0 && h->n[2] <= 1024 && want_fused && !h->fine () && cfg->layout == 0)
   {
     bool ok = true;
     for (int ax = 0; ax < 3; ++ax)
@@ -1268,10 +1288,12 @@ int yee_set_material_sphere (yee_handle *h, int32_t which, double value, double 
             }
             else if (h->miss >= 0)
             {
-              // a 2D scheme on the (y, z) or (x, z) plane: the 2D sphere on its two coordinates
+              // a 2D scheme on the 3D kernels ((y, z), (x, z), or (x, y) under H_CENTERED): the 2D
+              // sphere on its two coordinates
               const double ma = h->miss == 0 ? my : mx, ca = h->miss == 0 ? cy : cx;
-              val = stair ? sphere_stair_2d (ma, mz, ca, cz, radius, value, outside)
-                          : sphere_accurate_2d (ma, mz, ca, cz, radius, value, outside, accuracy);
+              const double mb = h->miss == 2 ? my : mz, cb = h->miss == 2 ? cy : cz;
+              val = stair ? sphere_stair_2d (ma, mb, ca, cb, radius, value, outside)
+                          : sphere_accurate_2d (ma, mb, ca, cb, radius, value, outside, accuracy);
             }
             else if (h->dim == 3)
               val = stair ? sphere_stair_3d (mx, my, mz, cx, cy, cz, radius, value, outside)
@@ -1768,6 +1790,7 @@ int scattered_checks (yee_handle *h, int32_t comp, const double *out)
   if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
   // the reference interpolates EInc / HInc, which exist only with TF/SF
   if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
+  if (h->cfg.layout != 0) return fail (YEE_ENOTSUP, "the scattered field is computed for the E_CENTERED layout only");
   if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
   return YEE_OK;
 }
@@ -1852,6 +1875,7 @@ int yee_get_mask (yee_handle *h, int32_t kind, int32_t comp, uint8_t *out)
   if (kind == 0)
   {
     if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "bad component");
+    if (h->cfg.layout != 0) return fail (YEE_ENOTSUP, "the TF/SF mask export covers the E_CENTERED layout only");
     uint8_t *dev = nullptr;
     HIP_TRY (hipMalloc (&dev, (size_t) h->ncell));
     const int grid = (int) std::min<int64_t> (4096, (h->ncell + 255) / 256);

@@ -97,6 +97,8 @@ struct StepArgs
   // (k, -k or 2k) / (mu0 w), and the part this handle holds (0 real, 1 imaginary)
   int exp_pair, exp_part;
   double exp_k, exp_d, exp_w, exp_t0, exp_m;
+  // Yee layout: 0 E_CENTERED, 1 H_CENTERED (component positions of the test problems)
+  int layout;
 };
 
 // Near-to-far-field transform (yee_ntff): the complex fields' two parts, the snapshot of H before

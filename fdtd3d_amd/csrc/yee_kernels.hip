@@ -190,7 +190,7 @@ __device__ __forceinline__ void load_old (const StepArgs<T> &a, int64_t q, bool 
 // stencil are served by L1/L2 (half_cell).  Reads f, writes f_n (the same arrays when
 // updating in place).
 // ---------------------------------------------------------------------------------------
-template <typename T, int MODE>
+template <typename T, int MODE, int LAY = 0>
 __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j, int k)
 {
   const int64_t sx = a.sx;
@@ -206,9 +206,20 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
   load_old<T, MODE, 2> (a, q, special, fo2, do2);
   const T *Hx = a.f[3], *Hy = a.f[4], *Hz = a.f[5];
   const T hx = gld (Hx + q), hy = gld (Hy + q), hz = gld (Hz + q);
-  T vx[4] = { gld (Hz + q - sy), hz, gld (Hy + q - sz), hy };
-  T vy[4] = { gld (Hx + q - sz), hx, gld (Hz + q - sx), hz };
-  T vz[4] = { gld (Hy + q - sx), hy, gld (Hx + q - sy), hx };
+  // E_CENTERED: H at {-1, 0} along the derivative axis; H_CENTERED: at {0, +1}
+  T vx[4], vy[4], vz[4];
+  if constexpr (LAY == 0)
+  {
+    vx[0] = gld (Hz + q - sy); vx[1] = hz; vx[2] = gld (Hy + q - sz); vx[3] = hy;
+    vy[0] = gld (Hx + q - sz); vy[1] = hx; vy[2] = gld (Hz + q - sx); vy[3] = hz;
+    vz[0] = gld (Hy + q - sx); vz[1] = hy; vz[2] = gld (Hx + q - sy); vz[3] = hx;
+  }
+  else
+  {
+    vx[0] = hz; vx[1] = gld (Hz + q + sy); vx[2] = hy; vx[3] = gld (Hy + q + sz);
+    vy[0] = hx; vy[1] = gld (Hx + q + sz); vy[2] = hz; vy[3] = gld (Hz + q + sx);
+    vz[0] = hy; vz[1] = gld (Hy + q + sx); vz[2] = hx; vz[3] = gld (Hx + q + sy);
+  }
   T ex_m = a.eps_u, ey_m = a.eps_u, ez_m = a.eps_u;
   if (a.matc[0])
   {
@@ -219,17 +230,28 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
   }
   else if (a.eps)
   {
-    // 2-cell averages along each component's axis (getMaterial, E_CENTERED)
-    const T e = gld (a.eps + q);
-    ex_m = avg2 (e, gld (a.eps + q + sx));
-    ey_m = avg2 (e, gld (a.eps + q + sy));
-    ez_m = avg2 (e, gld (a.eps + q + sz));
+    const T *m = a.eps;
+    if constexpr (LAY == 0)
+    {
+      // 2-cell averages along each component's axis (getMaterial, E_CENTERED)
+      const T e = gld (m + q);
+      ex_m = avg2 (e, gld (m + q + sx));
+      ey_m = avg2 (e, gld (m + q + sy));
+      ez_m = avg2 (e, gld (m + q + sz));
+    }
+    else
+    {
+      // 4-cell transverse averages (getMaterial, H_CENTERED, E components)
+      ex_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sy), gld (m + q + sy + sz));
+      ey_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sx), gld (m + q + sx + sz));
+      ez_m = avg4 (gld (m + q), gld (m + q + sy), gld (m + q + sx), gld (m + q + sx + sy));
+    }
   }
   if (a.tfsf && tfsf_near<T, 3> (a, i, j, k))
   {
-    tfsf_apply<T, 0, 3> (a, i, j, k, vx);
-    tfsf_apply<T, 1, 3> (a, i, j, k, vy);
-    tfsf_apply<T, 2, 3> (a, i, j, k, vz);
+    tfsf_apply<T, 0, 3, LAY> (a, i, j, k, vx);
+    tfsf_apply<T, 1, 3, LAY> (a, i, j, k, vy);
+    tfsf_apply<T, 2, 3, LAY> (a, i, j, k, vz);
   }
   T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
   if (MODE == 2 && special)
@@ -257,7 +279,7 @@ __device__ __forceinline__ void e_half_cell (const StepArgs<T> &a, int i, int j,
 // 3D H half-step: Hx, Hy, Hz on the run box (one thread per cell).  E is read from f[0..2],
 // which the host points at the new E when the state is double-buffered.
 // ---------------------------------------------------------------------------------------
-template <typename T, int MODE>
+template <typename T, int MODE, int LAY = 0>
 __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j, int k)
 {
   const int64_t sx = a.sx;
@@ -273,16 +295,26 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
   load_old<T, MODE, 5> (a, q, special, fo5, do5);
   const T *Ex = a.f[0], *Ey = a.f[1], *Ez = a.f[2];
   const T ex = gld (Ex + q), ey = gld (Ey + q), ez = gld (Ez + q);
-  T vx[4] = { ey, gld (Ey + q + sz), ez, gld (Ez + q + sy) };
-  T vy[4] = { ez, gld (Ez + q + sx), ex, gld (Ex + q + sz) };
-  T vz[4] = { ex, gld (Ex + q + sy), ey, gld (Ey + q + sx) };
+  // E_CENTERED: E at {0, +1} along the derivative axis; H_CENTERED: at {-1, 0}
+  T vx[4], vy[4], vz[4];
+  if constexpr (LAY == 0)
+  {
+    vx[0] = ey; vx[1] = gld (Ey + q + sz); vx[2] = ez; vx[3] = gld (Ez + q + sy);
+    vy[0] = ez; vy[1] = gld (Ez + q + sx); vy[2] = ex; vy[3] = gld (Ex + q + sz);
+    vz[0] = ex; vz[1] = gld (Ex + q + sy); vz[2] = ey; vz[3] = gld (Ey + q + sx);
+  }
+  else
+  {
+    vx[0] = gld (Ey + q - sz); vx[1] = ey; vx[2] = gld (Ez + q - sy); vx[3] = ez;
+    vy[0] = gld (Ez + q - sx); vy[1] = ez; vy[2] = gld (Ex + q - sz); vy[3] = ex;
+    vz[0] = gld (Ex + q - sy); vz[1] = ex; vz[2] = gld (Ey + q - sx); vz[3] = ey;
+  }
   if (a.tfsf && tfsf_near<T, 3> (a, i, j, k))
   {
-    tfsf_apply<T, 3, 3> (a, i, j, k, vx);
-    tfsf_apply<T, 4, 3> (a, i, j, k, vy);
-    tfsf_apply<T, 5, 3> (a, i, j, k, vz);
+    tfsf_apply<T, 3, 3, LAY> (a, i, j, k, vx);
+    tfsf_apply<T, 4, 3, LAY> (a, i, j, k, vy);
+    tfsf_apply<T, 5, 3, LAY> (a, i, j, k, vz);
   }
-  // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
   T hx_m = a.mu_u, hy_m = a.mu_u, hz_m = a.mu_u;
   if (a.matc[3])
   {
@@ -293,9 +325,21 @@ __device__ __forceinline__ void h_half_cell (const StepArgs<T> &a, int i, int j,
   else if (a.mu)
   {
     const T *m = a.mu;
-    hx_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sy), gld (m + q + sy + sz));
-    hy_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sx), gld (m + q + sx + sz));
-    hz_m = avg4 (gld (m + q), gld (m + q + sy), gld (m + q + sx), gld (m + q + sx + sy));
+    if constexpr (LAY == 0)
+    {
+      // 4-cell transverse averages (getMaterial, E_CENTERED, H components)
+      hx_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sy), gld (m + q + sy + sz));
+      hy_m = avg4 (gld (m + q), gld (m + q + sz), gld (m + q + sx), gld (m + q + sx + sz));
+      hz_m = avg4 (gld (m + q), gld (m + q + sy), gld (m + q + sx), gld (m + q + sx + sy));
+    }
+    else
+    {
+      // 2-cell averages along each component's axis (getMaterial, H_CENTERED, H components)
+      const T u = gld (m + q);
+      hx_m = avg2 (u, gld (m + q + sx));
+      hy_m = avg2 (u, gld (m + q + sy));
+      hz_m = avg2 (u, gld (m + q + sz));
+    }
   }
   T oxm = 0, oym = 0, ozm = 0, gxm = 0, gym = 0, gzm = 0;
   if (MODE == 2 && special)
@@ -343,6 +387,24 @@ __global__ __launch_bounds__ (256) void h_half_3d (StepArgs<T> a)
   const int p[3] = { i, j, k };
   if (MODE == 2 && !in_drude_box (a, p, 3)) h_half_cell<T, 1> (a, i, j, k);
   else h_half_cell<T, MODE> (a, i, j, k);
+}
+
+// The H_CENTERED layout (--layout-type 1) on the same half-step cells: plain and PML runs
+// (MODE 0 / 1; the host refuses Drude media and double material precision with it).
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void e_half_3d_hc (StepArgs<T> a)
+{
+  int i, j, k;
+  if (!half_cell (a, i, j, k)) return;
+  e_half_cell<T, MODE, 1> (a, i, j, k);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__ (256) void h_half_3d_hc (StepArgs<T> a)
+{
+  int i, j, k;
+  if (!half_cell (a, i, j, k)) return;
+  h_half_cell<T, MODE, 1> (a, i, j, k);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1974,6 +2036,10 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
   template __global__ void h_half_3d<T, 0> (StepArgs<T>);                                  \
   template __global__ void h_half_3d<T, 1> (StepArgs<T>);                                  \
   template __global__ void h_half_3d<T, 2> (StepArgs<T>);                                  \
+  template __global__ void e_half_3d_hc<T, 0> (StepArgs<T>);                               \
+  template __global__ void e_half_3d_hc<T, 1> (StepArgs<T>);                               \
+  template __global__ void h_half_3d_hc<T, 0> (StepArgs<T>);                               \
+  template __global__ void h_half_3d_hc<T, 1> (StepArgs<T>);                               \
   template __global__ void e_half_tez<T, 0> (StepArgs<T>);                                 \
   template __global__ void e_half_tez<T, 1> (StepArgs<T>);                                 \
   template __global__ void e_half_tez<T, 2> (StepArgs<T>);                                 \

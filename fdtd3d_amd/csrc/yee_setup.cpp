@@ -175,6 +175,13 @@ Scalars compute_scalars (const yee_config &cfg)
     s.zero_inc[1] = cfg.tfsf_left[1] - 2.5 * std::sin (a2);
     s.zero_inc[2] = 0;
   }
+  if (missing_axis (cfg) == 2 && cfg.layout == 1)
+  {
+    // H_CENTERED TEz / TMz run on the 3D kernels: their d = x C2 + y S2 is the 3D expression
+    // x S1 C2 + y S1 S2 + z C1 with S1 = 1 and C1 = 0 (exact)
+    s.S1 = 1.0;
+    s.C1 = 0.0;
+  }
   return s;
 }
 
@@ -255,10 +262,12 @@ std::vector<double> sigma_profile (const yee_config &cfg, int axis)
 // component of that axis or an H component of another axis; getMaterial then averages the
 // two eps cells p and p+1.  At a half-integer coordinate the cells coincide and the average
 // reproduces the cell value exactly, as do the 4-cell averages of H components.
-double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p, int64_t n)
+double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p, int64_t n, int layout)
 {
   const CompDesc &c = kComp[comp];
-  const bool integer_coord = c.is_h ? (axis != c.axis) : (axis == c.axis);
+  // H_CENTERED swaps the E and H positions (YeeGridLayout.cpp:1962-1967, 2016-2021)
+  const bool h_like = c.is_h != (layout == 1);
+  const bool integer_coord = h_like ? (axis != c.axis) : (axis == c.axis);
   if ((int64_t) prof.size () == 2 * n)
   {
     // fine sigma grid (double material precision): the 8-cell average of a profile that varies
@@ -290,7 +299,7 @@ std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len)
   const double e0 = eps0 (), dt = s.dt, dx = cfg.dx;
   auto at = [&] (int comp, int axis, int64_t p) -> double {
     if (!axis_active (cfg, axis)) return 0.0;   // no sigma along an axis the scheme lacks: reads as 0
-    return sigma_at (prof[axis], comp, axis, p, cfg.size[axis]);
+    return sigma_at (prof[axis], comp, axis, p, cfg.size[axis], cfg.layout);
   };
   for (int comp = 0; comp < 6; ++comp)
   {

@@ -80,7 +80,7 @@ enum PmlRole { kCAC = 0, kCBC = 1, kP4 = 2, kM4 = 3, kCA5 = 4, kDD5 = 5, kNumRol
 std::vector<double> pml_tables (const yee_config &cfg, int64_t tab_len);
 
 // Averaged sigma of `comp` along `axis` at index p (InternalScheme::getMaterial on a sigma grid).
-double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p, int64_t n);
+double sigma_at (const std::vector<double> &prof, int comp, int axis, int64_t p, int64_t n, int layout = 0);
 
 // Material generators restating Approximation::approximateSphere* (Approximation.cpp:563-764).
 double sphere_accurate_3d (double mx, double my, double mz, double cx, double cy, double cz,

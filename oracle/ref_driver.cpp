@@ -119,14 +119,24 @@ bool readRaw (const std::string &name, std::vector<FPValue> &dst, size_t n)
   return true;
 }
 
-template <SchemeType_t Type, template <typename, bool> class TCoord>
+/* Coordinate slot sl of a 1D, 2D or 3D coordinate. */
+inline FPValue fpSlot (const GridCoordinateFP1D &c, int) { return c.get1 (); }
+inline FPValue fpSlot (const GridCoordinateFP2D &c, int sl) { return sl == 0 ? c.get1 () : c.get2 (); }
+inline FPValue fpSlot (const GridCoordinateFP3D &c, int sl) { return sl == 0 ? c.get1 () : (sl == 1 ? c.get2 () : c.get3 ()); }
+inline grid_coord tcSlot (const GridCoordinate1D &c, int) { return c.get1 (); }
+inline grid_coord tcSlot (const GridCoordinate2D &c, int sl) { return sl == 0 ? c.get1 () : c.get2 (); }
+inline grid_coord tcSlot (const GridCoordinate3D &c, int sl) { return sl == 0 ? c.get1 () : (sl == 1 ? c.get2 () : c.get3 ()); }
+
+/* One run of the reference's InternalScheme for scheme Type with Yee layout LT
+ * (E_CENTERED or H_CENTERED, --layout-type 0 / 1). */
+template <SchemeType_t Type, template <typename, bool> class TCoord, LayoutType LT>
 class RefRun
 {
   typedef TCoord<grid_coord, true> TC;
   typedef TCoord<grid_coord, false> TCS;
   typedef TCoord<FPValue, true> TCFP;
-  typedef InternalScheme<Type, TCoord, E_CENTERED> IS;
-  typedef YeeGridLayout<Type, TCoord, E_CENTERED> Layout;
+  typedef InternalScheme<Type, TCoord, LT> IS;
+  typedef YeeGridLayout<Type, TCoord, LT> Layout;
 
   CoordinateType ct1, ct2, ct3;
   TC size;
@@ -150,8 +160,6 @@ class RefRun
     if (dim == 1) return 0;
     return (miss >= 0 && a > miss) ? a - 1 : a;
   }
-  FPValue fpSlot (const TCFP &c, int sl) const;
-  grid_coord tcSlot (const TC &c, int sl) const;
 
   TC coordOf (grid_coord i, grid_coord j, grid_coord k) const
   {
@@ -942,78 +950,30 @@ public:
   }
 };
 
-#define REF_SLOTS_3D(MODE)                                                                                   \
-  template <>                                                                                                \
-  FPValue RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate3DTemplate>::fpSlot (const GridCoordinateFP3D &c, int sl) const \
-  {                                                                                                          \
-    return sl == 0 ? c.get1 () : (sl == 1 ? c.get2 () : c.get3 ());                                          \
-  }                                                                                                          \
-  template <>                                                                                                \
-  grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate3DTemplate>::tcSlot (const GridCoordinate3D &c, int sl) const \
-  {                                                                                                          \
-    return sl == 0 ? c.get1 () : (sl == 1 ? c.get2 () : c.get3 ());                                          \
+/* Builds, runs and dumps one RefRun with the layout --layout-type selects. */
+template <SchemeType_t Type, template <typename, bool> class TCoord, LayoutType LT>
+double driveLayout (TCoord<grid_coord, true> s, TCoord<grid_coord, true> p, TCoord<grid_coord, true> l,
+                    TCoord<grid_coord, true> r, int dim, int axis, bool text)
+{
+  RefRun<Type, TCoord, LT> run (s, p, l, r, dim, axis);
+  run.initMaterials ();
+  double secs = run.run ();
+  run.dump ();
+  if (text)
+  {
+    if (const char *txt = getenv ("YEE_REF_TXT")) run.dumpText (txt);
   }
-#define REF_SLOTS_2D(MODE)                                                                                   \
-  template <>                                                                                                \
-  FPValue RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate>::fpSlot (const GridCoordinateFP2D &c, int sl) const \
-  {                                                                                                          \
-    return sl == 0 ? c.get1 () : c.get2 ();                                                                  \
-  }                                                                                                          \
-  template <>                                                                                                \
-  grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate>::tcSlot (const GridCoordinate2D &c, int sl) const \
-  {                                                                                                          \
-    return sl == 0 ? c.get1 () : c.get2 ();                                                                  \
-  }
-#define REF_SLOTS_1D(MODE)                                                                                   \
-  template <>                                                                                                \
-  FPValue RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate>::fpSlot (const GridCoordinateFP1D &c, int) const \
-  {                                                                                                          \
-    return c.get1 ();                                                                                        \
-  }                                                                                                          \
-  template <>                                                                                                \
-  grid_coord RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate>::tcSlot (const GridCoordinate1D &c, int) const \
-  {                                                                                                          \
-    return c.get1 ();                                                                                        \
-  }
-#ifdef MODE_DIM3
-REF_SLOTS_3D (Dim3)
-#endif
-#ifdef MODE_EX_HY
-REF_SLOTS_1D (Dim1_ExHy)
-#endif
-#ifdef MODE_EX_HZ
-REF_SLOTS_1D (Dim1_ExHz)
-#endif
-#ifdef MODE_EY_HX
-REF_SLOTS_1D (Dim1_EyHx)
-#endif
-#ifdef MODE_EY_HZ
-REF_SLOTS_1D (Dim1_EyHz)
-#endif
-#ifdef MODE_EZ_HX
-REF_SLOTS_1D (Dim1_EzHx)
-#endif
-#ifdef MODE_EZ_HY
-REF_SLOTS_1D (Dim1_EzHy)
-#endif
-#ifdef MODE_TEX
-REF_SLOTS_2D (Dim2_TEx)
-#endif
-#ifdef MODE_TEY
-REF_SLOTS_2D (Dim2_TEy)
-#endif
-#ifdef MODE_TEZ
-REF_SLOTS_2D (Dim2_TEz)
-#endif
-#ifdef MODE_TMX
-REF_SLOTS_2D (Dim2_TMx)
-#endif
-#ifdef MODE_TMY
-REF_SLOTS_2D (Dim2_TMy)
-#endif
-#ifdef MODE_TMZ
-REF_SLOTS_2D (Dim2_TMz)
-#endif
+  return secs;
+}
+
+template <SchemeType_t Type, template <typename, bool> class TCoord>
+double drive (TCoord<grid_coord, true> s, TCoord<grid_coord, true> p, TCoord<grid_coord, true> l,
+              TCoord<grid_coord, true> r, int dim, int axis = -1, bool text = false)
+{
+  if (SOLVER_SETTINGS.getLayoutType () == H_CENTERED)
+    return driveLayout<Type, TCoord, H_CENTERED> (s, p, l, r, dim, axis, text);
+  return driveLayout<Type, TCoord, E_CENTERED> (s, p, l, r, dim, axis, text);
+}
 
 } /* namespace */
 
@@ -1021,11 +981,6 @@ int main (int argc, char **argv)
 {
   solverSettings.SetupFromCmd (argc, argv);
   solverSettings.Initialize ();
-  if (SOLVER_SETTINGS.getLayoutType () != E_CENTERED)
-  {
-    fprintf (stderr, "only --layout-type 0 (E_CENTERED) is driven\n");
-    return 3;
-  }
   if (SOLVER_SETTINGS.getDoUseCaCbPMLMetaGrids ())
   {
     fprintf (stderr, "--use-ca-cb-pml-metamaterials: the reference never binds its a1 / a2 grids\n");
@@ -1055,11 +1010,7 @@ int main (int argc, char **argv)
                                              CoordinateType::X, CoordinateType::Y, CoordinateType::Z);
     GridCoordinate3D r = GRID_COORDINATE_3D (SOLVER_SETTINGS.getTFSFSizeRightX (), SOLVER_SETTINGS.getTFSFSizeRightY (), SOLVER_SETTINGS.getTFSFSizeRightZ (),
                                              CoordinateType::X, CoordinateType::Y, CoordinateType::Z);
-    RefRun<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate> run (s, p, l, r, 3);
-    run.initMaterials ();
-    secs = run.run ();
-    run.dump ();
-    if (const char *txt = getenv ("YEE_REF_TXT")) run.dumpText (txt);
+    secs = drive<static_cast<SchemeType_t> (SchemeType::Dim3), GridCoordinate3DTemplate> (s, p, l, r, 3, -1, true);
   }
   else
 #endif
@@ -1070,10 +1021,7 @@ int main (int argc, char **argv)
     GridCoordinate2D p = GRID_COORDINATE_2D (SOLVER_SETTINGS.getPMLSize##GA (), SOLVER_SETTINGS.getPMLSize##GB (), CA, CB); \
     GridCoordinate2D l = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeLeft##GA (), SOLVER_SETTINGS.getTFSFSizeLeft##GB (), CA, CB); \
     GridCoordinate2D r = GRID_COORDINATE_2D (SOLVER_SETTINGS.getTFSFSizeRight##GA (), SOLVER_SETTINGS.getTFSFSizeRight##GB (), CA, CB); \
-    RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate> run (s, p, l, r, 2, MISSING); \
-    run.initMaterials ();                                                                                    \
-    secs = run.run ();                                                                                       \
-    run.dump ();                                                                                             \
+    secs = drive<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate2DTemplate> (s, p, l, r, 2, MISSING); \
   }                                                                                                          \
   else
   /* the lines of main.cpp:629-800 */
@@ -1084,10 +1032,7 @@ int main (int argc, char **argv)
     GridCoordinate1D p = GRID_COORDINATE_1D (SOLVER_SETTINGS.getPMLSize##GA (), CA);                         \
     GridCoordinate1D l = GRID_COORDINATE_1D (SOLVER_SETTINGS.getTFSFSizeLeft##GA (), CA);                    \
     GridCoordinate1D r = GRID_COORDINATE_1D (SOLVER_SETTINGS.getTFSFSizeRight##GA (), CA);                   \
-    RefRun<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate> run (s, p, l, r, 1, AXIS); \
-    run.initMaterials ();                                                                                    \
-    secs = run.run ();                                                                                       \
-    run.dump ();                                                                                             \
+    secs = drive<static_cast<SchemeType_t> (SchemeType::MODE), GridCoordinate1DTemplate> (s, p, l, r, 1, AXIS); \
   }                                                                                                          \
   else
 #ifdef MODE_EX_HY

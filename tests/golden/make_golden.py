@@ -111,6 +111,19 @@ def cases(tmp):
         "--1d-ezhy --size x:90 --use-pml --pml-size x:8 --point-source x:30 --point-source-ez "
         "--current-source x:55 --current-source-jz --current-source-my --angle-teta 90 --angle-phi 0 "
         "--angle-psi 90 --time-steps 200").split()
+    # the H_CENTERED layout (--layout-type 1; Tests/suite t1.2, t2.3, t6.x, t7.x and t8 run both
+    # layouts): 3D with and without PML, the TMz / TEz / TEy planes and two 1D lines
+    hc = ["--layout-type", "1"]
+    out["hc_d3_pml_tfsf_oblique_sphere"] = out["d3_pml_tfsf_oblique_sphere"] + hc
+    out["hc_d3_tfsf_axial_pointsource"] = out["d3_tfsf_axial_pointsource"] + hc
+    out["hc_tmz_pml_tfsf_oblique"] = out["tmz_pml_tfsf_oblique"] + hc
+    out["hc_tez_pml_tfsf_oblique_sphere"] = out["tez_pml_tfsf_oblique_sphere"] + hc
+    out["hc_tey_pml_tfsf_sphere"] = out["tey_pml_tfsf_sphere"] + hc
+    out["hc_exhy_pml_tfsf"] = (
+        f"--1d-exhy --size z:{n1} --use-pml --pml-size z:10 --use-tfsf --tfsf-size-left z:20 "
+        "--tfsf-size-right z:20 --angle-teta 0 --angle-phi 0 --angle-psi 90 --eps-sphere 2 "
+        "--eps-sphere-center z:75 --eps-sphere-radius 10 --time-steps 300").split() + hc
+    out["hc_ezhy_pml_sources"] = out["ezhy_pml_sources"] + hc
     return out
 
 

@@ -137,7 +137,8 @@ def test_cli_rejects_options_outside_the_path():
 
 INVALID_CONFIGS = [
     (dict(precision=2), b"precision"),
-    (dict(layout=1), b"E_CENTERED"),
+    (dict(layout=2), b"H_CENTERED"),
+    (dict(size=[10, 10, 10], layout=1, use_pml=True, use_metamaterials=True), b"H_CENTERED"),
     (dict(size=[2, 10, 10]), b"size"),
     (dict(size=[10, 10, 1 << 21]), b"size"),
     (dict(size=[8, 8, 8], pml=[5, 1, 1], use_pml=True), b"pml"),

@@ -66,26 +66,29 @@ def test_test_functions_vs_oracle(args, precision, engine_lib):
 _FN = "--use-{k}-border-condition --use-{k}-start-values --use-{k}-right-side --calc-{k}-diff-norm"
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("precision", [8])
 @pytest.mark.parametrize("scheme,kind,comps", [
     ("--3d", "polinom1", ["Ez"]), ("--2d-tmz", "polinom1", ["Ez"]), ("--2d-tey", "polinom1", ["Ez"]),
     ("--3d", "polinom2", ["Ex", "Ey", "Ez"]), ("--3d", "polinom3", ["Ez"]), ("--2d-tmz", "polinom3", ["Ez"]),
     ("--2d-tey", "polinom3", ["Ez"]),
 ])
-def test_suite_polynomial_bar(scheme, kind, comps, precision, engine_lib):
-    """Tests/suite t2.x / t3 / t4.x: the largest diff-norm percentage over the run <= 1e-4 %."""
+def test_suite_polynomial_bar(scheme, kind, comps, precision, layout, engine_lib):
+    """Tests/suite t2.x / t3 / t4.x: the largest diff-norm percentage over the run <= 1e-4 %, in
+    both Yee layouts (t2.3 runs --layout-type 0 and 1)."""
     worst = 0.0
     for size, dx, steps in ((20, "0.001", 101), (40, "0.0005", 201)):
         args = (f"{scheme} --size x:{size} --same-size --angle-phi 0 --dx {dx} --wavelength 0.02 "
                 + _FN.format(k=kind) + f" --time-steps {steps}")
         cfg = parse_args(args.split())
         cfg.precision = precision
+        cfg.layout = layout
         eng = Engine(cfg)
         eng.prepare()
         norms = run_norms(eng, cfg)
         eng.close()
         worst = max(worst, max(float(norms[c][:, 1].max()) for c in comps))
-    say(f"[gpu] suite {scheme} {kind} precision {precision}: largest percent {worst:.3e}")
+    say(f"[gpu] suite {scheme} {kind} precision {precision} layout {layout}: largest percent {worst:.3e}")
     assert worst <= 1e-4
 
 
@@ -201,20 +204,21 @@ def test_exp_solutions_vs_oracle(args, engine_lib):
     eng.close()
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("pair", ["exhy", "exhz", "eyhx", "eyhz", "ezhx", "ezhy"])
-def test_suite_t7_plane_wave_and_dielectric(pair, engine_lib):
+def test_suite_t7_plane_wave_and_dielectric(pair, layout, engine_lib):
     """Tests/suite t7.1 - t7.6 (one 1D pair each, complex build, 600 cells, dz 0.1 mm, lambda
     2 cm, PML 50, TF/SF border at 150 on the left only, the dielectric from 300, 10000 steps):
     exp1 in vacuum over [155, 545), exp3 inside eps = 4 over [305, 545), exp2 reflected over
     [55, 145); at the last step the real, imaginary and module percentages of both components
-    stay under 0.12 / 0.12 / 0.17 %."""
+    stay under 0.12 / 0.12 / 0.17 %, in both Yee layouts (t7.x run --layout-type 0 and 1)."""
     import dataclasses
     from tests.helpers import exp_case
     for kind, norm, eps, bar in (("exp1", (155, 545), 1, 0.12), ("exp3", (305, 545), 4, 0.12),
                                  ("exp2", (55, 145), 4, 0.17)):
         args = exp_case(pair, kind, size=600, pml=50, tfsf=150, center=450, radius=150, steps=10000, dx="0.0001",
                         wavelength="0.02", eps=eps, norm=norm)
-        cfg = dataclasses.replace(parse_args(args.split()), complex_fields=True)
+        cfg = dataclasses.replace(parse_args(args.split()), complex_fields=True, layout=layout)
         eng = _exp_engine(cfg)
         eng.step(cfg.time_steps)
         worst = 0.0
@@ -222,7 +226,7 @@ def test_suite_t7_plane_wave_and_dielectric(pair, engine_lib):
             v = eng.diff_norm_complex(n)
             worst = max(worst, v[3], v[4], v[5])
         eng.close()
-        say(f"[gpu] t7 {pair} {kind}: largest percentage {worst:.4f} (bar {bar})")
+        say(f"[gpu] t7 {pair} {kind} layout {layout}: largest percentage {worst:.4f} (bar {bar})")
         assert 0 < worst <= bar, (pair, kind, worst)
 
 
