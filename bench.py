@@ -62,6 +62,8 @@ WORKLOADS = {
                  make=lambda: cfg2()),
     "cfg2-768-fp64": dict(desc="3D 768^3 FP64, otherwise as cfg2", make=lambda: cfg2(n=768, precision=8)),
     "cfg2-512": dict(desc="3D 512^3 FP32, otherwise as cfg2", make=lambda: cfg2(n=512)),
+    "cfg2-512-hc": dict(desc="3D 512^3 FP32, H_CENTERED layout (--layout-type 1, half-step kernels), otherwise as cfg2",
+                        make=lambda: _hc(cfg2(n=512))),
     "cfg2-plain": dict(desc="3D 1024^3 FP32 without PML and TF/SF (stencil only), eps as cfg2",
                        make=lambda: _plain(cfg2())),
     "cfg3-drude": dict(desc="3D 512^3 FP32, PML 16, TF/SF lambda=40 cells theta=phi=psi=0, central 128^3 Drude slab "
@@ -350,6 +352,11 @@ def precision_check(args) -> dict:
 def _plain(c: Config) -> Config:
     c.use_pml = False
     c.use_tfsf = False
+    return c
+
+
+def _hc(c: Config) -> Config:
+    c.layout = 1
     return c
 
 
