@@ -496,3 +496,31 @@ def test_cli_complex_dumps(engine_lib, tmp_path):
     assert np.allclose(inc[:, 1], e_re[: 500 * 16], rtol=1e-14, atol=1e-300)
     assert np.allclose(inc[:, 2], e_im[: 500 * 16], rtol=1e-14, atol=1e-300)
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["hc_d3_tfsf_axial_pointsource", "hc_ezhy_pml_sources"])
+def test_cli_h_centered_dumps_match_reference_fixture(name, engine_lib, tmp_path):
+    """The CLI with --layout-type 1 on a fixture's own command line: the binary dumps of the whole
+    grid equal the H_CENTERED fields the reference wrote (3D without PML: bit for bit; the 1D line
+    with PML: relative L2 <= 1e-12)."""
+    exe = os.path.join(BIN, "fdtd3d")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    cfg, _, data = load_golden(name)
+    assert cfg.layout == 1
+    argv = str(data["argv"]).split()
+    names = "xyz"
+    start = ",".join(f"{names[a]}:0" for a in cfg.active_axes)
+    end = ",".join(f"{names[a]}:{cfg.size[a]}" for a in cfg.active_axes)
+    say(f"[gpu] cli H_CENTERED dumps {name}")
+    out = subprocess.run([exe] + argv + ["--save-res", "--save-as-dat", "--manual-save-coords", "--save-start-coord",
+                                         start, "--save-end-coord", end],
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    for f in field_names(cfg):
+        got = np.fromfile(tmp_path / f"previous-1_[timestep={cfg.time_steps}]_[pid=0]_[name={f}].dat", dtype=np.float64)
+        ref = data[f].ravel()
+        if cfg.use_pml:
+            assert rel_l2(got, ref) <= TOL_FP64, (f, rel_l2(got, ref))
+        else:
+            np.testing.assert_array_equal(got, ref, err_msg=f)
