@@ -1790,7 +1790,6 @@ int scattered_checks (yee_handle *h, int32_t comp, const double *out)
   if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "component not present in this scheme");
   // the reference interpolates EInc / HInc, which exist only with TF/SF
   if (!h->use_tfsf ()) return fail (YEE_EINVAL, "TF/SF is not enabled");
-  if (h->cfg.layout != 0) return fail (YEE_ENOTSUP, "the scattered field is computed for the E_CENTERED layout only");
   if (!h->prepared) return fail (YEE_ESTATE, "yee_prepare has not been called");
   return YEE_OK;
 }
@@ -1875,7 +1874,6 @@ int yee_get_mask (yee_handle *h, int32_t kind, int32_t comp, uint8_t *out)
   if (kind == 0)
   {
     if (comp < 0 || comp > 5 || !h->exposed[comp]) return fail (YEE_EINVAL, "bad component");
-    if (h->cfg.layout != 0) return fail (YEE_ENOTSUP, "the TF/SF mask export covers the E_CENTERED layout only");
     uint8_t *dev = nullptr;
     HIP_TRY (hipMalloc (&dev, (size_t) h->ncell));
     const int grid = (int) std::min<int64_t> (4096, (h->ncell + 255) / 256);

@@ -1968,6 +1968,17 @@ __global__ __launch_bounds__ (256) void ntff_faces (NtffArgs<T> a)
 // correction does.
 // Cells [q0, q0 + count) of the flat grid; out[q - q0] receives the value in the field
 // precision, widened to FP64.
+// The incident value a cell of component COMP holds inside [L, R] of the TF/SF box (at the
+// component's position in layout LAY), 0 outside.
+template <typename T, int COMP, int DIM, int LAY>
+__device__ double scattered_incident (const StepArgs<T> &a, int i, int j, int k, const T *inc)
+{
+  bool in = twice_coord<COMP, 0, LAY> (i) >= 2 * a.tl[0] && twice_coord<COMP, 0, LAY> (i) <= 2 * a.tr[0]
+            && twice_coord<COMP, 1, LAY> (j) >= 2 * a.tl[1] && twice_coord<COMP, 1, LAY> (j) <= 2 * a.tr[1];
+  if (DIM == 3) in = in && twice_coord<COMP, 2, LAY> (k) >= 2 * a.tl[2] && twice_coord<COMP, 2, LAY> (k) <= 2 * a.tr[2];
+  return in ? incident_at<T, COMP, DIM, LAY> (a, i, j, k, inc, COMP >= 3 ? 0.5 : 0.0) : 0.0;
+}
+
 template <typename T, int COMP, int DIM>
 __global__ void scattered_kernel (StepArgs<T> a, const T *f, double *out, int hsel, int64_t q0, int64_t count)
 {
@@ -1979,14 +1990,9 @@ __global__ void scattered_kernel (StepArgs<T> a, const T *f, double *out, int hs
   const int j = (int) ((q / nzz) % a.ny);
   const int i = (int) (q / ((int64_t) nzz * a.ny));
   double v = (double) f[q];
-  bool in = twice_coord<COMP, 0> (i) >= 2 * a.tl[0] && twice_coord<COMP, 0> (i) <= 2 * a.tr[0]
-            && twice_coord<COMP, 1> (j) >= 2 * a.tl[1] && twice_coord<COMP, 1> (j) <= 2 * a.tr[1];
-  if (DIM == 3) in = in && twice_coord<COMP, 2> (k) >= 2 * a.tl[2] && twice_coord<COMP, 2> (k) <= 2 * a.tr[2];
-  if (in)
-  {
-    const T *inc = COMP >= 3 ? a.tg->hinc[hsel] : a.tg->einc;
-    v -= incident_at<T, COMP, DIM> (a, i, j, k, inc, COMP >= 3 ? 0.5 : 0.0);
-  }
+  const T *inc = COMP >= 3 ? a.tg->hinc[hsel] : a.tg->einc;
+  if (a.layout == 0) v -= scattered_incident<T, COMP, DIM, 0> (a, i, j, k, inc);
+  else v -= scattered_incident<T, COMP, DIM, 1> (a, i, j, k, inc);
   out[u] = (double) (T) v;
 }
 
@@ -2000,7 +2006,19 @@ __global__ void tfsf_mask_kernel (StepArgs<T> a, int comp, int dim, uint8_t *out
     const int j = (int) ((q / a.nz) % a.ny);
     const int i = (int) (q / ((int64_t) a.nz * a.ny));
     uint8_t m = 0;
-    if (dim == 3)
+    if (dim == 3 && a.layout == 1)
+    {
+      switch (comp)
+      {
+        case 0: m = tfsf_bits<T, 0, 3, 1> (a, i, j, k); break;
+        case 1: m = tfsf_bits<T, 1, 3, 1> (a, i, j, k); break;
+        case 2: m = tfsf_bits<T, 2, 3, 1> (a, i, j, k); break;
+        case 3: m = tfsf_bits<T, 3, 3, 1> (a, i, j, k); break;
+        case 4: m = tfsf_bits<T, 4, 3, 1> (a, i, j, k); break;
+        default: m = tfsf_bits<T, 5, 3, 1> (a, i, j, k); break;
+      }
+    }
+    else if (dim == 3)
     {
       switch (comp)
       {

@@ -119,8 +119,7 @@ typedef struct yee_config
                                  axis (main.cpp:629-800) and ignore the other two */
   int32_t layout;             /* --layout-type: 0 = E_CENTERED, 1 = H_CENTERED (E and H positions swapped,
                                  YeeGridLayout.cpp:1937-2043; runs without metamaterials and double
-                                 material precision; NTFF, the scattered field, the TF/SF mask export
-                                 and the adjoint take E_CENTERED runs only) */
+                                 material precision; NTFF and the adjoint take E_CENTERED runs only) */
   int32_t precision;          /* 4 = FP32 fields, 8 = FP64 fields (reference: -DFLOAT_VALUES / -DDOUBLE_VALUES) */
   int32_t time_steps;         /* --time-steps T; step t = 0 performs no field update (Scheme.inc.h:1032) */
   int64_t size[3];            /* --size x:,y:,z: */

@@ -443,3 +443,53 @@ def test_complex_fields_vs_oracle(args, precision, engine_lib):
         else:
             check_fp32(f"complex {label}", field_names(cfg), part.field, ref.field)
     eng.close()
+
+
+@pytest.mark.parametrize("args", [MASK_CASES[0], MASK_CASES[1]])
+def test_h_centered_masks_are_the_role_swapped_e_centered_masks(args, engine_lib):
+    """H_CENTERED puts each E component where E_CENTERED puts the H component of its axis, and its
+    TF/SF predicates are the ones of that position (YeeGridLayout.cpp:292-306 for Ex against
+    :1130-1147 for Hx): the H_CENTERED mask of Ex equals the E_CENTERED mask of Hx bit for bit, and
+    so on for every pair of the scheme (3D, and TMz on the 3D kernels)."""
+    import dataclasses
+    cfg0 = parse_args(args.split())
+    cfg1 = dataclasses.replace(cfg0, layout=1)
+    say(f"[gpu] H_CENTERED masks {args[:40]}")
+    e0, e1 = Engine(cfg0), Engine(cfg1)
+    e0.prepare()
+    e1.prepare()
+    swap = {"Ex": "Hx", "Ey": "Hy", "Ez": "Hz", "Hx": "Ex", "Hy": "Ey", "Hz": "Ez"}
+    for comp in field_names(cfg0):
+        if swap[comp] not in field_names(cfg0):
+            continue
+        m1 = e1.tfsf_mask(comp)
+        np.testing.assert_array_equal(m1, e0.tfsf_mask(swap[comp]), err_msg=comp)
+        assert m1.any()
+    e0.close()
+    e1.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_scattered_field_vanishes_in_vacuum(layout, engine_lib):
+    """Without a scatterer the total field inside the TF/SF box is the incident wave: total minus
+    the interpolated incident value at each component's position stays at the discretisation
+    residue (a position off by half a cell leaves ~k dx / 2 of the amplitude)."""
+    import dataclasses
+    args = ("--3d --size x:40,y:36,z:44 --use-pml --pml-size x:6 --same-size-pml --use-tfsf "
+            "--tfsf-size-left x:10 --tfsf-size-right x:10 --same-size-tfsf --angle-teta 30 --angle-phi 45 "
+            "--angle-psi 90 --dx 0.0005 --wavelength 0.02 --time-steps 120")
+    cfg = dataclasses.replace(parse_args(args.split()), layout=layout)
+    say(f"[gpu] scattered field in vacuum, layout {layout}")
+    eng = Engine(cfg)
+    eng.prepare()
+    eng.step(cfg.time_steps)
+    eng.sync()
+    for comp in ("Ex", "Ey", "Ez"):
+        tot = eng.field(comp)
+        sc = eng.scattered(comp)
+        inner = (slice(12, 28), slice(12, 24), slice(12, 32))
+        ratio = np.abs(sc[inner]).max() / np.abs(tot[inner]).max()
+        say(f"[gpu]   {comp}: max |scattered| / max |total| inside the box = {ratio:.2e}")
+        assert np.abs(tot[inner]).max() > 0.1
+        assert ratio < 0.02, (comp, ratio)
+    eng.close()
