@@ -140,6 +140,13 @@ def complex_cases():
             "--2d-tmz --size x:30,y:26 --use-pml --pml-size x:4,y:4 --use-tfsf --tfsf-size-left x:7,y:7 "
             "--tfsf-size-right x:7,y:7 --angle-teta 90 --angle-phi 30 --angle-psi 90 --current-source x:14,y:12 "
             "--current-source-jz --current-source-mx --point-source x:10,y:9 --point-source-ez --time-steps 50").split(),
+        # the H_CENTERED layout with complex sources
+        "hc_c3d_pml_tfsf_sources": (
+            "--3d --size x:16,y:15,z:17 --use-pml --pml-size x:3,y:3,z:3 --use-tfsf --tfsf-size-left x:5,y:5,z:5 "
+            "--tfsf-size-right x:5,y:5,z:5 --angle-teta 30 --angle-phi 45 --angle-psi 90 --eps-sphere 3 "
+            "--eps-sphere-center x:8,y:7,z:8 --eps-sphere-radius 3 --point-source x:7,y:7,z:9 --point-source-hy "
+            "--current-source x:9,y:8,z:8 --current-source-jx --current-source-mz --time-steps 30 "
+            "--layout-type 1").split(),
     }
 
 
